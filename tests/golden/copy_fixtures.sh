@@ -1,0 +1,21 @@
+#!/usr/bin/env bash
+# Copies the reference's committed sssp golden data (inputs and MOSP-OpenMP outputs) into
+# tests/golden/. Source: cpp/tests/data/mosp_sssp (34 cases, cases.txt) and
+# cpp/tests/data/mosp_graph_io (the inputs of 27 of them, plus the MOSP applyChangeBatch goldens
+# in <case>/applied/). Data only: no source file of the reference is copied.
+set -euo pipefail
+SRC=${1:-/root/reference/cpp/tests/data}
+DST=$(cd "$(dirname "$0")" && pwd)
+rm -rf "$DST/mosp_sssp" "$DST/mosp_graph_io"
+mkdir -p "$DST/mosp_sssp" "$DST/mosp_graph_io"
+cp "$SRC/mosp_sssp/cases.txt" "$DST/mosp_sssp/"
+for d in "$SRC"/mosp_sssp/*/; do
+  cp -r "$d" "$DST/mosp_sssp/"
+done
+cp "$SRC/mosp_graph_io/cases.txt" "$DST/mosp_graph_io/"
+for d in "$SRC"/mosp_graph_io/*/; do
+  name=$(basename "$d")
+  [ "$name" = mtx ] && continue
+  cp -r "$d" "$DST/mosp_graph_io/"
+done
+echo "copied $(ls "$DST/mosp_sssp" | wc -l) mosp_sssp entries and $(ls "$DST/mosp_graph_io" | wc -l) mosp_graph_io entries"
