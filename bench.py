@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""bench.py -- edges/s and p50 ms/batch of dyng_amd.sssp.update on one MI355X.
+
+Workload (the headline configuration): RMAT scale 22 (4,194,304 vertices, 67,108,864 distinct
+directed non-loop edges, a=0.57 b=0.19 c=0.19 d=0.05), weights U[1,255], source = the vertex with
+the largest out-degree; mixed batches of 100,000 operations (40% insertions of new edges with
+RMAT endpoints, 40% deletions of existing edges, 20% reweights of existing edges to U[1,255]).
+Graph seed 12345 ^ 1, batch seed 777 ^ 1 (splitmix64 counter-based draws).
+
+A step is one sssp.update call (device batch apply + repair) on a batch already resident in HBM.
+`value` = batch operations / p50 wall seconds per update. The JSON line also carries the
+persistent kernel's roofline (algorithmic bytes / HIP-event duration against 8 TB/s), the
+OpenMP CPU baseline (the oracle's restatement of the reference engine, timed on this host on a
+bounded sample) and the bit-exactness check of the GPU trees against that CPU engine.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "edges/s & p50 ms/batch, RMAT-22 100K mixed batches; learned vs plain GPU"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (kind, size, edges, wmax, config index, batches, (ins, del, rew))
+    "rmat22_mixed100k": ("rmat", 22, 67_108_864, 255, 1, 20, (40_000, 40_000, 20_000)),
+    "er64k_mixed1k": ("er", 65_536, 1_048_576, 100, 0, 10, (500, 500, 0)),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fused_kernel_bytes(c, n: int, num_changes: int, num_ins: int) -> int:
+    """Algorithmic HBM bytes of one persistent-kernel launch (update), from its work counters:
+    pack 24 B/vertex + unpack 20 B/vertex, 20 B per pointer-jumping visit, 20 B per invalidated
+    vertex, 12 B per change-list entry, 8 B per insertion head, 20 B per pull candidate plus
+    16 B per in-edge it scans, 20 B per expanded vertex + 16 B per scanned out-edge + 16 B per
+    improving atomicMin, 12 B per changed (distance, parent) pair written."""
+    return (44 * n + 20 * c.jump_items + 20 * c.invalidated + 12 * num_changes + 8 * num_ins
+            + 20 * c.pull_vertices + 16 * c.pull_edges
+            + 20 * c.relax_vertices + 16 * c.relax_edges + 16 * c.relax_improved + 12 * c.affected)
+
+
+def relax_bytes(c) -> int:
+    """B_relax = 20 P + 16 E_s + 16 I (the near-far propagation)."""
+    return 20 * c.relax_vertices + 16 * c.relax_edges + 16 * c.relax_improved
+
+
+def make_inputs(name: str, batches_needed: int):
+    from dyng_amd import generators as G
+
+    kind, size, m, wmax, cfg, nb, (ni, nd, nr) = WORKLOADS[name]
+    nb = max(nb, batches_needed)
+    t0 = time.perf_counter()
+    if kind == "rmat":
+        n = 1 << size
+        src, dst, w = G.rmat(size, m, wmax, G.GRAPH_SEED ^ cfg)
+        batches = G.mixed_batches(src, dst, n, nb, ni, nd, nr, wmax, G.BATCH_SEED ^ cfg, rmat_scale=size)
+    else:
+        n = size
+        src, dst, w = G.er(n, m, wmax, G.GRAPH_SEED ^ cfg)
+        batches = G.mixed_batches(src, dst, n, nb, ni, nd, nr, wmax, G.BATCH_SEED ^ cfg)
+    source = G.max_out_degree_vertex(src, n) if kind == "rmat" else 0
+    log(f"[bench] inputs {name}: n={n} m={m} batches={nb}x{ni + nd + nr} in {time.perf_counter() - t0:.1f}s")
+    return n, src, dst, w, batches, source, ni + nd + nr
+
+
+def cpu_baseline(n, src, dst, w, batches, source, sample: int, threads: int):
+    """The oracle's OpenMP engine (openmp.cpp restated) on `sample` batches: repair time per
+    batch (identify_affected + seed + loop + finalize) and the full host update (MOSP apply +
+    transpose + repair). Returns (summary, trees after each sampled batch)."""
+    import oracle
+
+    row = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(src, minlength=n), out=row[1:])
+    g = oracle.Csr(row, dst.astype(np.int32), w.astype(np.int32).reshape(1, -1))
+    ws = oracle.Workspace()
+    t0 = time.perf_counter()
+    tree, _ = oracle.compute(g, source, 0, "omp", threads=threads, ws=ws)
+    t_compute = time.perf_counter() - t0
+    repair, full, trees = [], [], []
+    for b in range(sample):
+        ins_s, ins_d, ins_w, del_s, del_d = batches[b]
+        t1 = time.perf_counter()
+        g, _, c = oracle.update(g, tree, ins_s, ins_d, ins_w.reshape(-1, 1), del_s, del_d, engine="omp",
+                                threads=threads, ws=ws)
+        full.append(time.perf_counter() - t1)
+        repair.append(sum(c.t_hooks))
+        trees.append((tree.dist.copy(), tree.parent.copy(), c.invalidated, c.affected))
+        log(f"[bench] cpu batch {b}: repair {repair[-1] * 1e3:.1f} ms, full update {full[-1] * 1e3:.0f} ms")
+    return {"compute_s": t_compute, "repair_ms": [x * 1e3 for x in repair], "update_ms": [x * 1e3 for x in full]}, trees
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=19)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat22_mixed100k", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-sample", type=int, default=3, help="batches timed on the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core of this host")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (the path does "
+              "not shard across GPUs in this release)", file=sys.stderr)
+        return 2
+
+    import dyng_amd
+    from dyng_amd import sssp
+
+    if dyng_amd.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        return 1
+    K, W = args.steps, args.warmup
+    n, src, dst, w, batches, source, ops = make_inputs(args.workload, K + W)
+
+    # ---- GPU: graph, initial tree, batches resident in HBM ----
+    t0 = time.perf_counter()
+    g = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
+    tree = sssp.compute(g, source)
+    dyng_amd.synchronize()
+    log(f"[bench] device graph + compute in {time.perf_counter() - t0:.1f}s")
+    D = dyng_amd.DeviceArray
+    dev_batches = [dyng_amd.EdgeBatch(insert=(D(b[0]), D(b[1]), D(b[2])), delete=(D(b[3]), D(b[4])))
+                   for b in batches[: K + W]]
+    for b in range(W):
+        sssp.update(g, dev_batches[b], tree)
+    dyng_amd.synchronize()
+    wall, stats = [], []
+    t_start = time.perf_counter()
+    for b in range(W, W + K):
+        t1 = time.perf_counter()
+        st = sssp.update(g, dev_batches[b], tree)
+        wall.append((time.perf_counter() - t1) * 1e3)
+        stats.append(st)
+    dyng_amd.synchronize()
+    total_s = time.perf_counter() - t_start
+    p50 = statistics.median(wall)
+    value = ops / (p50 / 1e3)
+    log(f"[bench] gpu: p50 {p50:.3f} ms/batch, mean {total_s / K * 1e3:.3f} ms, {value:.3e} edges/s")
+
+    # ---- roofline of the persistent kernel (HIP events on its stream) ----
+    kbytes = [fused_kernel_bytes(s.device, n, s.batch.deleted_edges + s.batch.ignored_deletions + ops,
+                                 ops) for s in stats]
+    kms = [s.device.kernel_ms for s in stats]
+    achieved = (sum(kbytes) / len(kbytes)) / (statistics.mean(kms) / 1e3) / 1e9
+    rbytes = [relax_bytes(s.device) for s in stats]
+    rms = [s.device.relax_ms for s in stats]
+    relax_gbs = (sum(rbytes) / max(sum(rms), 1e-9)) * 1e3 / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "sssp_fused_kernel", "kernel_ms_mean": round(statistics.mean(kms), 4),
+                "algorithmic_bytes_mean": int(sum(kbytes) / len(kbytes))}
+    relax_phase = {"achieved_gbs": round(relax_gbs, 1), "frac": round(relax_gbs / HBM_PEAK_GBS, 4),
+                   "relax_ms_mean": round(statistics.mean(rms), 4), "bytes_mean": int(sum(rbytes) / len(rbytes)),
+                   "teps": round(sum(s.device.relax_edges for s in stats) / max(sum(rms), 1e-9) * 1e3, 1)}
+
+    # ---- CPU baseline + bit-exactness on the sampled batches ----
+    cpu = None
+    exact = None
+    if not args.no_cpu and args.cpu_sample > 0:
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        summary, trees = cpu_baseline(n, src, dst, w, batches, source, args.cpu_sample, threads)
+        rep = statistics.median(summary["repair_ms"])
+        cpu = {"value": round(ops / (rep / 1e3), 1), "unit": "edges/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_sample} batches x {ops} ops after compute(); repair = identify_affected + "
+                         f"seed + loop + finalize of the OpenMP engine",
+               "p50_ms_repair": round(rep, 3), "p50_ms_update_incl_host_apply": round(
+                   statistics.median(summary["update_ms"]), 1), "compute_s": round(summary["compute_s"], 2)}
+        # the same batches on a fresh device graph, compared after each
+        g2 = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
+        t2 = sssp.compute(g2, source)
+        ok = True
+        for b in range(args.cpu_sample):
+            st = sssp.update(g2, dev_batches[b], t2)
+            d, p = t2.to_numpy()
+            cd, cp, inv, aff = trees[b]
+            same = bool(np.array_equal(d, cd) and np.array_equal(p, cp) and st.invalidated == inv
+                        and st.affected == aff)
+            ok = ok and same
+            log(f"[bench] parity batch {b}: {'bit-exact' if same else 'MISMATCH'}")
+        exact = {"batches_checked": args.cpu_sample, "bit_exact": ok, "against": "OpenMP engine (oracle)"}
+
+    s0 = stats[0].device
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "edges/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(total_s / K * 1e3, 4), "p50_ms": round(p50, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic",
+        "config": {"workload": args.workload, "vertices": int(n), "edges": int(len(src)), "batch_ops": ops,
+                   "source": int(source), "engine": "fused persistent (HIP, gfx950)",
+                   "apply_ms_p50": round(statistics.median(s.apply_ms for s in stats), 4),
+                   "engine_ms_p50": round(statistics.median(s.engine_ms for s in stats), 4),
+                   "blocks": int(s0.blocks)},
+        "roofline": roofline, "relax_phase": relax_phase, "cpu_baseline": cpu, "parity": exact,
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if (exact is None or exact["bit_exact"]) else 3
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,31 @@
+"""dyng_amd: an MI355X-native implementation of dynG's batch-dynamic SSSP path.
+
+``dyng_amd.sssp.compute`` / ``dyng_amd.sssp.update`` keep the reference's Python API
+(python/dyng/sssp.py) on a device-resident graph; the work runs in libdyng_hip.so (hand-written
+HIP for gfx950). There is no CPU fallback: without the built library every call raises
+NotSupportedError.
+"""
+
+from . import sssp
+from .batch import EdgeBatch
+from .device import DeviceArray, device_count, synchronize
+from .errors import (DeviceError, DyngError, InvalidArgumentError, NotSupportedError, OutOfMemoryError,
+                     StaleResultError)
+from .graph import ApplySummary, BatchSemantics, Graph, GraphProperties
+from .resources import Resources
+
+__version__ = "0.1.0"
+
+__all__ = ["sssp", "Graph", "GraphProperties", "BatchSemantics", "ApplySummary", "EdgeBatch", "Resources",
+           "DeviceArray", "device_count", "synchronize",
+           "DyngError", "InvalidArgumentError", "StaleResultError", "NotSupportedError", "OutOfMemoryError",
+           "DeviceError", "available", "__version__"]
+
+
+def available() -> bool:
+    """Whether the HIP library is built and a HIP device is visible (plugin protocol of
+    python/plugin/dyng_plugin/__init__.py)."""
+    try:
+        return device_count() > 0
+    except Exception:
+        return False

@@ -1,0 +1,580 @@
+// The host side of the HIP sssp path: the semantics of dyng::sssp::compute / update /
+// result::from_arrays (cpp/src/algorithms/sssp/sssp.cpp) around the device graph and the engine,
+// exported as the API layer of include/dyng_hip.h (the calls python/bindings/sssp.cpp binds).
+#include "internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace dyng_hip {
+
+namespace {
+thread_local std::string last_error;
+}
+
+void set_last_error(const std::string& m) { last_error = m; }
+
+void throw_hip(hipError_t e, const char* what, const char* file, int line) {
+  (void)hipGetLastError();
+  const int code = e == hipErrorOutOfMemory ? DYNG_HIP_OUT_OF_MEMORY : DYNG_HIP_RUNTIME_ERROR;
+  throw status_error(code, std::string("dyng: HIP error ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) +
+                               ") in " + what + " at " + file + ":" + std::to_string(line));
+}
+
+namespace {
+
+constexpr int64_t kInf = sssp_infinity;
+
+struct scoped_device {
+  int previous = 0;
+  explicit scoped_device(int d) {
+    DYNG_HIP_TRY(hipGetDevice(&previous));
+    if (previous != d) DYNG_HIP_TRY(hipSetDevice(d));
+  }
+  ~scoped_device() { (void)hipSetDevice(previous); }
+};
+
+/// The pooled device workspace of each device (ADR 0015: sized once, shared by every result run
+/// on the device; runs on one device are serialized by the pool's mutex).
+struct workspace_pool {
+  std::mutex lock;
+  std::map<int, dyng_hip_workspace*> per_device;
+  dyng_hip_workspace* get(int device) {
+    auto it = per_device.find(device);
+    if (it != per_device.end()) return it->second;
+    dyng_hip_workspace* ws = nullptr;
+    if (dyng_hip_workspace_create(device, &ws) != DYNG_HIP_OK) fail(DYNG_HIP_OUT_OF_MEMORY, dyng_hip_last_error());
+    per_device[device] = ws;
+    return ws;
+  }
+};
+
+workspace_pool& pool() {
+  static workspace_pool* p = new workspace_pool();  // lives until the process ends
+  return *p;
+}
+
+/// MOSP's defaultDelta (sssp.cpp:136-145).
+int64_t default_delta(int64_t m, int64_t n, int64_t weight_sum) {
+  if (m <= 0 || n <= 0) return 1;
+  const double average_weight = static_cast<double>(weight_sum) / static_cast<double>(m);
+  const double average_degree = static_cast<double>(m) / static_cast<double>(n);
+  return std::max<int64_t>(1, static_cast<int64_t>(32.0 * average_weight / average_degree));
+}
+
+/// sssp_distances_fit (sssp.cpp:147-152).
+bool distances_fit(int64_t n, int64_t max_weight) {
+  const u64 weight = static_cast<u64>(std::max<int64_t>(max_weight, 1));
+  const u64 hops = static_cast<u64>(std::max<int64_t>(n - 1, 1));
+  return weight <= output_max_distance / hops;
+}
+
+void call(int status) {
+  if (status != DYNG_HIP_OK) throw status_error(status, dyng_hip_last_error());
+}
+
+void expect_options(const dyng_hip_options& opt) {
+  if (opt.delta < 0) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: options.delta must be >= 0 (0 = automatic), got " + std::to_string(opt.delta));
+  if (opt.objective < 0) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: options.objective must be >= 0, got " + std::to_string(opt.objective));
+  if (opt.cuda_engine < 0 || opt.cuda_engine > 2) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: options.cuda_engine is not an engine");
+}
+
+void expect_graph(const dyng_hip_graph* g, const dyng_hip_options& opt, const char* what) {
+  if (g == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, std::string(what) + ": null graph");
+  if (opt.objective >= g->K) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, std::string(what) + ": options.objective " + std::to_string(opt.objective) +
+                                        " is out of range for a graph with " + std::to_string(g->K) + " weight column(s)");
+  }
+}
+
+}  // namespace
+
+}  // namespace dyng_hip
+
+/// The state behind a result handle (sssp_state, problem.hpp:275-312).
+struct dyng_hip_result {
+  int device = 0;
+  int32_t source = 0;
+  dyng_hip_options opt{};
+  uint64_t version = 0;
+  uint64_t graph_state = 0;
+  bool poisoned = false;
+  bool trusted = true;  ///< a tree this library computed, or an imported one that was validated
+  int64_t n = 0;
+  dyng_hip::dbuf<int64_t> distances;
+  dyng_hip::dbuf<int32_t> parents;
+  dyng_hip_tuning tuning{};
+};
+
+using namespace dyng_hip;
+
+namespace {
+
+void expect_usable(const dyng_hip_result* r, const char* what) {
+  if (r == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, std::string(what) + ": use of a moved-from result");
+  if (r->poisoned) {
+    throw status_error(DYNG_HIP_STALE_RESULT,
+                       std::string("dyng: ") + what +
+                           ": the result was left unusable by a failed update; recompute it with sssp::compute() "
+                           "(or import a valid tree with sssp::result::from_arrays())");
+  }
+}
+
+dyng_hip_sssp_run make_run(const dyng_hip_graph* g, dyng_hip_result* r, int64_t delta, int64_t max_weight) {
+  dyng_hip_sssp_run run{};
+  run.graph = g;
+  run.objective = r->opt.objective;
+  run.source = r->source;
+  run.delta = delta;
+  run.max_weight = max_weight;
+  run.distances = r->distances.p;
+  run.parents = r->parents.p;
+  run.check_cycles = r->trusted ? 0 : 1;
+  run.workspace = pool().get(g->device);
+  run.tuning = r->tuning;
+  if (r->opt.cuda_engine != DYNG_HIP_ENGINE_AUTOMATIC && run.tuning.engine == 0) run.tuning.engine = r->opt.cuda_engine;
+  return run;
+}
+
+/// Vertex growth: new vertices start unreachable (sssp.cpp:579-612).
+void grow_result(dyng_hip_result* r, int64_t n, hipStream_t s) {
+  if (n <= r->n) return;
+  dbuf<int64_t> d;
+  dbuf<int32_t> p;
+  d.reserve(n);
+  p.reserve(n);
+  if (r->n > 0) {
+    DYNG_HIP_TRY(hipMemcpyAsync(d.p, r->distances.p, r->n * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    DYNG_HIP_TRY(hipMemcpyAsync(p.p, r->parents.p, r->n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  }
+  std::vector<int64_t> hd(static_cast<size_t>(n - r->n), kInf);
+  std::vector<int32_t> hp(static_cast<size_t>(n - r->n), -1);
+  DYNG_HIP_TRY(hipMemcpyAsync(d.p + r->n, hd.data(), hd.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  DYNG_HIP_TRY(hipMemcpyAsync(p.p + r->n, hp.data(), hp.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  std::swap(r->distances.p, d.p);
+  std::swap(r->distances.n, d.n);
+  std::swap(r->parents.p, p.p);
+  std::swap(r->parents.n, p.n);
+  r->n = n;
+}
+
+void fill_stats(dyng_hip_stats* st, const dyng_hip_counters& c) {
+  st->affected = c.affected;
+  st->iterations = c.iterations;
+  st->frontier_visits = c.pushes;
+  st->fallback_used = 0;
+  st->converged = 1;
+  st->engine_used = c.engine_used;
+  st->packed_parents = c.packed_parents;
+  st->invalidated = c.invalidated;
+  st->epochs = c.epochs;
+  st->pushes = c.pushes;
+  st->device = c;
+}
+
+/// MOSP's canonicalizeTree() (sssp.cpp:385-404) on host rows.
+void canonicalize(int64_t n, const std::vector<int64_t>& row, const std::vector<int32_t>& col,
+                  const int32_t* w, int32_t source, const std::vector<int64_t>& d, std::vector<int32_t>& p) {
+  for (int64_t u = 0; u < n; ++u) {
+    const int64_t du = d[u];
+    if (du >= kInf / 2) continue;
+    for (int64_t e = row[u]; e < row[u + 1]; ++e) {
+      const int32_t v = col[e];
+      if (v != source && u < p[v] && du + static_cast<int64_t>(w[e]) == d[v]) p[v] = static_cast<int32_t>(u);
+    }
+  }
+}
+
+/// validate_tree_sequential (sssp.cpp:408-456): the first violation in vertex order.
+void validate_tree(int32_t source, const std::vector<int64_t>& d, const std::vector<int32_t>& p, int64_t bound) {
+  const auto n = static_cast<int64_t>(d.size());
+  if (d[source] != 0) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: the source " + std::to_string(source) + " must have distance 0, got " +
+                                        std::to_string(d[source]));
+  }
+  if (p[source] != -1) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: the source " + std::to_string(source) + " must have parent -1, got " +
+                                        std::to_string(p[source]));
+  }
+  for (int64_t v = 0; v < n; ++v) {
+    const int64_t dv = d[v];
+    const int32_t pv = p[v];
+    if (dv >= kInf / 2) {
+      if (pv != -1) {
+        fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: unreachable vertex " + std::to_string(v) + " has parent " +
+                                            std::to_string(pv) + " (expected -1)");
+      }
+      continue;
+    }
+    if (dv < 0 || dv > bound) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: the distance " + std::to_string(dv) + " of vertex " + std::to_string(v) +
+                                          " is outside [0, (n - 1) * max weight = " + std::to_string(bound) + "]");
+    }
+    if (v == source) continue;
+    if (pv < 0) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: reachable vertex " + std::to_string(v) + " (distance " +
+                                          std::to_string(dv) + ") has no parent");
+    }
+    if (d[pv] >= kInf / 2) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: vertex " + std::to_string(v) + " has the unreachable parent " +
+                                          std::to_string(pv));
+    }
+  }
+  std::vector<char> state(static_cast<size_t>(n), 0);
+  state[source] = 2;
+  std::vector<int32_t> path;
+  for (int64_t v = 0; v < n; ++v) {
+    if (state[v] != 0 || d[v] >= kInf / 2) continue;
+    path.clear();
+    int32_t u = static_cast<int32_t>(v);
+    while (state[u] == 0) {
+      state[u] = 1;
+      path.push_back(u);
+      u = p[u];
+    }
+    if (state[u] != 2) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: the input shortest-path tree has a parent cycle through vertex " +
+                                          std::to_string(u));
+    }
+    for (const int32_t x : path) state[x] = 2;
+  }
+}
+
+struct event_pair {
+  hipEvent_t a = nullptr, b = nullptr;
+  event_pair() {
+    DYNG_HIP_TRY(hipEventCreate(&a));
+    DYNG_HIP_TRY(hipEventCreate(&b));
+  }
+  ~event_pair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  double ms() const {
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, a, b);
+    return t;
+  }
+};
+
+}  // namespace
+
+extern "C" const char* dyng_hip_last_error(void) { return dyng_hip::last_error.c_str(); }
+
+extern "C" int dyng_hip_api_compute(dyng_hip_graph* g, int32_t source, const dyng_hip_options* opt_in,
+                                    void* stream, dyng_hip_result** out) {
+  return guarded([&] {
+    const dyng_hip_options opt = opt_in ? *opt_in : dyng_hip_options{0, 0, 0, 1};
+    expect_options(opt);
+    expect_graph(g, opt, "sssp::compute");
+    const int64_t n = g->n;
+    if (source < 0 || source >= n) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::compute: source " + std::to_string(source) + " is out of range [0, " +
+                                          std::to_string(n) + ")");
+    }
+    const int k = opt.objective;
+    if (g->wmin[k] < 1) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::compute: an edge has a weight below 1 in objective " + std::to_string(k) +
+                                          "; sssp needs weights in [1, 2^31 - 1]");
+    }
+    if (!distances_fit(n, g->wmax[k])) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::compute: distances up to " + std::to_string(g->wmax[k]) + " * " +
+                                          std::to_string(n - 1) + " do not fit in 62 bits");
+    }
+    scoped_device guard(g->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto r = std::make_unique<dyng_hip_result>();
+    r->device = g->device;
+    r->source = source;
+    r->opt = opt;
+    r->version = g->version;
+    r->graph_state = g->state_id;
+    r->n = n;
+    r->distances.reserve(n);
+    r->parents.reserve(n);
+    const int64_t delta = opt.delta > 0 ? opt.delta : default_delta(std::max<int64_t>(g->m, 1), n, g->wsum[k]);
+    std::lock_guard<std::mutex> lk(pool().lock);
+    dyng_hip_sssp_run run = make_run(g, r.get(), delta, g->wmax[k]);
+    dyng_hip_counters c{};
+    call(dyng_hip_sssp_compute(&run, s, &c));
+    *out = r.release();
+  });
+}
+
+extern "C" int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batch, dyng_hip_result* r,
+                                   void* stream, dyng_hip_stats* stats) {
+  return guarded([&] {
+    // ---- begin_update (sssp.cpp:620-640): nothing changes before these checks ----
+    expect_usable(r, "sssp::update");
+    if (g == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: null graph");
+    if (r->graph_state != g->state_id || r->version != g->version) {
+      throw status_error(DYNG_HIP_STALE_RESULT,
+                         "dyng: sssp::update: the result matches graph version " + std::to_string(r->version) +
+                             " but the graph is at version " + std::to_string(g->version) +
+                             " (or it belongs to another graph); recompute it");
+    }
+    if (r->device != g->device) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: the result is on another device");
+    expect_graph(g, r->opt, "sssp::update");
+    if (r->n != g->n) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: the result has " + std::to_string(r->n) +
+                                          " vertices but the graph has " + std::to_string(g->n));
+    }
+    scoped_device guard(g->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int k = r->opt.objective;
+    // ---- prepare on G_t (sssp.cpp:643-689) ----
+    batch_scan sc;
+    scan_batch(g, batch, s, sc);
+    if (batch->num_ins > 0 && sc.wmin[k] < 1) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: an insertion has a weight below 1 in objective " +
+                                          std::to_string(k) + "; sssp needs weights in [1, 2^31 - 1]");
+    }
+    const int64_t max_weight = std::max<int64_t>(std::max<int64_t>(sc.wmax[k], 1), g->wmax[k]);
+    const int64_t delta =
+        r->opt.delta > 0 ? r->opt.delta : default_delta(std::max<int64_t>(g->m, 1), g->n, g->wsum[k]);
+    if (!distances_fit(sc.n_after, max_weight)) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: distances up to " + std::to_string(max_weight) + " * " +
+                                          std::to_string(sc.n_after - 1) + " do not fit in 62 bits");
+    }
+    // ---- commit: the device apply ----
+    event_pair ev_apply, ev_engine;
+    dyng_hip_apply_summary summary{};
+    DYNG_HIP_TRY(hipEventRecord(ev_apply.a, s));
+    try {
+      apply_scanned(g, sc, s, &summary);
+    } catch (...) {
+      r->poisoned = true;  // the graph may be partly updated
+      throw;
+    }
+    DYNG_HIP_TRY(hipEventRecord(ev_apply.b, s));
+    // ---- resume + the engine; a failure from here on poisons the result ----
+    dyng_hip_counters c{};
+    try {
+      grow_result(r, g->n, s);
+      std::lock_guard<std::mutex> lk(pool().lock);
+      dyng_hip_sssp_run run = make_run(g, r, delta, max_weight);
+      DYNG_HIP_TRY(hipEventRecord(ev_engine.a, s));
+      call(dyng_hip_sssp_update(&run, s, &c));
+      DYNG_HIP_TRY(hipEventRecord(ev_engine.b, s));
+      DYNG_HIP_TRY(hipEventSynchronize(ev_engine.b));
+    } catch (...) {
+      r->poisoned = true;
+      throw;
+    }
+    r->version = g->version;
+    r->trusted = true;
+    if (stats != nullptr) {
+      std::memset(stats, 0, sizeof(*stats));
+      fill_stats(stats, c);
+      stats->batch = summary;
+      stats->apply_ms = ev_apply.ms();
+      stats->engine_ms = ev_engine.ms();
+    }
+  });
+}
+
+extern "C" int dyng_hip_api_from_arrays(dyng_hip_graph* g, int32_t source, const int64_t* distances,
+                                        const int32_t* parents, int32_t canonicalize_tree, const dyng_hip_options* opt_in,
+                                        void* stream, dyng_hip_result** out) {
+  return guarded([&] {
+    const dyng_hip_options opt = opt_in ? *opt_in : dyng_hip_options{0, 0, 0, 1};
+    expect_options(opt);
+    expect_graph(g, opt, "sssp::result::from_arrays");
+    const int64_t n = g->n;
+    if (source < 0 || source >= n) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::result::from_arrays: source " + std::to_string(source) +
+                                          " is out of range [0, " + std::to_string(n) + ")");
+    }
+    const int k = opt.objective;
+    if (g->wmin[k] < 1) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::result::from_arrays: an edge has a weight below 1 in objective " +
+                                          std::to_string(k));
+    }
+    // import (normalize_imported_tree, sssp.cpp:533-558)
+    std::vector<int64_t> d(distances, distances + n);
+    std::vector<int32_t> p(parents, parents + n);
+    for (int64_t v = 0; v < n; ++v) {
+      if (d[v] >= kInf / 2) d[v] = kInf;
+      if (p[v] < -1 || p[v] >= n) {
+        fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::result::from_arrays: vertex " + std::to_string(v) + " has parent " +
+                                            std::to_string(p[v]) + ", outside [-1, " + std::to_string(n) + ")");
+      }
+    }
+    if (canonicalize_tree) {
+      std::vector<int64_t> row(static_cast<size_t>(n) + 1);
+      std::vector<int32_t> col(static_cast<size_t>(std::max<int64_t>(g->m, 1)));
+      std::vector<int32_t> w(static_cast<size_t>(std::max<int64_t>(g->m, 1)) * g->K);
+      call(dyng_hip_graph_download(g, row.data(), col.data(), w.data()));
+      canonicalize(n, row, col, w.data() + static_cast<int64_t>(k) * g->m, source, d, p);
+    }
+    if (opt.validate_inputs) {
+      if (!distances_fit(n, g->wmax[k])) {
+        fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::result::from_arrays: distances up to " + std::to_string(g->wmax[k]) +
+                                            " * " + std::to_string(n - 1) + " do not fit in 62 bits");
+      }
+      validate_tree(source, d, p, g->wmax[k] * std::max<int64_t>(n - 1, 1));
+    }
+    scoped_device guard(g->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto r = std::make_unique<dyng_hip_result>();
+    r->device = g->device;
+    r->source = source;
+    r->opt = opt;
+    r->version = g->version;
+    r->graph_state = g->state_id;
+    r->trusted = opt.validate_inputs != 0;
+    r->n = n;
+    r->distances.reserve(n);
+    r->parents.reserve(n);
+    if (n > 0) {
+      DYNG_HIP_TRY(hipMemcpyAsync(r->distances.p, d.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, s));
+      DYNG_HIP_TRY(hipMemcpyAsync(r->parents.p, p.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    }
+    DYNG_HIP_TRY(hipStreamSynchronize(s));
+    *out = r.release();
+  });
+}
+
+extern "C" int dyng_hip_result_destroy(dyng_hip_result* r) {
+  return guarded([&] { delete r; });
+}
+
+extern "C" int dyng_hip_result_clone(const dyng_hip_result* r, void* stream, dyng_hip_result** out) {
+  return guarded([&] {
+    expect_usable(r, "sssp::result::clone");
+    scoped_device guard(r->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto c = std::make_unique<dyng_hip_result>();
+    c->device = r->device;
+    c->source = r->source;
+    c->opt = r->opt;
+    c->version = r->version;
+    c->graph_state = r->graph_state;
+    c->trusted = r->trusted;
+    c->tuning = r->tuning;
+    c->n = r->n;
+    c->distances.reserve(r->n);
+    c->parents.reserve(r->n);
+    if (r->n > 0) {
+      DYNG_HIP_TRY(hipMemcpyAsync(c->distances.p, r->distances.p, r->n * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+      DYNG_HIP_TRY(hipMemcpyAsync(c->parents.p, r->parents.p, r->n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    }
+    DYNG_HIP_TRY(hipStreamSynchronize(s));
+    *out = c.release();
+  });
+}
+
+extern "C" int dyng_hip_result_arrays(const dyng_hip_result* r, const int64_t** distances, const int32_t** parents,
+                                      int64_t* n) {
+  return guarded([&] {
+    expect_usable(r, "sssp::result::distances");
+    *distances = r->distances.p;
+    *parents = r->parents.p;
+    *n = r->n;
+  });
+}
+
+extern "C" int dyng_hip_result_source(const dyng_hip_result* r, int32_t* source) {
+  return guarded([&] { *source = r ? r->source : -1; });
+}
+
+extern "C" int dyng_hip_result_version(const dyng_hip_result* r, uint64_t* version) {
+  return guarded([&] { *version = r ? r->version : 0; });
+}
+
+extern "C" int dyng_hip_result_options(const dyng_hip_result* r, dyng_hip_options* opt) {
+  return guarded([&] {
+    expect_usable(r, "sssp::result::get_options");
+    *opt = r->opt;
+  });
+}
+
+extern "C" int dyng_hip_result_set_options(dyng_hip_result* r, const dyng_hip_options* opt) {
+  return guarded([&] {
+    expect_usable(r, "sssp::result::set_options");
+    expect_options(*opt);
+    if (opt->objective != r->opt.objective) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::result::set_options: the objective is fixed at compute() (" +
+                                          std::to_string(r->opt.objective) + "); got " + std::to_string(opt->objective));
+    }
+    r->opt = *opt;
+  });
+}
+
+extern "C" int dyng_hip_result_download(const dyng_hip_result* r, int64_t* distances, int32_t* parents) {
+  return guarded([&] {
+    expect_usable(r, "sssp::result::to_vector");
+    scoped_device guard(r->device);
+    if (r->n > 0) {
+      DYNG_HIP_TRY(hipMemcpy(distances, r->distances.p, r->n * sizeof(int64_t), hipMemcpyDeviceToHost));
+      DYNG_HIP_TRY(hipMemcpy(parents, r->parents.p, r->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+extern "C" int dyng_hip_result_set_tuning(dyng_hip_result* r, const dyng_hip_tuning* tuning) {
+  return guarded([&] {
+    if (r == nullptr || tuning == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: null argument");
+    r->tuning = *tuning;
+  });
+}
+
+extern "C" int dyng_hip_device_count(int* count) {
+  return guarded([&] {
+    int c = 0;
+    const hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      c = 0;
+    }
+    *count = c;
+  });
+}
+
+extern "C" int dyng_hip_malloc(int device, int64_t bytes, void** out) {
+  return guarded([&] {
+    scoped_device guard(device);
+    void* p = nullptr;
+    if (bytes > 0) {
+      const hipError_t e = hipMalloc(&p, static_cast<size_t>(bytes));
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(DYNG_HIP_OUT_OF_MEMORY, "dyng: out of device memory (" + std::to_string(bytes) + " bytes)");
+      }
+    }
+    *out = p;
+  });
+}
+
+extern "C" int dyng_hip_free(void* p) {
+  return guarded([&] {
+    if (p != nullptr) DYNG_HIP_TRY(hipFree(p));
+  });
+}
+
+extern "C" int dyng_hip_memcpy_htod(void* dst, const void* src, int64_t bytes) {
+  return guarded([&] {
+    if (bytes > 0) DYNG_HIP_TRY(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice));
+  });
+}
+
+extern "C" int dyng_hip_memcpy_dtoh(void* dst, const void* src, int64_t bytes) {
+  return guarded([&] {
+    if (bytes > 0) DYNG_HIP_TRY(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" int dyng_hip_synchronize(int device) {
+  return guarded([&] {
+    scoped_device guard(device);
+    DYNG_HIP_TRY(hipDeviceSynchronize());
+  });
+}

@@ -1,0 +1,114 @@
+// Shared device and host helpers of the HIP sssp path (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dyng_hip {
+
+using u64 = unsigned long long;
+constexpr u64 packed_inf = ~0ULL;
+/// DISTANCE_INF of the reference (INT64_MAX / 4): distances at or above half of it are
+/// unreachable (cpp/include/dyng/core/types.hpp:91).
+constexpr int64_t sssp_infinity = INT64_MAX / 4;
+/// The largest finite distance an output may hold (kernels.cuh:40).
+constexpr u64 output_max_distance = static_cast<u64>(sssp_infinity / 2 - 1);
+
+#define DYNG_DEV __device__ __forceinline__
+
+/// Packed (distance << bits | parent) words; bits == 0 means distance-only words
+/// (kernels.cuh:52-78). The all-ones parent field is "no parent"; the all-ones word is INF.
+struct packing {
+  int bits;
+  u64 no_parent;
+  DYNG_DEV u64 pack(u64 distance, int32_t parent) const {
+    if (bits == 0) return distance;
+    return (distance << bits) | (parent < 0 ? no_parent : static_cast<u64>(static_cast<uint32_t>(parent)));
+  }
+  DYNG_DEV u64 distance(u64 word) const { return word >> bits; }
+  DYNG_DEV int32_t parent(u64 word) const {
+    const u64 p = word & no_parent;
+    return p == no_parent ? -1 : static_cast<int32_t>(p);
+  }
+};
+
+// ---- agent-scope loads and stores of data other workgroups write during the same kernel ----
+
+template <typename T>
+DYNG_DEV T ld(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+DYNG_DEV void st(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+DYNG_DEV int lane() { return static_cast<int>(__lane_id()); }
+
+DYNG_DEV u64 lanes_below() {
+  const int l = lane();
+  return l == 0 ? 0ULL : (~0ULL >> (64 - l));
+}
+
+/// Reserve one slot of a shared list per lane with `pred`: one atomicAdd per wave (the active
+/// lanes with pred vote, the lowest of them reserves for all). Every active lane must call it.
+DYNG_DEV int wave_append(int* counter, bool pred) {
+  const u64 mask = __ballot(pred);
+  if (!pred) return -1;
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  const int rank = __popcll(mask & lanes_below());
+  int base = 0;
+  if (lane() == leader) base = atomicAdd(counter, __popcll(mask));
+  base = __shfl(base, leader);
+  return base + rank;
+}
+
+DYNG_DEV u64 wave_min(u64 v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const u64 o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+DYNG_DEV u64 wave_sum(u64 v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+/// Claim a vertex for a list in this generation: true for the first claimant only.
+DYNG_DEV bool claim(int* stamp, int32_t v, int generation) {
+  return atomicExch(&stamp[v], generation) != generation;
+}
+
+/// Grid-wide barrier of a persistent kernel whose workgroups are all resident (cooperative
+/// launch). Release fence, one arrival per workgroup on an agent-scope counter, the last arrival
+/// bumps the generation, acquire fence. Per-XCD L2s are written back and invalidated by the
+/// agent-scope fences, so data written before the barrier is visible after it on every XCD.
+struct grid_sync_state {
+  unsigned int count;
+  unsigned int generation;
+};
+
+DYNG_DEV void grid_barrier(grid_sync_state* s) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int g = __hip_atomic_load(&s->generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int arrived =
+        __hip_atomic_fetch_add(&s->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == gridDim.x - 1) {
+      __hip_atomic_store(&s->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&s->generation, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      while (__hip_atomic_load(&s->generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+}  // namespace dyng_hip

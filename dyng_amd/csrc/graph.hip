@@ -1,0 +1,1532 @@
+// The device graph of the HIP sssp path: out- and in-rows in slack storage, built on the device
+// and updated in place by each batch.
+//
+// Layout (HBM): per vertex an int2 {offset, length} and a capacity; one slot arena per direction
+// holding the neighbour ids, then one int32 weight column per objective (column k at
+// w + k * slot_cap). Every row is kept sorted by neighbour id, parallel edges in the order they
+// were added, so the first (u,v) of row u in MOSP's append order is the first entry of the run of
+// v in the sorted row. A row's capacity is len + ceil(len * headroom) + 2; a row that outgrows it
+// moves to the free tail of the arena; compaction rebuilds the arena when the tail runs short.
+//
+// A batch (MOSP applyChangeBatch under upsert_last_wins, cpp/src/graph/apply_host.cpp:57-401) is
+// applied without leaving the device:
+//   1. Step 0 (apply_host.cpp:80-147): self-loops, vertex growth, undirected expansion; one key
+//      (u << b | v) per effective operation, in application order (deletions first).
+//   2. A stable radix sort by key groups the operations of each pair (u,v) in batch order.
+//   3. One thread per pair replays its operations on the run of v in row u (binary search):
+//      deletions mark the first alive match dead, insertions overwrite the first alive match
+//      (upsert) or add a pending entry; then the weight-increase flags of :348-394 (final first
+//      (u,v) against the original first (u,v)). Each operation that changes an in-edge emits an
+//      in-operation for the in-rows.
+//   4. One workgroup per touched row merges the alive entries with the pending ones (positions
+//      from prefix sums and binary searches), staged through a scratch arena and written back in
+//      place, or moved to the tail when the row outgrew its capacity.
+//   5. Steps 2-4 again for the in-rows with the in-operations (a deletion or a weight change
+//      matches the first alive in-entry of u with the old weights).
+#include "internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace dyng_hip {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int32_t kDead = static_cast<int32_t>(0x80000000u);
+constexpr u64 kNoKey = ~0ULL;
+
+unsigned int blocks_for(int64_t work, int block = kBlock) {
+  int64_t b = (work + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > 65535LL * 16) b = 65535LL * 16;
+  return static_cast<unsigned int>(b);
+}
+
+int bits_for(int64_t n_after) {
+  int b = 1;
+  while ((int64_t{1} << b) <= n_after) ++b;  // n_after < 2^b, so ids < 2^b - 1 < the all-ones key
+  return b;
+}
+
+DYNG_DEV int32_t masked(int32_t x) { return x & 0x7fffffff; }
+
+/// First index in [0, len) whose (masked) neighbour is >= v.
+DYNG_DEV int lower_nbr(const int32_t* a, int len, int32_t v) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (masked(a[mid]) < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+/// First index in [0, len) whose (masked) neighbour is > v.
+DYNG_DEV int upper_nbr(const int32_t* a, int len, int32_t v) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (masked(a[mid]) <= v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+DYNG_DEV int64_t lower_key(const u64* keys, int64_t lo, int64_t hi, u64 key) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+DYNG_DEV int64_t upper_key(const u64* keys, int64_t lo, int64_t hi, u64 key) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] <= key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------------------------
+// rocPRIM wrappers
+// ---------------------------------------------------------------------------------------------
+
+void sort_pairs(dbuf<unsigned char>& tmp, const u64* kin, u64* kout, const uint32_t* vin,
+                uint32_t* vout, size_t n, int end_bit, hipStream_t s) {
+  if (n == 0) return;
+  size_t bytes = 0;
+  DYNG_HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0, end_bit, s));
+  tmp.reserve(bytes);
+  DYNG_HIP_TRY(rocprim::radix_sort_pairs(static_cast<void*>(tmp.p), bytes, kin, kout, vin, vout, n, 0,
+                                         end_bit, s));
+}
+
+template <typename T>
+void exclusive_scan(dbuf<unsigned char>& tmp, const T* in, T* out, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  size_t bytes = 0;
+  DYNG_HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, in, out, T{0}, n, rocprim::plus<T>(), s));
+  tmp.reserve(bytes);
+  DYNG_HIP_TRY(rocprim::exclusive_scan(static_cast<void*>(tmp.p), bytes, in, out, T{0}, n,
+                                       rocprim::plus<T>(), s));
+}
+
+template <typename T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < n; i += int64_t{gridDim.x} * blockDim.x) p[i] = v;
+}
+
+template <typename T>
+void fill(T* p, int64_t n, T v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<T>, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, n, v);
+  DYNG_HIP_TRY(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+// Construction
+// ---------------------------------------------------------------------------------------------
+
+/// One key per (expanded) input edge: (u << b | v), or kNoKey for a dropped self-loop or the
+/// missing reverse arc of a directed edge. val = slot (edge index * stride + reversed).
+__global__ void k_edge_keys(const int32_t* src, const int32_t* dst, int64_t count, int stride,
+                            int directed, int drop_self_loops, int vbits, u64* keys, uint32_t* vals) {
+  const int64_t total = count * stride;
+  for (int64_t e = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; e < total; e += int64_t{gridDim.x} * blockDim.x) {
+    const int64_t j = e / stride;
+    const int r = static_cast<int>(e % stride);
+    const int32_t u0 = src[j], v0 = dst[j];
+    bool valid = true;
+    if (r == 1 && (directed || u0 == v0)) valid = false;
+    if (u0 == v0 && drop_self_loops) valid = false;
+    const int32_t u = r ? v0 : u0, v = r ? u0 : v0;
+    keys[e] = valid ? ((static_cast<u64>(u) << vbits) | static_cast<u64>(v)) : kNoKey;
+    vals[e] = static_cast<uint32_t>(e);
+  }
+}
+
+/// keep[i]: sorted edge i survives (a simple graph keeps the last of a run of equal pairs).
+__global__ void k_keep(const u64* keys, int64_t N, int allow_parallel, int32_t* keep) {
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i <= N; i += int64_t{gridDim.x} * blockDim.x) {
+    if (i == N) { keep[i] = 0; continue; }
+    const u64 k = keys[i];
+    keep[i] = (k != kNoKey && (allow_parallel || i + 1 == N || keys[i + 1] != k)) ? 1 : 0;
+  }
+}
+
+/// Degree and capacity of every row from the sorted keys and the keep prefix.
+__global__ void k_row_degrees(const u64* keys, int64_t N, const int32_t* keep_prefix, int64_t n,
+                              int vbits, double headroom, int2* meta, int32_t* cap, int64_t* cap64) {
+  for (int64_t u = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; u < n; u += int64_t{gridDim.x} * blockDim.x) {
+    const int64_t a = lower_key(keys, 0, N, static_cast<u64>(u) << vbits);
+    const int64_t b = lower_key(keys, a, N, static_cast<u64>(u + 1) << vbits);
+    const int deg = keep_prefix[b] - keep_prefix[a];
+    const int c = deg + static_cast<int>(ceil(deg * headroom)) + 2;
+    meta[u] = make_int2(0, deg);
+    cap[u] = c;
+    cap64[u] = c;
+  }
+}
+
+__global__ void k_set_offsets(const int64_t* off64, int64_t n, int2* meta) {
+  for (int64_t u = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; u < n; u += int64_t{gridDim.x} * blockDim.x) {
+    meta[u].x = static_cast<int32_t>(off64[u]);
+  }
+}
+
+/// Scatter the kept sorted edges into their rows. Weights come from the input list
+/// (w_in[edge * K + k], edge = slot / stride) or, when `from_slots`, from an arena column
+/// (w_in[k * in_slot_cap + slot]).
+__global__ void k_fill_rows(const u64* keys, const uint32_t* vals, int64_t N, const int32_t* keep,
+                            const int32_t* keep_prefix, int vbits, int stride, const int2* meta,
+                            int32_t* nbr, int32_t* w, int64_t slot_cap, int K, const int32_t* w_in,
+                            int from_slots, int64_t in_slot_cap) {
+  const u64 mask = (1ULL << vbits) - 1;
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < N; i += int64_t{gridDim.x} * blockDim.x) {
+    if (!keep[i]) continue;
+    const u64 key = keys[i];
+    const int64_t u = static_cast<int64_t>(key >> vbits);
+    const int32_t v = static_cast<int32_t>(key & mask);
+    const int64_t a = lower_key(keys, 0, i + 1, static_cast<u64>(u) << vbits);
+    const int64_t slot = meta[u].x + (keep_prefix[i] - keep_prefix[a]);
+    nbr[slot] = v;
+    const uint32_t val = vals[i];
+    for (int k = 0; k < K; ++k) {
+      w[k * slot_cap + slot] = from_slots ? w_in[k * in_slot_cap + val] : w_in[(static_cast<int64_t>(val) / stride) * K + k];
+    }
+  }
+}
+
+/// Keys of the in-rows from the out-rows: (v << b | u), val = out slot.
+__global__ void k_reverse_keys(const int2* meta, const int32_t* nbr, int64_t n, int vbits, u64* keys,
+                               uint32_t* vals, const int64_t* row_pos) {
+  // one wave per row
+  const int64_t wave = (blockIdx.x * int64_t{blockDim.x} + threadIdx.x) / 64;
+  const int64_t waves = (int64_t{gridDim.x} * blockDim.x) / 64;
+  for (int64_t u = wave; u < n; u += waves) {
+    const int2 mt = meta[u];
+    const int64_t base = row_pos[u];
+    for (int j = lane(); j < mt.y; j += 64) {
+      const int32_t v = nbr[mt.x + j];
+      keys[base + j] = (static_cast<u64>(v) << vbits) | static_cast<u64>(u);
+      vals[base + j] = static_cast<uint32_t>(mt.x + j);
+    }
+  }
+}
+
+__global__ void k_len64(const int2* meta, int64_t n, int64_t* out) {
+  for (int64_t u = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; u < n; u += int64_t{gridDim.x} * blockDim.x) out[u] = meta[u].y;
+}
+
+/// Column summary of every objective: pass 0 max and sum, pass 1 the count of maximal weights.
+__global__ void k_column_summary(const int2* meta, const int32_t* w, int64_t n, int64_t slot_cap, int K,
+                                 int pass, apply_control* ctl) {
+  const int64_t wave = (blockIdx.x * int64_t{blockDim.x} + threadIdx.x) / 64;
+  const int64_t waves = (int64_t{gridDim.x} * blockDim.x) / 64;
+  for (int k = 0; k < K; ++k) {
+    long long mx = 0, sum = 0, cnt = 0, bad = 0;
+    const long long target = ctl->col_max[k];
+    for (int64_t u = wave; u < n; u += waves) {
+      const int2 mt = meta[u];
+      for (int j = lane(); j < mt.y; j += 64) {
+        const long long x = w[k * slot_cap + mt.x + j];
+        if (pass == 0) {
+          mx = x > mx ? x : mx;
+          sum += x;
+          bad += x < 1 ? 1 : 0;
+        } else {
+          cnt += x == target ? 1 : 0;
+        }
+      }
+    }
+    if (pass == 0) {
+      mx = static_cast<long long>(wave_min(~static_cast<u64>(mx)));
+      mx = static_cast<long long>(~static_cast<u64>(mx));
+      sum = static_cast<long long>(wave_sum(static_cast<u64>(sum)));
+      bad = static_cast<long long>(wave_sum(static_cast<u64>(bad)));
+      if (lane() == 0) {
+        atomicMax(reinterpret_cast<unsigned long long*>(&ctl->col_max[k]), static_cast<unsigned long long>(mx));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->col_dsum[k]), static_cast<unsigned long long>(sum));
+        if (bad) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->col_bad[k]), static_cast<unsigned long long>(bad));
+      }
+    } else {
+      cnt = static_cast<long long>(wave_sum(static_cast<u64>(cnt)));
+      if (lane() == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->col_dcount[k]), static_cast<unsigned long long>(cnt));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Apply
+// ---------------------------------------------------------------------------------------------
+
+/// Batch scan (before anything changes): id range, self-loops, weight range per objective.
+__global__ void k_batch_scan(const int32_t* ins_src, const int32_t* ins_dst, const int32_t* ins_w,
+                             int64_t num_ins, const int32_t* del_src, const int32_t* del_dst,
+                             int64_t num_del, int K, apply_control* ctl) {
+  int mn = INT32_MAX, mx = -1, sl_i = 0, sl_d = 0, wmin = INT32_MAX;
+  int lmax[4] = {0, 0, 0, 0};
+  const int64_t stride = int64_t{gridDim.x} * blockDim.x;
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < num_ins; i += stride) {
+    const int32_t u = ins_src[i], v = ins_dst[i];
+    mn = min(mn, min(u, v));
+    mx = max(mx, max(u, v));
+    sl_i += u == v ? 1 : 0;
+    for (int k = 0; k < K; ++k) {
+      const int32_t x = ins_w[i * K + k];
+      wmin = min(wmin, x);
+      if (k < 4) lmax[k] = max(lmax[k], x); else atomicMax(&ctl->batch_wmax[k], x);
+    }
+  }
+  for (int k = 0; k < 4 && k < K; ++k) {
+    if (lmax[k] > 0) atomicMax(&ctl->batch_wmax[k], lmax[k]);
+  }
+  // per-column minimum (rarely below 1; objective checks happen on the host)
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < num_ins; i += stride) {
+    for (int k = 0; k < K; ++k) {
+      const int32_t x = ins_w[i * K + k];
+      if (x < 1) atomicMin(&ctl->batch_wmin[k], x);
+    }
+  }
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < num_del; i += stride) {
+    const int32_t u = del_src[i], v = del_dst[i];
+    mn = min(mn, min(u, v));
+    sl_d += u == v ? 1 : 0;
+  }
+  atomicMin(&ctl->min_id, mn);
+  atomicMax(&ctl->max_id, mx);
+  if (sl_i) atomicAdd(&ctl->self_loops_ins, sl_i);
+  if (sl_d) atomicAdd(&ctl->self_loops_del, sl_d);
+  atomicMin(&ctl->min_weight, wmin);
+}
+
+struct op_layout {
+  int64_t num_first, num_second;  ///< batch operations of the first / second kind
+  int first_is_del;               ///< deletions_first
+  int stride;                     ///< 1 directed, 2 undirected (reverse arc slot)
+  DYNG_DEV bool is_del(int64_t e) const {
+    return (e < num_first * stride) ? first_is_del != 0 : first_is_del == 0;
+  }
+  DYNG_DEV int64_t local(int64_t e) const { return e < num_first * stride ? e : e - num_first * stride; }
+  DYNG_DEV int64_t del_base() const { return first_is_del ? 0 : num_first * stride; }
+  DYNG_DEV int64_t ins_base() const { return first_is_del ? num_first * stride : 0; }
+};
+
+__global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* ins_dst,
+                           const int32_t* del_src, const int32_t* del_dst, int directed,
+                           int drop_self_loops, int64_t n_after, int vbits, u64* keys, uint32_t* vals,
+                           int32_t* e_del_src, int32_t* e_del_dst, int32_t* e_ins_src,
+                           int32_t* e_ins_dst, uint8_t* e_inc, int K, apply_control* ctl) {
+  const int64_t total = (L.num_first + L.num_second) * L.stride;
+  unsigned long long dropped = 0, ignored = 0;
+  for (int64_t e = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; e < total; e += int64_t{gridDim.x} * blockDim.x) {
+    const bool del = L.is_del(e);
+    const int64_t loc = L.local(e);
+    const int64_t j = loc / L.stride;
+    const int r = static_cast<int>(loc % L.stride);
+    const int32_t u0 = del ? del_src[j] : ins_src[j];
+    const int32_t v0 = del ? del_dst[j] : ins_dst[j];
+    bool valid = true;
+    if (r == 1 && (directed || u0 == v0)) valid = false;
+    if (valid && u0 == v0 && drop_self_loops) {
+      valid = false;
+      dropped += 1;
+    }
+    const int32_t u = r ? v0 : u0, v = r ? u0 : v0;
+    if (valid && del && (u >= n_after || v >= n_after)) {
+      valid = false;
+      ignored += 1;
+    }
+    keys[e] = valid ? ((static_cast<u64>(u) << vbits) | static_cast<u64>(v)) : kNoKey;
+    vals[e] = static_cast<uint32_t>(e);
+    if (del) {
+      e_del_src[loc] = valid ? u : -1;
+      e_del_dst[loc] = valid ? v : -1;
+    } else {
+      e_ins_src[loc] = valid ? u : -1;
+      e_ins_dst[loc] = valid ? v : -1;
+      for (int k = 0; k < K; ++k) e_inc[loc * K + k] = 0;
+    }
+  }
+  dropped = wave_sum(dropped);
+  ignored = wave_sum(ignored);
+  if (lane() == 0) {
+    if (dropped) atomicAdd(&ctl->dropped_self_loops, dropped);
+    if (ignored) atomicAdd(&ctl->ignored_deletions, ignored);
+  }
+}
+
+struct out_pair_params {
+  op_layout L;
+  const u64* keys;
+  const uint32_t* vals;
+  int64_t N;
+  int vbits;
+  int K;
+  int64_t n_before;  ///< rows that existed before the batch (classification)
+  int upsert;        ///< on_existing_insert: 1 upsert, 0 ignore
+  const int32_t* ins_w;  ///< batch insertion weights (j * K + k)
+  int2* meta;
+  int32_t* nbr;
+  int32_t* w;
+  int64_t slot_cap;
+  int32_t* pend_alive;
+  int32_t* kill;
+  int32_t* pend_w;  ///< k * N + position
+  uint8_t* e_inc;
+  // in-operations, one per out-operation position
+  u64* ikeys;
+  uint32_t* ivals;
+  int32_t* iop_type;
+  int32_t* iop_wold;
+  int32_t* iop_wnew;
+  int32_t* mark;
+  int mark_gen;
+  int32_t* touched;
+  apply_control* ctl;
+};
+
+enum { kInDelete = 0, kInModify = 1, kInInsert = 2 };
+
+__global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
+  const int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x;
+  unsigned long long c_ins = 0, c_upd = 0, c_del = 0, c_ign_del = 0, c_ign_ins = 0;
+  long long dsum[4] = {0, 0, 0, 0}, dcnt[4] = {0, 0, 0, 0};
+  const u64 mask = (1ULL << p.vbits) - 1;
+  const int K = p.K;
+  if (i == p.N) {
+    p.pend_alive[i] = 0;
+    p.kill[i] = 0;
+  }
+  if (i < p.N) {
+    const u64 key = p.keys[i];
+    if (key == kNoKey) {
+      p.pend_alive[i] = 0;
+      p.kill[i] = 0;
+      p.ikeys[i] = kNoKey;
+      p.ivals[i] = static_cast<uint32_t>(i);
+    } else if (i == 0 || p.keys[i - 1] != key) {
+      int64_t j = i + 1;
+      while (j < p.N && p.keys[j] == key) ++j;
+      const int32_t u = static_cast<int32_t>(key >> p.vbits);
+      const int32_t v = static_cast<int32_t>(key & mask);
+      const int2 mt = p.meta[u];
+      const int32_t* row = p.nbr + mt.x;
+      const int lb = lower_nbr(row, mt.y, v);
+      const int ub = upper_nbr(row, mt.y, v);
+      int32_t first_init[max_weights];
+      const bool had = lb < ub && u < p.n_before;
+      if (had) {
+        for (int k = 0; k < K; ++k) first_init[k] = p.w[k * p.slot_cap + mt.x + lb];
+      }
+      bool structural = false;
+      auto account = [&](int k, long long wold, long long wnew, bool removed, bool added) {
+        const long long target = p.ctl->col_max[k];
+        const long long ds = (added ? wnew : 0) - (removed ? wold : 0);
+        const long long dc = ((added && wnew == target) ? 1 : 0) - ((removed && wold == target) ? 1 : 0);
+        if (k < 4) {
+          dsum[k] += ds;
+          dcnt[k] += dc;
+        } else {
+          if (ds) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dsum[k]), static_cast<unsigned long long>(ds));
+          if (dc) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dcount[k]), static_cast<unsigned long long>(dc));
+        }
+      };
+      for (int64_t q = i; q < j; ++q) {
+        const int64_t e = p.vals[q];
+        const bool del = p.L.is_del(e);
+        p.pend_alive[q] = 0;
+        p.kill[q] = 0;
+        p.ikeys[q] = kNoKey;
+        p.ivals[q] = static_cast<uint32_t>(q);
+        const u64 ikey = (static_cast<u64>(v) << p.vbits) | static_cast<u64>(u);
+        // first alive match: old entries of the run, then pending entries of this pair
+        int old_hit = -1;
+        for (int x = lb; x < ub; ++x) {
+          if (row[x] >= 0) { old_hit = x; break; }
+        }
+        int64_t pend_hit = -1;
+        if (old_hit < 0) {
+          for (int64_t x = i; x < q; ++x) {
+            if (p.pend_alive[x]) { pend_hit = x; break; }
+          }
+        }
+        if (del) {
+          if (old_hit >= 0) {
+            p.nbr[mt.x + old_hit] = v | kDead;
+            p.kill[q] = 1;
+            for (int k = 0; k < K; ++k) {
+              const int32_t wo = p.w[k * p.slot_cap + mt.x + old_hit];
+              p.iop_wold[k * p.N + q] = wo;
+              account(k, wo, 0, true, false);
+            }
+            p.ikeys[q] = ikey;
+            p.iop_type[q] = kInDelete;
+            structural = true;
+            ++c_del;
+          } else if (pend_hit >= 0) {
+            p.pend_alive[pend_hit] = 0;
+            for (int k = 0; k < K; ++k) {
+              const int32_t wo = p.pend_w[k * p.N + pend_hit];
+              p.iop_wold[k * p.N + q] = wo;
+              account(k, wo, 0, true, false);
+            }
+            p.ikeys[q] = ikey;
+            p.iop_type[q] = kInDelete;
+            ++c_del;
+          } else {
+            ++c_ign_del;
+          }
+        } else {
+          const int64_t bj = p.L.local(e) / p.L.stride;
+          if (old_hit >= 0 || pend_hit >= 0) {
+            if (!p.upsert) {
+              ++c_ign_ins;
+            } else {
+              for (int k = 0; k < K; ++k) {
+                const int32_t wn = p.ins_w[bj * K + k];
+                int32_t* slot = old_hit >= 0 ? &p.w[k * p.slot_cap + mt.x + old_hit] : &p.pend_w[k * p.N + pend_hit];
+                const int32_t wo = *slot;
+                *slot = wn;
+                p.iop_wold[k * p.N + q] = wo;
+                p.iop_wnew[k * p.N + q] = wn;
+                account(k, wo, wn, true, true);
+              }
+              p.ikeys[q] = ikey;
+              p.iop_type[q] = kInModify;
+              ++c_upd;
+            }
+          } else {
+            p.pend_alive[q] = 1;
+            for (int k = 0; k < K; ++k) {
+              const int32_t wn = p.ins_w[bj * K + k];
+              p.pend_w[k * p.N + q] = wn;
+              p.iop_wnew[k * p.N + q] = wn;
+              account(k, 0, wn, false, true);
+            }
+            p.ikeys[q] = ikey;
+            p.iop_type[q] = kInInsert;
+            structural = true;
+            ++c_ins;
+          }
+        }
+      }
+      // weight-increase classification: final first (u,v) against the original first (u,v)
+      if (had) {
+        const int32_t* wf = nullptr;
+        int64_t wf_stride = 0;
+        for (int x = lb; x < ub; ++x) {
+          if (row[x] >= 0) { wf = p.w + mt.x + x; wf_stride = p.slot_cap; break; }
+        }
+        if (wf == nullptr) {
+          for (int64_t x = i; x < j; ++x) {
+            if (p.pend_alive[x]) { wf = p.pend_w + x; wf_stride = p.N; break; }
+          }
+        }
+        if (wf != nullptr) {
+          for (int64_t q = i; q < j; ++q) {
+            const int64_t e = p.vals[q];
+            if (p.L.is_del(e)) continue;
+            const int64_t loc = p.L.local(e);
+            for (int k = 0; k < K; ++k) {
+              p.e_inc[loc * K + k] = wf[k * wf_stride] > first_init[k] ? 1 : 0;
+            }
+          }
+        }
+      }
+      if (structural && atomicExch(&p.mark[u], p.mark_gen) != p.mark_gen) {
+        const int t = atomicAdd(&p.ctl->touched_out, 1);
+        p.touched[t] = u;
+      }
+    }
+  }
+  c_ins = wave_sum(c_ins);
+  c_upd = wave_sum(c_upd);
+  c_del = wave_sum(c_del);
+  c_ign_del = wave_sum(c_ign_del);
+  c_ign_ins = wave_sum(c_ign_ins);
+  for (int k = 0; k < 4 && k < K; ++k) {
+    dsum[k] = static_cast<long long>(wave_sum(static_cast<u64>(dsum[k])));
+    dcnt[k] = static_cast<long long>(wave_sum(static_cast<u64>(dcnt[k])));
+  }
+  if (lane() == 0) {
+    if (c_ins) atomicAdd(&p.ctl->inserted_edges, c_ins);
+    if (c_upd) atomicAdd(&p.ctl->updated_edges, c_upd);
+    if (c_del) atomicAdd(&p.ctl->deleted_edges, c_del);
+    if (c_ign_del) atomicAdd(&p.ctl->ignored_deletions, c_ign_del);
+    if (c_ign_ins) atomicAdd(&p.ctl->ignored_insertions, c_ign_ins);
+    for (int k = 0; k < 4 && k < K; ++k) {
+      if (dsum[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dsum[k]), static_cast<unsigned long long>(dsum[k]));
+      if (dcnt[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dcount[k]), static_cast<unsigned long long>(dcnt[k]));
+    }
+  }
+}
+
+struct in_pair_params {
+  const u64* keys;  ///< sorted in-operation keys (v << b | u)
+  const uint32_t* vals;  ///< out-operation position of each
+  int64_t N;
+  int vbits;
+  int K;
+  const int32_t* iop_type;
+  const int32_t* iop_wold;
+  const int32_t* iop_wnew;
+  int2* meta;
+  int32_t* nbr;
+  int32_t* w;
+  int64_t slot_cap;
+  int32_t* pend_alive;
+  int32_t* kill;
+  int32_t* pend_w;
+  int32_t* mark;
+  int mark_gen;
+  int32_t* touched;
+  apply_control* ctl;
+};
+
+__global__ void __launch_bounds__(kBlock) k_in_pairs(in_pair_params p) {
+  const int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x;
+  if (i == p.N) {
+    p.pend_alive[i] = 0;
+    p.kill[i] = 0;
+  }
+  if (i >= p.N) return;
+  const u64 key = p.keys[i];
+  if (key == kNoKey) {
+    p.pend_alive[i] = 0;
+    p.kill[i] = 0;
+    return;
+  }
+  if (i > 0 && p.keys[i - 1] == key) return;
+  int64_t j = i + 1;
+  while (j < p.N && p.keys[j] == key) ++j;
+  const u64 mask = (1ULL << p.vbits) - 1;
+  const int32_t v = static_cast<int32_t>(key >> p.vbits);
+  const int32_t u = static_cast<int32_t>(key & mask);
+  const int2 mt = p.meta[v];
+  const int32_t* row = p.nbr + mt.x;
+  const int lb = lower_nbr(row, mt.y, u);
+  const int ub = upper_nbr(row, mt.y, u);
+  const int K = p.K;
+  bool structural = false;
+  for (int64_t q = i; q < j; ++q) {
+    p.pend_alive[q] = 0;
+    p.kill[q] = 0;
+    const uint32_t o = p.vals[q];
+    const int type = p.iop_type[o];
+    if (type == kInInsert) {
+      p.pend_alive[q] = 1;
+      for (int k = 0; k < K; ++k) p.pend_w[k * p.N + q] = p.iop_wnew[k * p.N + o];
+      structural = true;
+      continue;
+    }
+    // first alive entry of u with the old weights: old entries, then pending ones
+    int old_hit = -1;
+    for (int x = lb; x < ub && old_hit < 0; ++x) {
+      if (row[x] < 0) continue;
+      bool same = true;
+      for (int k = 0; k < K && same; ++k) same = p.w[k * p.slot_cap + mt.x + x] == p.iop_wold[k * p.N + o];
+      if (same) old_hit = x;
+    }
+    int64_t pend_hit = -1;
+    if (old_hit < 0) {
+      for (int64_t x = i; x < q && pend_hit < 0; ++x) {
+        if (!p.pend_alive[x]) continue;
+        bool same = true;
+        for (int k = 0; k < K && same; ++k) same = p.pend_w[k * p.N + x] == p.iop_wold[k * p.N + o];
+        if (same) pend_hit = x;
+      }
+    }
+    if (old_hit < 0 && pend_hit < 0) {
+      atomicOr(&p.ctl->error_bits, static_cast<int>(DYNG_HIP_DEVICE_ERROR_INTERNAL));
+      continue;
+    }
+    if (type == kInDelete) {
+      if (old_hit >= 0) {
+        p.nbr[mt.x + old_hit] = u | kDead;
+        p.kill[q] = 1;
+        structural = true;
+      } else {
+        p.pend_alive[pend_hit] = 0;
+      }
+    } else {
+      for (int k = 0; k < K; ++k) {
+        const int32_t wn = p.iop_wnew[k * p.N + o];
+        if (old_hit >= 0) p.w[k * p.slot_cap + mt.x + old_hit] = wn; else p.pend_w[k * p.N + pend_hit] = wn;
+      }
+    }
+  }
+  if (structural && atomicExch(&p.mark[v], p.mark_gen) != p.mark_gen) {
+    const int t = atomicAdd(&p.ctl->touched_in, 1);
+    p.touched[t] = v;
+  }
+}
+
+struct merge_params {
+  int2* meta;
+  int32_t* cap;
+  int32_t* nbr;
+  int32_t* w;
+  int64_t slot_cap;
+  int K;
+  const u64* keys;
+  int64_t N;
+  int vbits;
+  const int32_t* pend_alive;
+  const int32_t* pend_prefix;
+  const int32_t* kill_prefix;
+  const int32_t* pend_w;
+  const int32_t* touched;
+  const int* touched_count;
+  int32_t* stage_nbr;
+  int32_t* stage_w;
+  int64_t stage_cap;
+  long long* stage_used;
+  long long* tail;
+  long long* garbage;
+  unsigned long long* relocated;
+  int* error_bits;
+  double headroom;
+};
+
+/// One workgroup per touched row: the alive entries and the pending ones, merged by neighbour id
+/// (existing entries before new ones of the same neighbour), written in place through the
+/// staging arena, or to a new slot range at the tail when the row outgrew its capacity.
+__global__ void __launch_bounds__(kBlock) k_merge_rows(merge_params p) {
+  __shared__ int s_wave_tot[kBlock / 64];
+  __shared__ long long s_dest;
+  __shared__ int s_staged, s_newcap, s_relocated;
+  __shared__ int64_t s_a, s_b;
+  const int count = *p.touched_count;
+  const u64 mask = (1ULL << p.vbits) - 1;
+  const int wid = threadIdx.x / 64;
+  for (int t = blockIdx.x; t < count; t += gridDim.x) {
+    const int32_t u = p.touched[t];
+    const int2 mt = p.meta[u];
+    if (threadIdx.x == 0) {
+      const int64_t a = lower_key(p.keys, 0, p.N, static_cast<u64>(u) << p.vbits);
+      const int64_t b = lower_key(p.keys, a, p.N, static_cast<u64>(u + 1) << p.vbits);
+      const int P = p.pend_prefix[b] - p.pend_prefix[a];
+      const int Kc = p.kill_prefix[b] - p.kill_prefix[a];
+      const int new_len = mt.y - Kc + P;
+      const int c = p.cap[u];
+      s_a = a;
+      s_b = b;
+      s_staged = 0;
+      s_relocated = 0;
+      s_newcap = c;
+      bool done = false;
+      if (new_len <= c) {
+        const long long s = atomicAdd(reinterpret_cast<unsigned long long*>(p.stage_used), static_cast<unsigned long long>(new_len));
+        if (s + new_len <= p.stage_cap) {
+          s_staged = 1;
+          s_dest = s;
+          done = true;
+        }
+      }
+      if (!done) {
+        const int nc = new_len + static_cast<int>(ceil(new_len * p.headroom)) + 2;
+        s_newcap = nc;
+        s_dest = static_cast<long long>(atomicAdd(reinterpret_cast<unsigned long long*>(p.tail), static_cast<unsigned long long>(nc)));
+        s_relocated = 1;
+        if (s_dest + nc > p.slot_cap) {
+          // the host sizes the tail for the worst case; never write past the arena
+          atomicOr(p.error_bits, static_cast<int>(DYNG_HIP_DEVICE_ERROR_CAPACITY));
+          s_dest = -1;
+        } else {
+          atomicAdd(reinterpret_cast<unsigned long long*>(p.garbage), static_cast<unsigned long long>(c));
+          atomicAdd(p.relocated, 1ULL);
+        }
+      }
+    }
+    __syncthreads();
+    if (s_dest < 0) {
+      __syncthreads();
+      continue;
+    }
+    const int64_t a = s_a, b = s_b;
+    const long long dest = s_dest;
+    const bool staged = s_staged != 0;
+    int32_t* tn = staged ? p.stage_nbr : p.nbr;
+    int32_t* tw = staged ? p.stage_w : p.w;
+    const int64_t tstride = staged ? p.stage_cap : p.slot_cap;
+    const int pa = p.pend_prefix[a];
+    const int ka = p.kill_prefix[a];
+    // alive old entries
+    int carry = 0;
+    for (int base = 0; base < mt.y; base += kBlock) {
+      const int jx = base + static_cast<int>(threadIdx.x);
+      const int32_t x = jx < mt.y ? p.nbr[mt.x + jx] : -1;
+      const bool alive = x >= 0;
+      const u64 ballot = __ballot(alive);
+      if (lane() == 0) s_wave_tot[wid] = __popcll(ballot);
+      __syncthreads();
+      int before = 0, total = 0;
+      for (int q = 0; q < kBlock / 64; ++q) {
+        before += q < wid ? s_wave_tot[q] : 0;
+        total += s_wave_tot[q];
+      }
+      if (alive) {
+        const int pre = before + __popcll(ballot & lanes_below());
+        const int64_t lt = lower_key(p.keys, a, b, (static_cast<u64>(u) << p.vbits) | static_cast<u64>(x));
+        const int64_t pos = dest + carry + pre + (p.pend_prefix[lt] - pa);
+        tn[pos] = x;
+        for (int k = 0; k < p.K; ++k) tw[k * tstride + pos] = p.w[k * p.slot_cap + mt.x + jx];
+      }
+      carry += total;
+      __syncthreads();
+    }
+    // pending entries
+    for (int64_t q = a + threadIdx.x; q < b; q += kBlock) {
+      if (!p.pend_alive[q]) continue;
+      const u64 key = p.keys[q];
+      const int32_t v = static_cast<int32_t>(key & mask);
+      const int y = upper_nbr(p.nbr + mt.x, mt.y, v);
+      const int64_t se = upper_key(p.keys, q, b, key);
+      const int killed = p.kill_prefix[se] - ka;
+      const int64_t pos = dest + (y - killed) + (p.pend_prefix[q] - pa);
+      tn[pos] = v;
+      for (int k = 0; k < p.K; ++k) tw[k * tstride + pos] = p.pend_w[k * p.N + q];
+    }
+    __syncthreads();
+    const int new_len = carry + (p.pend_prefix[b] - pa);
+    if (staged) {
+      for (int x = threadIdx.x; x < new_len; x += kBlock) {
+        p.nbr[mt.x + x] = p.stage_nbr[dest + x];
+        for (int k = 0; k < p.K; ++k) p.w[k * p.slot_cap + mt.x + x] = p.stage_w[k * p.stage_cap + dest + x];
+      }
+    }
+    if (threadIdx.x == 0) {
+      p.meta[u] = make_int2(staged ? mt.x : static_cast<int32_t>(dest), new_len);
+      if (s_relocated) p.cap[u] = s_newcap;
+    }
+    __syncthreads();
+  }
+}
+
+/// Copy one row store into a fresh arena (compaction): one wave per row.
+__global__ void k_compact_rows(const int2* meta, const int32_t* nbr, const int32_t* w, int64_t slot_cap,
+                               int64_t n, int K, const int64_t* new_off, int2* new_meta, int32_t* new_nbr,
+                               int32_t* new_w, int64_t new_slot_cap) {
+  const int64_t wave = (blockIdx.x * int64_t{blockDim.x} + threadIdx.x) / 64;
+  const int64_t waves = (int64_t{gridDim.x} * blockDim.x) / 64;
+  for (int64_t u = wave; u < n; u += waves) {
+    const int2 mt = meta[u];
+    const int64_t o = new_off[u];
+    for (int j = lane(); j < mt.y; j += 64) {
+      new_nbr[o + j] = nbr[mt.x + j];
+      for (int k = 0; k < K; ++k) new_w[k * new_slot_cap + o + j] = w[k * slot_cap + mt.x + j];
+    }
+    if (lane() == 0) new_meta[u] = make_int2(static_cast<int32_t>(o), mt.y);
+  }
+}
+
+__global__ void k_caps_from_len(const int2* meta, int64_t n, double headroom, int32_t* cap, int64_t* cap64) {
+  for (int64_t u = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; u < n; u += int64_t{gridDim.x} * blockDim.x) {
+    const int len = meta[u].y;
+    const int c = len + static_cast<int>(ceil(len * headroom)) + 2;
+    cap[u] = c;
+    cap64[u] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------------
+
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+struct scoped_device {
+  int previous = 0;
+  explicit scoped_device(int d) {
+    DYNG_HIP_TRY(hipGetDevice(&previous));
+    if (previous != d) DYNG_HIP_TRY(hipSetDevice(d));
+  }
+  ~scoped_device() { (void)hipSetDevice(previous); }
+};
+
+int64_t arena_slots(int64_t rows_slots, int64_t m, int64_t extra) {
+  return rows_slots + std::max<int64_t>(m + m / 2, 1 << 16) + extra;
+}
+
+void check_slots(int64_t slots) {
+  if (slots >= (int64_t{1} << 31) - 1) {
+    fail(DYNG_HIP_NOT_SUPPORTED,
+         "dyng: the HIP device graph holds at most 2^31 - 1 edge slots per direction (needed " +
+             std::to_string(slots) + ")");
+  }
+}
+
+/// Build one row store from sorted keys (u << b | v) with keep flags and weights.
+void build_store(dyng_hip_graph* g, row_store& rs, const u64* keys, const uint32_t* vals, int64_t N,
+                 const int32_t* keep, int vbits, int stride, const int32_t* w_in, int from_slots,
+                 int64_t in_slot_cap, hipStream_t s) {
+  const int64_t n = g->n_cap;
+  dbuf<int32_t> keep_prefix;
+  keep_prefix.reserve(N + 1);
+  exclusive_scan<int32_t>(g->scan_tmp, keep, keep_prefix.p, N + 1, s);
+  rs.meta.reserve(n);
+  rs.cap.reserve(n);
+  fill<int2>(rs.meta.p, n, make_int2(0, 0), s);
+  fill<int32_t>(rs.cap.p, n, 0, s);
+  dbuf<int64_t> cap64, off64;
+  cap64.reserve(g->n + 1);
+  off64.reserve(g->n + 1);
+  fill<int64_t>(cap64.p, g->n + 1, 0, s);
+  hipLaunchKernelGGL(k_row_degrees, dim3(blocks_for(g->n)), dim3(kBlock), 0, s, keys, N, keep_prefix.p,
+                     g->n, vbits, g->props.headroom, rs.meta.p, rs.cap.p, cap64.p);
+  DYNG_HIP_TRY(hipGetLastError());
+  exclusive_scan<int64_t>(g->scan_tmp, cap64.p, off64.p, g->n + 1, s);
+  int64_t rows_slots = 0;
+  DYNG_HIP_TRY(hipMemcpyAsync(&rows_slots, off64.p + g->n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  hipLaunchKernelGGL(k_set_offsets, dim3(blocks_for(g->n)), dim3(kBlock), 0, s, off64.p, g->n, rs.meta.p);
+  const int64_t slots = arena_slots(rows_slots, N, 0);
+  check_slots(slots);
+  rs.slot_cap = slots;
+  rs.tail = rows_slots;
+  rs.garbage = 0;
+  rs.nbr.reserve(slots);
+  rs.w.reserve(slots * g->K);
+  hipLaunchKernelGGL(k_fill_rows, dim3(blocks_for(N)), dim3(kBlock), 0, s, keys, vals, N, keep,
+                     keep_prefix.p, vbits, stride, rs.meta.p, rs.nbr.p, rs.w.p, rs.slot_cap, g->K, w_in,
+                     from_slots, in_slot_cap);
+  DYNG_HIP_TRY(hipGetLastError());
+}
+
+/// In-rows from the out-rows.
+void build_in_store(dyng_hip_graph* g, hipStream_t s) {
+  const int64_t n = g->n;
+  const int vbits = bits_for(std::max<int64_t>(n, 1));
+  dbuf<int64_t> len64, pos64;
+  len64.reserve(n + 1);
+  pos64.reserve(n + 1);
+  fill<int64_t>(len64.p, n + 1, 0, s);
+  hipLaunchKernelGGL(k_len64, dim3(blocks_for(n)), dim3(kBlock), 0, s, g->out.meta.p, n, len64.p);
+  exclusive_scan<int64_t>(g->scan_tmp, len64.p, pos64.p, n + 1, s);
+  const int64_t M = g->m;
+  dbuf<u64> ka, kb;
+  dbuf<uint32_t> va, vb;
+  dbuf<int32_t> keep;
+  ka.reserve(M);
+  kb.reserve(M);
+  va.reserve(M);
+  vb.reserve(M);
+  keep.reserve(M + 1);
+  hipLaunchKernelGGL(k_reverse_keys, dim3(blocks_for(n * 64)), dim3(kBlock), 0, s, g->out.meta.p, g->out.nbr.p, n,
+                     vbits, ka.p, va.p, pos64.p);
+  DYNG_HIP_TRY(hipGetLastError());
+  sort_pairs(g->sort_tmp, ka.p, kb.p, va.p, vb.p, M, 2 * vbits, s);
+  hipLaunchKernelGGL(k_keep, dim3(blocks_for(M + 1)), dim3(kBlock), 0, s, kb.p, M, 1, keep.p);
+  build_store(g, g->in, kb.p, vb.p, M, keep.p, vbits, 1, g->out.w.p, 1, g->out.slot_cap, s);
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+}
+
+/// Exact column summaries (max, sum, count of the max) of every objective.
+void recompute_columns(dyng_hip_graph* g, hipStream_t s) {
+  apply_control* c = g->host_ctl.p;
+  std::memset(c, 0, sizeof(apply_control));
+  DYNG_HIP_TRY(hipMemcpyAsync(g->ctl.p, c, sizeof(apply_control), hipMemcpyHostToDevice, s));
+  const unsigned grid = blocks_for(g->n * 64);
+  hipLaunchKernelGGL(k_column_summary, dim3(grid), dim3(kBlock), 0, s, g->out.meta.p, g->out.w.p, g->n,
+                     g->out.slot_cap, g->K, 0, g->ctl.p);
+  DYNG_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_column_summary, dim3(grid), dim3(kBlock), 0, s, g->out.meta.p, g->out.w.p, g->n,
+                     g->out.slot_cap, g->K, 1, g->ctl.p);
+  DYNG_HIP_TRY(hipGetLastError());
+  DYNG_HIP_TRY(hipMemcpyAsync(c, g->ctl.p, sizeof(apply_control), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  g->wmax.assign(g->K, 1);
+  g->wsum.assign(g->K, 0);
+  g->wmax_count.assign(g->K, 0);
+  g->wmin.assign(g->K, 1);
+  for (int k = 0; k < g->K; ++k) {
+    g->wmin[k] = c->col_bad[k] == 0 ? 1 : 0;
+    g->wmax[k] = std::max<long long>(1, c->col_max[k]);
+    g->wsum[k] = c->col_dsum[k];
+    g->wmax_count[k] = c->col_max[k] >= 1 ? c->col_dcount[k] : 0;
+  }
+}
+
+void compact_store(dyng_hip_graph* g, row_store& rs, int64_t extra, hipStream_t s) {
+  const int64_t n = g->n;
+  dbuf<int64_t> cap64, off64;
+  cap64.reserve(n + 1);
+  off64.reserve(n + 1);
+  fill<int64_t>(cap64.p, n + 1, 0, s);
+  hipLaunchKernelGGL(k_caps_from_len, dim3(blocks_for(n)), dim3(kBlock), 0, s, rs.meta.p, n, g->props.headroom,
+                     rs.cap.p, cap64.p);
+  exclusive_scan<int64_t>(g->scan_tmp, cap64.p, off64.p, n + 1, s);
+  int64_t rows_slots = 0;
+  DYNG_HIP_TRY(hipMemcpyAsync(&rows_slots, off64.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  const int64_t slots = arena_slots(rows_slots, g->m, extra);
+  check_slots(slots);
+  row_store fresh;
+  fresh.meta.reserve(g->n_cap);
+  fresh.nbr.reserve(slots);
+  fresh.w.reserve(slots * g->K);
+  fill<int2>(fresh.meta.p, g->n_cap, make_int2(0, 0), s);
+  hipLaunchKernelGGL(k_compact_rows, dim3(blocks_for(n * 64)), dim3(kBlock), 0, s, rs.meta.p, rs.nbr.p, rs.w.p,
+                     rs.slot_cap, n, g->K, off64.p, fresh.meta.p, fresh.nbr.p, fresh.w.p, slots);
+  DYNG_HIP_TRY(hipGetLastError());
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  std::swap(rs.meta.p, fresh.meta.p);
+  std::swap(rs.meta.n, fresh.meta.n);
+  std::swap(rs.nbr.p, fresh.nbr.p);
+  std::swap(rs.nbr.n, fresh.nbr.n);
+  std::swap(rs.w.p, fresh.w.p);
+  std::swap(rs.w.n, fresh.w.n);
+  rs.slot_cap = slots;
+  rs.tail = rows_slots;
+  rs.garbage = 0;
+}
+
+void grow_vertices(dyng_hip_graph* g, int64_t n_after, hipStream_t s) {
+  if (n_after <= g->n) return;
+  if (n_after > g->n_cap) {
+    const int64_t cap = std::max<int64_t>(n_after, g->n_cap + g->n_cap / 2);
+    for (row_store* rs : {&g->out, &g->in}) {
+      rs->meta.reserve(cap, true, s);
+      rs->cap.reserve(cap, true, s);
+    }
+    g->mark_out.reserve(cap, true, s);
+    g->mark_in.reserve(cap, true, s);
+    fill<int32_t>(g->mark_out.p + g->n_cap, cap - g->n_cap, 0, s);
+    fill<int32_t>(g->mark_in.p + g->n_cap, cap - g->n_cap, 0, s);
+    g->n_cap = cap;
+  }
+  for (row_store* rs : {&g->out, &g->in}) {
+    fill<int2>(rs->meta.p + g->n, n_after - g->n, make_int2(0, 0), s);
+    fill<int32_t>(rs->cap.p + g->n, n_after - g->n, 0, s);
+  }
+}
+
+dyng_hip_graph* create_from_edge_list(int device, const dyng_hip_graph_props* props, int64_t n, int64_t count,
+                                      int32_t K, const int32_t* src, const int32_t* dst, const int32_t* w,
+                                      int32_t from_device, hipStream_t s) {
+  if (props == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: graph properties are required");
+  if (n < 0 || n >= INT32_MAX) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: the HIP backend takes int32 vertex ids");
+  if (K < 1 || K > max_weights) {
+    fail(DYNG_HIP_NOT_SUPPORTED, "dyng: the HIP sssp backend takes 1.." + std::to_string(max_weights) +
+                                     " weight columns, got " + std::to_string(K));
+  }
+  if (props->on_existing_insert == 1 || props->on_missing_delete == 1) {
+    fail(DYNG_HIP_NOT_SUPPORTED,
+         "dyng: the HIP device apply supports on_existing_insert = upsert or ignore and "
+         "on_missing_delete = ignore");
+  }
+  scoped_device guard(device);
+  auto g = std::make_unique<dyng_hip_graph>();
+  g->device = device;
+  g->props = *props;
+  if (!(g->props.headroom >= 0.0) || g->props.headroom > 16.0) g->props.headroom = 0.25;
+  g->K = K;
+  g->n = n;
+  g->n_cap = std::max<int64_t>(n, 1);
+  const int stride = props->directed ? 1 : 2;
+  const int64_t N = count * stride;
+  if (N >= (int64_t{1} << 32)) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: too many edges for the HIP device graph");
+  const int vbits = bits_for(std::max<int64_t>(n, 1));
+  // stage the edge list on the device
+  dbuf<int32_t> d_src, d_dst, d_w;
+  const int32_t* ps = src;
+  const int32_t* pd = dst;
+  const int32_t* pw = w;
+  if (!from_device) {
+    d_src.reserve(count);
+    d_dst.reserve(count);
+    d_w.reserve(count * K);
+    if (count > 0) {
+      DYNG_HIP_TRY(hipMemcpyAsync(d_src.p, src, count * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      DYNG_HIP_TRY(hipMemcpyAsync(d_dst.p, dst, count * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      DYNG_HIP_TRY(hipMemcpyAsync(d_w.p, w, count * K * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    }
+    ps = d_src.p;
+    pd = d_dst.p;
+    pw = d_w.p;
+  }
+  g->ctl.reserve(1);
+  dbuf<u64> ka, kb;
+  dbuf<uint32_t> va, vb;
+  dbuf<int32_t> keep;
+  ka.reserve(N);
+  kb.reserve(N);
+  va.reserve(N);
+  vb.reserve(N);
+  keep.reserve(N + 1);
+  if (N > 0) {
+    hipLaunchKernelGGL(k_edge_keys, dim3(blocks_for(N)), dim3(kBlock), 0, s, ps, pd, count, stride,
+                       props->directed, props->on_self_loop == 1 ? 1 : 0, vbits, ka.p, va.p);
+    DYNG_HIP_TRY(hipGetLastError());
+    sort_pairs(g->sort_tmp, ka.p, kb.p, va.p, vb.p, N, 2 * vbits, s);
+  }
+  hipLaunchKernelGGL(k_keep, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, kb.p, N, props->allow_parallel, keep.p);
+  DYNG_HIP_TRY(hipGetLastError());
+  build_store(g.get(), g->out, kb.p, vb.p, N, keep.p, vbits, stride, pw, 0, 0, s);
+  // edges kept
+  {
+    dbuf<int32_t> kp;
+    kp.reserve(N + 1);
+    exclusive_scan<int32_t>(g->scan_tmp, keep.p, kp.p, N + 1, s);
+    int32_t mm = 0;
+    DYNG_HIP_TRY(hipMemcpyAsync(&mm, kp.p + N, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    DYNG_HIP_TRY(hipStreamSynchronize(s));
+    g->m = mm;
+  }
+  g->mark_out.reserve(g->n_cap);
+  g->mark_in.reserve(g->n_cap);
+  fill<int32_t>(g->mark_out.p, g->n_cap, 0, s);
+  fill<int32_t>(g->mark_in.p, g->n_cap, 0, s);
+  build_in_store(g.get(), s);
+  recompute_columns(g.get(), s);
+  static uint64_t next_state = 1;
+  g->state_id = __atomic_fetch_add(&next_state, 1, __ATOMIC_RELAXED);
+  return g.release();
+}
+
+void copy_rows_to_host(const row_store& rs, int64_t n, int K, int64_t* row_ptr, int32_t* col, int32_t* w) {
+  std::vector<int2> meta(static_cast<size_t>(n));
+  if (n > 0) DYNG_HIP_TRY(hipMemcpy(meta.data(), rs.meta.p, n * sizeof(int2), hipMemcpyDeviceToHost));
+  std::vector<int32_t> nbr(static_cast<size_t>(rs.slot_cap));
+  std::vector<int32_t> wt(static_cast<size_t>(rs.slot_cap) * K);
+  if (rs.slot_cap > 0) {
+    DYNG_HIP_TRY(hipMemcpy(nbr.data(), rs.nbr.p, rs.slot_cap * sizeof(int32_t), hipMemcpyDeviceToHost));
+    DYNG_HIP_TRY(hipMemcpy(wt.data(), rs.w.p, rs.slot_cap * K * sizeof(int32_t), hipMemcpyDeviceToHost));
+  }
+  row_ptr[0] = 0;
+  for (int64_t u = 0; u < n; ++u) row_ptr[u + 1] = row_ptr[u] + meta[u].y;
+  const int64_t m = row_ptr[n];
+  for (int64_t u = 0; u < n; ++u) {
+    for (int j = 0; j < meta[u].y; ++j) {
+      const int64_t o = row_ptr[u] + j;
+      col[o] = nbr[meta[u].x + j];
+      for (int k = 0; k < K; ++k) w[k * m + o] = wt[k * rs.slot_cap + meta[u].x + j];
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// The apply (host orchestration)
+// ---------------------------------------------------------------------------------------------
+
+void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch_scan& sc) {
+  if (b == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: graph apply: no batch");
+  const int K = g->K;
+  const int64_t num_ins = b->num_ins, num_del = b->num_del;
+  if (num_ins < 0 || num_del < 0) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: negative batch sizes");
+  if (num_ins > 0 && b->num_weights != K) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: the batch has " + std::to_string(b->num_weights) +
+                                        " weights per insertion but the graph has " + std::to_string(K));
+  }
+  // stage host batches on the device
+  const int32_t *ins_src = b->ins_src, *ins_dst = b->ins_dst, *ins_w = b->ins_w, *del_src = b->del_src,
+                *del_dst = b->del_dst;
+  if (!b->on_device) {
+    g->b_ins_src.reserve(num_ins);
+    g->b_ins_dst.reserve(num_ins);
+    g->b_ins_w.reserve(num_ins * K);
+    g->b_del_src.reserve(num_del);
+    g->b_del_dst.reserve(num_del);
+    if (num_ins > 0) {
+      DYNG_HIP_TRY(hipMemcpyAsync(g->b_ins_src.p, ins_src, num_ins * 4, hipMemcpyHostToDevice, s));
+      DYNG_HIP_TRY(hipMemcpyAsync(g->b_ins_dst.p, ins_dst, num_ins * 4, hipMemcpyHostToDevice, s));
+      DYNG_HIP_TRY(hipMemcpyAsync(g->b_ins_w.p, ins_w, num_ins * K * 4, hipMemcpyHostToDevice, s));
+    }
+    if (num_del > 0) {
+      DYNG_HIP_TRY(hipMemcpyAsync(g->b_del_src.p, del_src, num_del * 4, hipMemcpyHostToDevice, s));
+      DYNG_HIP_TRY(hipMemcpyAsync(g->b_del_dst.p, del_dst, num_del * 4, hipMemcpyHostToDevice, s));
+    }
+    ins_src = g->b_ins_src.p;
+    ins_dst = g->b_ins_dst.p;
+    ins_w = g->b_ins_w.p;
+    del_src = g->b_del_src.p;
+    del_dst = g->b_del_dst.p;
+  }
+  apply_control* hc = g->host_ctl.p;
+  // ---- scan the batch (ids, self-loops, weights) before anything changes ----
+  std::memset(hc, 0, sizeof(apply_control));
+  hc->min_id = INT32_MAX;
+  hc->max_id = -1;
+  hc->min_weight = INT32_MAX;
+  for (int k = 0; k < max_weights; ++k) hc->batch_wmin[k] = 1;
+  DYNG_HIP_TRY(hipMemcpyAsync(g->ctl.p, hc, sizeof(apply_control), hipMemcpyHostToDevice, s));
+  if (num_ins + num_del > 0) {
+    hipLaunchKernelGGL(k_batch_scan, dim3(std::min<unsigned>(blocks_for(std::max(num_ins, num_del)), 1024)),
+                       dim3(kBlock), 0, s, ins_src, ins_dst, ins_w, num_ins, del_src, del_dst, num_del, K, g->ctl.p);
+    DYNG_HIP_TRY(hipGetLastError());
+  }
+  DYNG_HIP_TRY(hipMemcpyAsync(hc, g->ctl.p, sizeof(apply_control), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  if (hc->min_id < 0) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: the batch names a negative vertex id");
+  if (g->props.on_self_loop == 2 && hc->self_loops_ins + hc->self_loops_del > 0) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: the batch has a self-loop and on_self_loop is error");
+  }
+  int64_t n_after = g->n;
+  if (hc->max_id >= 0) {
+    if (!g->props.allow_vertex_growth && hc->max_id >= g->n) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: an insertion names a vertex >= " + std::to_string(g->n) +
+                                          " and allow_vertex_growth is false");
+    }
+    if (hc->max_id >= INT32_MAX - 1) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: an insertion names the largest vertex id");
+    // a self-loop insertion that is dropped does not grow the vertex set; keep the rule simple by
+    // growing only for kept insertions (the scan covers them all; drop is rare)
+    n_after = std::max<int64_t>(n_after, static_cast<int64_t>(hc->max_id) + 1);
+  }
+  sc.n_after = n_after;
+  sc.num_ins = num_ins;
+  sc.num_del = num_del;
+  sc.ins_src = ins_src;
+  sc.ins_dst = ins_dst;
+  sc.ins_w = ins_w;
+  sc.del_src = del_src;
+  sc.del_dst = del_dst;
+  sc.wmax.assign(K, 0);
+  sc.wmin.assign(K, 1);
+  for (int k = 0; k < K; ++k) {
+    sc.wmax[k] = num_ins > 0 ? hc->batch_wmax[k] : 0;
+    sc.wmin[k] = hc->batch_wmin[k];
+  }
+}
+
+void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out) {
+  const int K = g->K;
+  const int64_t num_ins = sc.num_ins, num_del = sc.num_del;
+  const int32_t *ins_src = sc.ins_src, *ins_dst = sc.ins_dst, *ins_w = sc.ins_w, *del_src = sc.del_src,
+                *del_dst = sc.del_dst;
+  const int64_t n_after = sc.n_after;
+  apply_control* hc = g->host_ctl.p;
+  const int64_t n_before = g->n;
+  const int stride = g->props.directed ? 1 : 2;
+  const int64_t N = (num_ins + num_del) * stride;
+  if (N >= (int64_t{1} << 31)) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: batch too large");
+  // ---- make sure the tails can take every relocation this batch could cause ----
+  for (row_store* rs : {&g->out, &g->in}) {
+    // every touched row relocating: sum of (len + pending) * (1 + headroom) + 3 per row
+    const int64_t need = static_cast<int64_t>((g->m + N) * (1.0 + g->props.headroom)) + 4 * N + 1024;
+    if (rs->slot_cap - rs->tail < need) {
+      compact_store(g, *rs, need, s);
+      if (out) out->compactions += 1;
+    }
+  }
+  grow_vertices(g, n_after, s);
+  // deletions below the old vertex count that name vertices of the growth are fine; ids in the
+  // batch >= n_after only occur in deletions (handled in Step 0)
+  const int vbits = bits_for(std::max<int64_t>(n_after, 1));
+  op_layout L;
+  L.first_is_del = g->props.deletions_first ? 1 : 0;
+  L.num_first = L.first_is_del ? num_del : num_ins;
+  L.num_second = L.first_is_del ? num_ins : num_del;
+  L.stride = stride;
+  g->num_del_slots = num_del * stride;
+  g->num_ins_slots = num_ins * stride;
+  g->del_src.reserve(g->num_del_slots);
+  g->del_dst.reserve(g->num_del_slots);
+  g->ins_src.reserve(g->num_ins_slots);
+  g->ins_dst.reserve(g->num_ins_slots);
+  g->ins_inc.reserve(g->num_ins_slots * K);
+  g->keys_a.reserve(N);
+  g->keys_b.reserve(N);
+  g->vals_a.reserve(N);
+  g->vals_b.reserve(N);
+  g->pend_alive.reserve(N + 1);
+  g->kill.reserve(N + 1);
+  g->pend_prefix.reserve(N + 1);
+  g->kill_prefix.reserve(N + 1);
+  g->pend_w.reserve(N * K);
+  g->ikeys_a.reserve(N);
+  g->ikeys_b.reserve(N);
+  g->ivals_a.reserve(N);
+  g->ivals_b.reserve(N);
+  g->iop_type.reserve(N);
+  g->iop_wold.reserve(N * K);
+  g->iop_wnew.reserve(N * K);
+  g->ipend_alive.reserve(N + 1);
+  g->ikill.reserve(N + 1);
+  g->ipend_prefix.reserve(N + 1);
+  g->ikill_prefix.reserve(N + 1);
+  g->ipend_w.reserve(N * K);
+  g->touched_out.reserve(N + 1);
+  g->touched_in.reserve(N + 1);
+  g->stage_cap = std::max<int64_t>(int64_t{1} << 20, 2 * N);
+  g->stage_nbr.reserve(g->stage_cap);
+  g->stage_w.reserve(g->stage_cap * K);
+  if (++g->mark_generation == INT32_MAX) {
+    fill<int32_t>(g->mark_out.p, g->n_cap, 0, s);
+    fill<int32_t>(g->mark_in.p, g->n_cap, 0, s);
+    g->mark_generation = 1;
+  }
+  // ---- control block: counters, tails, the column targets ----
+  std::memset(hc, 0, sizeof(apply_control));
+  for (int k = 0; k < K; ++k) hc->col_max[k] = std::max<int64_t>(g->wmax[k], sc.wmax[k]);
+  hc->tail_out = g->out.tail;
+  hc->tail_in = g->in.tail;
+  DYNG_HIP_TRY(hipMemcpyAsync(g->ctl.p, hc, sizeof(apply_control), hipMemcpyHostToDevice, s));
+  apply_control* dc = g->ctl.p;
+  if (N > 0) {
+    hipLaunchKernelGGL(k_make_ops, dim3(blocks_for(N)), dim3(kBlock), 0, s, L, ins_src, ins_dst, del_src, del_dst,
+                       g->props.directed, g->props.on_self_loop == 1 ? 1 : 0, n_after, vbits, g->keys_a.p,
+                       g->vals_a.p, g->del_src.p, g->del_dst.p, g->ins_src.p, g->ins_dst.p, g->ins_inc.p, K, dc);
+    DYNG_HIP_TRY(hipGetLastError());
+    sort_pairs(g->sort_tmp, g->keys_a.p, g->keys_b.p, g->vals_a.p, g->vals_b.p, N, 2 * vbits, s);
+    out_pair_params op{};
+    op.L = L;
+    op.keys = g->keys_b.p;
+    op.vals = g->vals_b.p;
+    op.N = N;
+    op.vbits = vbits;
+    op.K = K;
+    op.n_before = n_before;
+    op.upsert = g->props.on_existing_insert == 0 ? 1 : 0;
+    op.ins_w = ins_w;
+    op.meta = g->out.meta.p;
+    op.nbr = g->out.nbr.p;
+    op.w = g->out.w.p;
+    op.slot_cap = g->out.slot_cap;
+    op.pend_alive = g->pend_alive.p;
+    op.kill = g->kill.p;
+    op.pend_w = g->pend_w.p;
+    op.e_inc = g->ins_inc.p;
+    op.ikeys = g->ikeys_a.p;
+    op.ivals = g->ivals_a.p;
+    op.iop_type = g->iop_type.p;
+    op.iop_wold = g->iop_wold.p;
+    op.iop_wnew = g->iop_wnew.p;
+    op.mark = g->mark_out.p;
+    op.mark_gen = g->mark_generation;
+    op.touched = g->touched_out.p;
+    op.ctl = dc;
+    hipLaunchKernelGGL(k_out_pairs, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, op);
+    DYNG_HIP_TRY(hipGetLastError());
+    exclusive_scan<int32_t>(g->scan_tmp, g->pend_alive.p, g->pend_prefix.p, N + 1, s);
+    exclusive_scan<int32_t>(g->scan_tmp, g->kill.p, g->kill_prefix.p, N + 1, s);
+    merge_params mp{};
+    mp.meta = g->out.meta.p;
+    mp.cap = g->out.cap.p;
+    mp.nbr = g->out.nbr.p;
+    mp.w = g->out.w.p;
+    mp.slot_cap = g->out.slot_cap;
+    mp.K = K;
+    mp.keys = g->keys_b.p;
+    mp.N = N;
+    mp.vbits = vbits;
+    mp.pend_alive = g->pend_alive.p;
+    mp.pend_prefix = g->pend_prefix.p;
+    mp.kill_prefix = g->kill_prefix.p;
+    mp.pend_w = g->pend_w.p;
+    mp.touched = g->touched_out.p;
+    mp.touched_count = &dc->touched_out;
+    mp.stage_nbr = g->stage_nbr.p;
+    mp.stage_w = g->stage_w.p;
+    mp.stage_cap = g->stage_cap;
+    mp.stage_used = &dc->stage_out;
+    mp.tail = &dc->tail_out;
+    mp.garbage = &dc->garbage_out;
+    mp.relocated = &dc->relocated_rows;
+    mp.error_bits = &dc->error_bits;
+    mp.headroom = g->props.headroom;
+    const unsigned merge_grid = static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>(N, 1), 4096));
+    hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kBlock), 0, s, mp);
+    DYNG_HIP_TRY(hipGetLastError());
+    // ---- the in-rows ----
+    sort_pairs(g->sort_tmp, g->ikeys_a.p, g->ikeys_b.p, g->ivals_a.p, g->ivals_b.p, N, 2 * vbits, s);
+    in_pair_params ip{};
+    ip.keys = g->ikeys_b.p;
+    ip.vals = g->ivals_b.p;
+    ip.N = N;
+    ip.vbits = vbits;
+    ip.K = K;
+    ip.iop_type = g->iop_type.p;
+    ip.iop_wold = g->iop_wold.p;
+    ip.iop_wnew = g->iop_wnew.p;
+    ip.meta = g->in.meta.p;
+    ip.nbr = g->in.nbr.p;
+    ip.w = g->in.w.p;
+    ip.slot_cap = g->in.slot_cap;
+    ip.pend_alive = g->ipend_alive.p;
+    ip.kill = g->ikill.p;
+    ip.pend_w = g->ipend_w.p;
+    ip.mark = g->mark_in.p;
+    ip.mark_gen = g->mark_generation;
+    ip.touched = g->touched_in.p;
+    ip.ctl = dc;
+    hipLaunchKernelGGL(k_in_pairs, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, ip);
+    DYNG_HIP_TRY(hipGetLastError());
+    exclusive_scan<int32_t>(g->scan_tmp, g->ipend_alive.p, g->ipend_prefix.p, N + 1, s);
+    exclusive_scan<int32_t>(g->scan_tmp, g->ikill.p, g->ikill_prefix.p, N + 1, s);
+    merge_params mi = mp;
+    mi.meta = g->in.meta.p;
+    mi.cap = g->in.cap.p;
+    mi.nbr = g->in.nbr.p;
+    mi.w = g->in.w.p;
+    mi.slot_cap = g->in.slot_cap;
+    mi.keys = g->ikeys_b.p;
+    mi.pend_alive = g->ipend_alive.p;
+    mi.pend_prefix = g->ipend_prefix.p;
+    mi.kill_prefix = g->ikill_prefix.p;
+    mi.pend_w = g->ipend_w.p;
+    mi.touched = g->touched_in.p;
+    mi.touched_count = &dc->touched_in;
+    mi.stage_used = &dc->stage_in;
+    mi.tail = &dc->tail_in;
+    mi.garbage = &dc->garbage_in;
+    hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kBlock), 0, s, mi);
+    DYNG_HIP_TRY(hipGetLastError());
+  }
+  DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  if (hc->error_bits != 0) {
+    throw status_error(DYNG_HIP_DEVICE_ERROR, "dyng: the device apply reported an internal error",
+                       static_cast<uint32_t>(hc->error_bits));
+  }
+  g->n = n_after;
+  g->m += static_cast<int64_t>(hc->inserted_edges) - static_cast<int64_t>(hc->deleted_edges);
+  for (int k = 0; k < K; ++k) g->wmin[k] = std::min<int64_t>(g->wmin[k], sc.wmin[k]);
+  g->out.tail = hc->tail_out;
+  g->in.tail = hc->tail_in;
+  g->out.garbage += hc->garbage_out;
+  g->in.garbage += hc->garbage_in;
+  bool dirty = false;
+  for (int k = 0; k < K; ++k) {
+    const int64_t target = hc->col_max[k];
+    const int64_t before = target == g->wmax[k] ? g->wmax_count[k] : 0;
+    g->wsum[k] += hc->col_dsum[k];
+    g->wmax[k] = target;
+    g->wmax_count[k] = before + hc->col_dcount[k];
+    if (g->wmax_count[k] <= 0) dirty = true;
+  }
+  g->version += 1;
+  if (out != nullptr) {
+    out->inserted_edges = static_cast<int64_t>(hc->inserted_edges);
+    out->updated_edges = static_cast<int64_t>(hc->updated_edges);
+    out->deleted_edges = static_cast<int64_t>(hc->deleted_edges);
+    out->ignored_deletions = static_cast<int64_t>(hc->ignored_deletions);
+    out->dropped_self_loops = static_cast<int64_t>(hc->dropped_self_loops);
+    out->ignored_insertions = static_cast<int64_t>(hc->ignored_insertions);
+    out->inserted_vertices = n_after - n_before;
+    out->num_vertices_after = n_after;
+    out->relocated_rows = static_cast<int64_t>(hc->relocated_rows);
+  }
+  // the last edge of the largest weight left: recompute the column summaries (reuses the control
+  // block, so only after the counters above were read)
+  if (dirty) recompute_columns(g, s);
+}
+
+}  // namespace dyng_hip
+
+// ---------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------
+
+using namespace dyng_hip;
+
+extern "C" int dyng_hip_graph_create_from_edges(int device, const dyng_hip_graph_props* props, int64_t n,
+                                                int64_t count, int32_t num_weights, const int32_t* src,
+                                                const int32_t* dst, const int32_t* w, int32_t from_device,
+                                                void* stream, dyng_hip_graph** out) {
+  return guarded([&] {
+    if (out == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: null output handle");
+    for (int64_t i = 0; !from_device && i < count; ++i) {
+      if (src[i] < 0 || src[i] >= n || dst[i] < 0 || dst[i] >= n) {
+        fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: edge " + std::to_string(i) + " (" + std::to_string(src[i]) + ", " +
+                                            std::to_string(dst[i]) + ") is out of range [0, " + std::to_string(n) + ")");
+      }
+    }
+    *out = create_from_edge_list(device, props, n, count, num_weights, src, dst, w, from_device, as_stream(stream));
+  });
+}
+
+extern "C" int dyng_hip_graph_create(int device, const dyng_hip_graph_props* props, int64_t n, int64_t m,
+                                     int32_t num_weights, const int64_t* row_ptr, const int32_t* col,
+                                     const int32_t* w, int32_t from_device, void* stream, dyng_hip_graph** out) {
+  return guarded([&] {
+    if (from_device) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: graph_create takes host rows (use create_from_edges)");
+    if (n < 0 || m < 0) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: negative sizes");
+    if (row_ptr[0] != 0 || row_ptr[n] != m) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: row offsets do not span the edges");
+    std::vector<int32_t> src(static_cast<size_t>(m));
+    std::vector<int32_t> wl(static_cast<size_t>(m) * num_weights);
+    for (int64_t u = 0; u < n; ++u) {
+      if (row_ptr[u + 1] < row_ptr[u]) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: row offsets decrease");
+      for (int64_t e = row_ptr[u]; e < row_ptr[u + 1]; ++e) src[e] = static_cast<int32_t>(u);
+    }
+    for (int64_t e = 0; e < m; ++e) {
+      if (col[e] < 0 || col[e] >= n) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: neighbour out of range");
+      for (int k = 0; k < num_weights; ++k) wl[e * num_weights + k] = w[k * m + e];
+    }
+    *out = create_from_edge_list(device, props, n, m, num_weights, src.data(), col, wl.data(), 0, as_stream(stream));
+  });
+}
+
+extern "C" int dyng_hip_graph_destroy(dyng_hip_graph* g) {
+  return guarded([&] { delete g; });
+}
+
+extern "C" int dyng_hip_graph_num_vertices(const dyng_hip_graph* g, int64_t* n) {
+  return guarded([&] { *n = g->n; });
+}
+
+extern "C" int dyng_hip_graph_num_edges(const dyng_hip_graph* g, int64_t* m) {
+  return guarded([&] { *m = g->m; });
+}
+
+extern "C" int dyng_hip_graph_version(const dyng_hip_graph* g, uint64_t* version) {
+  return guarded([&] { *version = g->version; });
+}
+
+extern "C" int dyng_hip_graph_column(const dyng_hip_graph* g, int32_t k, int64_t* max_weight, int64_t* weight_sum) {
+  return guarded([&] {
+    if (k < 0 || k >= g->K) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: objective out of range");
+    *max_weight = g->wmax[k];
+    *weight_sum = g->wsum[k];
+  });
+}
+
+extern "C" int dyng_hip_graph_download(const dyng_hip_graph* g, int64_t* row_ptr, int32_t* col, int32_t* w) {
+  return guarded([&] {
+    scoped_device guard(g->device);
+    DYNG_HIP_TRY(hipDeviceSynchronize());
+    copy_rows_to_host(g->out, g->n, g->K, row_ptr, col, w);
+  });
+}
+
+extern "C" int dyng_hip_graph_download_in(const dyng_hip_graph* g, int64_t* row_ptr, int32_t* col, int32_t* w) {
+  return guarded([&] {
+    scoped_device guard(g->device);
+    DYNG_HIP_TRY(hipDeviceSynchronize());
+    copy_rows_to_host(g->in, g->n, g->K, row_ptr, col, w);
+  });
+}
+
+extern "C" int dyng_hip_graph_apply(dyng_hip_graph* g, const dyng_hip_batch* batch, void* stream,
+                                    dyng_hip_apply_summary* summary) {
+  return guarded([&] {
+    if (g == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: null graph");
+    scoped_device guard(g->device);
+    dyng_hip_apply_summary tmp{};
+    batch_scan sc;
+    scan_batch(g, batch, as_stream(stream), sc);
+    apply_scanned(g, sc, as_stream(stream), &tmp);
+    if (summary != nullptr) *summary = tmp;
+  });
+}
+
+extern "C" int dyng_hip_graph_changes(const dyng_hip_graph* g, dyng_hip_changes* out) {
+  return guarded([&] {
+    out->del_src = g->del_src.p;
+    out->del_dst = g->del_dst.p;
+    out->num_del = g->num_del_slots;
+    out->ins_src = g->ins_src.p;
+    out->ins_dst = g->ins_dst.p;
+    out->ins_increased = g->ins_inc.p;
+    out->num_ins = g->num_ins_slots;
+    out->num_weights = g->K;
+  });
+}

@@ -1,0 +1,198 @@
+// Internal declarations shared by the translation units of libdyng_hip.so.
+#pragma once
+
+#include "../../include/dyng_hip.h"
+#include "common.hpp"
+
+#include <cstddef>
+#include <memory>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace dyng_hip {
+
+constexpr int max_weights = 64;  ///< weight columns a graph may have on this backend
+
+/// An error on its way to the C ABI (translated into a dyng_hip_status and the last message).
+struct status_error : std::runtime_error {
+  int code;
+  uint32_t device_bits;
+  status_error(int c, const std::string& m, uint32_t bits = 0)
+      : std::runtime_error(m), code(c), device_bits(bits) {}
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
+[[noreturn]] inline void fail(int code, const std::string& m) { throw status_error(code, m); }
+void set_last_error(const std::string& m);
+
+#define DYNG_HIP_TRY(expr)                                                   \
+  do {                                                                       \
+    hipError_t dyng_e_ = (expr);                                             \
+    if (dyng_e_ != hipSuccess) ::dyng_hip::throw_hip(dyng_e_, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+/// Run `fn` and translate exceptions into a status (the C ABI never throws).
+template <typename fn_t>
+int guarded(fn_t&& fn) {
+  try {
+    fn();
+    set_last_error("");
+    return DYNG_HIP_OK;
+  } catch (const status_error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("dyng: out of host memory");
+    return DYNG_HIP_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return DYNG_HIP_RUNTIME_ERROR;
+  }
+}
+
+/// A device array that grows on demand (hipMalloc; the contents are not kept on growth unless
+/// `keep` is set).
+template <typename T>
+struct dbuf {
+  T* p = nullptr;
+  size_t n = 0;
+  dbuf() = default;
+  dbuf(const dbuf&) = delete;
+  dbuf& operator=(const dbuf&) = delete;
+  ~dbuf() { release(); }
+  void release() {
+    if (p != nullptr) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T* reserve(size_t count, bool keep = false, hipStream_t stream = nullptr) {
+    if (count <= n && p != nullptr) return p;
+    const size_t want = count > 0 ? count : 1;
+    T* q = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&q), want * sizeof(T));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw status_error(DYNG_HIP_OUT_OF_MEMORY,
+                         "dyng: out of device memory (" + std::to_string(want * sizeof(T)) + " bytes)");
+    }
+    if (keep && p != nullptr && n > 0) {
+      DYNG_HIP_TRY(hipMemcpyAsync(q, p, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
+      DYNG_HIP_TRY(hipStreamSynchronize(stream));
+    }
+    release();
+    p = q;
+    n = want;
+    return p;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+/// Pinned host memory (hipHostMalloc).
+template <typename T>
+struct pinned {
+  T* p = nullptr;
+  pinned() {
+    void* q = nullptr;
+    if (hipHostMalloc(&q, sizeof(T), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      throw status_error(DYNG_HIP_OUT_OF_MEMORY, "dyng: out of pinned host memory");
+    }
+    p = static_cast<T*>(q);
+  }
+  ~pinned() {
+    if (p != nullptr) (void)hipHostFree(p);
+  }
+  pinned(const pinned&) = delete;
+  pinned& operator=(const pinned&) = delete;
+};
+
+/// One direction of the device graph: per-vertex {offset, length} and capacity, and the slot
+/// arena (neighbour ids, then num_weights weight columns of slot_cap entries each).
+struct row_store {
+  dbuf<int2> meta;
+  dbuf<int32_t> cap;
+  dbuf<int32_t> nbr;
+  dbuf<int32_t> w;
+  int64_t slot_cap = 0;  ///< slots of the arena
+  int64_t tail = 0;      ///< first slot of the free tail
+  int64_t garbage = 0;   ///< slots left behind by relocated rows
+};
+
+/// The device-side counters of one apply, read back once at its end.
+struct apply_control {
+  unsigned long long inserted_edges, updated_edges, deleted_edges, ignored_deletions;
+  unsigned long long dropped_self_loops, ignored_insertions;
+  unsigned long long relocated_rows;
+  int in_ops;
+  int touched_out, touched_in;
+  int error_bits;
+  long long tail_out, tail_in;        ///< arena tails (atomic bump allocation)
+  long long stage_out, stage_in;      ///< staging slots used
+  long long garbage_out, garbage_in;  ///< slots left behind by relocations
+  // batch scan
+  int min_id;
+  int max_id;
+  int self_loops_ins, self_loops_del;
+  int min_weight;
+  int pad0;
+  // column summaries: target max per objective (set by the host), deltas (device)
+  long long col_max[max_weights];
+  long long col_dsum[max_weights];
+  long long col_dcount[max_weights];
+  int batch_wmax[max_weights];
+  int batch_wmin[max_weights];
+  long long col_bad[max_weights];  ///< weights below 1 per column (column summary)
+};
+
+/// What the batch scan found (before anything changes) and where the batch is on the device.
+struct batch_scan {
+  int64_t n_after = 0;
+  int64_t num_ins = 0, num_del = 0;
+  const int32_t *ins_src = nullptr, *ins_dst = nullptr, *ins_w = nullptr, *del_src = nullptr, *del_dst = nullptr;
+  std::vector<int64_t> wmax, wmin;  ///< per objective: largest / smallest insertion weight (min clamped to 1)
+};
+
+}  // namespace dyng_hip
+
+struct dyng_hip_graph;
+namespace dyng_hip {
+void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch_scan& sc);
+void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out);
+}  // namespace dyng_hip
+
+/// The device graph behind the dyng_hip_graph handle.
+struct dyng_hip_graph {
+  int device = 0;
+  dyng_hip_graph_props props{};
+  int32_t K = 1;
+  int64_t n = 0;      ///< vertices
+  int64_t n_cap = 0;  ///< capacity of the per-vertex arrays
+  int64_t m = 0;      ///< edges
+  uint64_t version = 0;
+  uint64_t state_id = 0;
+  dyng_hip::row_store out, in;
+  std::vector<int64_t> wmax, wsum, wmax_count;
+  std::vector<int64_t> wmin;  ///< 1 when every weight of the column is >= 1, else 0 (or below)
+
+  // last apply: the effective operations (with holes, -1) and their weight-increase flags
+  dyng_hip::dbuf<int32_t> del_src, del_dst, ins_src, ins_dst;
+  dyng_hip::dbuf<uint8_t> ins_inc;
+  int64_t num_del_slots = 0, num_ins_slots = 0;
+
+  // apply scratch
+  dyng_hip::dbuf<unsigned long long> keys_a, keys_b, ikeys_a, ikeys_b;
+  dyng_hip::dbuf<uint32_t> vals_a, vals_b, ivals_a, ivals_b;
+  dyng_hip::dbuf<unsigned char> sort_tmp, scan_tmp;
+  dyng_hip::dbuf<int32_t> pend_alive, kill, pend_prefix, kill_prefix, pend_w;
+  dyng_hip::dbuf<int32_t> ipend_alive, ikill, ipend_prefix, ikill_prefix, ipend_w;
+  dyng_hip::dbuf<int32_t> iop_type, iop_wold, iop_wnew;
+  dyng_hip::dbuf<int32_t> touched_out, touched_in, mark_out, mark_in;
+  dyng_hip::dbuf<int32_t> stage_nbr, stage_w;
+  int64_t stage_cap = 0;
+  int mark_generation = 0;
+  dyng_hip::dbuf<int32_t> b_ins_src, b_ins_dst, b_ins_w, b_del_src, b_del_dst;
+  dyng_hip::dbuf<dyng_hip::apply_control> ctl;
+  dyng_hip::pinned<dyng_hip::apply_control> host_ctl;
+};

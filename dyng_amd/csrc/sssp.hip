@@ -1,0 +1,866 @@
+// The sssp engine of the HIP path: one persistent cooperative kernel per compute() / update()
+// (the fused engine of cpp/src/algorithms/sssp/fused.cuh:155-495, re-designed for CDNA4).
+//
+// Phases, separated by grid barriers and decided on the device:
+//   A  pack the old tree into (distance << b | parent) words; roots = heads v of changed edges
+//      (u,v) with parent[v] == u (fused.cuh:178-197); roots start the invalidated list.
+//   B  subtree invalidation by pointer jumping (fused.cuh:199-234), each round over the
+//      compacted list of vertices still jumping (wave64 ballot appends) instead of all n; every
+//      newly flagged vertex joins the invalidated list, so phase C touches only those.
+//   C  invalidate (word INF, stamp), then the insertion heads (fused.cuh:236-256).
+//   D  pull: each candidate takes its best in-neighbour offer (fused.cuh:258-285).
+//   E  near-far propagation (fused.cuh:288-417): the relax iteration expands the near frontier
+//      with a packed 64-bit atomicMin per edge; short rows run one thread per vertex, longer rows
+//      are cut into chunks of edges that whole waves expand.
+//   F  unpack, writing only changed (distance, parent) pairs and counting them
+//      (fused.cuh:419-494); the distance-only mode recovers lowest-id parents.
+//
+// Per-iteration list counters live in rings of three slots, so an iteration needs one barrier
+// (two when it has chunked rows) instead of the reference's two.
+#include "internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+namespace dyng_hip {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct control {
+  int ring_near[3];
+  int ring_chunk[3];
+  int ring_jump[3];
+  int far_count[2];
+  int cand_count;
+  int frontier_count;
+  int roots;
+  int overflow;
+  int invalidated;
+  int rounds;
+  int still_jumping;
+  int iterations;
+  int epochs;
+  int generation;
+  long long pushes;
+  u64 min_frontier;
+  u64 minimum[2];
+  unsigned long long affected;
+  unsigned long long relax_edges;
+  unsigned long long relax_improved;
+  unsigned long long pull_edges;
+  unsigned long long jump_items;
+  unsigned long long pull_vertices;
+  unsigned long long t_phase[4];  ///< s_memrealtime ticks: start, before pull, before near-far, end
+  unsigned int error_bits;
+};
+
+struct engine_params {
+  int32_t n;
+  const int2* out_meta;
+  const int32_t* out_nbr;
+  const int32_t* out_w;
+  const int2* in_meta;
+  const int32_t* in_nbr;
+  const int32_t* in_w;
+  const int32_t* del_src;
+  const int32_t* del_dst;
+  int64_t num_del;
+  const int32_t* ins_src;
+  const int32_t* ins_dst;
+  const uint8_t* ins_inc;
+  int32_t inc_stride;
+  int32_t inc_k;
+  int64_t num_ins;
+  int32_t source;
+  int32_t from_scratch;
+  int32_t count_changes;
+  int32_t check_cycles;
+  packing pk;
+  u64 max_distance;
+  u64 delta;
+  int32_t max_rounds;
+  int32_t generation;
+  long long* distances;
+  int32_t* parents;
+  u64* packed;
+  int* stamp;
+  int* in_far;
+  int* flag;
+  int* ancestor;
+  int* cand;
+  int* frontier;
+  int* near_a;
+  int* near_b;
+  int* far_a;
+  int* far_b;
+  int4* chunks;
+  int64_t chunk_cap;
+  control* ctl;
+  grid_sync_state* bar;
+  int32_t light;
+  int32_t chunk_edges;
+};
+
+/// Append v to a list when pred (wave-aggregated reservation).
+DYNG_DEV void append(int* list, int* counter, bool pred, int32_t v) {
+  const int slot = wave_append(counter, pred);
+  if (pred) list[slot] = v;
+}
+
+/// Cut the row [begin, begin + len) of vertex v into chunks of `chunk` edges (one wave each).
+DYNG_DEV void emit_chunks(const engine_params& p, int* counter, int32_t v, int32_t begin, int32_t len) {
+  const int pieces = (len + p.chunk_edges - 1) / p.chunk_edges;
+  const int base = atomicAdd(counter, pieces);
+  for (int c = 0; c < pieces; ++c) {
+    if (base + c >= p.chunk_cap) {
+      atomicOr(&p.ctl->error_bits, DYNG_HIP_DEVICE_ERROR_CAPACITY);
+      return;
+    }
+    const int b = begin + c * p.chunk_edges;
+    const int e = min(begin + len, b + p.chunk_edges);
+    p.chunks[base + c] = make_int4(v, b, e, 0);
+  }
+}
+
+/// One relaxation of edge (u, w) of weight wt with u at distance du (fused.cuh:341-362): the
+/// candidate word joins the next near list or the far pile when it lowered w's distance.
+DYNG_DEV void relax(const engine_params& p, bool active, int32_t u, u64 du, int32_t w, int32_t wt,
+                    u64 threshold, int generation, int* next, int* next_count, int* far, int* far_count,
+                    unsigned long long& improved) {
+  bool to_near = false, to_far = false;
+  if (active && w != p.source) {
+    const u64 nd = du + static_cast<u64>(wt);
+    if (nd <= p.max_distance) {
+      const u64 cand = p.pk.pack(nd, u);
+      if (cand < ld(&p.packed[w])) {
+        const u64 old = atomicMin(&p.packed[w], cand);
+        improved += cand < old ? 1 : 0;
+        if (nd < p.pk.distance(old)) {
+          if (nd < threshold) {
+            to_near = claim(p.stamp, w, generation);
+          } else {
+            to_far = atomicExch(&p.in_far[w], 1) == 0;
+          }
+        }
+      }
+    }
+  }
+  append(next, next_count, to_near, w);
+  append(far, far_count, to_far, w);
+}
+
+/// The relax iteration over `count` near vertices: light rows inline, heavy rows as chunks.
+DYNG_DEV void expand_light(const engine_params& p, const int* near, int count, u64 threshold, int generation,
+                           int* next, int* next_count, int* far, int* far_count, int* chunk_count,
+                           unsigned long long& edges, unsigned long long& improved) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int threads = gridDim.x * blockDim.x;
+  for (int base = tid - lane(); base < count; base += threads) {
+    const int i = base + lane();
+    int32_t u = -1;
+    u64 word = packed_inf;
+    int2 mt = make_int2(0, 0);
+    if (i < count) {
+      u = ld(&near[i]);
+      word = ld(&p.packed[u]);
+      if (word != packed_inf) mt = p.out_meta[u];
+    }
+    const bool light = word != packed_inf && mt.y <= p.light;
+    if (word != packed_inf && !light) emit_chunks(p, chunk_count, u, mt.x, mt.y);
+    const u64 du = p.pk.distance(word);
+    const int len = light ? mt.y : 0;
+    edges += len;
+    // lanes walk their rows together; a lane past its row's end is idle
+    int most = len;
+    for (int off = 32; off > 0; off >>= 1) most = max(most, __shfl_xor(most, off));
+    for (int j = 0; j < most; ++j) {
+      const bool act = j < len;
+      const int32_t w = act ? p.out_nbr[mt.x + j] : 0;
+      const int32_t wt = act ? p.out_w[mt.x + j] : 0;
+      relax(p, act, u, du, w, wt, threshold, generation, next, next_count, far, far_count, improved);
+    }
+  }
+}
+
+DYNG_DEV void expand_chunks(const engine_params& p, int nchunks, u64 threshold, int generation, int* next,
+                            int* next_count, int* far, int* far_count, unsigned long long& edges,
+                            unsigned long long& improved) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int waves = (gridDim.x * blockDim.x) / 64;
+  for (int c = wave; c < nchunks; c += waves) {
+    const int4 ch = p.chunks[c];
+    const int32_t u = ch.x;
+    const u64 word = ld(&p.packed[u]);
+    const u64 du = p.pk.distance(word);
+    if (lane() == 0) edges += ch.z - ch.y;
+    for (int b = ch.y; b < ch.z; b += 64) {
+      const int e = b + lane();
+      const bool act = word != packed_inf && e < ch.z;
+      const int32_t w = act ? p.out_nbr[e] : 0;
+      const int32_t wt = act ? p.out_w[e] : 0;
+      relax(p, act, u, du, w, wt, threshold, generation, next, next_count, far, far_count, improved);
+    }
+  }
+}
+
+/// Pull offer of in-edge (u -> v, wt): the candidate word, or INF.
+DYNG_DEV u64 pull_offer(const engine_params& p, int32_t u, int32_t wt) {
+  const u64 word = ld(&p.packed[u]);
+  if (word == packed_inf) return packed_inf;
+  const u64 nd = p.pk.distance(word) + static_cast<u64>(wt);
+  return nd <= p.max_distance ? p.pk.pack(nd, u) : packed_inf;
+}
+
+DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best, int generation, int* counter) {
+  bool add = false;
+  if (pred) {
+    const u64 current = ld(&p.packed[v]);
+    if (best < current) {
+      const u64 old = atomicMin(&p.packed[v], best);
+      add = p.pk.distance(best) < p.pk.distance(old) && claim(p.stamp, v, generation);
+    }
+  }
+  append(p.frontier, counter, add, v);
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(kBlock) sssp_fused_kernel(engine_params p) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int threads = gridDim.x * blockDim.x;
+  const int n = p.n;
+  control* c = p.ctl;
+  int generation = p.generation;
+  int frontier_count = 0;
+  unsigned long long relax_edges = 0, relax_improved = 0, pull_edges = 0, affected = 0;
+  int step = 0, cstep = 0;  // ring positions (uniform across the grid)
+  unsigned long long jump_items = 0;
+  if (tid == 0) c->t_phase[0] = __builtin_amdgcn_s_memrealtime();
+
+  if (p.from_scratch) {
+    for (int v = tid; v < n; v += threads) p.packed[v] = v == p.source ? p.pk.pack(0, -1) : packed_inf;
+    if (tid == 0) p.frontier[0] = p.source;
+    frontier_count = 1;
+    grid_barrier(p.bar);
+  } else {
+    // ---- A: pack the old tree; ancestors; roots ----
+    for (int v = tid; v < n; v += threads) {
+      const long long d = p.distances[v];
+      const int32_t par = p.parents[v];
+      u64 word = packed_inf;
+      if (d < sssp_infinity / 2) {
+        if (d < 0 || static_cast<u64>(d) > p.max_distance) {
+          st(&c->overflow, 1);
+        } else {
+          word = p.pk.pack(static_cast<u64>(d), par);
+        }
+      }
+      p.packed[v] = word;
+      p.ancestor[v] = par;
+    }
+    for (int64_t i = tid; i < p.num_del + p.num_ins; i += threads) {
+      int32_t u, v;
+      bool changed;
+      if (i < p.num_del) {
+        u = p.del_src[i];
+        v = p.del_dst[i];
+        changed = v >= 0;
+      } else {
+        const int64_t j = i - p.num_del;
+        u = p.ins_src[j];
+        v = p.ins_dst[j];
+        changed = v >= 0 && p.ins_inc[j * p.inc_stride + p.inc_k] != 0;
+      }
+      const bool root = changed && p.parents[v] == u && atomicExch(&p.flag[v], 1) == 0;
+      append(p.cand, &c->cand_count, root, v);
+    }
+    grid_barrier(p.bar);
+    if (ld(&c->overflow) != 0) return;  // uniform: the host reports invalid_input
+    const int roots = ld(&c->cand_count);
+    const bool jump = (p.num_del + p.num_ins > 0) && (roots > 0 || p.check_cycles);
+    // ---- B: pointer jumping over compacted lists of vertices still jumping ----
+    if (jump) {
+      int count = n;
+      const int* list = nullptr;
+      int* outs[2] = {p.near_a, p.near_b};
+      for (int r = 0; r < p.max_rounds; ++r) {
+        jump_items += count;
+        int* out = outs[r & 1];
+        int* out_count = &c->ring_jump[r % 3];
+        if (tid == 0) {
+          st(&c->ring_jump[(r + 1) % 3], 0);
+          st(&c->rounds, r + 1);
+        }
+        for (int base = tid - lane(); base < count; base += threads) {
+          const int i = base + lane();
+          bool keep = false, flagged = false;
+          int32_t v = -1;
+          if (i < count) {
+            v = list ? ld(&list[i]) : i;
+            const int32_t a = ld(&p.ancestor[v]);
+            if (a >= 0 && ld(&p.flag[v]) == 0) {
+              if (ld(&p.flag[a]) != 0) {
+                st(&p.flag[v], 1);
+                flagged = true;
+              } else {
+                const int32_t next = ld(&p.ancestor[a]);
+                st(&p.ancestor[v], next);
+                keep = next >= 0;
+              }
+            }
+          }
+          append(p.cand, &c->cand_count, flagged, v);
+          append(out, out_count, keep, v);
+        }
+        grid_barrier(p.bar);
+        count = ld(out_count);
+        list = out;
+        if (count == 0) break;
+      }
+      if (count > 0) {
+        if (tid == 0) atomicOr(&c->error_bits, DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE);
+        return;  // uniform
+      }
+    }
+    // ---- C: invalidate the flagged vertices; then the insertion heads ----
+    ++generation;
+    const int inv = ld(&c->cand_count);
+    for (int i = tid; i < inv; i += threads) {
+      const int32_t v = p.cand[i];
+      p.flag[v] = 0;
+      p.packed[v] = packed_inf;
+      p.stamp[v] = generation;
+    }
+    if (tid == 0) st(&c->invalidated, inv);
+    grid_barrier(p.bar);
+    for (int64_t base = tid - lane(); base < p.num_ins; base += threads) {
+      const int64_t i = base + lane();
+      int32_t v = -1;
+      bool add = false;
+      if (i < p.num_ins) {
+        v = p.ins_dst[i];
+        add = v >= 0 && v != p.source && claim(p.stamp, v, generation);
+      }
+      append(p.cand, &c->cand_count, add, v);
+    }
+    grid_barrier(p.bar);
+    // ---- D: pull ----
+    ++generation;
+    const int cands = ld(&c->cand_count);
+    if (tid == 0) {
+      c->t_phase[1] = __builtin_amdgcn_s_memrealtime();
+      c->jump_items = jump_items;
+      c->pull_vertices = cands;
+    }
+    int* chunk_count = &c->ring_chunk[cstep % 3];
+    if (tid == 0) st(&c->ring_chunk[(cstep + 1) % 3], 0);
+    for (int base = tid - lane(); base < cands; base += threads) {
+      const int i = base + lane();
+      int32_t v = -1;
+      int2 mt = make_int2(0, 0);
+      if (i < cands) {
+        v = p.cand[i];
+        mt = p.in_meta[v];
+      }
+      const bool light = i < cands && mt.y <= p.light;
+      if (i < cands && !light) emit_chunks(p, chunk_count, v, mt.x, mt.y);
+      u64 best = packed_inf;
+      if (light) {
+        pull_edges += mt.y;
+        for (int j = 0; j < mt.y; ++j) {
+          const u64 o = pull_offer(p, p.in_nbr[mt.x + j], p.in_w[mt.x + j]);
+          best = o < best ? o : best;
+        }
+      }
+      pull_commit(p, light, v, best, generation, &c->frontier_count);
+    }
+    grid_barrier(p.bar);
+    const int nchunks = static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap));
+    ++cstep;
+    if (nchunks > 0) {
+      const int wave = tid / 64;
+      const int waves = threads / 64;
+      for (int q = wave; q < nchunks; q += waves) {
+        const int4 ch = p.chunks[q];
+        u64 best = packed_inf;
+        for (int e = ch.y + lane(); e < ch.z; e += 64) {
+          const u64 o = pull_offer(p, p.in_nbr[e], p.in_w[e]);
+          best = o < best ? o : best;
+        }
+        if (lane() == 0) pull_edges += ch.z - ch.y;
+        best = wave_min(best);
+        if (lane() == 0) {
+          const u64 current = ld(&p.packed[ch.x]);
+          if (best < current) {
+            const u64 old = atomicMin(&p.packed[ch.x], best);
+            if (p.pk.distance(best) < p.pk.distance(old) && claim(p.stamp, ch.x, generation)) {
+              const int slot = atomicAdd(&c->frontier_count, 1);
+              p.frontier[slot] = ch.x;
+            }
+          }
+        }
+      }
+      grid_barrier(p.bar);
+    }
+    frontier_count = ld(&c->frontier_count);
+  }
+
+  // ---- E: near-far propagation ----
+  if (tid == 0) c->t_phase[2] = __builtin_amdgcn_s_memrealtime();
+  u64 local = packed_inf;
+  for (int i = tid; i < frontier_count; i += threads) {
+    const u64 word = ld(&p.packed[ld(&p.frontier[i])]);
+    if (word != packed_inf) local = min(local, p.pk.distance(word));
+  }
+  local = wave_min(local);
+  if (lane() == 0 && local != packed_inf) atomicMin(&c->min_frontier, local);
+  grid_barrier(p.bar);
+  const u64 smallest = ld(&c->min_frontier);
+  u64 threshold = (smallest == packed_inf ? 0 : smallest) + p.delta;
+  int* current = p.near_a;
+  int* next = p.near_b;
+  int* far[2] = {p.far_a, p.far_b};
+  int fb = 0;
+  ++generation;
+  {
+    int* near_count = &c->ring_near[step % 3];
+    if (tid == 0) st(&c->ring_near[(step + 1) % 3], 0);
+    for (int base = tid - lane(); base < frontier_count; base += threads) {
+      const int i = base + lane();
+      int32_t v = -1;
+      bool to_near = false, to_far = false;
+      if (i < frontier_count) {
+        v = ld(&p.frontier[i]);
+        const u64 word = ld(&p.packed[v]);
+        const u64 d = word == packed_inf ? packed_inf : p.pk.distance(word);
+        if (d < threshold) {
+          to_near = claim(p.stamp, v, generation);
+        } else {
+          to_far = atomicExch(&p.in_far[v], 1) == 0;
+        }
+      }
+      append(current, near_count, to_near, v);
+      append(far[fb], &c->far_count[fb], to_far, v);
+    }
+    grid_barrier(p.bar);
+  }
+  int near_count = ld(&c->ring_near[step % 3]);
+  ++step;
+  int iterations = 0, epochs = 0;
+  long long pushes = 0;
+  // every relax iteration expands a vertex whose distance fell since its last expansion; a run
+  // past this bound is a bug: stop with an internal error instead of spinning
+  const long long guard = 4LL * n + 100000;
+  for (;;) {
+    if (static_cast<long long>(iterations) + epochs > guard) {
+      if (tid == 0) atomicOr(&c->error_bits, DYNG_HIP_DEVICE_ERROR_INTERNAL);
+      break;
+    }
+    if (near_count > 0) {
+      // -- one relax iteration --
+      ++generation;
+      ++iterations;
+      pushes += near_count;
+      int* next_count = &c->ring_near[step % 3];
+      int* chunk_count = &c->ring_chunk[cstep % 3];
+      if (tid == 0) {
+        st(&c->ring_near[(step + 1) % 3], 0);
+        st(&c->ring_chunk[(cstep + 1) % 3], 0);
+      }
+      expand_light(p, current, near_count, threshold, generation, next, next_count, far[fb], &c->far_count[fb],
+                   chunk_count, relax_edges, relax_improved);
+      grid_barrier(p.bar);
+      const int nchunks = static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap));
+      ++cstep;
+      if (nchunks > 0) {
+        expand_chunks(p, nchunks, threshold, generation, next, next_count, far[fb], &c->far_count[fb], relax_edges,
+                      relax_improved);
+        grid_barrier(p.bar);
+      }
+      near_count = ld(next_count);
+      ++step;
+      int* t = current;
+      current = next;
+      next = t;
+      continue;
+    }
+    // -- near frontier empty: raise the threshold past the far pile --
+    const int far_count = ld(&c->far_count[fb]);
+    if (far_count == 0) break;
+    ++epochs;
+    const int ms = epochs & 1;
+    if (tid == 0) st(&c->minimum[ms ^ 1], packed_inf);
+    local = packed_inf;
+    for (int i = tid; i < far_count; i += threads) {
+      const u64 word = ld(&p.packed[ld(&far[fb][i])]);
+      if (word != packed_inf) local = min(local, p.pk.distance(word));
+    }
+    local = wave_min(local);
+    if (lane() == 0 && local != packed_inf) atomicMin(&c->minimum[ms], local);
+    grid_barrier(p.bar);
+    const u64 mn = ld(&c->minimum[ms]);
+    threshold = (threshold > mn ? threshold : mn) + p.delta;
+    ++generation;
+    int* split_count = &c->ring_near[step % 3];
+    if (tid == 0) st(&c->ring_near[(step + 1) % 3], 0);
+    for (int base = tid - lane(); base < far_count; base += threads) {
+      const int i = base + lane();
+      int32_t v = -1;
+      bool to_near = false, keep = false;
+      if (i < far_count) {
+        v = ld(&far[fb][i]);
+        const u64 word = ld(&p.packed[v]);
+        const u64 d = word == packed_inf ? packed_inf : p.pk.distance(word);
+        if (d < threshold) {
+          p.in_far[v] = 0;
+          to_near = claim(p.stamp, v, generation);
+        } else {
+          keep = true;
+        }
+      }
+      append(current, split_count, to_near, v);
+      append(far[fb ^ 1], &c->far_count[fb ^ 1], keep, v);
+    }
+    grid_barrier(p.bar);
+    near_count = ld(split_count);
+    ++step;
+    if (tid == 0) st(&c->far_count[fb], 0);
+    fb ^= 1;
+  }
+
+  // ---- F: unpack ----
+  if (tid == 0) c->t_phase[3] = __builtin_amdgcn_s_memrealtime();
+  if (p.pk.bits > 0) {
+    for (int v = tid; v < n; v += threads) {
+      const u64 word = p.packed[v];
+      const long long d = word == packed_inf ? sssp_infinity : static_cast<long long>(p.pk.distance(word));
+      const int32_t par = word == packed_inf ? -1 : p.pk.parent(word);
+      if (!p.count_changes) {
+        p.distances[v] = d;
+        p.parents[v] = par;
+      } else if (p.distances[v] != d || p.parents[v] != par) {
+        p.distances[v] = d;
+        p.parents[v] = par;
+        ++affected;
+      }
+    }
+  } else {
+    // distance-only words: lowest-id tight parents (fused.cuh:452-485)
+    for (int v = tid; v < n; v += threads) {
+      const u64 word = p.packed[v];
+      const long long d = word == packed_inf ? sssp_infinity : static_cast<long long>(word);
+      if (p.count_changes) {
+        p.ancestor[v] = p.parents[v];
+        p.flag[v] = d != p.distances[v] ? 1 : 0;
+      }
+      p.distances[v] = d;
+      p.parents[v] = (word == packed_inf || v == p.source) ? -1 : INT32_MAX;
+    }
+    grid_barrier(p.bar);
+    for (int u = tid; u < n; u += threads) {
+      const u64 du = p.packed[u];
+      if (du == packed_inf) continue;
+      const int2 mt = p.out_meta[u];
+      for (int j = 0; j < mt.y; ++j) {
+        const int32_t w = p.out_nbr[mt.x + j];
+        if (w != p.source && du + static_cast<u64>(p.out_w[mt.x + j]) == p.packed[w]) atomicMin(&p.parents[w], u);
+      }
+    }
+    if (p.count_changes) {
+      grid_barrier(p.bar);
+      for (int v = tid; v < n; v += threads) {
+        affected += (p.flag[v] != 0 || p.ancestor[v] != ld(&p.parents[v])) ? 1 : 0;
+        p.flag[v] = 0;
+      }
+    }
+  }
+  affected = wave_sum(affected);
+  relax_edges = wave_sum(relax_edges);
+  relax_improved = wave_sum(relax_improved);
+  pull_edges = wave_sum(pull_edges);
+  if (lane() == 0) {
+    if (affected) atomicAdd(&c->affected, affected);
+    if (relax_edges) atomicAdd(&c->relax_edges, relax_edges);
+    if (relax_improved) atomicAdd(&c->relax_improved, relax_improved);
+    if (pull_edges) atomicAdd(&c->pull_edges, pull_edges);
+  }
+  if (tid == 0) {
+    c->iterations = iterations;
+    c->epochs = epochs;
+    c->pushes = pushes;
+    c->generation = generation;
+  }
+}
+
+}  // namespace dyng_hip
+
+// ---------------------------------------------------------------------------------------------
+// Workspace and launch
+// ---------------------------------------------------------------------------------------------
+
+struct dyng_hip_workspace {
+  int device = 0;
+  int64_t capacity = 0;
+  int generation = 0;
+  dyng_hip::dbuf<dyng_hip::u64> packed;
+  dyng_hip::dbuf<int> stamp, in_far, flag, ancestor;
+  dyng_hip::dbuf<int> cand, frontier, near_a, near_b, far_a, far_b;
+  dyng_hip::dbuf<int4> chunks;
+  int64_t chunk_cap = 0;
+  dyng_hip::dbuf<dyng_hip::control> ctl;
+  dyng_hip::pinned<dyng_hip::control> host_ctl;
+  dyng_hip::dbuf<dyng_hip::grid_sync_state> bar;
+  int grid_blocks = 0;
+  int cu_count = 0;
+  int occupancy = 0;
+  hipEvent_t ev_start = nullptr;
+  hipEvent_t ev_stop = nullptr;
+  ~dyng_hip_workspace() {
+    if (ev_start != nullptr) (void)hipEventDestroy(ev_start);
+    if (ev_stop != nullptr) (void)hipEventDestroy(ev_stop);
+  }
+};
+
+namespace dyng_hip {
+
+namespace {
+
+void ws_reserve(dyng_hip_workspace* ws, int64_t n, int64_t slots, hipStream_t s) {
+  const int64_t want_chunks = slots / 16 + n + 4096;
+  if (n > ws->capacity) {
+    const int64_t cap = std::max<int64_t>(n, 1);
+    ws->packed.reserve(cap);
+    ws->stamp.reserve(cap);
+    ws->in_far.reserve(cap);
+    ws->flag.reserve(cap);
+    ws->ancestor.reserve(cap);
+    ws->cand.reserve(2 * cap + 64);
+    ws->frontier.reserve(cap);
+    ws->near_a.reserve(cap);
+    ws->near_b.reserve(cap);
+    ws->far_a.reserve(cap);
+    ws->far_b.reserve(cap);
+    DYNG_HIP_TRY(hipMemsetAsync(ws->stamp.p, 0, cap * sizeof(int), s));
+    DYNG_HIP_TRY(hipMemsetAsync(ws->in_far.p, 0, cap * sizeof(int), s));
+    DYNG_HIP_TRY(hipMemsetAsync(ws->flag.p, 0, cap * sizeof(int), s));
+    ws->capacity = cap;
+    ws->generation = 0;
+  }
+  if (want_chunks > ws->chunk_cap) {
+    ws->chunks.reserve(want_chunks);
+    ws->chunk_cap = want_chunks;
+  }
+  ws->ctl.reserve(1);
+  ws->bar.reserve(1);
+  if (ws->cu_count == 0) {
+    DYNG_HIP_TRY(hipDeviceGetAttribute(&ws->cu_count, hipDeviceAttributeMultiprocessorCount, ws->device));
+    DYNG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ws->occupancy,
+                                                              reinterpret_cast<const void*>(sssp_fused_kernel),
+                                                              kBlock, 0));
+    int coop = 0;
+    DYNG_HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ws->device));
+    if (!coop) {
+      fail(DYNG_HIP_NOT_SUPPORTED, "dyng: sssp: the fused engine needs cooperative launch, which this device lacks");
+    }
+    if (ws->occupancy < 1) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: sssp: the persistent kernel does not fit on a CU");
+  }
+}
+
+int bits_for_parents(int64_t n) {
+  int bits = 1;
+  while (((1ULL << bits) - 1) < static_cast<u64>(n)) ++bits;
+  return bits;
+}
+
+void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_hip_counters* out) {
+  if (run == nullptr || run->graph == nullptr || run->workspace == nullptr) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: sssp run: graph and workspace are required");
+  }
+  const dyng_hip_graph* g = run->graph;
+  dyng_hip_workspace* ws = run->workspace;
+  const int64_t n = g->n;
+  dyng_hip_counters counters{};
+  counters.engine_used = DYNG_HIP_ENGINE_FUSED;
+  counters.packed_parents = 1;
+  if (n == 0) {
+    if (out) *out = counters;
+    return;
+  }
+  if (run->delta <= 0) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: the near-far width delta must be > 0, got " + std::to_string(run->delta));
+  if (run->source < 0 || run->source >= n) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: source " + std::to_string(run->source) + " is out of range [0, " +
+                                        std::to_string(n) + ")");
+  }
+  if (run->objective < 0 || run->objective >= g->K) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: objective out of range");
+  if (run->tuning.engine == DYNG_HIP_ENGINE_OPERATORS) {
+    fail(DYNG_HIP_NOT_SUPPORTED, "dyng: sssp: the operators engine is not built in this release; use fused");
+  }
+  // choose_packing (cuda.cu:226-234): (n - 1) * max_weight must fit next to the parent bits
+  const u64 weight = static_cast<u64>(std::max<int64_t>(run->max_weight, 1));
+  const u64 hops = static_cast<u64>(std::max<int64_t>(n - 1, 1));
+  if (weight > output_max_distance / hops) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: distances up to " + std::to_string(weight) + " * " + std::to_string(hops) +
+                                        " do not fit in 62 bits");
+  }
+  const u64 bound = weight * hops;
+  const int bits = bits_for_parents(n);
+  const u64 max_packed = (packed_inf >> bits) - 1;
+  packing pk{};
+  if (bound <= max_packed) {
+    pk.bits = bits;
+    pk.no_parent = (1ULL << bits) - 1;
+  } else {
+    pk.bits = 0;
+    pk.no_parent = 0;
+  }
+  ws_reserve(ws, n, std::max(g->out.slot_cap, g->in.slot_cap), s);
+  if (ws->generation > INT32_MAX / 2) {
+    DYNG_HIP_TRY(hipMemsetAsync(ws->stamp.p, 0, ws->capacity * sizeof(int), s));
+    ws->generation = 0;
+  }
+  engine_params p{};
+  p.n = static_cast<int32_t>(n);
+  const int k = run->objective;
+  p.out_meta = g->out.meta.p;
+  p.out_nbr = g->out.nbr.p;
+  p.out_w = g->out.w.p + static_cast<int64_t>(k) * g->out.slot_cap;
+  p.in_meta = g->in.meta.p;
+  p.in_nbr = g->in.nbr.p;
+  p.in_w = g->in.w.p + static_cast<int64_t>(k) * g->in.slot_cap;
+  if (update) {
+    p.del_src = g->del_src.p;
+    p.del_dst = g->del_dst.p;
+    p.num_del = g->num_del_slots;
+    p.ins_src = g->ins_src.p;
+    p.ins_dst = g->ins_dst.p;
+    p.ins_inc = g->ins_inc.p;
+    p.inc_stride = g->K;
+    p.inc_k = k;
+    p.num_ins = g->num_ins_slots;
+  }
+  p.source = run->source;
+  p.from_scratch = update ? 0 : 1;
+  p.count_changes = update ? 1 : 0;
+  p.check_cycles = run->check_cycles;
+  p.pk = pk;
+  p.max_distance = bound;
+  p.delta = static_cast<u64>(run->delta);
+  int rounds = 0;
+  while ((1LL << rounds) < static_cast<long long>(n)) ++rounds;
+  p.max_rounds = rounds + 1;
+  p.generation = ++ws->generation;
+  p.distances = reinterpret_cast<long long*>(run->distances);
+  p.parents = run->parents;
+  p.packed = ws->packed.p;
+  p.stamp = ws->stamp.p;
+  p.in_far = ws->in_far.p;
+  p.flag = ws->flag.p;
+  p.ancestor = ws->ancestor.p;
+  p.cand = ws->cand.p;
+  p.frontier = ws->frontier.p;
+  p.near_a = ws->near_a.p;
+  p.near_b = ws->near_b.p;
+  p.far_a = ws->far_a.p;
+  p.far_b = ws->far_b.p;
+  p.chunks = ws->chunks.p;
+  p.chunk_cap = ws->chunk_cap;
+  p.ctl = ws->ctl.p;
+  p.bar = ws->bar.p;
+  p.light = run->tuning.light_degree > 0 ? run->tuning.light_degree : 32;
+  p.chunk_edges = run->tuning.chunk_edges > 0 ? std::max(16, run->tuning.chunk_edges) : 256;
+  int per_cu = run->tuning.blocks_per_cu > 0 ? std::min(run->tuning.blocks_per_cu, ws->occupancy) : std::min(ws->occupancy, 4);
+  const int blocks = std::max(1, per_cu * ws->cu_count);
+  control* hc = ws->host_ctl.p;
+  std::memset(hc, 0, sizeof(control));
+  hc->min_frontier = packed_inf;
+  hc->minimum[0] = packed_inf;
+  hc->minimum[1] = packed_inf;
+  DYNG_HIP_TRY(hipMemcpyAsync(ws->ctl.p, hc, sizeof(control), hipMemcpyHostToDevice, s));
+  DYNG_HIP_TRY(hipMemsetAsync(ws->bar.p, 0, sizeof(grid_sync_state), s));
+  void* args[] = {&p};
+  if (ws->ev_start == nullptr) {
+    DYNG_HIP_TRY(hipEventCreate(&ws->ev_start));
+    DYNG_HIP_TRY(hipEventCreate(&ws->ev_stop));
+  }
+  DYNG_HIP_TRY(hipEventRecord(ws->ev_start, s));
+  DYNG_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(sssp_fused_kernel), dim3(blocks), dim3(kBlock),
+                                          args, 0, s));
+  DYNG_HIP_TRY(hipEventRecord(ws->ev_stop, s));
+  DYNG_HIP_TRY(hipMemcpyAsync(hc, ws->ctl.p, sizeof(control), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  float kernel_ms = 0.0f;
+  DYNG_HIP_TRY(hipEventElapsedTime(&kernel_ms, ws->ev_start, ws->ev_stop));
+  if (hc->overflow != 0) {
+    throw status_error(DYNG_HIP_DEVICE_ERROR,
+                       "sssp: an input distance is negative or larger than (n - 1) * max weight; the initial tree "
+                       "does not belong to this graph",
+                       DYNG_HIP_DEVICE_ERROR_INVALID_INPUT);
+  }
+  if (hc->error_bits & DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE) {
+    throw status_error(DYNG_HIP_DEVICE_ERROR,
+                       "sssp: the input shortest-path tree has a parent cycle (pointer jumping did not reach a root "
+                       "in ceil(log2 n) + 1 rounds)",
+                       DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE);
+  }
+  if (hc->error_bits != 0) {
+    throw status_error(DYNG_HIP_DEVICE_ERROR, "sssp: the device reported error bits " + std::to_string(hc->error_bits),
+                       hc->error_bits);
+  }
+  ws->generation = std::max(ws->generation, hc->generation);
+  counters.invalidated = hc->invalidated;
+  counters.iterations = hc->iterations;
+  counters.epochs = hc->epochs;
+  counters.pushes = hc->pushes;
+  counters.affected = static_cast<int64_t>(hc->affected);
+  counters.packed_parents = pk.bits > 0 ? 1 : 0;
+  counters.relax_vertices = hc->pushes;
+  counters.relax_edges = static_cast<int64_t>(hc->relax_edges);
+  counters.relax_improved = static_cast<int64_t>(hc->relax_improved);
+  counters.pull_edges = static_cast<int64_t>(hc->pull_edges);
+  counters.jump_rounds = hc->rounds;
+  counters.jump_items = static_cast<int64_t>(hc->jump_items);
+  counters.pull_vertices = static_cast<int64_t>(hc->pull_vertices);
+  // s_memrealtime runs at 100 MHz
+  const double tick_ms = 1e-5;
+  const unsigned long long t0 = hc->t_phase[0];
+  const unsigned long long t1 = update ? hc->t_phase[1] : t0;
+  const unsigned long long t2 = hc->t_phase[2];
+  const unsigned long long t3 = hc->t_phase[3];
+  counters.step1_ms = (t2 >= t0 ? t2 - t0 : 0) * tick_ms;
+  counters.pull_ms = (update && t2 >= t1 ? t2 - t1 : 0) * tick_ms;
+  counters.relax_ms = (t3 >= t2 ? t3 - t2 : 0) * tick_ms;
+  counters.blocks = blocks;
+  counters.kernel_ms = kernel_ms;
+  if (out) *out = counters;
+}
+
+}  // namespace
+
+}  // namespace dyng_hip
+
+using namespace dyng_hip;
+
+extern "C" int dyng_hip_workspace_create(int device, dyng_hip_workspace** out) {
+  return guarded([&] {
+    auto ws = std::make_unique<dyng_hip_workspace>();
+    ws->device = device;
+    *out = ws.release();
+  });
+}
+
+extern "C" int dyng_hip_workspace_destroy(dyng_hip_workspace* ws) {
+  return guarded([&] { delete ws; });
+}
+
+extern "C" int dyng_hip_sssp_update(const dyng_hip_sssp_run* run, void* stream, dyng_hip_counters* out) {
+  return guarded([&] { run_engine(run, true, static_cast<hipStream_t>(stream), out); });
+}
+
+extern "C" int dyng_hip_sssp_compute(const dyng_hip_sssp_run* run, void* stream, dyng_hip_counters* out) {
+  return guarded([&] { run_engine(run, false, static_cast<hipStream_t>(stream), out); });
+}

@@ -1,0 +1,268 @@
+/*
+ * dyng_hip.h -- the C ABI of the MI355X-native sssp path (libdyng_hip.so).
+ *
+ * Plain pointers, sizes and POD structs; no C++ or torch types cross it. Two layers:
+ *
+ *  (1) Engine layer: what dynG's backend seam calls for the cuda backend today
+ *      (cpp/src/algorithms/sssp/problem.hpp:557-571 sssp_cuda_update / sssp_cuda_compute with
+ *      sssp_run, problem.hpp:380-400; the device graph upload of cpp/src/graph/graph.cpp:718-745
+ *      and cpp/src/graph/device_graph.cu:55-150; the host batch apply of
+ *      cpp/src/graph/apply_host.cpp:57-401, which this layer replaces with an in-place device
+ *      apply). A C++ backend built on dynG's resources/framework calls these.
+ *
+ *  (2) API layer: what dynG's Python binding binds (python/bindings/sssp.cpp:77-135:
+ *      native.sssp_compute / sssp_update / sssp_from_arrays and the result accessors; the graph
+ *      of python/dyng/graph.py and the batch of python/dyng/batch.py). Handles are opaque; the
+ *      host logic behind them (Step 0 checks, versions, poisoning, the change lists, the packing
+ *      rule, delta) is the C++ of dyng_amd/csrc/api.cpp, which calls layer (1).
+ *
+ * Every function returns a dyng_hip_status; on failure dyng_hip_last_error() gives the message
+ * (thread-local). The status values mirror dynG's exception classes (cpp/include/dyng/core/
+ * error.hpp:28-164): invalid_argument_error, stale_result_error, not_supported_error,
+ * out_of_memory_error, and a device error carrying the bits of util/device_error_flags.hpp.
+ */
+#ifndef DYNG_HIP_H
+#define DYNG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dyng_hip_status {
+  DYNG_HIP_OK = 0,
+  DYNG_HIP_INVALID_ARGUMENT = 1, /* invalid_argument_error */
+  DYNG_HIP_STALE_RESULT = 2,     /* stale_result_error */
+  DYNG_HIP_NOT_SUPPORTED = 3,    /* not_supported_error */
+  DYNG_HIP_OUT_OF_MEMORY = 4,    /* out_of_memory_error */
+  DYNG_HIP_DEVICE_ERROR = 5,     /* a device error bit was raised (see device_errors) */
+  DYNG_HIP_RUNTIME_ERROR = 6     /* the HIP runtime reported an error (cuda_error) */
+} dyng_hip_status;
+
+/* Device error bits (cpp/src/util/device_error_flags.hpp). */
+#define DYNG_HIP_DEVICE_ERROR_CAPACITY 1u
+#define DYNG_HIP_DEVICE_ERROR_INVALID_INPUT 2u
+#define DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE 4u
+#define DYNG_HIP_DEVICE_ERROR_INTERNAL 0x80000000u
+
+/* Engines (cpp/include/dyng/core/types.hpp, dyng::engine). */
+#define DYNG_HIP_ENGINE_AUTOMATIC 0
+#define DYNG_HIP_ENGINE_FUSED 1
+#define DYNG_HIP_ENGINE_OPERATORS 2
+
+/* The last error message of this thread ("" if none). */
+const char* dyng_hip_last_error(void);
+
+/* ============================================================================================
+ * (1) Engine layer
+ * ==========================================================================================*/
+
+/* graph_properties + batch_semantics (cpp/include/dyng/graph/graph_properties.hpp:67-221). */
+typedef struct dyng_hip_graph_props {
+  int32_t directed;            /* 1: directed; 0: a batch edge (u,v) changes both arcs */
+  int32_t sorted_rows;         /* row_order::sorted (1) or append (0); see the layout note */
+  int32_t allow_parallel;      /* multi_edges::allow (1) or forbid (0) */
+  int32_t on_existing_insert;  /* 0 upsert, 2 ignore (1 = error is not supported here) */
+  int32_t on_missing_delete;   /* 0 ignore (1 = error is not supported here) */
+  int32_t on_self_loop;        /* 0 keep, 1 drop (2 = error: checked on the host side) */
+  int32_t deletions_first;     /* 1: deletions, then insertions */
+  int32_t allow_vertex_growth; /* 1: an insertion naming v >= n grows the vertex set */
+  double headroom;             /* spare slots per row: ceil(len * headroom) + 4 */
+} dyng_hip_graph_props;
+
+/* The device graph: out- and in-rows in slack storage (one slot range per row with spare
+ * capacity, rows kept sorted by neighbour id, parallel edges in insertion order), so a batch
+ * applies in place. Replaces the compact device copy plus in-edge build of device_graph.cu. */
+typedef struct dyng_hip_graph dyng_hip_graph;
+
+/* apply_summary (cpp/include/dyng/graph/apply_summary.hpp:22-41). */
+typedef struct dyng_hip_apply_summary {
+  int64_t inserted_edges, updated_edges, deleted_edges, ignored_deletions, dropped_self_loops;
+  int64_t inserted_vertices, num_vertices_after, ignored_insertions, relocated_rows, compactions;
+} dyng_hip_apply_summary;
+
+/* A batch (cpp/include/dyng/graph/edge_batch.hpp:38-76): insertions (u, v, K weights,
+ * insertion-major), deletions (u, v). `on_device` = 1: the arrays are device pointers. */
+typedef struct dyng_hip_batch {
+  const int32_t* ins_src;
+  const int32_t* ins_dst;
+  const int32_t* ins_w;
+  int64_t num_ins;
+  const int32_t* del_src;
+  const int32_t* del_dst;
+  int64_t num_del;
+  int32_t num_weights;
+  int32_t on_device;
+} dyng_hip_batch;
+
+/* Build the device graph from compact rows (row_ptr n+1, col m, w K*m objective-major), host
+ * (from_device = 0) or device pointers. Replaces graph_access::prepare / build_device_graph
+ * (cpp/src/graph/graph.cpp:1011-1018, device_graph.hpp:140-157). */
+int dyng_hip_graph_create(int device, const dyng_hip_graph_props* props, int64_t n, int64_t m,
+                          int32_t num_weights, const int64_t* row_ptr, const int32_t* col,
+                          const int32_t* w, int32_t from_device, void* stream,
+                          dyng_hip_graph** out);
+/* Build the device graph from an edge list (src, dst, w count*K insertion-major), host or device
+ * pointers, with the construction rules of build_from_edges_host
+ * (cpp/src/graph/apply_host.cpp:608-690: repeated (u,v) of a simple graph merged, last wins). */
+int dyng_hip_graph_create_from_edges(int device, const dyng_hip_graph_props* props, int64_t n,
+                                     int64_t count, int32_t num_weights, const int32_t* src,
+                                     const int32_t* dst, const int32_t* w, int32_t from_device,
+                                     void* stream, dyng_hip_graph** out);
+int dyng_hip_graph_destroy(dyng_hip_graph* g);
+int dyng_hip_graph_num_vertices(const dyng_hip_graph* g, int64_t* n);
+int dyng_hip_graph_num_edges(const dyng_hip_graph* g, int64_t* m);
+/* Column summary of objective k (sssp.cpp:285-318 summarize_column, kept exact incrementally):
+ * largest weight (>= 1) and weight sum. */
+int dyng_hip_graph_column(const dyng_hip_graph* g, int32_t k, int64_t* max_weight,
+                          int64_t* weight_sum);
+/* Copy the compact out-rows (sorted by neighbour id) to host arrays sized n+1, m, K*m. */
+int dyng_hip_graph_download(const dyng_hip_graph* g, int64_t* row_ptr, int32_t* col, int32_t* w);
+/* Copy the compact in-rows (sorted by source id) to host arrays sized n+1, m, K*m. */
+int dyng_hip_graph_download_in(const dyng_hip_graph* g, int64_t* row_ptr, int32_t* col,
+                               int32_t* w);
+
+/* Apply a batch in place on the device (MOSP applyChangeBatch semantics under the graph's
+ * batch_semantics; apply_host.cpp:57-401 including the weight-increase classification of
+ * :348-394). The effective deletions and insertions (after Step 0) and the per-objective
+ * weight-increase flags stay on the device for the engine (dyng_hip_graph_changes). */
+int dyng_hip_graph_apply(dyng_hip_graph* g, const dyng_hip_batch* batch, void* stream,
+                         dyng_hip_apply_summary* summary);
+
+/* The change list of the last apply, on the device (sssp.cpp:563-576 build_changes, consumed in
+ * place: deletions, insertions and their weight-increase flags). Pointers stay valid until the
+ * next apply. */
+typedef struct dyng_hip_changes {
+  const int32_t* del_src;
+  const int32_t* del_dst;
+  int64_t num_del;
+  const int32_t* ins_src;
+  const int32_t* ins_dst;
+  const uint8_t* ins_increased; /* num_ins * num_weights flags */
+  int64_t num_ins;
+  int32_t num_weights;
+} dyng_hip_changes;
+int dyng_hip_graph_changes(const dyng_hip_graph* g, dyng_hip_changes* out);
+
+/* Device scratch of the engine (sssp_cuda_workspace, problem.hpp:219-268): sized once for n
+ * vertices, reused; stamps, in_far and flags are zero between runs. */
+typedef struct dyng_hip_workspace dyng_hip_workspace;
+int dyng_hip_workspace_create(int device, dyng_hip_workspace** out);
+int dyng_hip_workspace_destroy(dyng_hip_workspace* ws);
+
+/* Schedule knobs (the reference's only ones are delta and cuda_engine, sssp.hpp:78-94; they
+ * change the schedule, never the result). 0 = default. */
+typedef struct dyng_hip_tuning {
+  int32_t engine;          /* DYNG_HIP_ENGINE_* */
+  int32_t block_size;      /* threads per workgroup of the persistent kernel (256, 512, 1024) */
+  int32_t blocks_per_cu;   /* resident workgroups per CU (<= occupancy) */
+  int32_t light_degree;    /* rows up to this length: one thread per vertex */
+  int32_t chunk_edges;     /* longer rows: split in chunks of this many edges, one wave each */
+} dyng_hip_tuning;
+
+/* One engine run (sssp_run, problem.hpp:380-400), all arrays in device memory. */
+typedef struct dyng_hip_sssp_run {
+  const dyng_hip_graph* graph;
+  int32_t objective;
+  int32_t source;
+  int64_t delta;      /* near-far width (> 0) */
+  int64_t max_weight; /* largest weight of the objective before or after the batch (>= 1) */
+  int64_t* distances; /* n; in: old tree (update); out: new tree */
+  int32_t* parents;   /* n */
+  int32_t check_cycles; /* update: run the parent-cycle check even when no tree edge changed */
+  dyng_hip_workspace* workspace;
+  dyng_hip_tuning tuning;
+} dyng_hip_sssp_run;
+
+/* sssp_counters (problem.hpp:106-113) plus the relax kernel's work counters. */
+typedef struct dyng_hip_counters {
+  int64_t invalidated, iterations, epochs, pushes, affected;
+  int32_t packed_parents;
+  int32_t engine_used;
+  uint32_t device_errors;
+  int64_t relax_vertices; /* P: near-frontier vertices expanded */
+  int64_t relax_edges;    /* E_s: out-edges scanned by the expansions */
+  int64_t relax_improved; /* I: improving atomic minima */
+  int64_t pull_edges;     /* in-edges scanned by the pull pass */
+  int64_t jump_rounds;    /* pointer-jumping rounds run */
+  int64_t jump_items;     /* vertices visited over all pointer-jumping rounds */
+  int64_t pull_vertices;  /* candidates of the pull pass (invalidated + insertion heads) */
+  double step1_ms;        /* device time of Step 1 (pack, invalidation, pull), s_memrealtime */
+  double pull_ms;         /* of which the pull pass */
+  double relax_ms;        /* device time of the near-far propagation (the relax iterations) */
+  int64_t blocks;         /* workgroups of the persistent launch */
+  double kernel_ms;       /* the persistent kernel's duration (HIP events on its stream) */
+} dyng_hip_counters;
+
+/* sssp_cuda_update (cuda.cu:271-307): Step 1 + Step 2 on the graph after dyng_hip_graph_apply,
+ * using its change list. One host synchronization (the control block read-back). */
+int dyng_hip_sssp_update(const dyng_hip_sssp_run* run, void* stream, dyng_hip_counters* out);
+/* sssp_cuda_compute (cuda.cu:309-335): the canonical tree from the source. */
+int dyng_hip_sssp_compute(const dyng_hip_sssp_run* run, void* stream, dyng_hip_counters* out);
+
+/* Device memory and synchronization for callers without their own HIP runtime plumbing (the
+ * reference does this through rmm-like memory resources, cpp/include/dyng/core/memory.hpp):
+ * device count, allocation, host<->device copies, a device-wide synchronize. */
+int dyng_hip_device_count(int* count);
+int dyng_hip_malloc(int device, int64_t bytes, void** out);
+int dyng_hip_free(void* p);
+int dyng_hip_memcpy_htod(void* dst, const void* src, int64_t bytes);
+int dyng_hip_memcpy_dtoh(void* dst, const void* src, int64_t bytes);
+int dyng_hip_synchronize(int device);
+
+/* ============================================================================================
+ * (2) API layer (python/bindings/sssp.cpp:77-135)
+ * ==========================================================================================*/
+
+typedef struct dyng_hip_options { /* sssp::options (sssp.hpp:78-94) */
+  int64_t delta;
+  int32_t objective;
+  int32_t cuda_engine; /* DYNG_HIP_ENGINE_* */
+  int32_t validate_inputs;
+} dyng_hip_options;
+
+typedef struct dyng_hip_stats { /* sssp::stats (sssp.hpp:100-117) + update_stats */
+  int64_t affected, iterations, frontier_visits;
+  int32_t fallback_used, converged, engine_used, packed_parents;
+  dyng_hip_apply_summary batch;
+  int64_t invalidated, epochs, pushes;
+  dyng_hip_counters device; /* the engine run's counters */
+  double apply_ms, engine_ms; /* stream time of the device apply and of the engine (events) */
+} dyng_hip_stats;
+
+typedef struct dyng_hip_result dyng_hip_result;
+
+/* A graph handle with its version (graph::version(), graph.hpp). */
+int dyng_hip_graph_version(const dyng_hip_graph* g, uint64_t* version);
+
+/* native.sssp_compute (bindings/sssp.cpp:77-96 -> sssp::compute, sssp.hpp:415-421). */
+int dyng_hip_api_compute(dyng_hip_graph* g, int32_t source, const dyng_hip_options* opt,
+                         void* stream, dyng_hip_result** out);
+/* native.sssp_update (bindings/sssp.cpp:98-110 -> sssp::update, sssp.hpp:484-489). */
+int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batch, dyng_hip_result* r,
+                        void* stream, dyng_hip_stats* stats);
+/* native.sssp_from_arrays (bindings/sssp.cpp:112-135 -> result::from_arrays,
+ * sssp.cpp:1118-1198): host arrays, imported, canonicalized and validated on the host. */
+int dyng_hip_api_from_arrays(dyng_hip_graph* g, int32_t source, const int64_t* distances,
+                             const int32_t* parents, int32_t canonicalize,
+                             const dyng_hip_options* opt, void* stream, dyng_hip_result** out);
+int dyng_hip_result_destroy(dyng_hip_result* r);
+int dyng_hip_result_clone(const dyng_hip_result* r, void* stream, dyng_hip_result** out);
+/* result accessors (sssp.hpp:183-234): device arrays of n entries. */
+int dyng_hip_result_arrays(const dyng_hip_result* r, const int64_t** distances,
+                           const int32_t** parents, int64_t* n);
+int dyng_hip_result_source(const dyng_hip_result* r, int32_t* source);
+int dyng_hip_result_version(const dyng_hip_result* r, uint64_t* version);
+int dyng_hip_result_options(const dyng_hip_result* r, dyng_hip_options* opt);
+int dyng_hip_result_set_options(dyng_hip_result* r, const dyng_hip_options* opt);
+/* Copy the arrays to host memory (to_vector). */
+int dyng_hip_result_download(const dyng_hip_result* r, int64_t* distances, int32_t* parents);
+/* Schedule knobs of this result's runs (not part of sssp::options). */
+int dyng_hip_result_set_tuning(dyng_hip_result* r, const dyng_hip_tuning* tuning);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DYNG_HIP_H */

@@ -1,0 +1,365 @@
+"""Parity of the HIP path (libdyng_hip.so through dyng_amd) with the oracle and the reference's
+golden data. Needs an MI355X: every test here is marked gpu.
+
+The bar is bit-exactness: distances (int64), parents, `invalidated`, `affected` and the apply
+summary equal the oracle's (the sequential restatement of the reference engine) after every batch;
+the device rows equal the oracle's MOSP apply row for row (as multisets ordered by neighbour id).
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import fixture_io as fio
+
+pytestmark = pytest.mark.gpu
+
+INF = oracle.INF
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dg():
+    import dyng_amd
+
+    dyng_amd._lib.lib()  # loud failure if the HIP library is missing
+    assert dyng_amd.device_count() > 0, "no HIP device visible"
+    return dyng_amd
+
+
+def _props_pair(dg, directed=True, sorted_rows=True, allow_parallel=False, self_loop="keep",
+                growth=True, existing="upsert", deletions_first=True):
+    loops = {"keep": 0, "drop": 1, "error": 2}
+    ins = {"upsert": 0, "ignore": 2}
+    op = oracle.Props(directed=directed, sorted_rows=sorted_rows, allow_parallel=allow_parallel,
+                      on_existing_insert=ins[existing], on_self_loop=loops[self_loop],
+                      deletions_first=deletions_first, allow_vertex_growth=growth)
+    sem = dg.BatchSemantics(on_existing_insert=existing, on_self_loop=self_loop,
+                            deletions_first=deletions_first, allow_vertex_growth=growth)
+    gp = dg.GraphProperties(directed=directed, order="sorted" if sorted_rows else "append",
+                            parallel_edges="allow" if allow_parallel else "forbid", semantics=sem)
+    return op, gp
+
+
+def _sorted_rows(row, col, w, by_weight=False):
+    """Order every row by neighbour id, stable (or by (neighbour, weights) for in-rows)."""
+    n = len(row) - 1
+    rows = np.repeat(np.arange(n), np.diff(row))
+    keys = [np.arange(len(col))]
+    if by_weight:
+        keys = [w[k] for k in range(w.shape[0] - 1, -1, -1)]
+    order = np.lexsort(tuple(keys) + (col, rows))
+    return row, col[order], w[:, order]
+
+
+def _assert_same_rows(hg, og: oracle.Csr):
+    hr, hc, hw = hg.to_csr()
+    orow, ocol, ow = _sorted_rows(og.row_ptr, og.col, og.w)
+    assert np.array_equal(hr, orow), "row offsets"
+    assert np.array_equal(hc, ocol), "neighbours"
+    assert np.array_equal(hw, ow), "weights"
+    ir, ic, iw = hg.in_csr()
+    t = oracle.transpose(og)
+    trow, tcol, tw = _sorted_rows(t.row_ptr, t.col, t.w, by_weight=True)
+    _, ic2, iw2 = _sorted_rows(ir, ic, iw, by_weight=True)
+    assert np.array_equal(ir, trow) and np.array_equal(ic2, tcol) and np.array_equal(iw2, tw), "in-rows"
+
+
+# ------------------------------------------------------------------------------------------------
+# Golden data of the reference (cpp/tests/data/mosp_sssp, 34 cases)
+# ------------------------------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("name,inp,K", fio.sssp_cases(), ids=[c[0] for c in fio.sssp_cases()])
+def test_golden_compute_and_update(dg, name, inp, K):
+    _say(f"[golden] {name} K={K}")
+    row, col, w = fio.read_csr(os.path.join(inp, "graphCsr"), K)
+    ins_src, ins_dst, ins_w, del_src, del_dst = fio.read_batch(inp, K)
+    expected = os.path.join(fio.GOLDEN, "mosp_sssp", name)
+    props = dg.GraphProperties.mosp_compatible()
+    for k in range(K):
+        obj = f"obj{k}"
+        g = dg.Graph.from_csr(row, col, w.T, properties=props)
+        tree = dg.sssp.compute(g, 0, objective=k)
+        d, p = tree.to_numpy()
+        assert fio.distances_text(d) == fio.read_text(os.path.join(expected, "init", obj, "distancesOriginal.txt"))
+        assert fio.parents_text(p) == fio.read_text(os.path.join(expected, "init", obj, "SSSPTreeOriginal.txt"))
+        d0 = fio.read_distances(os.path.join(expected, "init", obj, "distancesOriginal.txt"), g.num_vertices)
+        p0 = fio.read_parents(os.path.join(expected, "init", obj, "SSSPTreeOriginal.txt"), g.num_vertices)
+        r = dg.sssp.Result.from_arrays(g, 0, d0, p0, canonicalize=False, objective=k)
+        batch = dg.EdgeBatch(insert=(ins_src, ins_dst, ins_w), delete=(del_src, del_dst))
+        st = dg.sssp.update(g, batch, r)
+        d1, p1 = r.to_numpy()
+        assert fio.distances_text(d1) == fio.read_text(os.path.join(expected, "updated", obj, "distancesUpdated.txt"))
+        assert fio.parents_text(p1) == fio.read_text(os.path.join(expected, "updated", obj, "SSSPTreeUpdated.txt"))
+        assert st.invalidated == fio.expected_invalidated(name, k)
+        assert r.graph_version == g.version == 1
+
+
+@pytest.mark.parametrize("case", fio.apply_cases())
+def test_golden_device_apply(dg, case):
+    _say(f"[apply] {case}")
+    base = os.path.join(fio.GOLDEN, "mosp_graph_io", case)
+    with open(os.path.join(base, "graphCsrValues.txt")) as f:
+        K = next(len(line.split()) for line in f if line.split())
+    row, col, w = fio.read_csr(os.path.join(base, "graphCsr"), K)
+    g = dg.Graph.from_csr(row, col, w.T, properties=dg.GraphProperties.mosp_compatible())
+    ins_src, ins_dst, ins_w, del_src, del_dst = fio.read_batch(base, K)
+    summary = g.apply(dg.EdgeBatch(insert=(ins_src, ins_dst, ins_w), delete=(del_src, del_dst)))
+    erow, ecol, ew = fio.read_csr(os.path.join(base, "applied", "graphCsr"), K)
+    _assert_same_rows(g, oracle.Csr(erow, ecol, ew))
+    a = oracle.apply_batch(oracle.Csr(row, col, w), ins_src, ins_dst, ins_w, del_src, del_dst,
+                           oracle.Props.mosp_compatible())
+    for f in ("inserted_edges", "updated_edges", "deleted_edges", "ignored_deletions"):
+        assert getattr(summary, f) == a.summary[f], f
+
+
+# ------------------------------------------------------------------------------------------------
+# Hand cases (cpp/tests/algorithms/sssp/sssp_test.cpp, python/tests/test_sssp.py)
+# ------------------------------------------------------------------------------------------------
+
+
+def _g(dg, n, edges, props=None):
+    e = np.array(edges, np.int64).reshape(-1, 3)
+    return dg.Graph.from_edges(e[:, 0], e[:, 1], e[:, 2], num_vertices=n,
+                               properties=props or dg.GraphProperties.mosp_compatible())
+
+
+def test_hand_ties(dg):
+    g = _g(dg, 5, [(0, 1, 5), (0, 2, 1), (2, 3, 1), (1, 4, 2), (3, 4, 2)])
+    r = dg.sssp.compute(g, 0)
+    assert r.distances.tolist() == [0, 5, 1, 2, 4] and r.parents.tolist() == [-1, 0, 0, 2, 3]
+    st = dg.sssp.update(g, dg.EdgeBatch(insert=([2], [1], [1])), r)
+    assert r.distances.tolist() == [0, 2, 1, 2, 4] and r.parents.tolist() == [-1, 2, 0, 2, 1]
+    assert st.invalidated == 0 and st.affected == 2 and st.batch.inserted_edges == 1 and st.packed_parents
+
+
+def test_hand_disconnect(dg):
+    g = _g(dg, 4, [(0, 1, 1), (0, 3, 50), (1, 2, 1), (2, 1, 1), (3, 1, 50)])
+    r = dg.sssp.compute(g, 0)
+    st = dg.sssp.update(g, dg.EdgeBatch(delete=([0], [1])), r)
+    assert r.distances.tolist() == [0, 100, 101, 50] and r.parents.tolist() == [-1, 3, 1, 0]
+    assert st.invalidated == 2 and st.affected == 2
+
+
+def test_hand_delete_all_and_reinsert(dg):
+    g = _g(dg, 5, [(0, 1, 3), (1, 2, 4), (0, 3, 2), (3, 4, 7)])
+    r = dg.sssp.compute(g, 0)
+    st = dg.sssp.update(g, dg.EdgeBatch(delete=([0, 1, 0, 3], [1, 2, 3, 4])), r)
+    assert g.num_edges == 0
+    assert r.distances.tolist() == [0, INF, INF, INF, INF] and r.parents.tolist() == [-1] * 5
+    assert st.invalidated == 4 and st.affected == 4
+    dg.sssp.update(g, dg.EdgeBatch(insert=([0, 4], [4, 2], [9, 1])), r)
+    assert r.distances.tolist() == [0, INF, 10, INF, 9] and r.parents.tolist() == [-1, -1, 4, -1, 0]
+
+
+def test_hand_empty_batch(dg):
+    g = _g(dg, 4, [(0, 1, 2), (1, 2, 2), (0, 2, 4), (2, 3, 1)])
+    r = dg.sssp.compute(g, 0)
+    d, p = r.to_numpy()
+    st = dg.sssp.update(g, dg.EdgeBatch(), r)
+    assert st.affected == 0 and st.invalidated == 0 and st.iterations == 0
+    d2, p2 = r.to_numpy()
+    assert np.array_equal(d, d2) and np.array_equal(p, p2)
+    assert g.version == 1 and r.graph_version == 1
+
+
+def test_hand_weight_increase(dg):
+    g = _g(dg, 3, [(0, 1, 1), (1, 2, 1), (0, 2, 5)])
+    r = dg.sssp.compute(g, 0)
+    st = dg.sssp.update(g, dg.EdgeBatch(insert=([0], [1], [10])), r)
+    assert st.invalidated == 2 and st.batch.updated_edges == 1
+    assert r.distances.tolist() == [0, 10, 5] and r.parents.tolist() == [-1, 0, 0]
+
+
+def test_hand_parallel_edges_and_self_loops(dg):
+    g = _g(dg, 3, [(0, 1, 7), (0, 1, 3), (1, 1, 1), (1, 2, 1)])
+    r = dg.sssp.compute(g, 0)
+    assert r.distances.tolist() == [0, 3, 4]
+    st = dg.sssp.update(g, dg.EdgeBatch(delete=([0], [1])), r)
+    assert st.invalidated == 2 and r.distances.tolist() == [0, 3, 4]
+    dg.sssp.update(g, dg.EdgeBatch(delete=([0], [1])), r)
+    assert r.distances.tolist() == [0, INF, INF]
+
+
+def test_hand_vertex_growth(dg):
+    g = _g(dg, 3, [(0, 1, 2), (1, 2, 2)], dg.GraphProperties())
+    r = dg.sssp.compute(g, 0)
+    st = dg.sssp.update(g, dg.EdgeBatch(insert=([2, 5], [5, 3], [1, 4])), r)
+    assert g.num_vertices == 6 and st.batch.num_vertices_after == 6
+    assert r.distances.tolist() == [0, 2, 4, 9, INF, 5] and r.parents.tolist() == [-1, 0, 1, 5, -1, 2]
+
+
+def test_hand_undirected(dg):
+    g = _g(dg, 4, [(0, 1, 1), (1, 2, 1), (2, 3, 1)], dg.GraphProperties(directed=False))
+    r = dg.sssp.compute(g, 3)
+    assert r.distances.tolist() == [3, 2, 1, 0]
+    dg.sssp.update(g, dg.EdgeBatch(insert=([0], [3], [5]), delete=([1], [2])), r)
+    assert r.distances.tolist() == [5, 6, 1, 0] and r.parents.tolist() == [3, 0, 3, -1]
+
+
+def test_hand_python_quickstart(dg):
+    g = dg.Graph.from_edges([0, 0, 1, 2], [1, 2, 2, 3], [4, 1, 1, 5])
+    r = dg.sssp.compute(g, 0)
+    assert r.distances.tolist() == [0, 4, 1, 6] and r.parents.tolist() == [-1, 0, 0, 2]
+    st = dg.sssp.update(g, dg.EdgeBatch(insert=([1, 3], [3, 4], [1, 2]), delete=([0], [2])), r)
+    assert st.batch.inserted_edges == 2 and st.batch.deleted_edges == 1 and st.invalidated == 2
+    fresh = dg.sssp.compute(g, 0)
+    assert r.distances.tolist() == fresh.distances.tolist() == [0, 4, 5, 5, 7]
+    assert r.parents.tolist() == fresh.parents.tolist()
+
+
+def test_errors_and_stale_results(dg):
+    g = _g(dg, 4, [(0, 1, 4), (0, 2, 1), (1, 2, 1), (2, 3, 5)], dg.GraphProperties())
+    with pytest.raises(dg.InvalidArgumentError, match="out of range"):
+        dg.sssp.compute(g, 9)
+    r = dg.sssp.compute(g, 0)
+    other = dg.sssp.compute(g, 0)
+    dg.sssp.update(g, dg.EdgeBatch(delete=([0], [2])), r)
+    with pytest.raises(dg.StaleResultError):
+        dg.sssp.update(g, dg.EdgeBatch(delete=([0], [1])), other)
+    with pytest.raises(dg.InvalidArgumentError):
+        dg.sssp.update(g, dg.EdgeBatch(insert=([0], [3], [0])), r)  # weight below 1: nothing changes
+    assert g.version == 1
+    with pytest.raises(dg.InvalidArgumentError):
+        dg.sssp.Result.from_arrays(g, 0, [3, 4, 1, 6], [-1, 0, 0, 2])  # source not at 0
+    with pytest.raises(dg.InvalidArgumentError):  # a parent cycle
+        dg.sssp.Result.from_arrays(g, 0, [0, 4, 1, 6], [-1, 3, 0, 1], canonicalize=False)
+    # an unvalidated cyclic tree: the device check reports the cycle and poisons the result
+    bad = dg.sssp.Result.from_arrays(g, 0, [0, 4, INF, 6], [-1, 3, -1, 1], canonicalize=False,
+                                     validate_inputs=False)
+    with pytest.raises(dg.InvalidArgumentError, match="cycle"):
+        dg.sssp.update(g, dg.EdgeBatch(delete=([0], [1])), bad)
+    with pytest.raises(dg.StaleResultError):
+        bad.to_numpy()
+
+
+def test_options_and_clone(dg):
+    e = np.array([(0, 1, 1, 10), (1, 2, 1, 10), (0, 2, 5, 1)])
+    g = dg.Graph.from_edges(e[:, 0], e[:, 1], e[:, 2:4], properties=dg.GraphProperties.mosp_compatible())
+    r1 = dg.sssp.compute(g, 0, objective=1, delta=5)
+    r0 = dg.sssp.compute(g, 0)
+    assert r0.distances.tolist() == [0, 1, 2] and r1.distances.tolist() == [0, 10, 1]
+    assert r1.options.objective == 1 and r1.options.delta == 5
+    r1.set_options(delta=7)
+    assert r1.options.delta == 7
+    with pytest.raises(dg.InvalidArgumentError):
+        r1.set_options(objective=0)
+    c = r0.clone()
+    st0 = dg.sssp.update(g, dg.EdgeBatch(insert=([0], [2], [[5, 30]])), r0)
+    assert st0.invalidated == 0
+    assert c.distances.tolist() == [0, 1, 2] and c.graph_version == 0
+
+
+# ------------------------------------------------------------------------------------------------
+# Random multi-batch streams against the oracle
+# ------------------------------------------------------------------------------------------------
+
+STREAM_CASES = [
+    # (seed, n, m, wmax, props)
+    (1, 40, 160, 9, {}),
+    (2, 200, 1200, 100, {}),
+    (3, 120, 900, 5, {"allow_parallel": True, "sorted_rows": False}),
+    (4, 150, 700, 20, {"directed": False}),
+    (5, 300, 2000, 1000, {"self_loop": "drop"}),
+    (6, 100, 500, 7, {"existing": "ignore"}),
+    (7, 80, 400, 12, {"deletions_first": False, "allow_parallel": True, "sorted_rows": False}),
+    (8, 2000, 16000, 255, {}),
+]
+
+
+@pytest.mark.parametrize("seed,n,m,wmax,kw", STREAM_CASES, ids=[f"s{c[0]}" for c in STREAM_CASES])
+def test_random_streams_match_the_oracle(dg, seed, n, m, wmax, kw):
+    _say(f"[stream] seed={seed} n={n} m={m} {kw}")
+    rng = np.random.default_rng(seed)
+    op, gp = _props_pair(dg, **kw)
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    w = rng.integers(1, wmax + 1, m)
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1), op)
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n, properties=gp)
+    _assert_same_rows(hg, og)
+    source = int(rng.integers(0, n))
+    ot, _ = oracle.compute(og, source)
+    hr = dg.sssp.compute(hg, source)
+    d, p = hr.to_numpy()
+    assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent)
+    for b in range(4):
+        ni, nd = int(rng.integers(0, m // 8 + 2)), int(rng.integers(0, m // 8 + 2))
+        grow = 3 if b == 2 else 0
+        ins = (rng.integers(0, n + grow, ni), rng.integers(0, n + grow, ni), rng.integers(1, wmax + 1, ni))
+        rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+        pick = rng.integers(0, og.m, nd) if og.m else np.zeros(0, np.int64)
+        dels = (np.concatenate([rows[pick], rng.integers(0, n, 2)]),
+                np.concatenate([og.col[pick], rng.integers(0, n, 2)]))  # plus possibly missing ones
+        og, a, c = oracle.update(og, ot, ins[0], ins[1], ins[2].reshape(-1, 1), dels[0], dels[1], props=op)
+        st = dg.sssp.update(hg, dg.EdgeBatch(insert=ins, delete=dels), hr)
+        d, p = hr.to_numpy()
+        assert np.array_equal(d, ot.dist), f"distances, batch {b}"
+        assert np.array_equal(p, ot.parent), f"parents, batch {b}"
+        assert st.invalidated == c.invalidated and st.affected == c.affected
+        for f in ("inserted_edges", "updated_edges", "deleted_edges", "ignored_deletions",
+                  "dropped_self_loops", "ignored_insertions", "num_vertices_after"):
+            assert getattr(st.batch, f) == a.summary[f], f
+        _assert_same_rows(hg, og)
+
+
+@pytest.mark.parametrize("light,chunk,bpc", [(1, 16, 1), (4, 64, 2), (64, 1024, 0)])
+def test_schedule_knobs_never_change_the_result(dg, light, chunk, bpc):
+    _say(f"[tuning] light={light} chunk={chunk} bpc={bpc}")
+    rng = np.random.default_rng(99)
+    n, m = 3000, 40000
+    src = np.concatenate([rng.integers(0, n, m), np.zeros(3000, np.int64)])  # a hub row
+    dst = np.concatenate([rng.integers(0, n, m), np.arange(3000)])
+    w = rng.integers(1, 50, len(src))
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    ot, _ = oracle.compute(og, 0)
+    hr = dg.sssp.compute(hg, 0)
+    hr.set_tuning(dg.sssp.Tuning(light_degree=light, chunk_edges=chunk, blocks_per_cu=bpc))
+    rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+    for _ in range(3):
+        pick = rng.integers(0, og.m, 300)
+        ins = (rng.integers(0, n, 300), rng.integers(0, n, 300), rng.integers(1, 50, 300))
+        dels = (rows[pick], og.col[pick])
+        og, a, c = oracle.update(og, ot, ins[0], ins[1], ins[2].reshape(-1, 1), dels[0], dels[1])
+        rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+        st = dg.sssp.update(hg, dg.EdgeBatch(insert=ins, delete=dels), hr)
+        d, p = hr.to_numpy()
+        assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent)
+        assert st.invalidated == c.invalidated and st.affected == c.affected
+
+
+def test_device_resident_batch(dg):
+    """A batch in device memory is applied in place and gives the host batch's result."""
+    rng = np.random.default_rng(17)
+    n, m = 500, 4000
+    src, dst, w = rng.integers(0, n, m), rng.integers(0, n, m), rng.integers(1, 30, m)
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    ot, _ = oracle.compute(og, 0)
+    hr = dg.sssp.compute(hg, 0)
+    rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+    pick = rng.integers(0, og.m, 200)
+    ins = (rng.integers(0, n, 150), rng.integers(0, n, 150), rng.integers(1, 30, 150))
+    dels = (rows[pick], og.col[pick])
+    og, a, c = oracle.update(og, ot, ins[0], ins[1], ins[2].reshape(-1, 1), dels[0], dels[1])
+
+    def D(x):
+        return dg.DeviceArray(np.asarray(x, np.int32))
+
+    batch = dg.EdgeBatch(insert=(D(ins[0]), D(ins[1]), D(ins[2])), delete=(D(dels[0]), D(dels[1])))
+    assert batch.on_device
+    st = dg.sssp.update(hg, batch, hr)
+    d, p = hr.to_numpy()
+    assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent)
+    assert st.invalidated == c.invalidated and st.affected == c.affected
