@@ -61,7 +61,8 @@ class Counters(C.Structure):
         ("relax_vertices", C.c_int64), ("relax_edges", C.c_int64), ("relax_improved", C.c_int64),
         ("pull_edges", C.c_int64), ("jump_rounds", C.c_int64), ("jump_items", C.c_int64),
         ("pull_vertices", C.c_int64), ("step1_ms", C.c_double), ("pull_ms", C.c_double),
-        ("relax_ms", C.c_double), ("blocks", C.c_int64), ("kernel_ms", C.c_double)]
+        ("relax_ms", C.c_double), ("blocks", C.c_int64), ("kernel_ms", C.c_double), ("pack_ms", C.c_double),
+        ("jump_ms", C.c_double), ("unpack_ms", C.c_double)]
 
 
 class Options(C.Structure):

@@ -92,9 +92,12 @@ struct grid_sync_state {
 };
 
 DYNG_DEV void grid_barrier(grid_sync_state* s) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  // The workgroup barrier orders every wave's accesses before thread 0's agent-scope release
+  // (which writes back the XCD's L2), and thread 0's acquire (which invalidates the caches) before
+  // every wave's later loads: one fence pair per workgroup, as HIP's own grid sync does.
   __syncthreads();
   if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned int g = __hip_atomic_load(&s->generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int arrived =
         __hip_atomic_fetch_add(&s->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -106,9 +109,9 @@ DYNG_DEV void grid_barrier(grid_sync_state* s) {
         __builtin_amdgcn_s_sleep(1);
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 }  // namespace dyng_hip

@@ -29,7 +29,7 @@ namespace dyng_hip {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 1024;  ///< 16 waves per workgroup; 8 waves/SIMD fit two per CU, one is launched
 
 struct control {
   int ring_near[3];
@@ -37,6 +37,7 @@ struct control {
   int ring_jump[3];
   int far_count[2];
   int cand_count;
+  int heads_count;
   int frontier_count;
   int roots;
   int overflow;
@@ -55,7 +56,7 @@ struct control {
   unsigned long long pull_edges;
   unsigned long long jump_items;
   unsigned long long pull_vertices;
-  unsigned long long t_phase[4];  ///< s_memrealtime ticks: start, before pull, before near-far, end
+  unsigned long long t_phase[8];  ///< s_memrealtime ticks: start, pull, near-far, unpack, after A, B, C, end
   unsigned int error_bits;
 };
 
@@ -216,21 +217,30 @@ DYNG_DEV u64 pull_offer(const engine_params& p, int32_t u, int32_t wt) {
   return nd <= p.max_distance ? p.pk.pack(nd, u) : packed_inf;
 }
 
-DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best, int generation, int* counter) {
+/// Lower v's word to its best pull offer; v joins the frontier if its distance fell. The
+/// smallest lowered distance goes to *minimum (the first near-far threshold).
+DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best, int generation, int* counter,
+                          u64* minimum) {
   bool add = false;
+  u64 low = packed_inf;
   if (pred) {
     const u64 current = ld(&p.packed[v]);
     if (best < current) {
       const u64 old = atomicMin(&p.packed[v], best);
-      add = p.pk.distance(best) < p.pk.distance(old) && claim(p.stamp, v, generation);
+      if (p.pk.distance(best) < p.pk.distance(old)) {
+        low = p.pk.distance(best);
+        add = claim(p.stamp, v, generation);
+      }
     }
   }
   append(p.frontier, counter, add, v);
+  low = wave_min(low);
+  if (lane() == 0 && low != packed_inf) atomicMin(minimum, low);
 }
 
 }  // namespace
 
-__global__ void __launch_bounds__(kBlock) sssp_fused_kernel(engine_params p) {
+__global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int threads = gridDim.x * blockDim.x;
   const int n = p.n;
@@ -277,9 +287,11 @@ __global__ void __launch_bounds__(kBlock) sssp_fused_kernel(engine_params p) {
         changed = v >= 0 && p.ins_inc[j * p.inc_stride + p.inc_k] != 0;
       }
       const bool root = changed && p.parents[v] == u && atomicExch(&p.flag[v], 1) == 0;
+      if (root) st(&p.stamp[v], generation + 1);
       append(p.cand, &c->cand_count, root, v);
     }
     grid_barrier(p.bar);
+    if (tid == 0) c->t_phase[4] = __builtin_amdgcn_s_memrealtime();
     if (ld(&c->overflow) != 0) return;  // uniform: the host reports invalid_input
     const int roots = ld(&c->cand_count);
     const bool jump = (p.num_del + p.num_ins > 0) && (roots > 0 || p.check_cycles);
@@ -306,6 +318,7 @@ __global__ void __launch_bounds__(kBlock) sssp_fused_kernel(engine_params p) {
             if (a >= 0 && ld(&p.flag[v]) == 0) {
               if (ld(&p.flag[a]) != 0) {
                 st(&p.flag[v], 1);
+                st(&p.stamp[v], generation + 1);
                 flagged = true;
               } else {
                 const int32_t next = ld(&p.ancestor[a]);
@@ -327,17 +340,18 @@ __global__ void __launch_bounds__(kBlock) sssp_fused_kernel(engine_params p) {
         return;  // uniform
       }
     }
-    // ---- C: invalidate the flagged vertices; then the insertion heads ----
+    // ---- C: invalidate the flagged vertices (stamped when flagged) and claim the insertion
+    // heads that are not invalidated, in one phase ----
+    if (tid == 0) c->t_phase[5] = __builtin_amdgcn_s_memrealtime();
     ++generation;
     const int inv = ld(&c->cand_count);
     for (int i = tid; i < inv; i += threads) {
       const int32_t v = p.cand[i];
       p.flag[v] = 0;
       p.packed[v] = packed_inf;
-      p.stamp[v] = generation;
     }
     if (tid == 0) st(&c->invalidated, inv);
-    grid_barrier(p.bar);
+    // the heads go after the invalidated vertices, counted apart: cand_count stays fixed here
     for (int64_t base = tid - lane(); base < p.num_ins; base += threads) {
       const int64_t i = base + lane();
       int32_t v = -1;
@@ -346,12 +360,12 @@ __global__ void __launch_bounds__(kBlock) sssp_fused_kernel(engine_params p) {
         v = p.ins_dst[i];
         add = v >= 0 && v != p.source && claim(p.stamp, v, generation);
       }
-      append(p.cand, &c->cand_count, add, v);
+      append(p.cand + inv, &c->heads_count, add, v);
     }
     grid_barrier(p.bar);
     // ---- D: pull ----
     ++generation;
-    const int cands = ld(&c->cand_count);
+    const int cands = inv + ld(&c->heads_count);
     if (tid == 0) {
       c->t_phase[1] = __builtin_amdgcn_s_memrealtime();
       c->jump_items = jump_items;
@@ -377,7 +391,7 @@ __global__ void __launch_bounds__(kBlock) sssp_fused_kernel(engine_params p) {
           best = o < best ? o : best;
         }
       }
-      pull_commit(p, light, v, best, generation, &c->frontier_count);
+      pull_commit(p, light, v, best, generation, &c->frontier_count, &c->min_frontier);
     }
     grid_barrier(p.bar);
     const int nchunks = static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap));
@@ -398,9 +412,12 @@ __global__ void __launch_bounds__(kBlock) sssp_fused_kernel(engine_params p) {
           const u64 current = ld(&p.packed[ch.x]);
           if (best < current) {
             const u64 old = atomicMin(&p.packed[ch.x], best);
-            if (p.pk.distance(best) < p.pk.distance(old) && claim(p.stamp, ch.x, generation)) {
-              const int slot = atomicAdd(&c->frontier_count, 1);
-              p.frontier[slot] = ch.x;
+            if (p.pk.distance(best) < p.pk.distance(old)) {
+              atomicMin(&c->min_frontier, p.pk.distance(best));
+              if (claim(p.stamp, ch.x, generation)) {
+                const int slot = atomicAdd(&c->frontier_count, 1);
+                p.frontier[slot] = ch.x;
+              }
             }
           }
         }
@@ -410,17 +427,11 @@ __global__ void __launch_bounds__(kBlock) sssp_fused_kernel(engine_params p) {
     frontier_count = ld(&c->frontier_count);
   }
 
-  // ---- E: near-far propagation ----
+  // ---- E: near-far propagation (the frontier's smallest distance came with the pull, or is the
+  // source's 0) ----
   if (tid == 0) c->t_phase[2] = __builtin_amdgcn_s_memrealtime();
   u64 local = packed_inf;
-  for (int i = tid; i < frontier_count; i += threads) {
-    const u64 word = ld(&p.packed[ld(&p.frontier[i])]);
-    if (word != packed_inf) local = min(local, p.pk.distance(word));
-  }
-  local = wave_min(local);
-  if (lane() == 0 && local != packed_inf) atomicMin(&c->min_frontier, local);
-  grid_barrier(p.bar);
-  const u64 smallest = ld(&c->min_frontier);
+  const u64 smallest = p.from_scratch ? 0 : ld(&c->min_frontier);
   u64 threshold = (smallest == packed_inf ? 0 : smallest) + p.delta;
   int* current = p.near_a;
   int* next = p.near_b;
@@ -590,6 +601,7 @@ __global__ void __launch_bounds__(kBlock) sssp_fused_kernel(engine_params p) {
     if (pull_edges) atomicAdd(&c->pull_edges, pull_edges);
   }
   if (tid == 0) {
+    c->t_phase[7] = __builtin_amdgcn_s_memrealtime();
     c->iterations = iterations;
     c->epochs = epochs;
     c->pushes = pushes;
@@ -773,8 +785,13 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   p.bar = ws->bar.p;
   p.light = run->tuning.light_degree > 0 ? run->tuning.light_degree : 32;
   p.chunk_edges = run->tuning.chunk_edges > 0 ? std::max(16, run->tuning.chunk_edges) : 256;
-  int per_cu = run->tuning.blocks_per_cu > 0 ? std::min(run->tuning.blocks_per_cu, ws->occupancy) : std::min(ws->occupancy, 4);
-  const int blocks = std::max(1, per_cu * ws->cu_count);
+  // Every workgroup must be resident for the grid barrier. The launch stays at half of the
+  // occupancy the runtime reports (one 1024-thread workgroup per CU by default), never at the
+  // edge of it.
+  const int resident = ws->occupancy * ws->cu_count;  // what the runtime says fits at once
+  const int cap = std::max(1, resident / 2);
+  const int per_cu = run->tuning.blocks_per_cu > 0 ? run->tuning.blocks_per_cu : 1;
+  const int blocks = std::max(1, std::min(per_cu * ws->cu_count, cap));
   control* hc = ws->host_ctl.p;
   std::memset(hc, 0, sizeof(control));
   hc->min_frontier = packed_inf;
@@ -834,6 +851,10 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   counters.step1_ms = (t2 >= t0 ? t2 - t0 : 0) * tick_ms;
   counters.pull_ms = (update && t2 >= t1 ? t2 - t1 : 0) * tick_ms;
   counters.relax_ms = (t3 >= t2 ? t3 - t2 : 0) * tick_ms;
+  const unsigned long long ta = hc->t_phase[4], tb = hc->t_phase[5], t7 = hc->t_phase[7];
+  counters.pack_ms = update && ta >= t0 ? (ta - t0) * tick_ms : 0.0;
+  counters.jump_ms = update && tb >= ta ? (tb - ta) * tick_ms : 0.0;
+  counters.unpack_ms = t7 >= t3 ? (t7 - t3) * tick_ms : 0.0;
   counters.blocks = blocks;
   counters.kernel_ms = kernel_ms;
   if (out) *out = counters;

@@ -84,13 +84,17 @@ class DeviceCounters:
     relax_ms: float
     blocks: int
     kernel_ms: float
+    pack_ms: float
+    jump_ms: float
+    unpack_ms: float
 
     @classmethod
     def _from_native(cls, d) -> "DeviceCounters":
         return cls(int(d.invalidated), int(d.iterations), int(d.epochs), int(d.pushes), int(d.affected),
                    bool(d.packed_parents), int(d.relax_vertices), int(d.relax_edges), int(d.relax_improved),
                    int(d.pull_edges), int(d.jump_rounds), int(d.jump_items), int(d.pull_vertices),
-                   float(d.step1_ms), float(d.pull_ms), float(d.relax_ms), int(d.blocks), float(d.kernel_ms))
+                   float(d.step1_ms), float(d.pull_ms), float(d.relax_ms), int(d.blocks), float(d.kernel_ms),
+                   float(d.pack_ms), float(d.jump_ms), float(d.unpack_ms))
 
 
 @dataclass(frozen=True)

@@ -193,6 +193,9 @@ typedef struct dyng_hip_counters {
   double relax_ms;        /* device time of the near-far propagation (the relax iterations) */
   int64_t blocks;         /* workgroups of the persistent launch */
   double kernel_ms;       /* the persistent kernel's duration (HIP events on its stream) */
+  double pack_ms;         /* Step 1 parts (s_memrealtime): packing the old tree and the roots, */
+  double jump_ms;         /* the pointer-jumping rounds, */
+  double unpack_ms;       /* and the final unpack */
 } dyng_hip_counters;
 
 /* sssp_cuda_update (cuda.cu:271-307): Step 1 + Step 2 on the graph after dyng_hip_graph_apply,
