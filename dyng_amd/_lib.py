@@ -37,8 +37,12 @@ SUMMARY_FIELDS = ("inserted_edges", "updated_edges", "deleted_edges", "ignored_d
                   "ignored_insertions", "relocated_rows", "compactions")
 
 
+APPLY_STAGES = ("scan", "make_ops", "sort_out", "pairs_out", "scans_out", "merge_out", "sort_in", "pairs_in",
+                "scans_in", "merge_in")
+
+
 class ApplySummary(C.Structure):
-    _fields_ = [(f, C.c_int64) for f in SUMMARY_FIELDS]
+    _fields_ = [(f, C.c_int64) for f in SUMMARY_FIELDS] + [("stage_ms", C.c_double * len(APPLY_STAGES))]
 
 
 class Batch(C.Structure):
@@ -49,7 +53,7 @@ class Batch(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("engine", C.c_int32), ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("light_degree", C.c_int32), ("chunk_edges", C.c_int32)]
+                ("light_degree", C.c_int32), ("chunk_edges", C.c_int32), ("barrier_timeout_us", C.c_int32)]
 
 
 COUNTER_FIELDS = ("invalidated", "iterations", "epochs", "pushes", "affected")

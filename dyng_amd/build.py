@@ -58,8 +58,23 @@ def build_generators() -> str:
     return GEN_OUT
 
 
+COPY_OUT = os.path.join(HERE, "hbm_copy")
+
+
+def build_copy_probe() -> str:
+    """hbm_copy: the known-size copy that calibrates the HBM traffic counters."""
+    tmp = COPY_OUT + ".tmp"
+    cmd = [hipcc(), "-x", "hip", os.path.join(CSRC, "hbm_copy.hip"), "-o", tmp] + flags()
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building hbm_copy failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, COPY_OUT)
+    return COPY_OUT
+
+
 def build(jobs: int = 3) -> str:
     build_generators()
+    build_copy_probe()
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as pool:
         objs = list(pool.map(_compile, SOURCES))

@@ -107,6 +107,8 @@ struct dyng_hip_result {
   uint64_t graph_state = 0;
   bool poisoned = false;
   bool trusted = true;  ///< a tree this library computed, or an imported one that was validated
+  /// the tree came from this engine (compute / update), so every tree edge is an edge of the graph
+  bool engine_tree = true;
   int64_t n = 0;
   dyng_hip::dbuf<int64_t> distances;
   dyng_hip::dbuf<int32_t> parents;
@@ -136,7 +138,7 @@ dyng_hip_sssp_run make_run(const dyng_hip_graph* g, dyng_hip_result* r, int64_t 
   run.max_weight = max_weight;
   run.distances = r->distances.p;
   run.parents = r->parents.p;
-  run.check_cycles = r->trusted ? 0 : 1;
+  run.check_cycles = r->trusted && r->engine_tree ? 0 : 1;
   run.workspace = pool().get(g->device);
   run.tuning = r->tuning;
   if (r->opt.cuda_engine != DYNG_HIP_ENGINE_AUTOMATIC && run.tuning.engine == 0) run.tuning.engine = r->opt.cuda_engine;
@@ -371,6 +373,7 @@ extern "C" int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batc
     }
     r->version = g->version;
     r->trusted = true;
+    r->engine_tree = true;
     if (stats != nullptr) {
       std::memset(stats, 0, sizeof(*stats));
       fill_stats(stats, c);
@@ -431,6 +434,7 @@ extern "C" int dyng_hip_api_from_arrays(dyng_hip_graph* g, int32_t source, const
     r->version = g->version;
     r->graph_state = g->state_id;
     r->trusted = opt.validate_inputs != 0;
+    r->engine_tree = false;
     r->n = n;
     r->distances.reserve(n);
     r->parents.reserve(n);
@@ -459,6 +463,7 @@ extern "C" int dyng_hip_result_clone(const dyng_hip_result* r, void* stream, dyn
     c->version = r->version;
     c->graph_state = r->graph_state;
     c->trusted = r->trusted;
+    c->engine_tree = r->engine_tree;
     c->tuning = r->tuning;
     c->n = r->n;
     c->distances.reserve(r->n);

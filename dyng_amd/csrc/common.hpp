@@ -46,6 +46,15 @@ DYNG_DEV void st(T* p, T v) {
 
 DYNG_DEV int lane() { return static_cast<int>(__lane_id()); }
 
+/// A value that is the same on every lane (a grid-wide count read after a barrier), moved to a
+/// scalar register.
+DYNG_DEV int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+DYNG_DEV unsigned long long uniform(unsigned long long v) {
+  const unsigned int lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(v));
+  const unsigned int hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(v >> 32));
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
 DYNG_DEV u64 lanes_below() {
   const int l = lane();
   return l == 0 ? 0ULL : (~0ULL >> (64 - l));
@@ -82,36 +91,58 @@ DYNG_DEV bool claim(int* stamp, int32_t v, int generation) {
   return atomicExch(&stamp[v], generation) != generation;
 }
 
-/// Grid-wide barrier of a persistent kernel whose workgroups are all resident (cooperative
-/// launch). Release fence, one arrival per workgroup on an agent-scope counter, the last arrival
-/// bumps the generation, acquire fence. Per-XCD L2s are written back and invalidated by the
-/// agent-scope fences, so data written before the barrier is visible after it on every XCD.
+/// Grid-wide barrier of a persistent kernel whose workgroups are all resident (the grid is sized
+/// to at most half of what the occupancy calculator says fits at once). Release fence, one
+/// arrival per workgroup on an agent-scope counter, the last arrival bumps the generation,
+/// acquire fence. Per-XCD L2s are written back and invalidated by the agent-scope fences, so data
+/// written before the barrier is visible after it on every XCD.
+///
+/// Every wait is bounded: a workgroup that waits longer than `limit` ticks of the 100 MHz
+/// real-time counter raises `abort`, and every workgroup that sees `abort` while waiting leaves
+/// too. The function then returns false and the caller leaves the kernel, so a grid whose
+/// workgroups could not all be resident ends with an error instead of spinning forever.
 struct grid_sync_state {
   unsigned int count;
   unsigned int generation;
+  unsigned int abort;
+  unsigned int pad;
 };
 
-DYNG_DEV void grid_barrier(grid_sync_state* s) {
+DYNG_DEV bool grid_barrier(grid_sync_state* s, unsigned long long limit) {
+  __shared__ int go;
   // The workgroup barrier orders every wave's accesses before thread 0's agent-scope release
   // (which writes back the XCD's L2), and thread 0's acquire (which invalidates the caches) before
-  // every wave's later loads: one fence pair per workgroup, as HIP's own grid sync does.
+  // every wave's later loads: one fence pair per workgroup.
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned int g = __hip_atomic_load(&s->generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int arrived =
         __hip_atomic_fetch_add(&s->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
     if (arrived == gridDim.x - 1) {
       __hip_atomic_store(&s->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&s->generation, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
+      const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
       while (__hip_atomic_load(&s->generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (__hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          ok = 0;
+          break;
+        }
+        if (__builtin_amdgcn_s_memrealtime() - t0 > limit) {
+          __hip_atomic_store(&s->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
         __builtin_amdgcn_s_sleep(1);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    go = ok;
   }
   __syncthreads();
+  return go != 0;
 }
 
 }  // namespace dyng_hip

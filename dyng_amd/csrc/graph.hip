@@ -1118,8 +1118,29 @@ void copy_rows_to_host(const row_store& rs, int64_t n, int K, int64_t* row_ptr, 
 // The apply (host orchestration)
 // ---------------------------------------------------------------------------------------------
 
+namespace {
+
+void mark_stage(dyng_hip_graph* g, int i, hipStream_t s) {
+  if (g->apply_ev[i] == nullptr) DYNG_HIP_TRY(hipEventCreate(&g->apply_ev[i]));
+  DYNG_HIP_TRY(hipEventRecord(g->apply_ev[i], s));
+}
+
+/// Stage durations from the recorded events (after the stream has been synchronized).
+void stage_times(dyng_hip_graph* g, bool rows, dyng_hip_apply_summary* out) {
+  for (double& t : out->stage_ms) t = 0.0;
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, g->apply_ev[0], g->apply_ev[1]) == hipSuccess) out->stage_ms[0] = ms;
+  if (!rows) return;
+  for (int i = 2; i < 11; ++i) {
+    if (hipEventElapsedTime(&ms, g->apply_ev[i], g->apply_ev[i + 1]) == hipSuccess) out->stage_ms[i - 1] = ms;
+  }
+}
+
+}  // namespace
+
 void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch_scan& sc) {
   if (b == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: graph apply: no batch");
+  mark_stage(g, 0, s);
   const int K = g->K;
   const int64_t num_ins = b->num_ins, num_del = b->num_del;
   if (num_ins < 0 || num_del < 0) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: negative batch sizes");
@@ -1164,6 +1185,7 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
                        dim3(kBlock), 0, s, ins_src, ins_dst, ins_w, num_ins, del_src, del_dst, num_del, K, g->ctl.p);
     DYNG_HIP_TRY(hipGetLastError());
   }
+  mark_stage(g, 1, s);
   DYNG_HIP_TRY(hipMemcpyAsync(hc, g->ctl.p, sizeof(apply_control), hipMemcpyDeviceToHost, s));
   DYNG_HIP_TRY(hipStreamSynchronize(s));
   if (hc->min_id < 0) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: the batch names a negative vertex id");
@@ -1272,11 +1294,14 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
   DYNG_HIP_TRY(hipMemcpyAsync(g->ctl.p, hc, sizeof(apply_control), hipMemcpyHostToDevice, s));
   apply_control* dc = g->ctl.p;
   if (N > 0) {
+    mark_stage(g, 2, s);
     hipLaunchKernelGGL(k_make_ops, dim3(blocks_for(N)), dim3(kBlock), 0, s, L, ins_src, ins_dst, del_src, del_dst,
                        g->props.directed, g->props.on_self_loop == 1 ? 1 : 0, n_after, vbits, g->keys_a.p,
                        g->vals_a.p, g->del_src.p, g->del_dst.p, g->ins_src.p, g->ins_dst.p, g->ins_inc.p, K, dc);
     DYNG_HIP_TRY(hipGetLastError());
+    mark_stage(g, 3, s);
     sort_pairs(g->sort_tmp, g->keys_a.p, g->keys_b.p, g->vals_a.p, g->vals_b.p, N, 2 * vbits, s);
+    mark_stage(g, 4, s);
     out_pair_params op{};
     op.L = L;
     op.keys = g->keys_b.p;
@@ -1306,8 +1331,10 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     op.ctl = dc;
     hipLaunchKernelGGL(k_out_pairs, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, op);
     DYNG_HIP_TRY(hipGetLastError());
+    mark_stage(g, 5, s);
     exclusive_scan<int32_t>(g->scan_tmp, g->pend_alive.p, g->pend_prefix.p, N + 1, s);
     exclusive_scan<int32_t>(g->scan_tmp, g->kill.p, g->kill_prefix.p, N + 1, s);
+    mark_stage(g, 6, s);
     merge_params mp{};
     mp.meta = g->out.meta.p;
     mp.cap = g->out.cap.p;
@@ -1336,8 +1363,10 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     const unsigned merge_grid = static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>(N, 1), 4096));
     hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kBlock), 0, s, mp);
     DYNG_HIP_TRY(hipGetLastError());
+    mark_stage(g, 7, s);
     // ---- the in-rows ----
     sort_pairs(g->sort_tmp, g->ikeys_a.p, g->ikeys_b.p, g->ivals_a.p, g->ivals_b.p, N, 2 * vbits, s);
+    mark_stage(g, 8, s);
     in_pair_params ip{};
     ip.keys = g->ikeys_b.p;
     ip.vals = g->ivals_b.p;
@@ -1360,8 +1389,10 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     ip.ctl = dc;
     hipLaunchKernelGGL(k_in_pairs, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, ip);
     DYNG_HIP_TRY(hipGetLastError());
+    mark_stage(g, 9, s);
     exclusive_scan<int32_t>(g->scan_tmp, g->ipend_alive.p, g->ipend_prefix.p, N + 1, s);
     exclusive_scan<int32_t>(g->scan_tmp, g->ikill.p, g->ikill_prefix.p, N + 1, s);
+    mark_stage(g, 10, s);
     merge_params mi = mp;
     mi.meta = g->in.meta.p;
     mi.cap = g->in.cap.p;
@@ -1380,6 +1411,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     mi.garbage = &dc->garbage_in;
     hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kBlock), 0, s, mi);
     DYNG_HIP_TRY(hipGetLastError());
+    mark_stage(g, 11, s);
   }
   DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
   DYNG_HIP_TRY(hipStreamSynchronize(s));
@@ -1414,6 +1446,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     out->inserted_vertices = n_after - n_before;
     out->num_vertices_after = n_after;
     out->relocated_rows = static_cast<int64_t>(hc->relocated_rows);
+    stage_times(g, N > 0, out);
   }
   // the last edge of the largest weight left: recompute the column summaries (reuses the control
   // block, so only after the counters above were read)

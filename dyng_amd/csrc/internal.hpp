@@ -195,4 +195,16 @@ struct dyng_hip_graph {
   dyng_hip::dbuf<int32_t> b_ins_src, b_ins_dst, b_ins_w, b_del_src, b_del_dst;
   dyng_hip::dbuf<dyng_hip::apply_control> ctl;
   dyng_hip::pinned<dyng_hip::apply_control> host_ctl;
+  /// events at the apply's stage boundaries (0-1 batch scan, 2-11 the update of the rows)
+  hipEvent_t apply_ev[12] = {};
+  bool apply_ev_recorded = false;
+
+  dyng_hip_graph() = default;
+  dyng_hip_graph(const dyng_hip_graph&) = delete;
+  dyng_hip_graph& operator=(const dyng_hip_graph&) = delete;
+  ~dyng_hip_graph() {
+    for (hipEvent_t& e : apply_ev) {
+      if (e != nullptr) (void)hipEventDestroy(e);
+    }
+  }
 };

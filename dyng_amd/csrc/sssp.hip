@@ -1,22 +1,26 @@
-// The sssp engine of the HIP path: one persistent cooperative kernel per compute() / update()
-// (the fused engine of cpp/src/algorithms/sssp/fused.cuh:155-495, re-designed for CDNA4).
+// The sssp engine of the HIP path: one persistent kernel per compute() / update() (the fused
+// engine of cpp/src/algorithms/sssp/fused.cuh:155-495, re-designed for CDNA4).
 //
 // Phases, separated by grid barriers and decided on the device:
 //   A  pack the old tree into (distance << b | parent) words; roots = heads v of changed edges
 //      (u,v) with parent[v] == u (fused.cuh:178-197); roots start the invalidated list.
-//   B  subtree invalidation by pointer jumping (fused.cuh:199-234), each round over the
-//      compacted list of vertices still jumping (wave64 ballot appends) instead of all n; every
-//      newly flagged vertex joins the invalidated list, so phase C touches only those.
+//   B  the subtrees under the roots (fused.cuh:199-234 computes them by pointer jumping over all
+//      n vertices). A tree this engine produced is walked top-down instead, level by level along
+//      its tree edges (out-edges (u, w) with parent[w] == u), touching only the invalidated
+//      region; a tree handed in from outside keeps pointer jumping (over compacted lists of the
+//      vertices still jumping), which also detects parent cycles.
 //   C  invalidate (word INF, stamp), then the insertion heads (fused.cuh:236-256).
 //   D  pull: each candidate takes its best in-neighbour offer (fused.cuh:258-285).
 //   E  near-far propagation (fused.cuh:288-417): the relax iteration expands the near frontier
-//      with a packed 64-bit atomicMin per edge; short rows run one thread per vertex, longer rows
-//      are cut into chunks of edges that whole waves expand.
+//      with a packed 64-bit atomicMin per edge.
 //   F  unpack, writing only changed (distance, parent) pairs and counting them
 //      (fused.cuh:419-494); the distance-only mode recovers lowest-id parents.
 //
-// Per-iteration list counters live in rings of three slots, so an iteration needs one barrier
-// (two when it has chunked rows) instead of the reference's two.
+// Rows are expanded edge-balanced: a workgroup takes a tile of the list, scans the row lengths in
+// LDS and deals the tile's edge slots out one per thread, so a thread's critical path is one edge
+// rather than one row. Rows longer than `light` are cut into chunks that whole waves expand after
+// a barrier. Per-iteration list counters live in rings of three slots, so an iteration needs one
+// barrier (two when it has chunked rows).
 #include "internal.hpp"
 
 #include <hip/hip_runtime.h>
@@ -30,6 +34,7 @@ namespace dyng_hip {
 namespace {
 
 constexpr int kBlock = 1024;  ///< 16 waves per workgroup; 8 waves/SIMD fit two per CU, one is launched
+constexpr int kWaves = kBlock / 64;
 
 struct control {
   int ring_near[3];
@@ -105,6 +110,7 @@ struct engine_params {
   grid_sync_state* bar;
   int32_t light;
   int32_t chunk_edges;
+  unsigned long long barrier_ticks;  ///< longest wait at a grid barrier (s_memrealtime ticks)
 };
 
 /// Append v to a list when pred (wave-aggregated reservation).
@@ -128,11 +134,95 @@ DYNG_DEV void emit_chunks(const engine_params& p, int* counter, int32_t v, int32
   }
 }
 
+/// The workgroup's view of one tile of a vertex list: per item the first edge slot of its rows in
+/// the tile (exclusive prefix of the lengths), the row offset, the vertex and a 64-bit word.
+struct tile_shared {
+  int start[kBlock + 1];
+  int off[kBlock];
+  int vert[kBlock];
+  u64 word[kBlock];
+  int scan[kWaves + 1];
+};
+
+/// Exclusive prefix sum of x over the workgroup; `total` gets the sum. Every thread calls it.
+DYNG_DEV int block_scan(tile_shared& L, int x, int& total) {
+  int v = x;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(v, off);
+    if (lane() >= off) v += y;
+  }
+  const int w = static_cast<int>(threadIdx.x) >> 6;
+  if (lane() == 63) L.scan[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int i = 0; i < kWaves; ++i) {
+      const int t = L.scan[i];
+      L.scan[i] = run;
+      run += t;
+    }
+    L.scan[kWaves] = run;
+  }
+  __syncthreads();
+  total = L.scan[kWaves];
+  return L.scan[w] + v - x;
+}
+
+/// Edge-balanced expansion of list items [0, count): the items are cut into one tile per
+/// workgroup (up to kBlock items), `load(index, v, off, len, word)` fills a thread's item (len 0:
+/// nothing to expand in the tile, e.g. a long row sent to the chunk list), and the edge slots of
+/// the tile's rows are dealt out to the workgroup's threads, one slot per thread per step:
+/// `edge(act, item, slot)` runs with every lane of a wave (act false on padding lanes). `done(valid,
+/// item)` runs on every thread after the tile's edges. Uniform per workgroup: every thread calls it.
+template <class Load, class Edge, class Done>
+DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edge edge, Done done) {
+  const int G = static_cast<int>(gridDim.x);
+  const int ts = min(kBlock, max(1, (count + G - 1) / G));
+  const int i = static_cast<int>(threadIdx.x);
+  for (long long t0 = static_cast<long long>(blockIdx.x) * ts; t0 < count; t0 += static_cast<long long>(G) * ts) {
+    const int cnt = static_cast<int>(min(static_cast<long long>(ts), count - t0));
+    int32_t v = -1, off = 0, len = 0;
+    u64 word = packed_inf;
+    if (i < cnt) load(static_cast<int>(t0) + i, v, off, len, word);
+    int total = 0;
+    const int start = block_scan(L, len, total);
+    if (i < cnt) {
+      L.start[i] = start;
+      L.off[i] = off;
+      L.vert[i] = v;
+      L.word[i] = word;
+    }
+    if (i == 0) L.start[cnt] = total;
+    __syncthreads();
+    for (int base = i - lane(); base < total; base += kBlock) {
+      const int e = base + lane();
+      const bool act = e < total;
+      int item = 0;
+      if (act) {
+        int lo = 0, hi = cnt;  // the last item whose first slot is <= e (its row holds e)
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (L.start[mid] <= e) {
+            lo = mid;
+          } else {
+            hi = mid;
+          }
+        }
+        item = lo;
+      }
+      edge(act, item, act ? L.off[item] + (e - L.start[item]) : 0);
+    }
+    __syncthreads();
+    done(i < cnt, i);
+    __syncthreads();
+  }
+}
+
 /// One relaxation of edge (u, w) of weight wt with u at distance du (fused.cuh:341-362): the
 /// candidate word joins the next near list or the far pile when it lowered w's distance.
 DYNG_DEV void relax(const engine_params& p, bool active, int32_t u, u64 du, int32_t w, int32_t wt,
                     u64 threshold, int generation, int* next, int* next_count, int* far, int* far_count,
-                    unsigned long long& improved) {
+                    unsigned long long* improved) {
   bool to_near = false, to_far = false;
   if (active && w != p.source) {
     const u64 nd = du + static_cast<u64>(wt);
@@ -140,7 +230,7 @@ DYNG_DEV void relax(const engine_params& p, bool active, int32_t u, u64 du, int3
       const u64 cand = p.pk.pack(nd, u);
       if (cand < ld(&p.packed[w])) {
         const u64 old = atomicMin(&p.packed[w], cand);
-        improved += cand < old ? 1 : 0;
+        if (cand < old) atomicAdd(improved, 1ULL);
         if (nd < p.pk.distance(old)) {
           if (nd < threshold) {
             to_near = claim(p.stamp, w, generation);
@@ -155,56 +245,16 @@ DYNG_DEV void relax(const engine_params& p, bool active, int32_t u, u64 du, int3
   append(far, far_count, to_far, w);
 }
 
-/// The relax iteration over `count` near vertices: light rows inline, heavy rows as chunks.
-DYNG_DEV void expand_light(const engine_params& p, const int* near, int count, u64 threshold, int generation,
-                           int* next, int* next_count, int* far, int* far_count, int* chunk_count,
-                           unsigned long long& edges, unsigned long long& improved) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int threads = gridDim.x * blockDim.x;
-  for (int base = tid - lane(); base < count; base += threads) {
-    const int i = base + lane();
-    int32_t u = -1;
-    u64 word = packed_inf;
-    int2 mt = make_int2(0, 0);
-    if (i < count) {
-      u = ld(&near[i]);
-      word = ld(&p.packed[u]);
-      if (word != packed_inf) mt = p.out_meta[u];
-    }
-    const bool light = word != packed_inf && mt.y <= p.light;
-    if (word != packed_inf && !light) emit_chunks(p, chunk_count, u, mt.x, mt.y);
-    const u64 du = p.pk.distance(word);
-    const int len = light ? mt.y : 0;
-    edges += len;
-    // lanes walk their rows together; a lane past its row's end is idle
-    int most = len;
-    for (int off = 32; off > 0; off >>= 1) most = max(most, __shfl_xor(most, off));
-    for (int j = 0; j < most; ++j) {
-      const bool act = j < len;
-      const int32_t w = act ? p.out_nbr[mt.x + j] : 0;
-      const int32_t wt = act ? p.out_w[mt.x + j] : 0;
-      relax(p, act, u, du, w, wt, threshold, generation, next, next_count, far, far_count, improved);
-    }
-  }
-}
-
-DYNG_DEV void expand_chunks(const engine_params& p, int nchunks, u64 threshold, int generation, int* next,
-                            int* next_count, int* far, int* far_count, unsigned long long& edges,
-                            unsigned long long& improved) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
-  const int waves = (gridDim.x * blockDim.x) / 64;
+/// Chunks of long rows, one wave per chunk: `edge(act, u, slot)` with every lane of the wave.
+template <class Edge>
+DYNG_DEV void chunk_rows(const engine_params& p, int nchunks, Edge edge) {
+  const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / 64);
+  const int waves = static_cast<int>((gridDim.x * blockDim.x) / 64);
   for (int c = wave; c < nchunks; c += waves) {
     const int4 ch = p.chunks[c];
-    const int32_t u = ch.x;
-    const u64 word = ld(&p.packed[u]);
-    const u64 du = p.pk.distance(word);
-    if (lane() == 0) edges += ch.z - ch.y;
     for (int b = ch.y; b < ch.z; b += 64) {
       const int e = b + lane();
-      const bool act = word != packed_inf && e < ch.z;
-      const int32_t w = act ? p.out_nbr[e] : 0;
-      const int32_t wt = act ? p.out_w[e] : 0;
-      relax(p, act, u, du, w, wt, threshold, generation, next, next_count, far, far_count, improved);
+      edge(e < ch.z, ch.x, e);
     }
   }
 }
@@ -240,14 +290,31 @@ DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best
 
 }  // namespace
 
+/// Leave the kernel when the grid barrier gave up (see grid_barrier); uniform per workgroup.
+#define DYNG_GRID_SYNC()                                                            \
+  do {                                                                              \
+    if (!grid_barrier(p.bar, p.barrier_ticks)) {                                    \
+      if (threadIdx.x == 0) atomicOr(&c->error_bits, DYNG_HIP_DEVICE_ERROR_TIMEOUT); \
+      return;                                                                       \
+    }                                                                               \
+  } while (0)
+
 __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) {
+  __shared__ tile_shared L;
+  // work counters of the workgroup, in LDS: relax edges, improving atomics, pull edges, affected
+  __shared__ unsigned long long counts[4];
+  if (threadIdx.x < 4) counts[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long* const relax_edges = &counts[0];
+  unsigned long long* const relax_improved = &counts[1];
+  unsigned long long* const pull_edges = &counts[2];
+  unsigned long long* const affected = &counts[3];
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int threads = gridDim.x * blockDim.x;
   const int n = p.n;
   control* c = p.ctl;
   int generation = p.generation;
   int frontier_count = 0;
-  unsigned long long relax_edges = 0, relax_improved = 0, pull_edges = 0, affected = 0;
   int step = 0, cstep = 0;  // ring positions (uniform across the grid)
   unsigned long long jump_items = 0;
   if (tid == 0) c->t_phase[0] = __builtin_amdgcn_s_memrealtime();
@@ -256,9 +323,9 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     for (int v = tid; v < n; v += threads) p.packed[v] = v == p.source ? p.pk.pack(0, -1) : packed_inf;
     if (tid == 0) p.frontier[0] = p.source;
     frontier_count = 1;
-    grid_barrier(p.bar);
+    DYNG_GRID_SYNC();
   } else {
-    // ---- A: pack the old tree; ancestors; roots ----
+    // ---- A: pack the old tree; ancestors (pointer jumping only); roots ----
     for (int v = tid; v < n; v += threads) {
       const long long d = p.distances[v];
       const int32_t par = p.parents[v];
@@ -271,7 +338,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         }
       }
       p.packed[v] = word;
-      p.ancestor[v] = par;
+      if (p.check_cycles) p.ancestor[v] = par;
     }
     for (int64_t i = tid; i < p.num_del + p.num_ins; i += threads) {
       int32_t u, v;
@@ -290,13 +357,15 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       if (root) st(&p.stamp[v], generation + 1);
       append(p.cand, &c->cand_count, root, v);
     }
-    grid_barrier(p.bar);
+    DYNG_GRID_SYNC();
     if (tid == 0) c->t_phase[4] = __builtin_amdgcn_s_memrealtime();
     if (ld(&c->overflow) != 0) return;  // uniform: the host reports invalid_input
-    const int roots = ld(&c->cand_count);
-    const bool jump = (p.num_del + p.num_ins > 0) && (roots > 0 || p.check_cycles);
-    // ---- B: pointer jumping over compacted lists of vertices still jumping ----
-    if (jump) {
+    const int roots = uniform(ld(&c->cand_count));
+    const bool work = p.num_del + p.num_ins > 0;
+    int inv = roots;  // the invalidated vertices: cand[0, inv)
+    if (work && p.check_cycles) {
+      // ---- B (tree from outside): pointer jumping over compacted lists of vertices still
+      // jumping; a list that is not empty after ceil(log2 n) + 1 rounds is a parent cycle ----
       int count = n;
       const int* list = nullptr;
       int* outs[2] = {p.near_a, p.near_b};
@@ -330,8 +399,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           append(p.cand, &c->cand_count, flagged, v);
           append(out, out_count, keep, v);
         }
-        grid_barrier(p.bar);
-        count = ld(out_count);
+        DYNG_GRID_SYNC();
+        count = uniform(ld(out_count));
         list = out;
         if (count == 0) break;
       }
@@ -339,14 +408,70 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         if (tid == 0) atomicOr(&c->error_bits, DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE);
         return;  // uniform
       }
+      inv = uniform(ld(&c->cand_count));  // nothing appends to it after the last round's barrier
+    } else if (work && roots > 0) {
+      // ---- B (tree from this engine): the subtrees under the roots, level by level down the
+      // tree edges (parent[w] == u for an out-edge (u, w)); every tree edge of the old tree is
+      // an edge of the graph unless this batch deleted it, and then its head is a root. Level
+      // L + 1 is appended right after level L in the candidate list and counted in a ring slot
+      // of its own, so every thread reads the same level bounds ----
+      int lo = 0, hi = roots, levels = 0;
+      auto visit = [&](bool act, int32_t u, int slot, int* level, int* found) {
+        bool add = false;
+        int32_t w = -1;
+        if (act) {
+          w = p.out_nbr[slot];
+          if (p.parents[w] == u && ld(&p.flag[w]) == 0 && atomicExch(&p.flag[w], 1) == 0) {
+            st(&p.stamp[w], generation + 1);
+            add = true;
+          }
+        }
+        append(level, found, add, w);
+      };
+      while (hi > lo) {
+        jump_items += hi - lo;
+        int* found = &c->ring_jump[levels % 3];
+        int* chunk_count = &c->ring_chunk[cstep % 3];
+        if (tid == 0) {
+          st(&c->ring_jump[(levels + 1) % 3], 0);
+          st(&c->ring_chunk[(cstep + 1) % 3], 0);
+        }
+        ++levels;
+        const int level_base = lo;
+        int* const level = p.cand + hi;
+        tile_rows(
+            L, hi - lo,
+            [&](int idx, int32_t& v, int32_t& off, int32_t& len, u64&) {
+              v = ld(&p.cand[level_base + idx]);
+              const int2 mt = p.out_meta[v];
+              if (mt.y <= p.light) {
+                off = mt.x;
+                len = mt.y;
+              } else {
+                emit_chunks(p, chunk_count, v, mt.x, mt.y);
+              }
+            },
+            [&](bool act, int item, int slot) { visit(act, act ? L.vert[item] : -1, slot, level, found); },
+            [&](bool, int) {});
+        DYNG_GRID_SYNC();
+        const int nchunks = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap)));
+        ++cstep;
+        if (nchunks > 0) {
+          chunk_rows(p, nchunks, [&](bool act, int32_t u, int slot) { visit(act, u, slot, level, found); });
+          DYNG_GRID_SYNC();
+        }
+        lo = hi;
+        hi += uniform(ld(found));
+      }
+      if (tid == 0) st(&c->rounds, levels);
+      inv = hi;
     }
     // ---- C: invalidate the flagged vertices (stamped when flagged) and claim the insertion
     // heads that are not invalidated, in one phase ----
     if (tid == 0) c->t_phase[5] = __builtin_amdgcn_s_memrealtime();
     ++generation;
-    const int inv = ld(&c->cand_count);
     for (int i = tid; i < inv; i += threads) {
-      const int32_t v = p.cand[i];
+      const int32_t v = ld(&p.cand[i]);
       p.flag[v] = 0;
       p.packed[v] = packed_inf;
     }
@@ -362,10 +487,10 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       }
       append(p.cand + inv, &c->heads_count, add, v);
     }
-    grid_barrier(p.bar);
+    DYNG_GRID_SYNC();
     // ---- D: pull ----
     ++generation;
-    const int cands = inv + ld(&c->heads_count);
+    const int cands = inv + uniform(ld(&c->heads_count));
     if (tid == 0) {
       c->t_phase[1] = __builtin_amdgcn_s_memrealtime();
       c->jump_items = jump_items;
@@ -373,28 +498,32 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     }
     int* chunk_count = &c->ring_chunk[cstep % 3];
     if (tid == 0) st(&c->ring_chunk[(cstep + 1) % 3], 0);
-    for (int base = tid - lane(); base < cands; base += threads) {
-      const int i = base + lane();
-      int32_t v = -1;
-      int2 mt = make_int2(0, 0);
-      if (i < cands) {
-        v = p.cand[i];
-        mt = p.in_meta[v];
-      }
-      const bool light = i < cands && mt.y <= p.light;
-      if (i < cands && !light) emit_chunks(p, chunk_count, v, mt.x, mt.y);
-      u64 best = packed_inf;
-      if (light) {
-        pull_edges += mt.y;
-        for (int j = 0; j < mt.y; ++j) {
-          const u64 o = pull_offer(p, p.in_nbr[mt.x + j], p.in_w[mt.x + j]);
-          best = o < best ? o : best;
-        }
-      }
-      pull_commit(p, light, v, best, generation, &c->frontier_count, &c->min_frontier);
-    }
-    grid_barrier(p.bar);
-    const int nchunks = static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap));
+    tile_rows(
+        L, cands,
+        [&](int idx, int32_t& v, int32_t& off, int32_t& len, u64& word) {
+          v = ld(&p.cand[idx]);
+          const int2 mt = p.in_meta[v];
+          word = packed_inf;  // the best offer so far
+          if (mt.y <= p.light) {
+            off = mt.x;
+            len = mt.y;
+            atomicAdd(pull_edges, static_cast<unsigned long long>(mt.y));
+          } else {
+            emit_chunks(p, chunk_count, v, mt.x, mt.y);
+          }
+        },
+        [&](bool act, int item, int slot) {
+          if (act) {
+            const u64 o = pull_offer(p, p.in_nbr[slot], p.in_w[slot]);
+            if (o < L.word[item]) atomicMin(&L.word[item], o);
+          }
+        },
+        [&](bool valid, int item) {
+          pull_commit(p, valid, valid ? L.vert[item] : -1, valid ? L.word[item] : packed_inf, generation,
+                      &c->frontier_count, &c->min_frontier);
+        });
+    DYNG_GRID_SYNC();
+    const int nchunks = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap)));
     ++cstep;
     if (nchunks > 0) {
       const int wave = tid / 64;
@@ -406,7 +535,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           const u64 o = pull_offer(p, p.in_nbr[e], p.in_w[e]);
           best = o < best ? o : best;
         }
-        if (lane() == 0) pull_edges += ch.z - ch.y;
+        if (lane() == 0) atomicAdd(pull_edges, static_cast<unsigned long long>(ch.z - ch.y));
         best = wave_min(best);
         if (lane() == 0) {
           const u64 current = ld(&p.packed[ch.x]);
@@ -422,16 +551,16 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           }
         }
       }
-      grid_barrier(p.bar);
+      DYNG_GRID_SYNC();
     }
-    frontier_count = ld(&c->frontier_count);
+    frontier_count = uniform(ld(&c->frontier_count));
   }
 
   // ---- E: near-far propagation (the frontier's smallest distance came with the pull, or is the
   // source's 0) ----
   if (tid == 0) c->t_phase[2] = __builtin_amdgcn_s_memrealtime();
   u64 local = packed_inf;
-  const u64 smallest = p.from_scratch ? 0 : ld(&c->min_frontier);
+  const u64 smallest = p.from_scratch ? 0 : uniform(ld(&c->min_frontier));
   u64 threshold = (smallest == packed_inf ? 0 : smallest) + p.delta;
   int* current = p.near_a;
   int* next = p.near_b;
@@ -458,9 +587,9 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       append(current, near_count, to_near, v);
       append(far[fb], &c->far_count[fb], to_far, v);
     }
-    grid_barrier(p.bar);
+    DYNG_GRID_SYNC();
   }
-  int near_count = ld(&c->ring_near[step % 3]);
+  int near_count = uniform(ld(&c->ring_near[step % 3]));
   ++step;
   int iterations = 0, epochs = 0;
   long long pushes = 0;
@@ -473,7 +602,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       break;
     }
     if (near_count > 0) {
-      // -- one relax iteration --
+      // -- one relax iteration: rows up to `light` edges edge-balanced per workgroup, longer
+      // rows in chunks after a barrier --
       ++generation;
       ++iterations;
       pushes += near_count;
@@ -483,17 +613,57 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         st(&c->ring_near[(step + 1) % 3], 0);
         st(&c->ring_chunk[(cstep + 1) % 3], 0);
       }
-      expand_light(p, current, near_count, threshold, generation, next, next_count, far[fb], &c->far_count[fb],
-                   chunk_count, relax_edges, relax_improved);
-      grid_barrier(p.bar);
-      const int nchunks = static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap));
+      int* far_list = far[fb];
+      int* far_counter = &c->far_count[fb];
+      const int* cur = current;
+      tile_rows(
+          L, near_count,
+          [&](int idx, int32_t& v, int32_t& off, int32_t& len, u64& word) {
+            v = ld(&cur[idx]);
+            word = ld(&p.packed[v]);
+            if (word == packed_inf) return;
+            const int2 mt = p.out_meta[v];
+            if (mt.y <= p.light) {
+              off = mt.x;
+              len = mt.y;
+              atomicAdd(relax_edges, static_cast<unsigned long long>(mt.y));
+            } else {
+              emit_chunks(p, chunk_count, v, mt.x, mt.y);
+            }
+          },
+          [&](bool act, int item, int slot) {
+            const int32_t u = act ? L.vert[item] : 0;
+            const u64 du = act ? p.pk.distance(L.word[item]) : 0;
+            const int32_t w = act ? p.out_nbr[slot] : 0;
+            const int32_t wt = act ? p.out_w[slot] : 0;
+            relax(p, act, u, du, w, wt, threshold, generation, next, next_count, far_list, far_counter,
+                  relax_improved);
+          },
+          [&](bool, int) {});
+      DYNG_GRID_SYNC();
+      const int nchunks = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap)));
       ++cstep;
       if (nchunks > 0) {
-        expand_chunks(p, nchunks, threshold, generation, next, next_count, far[fb], &c->far_count[fb], relax_edges,
-                      relax_improved);
-        grid_barrier(p.bar);
+        const int wave = tid / 64;
+        const int waves = threads / 64;
+        for (int q = wave; q < nchunks; q += waves) {
+          const int4 ch = p.chunks[q];
+          const int32_t u = ch.x;
+          const u64 word = ld(&p.packed[u]);
+          const u64 du = p.pk.distance(word);
+          if (lane() == 0) atomicAdd(relax_edges, static_cast<unsigned long long>(ch.z - ch.y));
+          for (int b = ch.y; b < ch.z; b += 64) {
+            const int e = b + lane();
+            const bool act = word != packed_inf && e < ch.z;
+            const int32_t w = act ? p.out_nbr[e] : 0;
+            const int32_t wt = act ? p.out_w[e] : 0;
+            relax(p, act, u, du, w, wt, threshold, generation, next, next_count, far_list, far_counter,
+                  relax_improved);
+          }
+        }
+        DYNG_GRID_SYNC();
       }
-      near_count = ld(next_count);
+      near_count = uniform(ld(next_count));
       ++step;
       int* t = current;
       current = next;
@@ -501,7 +671,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       continue;
     }
     // -- near frontier empty: raise the threshold past the far pile --
-    const int far_count = ld(&c->far_count[fb]);
+    const int far_count = uniform(ld(&c->far_count[fb]));
     if (far_count == 0) break;
     ++epochs;
     const int ms = epochs & 1;
@@ -513,8 +683,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     }
     local = wave_min(local);
     if (lane() == 0 && local != packed_inf) atomicMin(&c->minimum[ms], local);
-    grid_barrier(p.bar);
-    const u64 mn = ld(&c->minimum[ms]);
+    DYNG_GRID_SYNC();
+    const u64 mn = uniform(ld(&c->minimum[ms]));
     threshold = (threshold > mn ? threshold : mn) + p.delta;
     ++generation;
     int* split_count = &c->ring_near[step % 3];
@@ -537,8 +707,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       append(current, split_count, to_near, v);
       append(far[fb ^ 1], &c->far_count[fb ^ 1], keep, v);
     }
-    grid_barrier(p.bar);
-    near_count = ld(split_count);
+    DYNG_GRID_SYNC();
+    near_count = uniform(ld(split_count));
     ++step;
     if (tid == 0) st(&c->far_count[fb], 0);
     fb ^= 1;
@@ -546,6 +716,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
 
   // ---- F: unpack ----
   if (tid == 0) c->t_phase[3] = __builtin_amdgcn_s_memrealtime();
+  unsigned long long changed = 0;
   if (p.pk.bits > 0) {
     for (int v = tid; v < n; v += threads) {
       const u64 word = p.packed[v];
@@ -557,7 +728,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       } else if (p.distances[v] != d || p.parents[v] != par) {
         p.distances[v] = d;
         p.parents[v] = par;
-        ++affected;
+        ++changed;
       }
     }
   } else {
@@ -572,7 +743,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       p.distances[v] = d;
       p.parents[v] = (word == packed_inf || v == p.source) ? -1 : INT32_MAX;
     }
-    grid_barrier(p.bar);
+    DYNG_GRID_SYNC();
     for (int u = tid; u < n; u += threads) {
       const u64 du = p.packed[u];
       if (du == packed_inf) continue;
@@ -583,22 +754,21 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       }
     }
     if (p.count_changes) {
-      grid_barrier(p.bar);
+      DYNG_GRID_SYNC();
       for (int v = tid; v < n; v += threads) {
-        affected += (p.flag[v] != 0 || p.ancestor[v] != ld(&p.parents[v])) ? 1 : 0;
+        changed += (p.flag[v] != 0 || p.ancestor[v] != ld(&p.parents[v])) ? 1 : 0;
         p.flag[v] = 0;
       }
     }
   }
-  affected = wave_sum(affected);
-  relax_edges = wave_sum(relax_edges);
-  relax_improved = wave_sum(relax_improved);
-  pull_edges = wave_sum(pull_edges);
-  if (lane() == 0) {
-    if (affected) atomicAdd(&c->affected, affected);
-    if (relax_edges) atomicAdd(&c->relax_edges, relax_edges);
-    if (relax_improved) atomicAdd(&c->relax_improved, relax_improved);
-    if (pull_edges) atomicAdd(&c->pull_edges, pull_edges);
+  changed = wave_sum(changed);
+  if (lane() == 0 && changed) atomicAdd(affected, changed);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (*affected) atomicAdd(&c->affected, *affected);
+    if (*relax_edges) atomicAdd(&c->relax_edges, *relax_edges);
+    if (*relax_improved) atomicAdd(&c->relax_improved, *relax_improved);
+    if (*pull_edges) atomicAdd(&c->pull_edges, *pull_edges);
   }
   if (tid == 0) {
     c->t_phase[7] = __builtin_amdgcn_s_memrealtime();
@@ -608,6 +778,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     c->generation = generation;
   }
 }
+
+#undef DYNG_GRID_SYNC
 
 }  // namespace dyng_hip
 
@@ -674,11 +846,6 @@ void ws_reserve(dyng_hip_workspace* ws, int64_t n, int64_t slots, hipStream_t s)
     DYNG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ws->occupancy,
                                                               reinterpret_cast<const void*>(sssp_fused_kernel),
                                                               kBlock, 0));
-    int coop = 0;
-    DYNG_HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ws->device));
-    if (!coop) {
-      fail(DYNG_HIP_NOT_SUPPORTED, "dyng: sssp: the fused engine needs cooperative launch, which this device lacks");
-    }
     if (ws->occupancy < 1) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: sssp: the persistent kernel does not fit on a CU");
   }
 }
@@ -783,11 +950,17 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   p.chunk_cap = ws->chunk_cap;
   p.ctl = ws->ctl.p;
   p.bar = ws->bar.p;
-  p.light = run->tuning.light_degree > 0 ? run->tuning.light_degree : 32;
+  p.light = run->tuning.light_degree > 0 ? run->tuning.light_degree : 4096;
   p.chunk_edges = run->tuning.chunk_edges > 0 ? std::max(16, run->tuning.chunk_edges) : 256;
+  // the real-time counter runs at 100 MHz: 100 ticks per microsecond
+  const int64_t timeout_us = run->tuning.barrier_timeout_us > 0   ? run->tuning.barrier_timeout_us
+                             : run->tuning.barrier_timeout_us == 0 ? 4000000
+                                                                   : 0;
+  p.barrier_ticks = static_cast<unsigned long long>(timeout_us) * 100ULL;
   // Every workgroup must be resident for the grid barrier. The launch stays at half of the
   // occupancy the runtime reports (one 1024-thread workgroup per CU by default), never at the
-  // edge of it.
+  // edge of it; the barrier's bounded wait turns a grid that is not resident after all into an
+  // error.
   const int resident = ws->occupancy * ws->cu_count;  // what the runtime says fits at once
   const int cap = std::max(1, resident / 2);
   const int per_cu = run->tuning.blocks_per_cu > 0 ? run->tuning.blocks_per_cu : 1;
@@ -799,24 +972,38 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   hc->minimum[1] = packed_inf;
   DYNG_HIP_TRY(hipMemcpyAsync(ws->ctl.p, hc, sizeof(control), hipMemcpyHostToDevice, s));
   DYNG_HIP_TRY(hipMemsetAsync(ws->bar.p, 0, sizeof(grid_sync_state), s));
-  void* args[] = {&p};
   if (ws->ev_start == nullptr) {
     DYNG_HIP_TRY(hipEventCreate(&ws->ev_start));
     DYNG_HIP_TRY(hipEventCreate(&ws->ev_stop));
   }
   DYNG_HIP_TRY(hipEventRecord(ws->ev_start, s));
-  DYNG_HIP_TRY(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(sssp_fused_kernel), dim3(blocks), dim3(kBlock),
-                                          args, 0, s));
+  hipLaunchKernelGGL(sssp_fused_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
+  DYNG_HIP_TRY(hipGetLastError());
   DYNG_HIP_TRY(hipEventRecord(ws->ev_stop, s));
   DYNG_HIP_TRY(hipMemcpyAsync(hc, ws->ctl.p, sizeof(control), hipMemcpyDeviceToHost, s));
   DYNG_HIP_TRY(hipStreamSynchronize(s));
   float kernel_ms = 0.0f;
   DYNG_HIP_TRY(hipEventElapsedTime(&kernel_ms, ws->ev_start, ws->ev_stop));
+  if (hc->overflow != 0 || hc->error_bits != 0) {
+    // a run that stopped early can leave flags, far marks and stamps of any later generation
+    // behind in the shared workspace: clear them before any other run uses it
+    DYNG_HIP_TRY(hipMemsetAsync(ws->stamp.p, 0, ws->capacity * sizeof(int), s));
+    DYNG_HIP_TRY(hipMemsetAsync(ws->in_far.p, 0, ws->capacity * sizeof(int), s));
+    DYNG_HIP_TRY(hipMemsetAsync(ws->flag.p, 0, ws->capacity * sizeof(int), s));
+    DYNG_HIP_TRY(hipStreamSynchronize(s));
+    ws->generation = 0;
+  }
   if (hc->overflow != 0) {
     throw status_error(DYNG_HIP_DEVICE_ERROR,
                        "sssp: an input distance is negative or larger than (n - 1) * max weight; the initial tree "
                        "does not belong to this graph",
                        DYNG_HIP_DEVICE_ERROR_INVALID_INPUT);
+  }
+  if (hc->error_bits & DYNG_HIP_DEVICE_ERROR_TIMEOUT) {
+    throw status_error(DYNG_HIP_DEVICE_ERROR,
+                       "sssp: a grid barrier wait of the persistent kernel ran past " + std::to_string(timeout_us) +
+                           " us (its workgroups were not all resident: is other work filling the device?)",
+                       hc->error_bits);
   }
   if (hc->error_bits & DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE) {
     throw status_error(DYNG_HIP_DEVICE_ERROR,

@@ -75,10 +75,12 @@ class ApplySummary:
     ignored_insertions: int = 0
     relocated_rows: int = 0
     compactions: int = 0
+    stage_ms: dict = field(default_factory=dict, compare=False)  # device time per apply stage
 
     @classmethod
     def _from_native(cls, s: _lib.ApplySummary) -> "ApplySummary":
-        return cls(**{f: int(getattr(s, f)) for f in _lib.SUMMARY_FIELDS})
+        stages = {name: float(s.stage_ms[i]) for i, name in enumerate(_lib.APPLY_STAGES)}
+        return cls(**{f: int(getattr(s, f)) for f in _lib.SUMMARY_FIELDS}, stage_ms=stages)
 
 
 def _ids(x, what: str) -> np.ndarray:

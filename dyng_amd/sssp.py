@@ -58,10 +58,11 @@ class Tuning:
     blocks_per_cu: int = 0
     light_degree: int = 0
     chunk_edges: int = 0
+    barrier_timeout_us: int = 0
 
     def _native(self) -> _lib.Tuning:
         return _lib.Tuning(_lib.ENGINES[self.engine], 0, int(self.blocks_per_cu), int(self.light_degree),
-                           int(self.chunk_edges))
+                           int(self.chunk_edges), int(self.barrier_timeout_us))
 
 
 @dataclass(frozen=True)

@@ -44,6 +44,7 @@ typedef enum dyng_hip_status {
 #define DYNG_HIP_DEVICE_ERROR_CAPACITY 1u
 #define DYNG_HIP_DEVICE_ERROR_INVALID_INPUT 2u
 #define DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE 4u
+#define DYNG_HIP_DEVICE_ERROR_TIMEOUT 8u /* a grid barrier wait ran past tuning.barrier_timeout_us */
 #define DYNG_HIP_DEVICE_ERROR_INTERNAL 0x80000000u
 
 /* Engines (cpp/include/dyng/core/types.hpp, dyng::engine). */
@@ -80,6 +81,9 @@ typedef struct dyng_hip_graph dyng_hip_graph;
 typedef struct dyng_hip_apply_summary {
   int64_t inserted_edges, updated_edges, deleted_edges, ignored_deletions, dropped_self_loops;
   int64_t inserted_vertices, num_vertices_after, ignored_insertions, relocated_rows, compactions;
+  /* device time of the apply's stages (HIP events, ms): batch scan, operation keys, out-key sort,
+   * out-row pairing, out scans, out-row merge, in-key sort, in-row pairing, in scans, in-row merge */
+  double stage_ms[10];
 } dyng_hip_apply_summary;
 
 /* A batch (cpp/include/dyng/graph/edge_batch.hpp:38-76): insertions (u, v, K weights,
@@ -157,8 +161,11 @@ typedef struct dyng_hip_tuning {
   int32_t engine;          /* DYNG_HIP_ENGINE_* */
   int32_t block_size;      /* threads per workgroup of the persistent kernel (256, 512, 1024) */
   int32_t blocks_per_cu;   /* resident workgroups per CU (<= occupancy) */
-  int32_t light_degree;    /* rows up to this length: one thread per vertex */
-  int32_t chunk_edges;     /* longer rows: split in chunks of this many edges, one wave each */
+  int32_t light_degree;    /* rows up to this length: edge-balanced inside a workgroup (0: 4096) */
+  int32_t chunk_edges;     /* longer rows: split in chunks of this many edges, one wave each (0: 256) */
+  int32_t barrier_timeout_us; /* longest wait at a grid barrier before the run fails with
+                                 DYNG_HIP_DEVICE_ERROR_TIMEOUT (0: 4 s; < 0: no wait at all, so
+                                 the run fails at its first barrier: exercises the error path) */
 } dyng_hip_tuning;
 
 /* One engine run (sssp_run, problem.hpp:380-400), all arrays in device memory. */
@@ -170,7 +177,10 @@ typedef struct dyng_hip_sssp_run {
   int64_t max_weight; /* largest weight of the objective before or after the batch (>= 1) */
   int64_t* distances; /* n; in: old tree (update); out: new tree */
   int32_t* parents;   /* n */
-  int32_t check_cycles; /* update: run the parent-cycle check even when no tree edge changed */
+  int32_t check_cycles; /* update: 1 for a tree that did not come from this engine: its subtrees are
+                           found by pointer jumping, which also detects a parent cycle (run even when
+                           no tree edge changed); 0: the engine's own tree, walked top-down along its
+                           tree edges */
   dyng_hip_workspace* workspace;
   dyng_hip_tuning tuning;
 } dyng_hip_sssp_run;

@@ -363,3 +363,99 @@ def test_device_resident_batch(dg):
     d, p = hr.to_numpy()
     assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent)
     assert st.invalidated == c.invalidated and st.affected == c.affected
+
+
+# ------------------------------------------------------------------------------------------------
+# Subtree invalidation: the engine's own tree (top-down walk) and an imported one (pointer jumping)
+# ------------------------------------------------------------------------------------------------
+
+def _stream(dg, og, hg, ot, hr, rng, batches, n_ops, wlo, whi, extra_dels=()):
+    rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+    for b in range(batches):
+        pick = rng.integers(0, og.m, n_ops)
+        ins = (rng.integers(0, og.n, n_ops), rng.integers(0, og.n, n_ops), rng.integers(wlo, whi, n_ops))
+        ds = np.concatenate([rows[pick], np.asarray([e[0] for e in extra_dels], np.int64)])
+        dd = np.concatenate([og.col[pick], np.asarray([e[1] for e in extra_dels], np.int64)])
+        og, a, c = oracle.update(og, ot, ins[0], ins[1], ins[2].reshape(-1, 1), ds, dd)
+        rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+        st = dg.sssp.update(hg, dg.EdgeBatch(insert=ins, delete=(ds, dd)), hr)
+        d, p = hr.to_numpy()
+        assert np.array_equal(d, ot.dist), f"distances, batch {b}"
+        assert np.array_equal(p, ot.parent), f"parents, batch {b}"
+        assert st.invalidated == c.invalidated and st.affected == c.affected, f"counters, batch {b}"
+        extra_dels = ()
+    return og
+
+
+def test_deep_subtrees_are_walked_level_by_level(dg):
+    """A chain of unit edges under long shortcuts: deleting chain edges near the source
+    invalidates subtrees thousands of levels deep."""
+    _say("[deep] chain n=6000")
+    rng = np.random.default_rng(5)
+    n = 6000
+    src = np.concatenate([np.arange(n - 1), rng.integers(0, n, 12000)])
+    dst = np.concatenate([np.arange(1, n), rng.integers(0, n, 12000)])
+    w = np.concatenate([np.ones(n - 1, np.int64), rng.integers(1500, 4000, 12000)])
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    ot, _ = oracle.compute(og, 0)
+    hr = dg.sssp.compute(hg, 0)
+    _stream(dg, og, hg, ot, hr, rng, 3, 40, 1500, 4000, extra_dels=[(10, 11), (700, 701), (2500, 2501)])
+
+
+def test_hub_subtree_with_chunked_rows(dg):
+    """The invalidated subtree holds a hub whose row is cut into chunks (light rows of 8 edges)."""
+    _say("[hub] n=4000")
+    rng = np.random.default_rng(6)
+    n = 4000
+    src = np.concatenate([[0], np.full(3000, 1), rng.integers(0, n, 20000)])
+    dst = np.concatenate([[1], np.arange(2, 3002), rng.integers(0, n, 20000)])
+    w = np.concatenate([[1], np.ones(3000, np.int64), rng.integers(20, 60, 20000)])
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    ot, _ = oracle.compute(og, 0)
+    hr = dg.sssp.compute(hg, 0)
+    hr.set_tuning(dg.sssp.Tuning(light_degree=8, chunk_edges=64))
+    _stream(dg, og, hg, ot, hr, rng, 2, 50, 20, 60, extra_dels=[(0, 1)])
+
+
+def test_imported_tree_updates_like_the_engine_tree(dg):
+    """A validated imported tree is updated by pointer jumping, the next batch by the top-down
+    walk: both equal the oracle."""
+    _say("[imported] n=3000")
+    rng = np.random.default_rng(8)
+    n, m = 3000, 30000
+    src, dst, w = rng.integers(0, n, m), rng.integers(0, n, m), rng.integers(1, 40, m)
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    ot, _ = oracle.compute(og, 7)
+    hr = dg.sssp.Result.from_arrays(hg, 7, ot.dist, ot.parent)
+    _stream(dg, og, hg, ot, hr, rng, 3, 200, 1, 40)
+
+
+def test_barrier_timeout_fails_cleanly(dg):
+    """A grid barrier whose wait limit is zero fails the run with a device error (every workgroup
+    leaves the kernel); the result is poisoned and the device stays usable."""
+    _say("[timeout] n=5000")
+    rng = np.random.default_rng(9)
+    n, m = 5000, 50000
+    src, dst, w = rng.integers(0, n, m), rng.integers(0, n, m), rng.integers(1, 40, m)
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    ot, _ = oracle.compute(og, 0)
+    hr = dg.sssp.compute(hg, 0)
+    hr.set_tuning(dg.sssp.Tuning(barrier_timeout_us=-1))
+    rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+    pick = rng.integers(0, og.m, 300)
+    ins = (rng.integers(0, n, 300), rng.integers(0, n, 300), rng.integers(1, 40, 300))
+    dels = (rows[pick], og.col[pick].copy())
+    og, _, _ = oracle.update(og, ot, ins[0], ins[1], ins[2].reshape(-1, 1), dels[0], dels[1])
+    with pytest.raises(dg.DeviceError, match="grid barrier"):
+        dg.sssp.update(hg, dg.EdgeBatch(insert=ins, delete=dels), hr)
+    with pytest.raises(dg.StaleResultError):
+        hr.to_numpy()
+    # the next run on the shared workspace starts clean
+    fresh = dg.sssp.compute(hg, 0)
+    d, p = fresh.to_numpy()
+    assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent)
+    _stream(dg, og, hg, ot, fresh, rng, 2, 200, 1, 40)
