@@ -362,6 +362,9 @@ __global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* i
   }
 }
 
+/// Rows up to this many entries are merged by one wave, longer ones by a workgroup.
+constexpr int kMergeSmall = 512;
+
 struct out_pair_params {
   op_layout L;
   const u64* keys;
@@ -541,8 +544,13 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
         }
       }
       if (structural && atomicExch(&p.mark[u], p.mark_gen) != p.mark_gen) {
-        const int t = atomicAdd(&p.ctl->touched_out, 1);
-        p.touched[t] = u;
+        if (mt.y > kMergeSmall) {
+          const int t = atomicAdd(&p.ctl->touched_out_big, 1);
+          p.touched[p.N - t] = u;  // long rows fill the list from its end
+        } else {
+          const int t = atomicAdd(&p.ctl->touched_out, 1);
+          p.touched[t] = u;
+        }
       }
     }
   }
@@ -663,8 +671,13 @@ __global__ void __launch_bounds__(kBlock) k_in_pairs(in_pair_params p) {
     }
   }
   if (structural && atomicExch(&p.mark[v], p.mark_gen) != p.mark_gen) {
-    const int t = atomicAdd(&p.ctl->touched_in, 1);
-    p.touched[t] = v;
+    if (mt.y > kMergeSmall) {
+      const int t = atomicAdd(&p.ctl->touched_in_big, 1);
+      p.touched[p.N - t] = v;  // long rows fill the list from its end
+    } else {
+      const int t = atomicAdd(&p.ctl->touched_in, 1);
+      p.touched[t] = v;
+    }
   }
 }
 
@@ -683,7 +696,8 @@ struct merge_params {
   const int32_t* kill_prefix;
   const int32_t* pend_w;
   const int32_t* touched;
-  const int* touched_count;
+  const int* touched_count;     ///< rows of at most kMergeSmall entries: touched[0, count)
+  const int* touched_big;       ///< longer rows: touched[N - t] for t in [0, count)
   int32_t* stage_nbr;
   int32_t* stage_w;
   int64_t stage_cap;
@@ -695,21 +709,139 @@ struct merge_params {
   double headroom;
 };
 
+constexpr int kMergeBlock = 1024;  ///< threads of a merge workgroup
+constexpr int kMergeSeg = 2048;    ///< operations of one row staged in LDS for the merge
+constexpr int kMergeE = 8;       ///< row entries per thread per merge step
+constexpr int kMergeWaves = kMergeBlock / 64;
+static_assert(kMergeE * kMergeWaves == 128, "the step prefix takes two counts per lane of one wave");
+
+/// First index in [0, len) of an LDS array whose value is >= v.
+DYNG_DEV int lds_lower(const int32_t* a, int len, int32_t v) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+/// One wave per touched row of at most kMergeSmall entries (most rows): the merge of
+/// k_merge_rows below, wave-synchronous, with the row's few operations searched in place.
+__global__ void __launch_bounds__(kMergeBlock) k_merge_small(merge_params p) {
+  const int count = *p.touched_count;
+  const u64 mask = (1ULL << p.vbits) - 1;
+  const int wave = static_cast<int>((blockIdx.x * int64_t{blockDim.x} + threadIdx.x) / 64);
+  const int waves = static_cast<int>((int64_t{gridDim.x} * blockDim.x) / 64);
+  for (int t = wave; t < count; t += waves) {
+    const int32_t u = p.touched[t];
+    const int2 mt = p.meta[u];
+    long long a = 0, b = 0, dest = 0;
+    int staged = 0, relocated = 0, newcap = 0;
+    if (lane() == 0) {
+      a = lower_key(p.keys, 0, p.N, static_cast<u64>(u) << p.vbits);
+      b = lower_key(p.keys, a, p.N, static_cast<u64>(u + 1) << p.vbits);
+      const int P = p.pend_prefix[b] - p.pend_prefix[a];
+      const int Kc = p.kill_prefix[b] - p.kill_prefix[a];
+      const int new_len = mt.y - Kc + P;
+      const int c = p.cap[u];
+      newcap = c;
+      bool done = false;
+      if (new_len <= c) {
+        const long long s = atomicAdd(reinterpret_cast<unsigned long long*>(p.stage_used), static_cast<unsigned long long>(new_len));
+        if (s + new_len <= p.stage_cap) {
+          staged = 1;
+          dest = s;
+          done = true;
+        }
+      }
+      if (!done) {
+        const int nc = new_len + static_cast<int>(ceil(new_len * p.headroom)) + 2;
+        newcap = nc;
+        dest = static_cast<long long>(atomicAdd(reinterpret_cast<unsigned long long*>(p.tail), static_cast<unsigned long long>(nc)));
+        relocated = 1;
+        if (dest + nc > p.slot_cap) {
+          atomicOr(p.error_bits, static_cast<int>(DYNG_HIP_DEVICE_ERROR_CAPACITY));
+          dest = -1;
+        } else {
+          atomicAdd(reinterpret_cast<unsigned long long*>(p.garbage), static_cast<unsigned long long>(c));
+          atomicAdd(p.relocated, 1ULL);
+        }
+      }
+    }
+    a = __shfl(a, 0);
+    b = __shfl(b, 0);
+    dest = __shfl(dest, 0);
+    staged = __shfl(staged, 0);
+    if (dest < 0) continue;  // uniform over the wave
+    int32_t* tn = staged ? p.stage_nbr : p.nbr;
+    int32_t* tw = staged ? p.stage_w : p.w;
+    const int64_t tstride = staged ? p.stage_cap : p.slot_cap;
+    const int pa = p.pend_prefix[a];
+    const int ka = p.kill_prefix[a];
+    int carry = 0;
+    for (int base = 0; base < mt.y; base += 64) {
+      const int jx = base + lane();
+      const int32_t x = jx < mt.y ? p.nbr[mt.x + jx] : -1;
+      const bool alive = x >= 0;
+      const u64 ballot = __ballot(alive);
+      if (alive) {
+        const int64_t lt = lower_key(p.keys, a, b, (static_cast<u64>(u) << p.vbits) | static_cast<u64>(x));
+        const int64_t pos = dest + carry + __popcll(ballot & lanes_below()) + (p.pend_prefix[lt] - pa);
+        tn[pos] = x;
+        for (int k = 0; k < p.K; ++k) tw[k * tstride + pos] = p.w[k * p.slot_cap + mt.x + jx];
+      }
+      carry += __popcll(ballot);
+    }
+    for (int64_t q = a + lane(); q < b; q += 64) {
+      if (!p.pend_alive[q]) continue;
+      const u64 key = p.keys[q];
+      const int32_t v = static_cast<int32_t>(key & mask);
+      const int y = upper_nbr(p.nbr + mt.x, mt.y, v);
+      const int64_t se = upper_key(p.keys, q, b, key);
+      const int killed = p.kill_prefix[se] - ka;
+      const int64_t pos = dest + (y - killed) + (p.pend_prefix[q] - pa);
+      tn[pos] = v;
+      for (int k = 0; k < p.K; ++k) tw[k * tstride + pos] = p.pend_w[k * p.N + q];
+    }
+    const int new_len = carry + (p.pend_prefix[b] - pa);
+    if (staged) {
+      __threadfence_block();  // the wave's staging stores before its reads of them
+      for (int x = lane(); x < new_len; x += 64) {
+        p.nbr[mt.x + x] = p.stage_nbr[dest + x];
+        for (int k = 0; k < p.K; ++k) p.w[k * p.slot_cap + mt.x + x] = p.stage_w[k * p.stage_cap + dest + x];
+      }
+    }
+    if (lane() == 0) {
+      p.meta[u] = make_int2(staged ? mt.x : static_cast<int32_t>(dest), new_len);
+      if (relocated) p.cap[u] = newcap;
+    }
+  }
+}
+
 /// One workgroup per touched row: the alive entries and the pending ones, merged by neighbour id
 /// (existing entries before new ones of the same neighbour), written in place through the
 /// staging arena, or to a new slot range at the tail when the row outgrew its capacity.
-__global__ void __launch_bounds__(kBlock) k_merge_rows(merge_params p) {
-  __shared__ int s_wave_tot[kBlock / 64];
+///
+/// The row's operations (a sorted key segment) are staged in LDS with the number of pending
+/// entries before each, so an alive entry finds its place with a binary search in LDS. The row
+/// is read kMergeE slabs of 1024 entries per step, the loads of a step issued together; the
+/// alive entries before each slab and wave come from one LDS scan per step.
+__global__ void __launch_bounds__(kMergeBlock) k_merge_rows(merge_params p) {
+  __shared__ int s_tot[kMergeE * kMergeWaves];  // alive entries per (slab, wave), then their prefix
+  __shared__ int s_step_total;
+  __shared__ int32_t s_v[kMergeSeg];       // neighbour of each operation of the row
+  __shared__ int32_t s_pp[kMergeSeg + 1];  // pending entries before each operation
   __shared__ long long s_dest;
   __shared__ int s_staged, s_newcap, s_relocated;
   __shared__ int64_t s_a, s_b;
-  const int count = *p.touched_count;
+  const int count = *p.touched_big;
   const u64 mask = (1ULL << p.vbits) - 1;
-  const int wid = threadIdx.x / 64;
+  const int tid = static_cast<int>(threadIdx.x);
+  const int wid = tid / 64;
   for (int t = blockIdx.x; t < count; t += gridDim.x) {
-    const int32_t u = p.touched[t];
+    const int32_t u = p.touched[p.N - t];
     const int2 mt = p.meta[u];
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       const int64_t a = lower_key(p.keys, 0, p.N, static_cast<u64>(u) << p.vbits);
       const int64_t b = lower_key(p.keys, a, p.N, static_cast<u64>(u + 1) << p.vbits);
       const int P = p.pend_prefix[b] - p.pend_prefix[a];
@@ -758,32 +890,75 @@ __global__ void __launch_bounds__(kBlock) k_merge_rows(merge_params p) {
     const int64_t tstride = staged ? p.stage_cap : p.slot_cap;
     const int pa = p.pend_prefix[a];
     const int ka = p.kill_prefix[a];
+    const int nops = static_cast<int>(b - a);
+    const bool in_lds = nops <= kMergeSeg;
+    if (in_lds) {
+      for (int j = tid; j < nops; j += kMergeBlock) {
+        s_v[j] = static_cast<int32_t>(p.keys[a + j] & mask);
+        s_pp[j] = p.pend_prefix[a + j] - pa;
+      }
+      if (tid == 0) s_pp[nops] = p.pend_prefix[b] - pa;
+    }
     // alive old entries
     int carry = 0;
-    for (int base = 0; base < mt.y; base += kBlock) {
-      const int jx = base + static_cast<int>(threadIdx.x);
-      const int32_t x = jx < mt.y ? p.nbr[mt.x + jx] : -1;
-      const bool alive = x >= 0;
-      const u64 ballot = __ballot(alive);
-      if (lane() == 0) s_wave_tot[wid] = __popcll(ballot);
+    for (int base = 0; base < mt.y; base += kMergeBlock * kMergeE) {
+      int32_t x[kMergeE];
+      int32_t w0[kMergeE];
+#pragma unroll
+      for (int e = 0; e < kMergeE; ++e) {
+        const int jx = base + e * kMergeBlock + tid;
+        x[e] = jx < mt.y ? p.nbr[mt.x + jx] : -1;
+        w0[e] = jx < mt.y ? p.w[mt.x + jx] : 0;
+      }
+#pragma unroll
+      for (int e = 0; e < kMergeE; ++e) {
+        const u64 ballot = __ballot(x[e] >= 0);
+        if (lane() == 0) s_tot[e * kMergeWaves + wid] = __popcll(ballot);
+      }
       __syncthreads();
-      int before = 0, total = 0;
-      for (int q = 0; q < kBlock / 64; ++q) {
-        before += q < wid ? s_wave_tot[q] : 0;
-        total += s_wave_tot[q];
+      if (wid == 0) {
+        // exclusive prefix of the kMergeE * 16 counts in row order (two per lane)
+        const int l = lane();
+        const int c0 = s_tot[l], c1 = s_tot[l + 64];
+        int i0 = c0, i1 = c1;
+        for (int off = 1; off < 64; off <<= 1) {
+          const int y0 = __shfl_up(i0, off), y1 = __shfl_up(i1, off);
+          if (l >= off) {
+            i0 += y0;
+            i1 += y1;
+          }
+        }
+        const int half = __shfl(i0, 63);
+        s_tot[l] = i0 - c0;
+        s_tot[l + 64] = half + i1 - c1;
+        if (l == 63) s_step_total = half + i1;
       }
-      if (alive) {
-        const int pre = before + __popcll(ballot & lanes_below());
-        const int64_t lt = lower_key(p.keys, a, b, (static_cast<u64>(u) << p.vbits) | static_cast<u64>(x));
-        const int64_t pos = dest + carry + pre + (p.pend_prefix[lt] - pa);
-        tn[pos] = x;
-        for (int k = 0; k < p.K; ++k) tw[k * tstride + pos] = p.w[k * p.slot_cap + mt.x + jx];
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < kMergeE; ++e) {
+        const bool alive = x[e] >= 0;
+        const u64 ballot = __ballot(alive);
+        if (alive) {
+          const int jx = base + e * kMergeBlock + tid;
+          const int pre = carry + s_tot[e * kMergeWaves + wid] + __popcll(ballot & lanes_below());
+          int before;  // pending entries of the row with a smaller neighbour
+          if (in_lds) {
+            before = s_pp[lds_lower(s_v, nops, x[e])];
+          } else {
+            const int64_t lt = lower_key(p.keys, a, b, (static_cast<u64>(u) << p.vbits) | static_cast<u64>(x[e]));
+            before = p.pend_prefix[lt] - pa;
+          }
+          const int64_t pos = dest + pre + before;
+          tn[pos] = x[e];
+          tw[pos] = w0[e];
+          for (int k = 1; k < p.K; ++k) tw[k * tstride + pos] = p.w[k * p.slot_cap + mt.x + jx];
+        }
       }
-      carry += total;
+      carry += s_step_total;
       __syncthreads();
     }
     // pending entries
-    for (int64_t q = a + threadIdx.x; q < b; q += kBlock) {
+    for (int64_t q = a + tid; q < b; q += kMergeBlock) {
       if (!p.pend_alive[q]) continue;
       const u64 key = p.keys[q];
       const int32_t v = static_cast<int32_t>(key & mask);
@@ -797,12 +972,33 @@ __global__ void __launch_bounds__(kBlock) k_merge_rows(merge_params p) {
     __syncthreads();
     const int new_len = carry + (p.pend_prefix[b] - pa);
     if (staged) {
-      for (int x = threadIdx.x; x < new_len; x += kBlock) {
-        p.nbr[mt.x + x] = p.stage_nbr[dest + x];
-        for (int k = 0; k < p.K; ++k) p.w[k * p.slot_cap + mt.x + x] = p.stage_w[k * p.stage_cap + dest + x];
+      for (int base = 0; base < new_len; base += kMergeBlock * kMergeE) {
+        int32_t v[kMergeE];
+#pragma unroll
+        for (int e = 0; e < kMergeE; ++e) {
+          const int x = base + e * kMergeBlock + tid;
+          v[e] = x < new_len ? p.stage_nbr[dest + x] : 0;
+        }
+#pragma unroll
+        for (int e = 0; e < kMergeE; ++e) {
+          const int x = base + e * kMergeBlock + tid;
+          if (x < new_len) p.nbr[mt.x + x] = v[e];
+        }
+        for (int k = 0; k < p.K; ++k) {
+#pragma unroll
+          for (int e = 0; e < kMergeE; ++e) {
+            const int x = base + e * kMergeBlock + tid;
+            v[e] = x < new_len ? p.stage_w[k * p.stage_cap + dest + x] : 0;
+          }
+#pragma unroll
+          for (int e = 0; e < kMergeE; ++e) {
+            const int x = base + e * kMergeBlock + tid;
+            if (x < new_len) p.w[k * p.slot_cap + mt.x + x] = v[e];
+          }
+        }
       }
     }
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       p.meta[u] = make_int2(staged ? mt.x : static_cast<int32_t>(dest), new_len);
       if (s_relocated) p.cap[u] = s_newcap;
     }
@@ -1351,6 +1547,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     mp.pend_w = g->pend_w.p;
     mp.touched = g->touched_out.p;
     mp.touched_count = &dc->touched_out;
+    mp.touched_big = &dc->touched_out_big;
     mp.stage_nbr = g->stage_nbr.p;
     mp.stage_w = g->stage_w.p;
     mp.stage_cap = g->stage_cap;
@@ -1360,8 +1557,12 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     mp.relocated = &dc->relocated_rows;
     mp.error_bits = &dc->error_bits;
     mp.headroom = g->props.headroom;
-    const unsigned merge_grid = static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>(N, 1), 4096));
-    hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kBlock), 0, s, mp);
+    // short rows: one wave each (16 per workgroup); long rows: one workgroup each
+    const unsigned small_grid = static_cast<unsigned>(std::min<int64_t>((N + 15) / 16, 4096));
+    const unsigned merge_grid = static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>(N, 1), 1024));
+    hipLaunchKernelGGL(k_merge_small, dim3(small_grid), dim3(kMergeBlock), 0, s, mp);
+    DYNG_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kMergeBlock), 0, s, mp);
     DYNG_HIP_TRY(hipGetLastError());
     mark_stage(g, 7, s);
     // ---- the in-rows ----
@@ -1406,10 +1607,13 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     mi.pend_w = g->ipend_w.p;
     mi.touched = g->touched_in.p;
     mi.touched_count = &dc->touched_in;
+    mi.touched_big = &dc->touched_in_big;
     mi.stage_used = &dc->stage_in;
     mi.tail = &dc->tail_in;
     mi.garbage = &dc->garbage_in;
-    hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kBlock), 0, s, mi);
+    hipLaunchKernelGGL(k_merge_small, dim3(small_grid), dim3(kMergeBlock), 0, s, mi);
+    DYNG_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kMergeBlock), 0, s, mi);
     DYNG_HIP_TRY(hipGetLastError());
     mark_stage(g, 11, s);
   }

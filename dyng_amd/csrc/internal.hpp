@@ -126,7 +126,8 @@ struct apply_control {
   unsigned long long dropped_self_loops, ignored_insertions;
   unsigned long long relocated_rows;
   int in_ops;
-  int touched_out, touched_in;
+  int touched_out, touched_in;          ///< rows of at most kMergeSmall entries
+  int touched_out_big, touched_in_big;  ///< longer rows
   int error_bits;
   long long tail_out, tail_in;        ///< arena tails (atomic bump allocation)
   long long stage_out, stage_in;      ///< staging slots used

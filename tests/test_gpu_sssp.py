@@ -459,3 +459,38 @@ def test_barrier_timeout_fails_cleanly(dg):
     d, p = fresh.to_numpy()
     assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent)
     _stream(dg, og, hg, ot, fresh, rng, 2, 200, 1, 40)
+
+
+def test_long_rows_with_many_operations(dg):
+    """Rows longer than a wave's share and batches with more operations on one row than the merge
+    stages in LDS (both directions: vertex 0's out-row and vertex 1's in-row)."""
+    _say("[long rows] n=12000")
+    rng = np.random.default_rng(12)
+    n = 12000
+    hub_out = rng.permutation(np.arange(2, n))[:6000]
+    hub_in = rng.permutation(np.arange(2, n))[:6000]
+    src = np.concatenate([np.zeros(6000, np.int64), hub_in, rng.integers(0, n, 30000)])
+    dst = np.concatenate([hub_out, np.ones(6000, np.int64), rng.integers(0, n, 30000)])
+    w = rng.integers(1, 100, len(src))
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    _assert_same_rows(hg, og)
+    ot, _ = oracle.compute(og, 0)
+    hr = dg.sssp.compute(hg, 0)
+    for b in range(3):
+        dels_out = rng.choice(hub_out, 1500, replace=False)
+        dels_in = rng.choice(hub_in, 1500, replace=False)
+        ins_out = rng.integers(2, n, 1500)
+        ins_in = rng.integers(2, n, 1500)
+        ins = (np.concatenate([np.zeros(1500, np.int64), ins_in]),
+               np.concatenate([ins_out, np.ones(1500, np.int64)]), rng.integers(1, 100, 3000))
+        dels = (np.concatenate([np.zeros(1500, np.int64), dels_in]),
+                np.concatenate([dels_out, np.ones(1500, np.int64)]))
+        og, a, c = oracle.update(og, ot, ins[0], ins[1], ins[2].reshape(-1, 1), dels[0], dels[1])
+        st = dg.sssp.update(hg, dg.EdgeBatch(insert=ins, delete=dels), hr)
+        d, p = hr.to_numpy()
+        assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent), f"batch {b}"
+        assert st.invalidated == c.invalidated and st.affected == c.affected
+        for f in ("inserted_edges", "updated_edges", "deleted_edges", "ignored_deletions"):
+            assert getattr(st.batch, f) == a.summary[f], f
+        _assert_same_rows(hg, og)
