@@ -37,8 +37,8 @@ SUMMARY_FIELDS = ("inserted_edges", "updated_edges", "deleted_edges", "ignored_d
                   "ignored_insertions", "relocated_rows", "compactions")
 
 
-APPLY_STAGES = ("scan", "make_ops", "sort_out", "pairs_out", "scans_out", "merge_out", "sort_in", "pairs_in",
-                "scans_in", "merge_in")
+APPLY_STAGES = ("scan", "make_ops", "sort_out", "pairs_out", "scans_out", "sort_in", "pairs_in", "scans_in",
+                "plan", "merge_out", "merge_in")
 
 
 class ApplySummary(C.Structure):

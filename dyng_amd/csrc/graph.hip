@@ -392,10 +392,20 @@ struct out_pair_params {
   int32_t* mark;
   int mark_gen;
   int32_t* touched;
+  int32_t* row_a;  ///< per vertex: first operation of its row in the sorted keys
+  int32_t* row_b;  ///< per vertex: one past its last operation
   apply_control* ctl;
 };
 
 enum { kInDelete = 0, kInModify = 1, kInInsert = 2 };
+
+/// Record where row r's operations start and end in the sorted keys (row = key >> vbits).
+DYNG_DEV void mark_row_range(const u64* keys, int64_t i, int64_t N, int vbits, int32_t* row_a, int32_t* row_b) {
+  const u64 key = keys[i];
+  const u64 r = key >> vbits;
+  if (i == 0 || (keys[i - 1] >> vbits) != r) row_a[r] = static_cast<int32_t>(i);
+  if (i + 1 == N || (keys[i + 1] >> vbits) != r) row_b[r] = static_cast<int32_t>(i + 1);
+}
 
 __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
   const int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x;
@@ -409,6 +419,7 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
   }
   if (i < p.N) {
     const u64 key = p.keys[i];
+    if (key != kNoKey) mark_row_range(p.keys, i, p.N, p.vbits, p.row_a, p.row_b);
     if (key == kNoKey) {
       p.pend_alive[i] = 0;
       p.kill[i] = 0;
@@ -595,6 +606,8 @@ struct in_pair_params {
   int32_t* mark;
   int mark_gen;
   int32_t* touched;
+  int32_t* row_a;
+  int32_t* row_b;
   apply_control* ctl;
 };
 
@@ -611,6 +624,7 @@ __global__ void __launch_bounds__(kBlock) k_in_pairs(in_pair_params p) {
     p.kill[i] = 0;
     return;
   }
+  mark_row_range(p.keys, i, p.N, p.vbits, p.row_a, p.row_b);
   if (i > 0 && p.keys[i - 1] == key) return;
   int64_t j = i + 1;
   while (j < p.N && p.keys[j] == key) ++j;
@@ -694,10 +708,13 @@ struct merge_params {
   const int32_t* pend_alive;
   const int32_t* pend_prefix;
   const int32_t* kill_prefix;
+  const int32_t* kill;
   const int32_t* pend_w;
   const int32_t* touched;
   const int* touched_count;     ///< rows of at most kMergeSmall entries: touched[0, count)
   const int* touched_big;       ///< longer rows: touched[N - t] for t in [0, count)
+  const int32_t* row_a;         ///< the row's operations: keys[row_a[u], row_b[u])
+  const int32_t* row_b;
   int32_t* stage_nbr;
   int32_t* stage_w;
   int64_t stage_cap;
@@ -711,9 +728,12 @@ struct merge_params {
 
 constexpr int kMergeBlock = 1024;  ///< threads of a merge workgroup
 constexpr int kMergeSeg = 2048;    ///< operations of one row staged in LDS for the merge
-constexpr int kMergeE = 8;       ///< row entries per thread per merge step
+constexpr int kMergeE = 8;         ///< row entries per thread per merge step
 constexpr int kMergeWaves = kMergeBlock / 64;
+constexpr int kRegChunks = kMergeSmall / 64;  ///< a short row held in registers, 64 entries per chunk
+constexpr int kRegWeights = 2;                ///< weight columns a short row can hold in registers
 static_assert(kMergeE * kMergeWaves == 128, "the step prefix takes two counts per lane of one wave");
+static_assert(kMergeSmall % 64 == 0, "short rows are whole chunks of 64");
 
 /// First index in [0, len) of an LDS array whose value is >= v.
 DYNG_DEV int lds_lower(const int32_t* a, int len, int32_t v) {
@@ -725,59 +745,199 @@ DYNG_DEV int lds_lower(const int32_t* a, int len, int32_t v) {
   return lo;
 }
 
-/// One wave per touched row of at most kMergeSmall entries (most rows): the merge of
-/// k_merge_rows below, wave-synchronous, with the row's few operations searched in place.
+/// Whether a short row is merged in registers, in place (no staging): its entries fit the
+/// registers and its operations one per lane.
+DYNG_DEV bool merge_in_registers(int len, int nops, int K) {
+  return len <= kMergeSmall && nops <= 64 && K <= kRegWeights;
+}
+
+/// The row's length after the batch and whether it still fits its capacity.
+struct row_plan {
+  int new_len;
+  bool fits;
+};
+
+DYNG_DEV row_plan plan_row(const merge_params& p, int32_t u, int2 mt, int64_t a, int64_t b) {
+  const int P = p.pend_prefix[b] - p.pend_prefix[a];
+  const int Kc = p.kill_prefix[b] - p.kill_prefix[a];
+  const int new_len = mt.y - Kc + P;
+  return {new_len, new_len <= p.cap[u]};
+}
+
+DYNG_DEV int grown_cap(int len, double headroom) { return len + static_cast<int>(ceil(len * headroom)) + 2; }
+
+/// Before the merges: the tail slots the relocating rows will take and the staging slots the
+/// staged rows will take, so the host can compact or grow first and no row is refused.
+__global__ void k_merge_need(merge_params p, long long* reloc_need, long long* stage_need) {
+  const int small = *p.touched_count, big = *p.touched_big;
+  long long reloc = 0, stage = 0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < small + big; t += gridDim.x * blockDim.x) {
+    const int32_t u = t < small ? p.touched[t] : p.touched[p.N - (t - small)];
+    const int2 mt = p.meta[u];
+    const int64_t a = p.row_a[u], b = p.row_b[u];
+    const row_plan rp = plan_row(p, u, mt, a, b);
+    if (!rp.fits) {
+      reloc += grown_cap(rp.new_len, p.headroom);
+    } else if (!(t < small && merge_in_registers(mt.y, static_cast<int>(b - a), p.K))) {
+      stage += rp.new_len;
+    }
+  }
+  reloc = static_cast<long long>(wave_sum(static_cast<u64>(reloc)));
+  stage = static_cast<long long>(wave_sum(static_cast<u64>(stage)));
+  if (lane() == 0) {
+    if (reloc) atomicAdd(reinterpret_cast<unsigned long long*>(reloc_need), static_cast<unsigned long long>(reloc));
+    if (stage) atomicAdd(reinterpret_cast<unsigned long long*>(stage_need), static_cast<unsigned long long>(stage));
+  }
+}
+
+/// Reserve the destination of a row: its own range through the staging arena (in place), or a
+/// new range at the tail when it outgrew its capacity. -1 when the arena is exhausted (never
+/// after the host's check).
+DYNG_DEV long long reserve_row(const merge_params& p, const row_plan& rp, int c, bool& staged, int& newcap) {
+  staged = false;
+  newcap = c;
+  if (rp.fits) {
+    const long long s = static_cast<long long>(
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.stage_used), static_cast<unsigned long long>(rp.new_len)));
+    if (s + rp.new_len <= p.stage_cap) {
+      staged = true;
+      return s;
+    }
+  }
+  const int nc = grown_cap(rp.new_len, p.headroom);
+  newcap = nc;
+  const long long d = static_cast<long long>(
+      atomicAdd(reinterpret_cast<unsigned long long*>(p.tail), static_cast<unsigned long long>(nc)));
+  if (d + nc > p.slot_cap) {
+    atomicOr(p.error_bits, static_cast<int>(DYNG_HIP_DEVICE_ERROR_CAPACITY));
+    return -1;
+  }
+  atomicAdd(reinterpret_cast<unsigned long long*>(p.garbage), static_cast<unsigned long long>(c));
+  atomicAdd(p.relocated, 1ULL);
+  return d;
+}
+
+/// One wave per touched row of at most kMergeSmall entries (most rows).
+///
+/// The usual case (at most 64 operations, at most kRegWeights weight columns) reads the whole row
+/// into registers first, so it writes the merged row straight into its own slots (or its new
+/// range) with no staging: an alive entry's place is the alive entries before it plus the
+/// pending entries with a smaller neighbour (one compare per operation, the operations one per
+/// lane); a pending entry's place is the row entries up to its neighbour (ballots over the
+/// registers) minus the deletions of its run plus the pending entries before it. Other short rows
+/// take the staged path of k_merge_rows, wave-sized.
 __global__ void __launch_bounds__(kMergeBlock) k_merge_small(merge_params p) {
   const int count = *p.touched_count;
   const u64 mask = (1ULL << p.vbits) - 1;
   const int wave = static_cast<int>((blockIdx.x * int64_t{blockDim.x} + threadIdx.x) / 64);
   const int waves = static_cast<int>((int64_t{gridDim.x} * blockDim.x) / 64);
+  const int K = p.K;
   for (int t = wave; t < count; t += waves) {
     const int32_t u = p.touched[t];
     const int2 mt = p.meta[u];
-    long long a = 0, b = 0, dest = 0;
-    int staged = 0, relocated = 0, newcap = 0;
-    if (lane() == 0) {
-      a = lower_key(p.keys, 0, p.N, static_cast<u64>(u) << p.vbits);
-      b = lower_key(p.keys, a, p.N, static_cast<u64>(u + 1) << p.vbits);
-      const int P = p.pend_prefix[b] - p.pend_prefix[a];
-      const int Kc = p.kill_prefix[b] - p.kill_prefix[a];
-      const int new_len = mt.y - Kc + P;
-      const int c = p.cap[u];
-      newcap = c;
-      bool done = false;
-      if (new_len <= c) {
-        const long long s = atomicAdd(reinterpret_cast<unsigned long long*>(p.stage_used), static_cast<unsigned long long>(new_len));
-        if (s + new_len <= p.stage_cap) {
-          staged = 1;
-          dest = s;
-          done = true;
-        }
+    const int64_t a = p.row_a[u], b = p.row_b[u];
+    const int nops = static_cast<int>(b - a);
+    const int pa = p.pend_prefix[a];
+    const int ka = p.kill_prefix[a];
+    const row_plan rp = plan_row(p, u, mt, a, b);
+    if (merge_in_registers(mt.y, nops, K)) {
+      // ---- registers: the row, then the operations one per lane ----
+      int32_t x[kRegChunks];
+      int32_t wr[kRegWeights][kRegChunks];
+#pragma unroll
+      for (int c = 0; c < kRegChunks; ++c) {
+        const int jx = c * 64 + lane();
+        const bool in = jx < mt.y;
+        x[c] = in ? p.nbr[mt.x + jx] : -1;
+#pragma unroll
+        for (int k = 0; k < kRegWeights; ++k) wr[k][c] = (in && k < K) ? p.w[k * p.slot_cap + mt.x + jx] : 0;
       }
-      if (!done) {
-        const int nc = new_len + static_cast<int>(ceil(new_len * p.headroom)) + 2;
-        newcap = nc;
-        dest = static_cast<long long>(atomicAdd(reinterpret_cast<unsigned long long*>(p.tail), static_cast<unsigned long long>(nc)));
+      const bool has = lane() < nops;
+      const int32_t ov = has ? static_cast<int32_t>(p.keys[a + lane()] & mask) : INT32_MAX;
+      const bool opend = has && p.pend_alive[a + lane()] != 0;
+      const bool okill = has && p.kill[a + lane()] != 0;
+      // the destination (every lane reads the same row; lane 0 reserves)
+      long long dest = mt.x;
+      int newcap = p.cap[u];
+      int relocated = 0;
+      if (!rp.fits) {
+        if (lane() == 0) {
+          bool staged = false;
+          dest = reserve_row(p, rp, newcap, staged, newcap);
+        }
+        dest = __shfl(dest, 0);
+        newcap = __shfl(newcap, 0);
         relocated = 1;
-        if (dest + nc > p.slot_cap) {
-          atomicOr(p.error_bits, static_cast<int>(DYNG_HIP_DEVICE_ERROR_CAPACITY));
-          dest = -1;
-        } else {
-          atomicAdd(reinterpret_cast<unsigned long long*>(p.garbage), static_cast<unsigned long long>(c));
-          atomicAdd(p.relocated, 1ULL);
-        }
+        if (dest < 0) continue;  // uniform
       }
+      // pending entries before each alive entry
+      int before[kRegChunks];
+#pragma unroll
+      for (int c = 0; c < kRegChunks; ++c) before[c] = 0;
+      for (int j = 0; j < nops; ++j) {
+        const int32_t vj = __shfl(ov, j);
+        const bool pj = __shfl(static_cast<int>(opend), j) != 0;
+        if (!pj) continue;  // uniform
+#pragma unroll
+        for (int c = 0; c < kRegChunks; ++c) before[c] += (x[c] >= 0 && vj < x[c]) ? 1 : 0;
+      }
+      // pending entries: their places, computed before any store
+      int64_t ppos = -1;
+      const u64 pend_mask = __ballot(opend);
+      for (int j = 0; j < nops; ++j) {
+        if (!((pend_mask >> j) & 1)) continue;  // uniform
+        const int32_t vj = __shfl(ov, j);
+        int y = 0;
+#pragma unroll
+        for (int c = 0; c < kRegChunks; ++c) {
+          const int jx = c * 64 + lane();
+          y += __popcll(__ballot(jx < mt.y && masked(x[c]) <= vj));
+        }
+        const int killed = __popcll(__ballot(okill && ov <= vj));
+        const int pend_before = __popcll(pend_mask & ((1ULL << j) - 1));
+        if (lane() == j) ppos = dest + (y - killed) + pend_before;
+      }
+      // stores: alive entries, then pending ones
+      int carry = 0;
+#pragma unroll
+      for (int c = 0; c < kRegChunks; ++c) {
+        const bool alive = x[c] >= 0;
+        const u64 ballot = __ballot(alive);
+        if (alive) {
+          const int64_t pos = dest + carry + __popcll(ballot & lanes_below()) + before[c];
+          p.nbr[pos] = x[c];
+#pragma unroll
+          for (int k = 0; k < kRegWeights; ++k) {
+            if (k < K) p.w[k * p.slot_cap + pos] = wr[k][c];
+          }
+        }
+        carry += __popcll(ballot);
+      }
+      if (opend) {
+        p.nbr[ppos] = ov;
+        for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + ppos] = p.pend_w[k * p.N + a + lane()];
+      }
+      if (lane() == 0) {
+        p.meta[u] = make_int2(static_cast<int32_t>(dest), rp.new_len);
+        if (relocated) p.cap[u] = newcap;
+      }
+      continue;
     }
-    a = __shfl(a, 0);
-    b = __shfl(b, 0);
+    // ---- staged: the row's slots are rewritten through the staging arena ----
+    long long dest = 0;
+    int staged = 0, newcap = 0;
+    if (lane() == 0) {
+      bool st_ = false;
+      dest = reserve_row(p, rp, p.cap[u], st_, newcap);
+      staged = st_ ? 1 : 0;
+    }
     dest = __shfl(dest, 0);
     staged = __shfl(staged, 0);
+    newcap = __shfl(newcap, 0);
     if (dest < 0) continue;  // uniform over the wave
     int32_t* tn = staged ? p.stage_nbr : p.nbr;
     int32_t* tw = staged ? p.stage_w : p.w;
     const int64_t tstride = staged ? p.stage_cap : p.slot_cap;
-    const int pa = p.pend_prefix[a];
-    const int ka = p.kill_prefix[a];
     int carry = 0;
     for (int base = 0; base < mt.y; base += 64) {
       const int jx = base + lane();
@@ -788,7 +948,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_merge_small(merge_params p) {
         const int64_t lt = lower_key(p.keys, a, b, (static_cast<u64>(u) << p.vbits) | static_cast<u64>(x));
         const int64_t pos = dest + carry + __popcll(ballot & lanes_below()) + (p.pend_prefix[lt] - pa);
         tn[pos] = x;
-        for (int k = 0; k < p.K; ++k) tw[k * tstride + pos] = p.w[k * p.slot_cap + mt.x + jx];
+        for (int k = 0; k < K; ++k) tw[k * tstride + pos] = p.w[k * p.slot_cap + mt.x + jx];
       }
       carry += __popcll(ballot);
     }
@@ -801,26 +961,27 @@ __global__ void __launch_bounds__(kMergeBlock) k_merge_small(merge_params p) {
       const int killed = p.kill_prefix[se] - ka;
       const int64_t pos = dest + (y - killed) + (p.pend_prefix[q] - pa);
       tn[pos] = v;
-      for (int k = 0; k < p.K; ++k) tw[k * tstride + pos] = p.pend_w[k * p.N + q];
+      for (int k = 0; k < K; ++k) tw[k * tstride + pos] = p.pend_w[k * p.N + q];
     }
     const int new_len = carry + (p.pend_prefix[b] - pa);
     if (staged) {
       __threadfence_block();  // the wave's staging stores before its reads of them
       for (int x = lane(); x < new_len; x += 64) {
         p.nbr[mt.x + x] = p.stage_nbr[dest + x];
-        for (int k = 0; k < p.K; ++k) p.w[k * p.slot_cap + mt.x + x] = p.stage_w[k * p.stage_cap + dest + x];
+        for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + mt.x + x] = p.stage_w[k * p.stage_cap + dest + x];
       }
     }
     if (lane() == 0) {
       p.meta[u] = make_int2(staged ? mt.x : static_cast<int32_t>(dest), new_len);
-      if (relocated) p.cap[u] = newcap;
+      if (!staged) p.cap[u] = newcap;
     }
   }
 }
 
-/// One workgroup per touched row: the alive entries and the pending ones, merged by neighbour id
-/// (existing entries before new ones of the same neighbour), written in place through the
-/// staging arena, or to a new slot range at the tail when the row outgrew its capacity.
+/// One workgroup per touched row longer than kMergeSmall: the alive entries and the pending
+/// ones, merged by neighbour id (existing entries before new ones of the same neighbour),
+/// written in place through the staging arena, or to a new slot range at the tail when the row
+/// outgrew its capacity.
 ///
 /// The row's operations (a sorted key segment) are staged in LDS with the number of pending
 /// entries before each, so an alive entry finds its place with a binary search in LDS. The row
@@ -832,8 +993,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_merge_rows(merge_params p) {
   __shared__ int32_t s_v[kMergeSeg];       // neighbour of each operation of the row
   __shared__ int32_t s_pp[kMergeSeg + 1];  // pending entries before each operation
   __shared__ long long s_dest;
-  __shared__ int s_staged, s_newcap, s_relocated;
-  __shared__ int64_t s_a, s_b;
+  __shared__ int s_staged, s_newcap;
   const int count = *p.touched_big;
   const u64 mask = (1ULL << p.vbits) - 1;
   const int tid = static_cast<int>(threadIdx.x);
@@ -841,48 +1001,20 @@ __global__ void __launch_bounds__(kMergeBlock) k_merge_rows(merge_params p) {
   for (int t = blockIdx.x; t < count; t += gridDim.x) {
     const int32_t u = p.touched[p.N - t];
     const int2 mt = p.meta[u];
+    const int64_t a = p.row_a[u], b = p.row_b[u];
     if (tid == 0) {
-      const int64_t a = lower_key(p.keys, 0, p.N, static_cast<u64>(u) << p.vbits);
-      const int64_t b = lower_key(p.keys, a, p.N, static_cast<u64>(u + 1) << p.vbits);
-      const int P = p.pend_prefix[b] - p.pend_prefix[a];
-      const int Kc = p.kill_prefix[b] - p.kill_prefix[a];
-      const int new_len = mt.y - Kc + P;
-      const int c = p.cap[u];
-      s_a = a;
-      s_b = b;
-      s_staged = 0;
-      s_relocated = 0;
-      s_newcap = c;
-      bool done = false;
-      if (new_len <= c) {
-        const long long s = atomicAdd(reinterpret_cast<unsigned long long*>(p.stage_used), static_cast<unsigned long long>(new_len));
-        if (s + new_len <= p.stage_cap) {
-          s_staged = 1;
-          s_dest = s;
-          done = true;
-        }
-      }
-      if (!done) {
-        const int nc = new_len + static_cast<int>(ceil(new_len * p.headroom)) + 2;
-        s_newcap = nc;
-        s_dest = static_cast<long long>(atomicAdd(reinterpret_cast<unsigned long long*>(p.tail), static_cast<unsigned long long>(nc)));
-        s_relocated = 1;
-        if (s_dest + nc > p.slot_cap) {
-          // the host sizes the tail for the worst case; never write past the arena
-          atomicOr(p.error_bits, static_cast<int>(DYNG_HIP_DEVICE_ERROR_CAPACITY));
-          s_dest = -1;
-        } else {
-          atomicAdd(reinterpret_cast<unsigned long long*>(p.garbage), static_cast<unsigned long long>(c));
-          atomicAdd(p.relocated, 1ULL);
-        }
-      }
+      const row_plan rp = plan_row(p, u, mt, a, b);
+      bool staged = false;
+      int newcap = 0;
+      s_dest = reserve_row(p, rp, p.cap[u], staged, newcap);
+      s_staged = staged ? 1 : 0;
+      s_newcap = newcap;
     }
     __syncthreads();
     if (s_dest < 0) {
       __syncthreads();
       continue;
     }
-    const int64_t a = s_a, b = s_b;
     const long long dest = s_dest;
     const bool staged = s_staged != 0;
     int32_t* tn = staged ? p.stage_nbr : p.nbr;
@@ -1000,7 +1132,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_merge_rows(merge_params p) {
     }
     if (tid == 0) {
       p.meta[u] = make_int2(staged ? mt.x : static_cast<int32_t>(dest), new_len);
-      if (s_relocated) p.cap[u] = s_newcap;
+      if (!staged) p.cap[u] = s_newcap;
     }
     __syncthreads();
   }
@@ -1327,7 +1459,7 @@ void stage_times(dyng_hip_graph* g, bool rows, dyng_hip_apply_summary* out) {
   float ms = 0.0f;
   if (hipEventElapsedTime(&ms, g->apply_ev[0], g->apply_ev[1]) == hipSuccess) out->stage_ms[0] = ms;
   if (!rows) return;
-  for (int i = 2; i < 11; ++i) {
+  for (int i = 2; i < 12; ++i) {
     if (hipEventElapsedTime(&ms, g->apply_ev[i], g->apply_ev[i + 1]) == hipSuccess) out->stage_ms[i - 1] = ms;
   }
 }
@@ -1426,15 +1558,6 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
   const int stride = g->props.directed ? 1 : 2;
   const int64_t N = (num_ins + num_del) * stride;
   if (N >= (int64_t{1} << 31)) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: batch too large");
-  // ---- make sure the tails can take every relocation this batch could cause ----
-  for (row_store* rs : {&g->out, &g->in}) {
-    // every touched row relocating: sum of (len + pending) * (1 + headroom) + 3 per row
-    const int64_t need = static_cast<int64_t>((g->m + N) * (1.0 + g->props.headroom)) + 4 * N + 1024;
-    if (rs->slot_cap - rs->tail < need) {
-      compact_store(g, *rs, need, s);
-      if (out) out->compactions += 1;
-    }
-  }
   grow_vertices(g, n_after, s);
   // deletions below the old vertex count that name vertices of the growth are fine; ids in the
   // batch >= n_after only occur in deletions (handled in Step 0)
@@ -1474,9 +1597,10 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
   g->ipend_w.reserve(N * K);
   g->touched_out.reserve(N + 1);
   g->touched_in.reserve(N + 1);
-  g->stage_cap = std::max<int64_t>(int64_t{1} << 20, 2 * N);
-  g->stage_nbr.reserve(g->stage_cap);
-  g->stage_w.reserve(g->stage_cap * K);
+  g->row_a_out.reserve(g->n_cap);
+  g->row_b_out.reserve(g->n_cap);
+  g->row_a_in.reserve(g->n_cap);
+  g->row_b_in.reserve(g->n_cap);
   if (++g->mark_generation == INT32_MAX) {
     fill<int32_t>(g->mark_out.p, g->n_cap, 0, s);
     fill<int32_t>(g->mark_in.p, g->n_cap, 0, s);
@@ -1524,6 +1648,8 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     op.mark = g->mark_out.p;
     op.mark_gen = g->mark_generation;
     op.touched = g->touched_out.p;
+    op.row_a = g->row_a_out.p;
+    op.row_b = g->row_b_out.p;
     op.ctl = dc;
     hipLaunchKernelGGL(k_out_pairs, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, op);
     DYNG_HIP_TRY(hipGetLastError());
@@ -1531,43 +1657,9 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     exclusive_scan<int32_t>(g->scan_tmp, g->pend_alive.p, g->pend_prefix.p, N + 1, s);
     exclusive_scan<int32_t>(g->scan_tmp, g->kill.p, g->kill_prefix.p, N + 1, s);
     mark_stage(g, 6, s);
-    merge_params mp{};
-    mp.meta = g->out.meta.p;
-    mp.cap = g->out.cap.p;
-    mp.nbr = g->out.nbr.p;
-    mp.w = g->out.w.p;
-    mp.slot_cap = g->out.slot_cap;
-    mp.K = K;
-    mp.keys = g->keys_b.p;
-    mp.N = N;
-    mp.vbits = vbits;
-    mp.pend_alive = g->pend_alive.p;
-    mp.pend_prefix = g->pend_prefix.p;
-    mp.kill_prefix = g->kill_prefix.p;
-    mp.pend_w = g->pend_w.p;
-    mp.touched = g->touched_out.p;
-    mp.touched_count = &dc->touched_out;
-    mp.touched_big = &dc->touched_out_big;
-    mp.stage_nbr = g->stage_nbr.p;
-    mp.stage_w = g->stage_w.p;
-    mp.stage_cap = g->stage_cap;
-    mp.stage_used = &dc->stage_out;
-    mp.tail = &dc->tail_out;
-    mp.garbage = &dc->garbage_out;
-    mp.relocated = &dc->relocated_rows;
-    mp.error_bits = &dc->error_bits;
-    mp.headroom = g->props.headroom;
-    // short rows: one wave each (16 per workgroup); long rows: one workgroup each
-    const unsigned small_grid = static_cast<unsigned>(std::min<int64_t>((N + 15) / 16, 4096));
-    const unsigned merge_grid = static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>(N, 1), 1024));
-    hipLaunchKernelGGL(k_merge_small, dim3(small_grid), dim3(kMergeBlock), 0, s, mp);
-    DYNG_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kMergeBlock), 0, s, mp);
-    DYNG_HIP_TRY(hipGetLastError());
-    mark_stage(g, 7, s);
-    // ---- the in-rows ----
+    // ---- the in-rows' operations (they do not read the out-rows) ----
     sort_pairs(g->sort_tmp, g->ikeys_a.p, g->ikeys_b.p, g->ivals_a.p, g->ivals_b.p, N, 2 * vbits, s);
-    mark_stage(g, 8, s);
+    mark_stage(g, 7, s);
     in_pair_params ip{};
     ip.keys = g->ikeys_b.p;
     ip.vals = g->ivals_b.p;
@@ -1587,35 +1679,105 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     ip.mark = g->mark_in.p;
     ip.mark_gen = g->mark_generation;
     ip.touched = g->touched_in.p;
+    ip.row_a = g->row_a_in.p;
+    ip.row_b = g->row_b_in.p;
     ip.ctl = dc;
     hipLaunchKernelGGL(k_in_pairs, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, ip);
     DYNG_HIP_TRY(hipGetLastError());
-    mark_stage(g, 9, s);
+    mark_stage(g, 8, s);
     exclusive_scan<int32_t>(g->scan_tmp, g->ipend_alive.p, g->ipend_prefix.p, N + 1, s);
     exclusive_scan<int32_t>(g->scan_tmp, g->ikill.p, g->ikill_prefix.p, N + 1, s);
+    mark_stage(g, 9, s);
+    // ---- plan the merges: the exact tail and staging slots they take ----
+    auto merge_setup = [&](bool outs) {
+      row_store& rs = outs ? g->out : g->in;
+      merge_params mp{};
+      mp.meta = rs.meta.p;
+      mp.cap = rs.cap.p;
+      mp.nbr = rs.nbr.p;
+      mp.w = rs.w.p;
+      mp.slot_cap = rs.slot_cap;
+      mp.K = K;
+      mp.keys = outs ? g->keys_b.p : g->ikeys_b.p;
+      mp.N = N;
+      mp.vbits = vbits;
+      mp.pend_alive = outs ? g->pend_alive.p : g->ipend_alive.p;
+      mp.pend_prefix = outs ? g->pend_prefix.p : g->ipend_prefix.p;
+      mp.kill_prefix = outs ? g->kill_prefix.p : g->ikill_prefix.p;
+      mp.kill = outs ? g->kill.p : g->ikill.p;
+      mp.pend_w = outs ? g->pend_w.p : g->ipend_w.p;
+      mp.touched = outs ? g->touched_out.p : g->touched_in.p;
+      mp.touched_count = outs ? &dc->touched_out : &dc->touched_in;
+      mp.touched_big = outs ? &dc->touched_out_big : &dc->touched_in_big;
+      mp.row_a = outs ? g->row_a_out.p : g->row_a_in.p;
+      mp.row_b = outs ? g->row_b_out.p : g->row_b_in.p;
+      mp.stage_nbr = g->stage_nbr.p;
+      mp.stage_w = g->stage_w.p;
+      mp.stage_cap = g->stage_cap;
+      mp.stage_used = outs ? &dc->stage_out : &dc->stage_in;
+      mp.tail = outs ? &dc->tail_out : &dc->tail_in;
+      mp.garbage = outs ? &dc->garbage_out : &dc->garbage_in;
+      mp.relocated = &dc->relocated_rows;
+      mp.error_bits = &dc->error_bits;
+      mp.headroom = g->props.headroom;
+      return mp;
+    };
+    const unsigned need_grid = static_cast<unsigned>(std::min<int64_t>(blocks_for(N), 1024));
+    for (int attempt = 0;; ++attempt) {
+      hc->need_reloc_out = hc->need_reloc_in = hc->need_stage_out = hc->need_stage_in = 0;
+      DYNG_HIP_TRY(hipMemcpyAsync(&dc->need_reloc_out, &hc->need_reloc_out, 4 * sizeof(long long),
+                                  hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_merge_need, dim3(need_grid), dim3(kBlock), 0, s, merge_setup(true), &dc->need_reloc_out,
+                         &dc->need_stage_out);
+      hipLaunchKernelGGL(k_merge_need, dim3(need_grid), dim3(kBlock), 0, s, merge_setup(false), &dc->need_reloc_in,
+                         &dc->need_stage_in);
+      DYNG_HIP_TRY(hipGetLastError());
+      DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
+      DYNG_HIP_TRY(hipStreamSynchronize(s));
+      if (hc->error_bits != 0) break;  // reported below
+      bool compacted = false;
+      for (int side = 0; side < 2; ++side) {
+        row_store& rs = side == 0 ? g->out : g->in;
+        const int64_t need = (side == 0 ? hc->need_reloc_out : hc->need_reloc_in) + 1024;
+        if (rs.slot_cap - rs.tail < need) {
+          if (attempt > 2) fail(DYNG_HIP_RUNTIME_ERROR, "dyng: graph apply: the row arena cannot take the batch");
+          compact_store(g, rs, 2 * need, s);
+          if (out) out->compactions += 1;
+          compacted = true;
+        }
+      }
+      if (!compacted) break;
+      // compaction recomputed the capacities: plan again on the fresh arenas
+      hc->tail_out = g->out.tail;
+      hc->tail_in = g->in.tail;
+      hc->garbage_out = hc->garbage_in = 0;
+      DYNG_HIP_TRY(hipMemcpyAsync(&dc->tail_out, &hc->tail_out, 2 * sizeof(long long), hipMemcpyHostToDevice, s));
+      DYNG_HIP_TRY(hipMemcpyAsync(&dc->garbage_out, &hc->garbage_out, 2 * sizeof(long long), hipMemcpyHostToDevice, s));
+    }
+    const int64_t stage_need = std::max(hc->need_stage_out, hc->need_stage_in);
+    if (stage_need > g->stage_cap) {
+      g->stage_cap = stage_need + stage_need / 4 + 4096;
+      g->stage_nbr.reserve(g->stage_cap);
+      g->stage_w.reserve(g->stage_cap * K);
+    }
     mark_stage(g, 10, s);
-    merge_params mi = mp;
-    mi.meta = g->in.meta.p;
-    mi.cap = g->in.cap.p;
-    mi.nbr = g->in.nbr.p;
-    mi.w = g->in.w.p;
-    mi.slot_cap = g->in.slot_cap;
-    mi.keys = g->ikeys_b.p;
-    mi.pend_alive = g->ipend_alive.p;
-    mi.pend_prefix = g->ipend_prefix.p;
-    mi.kill_prefix = g->ikill_prefix.p;
-    mi.pend_w = g->ipend_w.p;
-    mi.touched = g->touched_in.p;
-    mi.touched_count = &dc->touched_in;
-    mi.touched_big = &dc->touched_in_big;
-    mi.stage_used = &dc->stage_in;
-    mi.tail = &dc->tail_in;
-    mi.garbage = &dc->garbage_in;
-    hipLaunchKernelGGL(k_merge_small, dim3(small_grid), dim3(kMergeBlock), 0, s, mi);
-    DYNG_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kMergeBlock), 0, s, mi);
-    DYNG_HIP_TRY(hipGetLastError());
-    mark_stage(g, 11, s);
+    // ---- the merges: short rows one wave each (16 per workgroup), long rows one workgroup each ----
+    const unsigned small_grid = static_cast<unsigned>(std::min<int64_t>((N + 15) / 16, 4096));
+    const unsigned merge_grid = static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>(N, 1), 1024));
+    if (hc->error_bits == 0) {
+      const merge_params mo = merge_setup(true);
+      hipLaunchKernelGGL(k_merge_small, dim3(small_grid), dim3(kMergeBlock), 0, s, mo);
+      DYNG_HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kMergeBlock), 0, s, mo);
+      DYNG_HIP_TRY(hipGetLastError());
+      mark_stage(g, 11, s);
+      const merge_params mi = merge_setup(false);
+      hipLaunchKernelGGL(k_merge_small, dim3(small_grid), dim3(kMergeBlock), 0, s, mi);
+      DYNG_HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kMergeBlock), 0, s, mi);
+      DYNG_HIP_TRY(hipGetLastError());
+      mark_stage(g, 12, s);
+    }
   }
   DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
   DYNG_HIP_TRY(hipStreamSynchronize(s));
@@ -1650,7 +1812,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     out->inserted_vertices = n_after - n_before;
     out->num_vertices_after = n_after;
     out->relocated_rows = static_cast<int64_t>(hc->relocated_rows);
-    stage_times(g, N > 0, out);
+    stage_times(g, N > 0 && hc->error_bits == 0, out);
   }
   // the last edge of the largest weight left: recompute the column summaries (reuses the control
   // block, so only after the counters above were read)

@@ -145,6 +145,8 @@ struct apply_control {
   int batch_wmax[max_weights];
   int batch_wmin[max_weights];
   long long col_bad[max_weights];  ///< weights below 1 per column (column summary)
+  // merge plan: tail slots of the relocating rows, staging slots of the staged rows
+  long long need_reloc_out, need_reloc_in, need_stage_out, need_stage_in;
 };
 
 /// What the batch scan found (before anything changes) and where the batch is on the device.
@@ -190,14 +192,15 @@ struct dyng_hip_graph {
   dyng_hip::dbuf<int32_t> ipend_alive, ikill, ipend_prefix, ikill_prefix, ipend_w;
   dyng_hip::dbuf<int32_t> iop_type, iop_wold, iop_wnew;
   dyng_hip::dbuf<int32_t> touched_out, touched_in, mark_out, mark_in;
+  dyng_hip::dbuf<int32_t> row_a_out, row_b_out, row_a_in, row_b_in;  ///< per vertex: its operations
   dyng_hip::dbuf<int32_t> stage_nbr, stage_w;
   int64_t stage_cap = 0;
   int mark_generation = 0;
   dyng_hip::dbuf<int32_t> b_ins_src, b_ins_dst, b_ins_w, b_del_src, b_del_dst;
   dyng_hip::dbuf<dyng_hip::apply_control> ctl;
   dyng_hip::pinned<dyng_hip::apply_control> host_ctl;
-  /// events at the apply's stage boundaries (0-1 batch scan, 2-11 the update of the rows)
-  hipEvent_t apply_ev[12] = {};
+  /// events at the apply's stage boundaries (0-1 batch scan, 2-12 the update of the rows)
+  hipEvent_t apply_ev[13] = {};
   bool apply_ev_recorded = false;
 
   dyng_hip_graph() = default;

@@ -82,8 +82,9 @@ typedef struct dyng_hip_apply_summary {
   int64_t inserted_edges, updated_edges, deleted_edges, ignored_deletions, dropped_self_loops;
   int64_t inserted_vertices, num_vertices_after, ignored_insertions, relocated_rows, compactions;
   /* device time of the apply's stages (HIP events, ms): batch scan, operation keys, out-key sort,
-   * out-row pairing, out scans, out-row merge, in-key sort, in-row pairing, in scans, in-row merge */
-  double stage_ms[10];
+   * out-row pairing, out scans, in-key sort, in-row pairing, in scans, merge planning (with its
+   * host round trip and any compaction), out-row merge, in-row merge */
+  double stage_ms[11];
 } dyng_hip_apply_summary;
 
 /* A batch (cpp/include/dyng/graph/edge_batch.hpp:38-76): insertions (u, v, K weights,
