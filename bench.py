@@ -36,6 +36,10 @@ WORKLOADS = {
     # name: (kind, size, edges, wmax, config index, batches, (ins, del, rew))
     "rmat22_mixed100k": ("rmat", 22, 67_108_864, 255, 1, 20, (40_000, 40_000, 20_000)),
     "er64k_mixed1k": ("er", 65_536, 1_048_576, 100, 0, 10, (500, 500, 0)),
+    # grid side 4096 (16,777,216 vertices, 4-neighbour arcs both ways + 0.1 % shortcuts); each
+    # batch is drawn against the current tree: (insertions, tree-edge deletions, random grid-arc
+    # deletions), so 80 % of the deletions hit tree edges
+    "grid4096_tree10k": ("grid", 4096, None, 1000, 2, 10, (5_000, 4_000, 1_000)),
 }
 
 
@@ -60,25 +64,42 @@ def relax_bytes(c) -> int:
 
 
 def make_inputs(name: str, batches_needed: int):
+    """The graph, the source and `batch(b, parents)`: batch b of the workload (mixed batches are
+    drawn up front; tree batches against the parents of the tree they will update)."""
     from dyng_amd import generators as G
 
     kind, size, m, wmax, cfg, nb, (ni, nd, nr) = WORKLOADS[name]
     nb = max(nb, batches_needed)
     t0 = time.perf_counter()
-    if kind == "rmat":
-        n = 1 << size
-        src, dst, w = G.rmat(size, m, wmax, G.GRAPH_SEED ^ cfg)
-        batches = G.mixed_batches(src, dst, n, nb, ni, nd, nr, wmax, G.BATCH_SEED ^ cfg, rmat_scale=size)
+    if kind == "grid":
+        n = size * size
+        src, dst, w = G.grid(size, 0.001, wmax, G.GRAPH_SEED ^ cfg)
+        m = len(src)
+
+        def batch(b, parents):
+            return G.tree_batch(size, parents, ni, nd, nr, wmax, G.BATCH_SEED ^ cfg, b)
+
+        batch.dynamic = True
     else:
-        n = size
-        src, dst, w = G.er(n, m, wmax, G.GRAPH_SEED ^ cfg)
-        batches = G.mixed_batches(src, dst, n, nb, ni, nd, nr, wmax, G.BATCH_SEED ^ cfg)
+        if kind == "rmat":
+            n = 1 << size
+            src, dst, w = G.rmat(size, m, wmax, G.GRAPH_SEED ^ cfg)
+            drawn = G.mixed_batches(src, dst, n, nb, ni, nd, nr, wmax, G.BATCH_SEED ^ cfg, rmat_scale=size)
+        else:
+            n = size
+            src, dst, w = G.er(n, m, wmax, G.GRAPH_SEED ^ cfg)
+            drawn = G.mixed_batches(src, dst, n, nb, ni, nd, nr, wmax, G.BATCH_SEED ^ cfg)
+
+        def batch(b, parents):
+            return drawn[b]
+
+        batch.dynamic = False
     source = G.max_out_degree_vertex(src, n) if kind == "rmat" else 0
     log(f"[bench] inputs {name}: n={n} m={m} batches={nb}x{ni + nd + nr} in {time.perf_counter() - t0:.1f}s")
-    return n, src, dst, w, batches, source, ni + nd + nr
+    return n, src, dst, w, batch, source, ni + nd + nr
 
 
-def cpu_baseline(n, src, dst, w, batches, source, sample: int, threads: int):
+def cpu_baseline(n, src, dst, w, batch, source, sample: int, threads: int):
     """The oracle's OpenMP engine (openmp.cpp restated) on `sample` batches: repair time per
     batch (identify_affected + seed + loop + finalize) and the full host update (MOSP apply +
     transpose + repair). Returns (summary, trees after each sampled batch)."""
@@ -91,9 +112,10 @@ def cpu_baseline(n, src, dst, w, batches, source, sample: int, threads: int):
     t0 = time.perf_counter()
     tree, _ = oracle.compute(g, source, 0, "omp", threads=threads, ws=ws)
     t_compute = time.perf_counter() - t0
-    repair, full, trees = [], [], []
+    repair, full, trees, drawn = [], [], [], []
     for b in range(sample):
-        ins_s, ins_d, ins_w, del_s, del_d = batches[b]
+        drawn.append(batch(b, tree.parent))
+        ins_s, ins_d, ins_w, del_s, del_d = drawn[-1]
         t1 = time.perf_counter()
         g, _, c = oracle.update(g, tree, ins_s, ins_d, ins_w.reshape(-1, 1), del_s, del_d, engine="omp",
                                 threads=threads, ws=ws)
@@ -101,7 +123,8 @@ def cpu_baseline(n, src, dst, w, batches, source, sample: int, threads: int):
         repair.append(sum(c.t_hooks))
         trees.append((tree.dist.copy(), tree.parent.copy(), c.invalidated, c.affected))
         log(f"[bench] cpu batch {b}: repair {repair[-1] * 1e3:.1f} ms, full update {full[-1] * 1e3:.0f} ms")
-    return {"compute_s": t_compute, "repair_ms": [x * 1e3 for x in repair], "update_ms": [x * 1e3 for x in full]}, trees
+    return ({"compute_s": t_compute, "repair_ms": [x * 1e3 for x in repair], "update_ms": [x * 1e3 for x in full]},
+            trees, drawn)
 
 
 def main() -> int:
@@ -126,7 +149,7 @@ def main() -> int:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
     K, W = args.steps, args.warmup
-    n, src, dst, w, batches, source, ops = make_inputs(args.workload, K + W)
+    n, src, dst, w, batch, source, ops = make_inputs(args.workload, K + W)
 
     # ---- GPU: graph, initial tree, batches resident in HBM ----
     t0 = time.perf_counter()
@@ -135,20 +158,27 @@ def main() -> int:
     dyng_amd.synchronize()
     log(f"[bench] device graph + compute in {time.perf_counter() - t0:.1f}s")
     D = dyng_amd.DeviceArray
-    dev_batches = [dyng_amd.EdgeBatch(insert=(D(b[0]), D(b[1]), D(b[2])), delete=(D(b[3]), D(b[4])))
-                   for b in batches[: K + W]]
-    for b in range(W):
-        sssp.update(g, dev_batches[b], tree)
-    dyng_amd.synchronize()
+
+    def device_batch(b):
+        x = batch(b, tree.to_numpy()[1] if batch.dynamic else None)
+        return dyng_amd.EdgeBatch(insert=(D(x[0]), D(x[1]), D(x[2])), delete=(D(x[3]), D(x[4])))
+
+    # mixed batches are uploaded up front; a tree batch is drawn (outside the timed call) against
+    # the tree the previous update left
+    dev_batches = [None] * (K + W) if batch.dynamic else [device_batch(b) for b in range(K + W)]
     wall, stats = [], []
-    t_start = time.perf_counter()
-    for b in range(W, W + K):
+    for b in range(W + K):
+        eb = dev_batches[b] if dev_batches[b] is not None else device_batch(b)
+        dyng_amd.synchronize()
         t1 = time.perf_counter()
-        st = sssp.update(g, dev_batches[b], tree)
-        wall.append((time.perf_counter() - t1) * 1e3)
-        stats.append(st)
+        st = sssp.update(g, eb, tree)
+        dt = (time.perf_counter() - t1) * 1e3
+        if b >= W:
+            wall.append(dt)
+            stats.append(st)
+        del eb
     dyng_amd.synchronize()
-    total_s = time.perf_counter() - t_start
+    total_s = sum(wall) / 1e3
     p50 = statistics.median(wall)
     value = ops / (p50 / 1e3)
     log(f"[bench] gpu: p50 {p50:.3f} ms/batch, mean {total_s / K * 1e3:.3f} ms, {value:.3e} edges/s")
@@ -174,7 +204,7 @@ def main() -> int:
     exact = None
     if not args.no_cpu and args.cpu_sample > 0:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        summary, trees = cpu_baseline(n, src, dst, w, batches, source, args.cpu_sample, threads)
+        summary, trees, drawn = cpu_baseline(n, src, dst, w, batch, source, args.cpu_sample, threads)
         rep = statistics.median(summary["repair_ms"])
         cpu = {"value": round(ops / (rep / 1e3), 1), "unit": "edges/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_sample} batches x {ops} ops after compute(); repair = identify_affected + "
@@ -186,7 +216,8 @@ def main() -> int:
         t2 = sssp.compute(g2, source)
         ok = True
         for b in range(args.cpu_sample):
-            st = sssp.update(g2, dev_batches[b], t2)
+            x = drawn[b]
+            st = sssp.update(g2, dyng_amd.EdgeBatch(insert=(x[0], x[1], x[2]), delete=(x[3], x[4])), t2)
             d, p = t2.to_numpy()
             cd, cp, inv, aff = trees[b]
             same = bool(np.array_equal(d, cd) and np.array_equal(p, cp) and st.invalidated == inv

@@ -92,10 +92,16 @@ DYNG_DEV bool claim(int* stamp, int32_t v, int generation) {
 }
 
 /// Grid-wide barrier of a persistent kernel whose workgroups are all resident (the grid is sized
-/// to at most half of what the occupancy calculator says fits at once). Release fence, one
-/// arrival per workgroup on an agent-scope counter, the last arrival bumps the generation,
-/// acquire fence. Per-XCD L2s are written back and invalidated by the agent-scope fences, so data
-/// written before the barrier is visible after it on every XCD.
+/// to at most half of what the occupancy calculator says fits at once): one arrival per
+/// workgroup on an agent-scope counter, the last arrival bumps the generation.
+///
+/// Two flavours. `Fenced` adds an agent-scope release fence before the arrival and an acquire
+/// fence after the wait (on MI355X they write back and invalidate the XCD's L2), so plain loads
+/// and stores on either side are ordered. Without it the barrier only waits for the thread's own
+/// memory operations to complete (s_waitcnt) before the arrival: the phases it separates must
+/// access every location that another workgroup writes during the kernel with agent-scope
+/// operations only (ld / st / atomics), which are coherent across the XCDs' L2s by themselves.
+/// That saves the two cache walks, the bulk of the barrier's cost.
 ///
 /// Every wait is bounded: a workgroup that waits longer than `limit` ticks of the 100 MHz
 /// real-time counter raises `abort`, and every workgroup that sees `abort` while waiting leaves
@@ -108,24 +114,27 @@ struct grid_sync_state {
   unsigned int pad;
 };
 
+template <bool Fenced>
 DYNG_DEV bool grid_barrier(grid_sync_state* s, unsigned long long limit) {
   __shared__ int go;
-  // The workgroup barrier orders every wave's accesses before thread 0's agent-scope release
-  // (which writes back the XCD's L2), and thread 0's acquire (which invalidates the caches) before
-  // every wave's later loads: one fence pair per workgroup.
+  constexpr int release = Fenced ? __ATOMIC_RELEASE : __ATOMIC_RELAXED;
+  constexpr int acquire = Fenced ? __ATOMIC_ACQUIRE : __ATOMIC_RELAXED;
+  constexpr int acq_rel = Fenced ? __ATOMIC_ACQ_REL : __ATOMIC_RELAXED;
+  // every wave's loads and stores have completed (at the scope they were issued with) before
+  // thread 0 arrives
+  if (!Fenced) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (Fenced) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned int g = __hip_atomic_load(&s->generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned int arrived =
-        __hip_atomic_fetch_add(&s->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int arrived = __hip_atomic_fetch_add(&s->count, 1u, acq_rel, __HIP_MEMORY_SCOPE_AGENT);
     int ok = 1;
     if (arrived == gridDim.x - 1) {
       __hip_atomic_store(&s->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&s->generation, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&s->generation, 1u, release, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-      while (__hip_atomic_load(&s->generation, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+      while (__hip_atomic_load(&s->generation, acquire, __HIP_MEMORY_SCOPE_AGENT) == g) {
         if (__hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
           ok = 0;
           break;
@@ -138,7 +147,7 @@ DYNG_DEV bool grid_barrier(grid_sync_state* s, unsigned long long limit) {
         __builtin_amdgcn_s_sleep(1);
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (Fenced) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     go = ok;
   }
   __syncthreads();

@@ -238,4 +238,59 @@ int dyng_gen_batches(const int32_t* src, const int32_t* dst, int64_t m, int64_t 
   return 0;
 }
 
+/// A batch aimed at the current shortest-path tree (the high-diameter grid configuration):
+/// `n_tree` deletions of distinct tree edges (parent[v], v) for random reachable v, `n_rand`
+/// deletions of random grid arcs (a grid arc deleted before makes an ignored deletion), and
+/// `n_ins` insertions of random non-loop arcs with w ~ U[1, wmax] (an arc already present is an
+/// upsert). Every draw is splitmix64 of (seed, batch, draw index), so the batch depends only on
+/// the tree it is drawn against. Returns 0, or -1 when the tree has too few edges.
+int dyng_gen_tree_batch(int64_t side, const int32_t* parents, int64_t n_ins, int64_t n_tree, int64_t n_rand,
+                        int32_t wmax, uint64_t seed, int64_t batch, int32_t* ins_src, int32_t* ins_dst,
+                        int32_t* ins_w, int32_t* del_src, int32_t* del_dst) {
+  const int64_t n = side * side;
+  const uint64_t s = seed ^ (0x7452ULL << 40) ^ static_cast<uint64_t>(batch) * 0x9E3779B97F4A7C15ULL;
+  int64_t with_parent = 0;
+  for (int64_t v = 0; v < n && with_parent <= n_tree; ++v) with_parent += parents[v] >= 0 ? 1 : 0;
+  if (with_parent < n_tree) return -1;
+  std::vector<uint8_t> taken(static_cast<size_t>(n), 0);
+  int64_t d = 0;
+  for (uint64_t i = 0; d < n_tree; ++i) {
+    const int64_t v = static_cast<int64_t>(splitmix(s, 3 * i) % static_cast<uint64_t>(n));
+    if (parents[v] < 0 || taken[v]) continue;
+    taken[v] = 1;
+    del_src[d] = parents[v];
+    del_dst[d] = static_cast<int32_t>(v);
+    ++d;
+  }
+  for (int64_t i = 0; i < n_rand; ++i) {
+    const int64_t u = static_cast<int64_t>(splitmix(s, 3 * i + 1) % static_cast<uint64_t>(n));
+    const int64_t r = u / side, c = u % side;
+    int64_t v = -1;
+    for (uint64_t k = 0; v < 0; ++k) {
+      switch (splitmix(s ^ 0x444952ULL, 4 * static_cast<uint64_t>(i) + k) % 4) {
+        case 0: if (c + 1 < side) v = u + 1; break;
+        case 1: if (c > 0) v = u - 1; break;
+        case 2: if (r + 1 < side) v = u + side; break;
+        default: if (r > 0) v = u - side; break;
+      }
+    }
+    del_src[d] = static_cast<int32_t>(u);
+    del_dst[d] = static_cast<int32_t>(v);
+    ++d;
+  }
+  for (int64_t i = 0; i < n_ins; ++i) {
+    uint64_t u, v;
+    uint64_t k = 0;
+    do {
+      u = splitmix(s ^ 0x494E53ULL, 2 * (static_cast<uint64_t>(i) * 8 + k)) % static_cast<uint64_t>(n);
+      v = splitmix(s ^ 0x494E53ULL, 2 * (static_cast<uint64_t>(i) * 8 + k) + 1) % static_cast<uint64_t>(n);
+      ++k;
+    } while (u == v);
+    ins_src[i] = static_cast<int32_t>(u);
+    ins_dst[i] = static_cast<int32_t>(v);
+    ins_w[i] = 1 + static_cast<int32_t>(splitmix(s ^ kWeightStream, static_cast<uint64_t>(i)) % static_cast<uint64_t>(wmax));
+  }
+  return 0;
+}
+
 }  // extern "C"

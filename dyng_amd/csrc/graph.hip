@@ -225,8 +225,8 @@ __global__ void k_len64(const int2* meta, int64_t n, int64_t* out) {
 }
 
 /// Column summary of every objective: pass 0 max and sum, pass 1 the count of maximal weights.
-__global__ void k_column_summary(const int2* meta, const int32_t* w, int64_t n, int64_t slot_cap, int K,
-                                 int pass, apply_control* ctl) {
+__global__ void k_column_summary(const int2* meta, const int32_t* nbr, const int32_t* w, int64_t n, int64_t slot_cap,
+                                 int K, int pass, apply_control* ctl) {
   const int64_t wave = (blockIdx.x * int64_t{blockDim.x} + threadIdx.x) / 64;
   const int64_t waves = (int64_t{gridDim.x} * blockDim.x) / 64;
   for (int k = 0; k < K; ++k) {
@@ -235,6 +235,7 @@ __global__ void k_column_summary(const int2* meta, const int32_t* w, int64_t n, 
     for (int64_t u = wave; u < n; u += waves) {
       const int2 mt = meta[u];
       for (int j = lane(); j < mt.y; j += 64) {
+        if (nbr[mt.x + j] < 0) continue;  // dead
         const long long x = w[k * slot_cap + mt.x + j];
         if (pass == 0) {
           mx = x > mx ? x : mx;
@@ -376,11 +377,13 @@ struct out_pair_params {
   int upsert;        ///< on_existing_insert: 1 upsert, 0 ignore
   const int32_t* ins_w;  ///< batch insertion weights (j * K + k)
   int2* meta;
+  const int32_t* base;
+  int32_t* dead;
   int32_t* nbr;
   int32_t* w;
   int64_t slot_cap;
   int32_t* pend_alive;
-  int32_t* kill;
+  int32_t* kill;    ///< 1: the operation killed a tail entry
   int32_t* pend_w;  ///< k * N + position
   uint8_t* e_inc;
   // in-operations, one per out-operation position
@@ -398,6 +401,22 @@ struct out_pair_params {
 };
 
 enum { kInDelete = 0, kInModify = 1, kInInsert = 2 };
+
+/// The entries of neighbour v in a row's two sorted runs: [lb, ub) in the base, [tl, tu) in the
+/// tail (row-relative indices).
+struct run_pair {
+  int bl, lb, ub, tl, tu;
+};
+
+DYNG_DEV run_pair find_runs(const int32_t* row, int base_len, int len, int32_t v) {
+  run_pair r;
+  r.bl = base_len;
+  r.lb = lower_nbr(row, base_len, v);
+  r.ub = upper_nbr(row, base_len, v);
+  r.tl = base_len + lower_nbr(row + base_len, len - base_len, v);
+  r.tu = base_len + upper_nbr(row + base_len, len - base_len, v);
+  return r;
+}
 
 /// Record where row r's operations start and end in the sorted keys (row = key >> vbits).
 DYNG_DEV void mark_row_range(const u64* keys, int64_t i, int64_t N, int vbits, int32_t* row_a, int32_t* row_b) {
@@ -432,12 +451,22 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
       const int32_t v = static_cast<int32_t>(key & mask);
       const int2 mt = p.meta[u];
       const int32_t* row = p.nbr + mt.x;
-      const int lb = lower_nbr(row, mt.y, v);
-      const int ub = upper_nbr(row, mt.y, v);
+      const run_pair rr = find_runs(row, u < p.n_before ? p.base[u] : 0, mt.y, v);
+      // the first alive (u,v) of the row in order: the base run's, then the tail's
+      auto first_alive = [&]() -> int {
+        for (int x = rr.lb; x < rr.ub; ++x) {
+          if (row[x] >= 0) return x;
+        }
+        for (int x = rr.tl; x < rr.tu; ++x) {
+          if (row[x] >= 0) return x;
+        }
+        return -1;
+      };
       int32_t first_init[max_weights];
-      const bool had = lb < ub && u < p.n_before;
+      const int init_hit = u < p.n_before ? first_alive() : -1;
+      const bool had = init_hit >= 0;
       if (had) {
-        for (int k = 0; k < K; ++k) first_init[k] = p.w[k * p.slot_cap + mt.x + lb];
+        for (int k = 0; k < K; ++k) first_init[k] = p.w[k * p.slot_cap + mt.x + init_hit];
       }
       bool structural = false;
       auto account = [&](int k, long long wold, long long wnew, bool removed, bool added) {
@@ -460,11 +489,8 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
         p.ikeys[q] = kNoKey;
         p.ivals[q] = static_cast<uint32_t>(q);
         const u64 ikey = (static_cast<u64>(v) << p.vbits) | static_cast<u64>(u);
-        // first alive match: old entries of the run, then pending entries of this pair
-        int old_hit = -1;
-        for (int x = lb; x < ub; ++x) {
-          if (row[x] >= 0) { old_hit = x; break; }
-        }
+        // first alive match: old entries of the runs, then pending entries of this pair
+        const int old_hit = first_alive();
         int64_t pend_hit = -1;
         if (old_hit < 0) {
           for (int64_t x = i; x < q; ++x) {
@@ -474,7 +500,11 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
         if (del) {
           if (old_hit >= 0) {
             p.nbr[mt.x + old_hit] = v | kDead;
-            p.kill[q] = 1;
+            if (old_hit < rr.bl) {
+              atomicAdd(&p.dead[u], 1);
+            } else {
+              p.kill[q] = 1;  // a tail entry
+            }
             for (int k = 0; k < K; ++k) {
               const int32_t wo = p.w[k * p.slot_cap + mt.x + old_hit];
               p.iop_wold[k * p.N + q] = wo;
@@ -535,8 +565,10 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
       if (had) {
         const int32_t* wf = nullptr;
         int64_t wf_stride = 0;
-        for (int x = lb; x < ub; ++x) {
-          if (row[x] >= 0) { wf = p.w + mt.x + x; wf_stride = p.slot_cap; break; }
+        const int final_hit = first_alive();
+        if (final_hit >= 0) {
+          wf = p.w + mt.x + final_hit;
+          wf_stride = p.slot_cap;
         }
         if (wf == nullptr) {
           for (int64_t x = i; x < j; ++x) {
@@ -555,13 +587,8 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
         }
       }
       if (structural && atomicExch(&p.mark[u], p.mark_gen) != p.mark_gen) {
-        if (mt.y > kMergeSmall) {
-          const int t = atomicAdd(&p.ctl->touched_out_big, 1);
-          p.touched[p.N - t] = u;  // long rows fill the list from its end
-        } else {
-          const int t = atomicAdd(&p.ctl->touched_out, 1);
-          p.touched[t] = u;
-        }
+        const int t = atomicAdd(&p.ctl->touched_out, 1);
+        p.touched[t] = u;
       }
     }
   }
@@ -597,6 +624,8 @@ struct in_pair_params {
   const int32_t* iop_wold;
   const int32_t* iop_wnew;
   int2* meta;
+  const int32_t* base;
+  int32_t* dead;
   int32_t* nbr;
   int32_t* w;
   int64_t slot_cap;
@@ -633,8 +662,7 @@ __global__ void __launch_bounds__(kBlock) k_in_pairs(in_pair_params p) {
   const int32_t u = static_cast<int32_t>(key & mask);
   const int2 mt = p.meta[v];
   const int32_t* row = p.nbr + mt.x;
-  const int lb = lower_nbr(row, mt.y, u);
-  const int ub = upper_nbr(row, mt.y, u);
+  const run_pair rr = find_runs(row, p.base[v], mt.y, u);
   const int K = p.K;
   bool structural = false;
   for (int64_t q = i; q < j; ++q) {
@@ -648,9 +676,11 @@ __global__ void __launch_bounds__(kBlock) k_in_pairs(in_pair_params p) {
       structural = true;
       continue;
     }
-    // first alive entry of u with the old weights: old entries, then pending ones
+    // first alive entry of u with the old weights: the base run's, the tail's, then pending ones
     int old_hit = -1;
-    for (int x = lb; x < ub && old_hit < 0; ++x) {
+    for (int x = rr.lb; x < rr.tu && old_hit < 0; ++x) {
+      if (x == rr.ub) x = rr.tl;
+      if (x >= rr.tu) break;
       if (row[x] < 0) continue;
       bool same = true;
       for (int k = 0; k < K && same; ++k) same = p.w[k * p.slot_cap + mt.x + x] == p.iop_wold[k * p.N + o];
@@ -672,7 +702,11 @@ __global__ void __launch_bounds__(kBlock) k_in_pairs(in_pair_params p) {
     if (type == kInDelete) {
       if (old_hit >= 0) {
         p.nbr[mt.x + old_hit] = u | kDead;
-        p.kill[q] = 1;
+        if (old_hit < rr.bl) {
+          atomicAdd(&p.dead[v], 1);
+        } else {
+          p.kill[q] = 1;  // a tail entry
+        }
         structural = true;
       } else {
         p.pend_alive[pend_hit] = 0;
@@ -685,19 +719,16 @@ __global__ void __launch_bounds__(kBlock) k_in_pairs(in_pair_params p) {
     }
   }
   if (structural && atomicExch(&p.mark[v], p.mark_gen) != p.mark_gen) {
-    if (mt.y > kMergeSmall) {
-      const int t = atomicAdd(&p.ctl->touched_in_big, 1);
-      p.touched[p.N - t] = v;  // long rows fill the list from its end
-    } else {
-      const int t = atomicAdd(&p.ctl->touched_in, 1);
-      p.touched[t] = v;
-    }
+    const int t = atomicAdd(&p.ctl->touched_in, 1);
+    p.touched[t] = v;
   }
 }
 
 struct merge_params {
   int2* meta;
   int32_t* cap;
+  int32_t* base;
+  int32_t* dead;
   int32_t* nbr;
   int32_t* w;
   int64_t slot_cap;
@@ -707,14 +738,14 @@ struct merge_params {
   int vbits;
   const int32_t* pend_alive;
   const int32_t* pend_prefix;
-  const int32_t* kill_prefix;
-  const int32_t* kill;
+  const int32_t* kill_prefix;  ///< prefix of the tail entries each operation killed
   const int32_t* pend_w;
   const int32_t* touched;
-  const int* touched_count;     ///< rows of at most kMergeSmall entries: touched[0, count)
-  const int* touched_big;       ///< longer rows: touched[N - t] for t in [0, count)
-  const int32_t* row_a;         ///< the row's operations: keys[row_a[u], row_b[u])
+  const int* touched_count;
+  const int32_t* row_a;  ///< the row's operations: keys[row_a[u], row_b[u])
   const int32_t* row_b;
+  int32_t* lists;        ///< the merge plan: kModes lists of rows, N + 1 entries each
+  int* list_count;       ///< rows per list
   int32_t* stage_nbr;
   int32_t* stage_w;
   int64_t stage_cap;
@@ -722,417 +753,404 @@ struct merge_params {
   long long* tail;
   long long* garbage;
   unsigned long long* relocated;
+  long long* need_reloc;
+  long long* need_stage;
   int* error_bits;
   double headroom;
 };
 
 constexpr int kMergeBlock = 1024;  ///< threads of a merge workgroup
-constexpr int kMergeSeg = 2048;    ///< operations of one row staged in LDS for the merge
-constexpr int kMergeE = 8;         ///< row entries per thread per merge step
 constexpr int kMergeWaves = kMergeBlock / 64;
-constexpr int kRegChunks = kMergeSmall / 64;  ///< a short row held in registers, 64 entries per chunk
-constexpr int kRegWeights = 2;                ///< weight columns a short row can hold in registers
-static_assert(kMergeE * kMergeWaves == 128, "the step prefix takes two counts per lane of one wave");
-static_assert(kMergeSmall % 64 == 0, "short rows are whole chunks of 64");
+constexpr int kMergeSeg = 2048;    ///< operations of one row a workgroup stages in LDS
+constexpr int kRegChunks = kMergeSmall / 64;  ///< a short tail held in registers, 64 entries per chunk
+constexpr int kRegWeights = 2;                ///< weight columns a tail merge holds in registers
+constexpr int kTailMin = 64;                  ///< a tail may always grow to this many entries
+constexpr int kTailMax = 4096;                ///< and never past this (a workgroup's registers)
+constexpr int kTailPerThread = kTailMax / kMergeBlock;
+static_assert(kMergeSmall % 64 == 0, "short tails are whole chunks of 64");
+static_assert(kTailMax % kMergeBlock == 0, "a long tail is whole slabs of a workgroup");
 
-/// First index in [0, len) of an LDS array whose value is >= v.
-DYNG_DEV int lds_lower(const int32_t* a, int len, int32_t v) {
+/// How a touched row is merged: its tail rewritten by one wave (short tail, few operations) or by
+/// a workgroup, or the whole row rewritten as one base run.
+enum merge_mode { kTailWave = 0, kTailBlock = 1, kFull = 2, kModes = 3 };
+
+DYNG_DEV int tail_limit(int base_len) { return min(kTailMax, max(kTailMin, base_len >> 3)); }
+DYNG_DEV int dead_limit(int base_len) { return max(16, base_len >> 2); }
+DYNG_DEV int grown_cap(int len, double headroom) { return len + static_cast<int>(ceil(len * headroom)) + 2; }
+
+/// First index in [0, len) of an LDS array whose value is >= v (strict: > v when `upper`).
+DYNG_DEV int lds_bound(const int32_t* a, int len, int32_t v, bool upper) {
   int lo = 0, hi = len;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1; else hi = mid;
+    if (a[mid] < v || (upper && a[mid] == v)) lo = mid + 1; else hi = mid;
   }
   return lo;
 }
 
-/// Whether a short row is merged in registers, in place (no staging): its entries fit the
-/// registers and its operations one per lane.
-DYNG_DEV bool merge_in_registers(int len, int nops, int K) {
-  return len <= kMergeSmall && nops <= 64 && K <= kRegWeights;
-}
-
-/// The row's length after the batch and whether it still fits its capacity.
+/// The shape of a touched row's merge.
 struct row_plan {
-  int new_len;
-  bool fits;
+  int mode;
+  int base_len, tail_len, new_tail;
+  int alive_base, alive_tail, pending;
+  int new_len;  ///< full merge: the rewritten row's length
 };
 
 DYNG_DEV row_plan plan_row(const merge_params& p, int32_t u, int2 mt, int64_t a, int64_t b) {
-  const int P = p.pend_prefix[b] - p.pend_prefix[a];
-  const int Kc = p.kill_prefix[b] - p.kill_prefix[a];
-  const int new_len = mt.y - Kc + P;
-  return {new_len, new_len <= p.cap[u]};
+  row_plan r;
+  r.base_len = p.base[u];
+  r.tail_len = mt.y - r.base_len;
+  r.pending = p.pend_prefix[b] - p.pend_prefix[a];
+  r.alive_tail = r.tail_len - (p.kill_prefix[b] - p.kill_prefix[a]);
+  r.alive_base = r.base_len - p.dead[u];
+  r.new_tail = r.alive_tail + r.pending;
+  r.new_len = r.alive_base + r.new_tail;
+  const int nops = static_cast<int>(b - a);
+  const bool tail_ok = r.new_tail <= tail_limit(r.base_len) && r.base_len + r.new_tail <= p.cap[u] &&
+                       p.dead[u] <= dead_limit(r.base_len) && p.K <= kRegWeights;
+  if (tail_ok && r.tail_len <= kMergeSmall && nops <= 64) {
+    r.mode = kTailWave;
+  } else if (tail_ok && nops <= kMergeSeg) {
+    r.mode = kTailBlock;
+  } else {
+    r.mode = kFull;
+  }
+  return r;
 }
 
-DYNG_DEV int grown_cap(int len, double headroom) { return len + static_cast<int>(ceil(len * headroom)) + 2; }
-
-/// Before the merges: the tail slots the relocating rows will take and the staging slots the
-/// staged rows will take, so the host can compact or grow first and no row is refused.
-__global__ void k_merge_need(merge_params p, long long* reloc_need, long long* stage_need) {
-  const int small = *p.touched_count, big = *p.touched_big;
+/// Before the merges: each touched row's mode (appended to its list) and, for full merges, the
+/// staging slots (its alive entries) and the tail slots (a row that outgrew its capacity), so the
+/// host can compact or grow first and no row is refused.
+__global__ void k_merge_plan(merge_params p) {
+  const int count = *p.touched_count;
   long long reloc = 0, stage = 0;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < small + big; t += gridDim.x * blockDim.x) {
-    const int32_t u = t < small ? p.touched[t] : p.touched[p.N - (t - small)];
-    const int2 mt = p.meta[u];
-    const int64_t a = p.row_a[u], b = p.row_b[u];
-    const row_plan rp = plan_row(p, u, mt, a, b);
-    if (!rp.fits) {
-      reloc += grown_cap(rp.new_len, p.headroom);
-    } else if (!(t < small && merge_in_registers(mt.y, static_cast<int>(b - a), p.K))) {
-      stage += rp.new_len;
+  const int stride = gridDim.x * blockDim.x;
+  // wave-uniform trip count: every lane takes part in the wave-aggregated list appends
+  for (int base = blockIdx.x * blockDim.x + threadIdx.x - lane(); base < count; base += stride) {
+    const int t = base + lane();
+    int mode = -1;
+    int32_t u = -1;
+    if (t < count) {
+      u = p.touched[t];
+      const int2 mt = p.meta[u];
+      const row_plan r = plan_row(p, u, mt, p.row_a[u], p.row_b[u]);
+      mode = r.mode;
+      if (r.mode == kFull) {
+        stage += r.alive_base + r.alive_tail;
+        if (r.new_len > p.cap[u]) reloc += grown_cap(r.new_len, p.headroom);
+      }
+    }
+    for (int m = 0; m < kModes; ++m) {
+      const int slot = wave_append(&p.list_count[m], mode == m);
+      if (mode == m) p.lists[m * (p.N + 1) + slot] = u;
     }
   }
   reloc = static_cast<long long>(wave_sum(static_cast<u64>(reloc)));
   stage = static_cast<long long>(wave_sum(static_cast<u64>(stage)));
   if (lane() == 0) {
-    if (reloc) atomicAdd(reinterpret_cast<unsigned long long*>(reloc_need), static_cast<unsigned long long>(reloc));
-    if (stage) atomicAdd(reinterpret_cast<unsigned long long*>(stage_need), static_cast<unsigned long long>(stage));
+    if (reloc) atomicAdd(reinterpret_cast<unsigned long long*>(p.need_reloc), static_cast<unsigned long long>(reloc));
+    if (stage) atomicAdd(reinterpret_cast<unsigned long long*>(p.need_stage), static_cast<unsigned long long>(stage));
   }
 }
 
-/// Reserve the destination of a row: its own range through the staging arena (in place), or a
-/// new range at the tail when it outgrew its capacity. -1 when the arena is exhausted (never
-/// after the host's check).
-DYNG_DEV long long reserve_row(const merge_params& p, const row_plan& rp, int c, bool& staged, int& newcap) {
-  staged = false;
-  newcap = c;
-  if (rp.fits) {
-    const long long s = static_cast<long long>(
-        atomicAdd(reinterpret_cast<unsigned long long*>(p.stage_used), static_cast<unsigned long long>(rp.new_len)));
-    if (s + rp.new_len <= p.stage_cap) {
-      staged = true;
-      return s;
-    }
-  }
-  const int nc = grown_cap(rp.new_len, p.headroom);
-  newcap = nc;
-  const long long d = static_cast<long long>(
-      atomicAdd(reinterpret_cast<unsigned long long*>(p.tail), static_cast<unsigned long long>(nc)));
-  if (d + nc > p.slot_cap) {
-    atomicOr(p.error_bits, static_cast<int>(DYNG_HIP_DEVICE_ERROR_CAPACITY));
-    return -1;
-  }
-  atomicAdd(reinterpret_cast<unsigned long long*>(p.garbage), static_cast<unsigned long long>(c));
-  atomicAdd(p.relocated, 1ULL);
-  return d;
-}
-
-/// One wave per touched row of at most kMergeSmall entries (most rows).
-///
-/// The usual case (at most 64 operations, at most kRegWeights weight columns) reads the whole row
-/// into registers first, so it writes the merged row straight into its own slots (or its new
-/// range) with no staging: an alive entry's place is the alive entries before it plus the
-/// pending entries with a smaller neighbour (one compare per operation, the operations one per
-/// lane); a pending entry's place is the row entries up to its neighbour (ballots over the
-/// registers) minus the deletions of its run plus the pending entries before it. Other short rows
-/// take the staged path of k_merge_rows, wave-sized.
-__global__ void __launch_bounds__(kMergeBlock) k_merge_small(merge_params p) {
-  const int count = *p.touched_count;
+/// One wave per row whose tail is short (at most kMergeSmall entries, 64 operations): the tail
+/// is read into registers and the merged tail (its alive entries and the pending ones, by
+/// neighbour id, old entries first among equal ones) written back in place. An alive entry's
+/// place is the alive entries before it plus the pending entries with a smaller neighbour (the
+/// operations one per lane); a pending entry's place is the alive entries up to its neighbour
+/// (ballots over the registers) plus the pending entries before it.
+__global__ void __launch_bounds__(kMergeBlock) k_tail_wave(merge_params p) {
+  const int count = p.list_count[kTailWave];
+  const int32_t* list = p.lists + kTailWave * (p.N + 1);
   const u64 mask = (1ULL << p.vbits) - 1;
   const int wave = static_cast<int>((blockIdx.x * int64_t{blockDim.x} + threadIdx.x) / 64);
   const int waves = static_cast<int>((int64_t{gridDim.x} * blockDim.x) / 64);
   const int K = p.K;
   for (int t = wave; t < count; t += waves) {
-    const int32_t u = p.touched[t];
+    const int32_t u = list[t];
     const int2 mt = p.meta[u];
     const int64_t a = p.row_a[u], b = p.row_b[u];
     const int nops = static_cast<int>(b - a);
-    const int pa = p.pend_prefix[a];
-    const int ka = p.kill_prefix[a];
-    const row_plan rp = plan_row(p, u, mt, a, b);
-    if (merge_in_registers(mt.y, nops, K)) {
-      // ---- registers: the row, then the operations one per lane ----
-      int32_t x[kRegChunks];
-      int32_t wr[kRegWeights][kRegChunks];
+    const int bl = p.base[u];
+    const int tl = mt.y - bl;
+    const int64_t at = mt.x + int64_t{bl};  // the tail's first slot
+    int32_t x[kRegChunks];
+    int32_t wr[kRegWeights][kRegChunks];
 #pragma unroll
-      for (int c = 0; c < kRegChunks; ++c) {
-        const int jx = c * 64 + lane();
-        const bool in = jx < mt.y;
-        x[c] = in ? p.nbr[mt.x + jx] : -1;
+    for (int c = 0; c < kRegChunks; ++c) {
+      const int jx = c * 64 + lane();
+      const bool in = jx < tl;
+      x[c] = in ? p.nbr[at + jx] : -1;
 #pragma unroll
-        for (int k = 0; k < kRegWeights; ++k) wr[k][c] = (in && k < K) ? p.w[k * p.slot_cap + mt.x + jx] : 0;
-      }
-      const bool has = lane() < nops;
-      const int32_t ov = has ? static_cast<int32_t>(p.keys[a + lane()] & mask) : INT32_MAX;
-      const bool opend = has && p.pend_alive[a + lane()] != 0;
-      const bool okill = has && p.kill[a + lane()] != 0;
-      // the destination (every lane reads the same row; lane 0 reserves)
-      long long dest = mt.x;
-      int newcap = p.cap[u];
-      int relocated = 0;
-      if (!rp.fits) {
-        if (lane() == 0) {
-          bool staged = false;
-          dest = reserve_row(p, rp, newcap, staged, newcap);
-        }
-        dest = __shfl(dest, 0);
-        newcap = __shfl(newcap, 0);
-        relocated = 1;
-        if (dest < 0) continue;  // uniform
-      }
-      // pending entries before each alive entry
-      int before[kRegChunks];
-#pragma unroll
-      for (int c = 0; c < kRegChunks; ++c) before[c] = 0;
-      for (int j = 0; j < nops; ++j) {
-        const int32_t vj = __shfl(ov, j);
-        const bool pj = __shfl(static_cast<int>(opend), j) != 0;
-        if (!pj) continue;  // uniform
-#pragma unroll
-        for (int c = 0; c < kRegChunks; ++c) before[c] += (x[c] >= 0 && vj < x[c]) ? 1 : 0;
-      }
-      // pending entries: their places, computed before any store
-      int64_t ppos = -1;
-      const u64 pend_mask = __ballot(opend);
-      for (int j = 0; j < nops; ++j) {
-        if (!((pend_mask >> j) & 1)) continue;  // uniform
-        const int32_t vj = __shfl(ov, j);
-        int y = 0;
-#pragma unroll
-        for (int c = 0; c < kRegChunks; ++c) {
-          const int jx = c * 64 + lane();
-          y += __popcll(__ballot(jx < mt.y && masked(x[c]) <= vj));
-        }
-        const int killed = __popcll(__ballot(okill && ov <= vj));
-        const int pend_before = __popcll(pend_mask & ((1ULL << j) - 1));
-        if (lane() == j) ppos = dest + (y - killed) + pend_before;
-      }
-      // stores: alive entries, then pending ones
-      int carry = 0;
-#pragma unroll
-      for (int c = 0; c < kRegChunks; ++c) {
-        const bool alive = x[c] >= 0;
-        const u64 ballot = __ballot(alive);
-        if (alive) {
-          const int64_t pos = dest + carry + __popcll(ballot & lanes_below()) + before[c];
-          p.nbr[pos] = x[c];
-#pragma unroll
-          for (int k = 0; k < kRegWeights; ++k) {
-            if (k < K) p.w[k * p.slot_cap + pos] = wr[k][c];
-          }
-        }
-        carry += __popcll(ballot);
-      }
-      if (opend) {
-        p.nbr[ppos] = ov;
-        for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + ppos] = p.pend_w[k * p.N + a + lane()];
-      }
-      if (lane() == 0) {
-        p.meta[u] = make_int2(static_cast<int32_t>(dest), rp.new_len);
-        if (relocated) p.cap[u] = newcap;
-      }
-      continue;
+      for (int k = 0; k < kRegWeights; ++k) wr[k][c] = (in && k < K) ? p.w[k * p.slot_cap + at + jx] : 0;
     }
-    // ---- staged: the row's slots are rewritten through the staging arena ----
-    long long dest = 0;
-    int staged = 0, newcap = 0;
-    if (lane() == 0) {
-      bool st_ = false;
-      dest = reserve_row(p, rp, p.cap[u], st_, newcap);
-      staged = st_ ? 1 : 0;
+    const bool has = lane() < nops;
+    const int32_t ov = has ? static_cast<int32_t>(p.keys[a + lane()] & mask) : INT32_MAX;
+    const bool opend = has && p.pend_alive[a + lane()] != 0;
+    int before[kRegChunks];
+#pragma unroll
+    for (int c = 0; c < kRegChunks; ++c) before[c] = 0;
+    const u64 pend_mask = __ballot(opend);
+    for (int j = 0; j < nops; ++j) {
+      if (!((pend_mask >> j) & 1)) continue;  // uniform
+      const int32_t vj = __shfl(ov, j);
+#pragma unroll
+      for (int c = 0; c < kRegChunks; ++c) before[c] += (x[c] >= 0 && vj < x[c]) ? 1 : 0;
     }
-    dest = __shfl(dest, 0);
-    staged = __shfl(staged, 0);
-    newcap = __shfl(newcap, 0);
-    if (dest < 0) continue;  // uniform over the wave
-    int32_t* tn = staged ? p.stage_nbr : p.nbr;
-    int32_t* tw = staged ? p.stage_w : p.w;
-    const int64_t tstride = staged ? p.stage_cap : p.slot_cap;
+    // pending entries: their places, computed before any store
+    int64_t ppos = -1;
+    for (int j = 0; j < nops; ++j) {
+      if (!((pend_mask >> j) & 1)) continue;  // uniform
+      const int32_t vj = __shfl(ov, j);
+      int y = 0;
+#pragma unroll
+      for (int c = 0; c < kRegChunks; ++c) y += __popcll(__ballot(x[c] >= 0 && x[c] <= vj));
+      if (lane() == j) ppos = at + y + __popcll(pend_mask & ((1ULL << j) - 1));
+    }
     int carry = 0;
-    for (int base = 0; base < mt.y; base += 64) {
-      const int jx = base + lane();
-      const int32_t x = jx < mt.y ? p.nbr[mt.x + jx] : -1;
-      const bool alive = x >= 0;
+#pragma unroll
+    for (int c = 0; c < kRegChunks; ++c) {
+      const bool alive = x[c] >= 0;
       const u64 ballot = __ballot(alive);
       if (alive) {
-        const int64_t lt = lower_key(p.keys, a, b, (static_cast<u64>(u) << p.vbits) | static_cast<u64>(x));
-        const int64_t pos = dest + carry + __popcll(ballot & lanes_below()) + (p.pend_prefix[lt] - pa);
-        tn[pos] = x;
-        for (int k = 0; k < K; ++k) tw[k * tstride + pos] = p.w[k * p.slot_cap + mt.x + jx];
+        const int64_t pos = at + carry + __popcll(ballot & lanes_below()) + before[c];
+        p.nbr[pos] = x[c];
+#pragma unroll
+        for (int k = 0; k < kRegWeights; ++k) {
+          if (k < K) p.w[k * p.slot_cap + pos] = wr[k][c];
+        }
       }
       carry += __popcll(ballot);
     }
-    for (int64_t q = a + lane(); q < b; q += 64) {
-      if (!p.pend_alive[q]) continue;
-      const u64 key = p.keys[q];
-      const int32_t v = static_cast<int32_t>(key & mask);
-      const int y = upper_nbr(p.nbr + mt.x, mt.y, v);
-      const int64_t se = upper_key(p.keys, q, b, key);
-      const int killed = p.kill_prefix[se] - ka;
-      const int64_t pos = dest + (y - killed) + (p.pend_prefix[q] - pa);
-      tn[pos] = v;
-      for (int k = 0; k < K; ++k) tw[k * tstride + pos] = p.pend_w[k * p.N + q];
+    if (opend) {
+      p.nbr[ppos] = ov;
+      for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + ppos] = p.pend_w[k * p.N + a + lane()];
     }
-    const int new_len = carry + (p.pend_prefix[b] - pa);
-    if (staged) {
-      __threadfence_block();  // the wave's staging stores before its reads of them
-      for (int x = lane(); x < new_len; x += 64) {
-        p.nbr[mt.x + x] = p.stage_nbr[dest + x];
-        for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + mt.x + x] = p.stage_w[k * p.stage_cap + dest + x];
-      }
-    }
-    if (lane() == 0) {
-      p.meta[u] = make_int2(staged ? mt.x : static_cast<int32_t>(dest), new_len);
-      if (!staged) p.cap[u] = newcap;
-    }
+    if (lane() == 0) p.meta[u] = make_int2(mt.x, bl + carry + __popcll(pend_mask));
   }
 }
 
-/// One workgroup per touched row longer than kMergeSmall: the alive entries and the pending
-/// ones, merged by neighbour id (existing entries before new ones of the same neighbour),
-/// written in place through the staging arena, or to a new slot range at the tail when the row
-/// outgrew its capacity.
-///
-/// The row's operations (a sorted key segment) are staged in LDS with the number of pending
-/// entries before each, so an alive entry finds its place with a binary search in LDS. The row
-/// is read kMergeE slabs of 1024 entries per step, the loads of a step issued together; the
-/// alive entries before each slab and wave come from one LDS scan per step.
-__global__ void __launch_bounds__(kMergeBlock) k_merge_rows(merge_params p) {
-  __shared__ int s_tot[kMergeE * kMergeWaves];  // alive entries per (slab, wave), then their prefix
-  __shared__ int s_step_total;
-  __shared__ int32_t s_v[kMergeSeg];       // neighbour of each operation of the row
+/// One workgroup per row whose tail needs more than a wave (up to kTailMax entries, kMergeSeg
+/// operations): the tail in registers (kTailPerThread per thread), the operations and the
+/// alive tail in LDS, the merged tail written back in place.
+__global__ void __launch_bounds__(kMergeBlock) k_tail_block(merge_params p) {
+  __shared__ int32_t s_v[kMergeSeg];       // neighbour of each operation
   __shared__ int32_t s_pp[kMergeSeg + 1];  // pending entries before each operation
-  __shared__ long long s_dest;
-  __shared__ int s_staged, s_newcap;
-  const int count = *p.touched_big;
+  __shared__ int32_t s_t[kTailMax];        // the alive tail entries, compacted
+  __shared__ int s_tot[kTailPerThread * kMergeWaves + 1];
+  const int count = p.list_count[kTailBlock];
+  const int32_t* list = p.lists + kTailBlock * (p.N + 1);
   const u64 mask = (1ULL << p.vbits) - 1;
   const int tid = static_cast<int>(threadIdx.x);
   const int wid = tid / 64;
+  const int K = p.K;
   for (int t = blockIdx.x; t < count; t += gridDim.x) {
-    const int32_t u = p.touched[p.N - t];
+    const int32_t u = list[t];
     const int2 mt = p.meta[u];
     const int64_t a = p.row_a[u], b = p.row_b[u];
+    const int nops = static_cast<int>(b - a);
+    const int bl = p.base[u];
+    const int tl = mt.y - bl;
+    const int64_t at = mt.x + int64_t{bl};
+    const int pa = p.pend_prefix[a];
+    for (int j = tid; j < nops; j += kMergeBlock) {
+      s_v[j] = static_cast<int32_t>(p.keys[a + j] & mask);
+      s_pp[j] = p.pend_prefix[a + j] - pa;
+    }
+    if (tid == 0) s_pp[nops] = p.pend_prefix[b] - pa;
+    int32_t x[kTailPerThread];
+    int32_t wr[kRegWeights][kTailPerThread];
+    int rank[kTailPerThread];
+#pragma unroll
+    for (int e = 0; e < kTailPerThread; ++e) {
+      const int jx = e * kMergeBlock + tid;
+      const bool in = jx < tl;
+      x[e] = in ? p.nbr[at + jx] : -1;
+#pragma unroll
+      for (int k = 0; k < kRegWeights; ++k) wr[k][e] = (in && k < K) ? p.w[k * p.slot_cap + at + jx] : 0;
+      const u64 ballot = __ballot(x[e] >= 0);
+      rank[e] = __popcll(ballot & lanes_below());
+      if (lane() == 0) s_tot[e * kMergeWaves + wid] = __popcll(ballot);
+    }
+    __syncthreads();
     if (tid == 0) {
-      const row_plan rp = plan_row(p, u, mt, a, b);
-      bool staged = false;
-      int newcap = 0;
-      s_dest = reserve_row(p, rp, p.cap[u], staged, newcap);
-      s_staged = staged ? 1 : 0;
-      s_newcap = newcap;
+      int run = 0;
+      for (int i = 0; i < kTailPerThread * kMergeWaves; ++i) {
+        const int c = s_tot[i];
+        s_tot[i] = run;
+        run += c;
+      }
+      s_tot[kTailPerThread * kMergeWaves] = run;
+    }
+    __syncthreads();
+    const int alive_tail = s_tot[kTailPerThread * kMergeWaves];
+    int apos[kTailPerThread];
+#pragma unroll
+    for (int e = 0; e < kTailPerThread; ++e) {
+      apos[e] = s_tot[e * kMergeWaves + wid] + rank[e];
+      if (x[e] >= 0) s_t[apos[e]] = x[e];
+    }
+    __syncthreads();
+    // pending entries: the alive tail entries up to their neighbour, the pending ones before them
+    for (int j = tid; j < nops; j += kMergeBlock) {
+      if (s_pp[j + 1] == s_pp[j]) continue;  // not pending
+      const int32_t v = s_v[j];
+      const int64_t pos = at + lds_bound(s_t, alive_tail, v, true) + s_pp[j];
+      p.nbr[pos] = v;
+      for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + pos] = p.pend_w[k * p.N + a + j];
+    }
+#pragma unroll
+    for (int e = 0; e < kTailPerThread; ++e) {
+      if (x[e] < 0) continue;
+      const int64_t pos = at + apos[e] + s_pp[lds_bound(s_v, nops, x[e], false)];
+      p.nbr[pos] = x[e];
+#pragma unroll
+      for (int k = 0; k < kRegWeights; ++k) {
+        if (k < K) p.w[k * p.slot_cap + pos] = wr[k][e];
+      }
+    }
+    if (tid == 0) p.meta[u] = make_int2(mt.x, bl + alive_tail + s_pp[nops]);
+    __syncthreads();
+  }
+}
+
+/// Count of the entries of a sorted run in global memory that are < v (or <= v when `upper`).
+DYNG_DEV int global_bound(const int32_t* a, int len, int32_t v, bool upper) {
+  int lo = 0, hi = len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int32_t x = a[mid];
+    if (x < v || (upper && x == v)) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+/// Compact the alive entries of [src, src + len) to dst (a workgroup; returns the count).
+DYNG_DEV int compact_alive(const merge_params& p, int64_t src, int len, int64_t dst, int* s_tot) {
+  const int tid = static_cast<int>(threadIdx.x);
+  const int wid = tid / 64;
+  int carry = 0;
+  for (int base = 0; base < len; base += kMergeBlock) {
+    const int jx = base + tid;
+    const int32_t x = jx < len ? p.nbr[src + jx] : -1;
+    const u64 ballot = __ballot(x >= 0);
+    if (lane() == 0) s_tot[wid] = __popcll(ballot);
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int q = 0; q < kMergeWaves; ++q) {
+      before += q < wid ? s_tot[q] : 0;
+      total += s_tot[q];
+    }
+    if (x >= 0) {
+      const int64_t pos = dst + carry + before + __popcll(ballot & lanes_below());
+      p.stage_nbr[pos] = x;
+      for (int k = 0; k < p.K; ++k) p.stage_w[k * p.stage_cap + pos] = p.w[k * p.slot_cap + src + jx];
+    }
+    carry += total;
+    __syncthreads();
+  }
+  return carry;
+}
+
+/// One workgroup per row rewritten whole (its tail or dead entries outgrew their limits, or the
+/// row its capacity): the alive base (A) and the alive tail (B) are compacted into the staging
+/// arena, then A, B and the pending entries (C) are merged into one base run, in place or at the
+/// tail of the arena. An entry's place is its index in its run plus the entries of the other two
+/// runs before it: A before B before C among equal neighbours.
+__global__ void __launch_bounds__(kMergeBlock) k_full_merge(merge_params p) {
+  __shared__ int32_t s_v[kMergeSeg];
+  __shared__ int32_t s_pp[kMergeSeg + 1];
+  __shared__ int32_t s_b[kTailMax];
+  __shared__ int s_tot[kMergeWaves];
+  __shared__ long long s_stage, s_dest;
+  __shared__ int s_newcap;
+  const int count = p.list_count[kFull];
+  const int32_t* list = p.lists + kFull * (p.N + 1);
+  const u64 mask = (1ULL << p.vbits) - 1;
+  const int tid = static_cast<int>(threadIdx.x);
+  const int K = p.K;
+  for (int t = blockIdx.x; t < count; t += gridDim.x) {
+    const int32_t u = list[t];
+    const int2 mt = p.meta[u];
+    const int64_t a = p.row_a[u], b = p.row_b[u];
+    const int nops = static_cast<int>(b - a);
+    const row_plan r = plan_row(p, u, mt, a, b);
+    if (tid == 0) {
+      s_stage = static_cast<long long>(atomicAdd(reinterpret_cast<unsigned long long*>(p.stage_used),
+                                                 static_cast<unsigned long long>(r.alive_base + r.alive_tail)));
+      s_newcap = p.cap[u];
+      s_dest = mt.x;
+      if (s_stage + r.alive_base + r.alive_tail > p.stage_cap) {
+        atomicOr(p.error_bits, static_cast<int>(DYNG_HIP_DEVICE_ERROR_CAPACITY));
+        s_dest = -1;
+      } else if (r.new_len > p.cap[u]) {
+        const int nc = grown_cap(r.new_len, p.headroom);
+        const long long d = static_cast<long long>(
+            atomicAdd(reinterpret_cast<unsigned long long*>(p.tail), static_cast<unsigned long long>(nc)));
+        if (d + nc > p.slot_cap) {
+          atomicOr(p.error_bits, static_cast<int>(DYNG_HIP_DEVICE_ERROR_CAPACITY));
+          s_dest = -1;
+        } else {
+          atomicAdd(reinterpret_cast<unsigned long long*>(p.garbage), static_cast<unsigned long long>(p.cap[u]));
+          atomicAdd(p.relocated, 1ULL);
+          s_dest = d;
+          s_newcap = nc;
+        }
+      }
     }
     __syncthreads();
     if (s_dest < 0) {
       __syncthreads();
       continue;
     }
+    const long long sa = s_stage;
     const long long dest = s_dest;
-    const bool staged = s_staged != 0;
-    int32_t* tn = staged ? p.stage_nbr : p.nbr;
-    int32_t* tw = staged ? p.stage_w : p.w;
-    const int64_t tstride = staged ? p.stage_cap : p.slot_cap;
     const int pa = p.pend_prefix[a];
-    const int ka = p.kill_prefix[a];
-    const int nops = static_cast<int>(b - a);
-    const bool in_lds = nops <= kMergeSeg;
-    if (in_lds) {
+    const bool ops_lds = nops <= kMergeSeg;
+    if (ops_lds) {
       for (int j = tid; j < nops; j += kMergeBlock) {
         s_v[j] = static_cast<int32_t>(p.keys[a + j] & mask);
         s_pp[j] = p.pend_prefix[a + j] - pa;
       }
       if (tid == 0) s_pp[nops] = p.pend_prefix[b] - pa;
     }
-    // alive old entries
-    int carry = 0;
-    for (int base = 0; base < mt.y; base += kMergeBlock * kMergeE) {
-      int32_t x[kMergeE];
-      int32_t w0[kMergeE];
-#pragma unroll
-      for (int e = 0; e < kMergeE; ++e) {
-        const int jx = base + e * kMergeBlock + tid;
-        x[e] = jx < mt.y ? p.nbr[mt.x + jx] : -1;
-        w0[e] = jx < mt.y ? p.w[mt.x + jx] : 0;
-      }
-#pragma unroll
-      for (int e = 0; e < kMergeE; ++e) {
-        const u64 ballot = __ballot(x[e] >= 0);
-        if (lane() == 0) s_tot[e * kMergeWaves + wid] = __popcll(ballot);
-      }
-      __syncthreads();
-      if (wid == 0) {
-        // exclusive prefix of the kMergeE * 16 counts in row order (two per lane)
-        const int l = lane();
-        const int c0 = s_tot[l], c1 = s_tot[l + 64];
-        int i0 = c0, i1 = c1;
-        for (int off = 1; off < 64; off <<= 1) {
-          const int y0 = __shfl_up(i0, off), y1 = __shfl_up(i1, off);
-          if (l >= off) {
-            i0 += y0;
-            i1 += y1;
-          }
-        }
-        const int half = __shfl(i0, 63);
-        s_tot[l] = i0 - c0;
-        s_tot[l + 64] = half + i1 - c1;
-        if (l == 63) s_step_total = half + i1;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int e = 0; e < kMergeE; ++e) {
-        const bool alive = x[e] >= 0;
-        const u64 ballot = __ballot(alive);
-        if (alive) {
-          const int jx = base + e * kMergeBlock + tid;
-          const int pre = carry + s_tot[e * kMergeWaves + wid] + __popcll(ballot & lanes_below());
-          int before;  // pending entries of the row with a smaller neighbour
-          if (in_lds) {
-            before = s_pp[lds_lower(s_v, nops, x[e])];
-          } else {
-            const int64_t lt = lower_key(p.keys, a, b, (static_cast<u64>(u) << p.vbits) | static_cast<u64>(x[e]));
-            before = p.pend_prefix[lt] - pa;
-          }
-          const int64_t pos = dest + pre + before;
-          tn[pos] = x[e];
-          tw[pos] = w0[e];
-          for (int k = 1; k < p.K; ++k) tw[k * tstride + pos] = p.w[k * p.slot_cap + mt.x + jx];
-        }
-      }
-      carry += s_step_total;
-      __syncthreads();
+    const int A = compact_alive(p, mt.x, r.base_len, sa, s_tot);
+    const int B = compact_alive(p, mt.x + int64_t{r.base_len}, r.tail_len, sa + A, s_tot);
+    const int32_t* SA = p.stage_nbr + sa;
+    const int32_t* SB = p.stage_nbr + sa + A;
+    for (int j = tid; j < B; j += kMergeBlock) s_b[j] = SB[j];  // B <= kTailMax
+    __syncthreads();  // the staged runs (and the row's old slots are free to overwrite)
+    // pending entries before v (strictly smaller neighbour)
+    auto pend_before = [&](int32_t v) -> int {
+      if (ops_lds) return s_pp[lds_bound(s_v, nops, v, false)];
+      const int64_t lt = lower_key(p.keys, a, b, (static_cast<u64>(u) << p.vbits) | static_cast<u64>(v));
+      return p.pend_prefix[lt] - pa;
+    };
+    for (int i = tid; i < A; i += kMergeBlock) {
+      const int32_t v = SA[i];
+      const int64_t pos = dest + i + lds_bound(s_b, B, v, false) + pend_before(v);
+      p.nbr[pos] = v;
+      for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + pos] = p.stage_w[k * p.stage_cap + sa + i];
     }
-    // pending entries
+    for (int j = tid; j < B; j += kMergeBlock) {
+      const int32_t v = s_b[j];
+      const int64_t pos = dest + global_bound(SA, A, v, true) + j + pend_before(v);
+      p.nbr[pos] = v;
+      for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + pos] = p.stage_w[k * p.stage_cap + sa + A + j];
+    }
     for (int64_t q = a + tid; q < b; q += kMergeBlock) {
       if (!p.pend_alive[q]) continue;
-      const u64 key = p.keys[q];
-      const int32_t v = static_cast<int32_t>(key & mask);
-      const int y = upper_nbr(p.nbr + mt.x, mt.y, v);
-      const int64_t se = upper_key(p.keys, q, b, key);
-      const int killed = p.kill_prefix[se] - ka;
-      const int64_t pos = dest + (y - killed) + (p.pend_prefix[q] - pa);
-      tn[pos] = v;
-      for (int k = 0; k < p.K; ++k) tw[k * tstride + pos] = p.pend_w[k * p.N + q];
-    }
-    __syncthreads();
-    const int new_len = carry + (p.pend_prefix[b] - pa);
-    if (staged) {
-      for (int base = 0; base < new_len; base += kMergeBlock * kMergeE) {
-        int32_t v[kMergeE];
-#pragma unroll
-        for (int e = 0; e < kMergeE; ++e) {
-          const int x = base + e * kMergeBlock + tid;
-          v[e] = x < new_len ? p.stage_nbr[dest + x] : 0;
-        }
-#pragma unroll
-        for (int e = 0; e < kMergeE; ++e) {
-          const int x = base + e * kMergeBlock + tid;
-          if (x < new_len) p.nbr[mt.x + x] = v[e];
-        }
-        for (int k = 0; k < p.K; ++k) {
-#pragma unroll
-          for (int e = 0; e < kMergeE; ++e) {
-            const int x = base + e * kMergeBlock + tid;
-            v[e] = x < new_len ? p.stage_w[k * p.stage_cap + dest + x] : 0;
-          }
-#pragma unroll
-          for (int e = 0; e < kMergeE; ++e) {
-            const int x = base + e * kMergeBlock + tid;
-            if (x < new_len) p.w[k * p.slot_cap + mt.x + x] = v[e];
-          }
-        }
-      }
+      const int32_t v = static_cast<int32_t>(p.keys[q] & mask);
+      const int64_t pos = dest + global_bound(SA, A, v, true) + lds_bound(s_b, B, v, true) + (p.pend_prefix[q] - pa);
+      p.nbr[pos] = v;
+      for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + pos] = p.pend_w[k * p.N + q];
     }
     if (tid == 0) {
-      p.meta[u] = make_int2(staged ? mt.x : static_cast<int32_t>(dest), new_len);
-      if (!staged) p.cap[u] = s_newcap;
+      const int len = A + B + r.pending;
+      p.meta[u] = make_int2(static_cast<int32_t>(dest), len);
+      p.base[u] = len;
+      p.dead[u] = 0;
+      p.cap[u] = s_newcap;
     }
     __syncthreads();
   }
@@ -1152,6 +1170,14 @@ __global__ void k_compact_rows(const int2* meta, const int32_t* nbr, const int32
       for (int k = 0; k < K; ++k) new_w[k * new_slot_cap + o + j] = w[k * slot_cap + mt.x + j];
     }
     if (lane() == 0) new_meta[u] = make_int2(static_cast<int32_t>(o), mt.y);
+  }
+}
+
+/// A freshly built row is one base run.
+__global__ void k_base_from_len(const int2* meta, int64_t n, int32_t* base, int32_t* dead) {
+  for (int64_t u = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; u < n; u += int64_t{gridDim.x} * blockDim.x) {
+    base[u] = meta[u].y;
+    dead[u] = 0;
   }
 }
 
@@ -1201,8 +1227,12 @@ void build_store(dyng_hip_graph* g, row_store& rs, const u64* keys, const uint32
   exclusive_scan<int32_t>(g->scan_tmp, keep, keep_prefix.p, N + 1, s);
   rs.meta.reserve(n);
   rs.cap.reserve(n);
+  rs.base.reserve(n);
+  rs.dead.reserve(n);
   fill<int2>(rs.meta.p, n, make_int2(0, 0), s);
   fill<int32_t>(rs.cap.p, n, 0, s);
+  fill<int32_t>(rs.base.p, n, 0, s);
+  fill<int32_t>(rs.dead.p, n, 0, s);
   dbuf<int64_t> cap64, off64;
   cap64.reserve(g->n + 1);
   off64.reserve(g->n + 1);
@@ -1225,6 +1255,9 @@ void build_store(dyng_hip_graph* g, row_store& rs, const u64* keys, const uint32
   hipLaunchKernelGGL(k_fill_rows, dim3(blocks_for(N)), dim3(kBlock), 0, s, keys, vals, N, keep,
                      keep_prefix.p, vbits, stride, rs.meta.p, rs.nbr.p, rs.w.p, rs.slot_cap, g->K, w_in,
                      from_slots, in_slot_cap);
+  DYNG_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_base_from_len, dim3(blocks_for(g->n)), dim3(kBlock), 0, s, rs.meta.p, g->n, rs.base.p,
+                     rs.dead.p);
   DYNG_HIP_TRY(hipGetLastError());
 }
 
@@ -1261,12 +1294,13 @@ void recompute_columns(dyng_hip_graph* g, hipStream_t s) {
   apply_control* c = g->host_ctl.p;
   std::memset(c, 0, sizeof(apply_control));
   DYNG_HIP_TRY(hipMemcpyAsync(g->ctl.p, c, sizeof(apply_control), hipMemcpyHostToDevice, s));
-  const unsigned grid = blocks_for(g->n * 64);
-  hipLaunchKernelGGL(k_column_summary, dim3(grid), dim3(kBlock), 0, s, g->out.meta.p, g->out.w.p, g->n,
-                     g->out.slot_cap, g->K, 0, g->ctl.p);
+  // a bounded grid (one reduction per wave, then one atomic per wave and column)
+  const unsigned grid = std::min(blocks_for(g->n * 64), 2048u);
+  hipLaunchKernelGGL(k_column_summary, dim3(grid), dim3(kBlock), 0, s, g->out.meta.p, g->out.nbr.p, g->out.w.p,
+                     g->n, g->out.slot_cap, g->K, 0, g->ctl.p);
   DYNG_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_column_summary, dim3(grid), dim3(kBlock), 0, s, g->out.meta.p, g->out.w.p, g->n,
-                     g->out.slot_cap, g->K, 1, g->ctl.p);
+  hipLaunchKernelGGL(k_column_summary, dim3(grid), dim3(kBlock), 0, s, g->out.meta.p, g->out.nbr.p, g->out.w.p,
+                     g->n, g->out.slot_cap, g->K, 1, g->ctl.p);
   DYNG_HIP_TRY(hipGetLastError());
   DYNG_HIP_TRY(hipMemcpyAsync(c, g->ctl.p, sizeof(apply_control), hipMemcpyDeviceToHost, s));
   DYNG_HIP_TRY(hipStreamSynchronize(s));
@@ -1323,6 +1357,8 @@ void grow_vertices(dyng_hip_graph* g, int64_t n_after, hipStream_t s) {
     for (row_store* rs : {&g->out, &g->in}) {
       rs->meta.reserve(cap, true, s);
       rs->cap.reserve(cap, true, s);
+      rs->base.reserve(cap, true, s);
+      rs->dead.reserve(cap, true, s);
     }
     g->mark_out.reserve(cap, true, s);
     g->mark_in.reserve(cap, true, s);
@@ -1333,6 +1369,8 @@ void grow_vertices(dyng_hip_graph* g, int64_t n_after, hipStream_t s) {
   for (row_store* rs : {&g->out, &g->in}) {
     fill<int2>(rs->meta.p + g->n, n_after - g->n, make_int2(0, 0), s);
     fill<int32_t>(rs->cap.p + g->n, n_after - g->n, 0, s);
+    fill<int32_t>(rs->base.p + g->n, n_after - g->n, 0, s);
+    fill<int32_t>(rs->dead.p + g->n, n_after - g->n, 0, s);
   }
 }
 
@@ -1421,21 +1459,45 @@ dyng_hip_graph* create_from_edge_list(int device, const dyng_hip_graph_props* pr
 
 void copy_rows_to_host(const row_store& rs, int64_t n, int K, int64_t* row_ptr, int32_t* col, int32_t* w) {
   std::vector<int2> meta(static_cast<size_t>(n));
-  if (n > 0) DYNG_HIP_TRY(hipMemcpy(meta.data(), rs.meta.p, n * sizeof(int2), hipMemcpyDeviceToHost));
+  std::vector<int32_t> base(static_cast<size_t>(n));
+  if (n > 0) {
+    DYNG_HIP_TRY(hipMemcpy(meta.data(), rs.meta.p, n * sizeof(int2), hipMemcpyDeviceToHost));
+    DYNG_HIP_TRY(hipMemcpy(base.data(), rs.base.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  }
   std::vector<int32_t> nbr(static_cast<size_t>(rs.slot_cap));
   std::vector<int32_t> wt(static_cast<size_t>(rs.slot_cap) * K);
   if (rs.slot_cap > 0) {
     DYNG_HIP_TRY(hipMemcpy(nbr.data(), rs.nbr.p, rs.slot_cap * sizeof(int32_t), hipMemcpyDeviceToHost));
     DYNG_HIP_TRY(hipMemcpy(wt.data(), rs.w.p, rs.slot_cap * K * sizeof(int32_t), hipMemcpyDeviceToHost));
   }
+  // each row in the reference's order: its base and tail runs merged by neighbour id (base
+  // entries first among equal ones), dead entries dropped
   row_ptr[0] = 0;
-  for (int64_t u = 0; u < n; ++u) row_ptr[u + 1] = row_ptr[u] + meta[u].y;
+  for (int64_t u = 0; u < n; ++u) {
+    int64_t alive = 0;
+    for (int j = 0; j < meta[u].y; ++j) alive += nbr[meta[u].x + j] >= 0 ? 1 : 0;
+    row_ptr[u + 1] = row_ptr[u] + alive;
+  }
   const int64_t m = row_ptr[n];
   for (int64_t u = 0; u < n; ++u) {
-    for (int j = 0; j < meta[u].y; ++j) {
-      const int64_t o = row_ptr[u] + j;
-      col[o] = nbr[meta[u].x + j];
-      for (int k = 0; k < K; ++k) w[k * m + o] = wt[k * rs.slot_cap + meta[u].x + j];
+    const int64_t off = meta[u].x;
+    int i = 0, j = base[u];
+    const int bend = base[u], tend = meta[u].y;
+    int64_t o = row_ptr[u];
+    auto put = [&](int x) {
+      col[o] = nbr[off + x];
+      for (int k = 0; k < K; ++k) w[k * m + o] = wt[k * rs.slot_cap + off + x];
+      ++o;
+    };
+    for (;;) {
+      while (i < bend && nbr[off + i] < 0) ++i;
+      while (j < tend && nbr[off + j] < 0) ++j;
+      if (i == bend && j == tend) break;
+      if (j == tend || (i < bend && nbr[off + i] <= nbr[off + j])) {
+        put(i++);
+      } else {
+        put(j++);
+      }
     }
   }
 }
@@ -1633,6 +1695,8 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     op.upsert = g->props.on_existing_insert == 0 ? 1 : 0;
     op.ins_w = ins_w;
     op.meta = g->out.meta.p;
+    op.base = g->out.base.p;
+    op.dead = g->out.dead.p;
     op.nbr = g->out.nbr.p;
     op.w = g->out.w.p;
     op.slot_cap = g->out.slot_cap;
@@ -1670,6 +1734,8 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     ip.iop_wold = g->iop_wold.p;
     ip.iop_wnew = g->iop_wnew.p;
     ip.meta = g->in.meta.p;
+    ip.base = g->in.base.p;
+    ip.dead = g->in.dead.p;
     ip.nbr = g->in.nbr.p;
     ip.w = g->in.w.p;
     ip.slot_cap = g->in.slot_cap;
@@ -1688,12 +1754,15 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     exclusive_scan<int32_t>(g->scan_tmp, g->ipend_alive.p, g->ipend_prefix.p, N + 1, s);
     exclusive_scan<int32_t>(g->scan_tmp, g->ikill.p, g->ikill_prefix.p, N + 1, s);
     mark_stage(g, 9, s);
-    // ---- plan the merges: the exact tail and staging slots they take ----
+    // ---- plan the merges: each row's mode, and the staging and tail slots they take ----
+    g->merge_lists.reserve(2 * kModes * (N + 1));
     auto merge_setup = [&](bool outs) {
       row_store& rs = outs ? g->out : g->in;
       merge_params mp{};
       mp.meta = rs.meta.p;
       mp.cap = rs.cap.p;
+      mp.base = rs.base.p;
+      mp.dead = rs.dead.p;
       mp.nbr = rs.nbr.p;
       mp.w = rs.w.p;
       mp.slot_cap = rs.slot_cap;
@@ -1704,13 +1773,13 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
       mp.pend_alive = outs ? g->pend_alive.p : g->ipend_alive.p;
       mp.pend_prefix = outs ? g->pend_prefix.p : g->ipend_prefix.p;
       mp.kill_prefix = outs ? g->kill_prefix.p : g->ikill_prefix.p;
-      mp.kill = outs ? g->kill.p : g->ikill.p;
       mp.pend_w = outs ? g->pend_w.p : g->ipend_w.p;
       mp.touched = outs ? g->touched_out.p : g->touched_in.p;
       mp.touched_count = outs ? &dc->touched_out : &dc->touched_in;
-      mp.touched_big = outs ? &dc->touched_out_big : &dc->touched_in_big;
       mp.row_a = outs ? g->row_a_out.p : g->row_a_in.p;
       mp.row_b = outs ? g->row_b_out.p : g->row_b_in.p;
+      mp.lists = g->merge_lists.p + (outs ? 0 : kModes * (N + 1));
+      mp.list_count = outs ? dc->modes_out : dc->modes_in;
       mp.stage_nbr = g->stage_nbr.p;
       mp.stage_w = g->stage_w.p;
       mp.stage_cap = g->stage_cap;
@@ -1718,19 +1787,21 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
       mp.tail = outs ? &dc->tail_out : &dc->tail_in;
       mp.garbage = outs ? &dc->garbage_out : &dc->garbage_in;
       mp.relocated = &dc->relocated_rows;
+      mp.need_reloc = outs ? &dc->need_reloc_out : &dc->need_reloc_in;
+      mp.need_stage = outs ? &dc->need_stage_out : &dc->need_stage_in;
       mp.error_bits = &dc->error_bits;
       mp.headroom = g->props.headroom;
       return mp;
     };
-    const unsigned need_grid = static_cast<unsigned>(std::min<int64_t>(blocks_for(N), 1024));
+    const unsigned plan_grid = static_cast<unsigned>(std::min<int64_t>(blocks_for(N), 1024));
     for (int attempt = 0;; ++attempt) {
       hc->need_reloc_out = hc->need_reloc_in = hc->need_stage_out = hc->need_stage_in = 0;
+      for (int i = 0; i < kModes; ++i) hc->modes_out[i] = hc->modes_in[i] = 0;
       DYNG_HIP_TRY(hipMemcpyAsync(&dc->need_reloc_out, &hc->need_reloc_out, 4 * sizeof(long long),
                                   hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_merge_need, dim3(need_grid), dim3(kBlock), 0, s, merge_setup(true), &dc->need_reloc_out,
-                         &dc->need_stage_out);
-      hipLaunchKernelGGL(k_merge_need, dim3(need_grid), dim3(kBlock), 0, s, merge_setup(false), &dc->need_reloc_in,
-                         &dc->need_stage_in);
+      DYNG_HIP_TRY(hipMemcpyAsync(dc->modes_out, hc->modes_out, 2 * sizeof(hc->modes_out), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_merge_plan, dim3(plan_grid), dim3(kBlock), 0, s, merge_setup(true));
+      hipLaunchKernelGGL(k_merge_plan, dim3(plan_grid), dim3(kBlock), 0, s, merge_setup(false));
       DYNG_HIP_TRY(hipGetLastError());
       DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
       DYNG_HIP_TRY(hipStreamSynchronize(s));
@@ -1761,22 +1832,26 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
       g->stage_w.reserve(g->stage_cap * K);
     }
     mark_stage(g, 10, s);
-    // ---- the merges: short rows one wave each (16 per workgroup), long rows one workgroup each ----
-    const unsigned small_grid = static_cast<unsigned>(std::min<int64_t>((N + 15) / 16, 4096));
-    const unsigned merge_grid = static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>(N, 1), 1024));
+    // ---- the merges, with grids sized from the plan ----
     if (hc->error_bits == 0) {
-      const merge_params mo = merge_setup(true);
-      hipLaunchKernelGGL(k_merge_small, dim3(small_grid), dim3(kMergeBlock), 0, s, mo);
-      DYNG_HIP_TRY(hipGetLastError());
-      hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kMergeBlock), 0, s, mo);
-      DYNG_HIP_TRY(hipGetLastError());
-      mark_stage(g, 11, s);
-      const merge_params mi = merge_setup(false);
-      hipLaunchKernelGGL(k_merge_small, dim3(small_grid), dim3(kMergeBlock), 0, s, mi);
-      DYNG_HIP_TRY(hipGetLastError());
-      hipLaunchKernelGGL(k_merge_rows, dim3(merge_grid), dim3(kMergeBlock), 0, s, mi);
-      DYNG_HIP_TRY(hipGetLastError());
-      mark_stage(g, 12, s);
+      for (int side = 0; side < 2; ++side) {
+        const merge_params mp = merge_setup(side == 0);
+        const int* modes = side == 0 ? hc->modes_out : hc->modes_in;
+        if (modes[kTailWave] > 0) {
+          const int64_t blocks = std::min<int64_t>((modes[kTailWave] + kMergeWaves - 1) / kMergeWaves, 8192);
+          hipLaunchKernelGGL(k_tail_wave, dim3(static_cast<unsigned>(blocks)), dim3(kMergeBlock), 0, s, mp);
+        }
+        if (modes[kTailBlock] > 0) {
+          hipLaunchKernelGGL(k_tail_block, dim3(static_cast<unsigned>(std::min(modes[kTailBlock], 4096))),
+                             dim3(kMergeBlock), 0, s, mp);
+        }
+        if (modes[kFull] > 0) {
+          hipLaunchKernelGGL(k_full_merge, dim3(static_cast<unsigned>(std::min(modes[kFull], 4096))),
+                             dim3(kMergeBlock), 0, s, mp);
+        }
+        DYNG_HIP_TRY(hipGetLastError());
+        mark_stage(g, 11 + side, s);
+      }
     }
   }
   DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));

@@ -110,9 +110,18 @@ struct pinned {
 
 /// One direction of the device graph: per-vertex {offset, length} and capacity, and the slot
 /// arena (neighbour ids, then num_weights weight columns of slot_cap entries each).
+///
+/// A row is two sorted runs in its slot range: the base [offset, offset + base) and the tail
+/// [offset + base, offset + length). A batch deletes by marking entries dead (sign bit of the
+/// neighbour id) and merges its new entries into the tail only; the row is rewritten as one
+/// base when the tail or the dead entries outgrow their limits, or the row its capacity. The
+/// row in the reference's order is the merge of the two runs by neighbour id, base entries first
+/// among equal ones (they are older), without the dead entries.
 struct row_store {
   dbuf<int2> meta;
   dbuf<int32_t> cap;
+  dbuf<int32_t> base;  ///< per vertex: length of the base run
+  dbuf<int32_t> dead;  ///< per vertex: dead entries in the base run
   dbuf<int32_t> nbr;
   dbuf<int32_t> w;
   int64_t slot_cap = 0;  ///< slots of the arena
@@ -126,8 +135,8 @@ struct apply_control {
   unsigned long long dropped_self_loops, ignored_insertions;
   unsigned long long relocated_rows;
   int in_ops;
-  int touched_out, touched_in;          ///< rows of at most kMergeSmall entries
-  int touched_out_big, touched_in_big;  ///< longer rows
+  int touched_out, touched_in;          ///< rows whose entries change
+  int modes_out[3], modes_in[3];        ///< rows per merge mode (wave tail, workgroup tail, full)
   int error_bits;
   long long tail_out, tail_in;        ///< arena tails (atomic bump allocation)
   long long stage_out, stage_in;      ///< staging slots used
@@ -193,6 +202,7 @@ struct dyng_hip_graph {
   dyng_hip::dbuf<int32_t> iop_type, iop_wold, iop_wnew;
   dyng_hip::dbuf<int32_t> touched_out, touched_in, mark_out, mark_in;
   dyng_hip::dbuf<int32_t> row_a_out, row_b_out, row_a_in, row_b_in;  ///< per vertex: its operations
+  dyng_hip::dbuf<int32_t> merge_lists;  ///< the merge plan's row lists, per direction and mode
   dyng_hip::dbuf<int32_t> stage_nbr, stage_w;
   int64_t stage_cap = 0;
   int mark_generation = 0;

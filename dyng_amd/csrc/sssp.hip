@@ -35,6 +35,7 @@ namespace {
 
 constexpr int kBlock = 1024;  ///< 16 waves per workgroup; 8 waves/SIMD fit two per CU, one is launched
 constexpr int kWaves = kBlock / 64;
+constexpr int kWalkLevels = 64;  ///< subtree walk levels before pointer jumping takes over
 
 struct control {
   int ring_near[3];
@@ -116,7 +117,20 @@ struct engine_params {
 /// Append v to a list when pred (wave-aggregated reservation).
 DYNG_DEV void append(int* list, int* counter, bool pred, int32_t v) {
   const int slot = wave_append(counter, pred);
-  if (pred) list[slot] = v;
+  if (pred) st(&list[slot], v);
+}
+
+/// A chunk {vertex, begin, end} of the chunk list, written and read with agent-scope operations.
+DYNG_DEV void st_chunk(int4* ch, int32_t v, int32_t b, int32_t e) {
+  u64* q = reinterpret_cast<u64*>(ch);
+  st(&q[0], static_cast<u64>(static_cast<uint32_t>(v)) | (static_cast<u64>(static_cast<uint32_t>(b)) << 32));
+  st(&q[1], static_cast<u64>(static_cast<uint32_t>(e)));
+}
+DYNG_DEV int4 ld_chunk(const int4* ch) {
+  const u64* q = reinterpret_cast<const u64*>(ch);
+  const u64 a = ld(&q[0]), b = ld(&q[1]);
+  return make_int4(static_cast<int32_t>(static_cast<uint32_t>(a)), static_cast<int32_t>(static_cast<uint32_t>(a >> 32)),
+                   static_cast<int32_t>(static_cast<uint32_t>(b)), 0);
 }
 
 /// Cut the row [begin, begin + len) of vertex v into chunks of `chunk` edges (one wave each).
@@ -130,7 +144,7 @@ DYNG_DEV void emit_chunks(const engine_params& p, int* counter, int32_t v, int32
     }
     const int b = begin + c * p.chunk_edges;
     const int e = min(begin + len, b + p.chunk_edges);
-    p.chunks[base + c] = make_int4(v, b, e, 0);
+    st_chunk(&p.chunks[base + c], v, b, e);
   }
 }
 
@@ -251,7 +265,7 @@ DYNG_DEV void chunk_rows(const engine_params& p, int nchunks, Edge edge) {
   const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / 64);
   const int waves = static_cast<int>((gridDim.x * blockDim.x) / 64);
   for (int c = wave; c < nchunks; c += waves) {
-    const int4 ch = p.chunks[c];
+    const int4 ch = ld_chunk(&p.chunks[c]);
     for (int b = ch.y; b < ch.z; b += 64) {
       const int e = b + lane();
       edge(e < ch.z, ch.x, e);
@@ -259,8 +273,9 @@ DYNG_DEV void chunk_rows(const engine_params& p, int nchunks, Edge edge) {
   }
 }
 
-/// Pull offer of in-edge (u -> v, wt): the candidate word, or INF.
+/// Pull offer of in-edge (u -> v, wt): the candidate word, or INF (also for a dead entry, u < 0).
 DYNG_DEV u64 pull_offer(const engine_params& p, int32_t u, int32_t wt) {
+  if (u < 0) return packed_inf;
   const u64 word = ld(&p.packed[u]);
   if (word == packed_inf) return packed_inf;
   const u64 nd = p.pk.distance(word) + static_cast<u64>(wt);
@@ -291,13 +306,17 @@ DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best
 }  // namespace
 
 /// Leave the kernel when the grid barrier gave up (see grid_barrier); uniform per workgroup.
-#define DYNG_GRID_SYNC()                                                            \
+/// DYNG_GRID_SYNC separates phases whose shared data is only touched with agent-scope
+/// operations; DYNG_GRID_SYNC_FENCED also orders plain loads and stores.
+#define DYNG_GRID_SYNC_AS(fenced)                                                   \
   do {                                                                              \
-    if (!grid_barrier(p.bar, p.barrier_ticks)) {                                    \
+    if (!grid_barrier<fenced>(p.bar, p.barrier_ticks)) {                            \
       if (threadIdx.x == 0) atomicOr(&c->error_bits, DYNG_HIP_DEVICE_ERROR_TIMEOUT); \
       return;                                                                       \
     }                                                                               \
   } while (0)
+#define DYNG_GRID_SYNC() DYNG_GRID_SYNC_AS(false)
+#define DYNG_GRID_SYNC_FENCED() DYNG_GRID_SYNC_AS(true)
 
 __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) {
   __shared__ tile_shared L;
@@ -320,8 +339,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   if (tid == 0) c->t_phase[0] = __builtin_amdgcn_s_memrealtime();
 
   if (p.from_scratch) {
-    for (int v = tid; v < n; v += threads) p.packed[v] = v == p.source ? p.pk.pack(0, -1) : packed_inf;
-    if (tid == 0) p.frontier[0] = p.source;
+    for (int v = tid; v < n; v += threads) st(&p.packed[v], v == p.source ? p.pk.pack(0, -1) : packed_inf);
+    if (tid == 0) st(&p.frontier[0], p.source);
     frontier_count = 1;
     DYNG_GRID_SYNC();
   } else {
@@ -337,8 +356,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           word = p.pk.pack(static_cast<u64>(d), par);
         }
       }
-      p.packed[v] = word;
-      if (p.check_cycles) p.ancestor[v] = par;
+      st(&p.packed[v], word);
+      if (p.check_cycles) st(&p.ancestor[v], par);
     }
     for (int64_t i = tid; i < p.num_del + p.num_ins; i += threads) {
       int32_t u, v;
@@ -363,72 +382,29 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     const int roots = uniform(ld(&c->cand_count));
     const bool work = p.num_del + p.num_ins > 0;
     int inv = roots;  // the invalidated vertices: cand[0, inv)
-    if (work && p.check_cycles) {
-      // ---- B (tree from outside): pointer jumping over compacted lists of vertices still
-      // jumping; a list that is not empty after ceil(log2 n) + 1 rounds is a parent cycle ----
-      int count = n;
-      const int* list = nullptr;
-      int* outs[2] = {p.near_a, p.near_b};
-      for (int r = 0; r < p.max_rounds; ++r) {
-        jump_items += count;
-        int* out = outs[r & 1];
-        int* out_count = &c->ring_jump[r % 3];
-        if (tid == 0) {
-          st(&c->ring_jump[(r + 1) % 3], 0);
-          st(&c->rounds, r + 1);
-        }
-        for (int base = tid - lane(); base < count; base += threads) {
-          const int i = base + lane();
-          bool keep = false, flagged = false;
-          int32_t v = -1;
-          if (i < count) {
-            v = list ? ld(&list[i]) : i;
-            const int32_t a = ld(&p.ancestor[v]);
-            if (a >= 0 && ld(&p.flag[v]) == 0) {
-              if (ld(&p.flag[a]) != 0) {
-                st(&p.flag[v], 1);
-                st(&p.stamp[v], generation + 1);
-                flagged = true;
-              } else {
-                const int32_t next = ld(&p.ancestor[a]);
-                st(&p.ancestor[v], next);
-                keep = next >= 0;
-              }
-            }
-          }
-          append(p.cand, &c->cand_count, flagged, v);
-          append(out, out_count, keep, v);
-        }
-        DYNG_GRID_SYNC();
-        count = uniform(ld(out_count));
-        list = out;
-        if (count == 0) break;
-      }
-      if (count > 0) {
-        if (tid == 0) atomicOr(&c->error_bits, DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE);
-        return;  // uniform
-      }
-      inv = uniform(ld(&c->cand_count));  // nothing appends to it after the last round's barrier
-    } else if (work && roots > 0) {
+    bool jump = work && p.check_cycles;
+    int jump_slot = 0;  // the first ring slot pointer jumping counts in (a slot already zero)
+    if (work && !p.check_cycles && roots > 0) {
       // ---- B (tree from this engine): the subtrees under the roots, level by level down the
       // tree edges (parent[w] == u for an out-edge (u, w)); every tree edge of the old tree is
       // an edge of the graph unless this batch deleted it, and then its head is a root. Level
       // L + 1 is appended right after level L in the candidate list and counted in a ring slot
-      // of its own, so every thread reads the same level bounds ----
+      // of its own, so every thread reads the same level bounds. A walk still going after
+      // kWalkLevels levels (a deep tree) is finished by pointer jumping ----
       int lo = 0, hi = roots, levels = 0;
       auto visit = [&](bool act, int32_t u, int slot, int* level, int* found) {
         bool add = false;
         int32_t w = -1;
         if (act) {
-          w = p.out_nbr[slot];
-          if (p.parents[w] == u && ld(&p.flag[w]) == 0 && atomicExch(&p.flag[w], 1) == 0) {
+          w = p.out_nbr[slot];  // a dead entry (deleted, not yet compacted) is negative
+          if (w >= 0 && p.parents[w] == u && ld(&p.flag[w]) == 0 && atomicExch(&p.flag[w], 1) == 0) {
             st(&p.stamp[w], generation + 1);
             add = true;
           }
         }
         append(level, found, add, w);
       };
-      while (hi > lo) {
+      while (hi > lo && levels < kWalkLevels) {
         jump_items += hi - lo;
         int* found = &c->ring_jump[levels % 3];
         int* chunk_count = &c->ring_chunk[cstep % 3];
@@ -464,7 +440,67 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         hi += uniform(ld(found));
       }
       if (tid == 0) st(&c->rounds, levels);
-      inv = hi;
+      if (hi > lo) {
+        // deep: pointer jumping from every vertex flagged so far (they are cand[0, hi)). Its
+        // rounds continue the ring where the walk stopped: slot `levels % 3` was cleared during
+        // the last level and nobody reads it, while the last level's count may still be unread
+        // by slower workgroups and must not be touched here.
+        for (int v = tid; v < n; v += threads) st(&p.ancestor[v], p.parents[v]);
+        if (tid == 0) st(&c->cand_count, hi);
+        DYNG_GRID_SYNC();
+        jump = true;
+        jump_slot = levels;
+      } else {
+        inv = hi;
+      }
+    }
+    if (jump) {
+      // ---- B (tree from outside, or a deep walk): pointer jumping over compacted lists of
+      // the vertices still jumping; a list that is not empty after ceil(log2 n) + 1 rounds is a
+      // parent cycle ----
+      int count = n;
+      const int* list = nullptr;
+      int* outs[2] = {p.near_a, p.near_b};
+      for (int r = 0; r < p.max_rounds; ++r) {
+        jump_items += count;
+        int* out = outs[r & 1];
+        int* out_count = &c->ring_jump[(r + jump_slot) % 3];
+        if (tid == 0) {
+          st(&c->ring_jump[(r + jump_slot + 1) % 3], 0);
+          st(&c->rounds, r + 1);
+        }
+        for (int base = tid - lane(); base < count; base += threads) {
+          const int i = base + lane();
+          bool keep = false, flagged = false;
+          int32_t v = -1;
+          if (i < count) {
+            v = list ? ld(&list[i]) : i;
+            const int32_t a = ld(&p.ancestor[v]);
+            if (a >= 0 && ld(&p.flag[v]) == 0) {
+              if (ld(&p.flag[a]) != 0) {
+                st(&p.flag[v], 1);
+                st(&p.stamp[v], generation + 1);
+                flagged = true;
+              } else {
+                const int32_t next = ld(&p.ancestor[a]);
+                st(&p.ancestor[v], next);
+                keep = next >= 0;
+              }
+            }
+          }
+          append(p.cand, &c->cand_count, flagged, v);
+          append(out, out_count, keep, v);
+        }
+        DYNG_GRID_SYNC();
+        count = uniform(ld(out_count));
+        list = out;
+        if (count == 0) break;
+      }
+      if (count > 0) {
+        if (tid == 0) atomicOr(&c->error_bits, DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE);
+        return;  // uniform
+      }
+      inv = uniform(ld(&c->cand_count));  // nothing appends to it after the last round's barrier
     }
     // ---- C: invalidate the flagged vertices (stamped when flagged) and claim the insertion
     // heads that are not invalidated, in one phase ----
@@ -472,8 +508,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     ++generation;
     for (int i = tid; i < inv; i += threads) {
       const int32_t v = ld(&p.cand[i]);
-      p.flag[v] = 0;
-      p.packed[v] = packed_inf;
+      st(&p.flag[v], 0);
+      st(&p.packed[v], packed_inf);
     }
     if (tid == 0) st(&c->invalidated, inv);
     // the heads go after the invalidated vertices, counted apart: cand_count stays fixed here
@@ -529,7 +565,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       const int wave = tid / 64;
       const int waves = threads / 64;
       for (int q = wave; q < nchunks; q += waves) {
-        const int4 ch = p.chunks[q];
+        const int4 ch = ld_chunk(&p.chunks[q]);
         u64 best = packed_inf;
         for (int e = ch.y + lane(); e < ch.z; e += 64) {
           const u64 o = pull_offer(p, p.in_nbr[e], p.in_w[e]);
@@ -545,7 +581,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
               atomicMin(&c->min_frontier, p.pk.distance(best));
               if (claim(p.stamp, ch.x, generation)) {
                 const int slot = atomicAdd(&c->frontier_count, 1);
-                p.frontier[slot] = ch.x;
+                st(&p.frontier[slot], ch.x);
               }
             }
           }
@@ -636,7 +672,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
             const u64 du = act ? p.pk.distance(L.word[item]) : 0;
             const int32_t w = act ? p.out_nbr[slot] : 0;
             const int32_t wt = act ? p.out_w[slot] : 0;
-            relax(p, act, u, du, w, wt, threshold, generation, next, next_count, far_list, far_counter,
+            relax(p, act && w >= 0, u, du, w, wt, threshold, generation, next, next_count, far_list, far_counter,
                   relax_improved);
           },
           [&](bool, int) {});
@@ -647,7 +683,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         const int wave = tid / 64;
         const int waves = threads / 64;
         for (int q = wave; q < nchunks; q += waves) {
-          const int4 ch = p.chunks[q];
+          const int4 ch = ld_chunk(&p.chunks[q]);
           const int32_t u = ch.x;
           const u64 word = ld(&p.packed[u]);
           const u64 du = p.pk.distance(word);
@@ -657,7 +693,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
             const bool act = word != packed_inf && e < ch.z;
             const int32_t w = act ? p.out_nbr[e] : 0;
             const int32_t wt = act ? p.out_w[e] : 0;
-            relax(p, act, u, du, w, wt, threshold, generation, next, next_count, far_list, far_counter,
+            relax(p, act && w >= 0, u, du, w, wt, threshold, generation, next, next_count, far_list, far_counter,
                   relax_improved);
           }
         }
@@ -698,7 +734,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         const u64 word = ld(&p.packed[v]);
         const u64 d = word == packed_inf ? packed_inf : p.pk.distance(word);
         if (d < threshold) {
-          p.in_far[v] = 0;
+          st(&p.in_far[v], 0);
           to_near = claim(p.stamp, v, generation);
         } else {
           keep = true;
@@ -719,7 +755,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   unsigned long long changed = 0;
   if (p.pk.bits > 0) {
     for (int v = tid; v < n; v += threads) {
-      const u64 word = p.packed[v];
+      const u64 word = ld(&p.packed[v]);
       const long long d = word == packed_inf ? sssp_infinity : static_cast<long long>(p.pk.distance(word));
       const int32_t par = word == packed_inf ? -1 : p.pk.parent(word);
       if (!p.count_changes) {
@@ -734,7 +770,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   } else {
     // distance-only words: lowest-id tight parents (fused.cuh:452-485)
     for (int v = tid; v < n; v += threads) {
-      const u64 word = p.packed[v];
+      const u64 word = ld(&p.packed[v]);
       const long long d = word == packed_inf ? sssp_infinity : static_cast<long long>(word);
       if (p.count_changes) {
         p.ancestor[v] = p.parents[v];
@@ -743,18 +779,18 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       p.distances[v] = d;
       p.parents[v] = (word == packed_inf || v == p.source) ? -1 : INT32_MAX;
     }
-    DYNG_GRID_SYNC();
+    DYNG_GRID_SYNC_FENCED();
     for (int u = tid; u < n; u += threads) {
-      const u64 du = p.packed[u];
+      const u64 du = ld(&p.packed[u]);
       if (du == packed_inf) continue;
       const int2 mt = p.out_meta[u];
       for (int j = 0; j < mt.y; ++j) {
         const int32_t w = p.out_nbr[mt.x + j];
-        if (w != p.source && du + static_cast<u64>(p.out_w[mt.x + j]) == p.packed[w]) atomicMin(&p.parents[w], u);
+        if (w >= 0 && w != p.source && du + static_cast<u64>(p.out_w[mt.x + j]) == ld(&p.packed[w])) atomicMin(&p.parents[w], u);
       }
     }
     if (p.count_changes) {
-      DYNG_GRID_SYNC();
+      DYNG_GRID_SYNC_FENCED();
       for (int v = tid; v < n; v += threads) {
         changed += (p.flag[v] != 0 || p.ancestor[v] != ld(&p.parents[v])) ? 1 : 0;
         p.flag[v] = 0;
@@ -780,6 +816,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
 }
 
 #undef DYNG_GRID_SYNC
+#undef DYNG_GRID_SYNC_FENCED
+#undef DYNG_GRID_SYNC_AS
 
 }  // namespace dyng_hip
 

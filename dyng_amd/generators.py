@@ -43,6 +43,9 @@ def lib() -> C.CDLL:
         L.dyng_gen_batches.argtypes = [vp, vp, C.c_int64, C.c_int64, C.c_int32, C.c_int64, C.c_int64, C.c_int64,
                                        C.c_int32, C.c_uint64, C.c_int32, C.c_double, C.c_double, C.c_double,
                                        vp, vp, vp, vp, vp]
+        L.dyng_gen_tree_batch.restype = C.c_int
+        L.dyng_gen_tree_batch.argtypes = [C.c_int64, vp, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_uint64,
+                                          C.c_int64, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -112,3 +115,21 @@ def max_out_degree_vertex(src: np.ndarray, n: int) -> int:
     """The vertex with the largest out-degree, lowest id on ties."""
     deg = np.bincount(src, minlength=n)
     return int(np.argmax(deg))
+
+
+def tree_batch(side: int, parents: np.ndarray, n_ins: int, n_tree: int, n_rand: int, wmax: int, seed: int,
+               batch: int):
+    """A batch against the current tree of a side x side grid graph: n_tree deletions of distinct
+    tree edges (parent[v], v), n_rand deletions of random grid arcs, n_ins random insertions with
+    w ~ U[1, wmax] (generators.cpp: dyng_gen_tree_batch)."""
+    parents = np.ascontiguousarray(parents, dtype=np.int32)
+    if len(parents) != side * side:
+        raise ValueError("tree_batch: parents must have side * side entries")
+    ins_s, ins_d, ins_w = (np.empty(n_ins, np.int32) for _ in range(3))
+    del_s, del_d = (np.empty(n_tree + n_rand, np.int32) for _ in range(2))
+    rc = lib().dyng_gen_tree_batch(side, parents.ctypes.data, n_ins, n_tree, n_rand, wmax, seed, batch,
+                                   ins_s.ctypes.data, ins_d.ctypes.data, ins_w.ctypes.data, del_s.ctypes.data,
+                                   del_d.ctypes.data)
+    if rc != 0:
+        raise ValueError("tree_batch: the tree has too few edges")
+    return ins_s, ins_d, ins_w, del_s, del_d

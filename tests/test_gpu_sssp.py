@@ -494,3 +494,31 @@ def test_long_rows_with_many_operations(dg):
         for f in ("inserted_edges", "updated_edges", "deleted_edges", "ignored_deletions"):
             assert getattr(st.batch, f) == a.summary[f], f
         _assert_same_rows(hg, og)
+
+
+def test_grid_with_deletions_on_the_tree(dg):
+    """The high-diameter configuration at a size the oracle runs in a moment: a 128 x 128 grid
+    (4-neighbour arcs both ways, 0.1 % shortcuts, w ~ U[1, 1000], source at the corner) and
+    batches whose deletions hit the current shortest-path tree (deep subtrees)."""
+    from dyng_amd import generators as G
+
+    side = 128
+    _say(f"[grid] side={side}")
+    src, dst, w = G.grid(side, 0.001, 1000, G.GRAPH_SEED ^ 2)
+    n = side * side
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    ot, _ = oracle.compute(og, 0)
+    hr = dg.sssp.compute(hg, 0)
+    d, p = hr.to_numpy()
+    assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent)
+    for b in range(4):
+        ins_s, ins_d, ins_w, del_s, del_d = G.tree_batch(side, ot.parent, 500, 400, 100, 1000,
+                                                         G.BATCH_SEED ^ 2, b)
+        og, a, c = oracle.update(og, ot, ins_s, ins_d, ins_w.reshape(-1, 1), del_s, del_d)
+        st = dg.sssp.update(hg, dg.EdgeBatch(insert=(ins_s, ins_d, ins_w), delete=(del_s, del_d)), hr)
+        d, p = hr.to_numpy()
+        assert np.array_equal(d, ot.dist), f"distances, batch {b}"
+        assert np.array_equal(p, ot.parent), f"parents, batch {b}"
+        assert st.invalidated == c.invalidated and st.affected == c.affected
+        assert st.invalidated > 0
