@@ -69,6 +69,7 @@ struct control {
 struct engine_params {
   int32_t n;
   const int2* out_meta;
+  const int32_t* out_base;  ///< per vertex: the length of the row's sorted base run
   const int32_t* out_nbr;
   const int32_t* out_w;
   const int2* in_meta;
@@ -280,6 +281,34 @@ DYNG_DEV u64 pull_offer(const engine_params& p, int32_t u, int32_t wt) {
   if (word == packed_inf) return packed_inf;
   const u64 nd = p.pk.distance(word) + static_cast<u64>(wt);
   return nd <= p.max_distance ? p.pk.pack(nd, u) : packed_inf;
+}
+
+/// The best offer to v over the alive entries (u, v) of u's out-row (both sorted runs), or INF.
+DYNG_DEV u64 edge_offer(const engine_params& p, int32_t u, int32_t v) {
+  const u64 word = ld(&p.packed[u]);
+  if (word == packed_inf) return packed_inf;
+  const u64 du = p.pk.distance(word);
+  const int2 mt = p.out_meta[u];
+  const int bl = p.out_base[u];
+  const int32_t* row = p.out_nbr + mt.x;
+  u64 best = packed_inf;
+  const int runs[3] = {0, bl, mt.y};
+  for (int r = 0; r < 2; ++r) {
+    int lo = runs[r], hi = runs[r + 1];
+    while (lo < hi) {  // first entry of the run whose neighbour (dead or alive) is >= v
+      const int mid = (lo + hi) >> 1;
+      if ((row[mid] & 0x7fffffff) < v) lo = mid + 1; else hi = mid;
+    }
+    for (int x = lo; x < runs[r + 1] && (row[x] & 0x7fffffff) == v; ++x) {
+      if (row[x] < 0) continue;  // dead
+      const u64 nd = du + static_cast<u64>(p.out_w[mt.x + x]);
+      if (nd <= p.max_distance) {
+        const u64 o = p.pk.pack(nd, u);
+        best = o < best ? o : best;
+      }
+    }
+  }
+  return best;
 }
 
 /// Lower v's word to its best pull offer; v joins the frontier if its distance fell. The
@@ -512,8 +541,10 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       st(&p.packed[v], packed_inf);
     }
     if (tid == 0) st(&c->invalidated, inv);
-    // the heads go after the invalidated vertices, counted apart: cand_count stays fixed here
-    for (int64_t base = tid - lane(); base < p.num_ins; base += threads) {
+    // A tree from outside: the insertion heads that are not invalidated pull over all their
+    // in-edges, as the reference does; they go after the invalidated vertices, counted apart
+    // (cand_count stays fixed here). The engine's own tree needs less, see phase D.
+    for (int64_t base = tid - lane(); p.check_cycles && base < p.num_ins; base += threads) {
       const int64_t i = base + lane();
       int32_t v = -1;
       bool add = false;
@@ -530,10 +561,28 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     if (tid == 0) {
       c->t_phase[1] = __builtin_amdgcn_s_memrealtime();
       c->jump_items = jump_items;
-      c->pull_vertices = cands;
+      c->pull_vertices = cands + (p.check_cycles ? 0 : p.num_ins);
     }
     int* chunk_count = &c->ring_chunk[cstep % 3];
     if (tid == 0) st(&c->ring_chunk[(cstep + 1) % 3], 0);
+    if (!p.check_cycles) {
+      // The engine's own tree is canonical: for a head v that was not invalidated, every in-edge
+      // that was there before the batch and not reweighted offers at least v's old word (no
+      // smaller distance, and no equal distance from a lower id), and v's old parent edge is
+      // intact, so v's word can only fall through the edges this batch inserted or reweighted.
+      // Each effective insertion (u, v) offers over the alive (u, v) entries of u's row, instead
+      // of a pull over all of v's in-edges.
+      for (int64_t base = tid - lane(); base < p.num_ins; base += threads) {
+        const int64_t i = base + lane();
+        int32_t v = -1;
+        u64 best = packed_inf;
+        if (i < p.num_ins) {
+          v = p.ins_dst[i];
+          if (v >= 0 && v != p.source) best = edge_offer(p, p.ins_src[i], v);
+        }
+        pull_commit(p, best != packed_inf, v, best, generation, &c->frontier_count, &c->min_frontier);
+      }
+    }
     tile_rows(
         L, cands,
         [&](int idx, int32_t& v, int32_t& off, int32_t& len, u64& word) {
@@ -944,6 +993,7 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   p.n = static_cast<int32_t>(n);
   const int k = run->objective;
   p.out_meta = g->out.meta.p;
+  p.out_base = g->out.base.p;
   p.out_nbr = g->out.nbr.p;
   p.out_w = g->out.w.p + static_cast<int64_t>(k) * g->out.slot_cap;
   p.in_meta = g->in.meta.p;
