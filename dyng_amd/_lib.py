@@ -53,7 +53,8 @@ class Batch(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("engine", C.c_int32), ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("light_degree", C.c_int32), ("chunk_edges", C.c_int32), ("barrier_timeout_us", C.c_int32)]
+                ("light_degree", C.c_int32), ("chunk_edges", C.c_int32), ("launch", C.c_int32),
+                ("barrier_timeout_us", C.c_int32)]
 
 
 COUNTER_FIELDS = ("invalidated", "iterations", "epochs", "pushes", "affected")
@@ -86,7 +87,7 @@ class SsspRun(C.Structure):
     _fields_ = [("graph", C.c_void_p), ("objective", C.c_int32), ("source", C.c_int32),
                 ("delta", C.c_int64), ("max_weight", C.c_int64), ("distances", C.c_void_p),
                 ("parents", C.c_void_p), ("check_cycles", C.c_int32), ("workspace", C.c_void_p),
-                ("tuning", Tuning)]
+                ("tuning", Tuning), ("sources", C.c_void_p), ("num_sources", C.c_int64)]
 
 
 class Changes(C.Structure):
@@ -107,7 +108,8 @@ EXPORTS = (
     "dyng_hip_result_arrays", "dyng_hip_result_source", "dyng_hip_result_version",
     "dyng_hip_result_options", "dyng_hip_result_set_options", "dyng_hip_result_download",
     "dyng_hip_result_set_tuning", "dyng_hip_device_count", "dyng_hip_malloc", "dyng_hip_free",
-    "dyng_hip_memcpy_htod", "dyng_hip_memcpy_dtoh", "dyng_hip_synchronize",
+    "dyng_hip_memcpy_htod", "dyng_hip_memcpy_dtoh", "dyng_hip_synchronize", "dyng_hip_api_compute_multi",
+    "dyng_hip_api_repair",
 )
 
 _lib = None
@@ -144,6 +146,8 @@ def lib() -> C.CDLL:
     L.dyng_hip_sssp_compute.argtypes = [C.POINTER(SsspRun), vp, C.POINTER(Counters)]
     L.dyng_hip_api_compute.argtypes = [vp, C.c_int32, C.POINTER(Options), vp, C.POINTER(vp)]
     L.dyng_hip_api_update.argtypes = [vp, C.POINTER(Batch), vp, vp, C.POINTER(Stats)]
+    L.dyng_hip_api_compute_multi.argtypes = [vp, vp, C.c_int64, C.POINTER(Options), vp, C.POINTER(vp)]
+    L.dyng_hip_api_repair.argtypes = [vp, vp, vp, C.POINTER(Stats)]
     L.dyng_hip_api_from_arrays.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, C.POINTER(Options), vp,
                                            C.POINTER(vp)]
     L.dyng_hip_result_destroy.argtypes = [vp]

@@ -113,6 +113,9 @@ struct dyng_hip_result {
   dyng_hip::dbuf<int64_t> distances;
   dyng_hip::dbuf<int32_t> parents;
   dyng_hip_tuning tuning{};
+  /// several sources at once (source = -1): their ids on the device
+  dyng_hip::dbuf<int32_t> sources;
+  int64_t num_sources = 0;
 };
 
 using namespace dyng_hip;
@@ -142,8 +145,14 @@ dyng_hip_sssp_run make_run(const dyng_hip_graph* g, dyng_hip_result* r, int64_t 
   run.workspace = pool().get(g->device);
   run.tuning = r->tuning;
   if (r->opt.cuda_engine != DYNG_HIP_ENGINE_AUTOMATIC && run.tuning.engine == 0) run.tuning.engine = r->opt.cuda_engine;
+  if (r->num_sources > 0) {
+    run.source = -1;
+    run.sources = r->sources.p;
+    run.num_sources = r->num_sources;
+  }
   return run;
 }
+
 
 /// Vertex growth: new vertices start unreachable (sssp.cpp:579-612).
 void grow_result(dyng_hip_result* r, int64_t n, hipStream_t s) {
@@ -267,6 +276,29 @@ struct event_pair {
   }
 };
 
+/// The engine part of an update (sssp_problem::resume + the engine run, sssp.cpp:691-823) for a
+/// result that is one apply behind the graph; a failure poisons the result.
+void repair_core(dyng_hip_graph* g, dyng_hip_result* r, int64_t delta, int64_t max_weight, hipStream_t s,
+                 dyng_hip_counters& c, double& engine_ms) {
+  event_pair ev;
+  try {
+    grow_result(r, g->n, s);
+    std::lock_guard<std::mutex> lk(pool().lock);
+    dyng_hip_sssp_run run = make_run(g, r, delta, max_weight);
+    DYNG_HIP_TRY(hipEventRecord(ev.a, s));
+    call(dyng_hip_sssp_update(&run, s, &c));
+    DYNG_HIP_TRY(hipEventRecord(ev.b, s));
+    DYNG_HIP_TRY(hipEventSynchronize(ev.b));
+  } catch (...) {
+    r->poisoned = true;
+    throw;
+  }
+  r->version = g->version;
+  r->trusted = true;
+  r->engine_tree = true;
+  engine_ms = ev.ms();
+}
+
 }  // namespace
 
 extern "C" const char* dyng_hip_last_error(void) { return dyng_hip::last_error.c_str(); }
@@ -302,6 +334,57 @@ extern "C" int dyng_hip_api_compute(dyng_hip_graph* g, int32_t source, const dyn
     r->n = n;
     r->distances.reserve(n);
     r->parents.reserve(n);
+    const int64_t delta = opt.delta > 0 ? opt.delta : default_delta(std::max<int64_t>(g->m, 1), n, g->wsum[k]);
+    std::lock_guard<std::mutex> lk(pool().lock);
+    dyng_hip_sssp_run run = make_run(g, r.get(), delta, g->wmax[k]);
+    dyng_hip_counters c{};
+    call(dyng_hip_sssp_compute(&run, s, &c));
+    *out = r.release();
+  });
+}
+
+extern "C" int dyng_hip_api_compute_multi(dyng_hip_graph* g, const int32_t* sources, int64_t count,
+                                          const dyng_hip_options* opt_in, void* stream, dyng_hip_result** out) {
+  return guarded([&] {
+    if (out == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::compute: null output handle");
+    const dyng_hip_options opt = opt_in ? *opt_in : dyng_hip_options{0, 0, 0, 1};
+    expect_options(opt);
+    expect_graph(g, opt, "sssp::compute");
+    const int64_t n = g->n;
+    if (count < 1 || count > n || sources == nullptr) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::compute: between 1 and n sources are needed, got " + std::to_string(count));
+    }
+    std::vector<char> seen(static_cast<size_t>(n), 0);
+    for (int64_t i = 0; i < count; ++i) {
+      if (sources[i] < 0 || sources[i] >= n) {
+        fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::compute: source " + std::to_string(sources[i]) +
+                                            " is out of range [0, " + std::to_string(n) + ")");
+      }
+      if (seen[sources[i]]++) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::compute: source " + std::to_string(sources[i]) + " is listed twice");
+    }
+    const int k = opt.objective;
+    if (g->wmin[k] < 1) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::compute: an edge has a weight below 1 in objective " + std::to_string(k) +
+                                          "; sssp needs weights in [1, 2^31 - 1]");
+    }
+    if (!distances_fit(n, g->wmax[k])) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::compute: distances up to " + std::to_string(g->wmax[k]) + " * " +
+                                          std::to_string(n - 1) + " do not fit in 62 bits");
+    }
+    scoped_device guard(g->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto r = std::make_unique<dyng_hip_result>();
+    r->device = g->device;
+    r->source = -1;
+    r->opt = opt;
+    r->version = g->version;
+    r->graph_state = g->state_id;
+    r->n = n;
+    r->distances.reserve(n);
+    r->parents.reserve(n);
+    r->sources.reserve(count);
+    r->num_sources = count;
+    DYNG_HIP_TRY(hipMemcpyAsync(r->sources.p, sources, count * sizeof(int32_t), hipMemcpyHostToDevice, s));
     const int64_t delta = opt.delta > 0 ? opt.delta : default_delta(std::max<int64_t>(g->m, 1), n, g->wsum[k]);
     std::lock_guard<std::mutex> lk(pool().lock);
     dyng_hip_sssp_run run = make_run(g, r.get(), delta, g->wmax[k]);
@@ -347,7 +430,7 @@ extern "C" int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batc
                                           std::to_string(sc.n_after - 1) + " do not fit in 62 bits");
     }
     // ---- commit: the device apply ----
-    event_pair ev_apply, ev_engine;
+    event_pair ev_apply;
     dyng_hip_apply_summary summary{};
     DYNG_HIP_TRY(hipEventRecord(ev_apply.a, s));
     try {
@@ -359,27 +442,53 @@ extern "C" int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batc
     DYNG_HIP_TRY(hipEventRecord(ev_apply.b, s));
     // ---- resume + the engine; a failure from here on poisons the result ----
     dyng_hip_counters c{};
-    try {
-      grow_result(r, g->n, s);
-      std::lock_guard<std::mutex> lk(pool().lock);
-      dyng_hip_sssp_run run = make_run(g, r, delta, max_weight);
-      DYNG_HIP_TRY(hipEventRecord(ev_engine.a, s));
-      call(dyng_hip_sssp_update(&run, s, &c));
-      DYNG_HIP_TRY(hipEventRecord(ev_engine.b, s));
-      DYNG_HIP_TRY(hipEventSynchronize(ev_engine.b));
-    } catch (...) {
-      r->poisoned = true;
-      throw;
-    }
-    r->version = g->version;
-    r->trusted = true;
-    r->engine_tree = true;
+    double engine_ms = 0.0;
+    repair_core(g, r, delta, max_weight, s, c, engine_ms);
     if (stats != nullptr) {
       std::memset(stats, 0, sizeof(*stats));
       fill_stats(stats, c);
       stats->batch = summary;
       stats->apply_ms = ev_apply.ms();
-      stats->engine_ms = ev_engine.ms();
+      stats->engine_ms = engine_ms;
+    }
+  });
+}
+
+extern "C" int dyng_hip_api_repair(dyng_hip_graph* g, dyng_hip_result* r, void* stream, dyng_hip_stats* stats) {
+  return guarded([&] {
+    expect_usable(r, "sssp::repair");
+    if (g == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::repair: null graph");
+    if (r->graph_state != g->state_id || r->version + 1 != g->version || !g->last.valid) {
+      throw status_error(DYNG_HIP_STALE_RESULT,
+                         "dyng: sssp::repair: the result matches graph version " + std::to_string(r->version) +
+                             " but repair needs a result exactly one apply behind the graph (version " +
+                             std::to_string(g->version) + ")");
+    }
+    if (r->device != g->device) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::repair: the result is on another device");
+    expect_graph(g, r->opt, "sssp::repair");
+    if (r->n > g->n) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::repair: the result has more vertices than the graph");
+    const int k = r->opt.objective;
+    if (g->last.num_ins > 0 && g->last.batch_wmin[k] < 1) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::repair: the last batch inserted a weight below 1 in objective " +
+                                          std::to_string(k) + "; sssp needs weights in [1, 2^31 - 1]");
+    }
+    const int64_t max_weight = g->last.max_weight[k];
+    const int64_t delta =
+        r->opt.delta > 0 ? r->opt.delta : default_delta(std::max<int64_t>(g->last.m, 1), g->last.n, g->last.wsum[k]);
+    if (!distances_fit(g->n, max_weight)) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::repair: distances up to " + std::to_string(max_weight) + " * " +
+                                          std::to_string(g->n - 1) + " do not fit in 62 bits");
+    }
+    scoped_device guard(g->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dyng_hip_counters c{};
+    double engine_ms = 0.0;
+    repair_core(g, r, delta, max_weight, s, c, engine_ms);
+    if (stats != nullptr) {
+      std::memset(stats, 0, sizeof(*stats));
+      fill_stats(stats, c);
+      stats->batch = g->last.summary;
+      stats->engine_ms = engine_ms;
     }
   });
 }
@@ -468,6 +577,12 @@ extern "C" int dyng_hip_result_clone(const dyng_hip_result* r, void* stream, dyn
     c->n = r->n;
     c->distances.reserve(r->n);
     c->parents.reserve(r->n);
+    if (r->num_sources > 0) {
+      c->sources.reserve(r->num_sources);
+      c->num_sources = r->num_sources;
+      DYNG_HIP_TRY(hipMemcpyAsync(c->sources.p, r->sources.p, r->num_sources * sizeof(int32_t),
+                                  hipMemcpyDeviceToDevice, s));
+    }
     if (r->n > 0) {
       DYNG_HIP_TRY(hipMemcpyAsync(c->distances.p, r->distances.p, r->n * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
       DYNG_HIP_TRY(hipMemcpyAsync(c->parents.p, r->parents.p, r->n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));

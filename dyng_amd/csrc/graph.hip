@@ -1611,6 +1611,19 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
 
 void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out) {
   const int K = g->K;
+  // G_t as the batch found it (the repair of a result one apply behind reads it)
+  g->last.valid = false;
+  g->last.max_weight.assign(K, 1);
+  g->last.batch_wmin.assign(K, 1);
+  g->last.wsum.assign(K, 0);
+  for (int k = 0; k < K; ++k) {
+    g->last.max_weight[k] = std::max<int64_t>(std::max<int64_t>(sc.wmax[k], 1), g->wmax[k]);
+    g->last.batch_wmin[k] = sc.wmin[k];
+    g->last.wsum[k] = g->wsum[k];
+  }
+  g->last.m = g->m;
+  g->last.n = g->n;
+  g->last.num_ins = sc.num_ins;
   const int64_t num_ins = sc.num_ins, num_del = sc.num_del;
   const int32_t *ins_src = sc.ins_src, *ins_dst = sc.ins_dst, *ins_w = sc.ins_w, *del_src = sc.del_src,
                 *del_dst = sc.del_dst;
@@ -1888,7 +1901,9 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     out->num_vertices_after = n_after;
     out->relocated_rows = static_cast<int64_t>(hc->relocated_rows);
     stage_times(g, N > 0 && hc->error_bits == 0, out);
+    g->last.summary = *out;
   }
+  g->last.valid = true;
   // the last edge of the largest weight left: recompute the column summaries (reuses the control
   // block, so only after the counters above were read)
   if (dirty) recompute_columns(g, s);

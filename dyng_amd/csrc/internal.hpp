@@ -188,6 +188,16 @@ struct dyng_hip_graph {
   std::vector<int64_t> wmax, wsum, wmax_count;
   std::vector<int64_t> wmin;  ///< 1 when every weight of the column is >= 1, else 0 (or below)
 
+  /// What the last apply found on G_t (before it changed anything), for the repair of a result
+  /// one apply behind: per objective max(batch max, G_t max) and the smallest batch weight, and
+  /// G_t's size and weight sums (the default delta); plus its summary.
+  struct last_apply_info {
+    bool valid = false;
+    std::vector<int64_t> max_weight, batch_wmin, wsum;
+    int64_t m = 0, n = 0, num_ins = 0;
+    dyng_hip_apply_summary summary{};
+  } last;
+
   // last apply: the effective operations (with holes, -1) and their weight-increase flags
   dyng_hip::dbuf<int32_t> del_src, del_dst, ins_src, ins_dst;
   dyng_hip::dbuf<uint8_t> ins_inc;

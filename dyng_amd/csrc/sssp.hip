@@ -84,7 +84,9 @@ struct engine_params {
   int32_t inc_stride;
   int32_t inc_k;
   int64_t num_ins;
-  int32_t source;
+  int32_t source;           ///< the source, or -1 with several (sources[0, num_sources))
+  const int32_t* sources;
+  int64_t num_sources;
   int32_t from_scratch;
   int32_t count_changes;
   int32_t check_cycles;
@@ -347,6 +349,11 @@ DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best
 #define DYNG_GRID_SYNC() DYNG_GRID_SYNC_AS(false)
 #define DYNG_GRID_SYNC_FENCED() DYNG_GRID_SYNC_AS(true)
 
+/// Clear the grid barrier (a kernel, not a memset, so the launch can live in a hipGraph).
+__global__ void k_reset_barrier(grid_sync_state* bar) {
+  if (threadIdx.x == 0) *bar = grid_sync_state{0u, 0u, 0u, 0u};
+}
+
 __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) {
   __shared__ tile_shared L;
   // work counters of the workgroup, in LDS: relax edges, improving atomics, pull edges, affected
@@ -368,9 +375,21 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   if (tid == 0) c->t_phase[0] = __builtin_amdgcn_s_memrealtime();
 
   if (p.from_scratch) {
-    for (int v = tid; v < n; v += threads) st(&p.packed[v], v == p.source ? p.pk.pack(0, -1) : packed_inf);
-    if (tid == 0) st(&p.frontier[0], p.source);
-    frontier_count = 1;
+    if (p.source >= 0) {
+      for (int v = tid; v < n; v += threads) st(&p.packed[v], v == p.source ? p.pk.pack(0, -1) : packed_inf);
+      if (tid == 0) st(&p.frontier[0], p.source);
+      frontier_count = 1;
+    } else {
+      // several sources (each at distance 0 with no parent; weights are >= 1, so no offer ever
+      // beats a source's word and no source needs skipping by id)
+      for (int v = tid; v < n; v += threads) st(&p.packed[v], packed_inf);
+      DYNG_GRID_SYNC();
+      for (int64_t i = tid; i < p.num_sources; i += threads) {
+        st(&p.packed[p.sources[i]], p.pk.pack(0, -1));
+        st(&p.frontier[i], p.sources[i]);
+      }
+      frontier_count = static_cast<int>(p.num_sources);
+    }
     DYNG_GRID_SYNC();
   } else {
     // ---- A: pack the old tree; ancestors (pointer jumping only); roots ----
@@ -826,7 +845,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         p.flag[v] = d != p.distances[v] ? 1 : 0;
       }
       p.distances[v] = d;
-      p.parents[v] = (word == packed_inf || v == p.source) ? -1 : INT32_MAX;
+      p.parents[v] = (word == packed_inf || word == 0) ? -1 : INT32_MAX;  // distance 0: a source
     }
     DYNG_GRID_SYNC_FENCED();
     for (int u = tid; u < n; u += threads) {
@@ -891,7 +910,14 @@ struct dyng_hip_workspace {
   int occupancy = 0;
   hipEvent_t ev_start = nullptr;
   hipEvent_t ev_stop = nullptr;
+  /// the launch as a hipGraph (control upload, barrier reset, events, the kernel, control
+  /// download), built once and replayed with the kernel node's arguments updated
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraphNode_t graph_kernel = nullptr;
   ~dyng_hip_workspace() {
+    if (graph_exec != nullptr) (void)hipGraphExecDestroy(graph_exec);
+    if (graph != nullptr) (void)hipGraphDestroy(graph);
     if (ev_start != nullptr) (void)hipEventDestroy(ev_start);
     if (ev_stop != nullptr) (void)hipEventDestroy(ev_stop);
   }
@@ -958,7 +984,11 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
     return;
   }
   if (run->delta <= 0) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: the near-far width delta must be > 0, got " + std::to_string(run->delta));
-  if (run->source < 0 || run->source >= n) {
+  if (run->source == -1) {
+    if (run->sources == nullptr || run->num_sources < 1 || run->num_sources > n) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: source -1 needs a device list of 1 to n sources");
+    }
+  } else if (run->source < 0 || run->source >= n) {
     fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: source " + std::to_string(run->source) + " is out of range [0, " +
                                         std::to_string(n) + ")");
   }
@@ -1011,6 +1041,8 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
     p.num_ins = g->num_ins_slots;
   }
   p.source = run->source;
+  p.sources = run->sources;
+  p.num_sources = run->num_sources;
   p.from_scratch = update ? 0 : 1;
   p.count_changes = update ? 1 : 0;
   p.check_cycles = run->check_cycles;
@@ -1058,17 +1090,52 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   hc->min_frontier = packed_inf;
   hc->minimum[0] = packed_inf;
   hc->minimum[1] = packed_inf;
-  DYNG_HIP_TRY(hipMemcpyAsync(ws->ctl.p, hc, sizeof(control), hipMemcpyHostToDevice, s));
-  DYNG_HIP_TRY(hipMemsetAsync(ws->bar.p, 0, sizeof(grid_sync_state), s));
   if (ws->ev_start == nullptr) {
     DYNG_HIP_TRY(hipEventCreate(&ws->ev_start));
     DYNG_HIP_TRY(hipEventCreate(&ws->ev_stop));
   }
-  DYNG_HIP_TRY(hipEventRecord(ws->ev_start, s));
-  hipLaunchKernelGGL(sssp_fused_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
-  DYNG_HIP_TRY(hipGetLastError());
-  DYNG_HIP_TRY(hipEventRecord(ws->ev_stop, s));
-  DYNG_HIP_TRY(hipMemcpyAsync(hc, ws->ctl.p, sizeof(control), hipMemcpyDeviceToHost, s));
+  void* args[] = {&p};
+  if (run->tuning.launch == DYNG_HIP_LAUNCH_GRAPH) {
+    hipKernelNodeParams kp{};
+    kp.func = reinterpret_cast<void*>(sssp_fused_kernel);
+    kp.gridDim = dim3(blocks);
+    kp.blockDim = dim3(kBlock);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    if (ws->graph_exec == nullptr) {
+      // built once per workspace: the control block, the barrier and the events never move
+      DYNG_HIP_TRY(hipGraphCreate(&ws->graph, 0));
+      hipGraphNode_t up, reset, start, stop, down;
+      DYNG_HIP_TRY(hipGraphAddMemcpyNode1D(&up, ws->graph, nullptr, 0, ws->ctl.p, hc, sizeof(control),
+                                           hipMemcpyHostToDevice));
+      grid_sync_state* bar = ws->bar.p;
+      void* reset_args[] = {&bar};
+      hipKernelNodeParams rp{};
+      rp.func = reinterpret_cast<void*>(k_reset_barrier);
+      rp.gridDim = dim3(1);
+      rp.blockDim = dim3(64);
+      rp.kernelParams = reset_args;
+      DYNG_HIP_TRY(hipGraphAddKernelNode(&reset, ws->graph, &up, 1, &rp));
+      DYNG_HIP_TRY(hipGraphAddEventRecordNode(&start, ws->graph, &reset, 1, ws->ev_start));
+      DYNG_HIP_TRY(hipGraphAddKernelNode(&ws->graph_kernel, ws->graph, &start, 1, &kp));
+      DYNG_HIP_TRY(hipGraphAddEventRecordNode(&stop, ws->graph, &ws->graph_kernel, 1, ws->ev_stop));
+      DYNG_HIP_TRY(hipGraphAddMemcpyNode1D(&down, ws->graph, &stop, 1, hc, ws->ctl.p, sizeof(control),
+                                           hipMemcpyDeviceToHost));
+      DYNG_HIP_TRY(hipGraphInstantiate(&ws->graph_exec, ws->graph, nullptr, nullptr, 0));
+    } else {
+      DYNG_HIP_TRY(hipGraphExecKernelNodeSetParams(ws->graph_exec, ws->graph_kernel, &kp));
+    }
+    DYNG_HIP_TRY(hipGraphLaunch(ws->graph_exec, s));
+  } else {
+    DYNG_HIP_TRY(hipMemcpyAsync(ws->ctl.p, hc, sizeof(control), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, s, ws->bar.p);
+    DYNG_HIP_TRY(hipEventRecord(ws->ev_start, s));
+    hipLaunchKernelGGL(sssp_fused_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
+    DYNG_HIP_TRY(hipGetLastError());
+    DYNG_HIP_TRY(hipEventRecord(ws->ev_stop, s));
+    DYNG_HIP_TRY(hipMemcpyAsync(hc, ws->ctl.p, sizeof(control), hipMemcpyDeviceToHost, s));
+  }
   DYNG_HIP_TRY(hipStreamSynchronize(s));
   float kernel_ms = 0.0f;
   DYNG_HIP_TRY(hipEventElapsedTime(&kernel_ms, ws->ev_start, ws->ev_stop));

@@ -58,11 +58,15 @@ class Tuning:
     blocks_per_cu: int = 0
     light_degree: int = 0
     chunk_edges: int = 0
+    launch: str = "plain"  # "plain" or "graph" (the launch built once as a hipGraph and replayed)
     barrier_timeout_us: int = 0
 
     def _native(self) -> _lib.Tuning:
+        launches = {"plain": 0, "graph": 1}
+        if self.launch not in launches:
+            raise InvalidArgumentError(f"Tuning.launch must be 'plain' or 'graph', got {self.launch!r}")
         return _lib.Tuning(_lib.ENGINES[self.engine], 0, int(self.blocks_per_cu), int(self.light_degree),
-                           int(self.chunk_edges), int(self.barrier_timeout_us))
+                           int(self.chunk_edges), launches[self.launch], int(self.barrier_timeout_us))
 
 
 @dataclass(frozen=True)
@@ -272,6 +276,46 @@ def compute(graph: Graph, source: int, *, options: Options | None = None, resour
     if tuning is not None:
         r.set_tuning(tuning)
     return r
+
+
+def compute_many(graph: Graph, sources, *, options: Options | None = None, resources: Resources | None = None,
+                 tuning: Tuning | None = None, **kwargs: Any) -> Result:
+    """The canonical trees of several distinct sources in one result and one launch (no
+    counterpart in the reference: the batched path, e.g. one source per component of a disjoint
+    union of graphs). Vertices reachable from several sources take the nearest one, lowest
+    (distance, parent) first; for a disjoint union each component holds its own tree."""
+    _check_graph(graph)
+    for key in kwargs:
+        if key not in {f.name for f in dataclasses.fields(Options)}:
+            raise TypeError(f"sssp.compute_many: unexpected keyword argument {key!r}")
+    opt = dataclasses.replace(options or Options(), **kwargs)
+    res = resolve(resources, graph.resources)
+    src = np.ascontiguousarray(np.asarray(sources, dtype=np.int64).reshape(-1))
+    if src.size and (src.min() < -(2**31) or src.max() >= 2**31):
+        raise InvalidArgumentError("sssp.compute_many: sources must fit int32")
+    src32 = np.ascontiguousarray(src.astype(np.int32))
+    o = opt._native()
+    h = C.c_void_p()
+    _lib.check(_lib.lib().dyng_hip_api_compute_multi(graph._handle, src32.ctypes.data, len(src32), C.byref(o),
+                                                     res.stream_handle, C.byref(h)))
+    r = Result._wrap(h, res)
+    if tuning is not None:
+        r.set_tuning(tuning)
+    return r
+
+
+def repair(graph: Graph, result: Result, *, resources: Resources | None = None) -> Stats:
+    """The repair half of update (sssp_problem::resume + the engine, sssp.cpp:691-823) for a
+    result exactly one ``Graph.apply`` behind the graph: ``graph.apply(batch)`` then
+    ``repair(graph, result)`` equals ``update(graph, batch, result)``; with clones of the result,
+    several schedules can be timed on one batch."""
+    _check_graph(graph)
+    if not isinstance(result, Result):
+        raise TypeError("dyng_amd.sssp.repair: result must be a dyng_amd.sssp.Result")
+    res = resolve(resources, result._resources)
+    out = _lib.Stats()
+    _lib.check(_lib.lib().dyng_hip_api_repair(graph._handle, result._handle, res.stream_handle, C.byref(out)))
+    return Stats._from_native(out)
 
 
 def update(graph: Graph, batch: EdgeBatch, result: Result, *, resources: Resources | None = None) -> Stats:

@@ -158,12 +158,16 @@ int dyng_hip_workspace_destroy(dyng_hip_workspace* ws);
 
 /* Schedule knobs (the reference's only ones are delta and cuda_engine, sssp.hpp:78-94; they
  * change the schedule, never the result). 0 = default. */
+#define DYNG_HIP_LAUNCH_PLAIN 0
+#define DYNG_HIP_LAUNCH_GRAPH 1 /* the engine's launch built once as a hipGraph and replayed */
+
 typedef struct dyng_hip_tuning {
   int32_t engine;          /* DYNG_HIP_ENGINE_* */
   int32_t block_size;      /* threads per workgroup of the persistent kernel (256, 512, 1024) */
   int32_t blocks_per_cu;   /* resident workgroups per CU (<= occupancy) */
   int32_t light_degree;    /* rows up to this length: edge-balanced inside a workgroup (0: 4096) */
   int32_t chunk_edges;     /* longer rows: split in chunks of this many edges, one wave each (0: 256) */
+  int32_t launch;             /* DYNG_HIP_LAUNCH_*: a plain launch, or a hipGraph replayed per run */
   int32_t barrier_timeout_us; /* longest wait at a grid barrier before the run fails with
                                  DYNG_HIP_DEVICE_ERROR_TIMEOUT (0: 4 s; < 0: no wait at all, so
                                  the run fails at its first barrier: exercises the error path) */
@@ -184,6 +188,11 @@ typedef struct dyng_hip_sssp_run {
                            tree edges */
   dyng_hip_workspace* workspace;
   dyng_hip_tuning tuning;
+  /* several sources at once (no counterpart in the reference: the batched multi-instance path,
+   * many independent trees of a disjoint union of graphs): source = -1 and these device ids;
+   * null / 0 for the single source above */
+  const int32_t* sources;
+  int64_t num_sources;
 } dyng_hip_sssp_run;
 
 /* sssp_counters (problem.hpp:106-113) plus the relax kernel's work counters. */
@@ -253,6 +262,15 @@ int dyng_hip_graph_version(const dyng_hip_graph* g, uint64_t* version);
 /* native.sssp_compute (bindings/sssp.cpp:77-96 -> sssp::compute, sssp.hpp:415-421). */
 int dyng_hip_api_compute(dyng_hip_graph* g, int32_t source, const dyng_hip_options* opt,
                          void* stream, dyng_hip_result** out);
+/* Several trees in one run (no counterpart in the reference; the batched multi-instance path):
+ * one result holding the trees of `count` distinct host sources, e.g. one per component of a
+ * disjoint union of graphs. Its source is -1; updates repair all the trees in one launch. */
+int dyng_hip_api_compute_multi(dyng_hip_graph* g, const int32_t* sources, int64_t count,
+                               const dyng_hip_options* opt, void* stream, dyng_hip_result** out);
+/* The repair half of sssp::update (sssp_problem::resume + the engine, sssp.cpp:691-823) for a
+ * result exactly one dyng_hip_graph_apply behind the graph: with a clone of the result, several
+ * schedules can be timed on one batch (the autotuner, the scheduler's training traces). */
+int dyng_hip_api_repair(dyng_hip_graph* g, dyng_hip_result* r, void* stream, dyng_hip_stats* stats);
 /* native.sssp_update (bindings/sssp.cpp:98-110 -> sssp::update, sssp.hpp:484-489). */
 int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batch, dyng_hip_result* r,
                         void* stream, dyng_hip_stats* stats);

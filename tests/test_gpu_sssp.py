@@ -522,3 +522,138 @@ def test_grid_with_deletions_on_the_tree(dg):
         assert np.array_equal(p, ot.parent), f"parents, batch {b}"
         assert st.invalidated == c.invalidated and st.affected == c.affected
         assert st.invalidated > 0
+
+
+# ------------------------------------------------------------------------------------------------
+# Batched multi-instance repair, repair after apply, the hipGraph launch, the autotuner
+# ------------------------------------------------------------------------------------------------
+
+def _rmat_instance(rng, n, wmax):
+    """A small RMAT-like instance (a=0.57, b=0.19, c=0.19) with average degree 16."""
+    m = 16 * n
+    bits = max(1, int(np.ceil(np.log2(n))))
+    u = np.zeros(m, np.int64)
+    v = np.zeros(m, np.int64)
+    for _ in range(bits):
+        r = rng.random(m)
+        u = 2 * u + (r >= 0.76)
+        v = 2 * v + (((r >= 0.57) & (r < 0.76)) | (r >= 0.95))
+    u %= n
+    v %= n
+    keep = u != v
+    return u[keep], v[keep], rng.integers(1, wmax + 1, int(keep.sum()))
+
+
+def _instance_batch(rng, og, n, frac, wmax):
+    rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+    k = max(1, int(frac * og.m) // 2)
+    pick = rng.choice(og.m, min(k, og.m), replace=False)
+    ins = (rng.integers(0, n, k), rng.integers(0, n, k), rng.integers(1, wmax + 1, k))
+    return ins[0], ins[1], ins[2], rows[pick], og.col[pick].copy()
+
+
+def test_batched_instances_match_each_instance_alone(dg):
+    """Independent RMAT graphs of uneven sizes as one device graph: one compute and one update
+    launch for all, each instance's tree equal to the oracle on the instance alone."""
+    _say("[batched] 12 instances")
+    rng = np.random.default_rng(21)
+    sizes = [int(2 ** rng.uniform(6, 12)) for _ in range(12)]
+    graphs, oracles, sources = [], [], []
+    for n in sizes:
+        s, d, w = _rmat_instance(rng, n, 100)
+        graphs.append((n, s, d, w))
+        og = oracle.build_from_edges(n, s, d, w.reshape(-1, 1))
+        src = int(rng.integers(0, n))
+        ot, _ = oracle.compute(og, src)
+        oracles.append([og, ot])
+        sources.append(src)
+    inst = dg.batched.Instances(graphs)
+    r = inst.compute(sources)
+    arrays = r.to_numpy()
+    for i, (og, ot) in enumerate(oracles):
+        d, p = inst.tree(r, i, arrays)
+        assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent), f"instance {i} compute"
+    for b in range(3):
+        batches = []
+        for i, n in enumerate(sizes):
+            x = _instance_batch(rng, oracles[i][0], n, 0.01, 100)
+            batches.append(x)
+            oracles[i][0], _, _ = oracle.update(oracles[i][0], oracles[i][1], x[0], x[1], x[2].reshape(-1, 1),
+                                                x[3], x[4])
+        inst.update(batches, r)
+        arrays = r.to_numpy()
+        for i, (og, ot) in enumerate(oracles):
+            d, p = inst.tree(r, i, arrays)
+            assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent), f"instance {i} batch {b}"
+
+
+def test_repair_after_apply_equals_update(dg):
+    rng = np.random.default_rng(22)
+    n, m = 3000, 30000
+    src, dst, w = rng.integers(0, n, m), rng.integers(0, n, m), rng.integers(1, 50, m)
+    g1 = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    g2 = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    r1, r2 = dg.sssp.compute(g1, 3), dg.sssp.compute(g2, 3)
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    ot, _ = oracle.compute(og, 3)
+    for b in range(3):
+        x = _instance_batch(rng, og, n + 2, 0.02, 50)  # a little vertex growth too
+        og, _, c = oracle.update(og, ot, x[0], x[1], x[2].reshape(-1, 1), x[3], x[4])
+        batch = dg.EdgeBatch(insert=(x[0], x[1], x[2]), delete=(x[3], x[4]))
+        st1 = dg.sssp.update(g1, batch, r1)
+        g2.apply(batch)
+        with pytest.raises(dg.StaleResultError):
+            dg.sssp.update(g2, dg.EdgeBatch(), r2)  # one apply behind: update refuses
+        st2 = dg.sssp.repair(g2, r2)
+        for r, st in ((r1, st1), (r2, st2)):
+            d, p = r.to_numpy()
+            assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent), f"batch {b}"
+            assert st.invalidated == c.invalidated and st.affected == c.affected
+    with pytest.raises(dg.StaleResultError):
+        dg.sssp.repair(g2, r2)  # already current
+
+
+def test_graph_launch_replays_match_plain_launches(dg):
+    """The engine's launch as a hipGraph, replayed for every update (with other eager work and
+    synchronizations in between), gives the oracle's tree every time."""
+    rng = np.random.default_rng(23)
+    n, m = 4000, 48000
+    src, dst, w = rng.integers(0, n, m), rng.integers(0, n, m), rng.integers(1, 60, m)
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    ot, _ = oracle.compute(og, 0)
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    hr = dg.sssp.compute(hg, 0)
+    hr.set_tuning(dg.sssp.Tuning(launch="graph"))
+    other = dg.Graph.from_edges(src, dst, w, num_vertices=n)  # eager work between the replays
+    ro = dg.sssp.compute(other, 1)
+    for b in range(5):
+        x = _instance_batch(rng, og, n, 0.02, 60)
+        og, _, c = oracle.update(og, ot, x[0], x[1], x[2].reshape(-1, 1), x[3], x[4])
+        st = dg.sssp.update(hg, dg.EdgeBatch(insert=(x[0], x[1], x[2]), delete=(x[3], x[4])), hr)
+        d, p = hr.to_numpy()
+        assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent), f"replay {b}"
+        assert st.invalidated == c.invalidated and st.affected == c.affected
+        dg.sssp.update(other, dg.EdgeBatch(insert=(x[1], x[0], x[2])), ro)
+        dg.synchronize()
+
+
+def test_autotune_keeps_the_result(dg):
+    rng = np.random.default_rng(24)
+    n, m = 5000, 80000
+    src, dst, w = rng.integers(0, n, m), rng.integers(0, n, m), rng.integers(1, 100, m)
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    ot, _ = oracle.compute(og, 0)
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    hr = dg.sssp.compute(hg, 0)
+    x = _instance_batch(rng, og, n, 0.02, 100)
+    og, _, _ = oracle.update(og, ot, x[0], x[1], x[2].reshape(-1, 1), x[3], x[4])
+    rep = dg.batched.autotune(hg, dg.EdgeBatch(insert=(x[0], x[1], x[2]), delete=(x[3], x[4])), hr)
+    assert len(rep.table) == len(dg.batched.candidate_tunings(1)) and rep.engine_ms > 0
+    d, p = hr.to_numpy()
+    assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent)
+    # the tuned schedule keeps giving the oracle's tree
+    x = _instance_batch(rng, og, n, 0.02, 100)
+    og, _, _ = oracle.update(og, ot, x[0], x[1], x[2].reshape(-1, 1), x[3], x[4])
+    dg.sssp.update(hg, dg.EdgeBatch(insert=(x[0], x[1], x[2]), delete=(x[3], x[4])), hr)
+    d, p = hr.to_numpy()
+    assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent)
