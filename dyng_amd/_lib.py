@@ -54,7 +54,7 @@ class Batch(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("engine", C.c_int32), ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("light_degree", C.c_int32), ("chunk_edges", C.c_int32), ("launch", C.c_int32),
-                ("barrier_timeout_us", C.c_int32)]
+                ("recompute", C.c_int32), ("barrier_timeout_us", C.c_int32)]
 
 
 COUNTER_FIELDS = ("invalidated", "iterations", "epochs", "pushes", "affected")
@@ -109,8 +109,18 @@ EXPORTS = (
     "dyng_hip_result_options", "dyng_hip_result_set_options", "dyng_hip_result_download",
     "dyng_hip_result_set_tuning", "dyng_hip_device_count", "dyng_hip_malloc", "dyng_hip_free",
     "dyng_hip_memcpy_htod", "dyng_hip_memcpy_dtoh", "dyng_hip_synchronize", "dyng_hip_api_compute_multi",
-    "dyng_hip_api_repair",
+    "dyng_hip_api_repair", "dyng_hip_batch_features", "dyng_hip_mlp_create", "dyng_hip_mlp_destroy",
+    "dyng_hip_mlp_get_params", "dyng_hip_mlp_set_params", "dyng_hip_mlp_gradients", "dyng_hip_mlp_train",
+    "dyng_hip_mlp_predict",
 )
+
+MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_TRAINED, MLP_PARAMS = 12, 64, 5, 5317, 5341
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("num_ins", C.c_int64), ("num_del", C.c_int64), ("tree_deletions", C.c_int64),
+                ("touched", C.c_int64), ("mean_log_degree", C.c_double), ("max_degree", C.c_int64),
+                ("num_vertices", C.c_int64), ("num_edges", C.c_int64)]
 
 _lib = None
 
@@ -148,6 +158,14 @@ def lib() -> C.CDLL:
     L.dyng_hip_api_update.argtypes = [vp, C.POINTER(Batch), vp, vp, C.POINTER(Stats)]
     L.dyng_hip_api_compute_multi.argtypes = [vp, vp, C.c_int64, C.POINTER(Options), vp, C.POINTER(vp)]
     L.dyng_hip_api_repair.argtypes = [vp, vp, vp, C.POINTER(Stats)]
+    L.dyng_hip_batch_features.argtypes = [vp, C.POINTER(Batch), vp, C.c_int64, vp, C.POINTER(BatchStats)]
+    L.dyng_hip_mlp_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
+    L.dyng_hip_mlp_destroy.argtypes = [vp]
+    L.dyng_hip_mlp_get_params.argtypes = [vp, vp, C.c_int64]
+    L.dyng_hip_mlp_set_params.argtypes = [vp, vp, C.c_int64]
+    L.dyng_hip_mlp_gradients.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
+    L.dyng_hip_mlp_train.argtypes = [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, vp]
+    L.dyng_hip_mlp_predict.argtypes = [vp, vp, C.c_int64, vp]
     L.dyng_hip_api_from_arrays.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, C.POINTER(Options), vp,
                                            C.POINTER(vp)]
     L.dyng_hip_result_destroy.argtypes = [vp]

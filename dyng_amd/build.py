@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdyng_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["graph.hip", "sssp.hip", "api.cpp"]
+SOURCES = ["graph.hip", "sssp.hip", "scheduler.hip", "api.cpp"]
 ARCH = os.environ.get("DYNG_HIP_ARCH", "gfx950")
 
 
@@ -72,7 +72,7 @@ def build_copy_probe() -> str:
     return COPY_OUT
 
 
-def build(jobs: int = 3) -> str:
+def build(jobs: int = 4) -> str:
     build_generators()
     build_copy_probe()
     os.makedirs(BUILD, exist_ok=True)
@@ -89,7 +89,7 @@ def build(jobs: int = 3) -> str:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--jobs", type=int, default=3)
+    ap.add_argument("--jobs", type=int, default=4)
     args = ap.parse_args()
     print(build(args.jobs))
     return 0

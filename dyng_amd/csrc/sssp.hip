@@ -1043,7 +1043,10 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   p.source = run->source;
   p.sources = run->sources;
   p.num_sources = run->num_sources;
-  p.from_scratch = update ? 0 : 1;
+  // an update may recompute from the sources instead of repairing (the same tree; `affected`
+  // still counts the changed pairs against the old tree)
+  const bool recompute = update && run->tuning.recompute != 0;
+  p.from_scratch = update && !recompute ? 0 : 1;
   p.count_changes = update ? 1 : 0;
   p.check_cycles = run->check_cycles;
   p.pk = pk;

@@ -59,6 +59,7 @@ class Tuning:
     light_degree: int = 0
     chunk_edges: int = 0
     launch: str = "plain"  # "plain" or "graph" (the launch built once as a hipGraph and replayed)
+    recompute: bool = False  # updates recompute the tree from scratch instead of repairing it
     barrier_timeout_us: int = 0
 
     def _native(self) -> _lib.Tuning:
@@ -66,7 +67,8 @@ class Tuning:
         if self.launch not in launches:
             raise InvalidArgumentError(f"Tuning.launch must be 'plain' or 'graph', got {self.launch!r}")
         return _lib.Tuning(_lib.ENGINES[self.engine], 0, int(self.blocks_per_cu), int(self.light_degree),
-                           int(self.chunk_edges), launches[self.launch], int(self.barrier_timeout_us))
+                           int(self.chunk_edges), launches[self.launch], int(bool(self.recompute)),
+                           int(self.barrier_timeout_us))
 
 
 @dataclass(frozen=True)

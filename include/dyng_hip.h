@@ -168,6 +168,8 @@ typedef struct dyng_hip_tuning {
   int32_t light_degree;    /* rows up to this length: edge-balanced inside a workgroup (0: 4096) */
   int32_t chunk_edges;     /* longer rows: split in chunks of this many edges, one wave each (0: 256) */
   int32_t launch;             /* DYNG_HIP_LAUNCH_*: a plain launch, or a hipGraph replayed per run */
+  int32_t recompute;          /* update: recompute the tree from the source(s) instead of repairing it
+                                 (the same tree; `invalidated` is then 0) */
   int32_t barrier_timeout_us; /* longest wait at a grid barrier before the run fails with
                                  DYNG_HIP_DEVICE_ERROR_TIMEOUT (0: 4 s; < 0: no wait at all, so
                                  the run fails at its first barrier: exercises the error path) */
@@ -292,6 +294,50 @@ int dyng_hip_result_set_options(dyng_hip_result* r, const dyng_hip_options* opt)
 int dyng_hip_result_download(const dyng_hip_result* r, int64_t* distances, int32_t* parents);
 /* Schedule knobs of this result's runs (not part of sssp::options). */
 int dyng_hip_result_set_tuning(dyng_hip_result* r, const dyng_hip_tuning* tuning);
+
+/* ============================================================================================
+ * (3) The learned scheduler (no counterpart in the reference): batch features and a
+ *     12 -> 64 -> 64 -> 5 MLP (ReLU) with FP32 master weights, trained on the device (FP32
+ *     forward, backward, Adam) and evaluated in BF16. It predicts log2 of the time of five
+ *     schedules of an update; every schedule gives the same tree.
+ * ==========================================================================================*/
+
+#define DYNG_HIP_MLP_INPUTS 12
+#define DYNG_HIP_MLP_HIDDEN 64
+#define DYNG_HIP_MLP_OUTPUTS 5
+#define DYNG_HIP_MLP_TRAINED 5317 /* W1 64x12, b1, W2 64x64, b2, W3 5x64, b3 */
+#define DYNG_HIP_MLP_PARAMS 5341  /* the trained ones, then x_mean[12], x_scale[12] */
+
+/* What a batch looks like on G_t, before it is applied (the scheduler's raw features). */
+typedef struct dyng_hip_batch_stats {
+  int64_t num_ins, num_del;
+  int64_t tree_deletions; /* deletions (u, v) with parent[v] == u in the given tree */
+  int64_t touched;        /* operations whose tail vertex exists */
+  double mean_log_degree; /* mean of log2(1 + out-degree) over those tails */
+  int64_t max_degree;
+  int64_t num_vertices, num_edges;
+} dyng_hip_batch_stats;
+
+/* `parents`: the tree's device parent array of n_tree entries (or null). */
+int dyng_hip_batch_features(const dyng_hip_graph* g, const dyng_hip_batch* batch, const int32_t* parents,
+                            int64_t n_tree, void* stream, dyng_hip_batch_stats* out);
+
+typedef struct dyng_hip_mlp dyng_hip_mlp;
+int dyng_hip_mlp_create(int device, uint64_t seed, dyng_hip_mlp** out);
+int dyng_hip_mlp_destroy(dyng_hip_mlp* m);
+/* All DYNG_HIP_MLP_PARAMS floats, host memory. */
+int dyng_hip_mlp_get_params(const dyng_hip_mlp* m, float* params, int64_t count);
+int dyng_hip_mlp_set_params(dyng_hip_mlp* m, const float* params, int64_t count);
+/* FP32 forward + backward of the mean squared error over `count` samples (x: count x 12,
+ * t: count x 5, host): the DYNG_HIP_MLP_TRAINED gradients and the loss. */
+int dyng_hip_mlp_gradients(dyng_hip_mlp* m, const float* x, const float* t, int64_t count, float* grad,
+                           float* loss);
+/* Adam (lr, 0.9, 0.999, 1e-8) over `steps` minibatches of `batch` samples drawn without
+ * replacement (reshuffled per pass); the final loss over all samples (FP32). */
+int dyng_hip_mlp_train(dyng_hip_mlp* m, const float* x, const float* t, int64_t count, int32_t steps,
+                       int32_t batch, float lr, uint64_t seed, float* final_loss);
+/* BF16 inference: y (count x 5) for x (count x 12), host memory. */
+int dyng_hip_mlp_predict(dyng_hip_mlp* m, const float* x, int64_t count, float* y);
 
 #ifdef __cplusplus
 }

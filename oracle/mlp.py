@@ -1,0 +1,88 @@
+"""CPU oracle of the learned scheduler's network (test infrastructure only: the tests use it to
+check the HIP training and inference kernels; nothing in the product imports it).
+
+A 12 -> 64 -> 64 -> 5 multilayer perceptron with ReLU hidden layers, the parameter layout of
+``dyng_hip.h`` (DYNG_HIP_MLP_*): W1[64][12], b1[64], W2[64][64], b2[64], W3[5][64], b3[5], then the
+input normalization x_mean[12], x_scale[12] (x_n = (x - mean) * scale). The loss is the mean
+squared error over all outputs of a minibatch. The reference project has no such network, so
+nothing here restates reference code; the backward pass is pinned by finite differences
+(tests/test_oracle.py) and the BF16 forward by rounding each stage as IEEE bfloat16 does (round
+to nearest, ties to even, on the upper 16 bits of binary32).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+IN, HIDDEN, OUT = 12, 64, 5
+W1, B1 = 0, HIDDEN * IN
+W2 = B1 + HIDDEN
+B2 = W2 + HIDDEN * HIDDEN
+W3 = B2 + HIDDEN
+B3 = W3 + OUT * HIDDEN
+TRAINED = B3 + OUT
+MEAN = TRAINED
+SCALE = MEAN + IN
+PARAMS = SCALE + IN
+
+
+def unpack(p: np.ndarray):
+    p = np.asarray(p)
+    return (p[W1:B1].reshape(HIDDEN, IN), p[B1:W2], p[W2:B2].reshape(HIDDEN, HIDDEN), p[B2:W3],
+            p[W3:B3].reshape(OUT, HIDDEN), p[B3:TRAINED], p[MEAN:SCALE], p[SCALE:PARAMS])
+
+
+def forward(p: np.ndarray, x: np.ndarray, dtype=np.float64):
+    """(y, cache) for inputs x (B x 12)."""
+    w1, b1, w2, b2, w3, b3, mean, scale = (a.astype(dtype) for a in unpack(p))
+    xn = (np.asarray(x, dtype) - mean) * scale
+    z1 = xn @ w1.T + b1
+    h1 = np.maximum(z1, 0)
+    z2 = h1 @ w2.T + b2
+    h2 = np.maximum(z2, 0)
+    y = h2 @ w3.T + b3
+    return y, (xn, h1, h2)
+
+
+def loss(p, x, t, dtype=np.float64) -> float:
+    y, _ = forward(p, x, dtype)
+    return float(np.mean((y - np.asarray(t, dtype)) ** 2))
+
+
+def gradients(p: np.ndarray, x: np.ndarray, t: np.ndarray, dtype=np.float64) -> np.ndarray:
+    """d loss / d (trained parameters), in the parameter layout."""
+    w1, b1, w2, b2, w3, b3, mean, scale = (a.astype(dtype) for a in unpack(p))
+    y, (xn, h1, h2) = forward(p, x, dtype)
+    B = y.shape[0]
+    dy = 2.0 * (y - np.asarray(t, dtype)) / (B * OUT)
+    g = np.zeros(TRAINED, dtype)
+    g[W3:B3] = (dy.T @ h2).reshape(-1)
+    g[B3:TRAINED] = dy.sum(0)
+    dh2 = (dy @ w3) * (h2 > 0)
+    g[W2:B2] = (dh2.T @ h1).reshape(-1)
+    g[B2:W3] = dh2.sum(0)
+    dh1 = (dh2 @ w2) * (h1 > 0)
+    g[W1:B1] = (dh1.T @ xn).reshape(-1)
+    g[B1:W2] = dh1.sum(0)
+    return g
+
+
+def bf16(a) -> np.ndarray:
+    """Round binary32 values to bfloat16 (nearest, ties to even), returned as binary32."""
+    f = np.ascontiguousarray(np.asarray(a, np.float32))
+    bits = f.view(np.uint32).astype(np.uint64)
+    nan = (bits & 0x7F800000) == 0x7F800000
+    nan &= (bits & 0x007FFFFF) != 0
+    rounded = ((bits + 0x7FFF + ((bits >> 16) & 1)) >> 16) << 16
+    out = np.where(nan, (bits | 0x400000) & 0xFFFF0000, rounded).astype(np.uint32)
+    return out.view(np.float32)
+
+
+def forward_bf16(p: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """The BF16 inference of the kernels: weights, the normalized input and each hidden layer
+    rounded to bfloat16, every dot product accumulated in binary32."""
+    w1, b1, w2, b2, w3, b3, mean, scale = unpack(np.asarray(p, np.float32))
+    xn = bf16((np.asarray(x, np.float32) - mean) * scale)
+    h1 = bf16(np.maximum(xn @ bf16(w1).T + bf16(b1), 0).astype(np.float32))
+    h2 = bf16(np.maximum(h1 @ bf16(w2).T + bf16(b2), 0).astype(np.float32))
+    return (h2 @ bf16(w3).T + bf16(b3)).astype(np.float32)

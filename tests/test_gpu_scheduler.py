@@ -1,0 +1,182 @@
+"""The learned scheduler on the GPU: its network's HIP kernels against the CPU oracle of the
+network (FP32 gradients, BF16 inference), training, bit-exact updates under every schedule it
+picks, and what the trained network gains over the plain fixed schedule. Needs an MI355X."""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dg():
+    import dyng_amd
+
+    dyng_amd._lib.lib()
+    assert dyng_amd.device_count() > 0, "no HIP device visible"
+    return dyng_amd
+
+
+def _random_params(rng):
+    p = rng.normal(0, 0.25, mlp.PARAMS).astype(np.float32)
+    p[mlp.MEAN:mlp.SCALE] = rng.normal(0, 1, mlp.IN)
+    p[mlp.SCALE:] = rng.uniform(0.5, 2.0, mlp.IN)
+    return p
+
+
+def test_network_gradients_match_the_oracle(dg):
+    _say("[mlp] gradients")
+    rng = np.random.default_rng(31)
+    net = dg.scheduler.Network(seed=5)
+    p = _random_params(rng)
+    net.params = p
+    assert np.array_equal(net.params, p)
+    for B in (1, 7, 300):
+        x = rng.normal(0, 2, (B, mlp.IN)).astype(np.float32)
+        t = rng.normal(0, 1, (B, mlp.OUT)).astype(np.float32)
+        g, loss = net.gradients(x, t)
+        want = mlp.gradients(p.astype(np.float64), x, t)
+        # FP32 sums in another order than the FP64 oracle: relative 1e-4 of the largest gradient
+        tol = 1e-4 * float(np.max(np.abs(want))) + 1e-7
+        assert np.max(np.abs(g - want)) <= tol, (B, float(np.max(np.abs(g - want))), tol)
+        assert abs(loss - mlp.loss(p.astype(np.float64), x, t)) <= 1e-5 * (1 + loss)
+
+
+def test_network_bf16_inference_matches_the_oracle(dg):
+    _say("[mlp] bf16 inference")
+    rng = np.random.default_rng(32)
+    net = dg.scheduler.Network(seed=6)
+    p = _random_params(rng)
+    net.params = p
+    x = rng.normal(0, 2, (200, mlp.IN)).astype(np.float32)
+    y = net.predict(x)
+    want = mlp.forward_bf16(p, x)
+    y64, _ = mlp.forward(p.astype(np.float64), x)
+    # the same BF16 rounding points as the oracle; FP32 summation order may move a hidden unit by
+    # one BF16 step, so the bound is a few BF16 steps of the output's scale
+    assert np.max(np.abs(y - want)) <= 0.02 * (1 + np.max(np.abs(want)))
+    assert np.max(np.abs(y - y64)) <= 0.05 * (1 + np.max(np.abs(y64)))
+
+
+def test_network_training_fits_a_known_function(dg):
+    _say("[mlp] training")
+    rng = np.random.default_rng(33)
+    x = rng.normal(0, 1, (2000, mlp.IN)).astype(np.float32)
+    t = np.stack([np.sin(x[:, 0]) + x[:, 1], x[:, 2] * x[:, 3], np.abs(x[:, 4]), x[:, 5] - x[:, 6], x[:, 7] ** 2],
+                 axis=1).astype(np.float32)
+    net = dg.scheduler.Network(seed=7)
+    net.normalize_inputs(x)
+    _, before = net.gradients(x, t)
+    after = net.train(x, t, steps=1500, batch=64, lr=3e-3, seed=2)
+    _say(f"[mlp] loss {before:.4f} -> {after:.4f}")
+    assert after < 0.2 * before
+    # the trained weights: BF16 inference close to the FP32 oracle with them
+    p = net.params
+    y = net.predict(x[:100])
+    y64, _ = mlp.forward(p.astype(np.float64), x[:100])
+    assert np.max(np.abs(y - y64)) <= 0.05 * (1 + np.max(np.abs(y64)))
+
+
+def test_every_schedule_gives_the_oracle_tree(dg):
+    """Updates under each of the five schedules (and under a scheduler with an untrained network,
+    which picks them arbitrarily) equal the oracle after every batch."""
+    _say("[scheduler] schedules")
+    rng = np.random.default_rng(34)
+    n, m = 6000, 70000
+    src, dst, w = rng.integers(0, n, m), rng.integers(0, n, m), rng.integers(1, 256, m)
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    ot, _ = oracle.compute(og, 0)
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    hr = dg.sssp.compute(hg, 0)
+    sched = dg.scheduler.Scheduler(dg.scheduler.Network(seed=11))
+    sched.network.params = _random_params(rng)
+    for b in range(10):
+        size = int(10 ** rng.uniform(1, 4))
+        nd = int(size * rng.uniform(0, 1))
+        rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+        pick = rng.choice(og.m, nd, replace=False)
+        x = (rng.integers(0, n, size - nd), rng.integers(0, n, size - nd), rng.integers(1, 256, size - nd),
+             rows[pick], og.col[pick].copy())
+        og, _, c = oracle.update(og, ot, x[0], x[1], x[2].reshape(-1, 1), x[3], x[4])
+        batch = dg.EdgeBatch(insert=(x[0], x[1], x[2]), delete=(x[3], x[4]))
+        if b < 5:
+            dg.scheduler.use(hr, b, dg.scheduler.default_delta(hg))
+            st = dg.sssp.update(hg, batch, hr)
+        else:
+            st = sched.update(hg, batch, hr)
+        d, p = hr.to_numpy()
+        assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent), f"batch {b}"
+        assert st.affected == c.affected
+    _say(f"[scheduler] picked {sched.choices}")
+
+
+def test_trained_scheduler_against_the_plain_schedule(dg):
+    """Train on traces of one stream, then on a held-out stream (other seeds) compare the
+    schedules the network picks with the plain fixed schedule, using the held-out stream's own
+    measured times of every schedule; the trees stay bit-exact with the oracle throughout."""
+    _say("[scheduler] train + held out")
+    from dyng_amd import generators as G
+
+    scale, n_edges = 16, 16 * 65536
+    n = 1 << scale
+
+    def stream(seed, count):
+        rng = np.random.default_rng(seed)
+        src, dst, w = G.rmat(scale, n_edges, 255, seed)
+        og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+        hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+        batches = []
+        cur = og
+        for _ in range(count):
+            size = int(10 ** rng.uniform(1, 5))
+            nd = min(int(size * rng.uniform(0, 1)), cur.m)
+            rows = np.repeat(np.arange(cur.n), np.diff(cur.row_ptr))
+            pick = rng.choice(cur.m, nd, replace=False)
+            b = (rng.integers(0, n, size - nd), rng.integers(0, n, size - nd), rng.integers(1, 256, size - nd),
+                 rows[pick], cur.col[pick].copy())
+            cur, _, _ = oracle.update(cur, oracle.compute(cur, 0)[0], b[0], b[1], b[2].reshape(-1, 1), b[3], b[4])
+            batches.append(b)
+        return og, hg, batches
+
+    og, hg, batches = stream(101, 40)
+    hr = dg.sssp.compute(hg, 0)
+    trace = dg.scheduler.collect(hg, hr, (dg.EdgeBatch(insert=b[:3], delete=b[3:]) for b in batches))
+    sched = dg.scheduler.Scheduler()
+    loss = sched.fit(trace, steps=2000)
+    _say(f"[scheduler] fit loss {loss:.3f} on {len(trace.x)} traces")
+
+    og, hg, batches = stream(202, 25)
+    ot, _ = oracle.compute(og, 0)
+    hr = dg.sssp.compute(hg, 0)
+    held = dg.scheduler.Trace()
+    history = dg.scheduler.History()
+    picked, plain, learned, best = [], 0.0, 0.0, 0.0
+    for b in batches:
+        batch = dg.EdgeBatch(insert=b[:3], delete=b[3:])
+        x = dg.scheduler.features(dg.scheduler.batch_stats(hg, batch, hr), history)
+        choice = int(np.argmin(sched.network.predict(x[None, :])[0]))
+        dg.scheduler.collect(hg, hr, [batch], held)  # times every schedule, then repairs hr
+        og, _, c = oracle.update(og, ot, b[0], b[1], b[2].reshape(-1, 1), b[3], b[4])
+        d, p = hr.to_numpy()
+        assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent)
+        history = dg.scheduler.History(c.invalidated, c.affected)
+        times = 2.0 ** np.asarray(held.t[-1])
+        picked.append(choice)
+        plain += times[0]
+        learned += times[choice]
+        best += times.min()
+    _say(f"[scheduler] held out: plain {plain:.2f} ms, learned {learned:.2f} ms, best possible {best:.2f} ms, "
+         f"gain {plain / learned:.3f}x, picks {np.bincount(picked, minlength=5).tolist()}")
+    assert learned <= 1.05 * plain

@@ -1,0 +1,62 @@
+"""The CPU oracle of the scheduler's network, pinned by mathematics: finite differences for the
+backward pass and the IEEE bfloat16 rounding rule for the BF16 forward pass."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import mlp
+
+
+def _params(rng):
+    p = rng.normal(0, 0.3, mlp.PARAMS)
+    p[mlp.MEAN:mlp.SCALE] = rng.normal(0, 1, mlp.IN)
+    p[mlp.SCALE:] = rng.uniform(0.5, 2.0, mlp.IN)
+    return p
+
+
+def test_gradients_match_finite_differences():
+    rng = np.random.default_rng(1)
+    p = _params(rng)
+    x = rng.normal(0, 2, (9, mlp.IN))
+    t = rng.normal(0, 1, (9, mlp.OUT))
+    g = mlp.gradients(p, x, t)
+    h = 1e-6
+    for q in rng.choice(mlp.TRAINED, 60, replace=False).tolist() + [mlp.B1, mlp.B2, mlp.B3, mlp.TRAINED - 1]:
+        e = np.zeros(mlp.PARAMS)
+        e[q] = h
+        fd = (mlp.loss(p + e, x, t) - mlp.loss(p - e, x, t)) / (2 * h)
+        assert abs(fd - g[q]) <= 1e-6 + 1e-5 * abs(fd), (q, fd, g[q])
+
+
+def test_planted_gradient_mistake_is_caught():
+    """A backward pass that forgets the ReLU mask disagrees with finite differences."""
+    rng = np.random.default_rng(2)
+    p = _params(rng)
+    x = rng.normal(0, 2, (6, mlp.IN))
+    t = rng.normal(0, 1, (6, mlp.OUT))
+    w1, b1, w2, b2, w3, b3, mean, scale = mlp.unpack(p)
+    y, (xn, h1, h2) = mlp.forward(p, x)
+    dy = 2.0 * (y - t) / (6 * mlp.OUT)
+    wrong = ((dy @ w3).T @ h1).reshape(-1)  # no mask on dh2
+    right = mlp.gradients(p, x, t)[mlp.W2:mlp.B2]
+    assert not np.allclose(wrong, right)
+
+
+def test_bf16_rounding_rule():
+    one = np.float32(1.0)
+    half_ulp = np.float32(1.0 + 2.0 ** -8)          # exactly between 1 and 1 + 2^-7: ties to even -> 1
+    above = np.float32(1.0 + 2.0 ** -8 + 2.0 ** -20)  # just above the tie -> up
+    odd_tie = np.float32(1.0 + 2.0 ** -7 + 2.0 ** -8)  # tie with an odd lower neighbour -> up to 1 + 2^-6
+    got = mlp.bf16([one, half_ulp, above, odd_tie, -3.0, 0.0])
+    assert got.tolist() == [1.0, 1.0, 1.0 + 2.0 ** -7, 1.0 + 2.0 ** -6, -3.0, 0.0]
+    assert np.isnan(mlp.bf16([np.nan]))[0]
+
+
+def test_bf16_forward_is_close_to_fp64():
+    rng = np.random.default_rng(3)
+    p = _params(rng)
+    x = rng.normal(0, 2, (50, mlp.IN))
+    y64, _ = mlp.forward(p, x)
+    yb = mlp.forward_bf16(p, x)
+    assert np.max(np.abs(yb - y64)) <= 0.05 * (1 + np.max(np.abs(y64)))
