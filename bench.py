@@ -40,6 +40,9 @@ WORKLOADS = {
     # batch is drawn against the current tree: (insertions, tree-edge deletions, random grid-arc
     # deletions), so 80 % of the deletions hit tree edges
     "grid4096_tree10k": ("grid", 4096, None, 1000, 2, 10, (5_000, 4_000, 1_000)),
+    # RMAT-23 (8,388,608 vertices, 134,217,728 edges) and a stream of batches of log-uniform size
+    # in [10, 10^6] with a uniform delete fraction: the learned scheduler against the plain one
+    "rmat23_learned": ("stream", 23, 134_217_728, 255, 4, 2000, (10, 1_000_000, 0)),
 }
 
 
@@ -127,6 +130,89 @@ def cpu_baseline(n, src, dst, w, batch, source, sample: int, threads: int):
             trees, drawn)
 
 
+def learned_vs_plain(n, src, dst, w, source, train, evaluate, log_prefix="learned"):
+    """Train the scheduler on traces of the `train` batches (held-out seeds) on one device graph,
+    then run the `evaluate` batches through two more graphs in lockstep, one with the plain fixed
+    schedule and one with the scheduler, comparing the two trees on the device after every batch.
+    Update times are the wall time of each sssp.update call (the scheduler's feature kernel and
+    BF16 inference included)."""
+    import dyng_amd
+    from dyng_amd import scheduler, sssp
+
+    D = dyng_amd.DeviceArray
+
+    def dev(x):
+        return dyng_amd.EdgeBatch(insert=(D(x[0]), D(x[1]), D(x[2])), delete=(D(x[3]), D(x[4])))
+
+    t0 = time.perf_counter()
+    gt = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
+    rt = sssp.compute(gt, source)
+    trace = scheduler.collect(gt, rt, (dev(x) for x in train), repeats=2)
+    del rt, gt
+    sched = scheduler.Scheduler()
+    loss = sched.fit(trace, steps=3000)
+    tx, tt = trace.arrays()
+    best_possible = float(np.sum(2.0 ** tt.min(1)) / np.sum(2.0 ** tt[:, 0]))
+    log(f"[bench] {log_prefix}: {len(train)} traces, fit loss {loss:.4f}, best choice per trace would take "
+        f"{best_possible:.3f}x of the plain time, in {time.perf_counter() - t0:.1f}s")
+    gp = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
+    gl = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
+    rp = sssp.compute(gp, source)
+    rl = sssp.compute(gl, source)
+    plain_ms, learned_ms, diffs = [], [], 0
+    ops = 0
+    for x in evaluate:
+        eb = dev(x)
+        ops += len(x[0]) + len(x[3])
+        dyng_amd.synchronize()
+        t1 = time.perf_counter()
+        sssp.update(gp, eb, rp)
+        plain_ms.append((time.perf_counter() - t1) * 1e3)
+        t1 = time.perf_counter()
+        sched.update(gl, eb, rl)
+        learned_ms.append((time.perf_counter() - t1) * 1e3)
+        diffs += rp.differences(rl)
+        del eb
+    picks = np.bincount(sched.choices, minlength=len(scheduler.CHOICES)).tolist()
+    out = {"batches": len(evaluate), "ops": ops, "plain_ms_total": round(sum(plain_ms), 3),
+           "learned_ms_total": round(sum(learned_ms), 3),
+           "speedup": round(sum(plain_ms) / max(sum(learned_ms), 1e-9), 4),
+           "plain_p50_ms": round(statistics.median(plain_ms), 4), "learned_p50_ms": round(statistics.median(learned_ms), 4),
+           "picks": dict(zip(scheduler.CHOICES, picks)), "train_traces": len(train),
+           "best_choice_per_trace_vs_plain": round(best_possible, 4),
+           "bit_exact": diffs == 0, "compared": "plain and learned trees on the device after every batch"}
+    log(f"[bench] {log_prefix}: plain {out['plain_ms_total']:.1f} ms, learned {out['learned_ms_total']:.1f} ms, "
+        f"speedup {out['speedup']:.3f}x, picks {out['picks']}, bit-exact {out['bit_exact']}")
+    del rp, rl, gp, gl
+    return out
+
+
+def stream_bench(args) -> int:
+    """The learned-scheduler configuration: RMAT-23 and a stream of uneven batches."""
+    from dyng_amd import generators as G
+
+    kind, scale, m, wmax, cfg, nb, (lo, hi, _) = WORKLOADS[args.workload]
+    n = 1 << scale
+    t0 = time.perf_counter()
+    src, dst, w = G.rmat(scale, m, wmax, G.GRAPH_SEED ^ cfg)
+    source = G.max_out_degree_vertex(src, n)
+    count = args.stream_batches or nb
+    train = G.stream(src, dst, n, args.train_batches, lo, hi, wmax, (G.BATCH_SEED ^ cfg) ^ 0x7EA1, scale)
+    evaluate = G.stream(src, dst, n, count, lo, hi, wmax, G.BATCH_SEED ^ cfg, scale)
+    log(f"[bench] inputs {args.workload}: n={n} m={m} train {len(train)} + evaluate {count} batches in "
+        f"{time.perf_counter() - t0:.1f}s")
+    res = learned_vs_plain(n, src, dst, w, source, train, evaluate)
+    out = {"metric": METRIC, "value": round(res["ops"] / (res["learned_ms_total"] / 1e3), 1), "unit": "edges/s",
+           "n_gpus": 1, "steps": count, "warmup": 0,
+           "ms_per_step": round(res["learned_ms_total"] / count, 4), "p50_ms": res["learned_p50_ms"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic",
+           "config": {"workload": args.workload, "vertices": n, "edges": m, "batch_ops": f"log-uniform [{lo}, {hi}]",
+                      "source": int(source)},
+           "learned_vs_plain": res, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+    return 0 if res["bit_exact"] else 3
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +222,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=3, help="batches timed on the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core of this host")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
+    ap.add_argument("--train-batches", type=int, default=120, help="trace batches the scheduler trains on")
+    ap.add_argument("--stream-batches", type=int, default=0, help="stream workloads: evaluated batches")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (the path does "
@@ -148,6 +237,8 @@ def main() -> int:
     if dyng_amd.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+    if WORKLOADS[args.workload][0] == "stream":
+        return stream_bench(args)
     K, W = args.steps, args.warmup
     n, src, dst, w, batch, source, ops = make_inputs(args.workload, K + W)
 
@@ -226,6 +317,16 @@ def main() -> int:
             log(f"[bench] parity batch {b}: {'bit-exact' if same else 'MISMATCH'}")
         exact = {"batches_checked": args.cpu_sample, "bit_exact": ok, "against": "OpenMP engine (oracle)"}
 
+    # ---- the learned scheduler against the plain schedule on the same batches ----
+    learned = None
+    if not args.no_learned and not batch.dynamic:
+        from dyng_amd import generators as G
+
+        kind, size, _, wmax, cfg, _, (ni, nd, nr) = WORKLOADS[args.workload]
+        train = G.mixed_batches(src, dst, n, 16, ni, nd, nr, wmax, (G.BATCH_SEED ^ cfg) ^ 0x7EA1,
+                                rmat_scale=size if kind == "rmat" else 0)
+        learned = learned_vs_plain(n, src, dst, w, source, train, [batch(b, None) for b in range(W + K)])
+
     s0 = stats[0].device
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "edges/s", "n_gpus": 1, "steps": K, "warmup": W,
@@ -237,6 +338,7 @@ def main() -> int:
                    "engine_ms_p50": round(statistics.median(s.engine_ms for s in stats), 4),
                    "blocks": int(s0.blocks)},
         "roofline": roofline, "relax_phase": relax_phase, "cpu_baseline": cpu, "parity": exact,
+        "learned_vs_plain": learned,
     }
     print(json.dumps(out), flush=True)
     # release every device object before the interpreter (and a profiler's runtime) shuts down
@@ -247,7 +349,8 @@ def main() -> int:
 
     gc.collect()
     dyng_amd.synchronize()
-    return 0 if (exact is None or exact["bit_exact"]) else 3
+    ok = (exact is None or exact["bit_exact"]) and (learned is None or learned["bit_exact"])
+    return 0 if ok else 3
 
 
 if __name__ == "__main__":

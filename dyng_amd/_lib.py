@@ -111,7 +111,7 @@ EXPORTS = (
     "dyng_hip_memcpy_htod", "dyng_hip_memcpy_dtoh", "dyng_hip_synchronize", "dyng_hip_api_compute_multi",
     "dyng_hip_api_repair", "dyng_hip_batch_features", "dyng_hip_mlp_create", "dyng_hip_mlp_destroy",
     "dyng_hip_mlp_get_params", "dyng_hip_mlp_set_params", "dyng_hip_mlp_gradients", "dyng_hip_mlp_train",
-    "dyng_hip_mlp_predict",
+    "dyng_hip_mlp_predict", "dyng_hip_result_equal", "dyng_hip_api_update_scheduled",
 )
 
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_TRAINED, MLP_PARAMS = 12, 64, 5, 5317, 5341
@@ -166,6 +166,9 @@ def lib() -> C.CDLL:
     L.dyng_hip_mlp_gradients.argtypes = [vp, vp, vp, C.c_int64, vp, vp]
     L.dyng_hip_mlp_train.argtypes = [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, vp]
     L.dyng_hip_mlp_predict.argtypes = [vp, vp, C.c_int64, vp]
+    L.dyng_hip_result_equal.argtypes = [vp, vp, vp, C.POINTER(C.c_int64)]
+    L.dyng_hip_api_update_scheduled.argtypes = [vp, C.POINTER(Batch), vp, vp, C.c_int64, C.c_int64, vp,
+                                                C.POINTER(Stats), C.POINTER(C.c_int32)]
     L.dyng_hip_api_from_arrays.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, C.POINTER(Options), vp,
                                            C.POINTER(vp)]
     L.dyng_hip_result_destroy.argtypes = [vp]

@@ -279,12 +279,20 @@ struct event_pair {
 /// The engine part of an update (sssp_problem::resume + the engine run, sssp.cpp:691-823) for a
 /// result that is one apply behind the graph; a failure poisons the result.
 void repair_core(dyng_hip_graph* g, dyng_hip_result* r, int64_t delta, int64_t max_weight, hipStream_t s,
-                 dyng_hip_counters& c, double& engine_ms) {
+                 dyng_hip_counters& c, double& engine_ms, int choice = 0) {
   event_pair ev;
   try {
     grow_result(r, g->n, s);
     std::lock_guard<std::mutex> lk(pool().lock);
     dyng_hip_sssp_run run = make_run(g, r, delta, max_weight);
+    // the learned scheduler's schedules (they change the time, never the tree)
+    switch (choice) {
+      case 1: run.delta = 4 * delta; break;
+      case 2: run.delta = std::max<int64_t>(1, delta / 4); break;
+      case 3: run.tuning.light_degree = 256; run.tuning.chunk_edges = 64; break;
+      case 4: run.tuning.recompute = 1; break;
+      default: break;
+    }
     DYNG_HIP_TRY(hipEventRecord(ev.a, s));
     call(dyng_hip_sssp_update(&run, s, &c));
     DYNG_HIP_TRY(hipEventRecord(ev.b, s));
@@ -454,6 +462,72 @@ extern "C" int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batc
   });
 }
 
+extern "C" int dyng_hip_api_update_scheduled(dyng_hip_graph* g, const dyng_hip_batch* batch, dyng_hip_result* r,
+                                             dyng_hip_mlp* mlp, int64_t last_invalidated, int64_t last_affected,
+                                             void* stream, dyng_hip_stats* stats, int32_t* choice) {
+  return guarded([&] {
+    // ---- begin_update (sssp.cpp:620-640): nothing changes before these checks ----
+    expect_usable(r, "sssp::update");
+    if (g == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: null graph");
+    if (r->graph_state != g->state_id || r->version != g->version) {
+      throw status_error(DYNG_HIP_STALE_RESULT,
+                         "dyng: sssp::update: the result matches graph version " + std::to_string(r->version) +
+                             " but the graph is at version " + std::to_string(g->version) +
+                             " (or it belongs to another graph); recompute it");
+    }
+    if (r->device != g->device) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: the result is on another device");
+    expect_graph(g, r->opt, "sssp::update");
+    if (r->n != g->n) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: the result has " + std::to_string(r->n) +
+                                          " vertices but the graph has " + std::to_string(g->n));
+    }
+    scoped_device guard(g->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int k = r->opt.objective;
+    // ---- prepare on G_t (sssp.cpp:643-689) ----
+    batch_scan sc;
+    scan_batch(g, batch, s, sc);
+    if (batch->num_ins > 0 && sc.wmin[k] < 1) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: an insertion has a weight below 1 in objective " +
+                                          std::to_string(k) + "; sssp needs weights in [1, 2^31 - 1]");
+    }
+    const int64_t max_weight = std::max<int64_t>(std::max<int64_t>(sc.wmax[k], 1), g->wmax[k]);
+    const int64_t delta =
+        r->opt.delta > 0 ? r->opt.delta : default_delta(std::max<int64_t>(g->m, 1), g->n, g->wsum[k]);
+    if (!distances_fit(sc.n_after, max_weight)) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: distances up to " + std::to_string(max_weight) + " * " +
+                                          std::to_string(sc.n_after - 1) + " do not fit in 62 bits");
+    }
+    // ---- the scheduler's features and inference, on G_t before anything changes; the apply's
+    // own synchronization brings the prediction back ----
+    if (mlp != nullptr) mlp_predict_async(mlp, g, sc, r->parents.p, r->n, last_invalidated, last_affected, s);
+    // ---- commit: the device apply ----
+    event_pair ev_apply;
+    dyng_hip_apply_summary summary{};
+    DYNG_HIP_TRY(hipEventRecord(ev_apply.a, s));
+    try {
+      apply_scanned(g, sc, s, &summary);
+    } catch (...) {
+      r->poisoned = true;  // the graph may be partly updated
+      throw;
+    }
+    DYNG_HIP_TRY(hipEventRecord(ev_apply.b, s));
+    // ---- resume + the engine; a failure from here on poisons the result ----
+    dyng_hip_counters c{};
+    double engine_ms = 0.0;
+    const int pick = mlp != nullptr ? mlp_pick(mlp) : 0;
+    if (choice != nullptr) *choice = pick;
+    repair_core(g, r, delta, max_weight, s, c, engine_ms, pick);
+    if (stats != nullptr) {
+      std::memset(stats, 0, sizeof(*stats));
+      fill_stats(stats, c);
+      stats->batch = summary;
+      stats->apply_ms = ev_apply.ms();
+      stats->engine_ms = engine_ms;
+    }
+  });
+}
+
 extern "C" int dyng_hip_api_repair(dyng_hip_graph* g, dyng_hip_result* r, void* stream, dyng_hip_stats* stats) {
   return guarded([&] {
     expect_usable(r, "sssp::repair");
@@ -553,6 +627,52 @@ extern "C" int dyng_hip_api_from_arrays(dyng_hip_graph* g, int32_t source, const
     }
     DYNG_HIP_TRY(hipStreamSynchronize(s));
     *out = r.release();
+  });
+}
+
+namespace {
+
+/// Count the positions where two arrays of the same length differ (one atomic per wave).
+template <typename T>
+__global__ void k_count_differences(const T* a, const T* b, int64_t n, unsigned long long* out) {
+  unsigned long long diff = 0;
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < n; i += int64_t{gridDim.x} * blockDim.x) {
+    diff += a[i] != b[i] ? 1 : 0;
+  }
+  diff = wave_sum(diff);
+  if (lane() == 0 && diff) atomicAdd(out, diff);
+}
+
+}  // namespace
+
+extern "C" int dyng_hip_result_equal(const dyng_hip_result* a, const dyng_hip_result* b, void* stream,
+                                     int64_t* differences) {
+  return guarded([&] {
+    expect_usable(a, "sssp::result::equal");
+    expect_usable(b, "sssp::result::equal");
+    if (differences == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::result::equal: null output");
+    if (a->device != b->device) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::result::equal: the results are on other devices");
+    if (a->n != b->n) {
+      *differences = std::max(a->n, b->n);
+      return;
+    }
+    scoped_device guard(a->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dbuf<unsigned long long> count;
+    count.reserve(1);
+    DYNG_HIP_TRY(hipMemsetAsync(count.p, 0, sizeof(unsigned long long), s));
+    if (a->n > 0) {
+      const unsigned grid = static_cast<unsigned>(std::min<int64_t>((a->n + 255) / 256, 4096));
+      hipLaunchKernelGGL(k_count_differences<int64_t>, dim3(grid), dim3(256), 0, s, a->distances.p, b->distances.p,
+                         a->n, count.p);
+      hipLaunchKernelGGL(k_count_differences<int32_t>, dim3(grid), dim3(256), 0, s, a->parents.p, b->parents.p, a->n,
+                         count.p);
+      DYNG_HIP_TRY(hipGetLastError());
+    }
+    unsigned long long h = 0;
+    DYNG_HIP_TRY(hipMemcpyAsync(&h, count.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    DYNG_HIP_TRY(hipStreamSynchronize(s));
+    *differences = static_cast<int64_t>(h);
   });
 }
 

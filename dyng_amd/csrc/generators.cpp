@@ -238,6 +238,61 @@ int dyng_gen_batches(const int32_t* src, const int32_t* dst, int64_t m, int64_t 
   return 0;
 }
 
+/// A stream of batches of uneven size (the learned-scheduler configuration): batch b has
+/// floor(10^U[log10 lo, log10 hi]) operations, a delete fraction U[0, 1], deletions of distinct
+/// edges of the graph (src, dst sorted; an edge is deleted at most once over the stream, and the
+/// deletions stop when the graph has none left) and insertions of new RMAT pairs (w ~ U[1, wmax];
+/// a pair inserted twice becomes an upsert). Sizes come first (sizes[b] = {insertions,
+/// deletions}, a call with null outputs fills only them); then the operations, batch after batch.
+int64_t dyng_gen_stream(const int32_t* src, const int32_t* dst, int64_t m, int64_t n, int32_t count, double lo,
+                        double hi, int32_t wmax, uint64_t seed, int32_t rmat_scale, double a, double b, double c,
+                        int64_t* sizes, int32_t* ins_src, int32_t* ins_dst, int32_t* ins_w, int32_t* del_src,
+                        int32_t* del_dst) {
+  int64_t left = m, total_ins = 0, total_del = 0;
+  for (int32_t bt = 0; bt < count; ++bt) {
+    const double e = std::log10(lo) + unit(splitmix(seed ^ 0x53495AULL, 2 * bt)) * (std::log10(hi) - std::log10(lo));
+    const int64_t size = std::max<int64_t>(1, static_cast<int64_t>(std::pow(10.0, e)));
+    int64_t nd = static_cast<int64_t>(static_cast<double>(size) * unit(splitmix(seed ^ 0x53495AULL, 2 * bt + 1)));
+    nd = std::min(nd, left);
+    left -= nd;
+    sizes[2 * bt] = size - nd;
+    sizes[2 * bt + 1] = nd;
+    total_ins += size - nd;
+    total_del += nd;
+  }
+  if (ins_src == nullptr) return total_ins + total_del;
+  std::vector<uint64_t> used((static_cast<size_t>(m) + 63) / 64, 0);
+  int64_t di = 0, ii = 0;
+  uint64_t draw = 0, idraw = 0;
+  const uint64_t ins_seed = seed ^ 0x5157ULL;
+  for (int32_t bt = 0; bt < count; ++bt) {
+    for (int64_t j = 0; j < sizes[2 * bt + 1]; ++j) {
+      for (;;) {
+        const int64_t e = static_cast<int64_t>(splitmix(seed ^ 0xDE15ULL, draw++) % static_cast<uint64_t>(m));
+        uint64_t& word = used[static_cast<size_t>(e) / 64];
+        const uint64_t bit = 1ULL << (e % 64);
+        if (word & bit) continue;
+        word |= bit;
+        del_src[di] = src[e];
+        del_dst[di] = dst[e];
+        ++di;
+        break;
+      }
+    }
+    for (int64_t j = 0; j < sizes[2 * bt]; ++j) {
+      uint32_t u, v;
+      do {
+        rmat_pair(ins_seed, idraw++, rmat_scale, a, b, c, u, v);
+      } while (u == v || u >= n || v >= n);
+      ins_src[ii] = static_cast<int32_t>(u);
+      ins_dst[ii] = static_cast<int32_t>(v);
+      ins_w[ii] = 1 + static_cast<int32_t>(splitmix(seed ^ 0x1B7ULL, static_cast<uint64_t>(ii)) % static_cast<uint64_t>(wmax));
+      ++ii;
+    }
+  }
+  return total_ins + total_del;
+}
+
 /// A batch aimed at the current shortest-path tree (the high-diameter grid configuration):
 /// `n_tree` deletions of distinct tree edges (parent[v], v) for random reachable v, `n_rand`
 /// deletions of random grid arcs (a grid arc deleted before makes an ignored deletion), and

@@ -169,8 +169,17 @@ struct batch_scan {
 }  // namespace dyng_hip
 
 struct dyng_hip_graph;
+struct dyng_hip_mlp;
 namespace dyng_hip {
 void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch_scan& sc);
+/// The scheduler's pick for a scanned batch on G_t: features, BF16 inference, argmin (one sync).
+int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents, int64_t n_tree,
+               int64_t last_invalidated, int64_t last_affected, hipStream_t s);
+/// The same without waiting: the prediction lands in pinned host memory, ready after the stream's
+/// next synchronization (the apply's); mlp_pick then reads the choice.
+void mlp_predict_async(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents,
+                       int64_t n_tree, int64_t last_invalidated, int64_t last_affected, hipStream_t s);
+int mlp_pick(const dyng_hip_mlp* m);
 void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out);
 }  // namespace dyng_hip
 

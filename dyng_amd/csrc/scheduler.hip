@@ -230,6 +230,32 @@ __global__ void k_batch_features(const int32_t* ins_src, int64_t num_ins, const 
   }
 }
 
+/// The network's 12 inputs from the batch features, on the device (the same formulas as the
+/// Python `scheduler.features`).
+__global__ void k_assemble_input(const feature_acc* acc, int64_t num_ins, int64_t num_del, int64_t n, int64_t m,
+                                 int64_t last_invalidated, int64_t last_affected, float* x) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double ops = static_cast<double>(num_ins + num_del);
+  const double nn = static_cast<double>(n > 1 ? n : 1), mm = static_cast<double>(m > 1 ? m : 1);
+  const double touched = static_cast<double>(acc->touched);
+  x[0] = static_cast<float>(log2(1.0 + ops));
+  x[1] = static_cast<float>(static_cast<double>(num_del) / (ops > 1.0 ? ops : 1.0));
+  x[2] = static_cast<float>(log2(1.0 + static_cast<double>(num_del)));
+  x[3] = static_cast<float>(log2(1.0 + static_cast<double>(num_ins)));
+  x[4] = static_cast<float>(log2(nn));
+  x[5] = static_cast<float>(log2(mm / nn));
+  x[6] = static_cast<float>(static_cast<double>(acc->tree_deletions) / static_cast<double>(num_del > 1 ? num_del : 1));
+  x[7] = static_cast<float>(log2(1.0 + static_cast<double>(acc->tree_deletions)));
+  x[8] = static_cast<float>(touched > 0.0 ? static_cast<double>(acc->log_degree_sum) / 1024.0 / touched : 0.0);
+  x[9] = static_cast<float>(log2(1.0 + static_cast<double>(acc->max_degree)));
+  x[10] = static_cast<float>(log2(1.0 + static_cast<double>(last_invalidated)));
+  x[11] = static_cast<float>(log2(1.0 + static_cast<double>(last_affected)));
+}
+
+__global__ void k_clear_acc(feature_acc* acc) {
+  if (threadIdx.x == 0) *acc = feature_acc{0, 0, 0, 0};
+}
+
 unsigned int grid_for(int64_t work, int block) {
   return static_cast<unsigned int>(std::max<int64_t>(1, std::min<int64_t>((work + block - 1) / block, 65535)));
 }
@@ -249,6 +275,11 @@ struct dyng_hip_mlp {
   bool bf16_fresh = false;
   dbuf<float> X, T, XN, H1, H2, DH1, DH2, Y, DY;
   dbuf<int32_t> idx;
+  dbuf<feature_acc> acc;
+  struct prediction {
+    float y[kOut];
+  };
+  pinned<prediction> host_y;  ///< the last prediction, read after the stream's next synchronization
 };
 
 namespace {
@@ -307,6 +338,60 @@ uint64_t mix(uint64_t z) {
 }
 
 }  // namespace
+
+namespace dyng_hip {
+
+void mlp_predict_async(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents,
+                       int64_t n_tree, int64_t last_invalidated, int64_t last_affected, hipStream_t s) {
+  refresh_bf16(m);
+  m->X.reserve(kIn);
+  m->Y.reserve(kOut);
+  m->acc.reserve(1);
+  hipLaunchKernelGGL(k_clear_acc, dim3(1), dim3(64), 0, s, m->acc.p);
+  const int64_t work = sc.num_ins + sc.num_del;
+  if (work > 0) {
+    hipLaunchKernelGGL(k_batch_features, dim3(grid_for(work, 256)), dim3(256), 0, s, sc.ins_src, sc.num_ins, sc.del_src,
+                       sc.del_dst, sc.num_del, parents, n_tree, g->out.meta.p, g->n, m->acc.p);
+  }
+  hipLaunchKernelGGL(k_assemble_input, dim3(1), dim3(64), 0, s, m->acc.p, sc.num_ins, sc.num_del, g->n, g->m,
+                     last_invalidated, last_affected, m->X.p);
+  hipLaunchKernelGGL(k_predict, dim3(1), dim3(kHidden), 0, s, m->bf16.p, m->params.p, m->X.p, 1, m->Y.p);
+  DYNG_HIP_TRY(hipGetLastError());
+  DYNG_HIP_TRY(hipMemcpyAsync(m->host_y.p->y, m->Y.p, kOut * sizeof(float), hipMemcpyDeviceToHost, s));
+}
+
+int mlp_pick(const dyng_hip_mlp* m) {
+  int best = 0;
+  const float* y = m->host_y.p->y;
+  for (int o = 1; o < kOut; ++o) best = y[o] < y[best] ? o : best;
+  return best;
+}
+
+int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents, int64_t n_tree,
+               int64_t last_invalidated, int64_t last_affected, hipStream_t s) {
+  refresh_bf16(m);
+  m->X.reserve(kIn);
+  m->Y.reserve(kOut);
+  m->acc.reserve(1);
+  DYNG_HIP_TRY(hipMemsetAsync(m->acc.p, 0, sizeof(feature_acc), s));
+  const int64_t work = sc.num_ins + sc.num_del;
+  if (work > 0) {
+    hipLaunchKernelGGL(k_batch_features, dim3(grid_for(work, 256)), dim3(256), 0, s, sc.ins_src, sc.num_ins, sc.del_src,
+                       sc.del_dst, sc.num_del, parents, n_tree, g->out.meta.p, g->n, m->acc.p);
+  }
+  hipLaunchKernelGGL(k_assemble_input, dim3(1), dim3(64), 0, s, m->acc.p, sc.num_ins, sc.num_del, g->n, g->m,
+                     last_invalidated, last_affected, m->X.p);
+  hipLaunchKernelGGL(k_predict, dim3(1), dim3(kHidden), 0, s, m->bf16.p, m->params.p, m->X.p, 1, m->Y.p);
+  DYNG_HIP_TRY(hipGetLastError());
+  float y[kOut];
+  DYNG_HIP_TRY(hipMemcpyAsync(y, m->Y.p, sizeof(y), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  int best = 0;
+  for (int o = 1; o < kOut; ++o) best = y[o] < y[best] ? o : best;
+  return best;
+}
+
+}  // namespace dyng_hip
 
 extern "C" int dyng_hip_mlp_create(int device, uint64_t seed, dyng_hip_mlp** out) {
   return guarded([&] {

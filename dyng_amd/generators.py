@@ -43,6 +43,10 @@ def lib() -> C.CDLL:
         L.dyng_gen_batches.argtypes = [vp, vp, C.c_int64, C.c_int64, C.c_int32, C.c_int64, C.c_int64, C.c_int64,
                                        C.c_int32, C.c_uint64, C.c_int32, C.c_double, C.c_double, C.c_double,
                                        vp, vp, vp, vp, vp]
+        L.dyng_gen_stream.restype = C.c_int64
+        L.dyng_gen_stream.argtypes = [vp, vp, C.c_int64, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                      C.c_uint64, C.c_int32, C.c_double, C.c_double, C.c_double, vp, vp, vp, vp,
+                                      vp, vp]
         L.dyng_gen_tree_batch.restype = C.c_int
         L.dyng_gen_tree_batch.argtypes = [C.c_int64, vp, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_uint64,
                                           C.c_int64, vp, vp, vp, vp, vp]
@@ -133,3 +137,30 @@ def tree_batch(side: int, parents: np.ndarray, n_ins: int, n_tree: int, n_rand: 
     if rc != 0:
         raise ValueError("tree_batch: the tree has too few edges")
     return ins_s, ins_d, ins_w, del_s, del_d
+
+
+def stream(src: np.ndarray, dst: np.ndarray, n: int, count: int, lo: float, hi: float, wmax: int, seed: int,
+           rmat_scale: int, abc=RMAT_ABC):
+    """Batches of log-uniform size in [lo, hi] with a uniform delete fraction against the graph
+    (src, dst sorted and distinct): deletions of distinct existing edges, insertions of new RMAT
+    pairs (generators.cpp: dyng_gen_stream). Returns a list of (ins_src, ins_dst, ins_w, del_src,
+    del_dst)."""
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    dst = np.ascontiguousarray(dst, dtype=np.int32)
+    sizes = np.zeros(2 * count, np.int64)
+    L = lib()
+    L.dyng_gen_stream(src.ctypes.data, dst.ctypes.data, len(src), n, count, lo, hi, wmax, seed, rmat_scale,
+                      abc[0], abc[1], abc[2], sizes.ctypes.data, None, None, None, None, None)
+    ni, nd = int(sizes[0::2].sum()), int(sizes[1::2].sum())
+    ins_s, ins_d, ins_w = (np.empty(max(ni, 1), np.int32) for _ in range(3))
+    del_s, del_d = (np.empty(max(nd, 1), np.int32) for _ in range(2))
+    L.dyng_gen_stream(src.ctypes.data, dst.ctypes.data, len(src), n, count, lo, hi, wmax, seed, rmat_scale,
+                      abc[0], abc[1], abc[2], sizes.ctypes.data, ins_s.ctypes.data, ins_d.ctypes.data,
+                      ins_w.ctypes.data, del_s.ctypes.data, del_d.ctypes.data)
+    out, i0, d0 = [], 0, 0
+    for b in range(count):
+        a, d = int(sizes[2 * b]), int(sizes[2 * b + 1])
+        out.append((ins_s[i0:i0 + a], ins_d[i0:i0 + a], ins_w[i0:i0 + a], del_s[d0:d0 + d], del_d[d0:d0 + d]))
+        i0 += a
+        d0 += d
+    return out

@@ -224,10 +224,18 @@ class Scheduler:
         return int(np.argmin(self.network.predict(x[None, :])[0]))
 
     def update(self, graph: Graph, batch: EdgeBatch, result: sssp.Result) -> sssp.Stats:
-        """sssp.update under the chosen schedule."""
-        choice = self.choose(graph, batch, result)
-        use(result, choice, default_delta(graph, result.options.objective))
-        st = sssp.update(graph, batch, result)
+        """sssp.update under the chosen schedule: the features and the BF16 inference run on the
+        device inside the update call (one extra tiny launch pair, no extra round trip)."""
+        nb, keep = batch._native(graph.num_weights)
+        out = _lib.Stats()
+        choice = C.c_int32(0)
+        net = self.network._handle if self.network is not None else None
+        _lib.check(_lib.lib().dyng_hip_api_update_scheduled(graph._handle, C.byref(nb), result._handle, net,
+                                                            int(self.history.invalidated), int(self.history.affected),
+                                                            result._resources.stream_handle, C.byref(out),
+                                                            C.byref(choice)))
+        del keep
+        st = sssp.Stats._from_native(out)
         self.history = History(st.invalidated, st.affected)
-        self.choices.append(choice)
+        self.choices.append(int(choice.value))
         return st

@@ -247,6 +247,15 @@ class Result:
                                                        res.stream_handle, C.byref(h)))
         return cls._wrap(h, res)
 
+    def differences(self, other: "Result") -> int:
+        """Vertices (counted per array) where this tree and ``other`` differ, compared on the device."""
+        if not isinstance(other, Result):
+            raise TypeError("dyng_amd.sssp.Result.differences: other must be a Result")
+        out = C.c_int64()
+        _lib.check(_lib.lib().dyng_hip_result_equal(self._handle, other._handle, self._resources.stream_handle,
+                                                    C.byref(out)))
+        return int(out.value)
+
     def __reduce__(self):
         raise TypeError("dyng_amd.sssp.Result cannot be pickled; rebuild it with Result.from_arrays()")
 

@@ -290,6 +290,10 @@ int dyng_hip_result_source(const dyng_hip_result* r, int32_t* source);
 int dyng_hip_result_version(const dyng_hip_result* r, uint64_t* version);
 int dyng_hip_result_options(const dyng_hip_result* r, dyng_hip_options* opt);
 int dyng_hip_result_set_options(dyng_hip_result* r, const dyng_hip_options* opt);
+/* Positions where two results' (distance, parent) pairs differ, counted on the device (0: the
+ * trees are identical); no counterpart in the reference (it compares host vectors). */
+int dyng_hip_result_equal(const dyng_hip_result* a, const dyng_hip_result* b, void* stream,
+                          int64_t* differences);
 /* Copy the arrays to host memory (to_vector). */
 int dyng_hip_result_download(const dyng_hip_result* r, int64_t* distances, int32_t* parents);
 /* Schedule knobs of this result's runs (not part of sssp::options). */
@@ -338,6 +342,14 @@ int dyng_hip_mlp_train(dyng_hip_mlp* m, const float* x, const float* t, int64_t 
                        int32_t batch, float lr, uint64_t seed, float* final_loss);
 /* BF16 inference: y (count x 5) for x (count x 12), host memory. */
 int dyng_hip_mlp_predict(dyng_hip_mlp* m, const float* x, int64_t count, float* y);
+
+/* sssp::update under the learned scheduler: the batch's features on G_t and the network's BF16
+ * inference run on the device before anything changes (mlp = null: the plain schedule), then the
+ * update runs under the schedule predicted fastest (*choice: its index). last_invalidated and
+ * last_affected are the previous update's counters (the network's history features). */
+int dyng_hip_api_update_scheduled(dyng_hip_graph* g, const dyng_hip_batch* batch, dyng_hip_result* r,
+                                  dyng_hip_mlp* mlp, int64_t last_invalidated, int64_t last_affected,
+                                  void* stream, dyng_hip_stats* stats, int32_t* choice);
 
 #ifdef __cplusplus
 }
