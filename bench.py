@@ -50,6 +50,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per persistent-kernel launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    passes of this workload (scaled by a copy of known size), or None if none was taken."""
+    path = os.path.join(ROOT, "profiles", f"sssp_fused_kernel_traffic_{workload}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch = "):
+                    return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def fused_kernel_bytes(c, n: int, num_changes: int, num_ins: int) -> int:
     """Algorithmic HBM bytes of one persistent-kernel launch (update), from its work counters:
     pack 24 B/vertex + unpack 20 B/vertex, 20 B per pointer-jumping visit, 20 B per invalidated
@@ -283,7 +297,7 @@ def main() -> int:
     rms = [s.device.relax_ms for s in stats]
     relax_gbs = (sum(rbytes) / max(sum(rms), 1e-9)) * 1e3 / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
                 "kernel": "sssp_fused_kernel", "kernel_ms_mean": round(statistics.mean(kms), 4),
                 "algorithmic_bytes_mean": int(sum(kbytes) / len(kbytes))}
     relax_phase = {"achieved_gbs": round(relax_gbs, 1), "frac": round(relax_gbs / HBM_PEAK_GBS, 4),
