@@ -1,26 +1,35 @@
 // The device graph of the HIP sssp path: out- and in-rows in slack storage, built on the device
 // and updated in place by each batch.
 //
-// Layout (HBM): per vertex an int2 {offset, length} and a capacity; one slot arena per direction
-// holding the neighbour ids, then one int32 weight column per objective (column k at
-// w + k * slot_cap). Every row is kept sorted by neighbour id, parallel edges in the order they
-// were added, so the first (u,v) of row u in MOSP's append order is the first entry of the run of
-// v in the sorted row. A row's capacity is len + ceil(len * headroom) + 2; a row that outgrows it
-// moves to the free tail of the arena; compaction rebuilds the arena when the tail runs short.
+// Layout (HBM): per vertex an int2 {offset, length}, a capacity, the length of the row's base run
+// and the count of dead entries in it; one slot arena per direction holding the neighbour ids,
+// then one int32 weight column per objective (column k at w + k * slot_cap). A row is two sorted
+// runs: the base [offset, offset + base) and the tail [offset + base, offset + length), each
+// sorted by neighbour id with parallel edges in the order they were added; among equal
+// neighbours the base entry is the older one. A deletion marks an entry dead (the sign bit of its
+// neighbour id) instead of moving the row. So the first alive (u,v) of row u in MOSP's append
+// order is the first alive entry of v's run in the base, else in the tail. A batch rewrites only
+// a row's tail while the tail stays short and the base has few dead entries (tail_limit,
+// dead_limit); otherwise the whole row is merged into one new base run. A row's capacity is
+// len + ceil(len * headroom) + 2; a row that outgrows it moves to the free tail of the arena;
+// compaction rebuilds the arena when the tail runs short.
 //
 // A batch (MOSP applyChangeBatch under upsert_last_wins, cpp/src/graph/apply_host.cpp:57-401) is
 // applied without leaving the device:
 //   1. Step 0 (apply_host.cpp:80-147): self-loops, vertex growth, undirected expansion; one key
 //      (u << b | v) per effective operation, in application order (deletions first).
 //   2. A stable radix sort by key groups the operations of each pair (u,v) in batch order.
-//   3. One thread per pair replays its operations on the run of v in row u (binary search):
-//      deletions mark the first alive match dead, insertions overwrite the first alive match
-//      (upsert) or add a pending entry; then the weight-increase flags of :348-394 (final first
-//      (u,v) against the original first (u,v)). Each operation that changes an in-edge emits an
-//      in-operation for the in-rows.
-//   4. One workgroup per touched row merges the alive entries with the pending ones (positions
-//      from prefix sums and binary searches), staged through a scratch arena and written back in
-//      place, or moved to the tail when the row outgrew its capacity.
+//   3. One thread per pair replays its operations on the runs of v in row u (binary searches in
+//      both runs): deletions mark the first alive match dead, insertions overwrite the first
+//      alive match (upsert) or add a pending entry; then the weight-increase flags of :348-394
+//      (final first (u,v) against the original first (u,v)). Each operation that changes an
+//      in-edge emits an in-operation for the in-rows.
+//   4. A planning pass sorts the touched rows into three merge modes (a short tail merged by one
+//      wave in registers, a longer tail by one workgroup, or the whole row into a new base) and
+//      sums the staging and relocation slots the full merges need, so the host can grow or
+//      compact the arena first; then one launch per mode merges the alive entries with the
+//      pending ones (positions from prefix sums and binary searches), in place, or at the arena's
+//      tail when the row outgrew its capacity.
 //   5. Steps 2-4 again for the in-rows with the in-operations (a deletion or a weight change
 //      matches the first alive in-entry of u with the old weights).
 #include "internal.hpp"
@@ -53,28 +62,6 @@ int bits_for(int64_t n_after) {
   int b = 1;
   while ((int64_t{1} << b) <= n_after) ++b;  // n_after < 2^b, so ids < 2^b - 1 < the all-ones key
   return b;
-}
-
-DYNG_DEV int32_t masked(int32_t x) { return x & 0x7fffffff; }
-
-/// First index in [0, len) whose (masked) neighbour is >= v.
-DYNG_DEV int lower_nbr(const int32_t* a, int len, int32_t v) {
-  int lo = 0, hi = len;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (masked(a[mid]) < v) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
-/// First index in [0, len) whose (masked) neighbour is > v.
-DYNG_DEV int upper_nbr(const int32_t* a, int len, int32_t v) {
-  int lo = 0, hi = len;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (masked(a[mid]) <= v) lo = mid + 1; else hi = mid;
-  }
-  return lo;
 }
 
 DYNG_DEV int64_t lower_key(const u64* keys, int64_t lo, int64_t hi, u64 key) {
@@ -411,10 +398,10 @@ struct run_pair {
 DYNG_DEV run_pair find_runs(const int32_t* row, int base_len, int len, int32_t v) {
   run_pair r;
   r.bl = base_len;
-  r.lb = lower_nbr(row, base_len, v);
-  r.ub = upper_nbr(row, base_len, v);
-  r.tl = base_len + lower_nbr(row + base_len, len - base_len, v);
-  r.tu = base_len + upper_nbr(row + base_len, len - base_len, v);
+  r.lb = row_lower_bound(row, 0, base_len, v);
+  r.ub = row_upper_bound_from(row, r.lb, base_len, v);
+  r.tl = row_lower_bound(row, base_len, len, v);
+  r.tu = row_upper_bound_from(row, r.tl, len, v);
   return r;
 }
 

@@ -7,10 +7,14 @@
 //   B  the subtrees under the roots (fused.cuh:199-234 computes them by pointer jumping over all
 //      n vertices). A tree this engine produced is walked top-down instead, level by level along
 //      its tree edges (out-edges (u, w) with parent[w] == u), touching only the invalidated
-//      region; a tree handed in from outside keeps pointer jumping (over compacted lists of the
+//      region; a walk deeper than kWalkLevels levels hands its unfinished part to pointer
+//      jumping. A tree handed in from outside keeps pointer jumping (over compacted lists of the
 //      vertices still jumping), which also detects parent cycles.
 //   C  invalidate (word INF, stamp), then the insertion heads (fused.cuh:236-256).
-//   D  pull: each candidate takes its best in-neighbour offer (fused.cuh:258-285).
+//   D  pull: each invalidated vertex takes its best in-neighbour offer (fused.cuh:258-285). For a
+//      tree this engine produced, an insertion head's old word was already the best over its
+//      old in-edges, so it is offered only the inserted edges into it; a tree from outside pulls
+//      over every in-edge of the heads too.
 //   E  near-far propagation (fused.cuh:288-417): the relax iteration expands the near frontier
 //      with a packed 64-bit atomicMin per edge.
 //   F  unpack, writing only changed (distance, parent) pairs and counting them
@@ -20,7 +24,10 @@
 // LDS and deals the tile's edge slots out one per thread, so a thread's critical path is one edge
 // rather than one row. Rows longer than `light` are cut into chunks that whole waves expand after
 // a barrier. Per-iteration list counters live in rings of three slots, so an iteration needs one
-// barrier (two when it has chunked rows).
+// barrier (two when it has chunked rows). Workgroups exchange data only through agent-scope
+// atomics and loads/stores that bypass the non-coherent caches, so the grid barrier itself needs
+// no cache write-back or invalidation (the distance-only unpack, which reads plain stores of
+// other workgroups, uses the fenced barrier).
 #include "internal.hpp"
 
 #include <hip/hip_runtime.h>
@@ -123,6 +130,55 @@ DYNG_DEV void append(int* list, int* counter, bool pred, int32_t v) {
   if (pred) st(&list[slot], v);
 }
 
+constexpr int kStage = 2048;  ///< entries a workgroup stages per list between flushes
+
+/// Appends staged in LDS. A list's counter is one address that every workgroup hits; device-scope
+/// atomics on one address serialize (about 10 ns each on MI355X), so one reservation per wave
+/// costs microseconds per phase on a big frontier and milliseconds on a pass over every vertex.
+/// Waves reserve LDS slots instead (an LDS atomic), and stage_flush reserves the global range of
+/// the workgroup's staged entries with one atomicAdd per list. Entries past the stage's capacity
+/// go to the list directly.
+struct stage_shared {
+  int count[2];
+  int base[2];
+  int item[2][kStage];
+};
+
+/// Stage v for list k (`list`, `counter`) when pred. Every active lane of the wave calls it.
+DYNG_DEV void stage_append(stage_shared& S, int k, bool pred, int32_t v, int* list, int* counter) {
+  const u64 mask = __ballot(pred);
+  if (mask == 0) return;  // uniform per wave
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  int b = 0;
+  if (lane() == leader) b = atomicAdd(&S.count[k], __popcll(mask));
+  b = __shfl(b, leader);
+  const int slot = b + __popcll(mask & lanes_below());
+  if (pred && slot < kStage) S.item[k][slot] = v;
+  append(list, counter, pred && slot >= kStage, v);
+}
+
+/// Publish the staged entries of list 0 (and of list 1 when list1 is not null). Every thread of
+/// the workgroup calls it; the stage is empty afterwards.
+DYNG_DEV void stage_flush(stage_shared& S, int* list0, int* counter0, int* list1 = nullptr, int* counter1 = nullptr) {
+  const int lists = list1 ? 2 : 1;
+  __syncthreads();  // every staged entry is in LDS
+  if (static_cast<int>(threadIdx.x) < lists) {
+    const int k = threadIdx.x;
+    const int cnt = min(S.count[k], kStage);
+    S.count[k] = cnt;
+    S.base[k] = cnt > 0 ? atomicAdd(k == 0 ? counter0 : counter1, cnt) : 0;
+  }
+  __syncthreads();
+  for (int k = 0; k < lists; ++k) {
+    int* list = k == 0 ? list0 : list1;
+    const int cnt = S.count[k], b = S.base[k];
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) st(&list[b + i], S.item[k][i]);
+  }
+  __syncthreads();  // every thread has read count and base
+  if (static_cast<int>(threadIdx.x) < lists) S.count[threadIdx.x] = 0;
+  __syncthreads();  // the reset lands before the next stage_append
+}
+
 /// A chunk {vertex, begin, end} of the chunk list, written and read with agent-scope operations.
 DYNG_DEV void st_chunk(int4* ch, int32_t v, int32_t b, int32_t e) {
   u64* q = reinterpret_cast<u64*>(ch);
@@ -190,9 +246,11 @@ DYNG_DEV int block_scan(tile_shared& L, int x, int& total) {
 /// nothing to expand in the tile, e.g. a long row sent to the chunk list), and the edge slots of
 /// the tile's rows are dealt out to the workgroup's threads, one slot per thread per step:
 /// `edge(act, item, slot)` runs with every lane of a wave (act false on padding lanes). `done(valid,
-/// item)` runs on every thread after the tile's edges. Uniform per workgroup: every thread calls it.
-template <class Load, class Edge, class Done>
-DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edge edge, Done done) {
+/// item)` runs on every thread after the tile's edges, and `flush()` on every thread of the
+/// workgroup after each tile (to publish staged appends). Uniform per workgroup: every thread
+/// calls it.
+template <class Load, class Edge, class Done, class Flush>
+DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edge edge, Done done, Flush flush) {
   const int G = static_cast<int>(gridDim.x);
   const int ts = min(kBlock, max(1, (count + G - 1) / G));
   const int i = static_cast<int>(threadIdx.x);
@@ -232,14 +290,15 @@ DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edge edge, Done do
     __syncthreads();
     done(i < cnt, i);
     __syncthreads();
+    flush();
   }
 }
 
 /// One relaxation of edge (u, w) of weight wt with u at distance du (fused.cuh:341-362): the
 /// candidate word joins the next near list or the far pile when it lowered w's distance.
 DYNG_DEV void relax(const engine_params& p, bool active, int32_t u, u64 du, int32_t w, int32_t wt,
-                    u64 threshold, int generation, int* next, int* next_count, int* far, int* far_count,
-                    unsigned long long* improved) {
+                    u64 threshold, int generation, stage_shared& S, int* next, int* next_count, int* far,
+                    int* far_count, unsigned long long* improved) {
   bool to_near = false, to_far = false;
   if (active && w != p.source) {
     const u64 nd = du + static_cast<u64>(wt);
@@ -258,8 +317,8 @@ DYNG_DEV void relax(const engine_params& p, bool active, int32_t u, u64 du, int3
       }
     }
   }
-  append(next, next_count, to_near, w);
-  append(far, far_count, to_far, w);
+  stage_append(S, 0, to_near, w, next, next_count);
+  stage_append(S, 1, to_far, w, far, far_count);
 }
 
 /// Chunks of long rows, one wave per chunk: `edge(act, u, slot)` with every lane of the wave.
@@ -296,11 +355,8 @@ DYNG_DEV u64 edge_offer(const engine_params& p, int32_t u, int32_t v) {
   u64 best = packed_inf;
   const int runs[3] = {0, bl, mt.y};
   for (int r = 0; r < 2; ++r) {
-    int lo = runs[r], hi = runs[r + 1];
-    while (lo < hi) {  // first entry of the run whose neighbour (dead or alive) is >= v
-      const int mid = (lo + hi) >> 1;
-      if ((row[mid] & 0x7fffffff) < v) lo = mid + 1; else hi = mid;
-    }
+    // the first entry of the run whose neighbour (dead or alive) is >= v
+    const int lo = row_lower_bound(row, runs[r], runs[r + 1], v);
     for (int x = lo; x < runs[r + 1] && (row[x] & 0x7fffffff) == v; ++x) {
       if (row[x] < 0) continue;  // dead
       const u64 nd = du + static_cast<u64>(p.out_w[mt.x + x]);
@@ -315,8 +371,8 @@ DYNG_DEV u64 edge_offer(const engine_params& p, int32_t u, int32_t v) {
 
 /// Lower v's word to its best pull offer; v joins the frontier if its distance fell. The
 /// smallest lowered distance goes to *minimum (the first near-far threshold).
-DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best, int generation, int* counter,
-                          u64* minimum) {
+DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best, int generation, stage_shared& S,
+                          int* counter, u64* minimum) {
   bool add = false;
   u64 low = packed_inf;
   if (pred) {
@@ -329,7 +385,7 @@ DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best
       }
     }
   }
-  append(p.frontier, counter, add, v);
+  stage_append(S, 0, add, v, p.frontier, counter);
   low = wave_min(low);
   if (lane() == 0 && low != packed_inf) atomicMin(minimum, low);
 }
@@ -358,7 +414,9 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   __shared__ tile_shared L;
   // work counters of the workgroup, in LDS: relax edges, improving atomics, pull edges, affected
   __shared__ unsigned long long counts[4];
+  __shared__ stage_shared S;
   if (threadIdx.x < 4) counts[threadIdx.x] = 0;
+  if (threadIdx.x < 2) S.count[threadIdx.x] = 0;
   __syncthreads();
   unsigned long long* const relax_edges = &counts[0];
   unsigned long long* const relax_improved = &counts[1];
@@ -450,7 +508,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
             add = true;
           }
         }
-        append(level, found, add, w);
+        stage_append(S, 0, add, w, level, found);
       };
       while (hi > lo && levels < kWalkLevels) {
         jump_items += hi - lo;
@@ -476,12 +534,13 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
               }
             },
             [&](bool act, int item, int slot) { visit(act, act ? L.vert[item] : -1, slot, level, found); },
-            [&](bool, int) {});
+            [&](bool, int) {}, [&] { stage_flush(S, level, found); });
         DYNG_GRID_SYNC();
         const int nchunks = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap)));
         ++cstep;
         if (nchunks > 0) {
           chunk_rows(p, nchunks, [&](bool act, int32_t u, int slot) { visit(act, u, slot, level, found); });
+          stage_flush(S, level, found);
           DYNG_GRID_SYNC();
         }
         lo = hi;
@@ -517,8 +576,10 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           st(&c->ring_jump[(r + jump_slot + 1) % 3], 0);
           st(&c->rounds, r + 1);
         }
-        for (int base = tid - lane(); base < count; base += threads) {
-          const int i = base + lane();
+        // whole workgroups per step (t0 is uniform per workgroup), so each step can publish its
+        // staged appends
+        for (long long t0 = static_cast<long long>(blockIdx.x) * kBlock; t0 < count; t0 += threads) {
+          const int i = static_cast<int>(t0) + static_cast<int>(threadIdx.x);
           bool keep = false, flagged = false;
           int32_t v = -1;
           if (i < count) {
@@ -536,8 +597,9 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
               }
             }
           }
-          append(p.cand, &c->cand_count, flagged, v);
-          append(out, out_count, keep, v);
+          stage_append(S, 0, flagged, v, p.cand, &c->cand_count);
+          stage_append(S, 1, keep, v, out, out_count);
+          stage_flush(S, p.cand, &c->cand_count, out, out_count);
         }
         DYNG_GRID_SYNC();
         count = uniform(ld(out_count));
@@ -599,8 +661,9 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           v = p.ins_dst[i];
           if (v >= 0 && v != p.source) best = edge_offer(p, p.ins_src[i], v);
         }
-        pull_commit(p, best != packed_inf, v, best, generation, &c->frontier_count, &c->min_frontier);
+        pull_commit(p, best != packed_inf, v, best, generation, S, &c->frontier_count, &c->min_frontier);
       }
+      stage_flush(S, p.frontier, &c->frontier_count);
     }
     tile_rows(
         L, cands,
@@ -623,9 +686,10 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           }
         },
         [&](bool valid, int item) {
-          pull_commit(p, valid, valid ? L.vert[item] : -1, valid ? L.word[item] : packed_inf, generation,
+          pull_commit(p, valid, valid ? L.vert[item] : -1, valid ? L.word[item] : packed_inf, generation, S,
                       &c->frontier_count, &c->min_frontier);
-        });
+        },
+        [&] { stage_flush(S, p.frontier, &c->frontier_count); });
     DYNG_GRID_SYNC();
     const int nchunks = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap)));
     ++cstep;
@@ -688,9 +752,10 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           to_far = atomicExch(&p.in_far[v], 1) == 0;
         }
       }
-      append(current, near_count, to_near, v);
-      append(far[fb], &c->far_count[fb], to_far, v);
+      stage_append(S, 0, to_near, v, current, near_count);
+      stage_append(S, 1, to_far, v, far[fb], &c->far_count[fb]);
     }
+    stage_flush(S, current, near_count, far[fb], &c->far_count[fb]);
     DYNG_GRID_SYNC();
   }
   int near_count = uniform(ld(&c->ring_near[step % 3]));
@@ -740,10 +805,10 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
             const u64 du = act ? p.pk.distance(L.word[item]) : 0;
             const int32_t w = act ? p.out_nbr[slot] : 0;
             const int32_t wt = act ? p.out_w[slot] : 0;
-            relax(p, act && w >= 0, u, du, w, wt, threshold, generation, next, next_count, far_list, far_counter,
+            relax(p, act && w >= 0, u, du, w, wt, threshold, generation, S, next, next_count, far_list, far_counter,
                   relax_improved);
           },
-          [&](bool, int) {});
+          [&](bool, int) {}, [&] { stage_flush(S, next, next_count, far_list, far_counter); });
       DYNG_GRID_SYNC();
       const int nchunks = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap)));
       ++cstep;
@@ -761,10 +826,11 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
             const bool act = word != packed_inf && e < ch.z;
             const int32_t w = act ? p.out_nbr[e] : 0;
             const int32_t wt = act ? p.out_w[e] : 0;
-            relax(p, act && w >= 0, u, du, w, wt, threshold, generation, next, next_count, far_list, far_counter,
+            relax(p, act && w >= 0, u, du, w, wt, threshold, generation, S, next, next_count, far_list, far_counter,
                   relax_improved);
           }
         }
+        stage_flush(S, next, next_count, far_list, far_counter);
         DYNG_GRID_SYNC();
       }
       near_count = uniform(ld(next_count));
@@ -808,9 +874,10 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           keep = true;
         }
       }
-      append(current, split_count, to_near, v);
-      append(far[fb ^ 1], &c->far_count[fb ^ 1], keep, v);
+      stage_append(S, 0, to_near, v, current, split_count);
+      stage_append(S, 1, keep, v, far[fb ^ 1], &c->far_count[fb ^ 1]);
     }
+    stage_flush(S, current, split_count, far[fb ^ 1], &c->far_count[fb ^ 1]);
     DYNG_GRID_SYNC();
     near_count = uniform(ld(split_count));
     ++step;

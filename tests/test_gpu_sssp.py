@@ -274,6 +274,9 @@ STREAM_CASES = [
     (6, 100, 500, 7, {"existing": "ignore"}),
     (7, 80, 400, 12, {"deletions_first": False, "allow_parallel": True, "sorted_rows": False}),
     (8, 2000, 16000, 255, {}),
+    # long rows (past the 16-entry bisection) with several parallel copies of each edge
+    (9, 64, 20000, 50, {"allow_parallel": True}),
+    (10, 400, 40000, 100, {"allow_parallel": True, "deletions_first": False}),
 ]
 
 
