@@ -121,8 +121,18 @@ DYNG_DEV bool claim(int* stamp, int32_t v, int generation) {
 }
 
 /// Grid-wide barrier of a persistent kernel whose workgroups are all resident (the grid is sized
-/// to at most half of what the occupancy calculator says fits at once): one arrival per
-/// workgroup on an agent-scope counter, the last arrival bumps the generation.
+/// to at most half of what the occupancy calculator says fits at once).
+///
+/// Arrivals go up a two-level tree of monotonic counters: workgroup b arrives on the counter of
+/// group b % G (G = min(16, workgroups)); the last member of a group to arrive (the one whose
+/// fetch-add returns a count one short of a whole round) arrives on the counter of groups, and
+/// the last group releases the round by storing its number. Device-scope atomics on one address
+/// serialize, so 16-way groups cut the arrival chain from one atomic per workgroup to about 16 +
+/// 16. The counters never reset (the round is the count divided by the members), so no arrival
+/// reads the state first and no reset store can race the next round's arrivals; the state is
+/// cleared once per launch. Every counter lives on its own 128-byte line, apart from the word
+/// the waiters poll. Measured on MI355X with 256 workgroups of 1024 threads: 1.9 us per barrier
+/// (7.7 us with one flat counter that is read, reset and bumped).
 ///
 /// Two flavours. `Fenced` adds an agent-scope release fence before the arrival and an acquire
 /// fence after the wait (on MI355X they write back and invalidate the XCD's L2), so plain loads
@@ -130,48 +140,64 @@ DYNG_DEV bool claim(int* stamp, int32_t v, int generation) {
 /// memory operations to complete (s_waitcnt) before the arrival: the phases it separates must
 /// access every location that another workgroup writes during the kernel with agent-scope
 /// operations only (ld / st / atomics), which are coherent across the XCDs' L2s by themselves.
-/// That saves the two cache walks, the bulk of the barrier's cost.
+/// That saves the two cache walks, the bulk of a fenced barrier's cost.
 ///
 /// Every wait is bounded: a workgroup that waits longer than `limit` ticks of the 100 MHz
 /// real-time counter raises `abort`, and every workgroup that sees `abort` while waiting leaves
-/// too. The function then returns false and the caller leaves the kernel, so a grid whose
-/// workgroups could not all be resident ends with an error instead of spinning forever.
+/// too (both are checked on the first poll and every 32nd after). The function then returns
+/// false and the caller leaves the kernel, so a grid whose workgroups could not all be resident
+/// ends with an error instead of spinning forever.
+constexpr unsigned int kBarrierGroups = 16;
+
+struct alignas(128) barrier_line {
+  unsigned long long v;
+  unsigned long long pad[15];
+};
+
 struct grid_sync_state {
-  unsigned int count;
-  unsigned int generation;
-  unsigned int abort;
-  unsigned int pad;
+  barrier_line groups;                 ///< groups arrived, over all rounds
+  barrier_line round;                  ///< the last released round
+  barrier_line abort;                  ///< nonzero: a wait timed out
+  barrier_line group[kBarrierGroups];  ///< arrivals of each group, over all rounds
 };
 
 template <bool Fenced>
 DYNG_DEV bool grid_barrier(grid_sync_state* s, unsigned long long limit) {
   __shared__ int go;
-  constexpr int release = Fenced ? __ATOMIC_RELEASE : __ATOMIC_RELAXED;
-  constexpr int acquire = Fenced ? __ATOMIC_ACQUIRE : __ATOMIC_RELAXED;
-  constexpr int acq_rel = Fenced ? __ATOMIC_ACQ_REL : __ATOMIC_RELAXED;
   // every wave's loads and stores have completed (at the scope they were issued with) before
   // thread 0 arrives
   if (!Fenced) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (Fenced) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned int g = __hip_atomic_load(&s->generation, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned int arrived = __hip_atomic_fetch_add(&s->count, 1u, acq_rel, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int blocks = gridDim.x;
+    const unsigned int G = blocks < kBarrierGroups ? blocks : kBarrierGroups;
+    const unsigned int grp = blockIdx.x % G;
+    const unsigned long long members = blocks / G + (blocks % G > grp ? 1u : 0u);
+    const unsigned long long a = __hip_atomic_fetch_add(&s->group[grp].v, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long target = a / members + 1;  // the round this arrival completes
+    bool last = false;
+    if (a % members == members - 1) {
+      const unsigned long long g = __hip_atomic_fetch_add(&s->groups.v, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = g % G == G - 1;
+    }
     int ok = 1;
-    if (arrived == gridDim.x - 1) {
-      __hip_atomic_store(&s->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&s->generation, 1u, release, __HIP_MEMORY_SCOPE_AGENT);
+    if (last) {
+      __hip_atomic_store(&s->round.v, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-      while (__hip_atomic_load(&s->generation, acquire, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        if (__hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-          ok = 0;
-          break;
-        }
-        if (__builtin_amdgcn_s_memrealtime() - t0 > limit) {
-          __hip_atomic_store(&s->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ok = 0;
-          break;
+      unsigned int polls = 0;
+      while (__hip_atomic_load(&s->round.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if ((polls++ & 31u) == 0) {
+          if (__hip_atomic_load(&s->abort.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            ok = 0;
+            break;
+          }
+          if (__builtin_amdgcn_s_memrealtime() - t0 > limit) {
+            __hip_atomic_store(&s->abort.v, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = 0;
+            break;
+          }
         }
         __builtin_amdgcn_s_sleep(1);
       }

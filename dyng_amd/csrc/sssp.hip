@@ -407,7 +407,8 @@ DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best
 
 /// Clear the grid barrier (a kernel, not a memset, so the launch can live in a hipGraph).
 __global__ void k_reset_barrier(grid_sync_state* bar) {
-  if (threadIdx.x == 0) *bar = grid_sync_state{0u, 0u, 0u, 0u};
+  unsigned long long* w = reinterpret_cast<unsigned long long*>(bar);
+  for (unsigned int i = threadIdx.x; i < sizeof(grid_sync_state) / 8; i += blockDim.x) w[i] = 0;
 }
 
 __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) {
