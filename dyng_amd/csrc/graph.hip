@@ -273,7 +273,8 @@ __global__ void k_batch_scan(const int32_t* ins_src, const int32_t* ins_dst, con
     }
   }
   for (int k = 0; k < 4 && k < K; ++k) {
-    if (lmax[k] > 0) atomicMax(&ctl->batch_wmax[k], lmax[k]);
+    for (int off = 32; off > 0; off >>= 1) lmax[k] = max(lmax[k], __shfl_xor(lmax[k], off));
+    if (lane() == 0 && lmax[k] > 0) atomicMax(&ctl->batch_wmax[k], lmax[k]);
   }
   // per-column minimum (rarely below 1; objective checks happen on the host)
   for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < num_ins; i += stride) {
@@ -287,11 +288,21 @@ __global__ void k_batch_scan(const int32_t* ins_src, const int32_t* ins_dst, con
     mn = min(mn, min(u, v));
     sl_d += u == v ? 1 : 0;
   }
-  atomicMin(&ctl->min_id, mn);
-  atomicMax(&ctl->max_id, mx);
-  if (sl_i) atomicAdd(&ctl->self_loops_ins, sl_i);
-  if (sl_d) atomicAdd(&ctl->self_loops_del, sl_d);
-  atomicMin(&ctl->min_weight, wmin);
+  // one atomic per wave and counter (the counters are single addresses every wave hits)
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = min(mn, __shfl_xor(mn, off));
+    mx = max(mx, __shfl_xor(mx, off));
+    sl_i += __shfl_xor(sl_i, off);
+    sl_d += __shfl_xor(sl_d, off);
+    wmin = min(wmin, __shfl_xor(wmin, off));
+  }
+  if (lane() == 0) {
+    if (mn != INT32_MAX) atomicMin(&ctl->min_id, mn);
+    if (mx >= 0) atomicMax(&ctl->max_id, mx);
+    if (sl_i) atomicAdd(&ctl->self_loops_ins, sl_i);
+    if (sl_d) atomicAdd(&ctl->self_loops_del, sl_d);
+    if (wmin != INT32_MAX) atomicMin(&ctl->min_weight, wmin);
+  }
 }
 
 struct op_layout {
@@ -309,6 +320,7 @@ struct op_layout {
 __global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* ins_dst,
                            const int32_t* del_src, const int32_t* del_dst, int directed,
                            int drop_self_loops, int64_t n_after, int vbits, u64* keys, uint32_t* vals,
+                           u64* ikeys, uint32_t* ivals,
                            int32_t* e_del_src, int32_t* e_del_dst, int32_t* e_ins_src,
                            int32_t* e_ins_dst, uint8_t* e_inc, int K, apply_control* ctl) {
   const int64_t total = (L.num_first + L.num_second) * L.stride;
@@ -333,6 +345,10 @@ __global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* i
     }
     keys[e] = valid ? ((static_cast<u64>(u) << vbits) | static_cast<u64>(v)) : kNoKey;
     vals[e] = static_cast<uint32_t>(e);
+    // the in-rows' key of the same operation; what the operation does to the in-row is decided by
+    // the out-row pass (iop_type), so the in-keys can be sorted meanwhile
+    ikeys[e] = valid ? ((static_cast<u64>(v) << vbits) | static_cast<u64>(u)) : kNoKey;
+    ivals[e] = static_cast<uint32_t>(e);
     if (del) {
       e_del_src[loc] = valid ? u : -1;
       e_del_dst[loc] = valid ? v : -1;
@@ -373,9 +389,7 @@ struct out_pair_params {
   int32_t* kill;    ///< 1: the operation killed a tail entry
   int32_t* pend_w;  ///< k * N + position
   uint8_t* e_inc;
-  // in-operations, one per out-operation position
-  u64* ikeys;
-  uint32_t* ivals;
+  // what each operation (by batch slot e) does to the in-rows: kInNone, or the type and weights
   int32_t* iop_type;
   int32_t* iop_wold;
   int32_t* iop_wnew;
@@ -387,7 +401,7 @@ struct out_pair_params {
   apply_control* ctl;
 };
 
-enum { kInDelete = 0, kInModify = 1, kInInsert = 2 };
+enum { kInDelete = 0, kInModify = 1, kInInsert = 2, kInNone = 3 };
 
 /// The entries of neighbour v in a row's two sorted runs: [lb, ub) in the base, [tl, tu) in the
 /// tail (row-relative indices).
@@ -416,7 +430,9 @@ DYNG_DEV void mark_row_range(const u64* keys, int64_t i, int64_t N, int vbits, i
 __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
   const int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x;
   unsigned long long c_ins = 0, c_upd = 0, c_del = 0, c_ign_del = 0, c_ign_ins = 0;
-  long long dsum[4] = {0, 0, 0, 0}, dcnt[4] = {0, 0, 0, 0};
+  // weight sum and count-at-maximum deltas of columns 0..3 (named, not an array: a private
+  // array indexed by a runtime column would live in scratch memory)
+  long long ds0 = 0, ds1 = 0, ds2 = 0, ds3 = 0, dc0 = 0, dc1 = 0, dc2 = 0, dc3 = 0;
   const u64 mask = (1ULL << p.vbits) - 1;
   const int K = p.K;
   if (i == p.N) {
@@ -429,8 +445,6 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
     if (key == kNoKey) {
       p.pend_alive[i] = 0;
       p.kill[i] = 0;
-      p.ikeys[i] = kNoKey;
-      p.ivals[i] = static_cast<uint32_t>(i);
     } else if (i == 0 || p.keys[i - 1] != key) {
       int64_t j = i + 1;
       while (j < p.N && p.keys[j] == key) ++j;
@@ -449,20 +463,28 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
         }
         return -1;
       };
-      int32_t first_init[max_weights];
       const int init_hit = u < p.n_before ? first_alive() : -1;
       const bool had = init_hit >= 0;
-      if (had) {
-        for (int k = 0; k < K; ++k) first_init[k] = p.w[k * p.slot_cap + mt.x + init_hit];
-      }
+      // the first operation that overwrote the original first (u,v)'s weights: its recorded old
+      // weights are the original ones (else the slot still holds them)
+      int64_t first_mod = -1;
       bool structural = false;
       auto account = [&](int k, long long wold, long long wnew, bool removed, bool added) {
         const long long target = p.ctl->col_max[k];
         const long long ds = (added ? wnew : 0) - (removed ? wold : 0);
         const long long dc = ((added && wnew == target) ? 1 : 0) - ((removed && wold == target) ? 1 : 0);
-        if (k < 4) {
-          dsum[k] += ds;
-          dcnt[k] += dc;
+        if (k == 0) {
+          ds0 += ds;
+          dc0 += dc;
+        } else if (k == 1) {
+          ds1 += ds;
+          dc1 += dc;
+        } else if (k == 2) {
+          ds2 += ds;
+          dc2 += dc;
+        } else if (k == 3) {
+          ds3 += ds;
+          dc3 += dc;
         } else {
           if (ds) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dsum[k]), static_cast<unsigned long long>(ds));
           if (dc) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dcount[k]), static_cast<unsigned long long>(dc));
@@ -473,9 +495,7 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
         const bool del = p.L.is_del(e);
         p.pend_alive[q] = 0;
         p.kill[q] = 0;
-        p.ikeys[q] = kNoKey;
-        p.ivals[q] = static_cast<uint32_t>(q);
-        const u64 ikey = (static_cast<u64>(v) << p.vbits) | static_cast<u64>(u);
+        p.iop_type[e] = kInNone;
         // first alive match: old entries of the runs, then pending entries of this pair
         const int old_hit = first_alive();
         int64_t pend_hit = -1;
@@ -494,22 +514,20 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
             }
             for (int k = 0; k < K; ++k) {
               const int32_t wo = p.w[k * p.slot_cap + mt.x + old_hit];
-              p.iop_wold[k * p.N + q] = wo;
+              p.iop_wold[k * p.N + e] = wo;
               account(k, wo, 0, true, false);
             }
-            p.ikeys[q] = ikey;
-            p.iop_type[q] = kInDelete;
+            p.iop_type[e] = kInDelete;
             structural = true;
             ++c_del;
           } else if (pend_hit >= 0) {
             p.pend_alive[pend_hit] = 0;
             for (int k = 0; k < K; ++k) {
               const int32_t wo = p.pend_w[k * p.N + pend_hit];
-              p.iop_wold[k * p.N + q] = wo;
+              p.iop_wold[k * p.N + e] = wo;
               account(k, wo, 0, true, false);
             }
-            p.ikeys[q] = ikey;
-            p.iop_type[q] = kInDelete;
+            p.iop_type[e] = kInDelete;
             ++c_del;
           } else {
             ++c_ign_del;
@@ -520,17 +538,17 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
             if (!p.upsert) {
               ++c_ign_ins;
             } else {
+              if (old_hit >= 0 && old_hit == init_hit && first_mod < 0) first_mod = e;
               for (int k = 0; k < K; ++k) {
                 const int32_t wn = p.ins_w[bj * K + k];
                 int32_t* slot = old_hit >= 0 ? &p.w[k * p.slot_cap + mt.x + old_hit] : &p.pend_w[k * p.N + pend_hit];
                 const int32_t wo = *slot;
                 *slot = wn;
-                p.iop_wold[k * p.N + q] = wo;
-                p.iop_wnew[k * p.N + q] = wn;
+                p.iop_wold[k * p.N + e] = wo;
+                p.iop_wnew[k * p.N + e] = wn;
                 account(k, wo, wn, true, true);
               }
-              p.ikeys[q] = ikey;
-              p.iop_type[q] = kInModify;
+              p.iop_type[e] = kInModify;
               ++c_upd;
             }
           } else {
@@ -538,11 +556,10 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
             for (int k = 0; k < K; ++k) {
               const int32_t wn = p.ins_w[bj * K + k];
               p.pend_w[k * p.N + q] = wn;
-              p.iop_wnew[k * p.N + q] = wn;
+              p.iop_wnew[k * p.N + e] = wn;
               account(k, 0, wn, false, true);
             }
-            p.ikeys[q] = ikey;
-            p.iop_type[q] = kInInsert;
+            p.iop_type[e] = kInInsert;
             structural = true;
             ++c_ins;
           }
@@ -568,15 +585,15 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
             if (p.L.is_del(e)) continue;
             const int64_t loc = p.L.local(e);
             for (int k = 0; k < K; ++k) {
-              p.e_inc[loc * K + k] = wf[k * wf_stride] > first_init[k] ? 1 : 0;
+              const int32_t orig = first_mod >= 0 ? p.iop_wold[k * p.N + first_mod] : p.w[k * p.slot_cap + mt.x + init_hit];
+              p.e_inc[loc * K + k] = wf[k * wf_stride] > orig ? 1 : 0;
             }
           }
         }
       }
-      if (structural && atomicExch(&p.mark[u], p.mark_gen) != p.mark_gen) {
-        const int t = atomicAdd(&p.ctl->touched_out, 1);
-        p.touched[t] = u;
-      }
+      const bool first_touch = structural && atomicExch(&p.mark[u], p.mark_gen) != p.mark_gen;
+      const int t = wave_append(&p.ctl->touched_out, first_touch);
+      if (first_touch) p.touched[t] = u;
     }
   }
   c_ins = wave_sum(c_ins);
@@ -584,19 +601,25 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
   c_del = wave_sum(c_del);
   c_ign_del = wave_sum(c_ign_del);
   c_ign_ins = wave_sum(c_ign_ins);
-  for (int k = 0; k < 4 && k < K; ++k) {
-    dsum[k] = static_cast<long long>(wave_sum(static_cast<u64>(dsum[k])));
-    dcnt[k] = static_cast<long long>(wave_sum(static_cast<u64>(dcnt[k])));
-  }
+  const long long dsums[4] = {static_cast<long long>(wave_sum(static_cast<u64>(ds0))),
+                              static_cast<long long>(wave_sum(static_cast<u64>(ds1))),
+                              static_cast<long long>(wave_sum(static_cast<u64>(ds2))),
+                              static_cast<long long>(wave_sum(static_cast<u64>(ds3)))};
+  const long long dcnts[4] = {static_cast<long long>(wave_sum(static_cast<u64>(dc0))),
+                              static_cast<long long>(wave_sum(static_cast<u64>(dc1))),
+                              static_cast<long long>(wave_sum(static_cast<u64>(dc2))),
+                              static_cast<long long>(wave_sum(static_cast<u64>(dc3)))};
   if (lane() == 0) {
     if (c_ins) atomicAdd(&p.ctl->inserted_edges, c_ins);
     if (c_upd) atomicAdd(&p.ctl->updated_edges, c_upd);
     if (c_del) atomicAdd(&p.ctl->deleted_edges, c_del);
     if (c_ign_del) atomicAdd(&p.ctl->ignored_deletions, c_ign_del);
     if (c_ign_ins) atomicAdd(&p.ctl->ignored_insertions, c_ign_ins);
-    for (int k = 0; k < 4 && k < K; ++k) {
-      if (dsum[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dsum[k]), static_cast<unsigned long long>(dsum[k]));
-      if (dcnt[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dcount[k]), static_cast<unsigned long long>(dcnt[k]));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= K) break;
+      if (dsums[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dsum[k]), static_cast<unsigned long long>(dsums[k]));
+      if (dcnts[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dcount[k]), static_cast<unsigned long long>(dcnts[k]));
     }
   }
 }
@@ -655,8 +678,9 @@ __global__ void __launch_bounds__(kBlock) k_in_pairs(in_pair_params p) {
   for (int64_t q = i; q < j; ++q) {
     p.pend_alive[q] = 0;
     p.kill[q] = 0;
-    const uint32_t o = p.vals[q];
+    const uint32_t o = p.vals[q];  // the operation's batch slot
     const int type = p.iop_type[o];
+    if (type == kInNone) continue;  // ignored by the out-row pass
     if (type == kInInsert) {
       p.pend_alive[q] = 1;
       for (int k = 0; k < K; ++k) p.pend_w[k * p.N + q] = p.iop_wnew[k * p.N + o];
@@ -705,10 +729,9 @@ __global__ void __launch_bounds__(kBlock) k_in_pairs(in_pair_params p) {
       }
     }
   }
-  if (structural && atomicExch(&p.mark[v], p.mark_gen) != p.mark_gen) {
-    const int t = atomicAdd(&p.ctl->touched_in, 1);
-    p.touched[t] = v;
-  }
+  const bool first_touch = structural && atomicExch(&p.mark[v], p.mark_gen) != p.mark_gen;
+  const int t = wave_append(&p.ctl->touched_in, first_touch);
+  if (first_touch) p.touched[t] = v;
 }
 
 struct merge_params {
@@ -757,9 +780,13 @@ constexpr int kTailPerThread = kTailMax / kMergeBlock;
 static_assert(kMergeSmall % 64 == 0, "short tails are whole chunks of 64");
 static_assert(kTailMax % kMergeBlock == 0, "a long tail is whole slabs of a workgroup");
 
-/// How a touched row is merged: its tail rewritten by one wave (short tail, few operations) or by
-/// a workgroup, or the whole row rewritten as one base run.
-enum merge_mode { kTailWave = 0, kTailBlock = 1, kFull = 2, kModes = 3 };
+constexpr int kLaneTail = 32;  ///< a tail one thread merges in registers: at most this many entries
+constexpr int kLaneOps = 8;    ///< and at most this many operations
+
+/// How a touched row is merged: its tail rewritten by one thread (a short tail and a few
+/// operations, the common case), by one wave or by a workgroup, or the whole row rewritten as one
+/// base run.
+enum merge_mode { kTailWave = 0, kTailBlock = 1, kFull = 2, kTailLane = 3, kModes = 4 };
 
 DYNG_DEV int tail_limit(int base_len) { return min(kTailMax, max(kTailMin, base_len >> 3)); }
 DYNG_DEV int dead_limit(int base_len) { return max(16, base_len >> 2); }
@@ -795,7 +822,9 @@ DYNG_DEV row_plan plan_row(const merge_params& p, int32_t u, int2 mt, int64_t a,
   const int nops = static_cast<int>(b - a);
   const bool tail_ok = r.new_tail <= tail_limit(r.base_len) && r.base_len + r.new_tail <= p.cap[u] &&
                        p.dead[u] <= dead_limit(r.base_len) && p.K <= kRegWeights;
-  if (tail_ok && r.tail_len <= kMergeSmall && nops <= 64) {
+  if (tail_ok && r.tail_len <= kLaneTail && nops <= kLaneOps) {
+    r.mode = kTailLane;
+  } else if (tail_ok && r.tail_len <= kMergeSmall && nops <= 64) {
     r.mode = kTailWave;
   } else if (tail_ok && nops <= kMergeSeg) {
     r.mode = kTailBlock;
@@ -837,6 +866,81 @@ __global__ void k_merge_plan(merge_params p) {
   if (lane() == 0) {
     if (reloc) atomicAdd(reinterpret_cast<unsigned long long*>(p.need_reloc), static_cast<unsigned long long>(reloc));
     if (stage) atomicAdd(reinterpret_cast<unsigned long long*>(p.need_stage), static_cast<unsigned long long>(stage));
+  }
+}
+
+/// One thread per row whose tail holds at most kLaneTail entries and that has at most kLaneOps
+/// operations (no more than kRegWeights weight columns): the tail and the operations are read
+/// into registers with independent loads, every entry's place is counted with static indices
+/// (an alive entry: the alive entries before it plus the pending entries with a smaller
+/// neighbour; a pending entry: the alive entries up to its neighbour plus the pending entries
+/// before it), then each is stored at its place. Old entries come first among equal neighbours,
+/// as in the wave merge below.
+__global__ void __launch_bounds__(256) k_tail_lane(merge_params p) {
+  const int count = p.list_count[kTailLane];
+  const int32_t* list = p.lists + kTailLane * (p.N + 1);
+  const u64 mask = (1ULL << p.vbits) - 1;
+  const int K = p.K;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+    const int32_t u = list[t];
+    const int2 mt = p.meta[u];
+    const int64_t a = p.row_a[u];
+    const int nops = static_cast<int>(p.row_b[u] - a);
+    const int bl = p.base[u];
+    const int tl = mt.y - bl;
+    const int64_t at = mt.x + int64_t{bl};  // the tail's first slot
+    int32_t x[kLaneTail];
+    int32_t wr[kRegWeights][kLaneTail];
+#pragma unroll
+    for (int j = 0; j < kLaneTail; ++j) {
+      const bool in = j < tl;
+      x[j] = in ? p.nbr[at + j] : -1;
+#pragma unroll
+      for (int k = 0; k < kRegWeights; ++k) wr[k][j] = (in && k < K) ? p.w[k * p.slot_cap + at + j] : 0;
+    }
+    int32_t ov[kLaneOps];
+    int32_t ow[kRegWeights][kLaneOps];
+    bool pend[kLaneOps];
+#pragma unroll
+    for (int j = 0; j < kLaneOps; ++j) {
+      pend[j] = j < nops && p.pend_alive[a + j] != 0;
+      ov[j] = pend[j] ? static_cast<int32_t>(p.keys[a + j] & mask) : 0;
+#pragma unroll
+      for (int k = 0; k < kRegWeights; ++k) ow[k][j] = (pend[j] && k < K) ? p.pend_w[k * p.N + a + j] : 0;
+    }
+    // alive entries, at their places (the tail is sorted, dead entries carry the sign bit)
+    int alive = 0;
+#pragma unroll
+    for (int i = 0; i < kLaneTail; ++i) {
+      if (x[i] >= 0) {
+        int pos = alive;
+#pragma unroll
+        for (int j = 0; j < kLaneOps; ++j) pos += (pend[j] && ov[j] < x[i]) ? 1 : 0;
+        p.nbr[at + pos] = x[i];
+#pragma unroll
+        for (int k = 0; k < kRegWeights; ++k) {
+          if (k < K) p.w[k * p.slot_cap + at + pos] = wr[k][i];
+        }
+        ++alive;
+      }
+    }
+    // pending entries (the operations are sorted by neighbour, then batch order)
+    int pending = 0;
+#pragma unroll
+    for (int j = 0; j < kLaneOps; ++j) {
+      if (pend[j]) {
+        int pos = pending;
+#pragma unroll
+        for (int i = 0; i < kLaneTail; ++i) pos += (x[i] >= 0 && x[i] <= ov[j]) ? 1 : 0;
+        p.nbr[at + pos] = ov[j];
+#pragma unroll
+        for (int k = 0; k < kRegWeights; ++k) {
+          if (k < K) p.w[k * p.slot_cap + at + pos] = ow[k][j];
+        }
+        ++pending;
+      }
+    }
+    p.meta[u] = make_int2(mt.x, bl + alive + pending);
   }
 }
 
@@ -1679,7 +1783,8 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     mark_stage(g, 2, s);
     hipLaunchKernelGGL(k_make_ops, dim3(blocks_for(N)), dim3(kBlock), 0, s, L, ins_src, ins_dst, del_src, del_dst,
                        g->props.directed, g->props.on_self_loop == 1 ? 1 : 0, n_after, vbits, g->keys_a.p,
-                       g->vals_a.p, g->del_src.p, g->del_dst.p, g->ins_src.p, g->ins_dst.p, g->ins_inc.p, K, dc);
+                       g->vals_a.p, g->ikeys_a.p, g->ivals_a.p, g->del_src.p, g->del_dst.p, g->ins_src.p,
+                       g->ins_dst.p, g->ins_inc.p, K, dc);
     DYNG_HIP_TRY(hipGetLastError());
     mark_stage(g, 3, s);
     sort_pairs(g->sort_tmp, g->keys_a.p, g->keys_b.p, g->vals_a.p, g->vals_b.p, N, 2 * vbits, s);
@@ -1704,8 +1809,6 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     op.kill = g->kill.p;
     op.pend_w = g->pend_w.p;
     op.e_inc = g->ins_inc.p;
-    op.ikeys = g->ikeys_a.p;
-    op.ivals = g->ivals_a.p;
     op.iop_type = g->iop_type.p;
     op.iop_wold = g->iop_wold.p;
     op.iop_wnew = g->iop_wnew.p;
@@ -1837,6 +1940,10 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
       for (int side = 0; side < 2; ++side) {
         const merge_params mp = merge_setup(side == 0);
         const int* modes = side == 0 ? hc->modes_out : hc->modes_in;
+        if (modes[kTailLane] > 0) {
+          const int64_t blocks = std::min<int64_t>((modes[kTailLane] + 255) / 256, 8192);
+          hipLaunchKernelGGL(k_tail_lane, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, mp);
+        }
         if (modes[kTailWave] > 0) {
           const int64_t blocks = std::min<int64_t>((modes[kTailWave] + kMergeWaves - 1) / kMergeWaves, 8192);
           hipLaunchKernelGGL(k_tail_wave, dim3(static_cast<unsigned>(blocks)), dim3(kMergeBlock), 0, s, mp);

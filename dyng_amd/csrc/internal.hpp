@@ -136,7 +136,7 @@ struct apply_control {
   unsigned long long relocated_rows;
   int in_ops;
   int touched_out, touched_in;          ///< rows whose entries change
-  int modes_out[3], modes_in[3];        ///< rows per merge mode (wave tail, workgroup tail, full)
+  int modes_out[4], modes_in[4];        ///< rows per merge mode (wave tail, workgroup tail, full, thread tail)
   int error_bits;
   long long tail_out, tail_in;        ///< arena tails (atomic bump allocation)
   long long stage_out, stage_in;      ///< staging slots used
