@@ -17,6 +17,7 @@ bounded sample) and the bit-exactness check of the GPU trees against that CPU en
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import statistics
@@ -43,6 +44,11 @@ WORKLOADS = {
     # RMAT-23 (8,388,608 vertices, 134,217,728 edges) and a stream of batches of log-uniform size
     # in [10, 10^6] with a uniform delete fraction: the learned scheduler against the plain one
     "rmat23_learned": ("stream", 23, 134_217_728, 255, 4, 2000, (10, 1_000_000, 0)),
+    # 256 independent RMAT graphs, n = 2^floor(U[10, 20)) each, average degree 16, weights U[1, 100],
+    # one source each (its largest out-degree vertex); every step applies each instance's own batch
+    # of 1 % of its edges (half insertions of new RMAT edges, half deletions) and repairs all 256
+    # trees in one persistent launch replayed from a hipGraph, with autotuned schedule knobs
+    "batched256_rmat": ("batched", 256, None, 100, 3, 6, (0, 0, 0)),
 }
 
 
@@ -227,6 +233,145 @@ def stream_bench(args) -> int:
     return 0 if res["bit_exact"] else 3
 
 
+def batched_bench(args) -> int:
+    """The batched multi-instance configuration: one device graph holding 256 RMAT instances."""
+    import dyng_amd
+    from dyng_amd import batched, sssp
+    from dyng_amd import generators as G
+
+    kind, count, _, wmax, cfg, nb, _ = WORKLOADS[args.workload]
+    K, W = args.steps, args.warmup
+    nb = max(nb, K + W)
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(G.GRAPH_SEED ^ cfg)
+    scales = np.floor(rng.uniform(10.0, 20.0, count)).astype(int)
+
+    def make(i):
+        k = int(scales[i])
+        n_i = 1 << k
+        src, dst, w = G.rmat(k, 16 * n_i, wmax, (G.GRAPH_SEED ^ cfg) + 7919 * (i + 1))
+        half = max(1, len(src) // 200)
+        drawn_i = G.mixed_batches(src, dst, n_i, nb, half, half, 0, wmax, (G.BATCH_SEED ^ cfg) + 7919 * (i + 1),
+                                  rmat_scale=k)
+        return (n_i, src, dst, w), G.max_out_degree_vertex(src, n_i), drawn_i
+
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=min(16, len(os.sched_getaffinity(0)))) as pool:
+        made = list(pool.map(make, range(count)))  # the generators release the GIL
+    graphs = [x[0] for x in made]
+    sources = [x[1] for x in made]
+    drawn = [x[2] for x in made]
+    del made
+    m_total = sum(len(g[1]) for g in graphs)
+    n_total = sum(g[0] for g in graphs)
+    log(f"[bench] inputs {args.workload}: {count} instances, n={n_total} m={m_total}, {nb} batches each, "
+        f"in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    inst = batched.Instances(graphs)
+    tree = inst.compute(sources)
+    D = dyng_amd.DeviceArray
+
+    def union_batch(b):
+        parts = [[], [], [], [], []]
+        for i in range(count):
+            x = drawn[i][b]
+            off = int(inst.offsets[i])
+            for f, a in enumerate(x):
+                parts[f].append(a if f == 2 else np.asarray(a, np.int64) + off)
+        cat = [np.concatenate(a) for a in parts]
+        return dyng_amd.EdgeBatch(insert=(D(cat[0].astype(np.int32)), D(cat[1].astype(np.int32)), D(cat[2])),
+                                  delete=(D(cat[3].astype(np.int32)), D(cat[4].astype(np.int32))))
+
+    dev = [union_batch(b) for b in range(W + K)]
+    ops = [sum(len(drawn[i][b][0]) + len(drawn[i][b][3]) for i in range(count)) for b in range(W + K)]
+    dyng_amd.synchronize()
+    log(f"[bench] device union graph + compute of {count} trees in {time.perf_counter() - t0:.1f}s")
+    # warm-up batches: the autotuner times its candidates on the first one (on clones of the tree)
+    report = None
+    for b in range(W):
+        if b == 0:
+            report = batched.autotune(inst.graph, dev[0], tree)
+            tree.set_tuning(dataclasses.replace(report.tuning, launch="graph"))
+            log(f"[bench] autotune: light_degree {report.tuning.light_degree}, chunk {report.tuning.chunk_edges}, "
+                f"delta {report.delta} ({report.engine_ms:.3f} ms; {len(report.table)} candidates)")
+        else:
+            sssp.update(inst.graph, dev[b], tree)
+    wall, stats = [], []
+    for b in range(W, W + K):
+        dyng_amd.synchronize()
+        t1 = time.perf_counter()
+        st = sssp.update(inst.graph, dev[b], tree)
+        wall.append((time.perf_counter() - t1) * 1e3)
+        stats.append(st)
+    p50 = statistics.median(wall)
+    opsK = ops[W:]
+    value = statistics.median(o / (t / 1e3) for o, t in zip(opsK, wall))
+    log(f"[bench] gpu: p50 {p50:.3f} ms/step ({statistics.median(opsK)} ops), {value:.3e} edges/s")
+    kbytes = [fused_kernel_bytes(s.device, n_total, s.batch.deleted_edges + s.batch.ignored_deletions + o, o)
+              for s, o in zip(stats, opsK)]
+    kms = [s.device.kernel_ms for s in stats]
+    achieved = (sum(kbytes) / len(kbytes)) / (statistics.mean(kms) / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
+                "kernel": "sssp_fused_kernel", "kernel_ms_mean": round(statistics.mean(kms), 4),
+                "algorithmic_bytes_mean": int(sum(kbytes) / len(kbytes))}
+    # ---- parity and the CPU baseline on a sample of instances (the smallest and a spread) ----
+    cpu, exact = None, None
+    if not args.no_cpu and args.cpu_sample > 0:
+        import oracle
+
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        order = np.argsort(scales, kind="stable")
+        sample = sorted(set(order[:4].tolist() + order[len(order) // 2::max(1, len(order) // 8)][:4].tolist()))
+        d_all, p_all = tree.to_numpy()
+        ok, rep_s, rep_ops = True, 0.0, 0
+        for i in sample:
+            n_i, src, dst, w = graphs[i]
+            row = np.zeros(n_i + 1, np.int64)
+            np.cumsum(np.bincount(src, minlength=n_i), out=row[1:])
+            og = oracle.Csr(row, dst.astype(np.int32), w.astype(np.int32).reshape(1, -1))
+            ws = oracle.Workspace()
+            ot, _ = oracle.compute(og, int(sources[i]), 0, "omp", threads=threads, ws=ws)
+            for b in range(W + K):
+                x = drawn[i][b]
+                og, _, c = oracle.update(og, ot, x[0], x[1], x[2].reshape(-1, 1), x[3], x[4], engine="omp",
+                                         threads=threads, ws=ws)
+                if b >= W:
+                    rep_s += sum(c.t_hooks)
+                    rep_ops += len(x[0]) + len(x[3])
+            di, pi = inst.tree(tree, i, (d_all, p_all))
+            same = bool(np.array_equal(di, ot.dist) and np.array_equal(pi, ot.parent))
+            ok = ok and same
+            log(f"[bench] parity instance {i} (n=2^{scales[i]}): {'bit-exact' if same else 'MISMATCH'} after "
+                f"{W + K} batches")
+        exact = {"instances_checked": len(sample), "batches_each": W + K, "bit_exact": ok,
+                 "against": "OpenMP engine (oracle), each instance alone"}
+        cpu = {"value": round(rep_ops / max(rep_s, 1e-9), 1), "unit": "edges/s", "cores": threads, "kind": "port",
+               "sample": f"{len(sample)} instances x {K} batches (their own 1 % batches), repair time of the OpenMP "
+                         f"engine per instance (identify_affected + seed + loop + finalize)"}
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "edges/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(sum(wall) / K, 4), "p50_ms": round(p50, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic",
+        "config": {"workload": args.workload, "instances": count, "vertices": int(n_total), "edges": int(m_total),
+                   "batch_ops_p50": int(statistics.median(opsK)), "launch": "hipGraph replay, one launch per step",
+                   "autotuned": {"light_degree": report.tuning.light_degree if report else None,
+                                 "chunk_edges": report.tuning.chunk_edges if report else None,
+                                 "delta": report.delta if report else None},
+                   "apply_ms_p50": round(statistics.median(s.apply_ms for s in stats), 4),
+                   "engine_ms_p50": round(statistics.median(s.engine_ms for s in stats), 4)},
+        "roofline": roofline, "cpu_baseline": cpu, "parity": exact,
+    }
+    print(json.dumps(out), flush=True)
+    del dev, tree, inst
+    import gc
+
+    gc.collect()
+    dyng_amd.synchronize()
+    return 0 if (exact is None or exact["bit_exact"]) else 3
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +398,8 @@ def main() -> int:
         return 1
     if WORKLOADS[args.workload][0] == "stream":
         return stream_bench(args)
+    if WORKLOADS[args.workload][0] == "batched":
+        return batched_bench(args)
     K, W = args.steps, args.warmup
     n, src, dst, w, batch, source, ops = make_inputs(args.workload, K + W)
 
