@@ -115,6 +115,36 @@ DYNG_DEV u64 wave_sum(u64 v) {
   return v;
 }
 
+/// Reduce v over the workgroup with op; every thread gets the result. Every thread of the
+/// workgroup calls it. `part` is an LDS array of at least blockDim.x / 64 entries (reused across
+/// calls: the leading barrier keeps a call from overwriting what the previous one still reads).
+/// Counters that every workgroup bumps are single addresses, and device-scope atomics on one
+/// address serialize (1-2 ns each), so kernels reduce per workgroup before their atomics.
+template <class T, class Op>
+DYNG_DEV T block_reduce(T v, T* part, Op op) {
+  for (int off = 32; off > 0; off >>= 1) v = op(v, static_cast<T>(__shfl_xor(v, off)));
+  const int waves = static_cast<int>((blockDim.x + 63) / 64);
+  __syncthreads();
+  if (lane() == 0) part[threadIdx.x / 64] = v;
+  __syncthreads();
+  T t = part[0];
+  for (int w = 1; w < waves; ++w) t = op(t, part[w]);
+  return t;
+}
+
+struct op_sum {
+  template <class T>
+  DYNG_DEV T operator()(T a, T b) const { return a + b; }
+};
+struct op_min {
+  template <class T>
+  DYNG_DEV T operator()(T a, T b) const { return b < a ? b : a; }
+};
+struct op_max {
+  template <class T>
+  DYNG_DEV T operator()(T a, T b) const { return b > a ? b : a; }
+};
+
 /// Claim a vertex for a list in this generation: true for the first claimant only.
 DYNG_DEV bool claim(int* stamp, int32_t v, int generation) {
   return atomicExch(&stamp[v], generation) != generation;

@@ -258,23 +258,28 @@ __global__ void k_column_summary(const int2* meta, const int32_t* nbr, const int
 __global__ void k_batch_scan(const int32_t* ins_src, const int32_t* ins_dst, const int32_t* ins_w,
                              int64_t num_ins, const int32_t* del_src, const int32_t* del_dst,
                              int64_t num_del, int K, apply_control* ctl) {
-  int mn = INT32_MAX, mx = -1, sl_i = 0, sl_d = 0, wmin = INT32_MAX;
-  int lmax[4] = {0, 0, 0, 0};
+  __shared__ long long part[kBlock / 64];
+  long long mn = INT32_MAX, mx = -1, sl_i = 0, sl_d = 0, wmin = INT32_MAX;
+  long long lmax[4] = {0, 0, 0, 0};
   const int64_t stride = int64_t{gridDim.x} * blockDim.x;
   for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < num_ins; i += stride) {
     const int32_t u = ins_src[i], v = ins_dst[i];
-    mn = min(mn, min(u, v));
-    mx = max(mx, max(u, v));
+    mn = min(mn, static_cast<long long>(min(u, v)));
+    mx = max(mx, static_cast<long long>(max(u, v)));
     sl_i += u == v ? 1 : 0;
-    for (int k = 0; k < K; ++k) {
-      const int32_t x = ins_w[i * K + k];
-      wmin = min(wmin, x);
-      if (k < 4) lmax[k] = max(lmax[k], x); else atomicMax(&ctl->batch_wmax[k], x);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k < K) {
+        const long long x = ins_w[i * K + k];
+        wmin = min(wmin, x);
+        lmax[k] = max(lmax[k], x);
+      }
     }
-  }
-  for (int k = 0; k < 4 && k < K; ++k) {
-    for (int off = 32; off > 0; off >>= 1) lmax[k] = max(lmax[k], __shfl_xor(lmax[k], off));
-    if (lane() == 0 && lmax[k] > 0) atomicMax(&ctl->batch_wmax[k], lmax[k]);
+    for (int k = 4; k < K; ++k) {
+      const int32_t x = ins_w[i * K + k];
+      wmin = min(wmin, static_cast<long long>(x));
+      atomicMax(&ctl->batch_wmax[k], x);
+    }
   }
   // per-column minimum (rarely below 1; objective checks happen on the host)
   for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < num_ins; i += stride) {
@@ -285,23 +290,27 @@ __global__ void k_batch_scan(const int32_t* ins_src, const int32_t* ins_dst, con
   }
   for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < num_del; i += stride) {
     const int32_t u = del_src[i], v = del_dst[i];
-    mn = min(mn, min(u, v));
+    mn = min(mn, static_cast<long long>(min(u, v)));
     sl_d += u == v ? 1 : 0;
   }
-  // one atomic per wave and counter (the counters are single addresses every wave hits)
-  for (int off = 32; off > 0; off >>= 1) {
-    mn = min(mn, __shfl_xor(mn, off));
-    mx = max(mx, __shfl_xor(mx, off));
-    sl_i += __shfl_xor(sl_i, off);
-    sl_d += __shfl_xor(sl_d, off);
-    wmin = min(wmin, __shfl_xor(wmin, off));
-  }
-  if (lane() == 0) {
-    if (mn != INT32_MAX) atomicMin(&ctl->min_id, mn);
-    if (mx >= 0) atomicMax(&ctl->max_id, mx);
-    if (sl_i) atomicAdd(&ctl->self_loops_ins, sl_i);
-    if (sl_d) atomicAdd(&ctl->self_loops_del, sl_d);
-    if (wmin != INT32_MAX) atomicMin(&ctl->min_weight, wmin);
+  // one atomic per workgroup and counter
+  mn = block_reduce(mn, part, op_min{});
+  mx = block_reduce(mx, part, op_max{});
+  sl_i = block_reduce(sl_i, part, op_sum{});
+  sl_d = block_reduce(sl_d, part, op_sum{});
+  wmin = block_reduce(wmin, part, op_min{});
+#pragma unroll
+  for (int k = 0; k < 4; ++k) lmax[k] = block_reduce(lmax[k], part, op_max{});
+  if (threadIdx.x == 0) {
+    if (mn != INT32_MAX) atomicMin(&ctl->min_id, static_cast<int>(mn));
+    if (mx >= 0) atomicMax(&ctl->max_id, static_cast<int>(mx));
+    if (sl_i) atomicAdd(&ctl->self_loops_ins, static_cast<int>(sl_i));
+    if (sl_d) atomicAdd(&ctl->self_loops_del, static_cast<int>(sl_d));
+    if (wmin != INT32_MAX) atomicMin(&ctl->min_weight, static_cast<int>(wmin));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k < K && lmax[k] > 0) atomicMax(&ctl->batch_wmax[k], static_cast<int>(lmax[k]));
+    }
   }
 }
 
@@ -320,7 +329,6 @@ struct op_layout {
 __global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* ins_dst,
                            const int32_t* del_src, const int32_t* del_dst, int directed,
                            int drop_self_loops, int64_t n_after, int vbits, u64* keys, uint32_t* vals,
-                           u64* ikeys, uint32_t* ivals,
                            int32_t* e_del_src, int32_t* e_del_dst, int32_t* e_ins_src,
                            int32_t* e_ins_dst, uint8_t* e_inc, int K, apply_control* ctl) {
   const int64_t total = (L.num_first + L.num_second) * L.stride;
@@ -343,12 +351,17 @@ __global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* i
       valid = false;
       ignored += 1;
     }
-    keys[e] = valid ? ((static_cast<u64>(u) << vbits) | static_cast<u64>(v)) : kNoKey;
+    // Both directions' keys go through one sort: the out-row key (u, v) at e and the in-row key
+    // (v, u) at total + e, tagged with bit 2 * vbits so that every out-key sorts before every
+    // in-key (what an operation does to the in-row is decided later by the out-row pass). A
+    // dropped operation takes the all-ones key of its half (no vertex id is all ones); k_untag
+    // restores kNoKey and removes the tag after the sort.
+    const u64 low = (u64{1} << (2 * vbits)) - 1;
+    const u64 tag = u64{1} << (2 * vbits);
+    keys[e] = valid ? ((static_cast<u64>(u) << vbits) | static_cast<u64>(v)) : low;
     vals[e] = static_cast<uint32_t>(e);
-    // the in-rows' key of the same operation; what the operation does to the in-row is decided by
-    // the out-row pass (iop_type), so the in-keys can be sorted meanwhile
-    ikeys[e] = valid ? ((static_cast<u64>(v) << vbits) | static_cast<u64>(u)) : kNoKey;
-    ivals[e] = static_cast<uint32_t>(e);
+    keys[total + e] = tag | (valid ? ((static_cast<u64>(v) << vbits) | static_cast<u64>(u)) : low);
+    vals[total + e] = static_cast<uint32_t>(e);
     if (del) {
       e_del_src[loc] = valid ? u : -1;
       e_del_dst[loc] = valid ? v : -1;
@@ -363,6 +376,15 @@ __global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* i
   if (lane() == 0) {
     if (dropped) atomicAdd(&ctl->dropped_self_loops, dropped);
     if (ignored) atomicAdd(&ctl->ignored_deletions, ignored);
+  }
+}
+
+/// After the joint sort: the keys without their direction tag, kNoKey for dropped operations.
+__global__ void k_untag(u64* keys, int64_t count, int vbits) {
+  const u64 low = (u64{1} << (2 * vbits)) - 1;
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < count; i += int64_t{gridDim.x} * blockDim.x) {
+    const u64 k = keys[i] & low;
+    keys[i] = k == low ? kNoKey : k;
   }
 }
 
@@ -596,20 +618,21 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
       if (first_touch) p.touched[t] = u;
     }
   }
-  c_ins = wave_sum(c_ins);
-  c_upd = wave_sum(c_upd);
-  c_del = wave_sum(c_del);
-  c_ign_del = wave_sum(c_ign_del);
-  c_ign_ins = wave_sum(c_ign_ins);
-  const long long dsums[4] = {static_cast<long long>(wave_sum(static_cast<u64>(ds0))),
-                              static_cast<long long>(wave_sum(static_cast<u64>(ds1))),
-                              static_cast<long long>(wave_sum(static_cast<u64>(ds2))),
-                              static_cast<long long>(wave_sum(static_cast<u64>(ds3)))};
-  const long long dcnts[4] = {static_cast<long long>(wave_sum(static_cast<u64>(dc0))),
-                              static_cast<long long>(wave_sum(static_cast<u64>(dc1))),
-                              static_cast<long long>(wave_sum(static_cast<u64>(dc2))),
-                              static_cast<long long>(wave_sum(static_cast<u64>(dc3)))};
-  if (lane() == 0) {
+  __shared__ unsigned long long part[kBlock / 64];
+  c_ins = block_reduce(c_ins, part, op_sum{});
+  c_upd = block_reduce(c_upd, part, op_sum{});
+  c_del = block_reduce(c_del, part, op_sum{});
+  c_ign_del = block_reduce(c_ign_del, part, op_sum{});
+  c_ign_ins = block_reduce(c_ign_ins, part, op_sum{});
+  const long long ds[4] = {static_cast<long long>(block_reduce(static_cast<u64>(ds0), part, op_sum{})),
+                           static_cast<long long>(block_reduce(static_cast<u64>(ds1), part, op_sum{})),
+                           static_cast<long long>(block_reduce(static_cast<u64>(ds2), part, op_sum{})),
+                           static_cast<long long>(block_reduce(static_cast<u64>(ds3), part, op_sum{}))};
+  const long long dc[4] = {static_cast<long long>(block_reduce(static_cast<u64>(dc0), part, op_sum{})),
+                           static_cast<long long>(block_reduce(static_cast<u64>(dc1), part, op_sum{})),
+                           static_cast<long long>(block_reduce(static_cast<u64>(dc2), part, op_sum{})),
+                           static_cast<long long>(block_reduce(static_cast<u64>(dc3), part, op_sum{}))};
+  if (threadIdx.x == 0) {
     if (c_ins) atomicAdd(&p.ctl->inserted_edges, c_ins);
     if (c_upd) atomicAdd(&p.ctl->updated_edges, c_upd);
     if (c_del) atomicAdd(&p.ctl->deleted_edges, c_del);
@@ -618,8 +641,8 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (k >= K) break;
-      if (dsums[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dsum[k]), static_cast<unsigned long long>(dsums[k]));
-      if (dcnts[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dcount[k]), static_cast<unsigned long long>(dcnts[k]));
+      if (ds[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dsum[k]), static_cast<unsigned long long>(ds[k]));
+      if (dc[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dcount[k]), static_cast<unsigned long long>(dc[k]));
     }
   }
 }
@@ -1662,7 +1685,7 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
   for (int k = 0; k < max_weights; ++k) hc->batch_wmin[k] = 1;
   DYNG_HIP_TRY(hipMemcpyAsync(g->ctl.p, hc, sizeof(apply_control), hipMemcpyHostToDevice, s));
   if (num_ins + num_del > 0) {
-    hipLaunchKernelGGL(k_batch_scan, dim3(std::min<unsigned>(blocks_for(std::max(num_ins, num_del)), 1024)),
+    hipLaunchKernelGGL(k_batch_scan, dim3(std::min<unsigned>(blocks_for(std::max(num_ins, num_del)), 256)),
                        dim3(kBlock), 0, s, ins_src, ins_dst, ins_w, num_ins, del_src, del_dst, num_del, K, g->ctl.p);
     DYNG_HIP_TRY(hipGetLastError());
   }
@@ -1740,19 +1763,15 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
   g->ins_src.reserve(g->num_ins_slots);
   g->ins_dst.reserve(g->num_ins_slots);
   g->ins_inc.reserve(g->num_ins_slots * K);
-  g->keys_a.reserve(N);
-  g->keys_b.reserve(N);
-  g->vals_a.reserve(N);
-  g->vals_b.reserve(N);
+  g->keys_a.reserve(2 * N);  // out-row keys, then in-row keys
+  g->keys_b.reserve(2 * N);
+  g->vals_a.reserve(2 * N);
+  g->vals_b.reserve(2 * N);
   g->pend_alive.reserve(N + 1);
   g->kill.reserve(N + 1);
   g->pend_prefix.reserve(N + 1);
   g->kill_prefix.reserve(N + 1);
   g->pend_w.reserve(N * K);
-  g->ikeys_a.reserve(N);
-  g->ikeys_b.reserve(N);
-  g->ivals_a.reserve(N);
-  g->ivals_b.reserve(N);
   g->iop_type.reserve(N);
   g->iop_wold.reserve(N * K);
   g->iop_wnew.reserve(N * K);
@@ -1783,11 +1802,14 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     mark_stage(g, 2, s);
     hipLaunchKernelGGL(k_make_ops, dim3(blocks_for(N)), dim3(kBlock), 0, s, L, ins_src, ins_dst, del_src, del_dst,
                        g->props.directed, g->props.on_self_loop == 1 ? 1 : 0, n_after, vbits, g->keys_a.p,
-                       g->vals_a.p, g->ikeys_a.p, g->ivals_a.p, g->del_src.p, g->del_dst.p, g->ins_src.p,
-                       g->ins_dst.p, g->ins_inc.p, K, dc);
+                       g->vals_a.p, g->del_src.p, g->del_dst.p, g->ins_src.p, g->ins_dst.p, g->ins_inc.p, K, dc);
     DYNG_HIP_TRY(hipGetLastError());
     mark_stage(g, 3, s);
-    sort_pairs(g->sort_tmp, g->keys_a.p, g->keys_b.p, g->vals_a.p, g->vals_b.p, N, 2 * vbits, s);
+    // one stable sort of both directions' keys (2N of them, 2 * vbits + 1 bits): one sort of 2N
+    // keys costs little more than one of N, since rocPRIM's small-size path is launch-bound
+    sort_pairs(g->sort_tmp, g->keys_a.p, g->keys_b.p, g->vals_a.p, g->vals_b.p, 2 * N, 2 * vbits + 1, s);
+    hipLaunchKernelGGL(k_untag, dim3(std::min<unsigned>(blocks_for(2 * N), 1024)), dim3(kBlock), 0, s, g->keys_b.p,
+                       2 * N, vbits);
     mark_stage(g, 4, s);
     out_pair_params op{};
     op.L = L;
@@ -1825,11 +1847,10 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     exclusive_scan<int32_t>(g->scan_tmp, g->kill.p, g->kill_prefix.p, N + 1, s);
     mark_stage(g, 6, s);
     // ---- the in-rows' operations (they do not read the out-rows) ----
-    sort_pairs(g->sort_tmp, g->ikeys_a.p, g->ikeys_b.p, g->ivals_a.p, g->ivals_b.p, N, 2 * vbits, s);
     mark_stage(g, 7, s);
     in_pair_params ip{};
-    ip.keys = g->ikeys_b.p;
-    ip.vals = g->ivals_b.p;
+    ip.keys = g->keys_b.p + N;
+    ip.vals = g->vals_b.p + N;
     ip.N = N;
     ip.vbits = vbits;
     ip.K = K;
@@ -1870,7 +1891,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
       mp.w = rs.w.p;
       mp.slot_cap = rs.slot_cap;
       mp.K = K;
-      mp.keys = outs ? g->keys_b.p : g->ikeys_b.p;
+      mp.keys = outs ? g->keys_b.p : g->keys_b.p + N;
       mp.N = N;
       mp.vbits = vbits;
       mp.pend_alive = outs ? g->pend_alive.p : g->ipend_alive.p;

@@ -213,8 +213,8 @@ struct dyng_hip_graph {
   int64_t num_del_slots = 0, num_ins_slots = 0;
 
   // apply scratch
-  dyng_hip::dbuf<unsigned long long> keys_a, keys_b, ikeys_a, ikeys_b;
-  dyng_hip::dbuf<uint32_t> vals_a, vals_b, ivals_a, ivals_b;
+  dyng_hip::dbuf<unsigned long long> keys_a, keys_b;  ///< the apply's sort: out-row keys, then in-row keys
+  dyng_hip::dbuf<uint32_t> vals_a, vals_b;
   dyng_hip::dbuf<unsigned char> sort_tmp, scan_tmp;
   dyng_hip::dbuf<int32_t> pend_alive, kill, pend_prefix, kill_prefix, pend_w;
   dyng_hip::dbuf<int32_t> ipend_alive, ikill, ipend_prefix, ikill_prefix, ipend_w;

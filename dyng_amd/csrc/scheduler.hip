@@ -204,7 +204,7 @@ struct feature_acc {
 
 /// Over the batch (on G_t, before it is applied): deletions that hit the old tree (parent[v] ==
 /// u) and the out-degrees of the operations' tail vertices.
-__global__ void k_batch_features(const int32_t* ins_src, int64_t num_ins, const int32_t* del_src,
+__global__ void __launch_bounds__(256) k_batch_features(const int32_t* ins_src, int64_t num_ins, const int32_t* del_src,
                                  const int32_t* del_dst, int64_t num_del, const int32_t* parents, int64_t n_tree,
                                  const int2* meta, int64_t n, feature_acc* acc) {
   unsigned long long hits = 0, touched = 0, logs = 0, mx = 0;
@@ -225,11 +225,13 @@ __global__ void k_batch_features(const int32_t* ins_src, int64_t num_ins, const 
       mx = d > mx ? d : mx;
     }
   }
-  hits = wave_sum(hits);
-  touched = wave_sum(touched);
-  logs = wave_sum(logs);
-  mx = ~wave_min(~mx);
-  if (lane() == 0) {
+  // one atomic per workgroup and counter
+  __shared__ unsigned long long part[4];
+  hits = block_reduce(hits, part, op_sum{});
+  touched = block_reduce(touched, part, op_sum{});
+  logs = block_reduce(logs, part, op_sum{});
+  mx = block_reduce(mx, part, op_max{});
+  if (threadIdx.x == 0) {
     if (hits) atomicAdd(&acc->tree_deletions, hits);
     if (touched) atomicAdd(&acc->touched, touched);
     if (logs) atomicAdd(&acc->log_degree_sum, logs);
@@ -237,32 +239,79 @@ __global__ void k_batch_features(const int32_t* ins_src, int64_t num_ins, const 
   }
 }
 
-/// The network's 12 inputs from the batch features, on the device (the same formulas as the
-/// Python `scheduler.features`).
-__global__ void k_assemble_input(feature_acc* acc, int64_t num_ins, int64_t num_del, int64_t n, int64_t m,
-                                 int64_t last_invalidated, int64_t last_affected, float* x) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+/// Input j of the network from the batch features (the formulas of the Python
+/// `scheduler.features`).
+DYNG_DEV float feature_input(const feature_acc& a, int j, int64_t num_ins, int64_t num_del, int64_t n, int64_t m,
+                             int64_t last_invalidated, int64_t last_affected) {
   const double ops = static_cast<double>(num_ins + num_del);
   const double nn = static_cast<double>(n > 1 ? n : 1), mm = static_cast<double>(m > 1 ? m : 1);
-  const double touched = static_cast<double>(acc->touched);
-  x[0] = static_cast<float>(log2(1.0 + ops));
-  x[1] = static_cast<float>(static_cast<double>(num_del) / (ops > 1.0 ? ops : 1.0));
-  x[2] = static_cast<float>(log2(1.0 + static_cast<double>(num_del)));
-  x[3] = static_cast<float>(log2(1.0 + static_cast<double>(num_ins)));
-  x[4] = static_cast<float>(log2(nn));
-  x[5] = static_cast<float>(log2(mm / nn));
-  x[6] = static_cast<float>(static_cast<double>(acc->tree_deletions) / static_cast<double>(num_del > 1 ? num_del : 1));
-  x[7] = static_cast<float>(log2(1.0 + static_cast<double>(acc->tree_deletions)));
-  x[8] = static_cast<float>(touched > 0.0 ? static_cast<double>(acc->log_degree_sum) / 1024.0 / touched : 0.0);
-  x[9] = static_cast<float>(log2(1.0 + static_cast<double>(acc->max_degree)));
-  x[10] = static_cast<float>(log2(1.0 + static_cast<double>(last_invalidated)));
-  x[11] = static_cast<float>(log2(1.0 + static_cast<double>(last_affected)));
-  // cleared for the next batch (the accumulators start at zero when allocated)
-  *acc = feature_acc{0, 0, 0, 0};
+  const double touched = static_cast<double>(a.touched);
+  double x = 0.0;
+  switch (j) {
+    case 0: x = log2(1.0 + ops); break;
+    case 1: x = static_cast<double>(num_del) / (ops > 1.0 ? ops : 1.0); break;
+    case 2: x = log2(1.0 + static_cast<double>(num_del)); break;
+    case 3: x = log2(1.0 + static_cast<double>(num_ins)); break;
+    case 4: x = log2(nn); break;
+    case 5: x = log2(mm / nn); break;
+    case 6: x = static_cast<double>(a.tree_deletions) / static_cast<double>(num_del > 1 ? num_del : 1); break;
+    case 7: x = log2(1.0 + static_cast<double>(a.tree_deletions)); break;
+    case 8: x = touched > 0.0 ? static_cast<double>(a.log_degree_sum) / 1024.0 / touched : 0.0; break;
+    case 9: x = log2(1.0 + static_cast<double>(a.max_degree)); break;
+    case 10: x = log2(1.0 + static_cast<double>(last_invalidated)); break;
+    default: x = log2(1.0 + static_cast<double>(last_affected)); break;
+  }
+  return static_cast<float>(x);
 }
 
-unsigned int grid_for(int64_t work, int block) {
-  return static_cast<unsigned int>(std::max<int64_t>(1, std::min<int64_t>((work + block - 1) / block, 65535)));
+/// The network's 12 inputs from the batch features, on the device; the accumulators are cleared
+/// for the next batch (they start at zero when allocated).
+__global__ void k_assemble_input(feature_acc* acc, int64_t num_ins, int64_t num_del, int64_t n, int64_t m,
+                                 int64_t last_invalidated, int64_t last_affected, float* x) {
+  if (blockIdx.x != 0) return;
+  const feature_acc a = *acc;
+  if (threadIdx.x < kIn) x[threadIdx.x] = feature_input(a, threadIdx.x, num_ins, num_del, n, m, last_invalidated, last_affected);
+  __syncthreads();
+  if (threadIdx.x == 0) *acc = feature_acc{0, 0, 0, 0};
+}
+
+/// Features to prediction in one workgroup (the scheduled update's path): the 12 inputs from the
+/// accumulators (then cleared), normalized, and the BF16 forward of k_predict.
+__global__ void __launch_bounds__(kHidden) k_infer(const uint16_t* H, const float* P, feature_acc* acc,
+                                                   int64_t num_ins, int64_t num_del, int64_t n, int64_t m,
+                                                   int64_t last_invalidated, int64_t last_affected, float* Y) {
+  __shared__ uint32_t hw[(kTrained + 1) / 2];
+  __shared__ float xs[kIn];
+  __shared__ float h1[kHidden];
+  __shared__ float h2[kHidden];
+  const int j = threadIdx.x;
+  const uint32_t* H2 = reinterpret_cast<const uint32_t*>(H);
+  for (int q = j; q < kTrained / 2; q += kHidden) hw[q] = H2[q];
+  if ((kTrained & 1) && j == 0) hw[kTrained / 2] = H[kTrained - 1];
+  if (j < kIn) {
+    const float x = feature_input(*acc, j, num_ins, num_del, n, m, last_invalidated, last_affected);
+    xs[j] = round_bf16((x - P[kMean + j]) * P[kScale + j]);
+  }
+  __syncthreads();
+  if (j == 0) *acc = feature_acc{0, 0, 0, 0};
+  const uint16_t* h = reinterpret_cast<const uint16_t*>(hw);
+  float a = from_bf16(h[kB1 + j]);
+  for (int i = 0; i < kIn; ++i) a += from_bf16(h[kW1 + j * kIn + i]) * xs[i];
+  h1[j] = round_bf16(a > 0.0f ? a : 0.0f);
+  __syncthreads();
+  a = from_bf16(h[kB2 + j]);
+  for (int k = 0; k < kHidden; ++k) a += from_bf16(h[kW2 + j * kHidden + k]) * h1[k];
+  h2[j] = round_bf16(a > 0.0f ? a : 0.0f);
+  __syncthreads();
+  if (j < kOut) {
+    a = from_bf16(h[kB3 + j]);
+    for (int k = 0; k < kHidden; ++k) a += from_bf16(h[kW3 + j * kHidden + k]) * h2[k];
+    Y[j] = a;
+  }
+}
+
+unsigned int grid_for(int64_t work, int block, int64_t cap = 65535) {
+  return static_cast<unsigned int>(std::max<int64_t>(1, std::min<int64_t>((work + block - 1) / block, cap)));
 }
 
 }  // namespace
@@ -357,12 +406,11 @@ void mlp_predict_async(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_sca
   }
   const int64_t work = sc.num_ins + sc.num_del;
   if (work > 0) {
-    hipLaunchKernelGGL(k_batch_features, dim3(grid_for(work, 256)), dim3(256), 0, s, sc.ins_src, sc.num_ins, sc.del_src,
-                       sc.del_dst, sc.num_del, parents, n_tree, g->out.meta.p, g->n, m->acc.p);
+    hipLaunchKernelGGL(k_batch_features, dim3(grid_for(work, 256, 256)), dim3(256), 0, s, sc.ins_src, sc.num_ins,
+                       sc.del_src, sc.del_dst, sc.num_del, parents, n_tree, g->out.meta.p, g->n, m->acc.p);
   }
-  hipLaunchKernelGGL(k_assemble_input, dim3(1), dim3(64), 0, s, m->acc.p, sc.num_ins, sc.num_del, g->n, g->m,
-                     last_invalidated, last_affected, m->X.p);
-  hipLaunchKernelGGL(k_predict, dim3(1), dim3(kHidden), 0, s, m->bf16.p, m->params.p, m->X.p, 1, m->Y.p);
+  hipLaunchKernelGGL(k_infer, dim3(1), dim3(kHidden), 0, s, m->bf16.p, m->params.p, m->acc.p, sc.num_ins, sc.num_del,
+                     g->n, g->m, last_invalidated, last_affected, m->Y.p);
   DYNG_HIP_TRY(hipGetLastError());
   DYNG_HIP_TRY(hipMemcpyAsync(m->host_y.p->y, m->Y.p, kOut * sizeof(float), hipMemcpyDeviceToHost, s));
 }
@@ -385,8 +433,8 @@ int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, c
   }
   const int64_t work = sc.num_ins + sc.num_del;
   if (work > 0) {
-    hipLaunchKernelGGL(k_batch_features, dim3(grid_for(work, 256)), dim3(256), 0, s, sc.ins_src, sc.num_ins, sc.del_src,
-                       sc.del_dst, sc.num_del, parents, n_tree, g->out.meta.p, g->n, m->acc.p);
+    hipLaunchKernelGGL(k_batch_features, dim3(grid_for(work, 256, 256)), dim3(256), 0, s, sc.ins_src, sc.num_ins,
+                       sc.del_src, sc.del_dst, sc.num_del, parents, n_tree, g->out.meta.p, g->n, m->acc.p);
   }
   hipLaunchKernelGGL(k_assemble_input, dim3(1), dim3(64), 0, s, m->acc.p, sc.num_ins, sc.num_del, g->n, g->m,
                      last_invalidated, last_affected, m->X.p);
@@ -575,7 +623,7 @@ extern "C" int dyng_hip_batch_features(const dyng_hip_graph* g, const dyng_hip_b
     DYNG_HIP_TRY(hipMemsetAsync(acc.p, 0, sizeof(feature_acc), s));
     const int64_t work = b->num_ins + b->num_del;
     if (work > 0) {
-      hipLaunchKernelGGL(k_batch_features, dim3(grid_for(work, 256)), dim3(256), 0, s, ins_src, b->num_ins, del_src,
+      hipLaunchKernelGGL(k_batch_features, dim3(grid_for(work, 256, 256)), dim3(256), 0, s, ins_src, b->num_ins, del_src,
                          del_dst, b->num_del, parents, n_tree, g->out.meta.p, g->n, acc.p);
       DYNG_HIP_TRY(hipGetLastError());
     }
