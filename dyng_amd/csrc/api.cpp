@@ -500,13 +500,16 @@ extern "C" int dyng_hip_api_update_scheduled(dyng_hip_graph* g, const dyng_hip_b
     }
     // ---- the scheduler's features and inference, on G_t before anything changes; the apply's
     // own synchronization brings the prediction back ----
-    if (mlp != nullptr) mlp_predict_async(mlp, g, sc, r->parents.p, r->n, last_invalidated, last_affected, s);
+    hipEvent_t predicted = nullptr;
+    if (mlp != nullptr) {
+      predicted = mlp_predict_async(mlp, g, sc, r->parents.p, r->n, last_invalidated, last_affected);
+    }
     // ---- commit: the device apply ----
     event_pair ev_apply;
     dyng_hip_apply_summary summary{};
     DYNG_HIP_TRY(hipEventRecord(ev_apply.a, s));
     try {
-      apply_scanned(g, sc, s, &summary);
+      apply_scanned(g, sc, s, &summary, predicted);
     } catch (...) {
       r->poisoned = true;  // the graph may be partly updated
       throw;

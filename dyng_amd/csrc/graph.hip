@@ -1723,7 +1723,8 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
   }
 }
 
-void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out) {
+void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out,
+                   hipEvent_t wait) {
   const int K = g->K;
   // G_t as the batch found it (the repair of a result one apply behind reads it)
   g->last.valid = false;
@@ -1918,6 +1919,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
       return mp;
     };
     const unsigned plan_grid = static_cast<unsigned>(std::min<int64_t>(blocks_for(N), 1024));
+    if (wait != nullptr) DYNG_HIP_TRY(hipStreamWaitEvent(s, wait, 0));  // before the rows change
     for (int attempt = 0;; ++attempt) {
       hc->need_reloc_out = hc->need_reloc_in = hc->need_stage_out = hc->need_stage_in = 0;
       for (int i = 0; i < kModes; ++i) hc->modes_out[i] = hc->modes_in[i] = 0;
@@ -1982,6 +1984,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
       }
     }
   }
+  if (wait != nullptr) DYNG_HIP_TRY(hipStreamWaitEvent(s, wait, 0));
   DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
   DYNG_HIP_TRY(hipStreamSynchronize(s));
   if (hc->error_bits != 0) {

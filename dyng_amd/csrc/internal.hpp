@@ -175,12 +175,16 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
 /// The scheduler's pick for a scanned batch on G_t: features, BF16 inference, argmin (one sync).
 int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents, int64_t n_tree,
                int64_t last_invalidated, int64_t last_affected, hipStream_t s);
-/// The same without waiting: the prediction lands in pinned host memory, ready after the stream's
-/// next synchronization (the apply's); mlp_pick then reads the choice.
-void mlp_predict_async(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents,
-                       int64_t n_tree, int64_t last_invalidated, int64_t last_affected, hipStream_t s);
+/// The same without waiting, on the network's own stream beside the apply: the prediction lands
+/// in pinned host memory; the returned event marks it (and the end of the features' reads of the
+/// rows). The apply waits for it before its merges and read-backs; mlp_pick then reads the choice.
+hipEvent_t mlp_predict_async(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents,
+                             int64_t n_tree, int64_t last_invalidated, int64_t last_affected);
 int mlp_pick(const dyng_hip_mlp* m);
-void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out);
+/// The apply after scan_batch. `wait` (optional): an event on another stream that must complete
+/// before the merges change the rows (and before the apply's read-backs).
+void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out,
+                   hipEvent_t wait = nullptr);
 }  // namespace dyng_hip
 
 /// The device graph behind the dyng_hip_graph handle.

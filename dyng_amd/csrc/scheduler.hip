@@ -321,6 +321,17 @@ using namespace dyng_hip;
 
 /// The network's state on one device: FP32 parameters, their BF16 copy, Adam's moments and the
 /// scratch of a training minibatch.
+/// The schedule to run from the predicted log2 engine times: the plain one (0) unless another is
+/// predicted at least kSwitchMargin faster (a misprediction costs more than the small gains the
+/// other schedules offer on most batches).
+constexpr float kSwitchMargin = 0.07f;  // log2(1.05)
+
+int pick_schedule(const float* y) {
+  int best = 0;
+  for (int o = 1; o < kOut; ++o) best = y[o] < y[best] ? o : best;
+  return (best != 0 && y[best] < y[0] - kSwitchMargin) ? best : 0;
+}
+
 struct dyng_hip_mlp {
   int device = 0;
   dbuf<float> params, grads, adam_m, adam_v;
@@ -334,13 +345,26 @@ struct dyng_hip_mlp {
     float y[kOut];
   };
   pinned<prediction> host_y;  ///< the last prediction, read after the stream's next synchronization
+  /// the scheduled update's features and inference run on this stream, beside the apply
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;  ///< recorded after the prediction's read-back
+  dyng_hip_mlp() = default;
+  dyng_hip_mlp(const dyng_hip_mlp&) = delete;
+  dyng_hip_mlp& operator=(const dyng_hip_mlp&) = delete;
+  ~dyng_hip_mlp() {
+    if (stream != nullptr) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+    if (done != nullptr) (void)hipEventDestroy(done);
+  }
 };
 
 namespace {
 
-void refresh_bf16(dyng_hip_mlp* m) {
+void refresh_bf16(dyng_hip_mlp* m, hipStream_t s) {
   if (m->bf16_fresh) return;
-  hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(kTrained, 256)), dim3(256), 0, 0, m->params.p, m->bf16.p);
+  hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(kTrained, 256)), dim3(256), 0, s, m->params.p, m->bf16.p);
   DYNG_HIP_TRY(hipGetLastError());
   m->bf16_fresh = true;
 }
@@ -395,9 +419,13 @@ uint64_t mix(uint64_t z) {
 
 namespace dyng_hip {
 
-void mlp_predict_async(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents,
-                       int64_t n_tree, int64_t last_invalidated, int64_t last_affected, hipStream_t s) {
-  refresh_bf16(m);
+hipEvent_t mlp_predict_async(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents,
+                             int64_t n_tree, int64_t last_invalidated, int64_t last_affected) {
+  // on the network's own stream: the batch scan's synchronization left the batch on the device,
+  // and the apply changes neither the parents nor the out-row lengths before its merges, which
+  // wait for `done`
+  const hipStream_t s = m->stream;
+  refresh_bf16(m, s);
   m->X.reserve(kIn);
   m->Y.reserve(kOut);
   if (m->acc.p == nullptr) {
@@ -413,18 +441,15 @@ void mlp_predict_async(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_sca
                      g->n, g->m, last_invalidated, last_affected, m->Y.p);
   DYNG_HIP_TRY(hipGetLastError());
   DYNG_HIP_TRY(hipMemcpyAsync(m->host_y.p->y, m->Y.p, kOut * sizeof(float), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipEventRecord(m->done, s));
+  return m->done;
 }
 
-int mlp_pick(const dyng_hip_mlp* m) {
-  int best = 0;
-  const float* y = m->host_y.p->y;
-  for (int o = 1; o < kOut; ++o) best = y[o] < y[best] ? o : best;
-  return best;
-}
+int mlp_pick(const dyng_hip_mlp* m) { return pick_schedule(m->host_y.p->y); }
 
 int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents, int64_t n_tree,
                int64_t last_invalidated, int64_t last_affected, hipStream_t s) {
-  refresh_bf16(m);
+  refresh_bf16(m, s);
   m->X.reserve(kIn);
   m->Y.reserve(kOut);
   if (m->acc.p == nullptr) {
@@ -443,9 +468,7 @@ int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, c
   float y[kOut];
   DYNG_HIP_TRY(hipMemcpyAsync(y, m->Y.p, sizeof(y), hipMemcpyDeviceToHost, s));
   DYNG_HIP_TRY(hipStreamSynchronize(s));
-  int best = 0;
-  for (int o = 1; o < kOut; ++o) best = y[o] < y[best] ? o : best;
-  return best;
+  return pick_schedule(y);
 }
 
 }  // namespace dyng_hip
@@ -461,6 +484,8 @@ extern "C" int dyng_hip_mlp_create(int device, uint64_t seed, dyng_hip_mlp** out
     m->adam_m.reserve(kTrained);
     m->adam_v.reserve(kTrained);
     m->bf16.reserve(kTrained);
+    DYNG_HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    DYNG_HIP_TRY(hipEventCreateWithFlags(&m->done, hipEventDisableTiming));
     // He initialization (uniform, variance 2 / fan_in), biases 0, identity normalization
     std::vector<float> h(kAll, 0.0f);
     uint64_t state = seed * 0x9E3779B97F4A7C15ULL + 1;
@@ -491,6 +516,7 @@ extern "C" int dyng_hip_mlp_get_params(const dyng_hip_mlp* m, float* out, int64_
   return guarded([&] {
     if (m == nullptr || out == nullptr || count != kAll) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: get_params needs DYNG_HIP_MLP_PARAMS floats");
     DYNG_HIP_TRY(hipSetDevice(m->device));
+    DYNG_HIP_TRY(hipStreamSynchronize(m->stream));  // a scheduled update may still read the buffers
     DYNG_HIP_TRY(hipMemcpy(out, m->params.p, kAll * sizeof(float), hipMemcpyDeviceToHost));
   });
 }
@@ -502,6 +528,7 @@ extern "C" int dyng_hip_mlp_set_params(dyng_hip_mlp* m, const float* in, int64_t
       if (!(in[kScale + i] > 0.0f) || !std::isfinite(in[kScale + i])) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: input scales must be positive");
     }
     DYNG_HIP_TRY(hipSetDevice(m->device));
+    DYNG_HIP_TRY(hipStreamSynchronize(m->stream));  // a scheduled update may still read the buffers
     DYNG_HIP_TRY(hipMemcpy(m->params.p, in, kAll * sizeof(float), hipMemcpyHostToDevice));
     DYNG_HIP_TRY(hipMemset(m->adam_m.p, 0, kTrained * sizeof(float)));
     DYNG_HIP_TRY(hipMemset(m->adam_v.p, 0, kTrained * sizeof(float)));
@@ -517,6 +544,7 @@ extern "C" int dyng_hip_mlp_gradients(dyng_hip_mlp* m, const float* x, const flo
       fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: gradients need 1 to 2^20 samples and output arrays");
     }
     DYNG_HIP_TRY(hipSetDevice(m->device));
+    DYNG_HIP_TRY(hipStreamSynchronize(m->stream));  // a scheduled update may still read the buffers
     upload_set(m, x, t, count);
     std::vector<int32_t> idx(static_cast<size_t>(count));
     for (int64_t i = 0; i < count; ++i) idx[i] = static_cast<int32_t>(i);
@@ -537,6 +565,7 @@ extern "C" int dyng_hip_mlp_train(dyng_hip_mlp* m, const float* x, const float* 
       fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: train needs samples, steps >= 0, a minibatch in [1, 65536], lr > 0");
     }
     DYNG_HIP_TRY(hipSetDevice(m->device));
+    DYNG_HIP_TRY(hipStreamSynchronize(m->stream));  // a scheduled update may still read the buffers
     upload_set(m, x, t, count);
     const int B = static_cast<int>(std::min<int64_t>(batch, count));
     std::vector<int32_t> order(static_cast<size_t>(count));
@@ -585,7 +614,8 @@ extern "C" int dyng_hip_mlp_predict(dyng_hip_mlp* m, const float* x, int64_t cou
       fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: predict needs 1 to 65535 samples");
     }
     DYNG_HIP_TRY(hipSetDevice(m->device));
-    refresh_bf16(m);
+    DYNG_HIP_TRY(hipStreamSynchronize(m->stream));  // a scheduled update may still read the buffers
+    refresh_bf16(m, nullptr);
     m->X.reserve(count * kIn);
     m->Y.reserve(count * kOut);
     DYNG_HIP_TRY(hipMemcpy(m->X.p, x, count * kIn * sizeof(float), hipMemcpyHostToDevice));

@@ -12,7 +12,8 @@ The network (12 -> 64 -> 64 -> 5, ReLU) keeps FP32 master weights and is trained
 the library's own forward, backward and Adam kernels; it is evaluated in BF16. It regresses log2
 of the engine time of each schedule, trained on traces the plain method generates: for a batch
 the graph is applied once and the update of the tree repaired under every schedule on clones of
-the tree (``sssp.repair``).
+the tree (``sssp.repair``). A schedule other than the plain one runs only when it is predicted
+at least 5 % faster.
 """
 
 from __future__ import annotations
@@ -104,6 +105,17 @@ class Network:
         y = np.empty((len(x), OUTPUTS), np.float32)
         _lib.check(_lib.lib().dyng_hip_mlp_predict(self._handle, x.ctypes.data, len(x), y.ctypes.data))
         return y
+
+
+SWITCH_MARGIN = 0.07  # log2(1.05)
+
+
+def pick(predicted_log2_ms) -> int:
+    """The schedule to run: the plain one (0) unless another is predicted at least 5 % faster (the
+    rule of the scheduled update entry point)."""
+    y = np.asarray(predicted_log2_ms, np.float32)
+    best = int(np.argmin(y))
+    return best if best != 0 and y[best] < y[0] - SWITCH_MARGIN else 0
 
 
 def batch_stats(graph: Graph, batch: EdgeBatch, result: sssp.Result | None) -> _lib.BatchStats:
@@ -221,7 +233,7 @@ class Scheduler:
         if self.network is None:
             return 0
         x = features(batch_stats(graph, batch, result), self.history)
-        return int(np.argmin(self.network.predict(x[None, :])[0]))
+        return pick(self.network.predict(x[None, :])[0])
 
     def update(self, graph: Graph, batch: EdgeBatch, result: sssp.Result) -> sssp.Stats:
         """sssp.update under the chosen schedule: the features and the BF16 inference run on the

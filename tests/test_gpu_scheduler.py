@@ -166,7 +166,7 @@ def test_trained_scheduler_against_the_plain_schedule(dg):
     for b in batches:
         batch = dg.EdgeBatch(insert=b[:3], delete=b[3:])
         x = dg.scheduler.features(dg.scheduler.batch_stats(hg, batch, hr), history)
-        choice = int(np.argmin(sched.network.predict(x[None, :])[0]))
+        choice = dg.scheduler.pick(sched.network.predict(x[None, :])[0])
         dg.scheduler.collect(hg, hr, [batch], held)  # times every schedule, then repairs hr
         og, _, c = oracle.update(og, ot, b[0], b[1], b[2].reshape(-1, 1), b[3], b[4])
         d, p = hr.to_numpy()
