@@ -67,7 +67,7 @@ class Counters(C.Structure):
         ("pull_edges", C.c_int64), ("jump_rounds", C.c_int64), ("jump_items", C.c_int64),
         ("pull_vertices", C.c_int64), ("step1_ms", C.c_double), ("pull_ms", C.c_double),
         ("relax_ms", C.c_double), ("blocks", C.c_int64), ("kernel_ms", C.c_double), ("pack_ms", C.c_double),
-        ("jump_ms", C.c_double), ("unpack_ms", C.c_double)]
+        ("jump_ms", C.c_double), ("unpack_ms", C.c_double), ("tree_token", C.c_uint64)]
 
 
 class Options(C.Structure):
@@ -87,7 +87,8 @@ class SsspRun(C.Structure):
     _fields_ = [("graph", C.c_void_p), ("objective", C.c_int32), ("source", C.c_int32),
                 ("delta", C.c_int64), ("max_weight", C.c_int64), ("distances", C.c_void_p),
                 ("parents", C.c_void_p), ("check_cycles", C.c_int32), ("workspace", C.c_void_p),
-                ("tuning", Tuning), ("sources", C.c_void_p), ("num_sources", C.c_int64)]
+                ("tuning", Tuning), ("sources", C.c_void_p), ("num_sources", C.c_int64),
+                ("tree_token", C.c_uint64)]
 
 
 class Changes(C.Structure):

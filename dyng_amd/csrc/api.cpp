@@ -116,6 +116,9 @@ struct dyng_hip_result {
   /// several sources at once (source = -1): their ids on the device
   dyng_hip::dbuf<int32_t> sources;
   int64_t num_sources = 0;
+  /// the token of the engine run that left this tree (its packed words may still be in the
+  /// device's workspace); 0 for a tree the engine did not leave, or that changed since
+  uint64_t tree_token = 0;
 };
 
 using namespace dyng_hip;
@@ -142,6 +145,7 @@ dyng_hip_sssp_run make_run(const dyng_hip_graph* g, dyng_hip_result* r, int64_t 
   run.distances = r->distances.p;
   run.parents = r->parents.p;
   run.check_cycles = r->trusted && r->engine_tree ? 0 : 1;
+  run.tree_token = r->tree_token;
   run.workspace = pool().get(g->device);
   run.tuning = r->tuning;
   if (r->opt.cuda_engine != DYNG_HIP_ENGINE_AUTOMATIC && run.tuning.engine == 0) run.tuning.engine = r->opt.cuda_engine;
@@ -294,7 +298,9 @@ void repair_core(dyng_hip_graph* g, dyng_hip_result* r, int64_t delta, int64_t m
       default: break;
     }
     DYNG_HIP_TRY(hipEventRecord(ev.a, s));
+    r->tree_token = 0;
     call(dyng_hip_sssp_update(&run, s, &c));
+    r->tree_token = c.tree_token;
     DYNG_HIP_TRY(hipEventRecord(ev.b, s));
     DYNG_HIP_TRY(hipEventSynchronize(ev.b));
   } catch (...) {
@@ -347,6 +353,7 @@ extern "C" int dyng_hip_api_compute(dyng_hip_graph* g, int32_t source, const dyn
     dyng_hip_sssp_run run = make_run(g, r.get(), delta, g->wmax[k]);
     dyng_hip_counters c{};
     call(dyng_hip_sssp_compute(&run, s, &c));
+    r->tree_token = c.tree_token;
     *out = r.release();
   });
 }
@@ -398,6 +405,7 @@ extern "C" int dyng_hip_api_compute_multi(dyng_hip_graph* g, const int32_t* sour
     dyng_hip_sssp_run run = make_run(g, r.get(), delta, g->wmax[k]);
     dyng_hip_counters c{};
     call(dyng_hip_sssp_compute(&run, s, &c));
+    r->tree_token = c.tree_token;
     *out = r.release();
   });
 }

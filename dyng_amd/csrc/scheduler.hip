@@ -86,7 +86,11 @@ __global__ void k_fwd3(const float* P, const float* H2, const float* T, const in
   float acc = P[kB3 + o];
   for (int k = 0; k < kHidden; ++k) acc += P[kW3 + o * kHidden + k] * H2[s * kHidden + k];
   Y[t] = acc;
-  if (T != nullptr) DY[t] = 2.0f * (acc - T[static_cast<int64_t>(idx[s]) * kOut + o]) / static_cast<float>(B * kOut);
+  if (T != nullptr) {
+    // a NaN target (a schedule not measured on that sample) adds nothing to the loss
+    const float target = T[static_cast<int64_t>(idx[s]) * kOut + o];
+    DY[t] = isnan(target) ? 0.0f : 2.0f * (acc - target) / static_cast<float>(B * kOut);
+  }
 }
 
 /// dh2[s][k] = [h2 > 0] * sum_o dy[s][o] W3[o][k].
@@ -402,7 +406,9 @@ float host_loss(dyng_hip_mlp* m, const float* t, const std::vector<int32_t>& idx
   double sum = 0.0;
   for (int s = 0; s < B; ++s) {
     for (int o = 0; o < kOut; ++o) {
-      const double d = static_cast<double>(y[s * kOut + o]) - t[static_cast<int64_t>(idx[s]) * kOut + o];
+      const float target = t[static_cast<int64_t>(idx[s]) * kOut + o];
+      if (std::isnan(target)) continue;
+      const double d = static_cast<double>(y[s * kOut + o]) - target;
       sum += d * d;
     }
   }

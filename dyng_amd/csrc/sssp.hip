@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <string>
 
@@ -109,6 +110,7 @@ struct engine_params {
   int* in_far;
   int* flag;
   int* ancestor;
+  int32_t resident; ///< update: `packed` already holds the old tree's words (no packing)
   int* cand;
   int* frontier;
   int* near_a;
@@ -190,6 +192,15 @@ DYNG_DEV int4 ld_chunk(const int4* ch) {
   const u64 a = ld(&q[0]), b = ld(&q[1]);
   return make_int4(static_cast<int32_t>(static_cast<uint32_t>(a)), static_cast<int32_t>(static_cast<uint32_t>(a >> 32)),
                    static_cast<int32_t>(static_cast<uint32_t>(b)), 0);
+}
+
+
+/// Four stamps at once, as two agent-scope 8-byte loads.
+DYNG_DEV int4 ld_int4(const int4* q4) {
+  const u64* q = reinterpret_cast<const u64*>(q4);
+  const u64 a = ld(&q[0]), b = ld(&q[1]);
+  return make_int4(static_cast<int32_t>(static_cast<uint32_t>(a)), static_cast<int32_t>(static_cast<uint32_t>(a >> 32)),
+                   static_cast<int32_t>(static_cast<uint32_t>(b)), static_cast<int32_t>(static_cast<uint32_t>(b >> 32)));
 }
 
 /// Cut the row [begin, begin + len) of vertex v into chunks of `chunk` edges (one wave each).
@@ -294,8 +305,15 @@ DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edge edge, Done do
   }
 }
 
+/// w's word changed in this run without its distance falling (a lower parent id at the same
+/// distance). Every vertex whose word a repair changes ends up with a stamp of this run (stamps
+/// of this run are >= its first generation p.generation): an invalidated vertex or one whose
+/// distance fell was claimed for a list; this one gets p.generation itself, which is below every
+/// generation a claim compares with, so it claims nothing. The unpack visits the stamped ones.
+DYNG_DEV void mark_changed(const engine_params& p, int32_t w) { atomicMax(&p.stamp[w], p.generation); }
+
 /// One relaxation of edge (u, w) of weight wt with u at distance du (fused.cuh:341-362): the
-/// candidate word joins the next near list or the far pile when it lowered w's distance.
+/// candidate word joins the next near list or the far pile when it lowered w's word.
 DYNG_DEV void relax(const engine_params& p, bool active, int32_t u, u64 du, int32_t w, int32_t wt,
                     u64 threshold, int generation, stage_shared& S, int* next, int* next_count, int* far,
                     int* far_count, unsigned long long* improved) {
@@ -306,12 +324,15 @@ DYNG_DEV void relax(const engine_params& p, bool active, int32_t u, u64 du, int3
       const u64 cand = p.pk.pack(nd, u);
       if (cand < ld(&p.packed[w])) {
         const u64 old = atomicMin(&p.packed[w], cand);
-        if (cand < old) atomicAdd(improved, 1ULL);
-        if (nd < p.pk.distance(old)) {
+        // Any lowered word joins a list, also one lowered only to a lower parent id at the same
+        // distance (fused.cuh:341-362 lists only a lower distance): the extra expansion is rare
+        // and offers nothing new, and being claimed stamps the vertex for the unpack.
+        if (cand < old) {
+          atomicAdd(improved, 1ULL);
           if (nd < threshold) {
             to_near = claim(p.stamp, w, generation);
           } else {
-            to_far = atomicExch(&p.in_far[w], 1) == 0;
+            to_far = atomicExch(&p.in_far[w], 1) == 0;  // stamped when it leaves the far pile
           }
         }
       }
@@ -382,6 +403,8 @@ DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best
       if (p.pk.distance(best) < p.pk.distance(old)) {
         low = p.pk.distance(best);
         add = claim(p.stamp, v, generation);
+      } else if (best < old) {
+        mark_changed(p, v);
       }
     }
   }
@@ -451,8 +474,9 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     }
     DYNG_GRID_SYNC();
   } else {
-    // ---- A: pack the old tree; ancestors (pointer jumping only); roots ----
-    for (int v = tid; v < n; v += threads) {
+    // ---- A: pack the old tree (unless the workspace still holds its words); ancestors (pointer
+    // jumping only); roots ----
+    for (int v = tid; !p.resident && v < n; v += threads) {
       const long long d = p.distances[v];
       const int32_t par = p.parents[v];
       u64 word = packed_inf;
@@ -710,6 +734,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           const u64 current = ld(&p.packed[ch.x]);
           if (best < current) {
             const u64 old = atomicMin(&p.packed[ch.x], best);
+            if (best < old && p.pk.distance(best) == p.pk.distance(old)) mark_changed(p, ch.x);
             if (p.pk.distance(best) < p.pk.distance(old)) {
               atomicMin(&c->min_frontier, p.pk.distance(best));
               if (claim(p.stamp, ch.x, generation)) {
@@ -890,7 +915,10 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   if (tid == 0) c->t_phase[3] = __builtin_amdgcn_s_memrealtime();
   unsigned long long changed = 0;
   if (p.pk.bits > 0) {
-    for (int v = tid; v < n; v += threads) {
+    // A repair changed only words of vertices it stamped (see mark_changed); every other word
+    // still equals its vertex's packed (distance, parent). The stamps cost 4 B per vertex to
+    // scan (four per load) instead of 20 B to unpack. A run from the sources visits every vertex.
+    auto unpack = [&](int32_t v) {
       const u64 word = ld(&p.packed[v]);
       const long long d = word == packed_inf ? sssp_infinity : static_cast<long long>(p.pk.distance(word));
       const int32_t par = word == packed_inf ? -1 : p.pk.parent(word);
@@ -901,6 +929,22 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         p.distances[v] = d;
         p.parents[v] = par;
         ++changed;
+      }
+    };
+    if (p.from_scratch) {
+      for (int v = tid; v < n; v += threads) unpack(v);
+    } else {
+      const int quads = n / 4;  // the stamp array is 16-byte aligned
+      const int4* stamp4 = reinterpret_cast<const int4*>(p.stamp);
+      for (int q = tid; q < quads; q += threads) {
+        const int4 s4 = ld_int4(&stamp4[q]);
+        if (s4.x >= p.generation) unpack(4 * q);
+        if (s4.y >= p.generation) unpack(4 * q + 1);
+        if (s4.z >= p.generation) unpack(4 * q + 2);
+        if (s4.w >= p.generation) unpack(4 * q + 3);
+      }
+      for (int v = 4 * quads + tid; v < n; v += threads) {
+        if (ld(&p.stamp[v]) >= p.generation) unpack(v);
       }
     }
   } else {
@@ -968,6 +1012,12 @@ struct dyng_hip_workspace {
   dyng_hip::dbuf<dyng_hip::u64> packed;
   dyng_hip::dbuf<int> stamp, in_far, flag, ancestor;
   dyng_hip::dbuf<int> cand, frontier, near_a, near_b, far_a, far_b;
+  /// the tree whose packed words `packed` holds after the last successful run: its token (0:
+  /// none), size, parent bits and distance array
+  uint64_t words_token = 0;
+  int64_t words_n = 0;
+  int words_bits = 0;
+  const void* words_distances = nullptr;
   dyng_hip::dbuf<int4> chunks;
   int64_t chunk_cap = 0;
   dyng_hip::dbuf<dyng_hip::control> ctl;
@@ -1015,6 +1065,7 @@ void ws_reserve(dyng_hip_workspace* ws, int64_t n, int64_t slots, hipStream_t s)
     DYNG_HIP_TRY(hipMemsetAsync(ws->flag.p, 0, cap * sizeof(int), s));
     ws->capacity = cap;
     ws->generation = 0;
+    ws->words_token = 0;
   }
   if (want_chunks > ws->chunk_cap) {
     ws->chunks.reserve(want_chunks);
@@ -1124,6 +1175,14 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   while ((1LL << rounds) < static_cast<long long>(n)) ++rounds;
   p.max_rounds = rounds + 1;
   p.generation = ++ws->generation;
+  // the workspace still holds this tree's words when the caller's token is the one the last
+  // successful run handed out for them (same size, packing and arrays)
+  p.resident = update && !recompute && !run->check_cycles && pk.bits > 0 && run->tree_token != 0 &&
+                       run->tree_token == ws->words_token && ws->words_n == n && ws->words_bits == pk.bits &&
+                       ws->words_distances == run->distances
+                   ? 1
+                   : 0;
+  ws->words_token = 0;  // until this run ends well
   p.distances = reinterpret_cast<long long*>(run->distances);
   p.parents = run->parents;
   p.packed = ws->packed.p;
@@ -1270,6 +1329,15 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   counters.unpack_ms = t7 >= t3 ? (t7 - t3) * tick_ms : 0.0;
   counters.blocks = blocks;
   counters.kernel_ms = kernel_ms;
+  if (pk.bits > 0) {
+    // `packed` now holds the words of the tree the run left: hand out a token for them
+    static std::atomic<uint64_t> next_token{0};
+    ws->words_token = ++next_token;
+    ws->words_n = n;
+    ws->words_bits = pk.bits;
+    ws->words_distances = run->distances;
+    counters.tree_token = ws->words_token;
+  }
   if (out) *out = counters;
 }
 

@@ -74,7 +74,8 @@ class Network:
         return x, t
 
     def gradients(self, x, t) -> tuple[np.ndarray, float]:
-        """FP32 gradients of the mean squared error (trained parameters) and the loss."""
+        """FP32 gradients of the squared error over the known targets (NaN: not measured), divided
+        by the number of outputs, and the loss (trained parameters)."""
         x, t = self._sets(x, t)
         g = np.empty(_lib.MLP_TRAINED, np.float32)
         loss = C.c_float()
@@ -186,30 +187,63 @@ class Trace:
         return np.asarray(self.x, np.float32), np.asarray(self.t, np.float32)
 
 
-def collect(graph: Graph, result: sssp.Result, batches, trace: Trace | None = None, repeats: int = 1) -> Trace:
+def collect(graph: Graph, result: sssp.Result, batches, trace: Trace | None = None, repeats: int = 1,
+            seed: int = 1) -> Trace:
     """Run the plain method over ``batches`` (an iterable of EdgeBatch), timing every choice on
-    clones of the tree for each batch; the result follows the default schedule."""
+    clones of the tree for each batch; the result follows the default schedule.
+
+    Every repair of a batch after the first finds the rows and words it touches in the caches, so
+    a fixed order would favour the later choices: one untimed repair warms them first, and the
+    choices are timed in a shuffled order."""
     trace = trace or Trace()
     history = History()
+    rng = np.random.default_rng(seed)
     for batch in batches:
         x = features(batch_stats(graph, batch, result), history)
         delta = default_delta(graph, result.options.objective)
         graph.apply(batch)
-        times = []
-        for choice in range(len(CHOICES)):
+        warm = result.clone()
+        use(warm, 0, delta)
+        sssp.repair(graph, warm)
+        del warm
+        times = [0.0] * len(CHOICES)
+        for choice in rng.permutation(len(CHOICES)):
             best = math.inf
             for _ in range(max(1, repeats)):
                 trial = result.clone()
-                use(trial, choice, delta)
+                use(trial, int(choice), delta)
                 st = sssp.repair(graph, trial)
                 best = min(best, st.engine_ms)
                 del trial
-            times.append(math.log2(max(best, 1e-3)))
+            times[int(choice)] = math.log2(max(best, 1e-3))
         use(result, 0, delta)
         st = sssp.repair(graph, result)
         history = History(st.invalidated, st.affected)
         trace.x.append(x)
         trace.t.append(times)
+    return trace
+
+
+def explore(graph: Graph, result: sssp.Result, batches, trace: Trace | None = None, seed: int = 1) -> Trace:
+    """Run every batch of ``batches`` for real (``sssp.update``) under a schedule drawn at random,
+    recording its engine time as the target of that schedule; the other schedules' targets are
+    NaN, which the loss ignores. Unlike ``collect``, these are the times of real updates (the
+    caches as a stream leaves them), one run each. The result is left on the plain schedule."""
+    trace = trace or Trace()
+    history = History()
+    rng = np.random.default_rng(seed)
+    for batch in batches:
+        x = features(batch_stats(graph, batch, result), history)
+        choice = int(rng.integers(len(CHOICES)))
+        use(result, choice, default_delta(graph, result.options.objective))
+        st = sssp.update(graph, batch, result)
+        t = [math.nan] * len(CHOICES)
+        t[choice] = math.log2(max(st.engine_ms, 1e-3))
+        history = History(st.invalidated, st.affected)
+        trace.x.append(x)
+        trace.t.append(t)
+    result.set_tuning(sssp.Tuning())
+    result.set_options(delta=0)
     return trace
 
 

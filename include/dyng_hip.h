@@ -81,9 +81,10 @@ typedef struct dyng_hip_graph dyng_hip_graph;
 typedef struct dyng_hip_apply_summary {
   int64_t inserted_edges, updated_edges, deleted_edges, ignored_deletions, dropped_self_loops;
   int64_t inserted_vertices, num_vertices_after, ignored_insertions, relocated_rows, compactions;
-  /* device time of the apply's stages (HIP events, ms): batch scan, operation keys, out-key sort,
-   * out-row pairing, out scans, in-key sort, in-row pairing, in scans, merge planning (with its
-   * host round trip and any compaction), out-row merge, in-row merge */
+  /* device time of the apply's stages (HIP events, ms): batch scan, operation keys, key sort
+   * (both directions' keys in one sort), out-row pairing, out scans, (unused: 0), in-row pairing,
+   * in scans, merge planning (with its host round trip and any compaction), out-row merge,
+   * in-row merge */
   double stage_ms[11];
 } dyng_hip_apply_summary;
 
@@ -195,6 +196,11 @@ typedef struct dyng_hip_sssp_run {
    * null / 0 for the single source above */
   const int32_t* sources;
   int64_t num_sources;
+  /* update: the token a previous run returned for the tree now in distances/parents (0: none).
+   * When it is the workspace's last run's token, the workspace still holds the tree's packed
+   * words and the run skips packing them (no counterpart in the reference, which packs the old
+   * tree every time, fused.cuh:178-197). A caller that changes the arrays in between passes 0. */
+  uint64_t tree_token;
 } dyng_hip_sssp_run;
 
 /* sssp_counters (problem.hpp:106-113) plus the relax kernel's work counters. */
@@ -218,6 +224,7 @@ typedef struct dyng_hip_counters {
   double pack_ms;         /* Step 1 parts (s_memrealtime): packing the old tree and the roots, */
   double jump_ms;         /* the pointer-jumping rounds, */
   double unpack_ms;       /* and the final unpack */
+  uint64_t tree_token;    /* the token of the tree this run left (0: its words are not kept) */
 } dyng_hip_counters;
 
 /* sssp_cuda_update (cuda.cu:271-307): Step 1 + Step 2 on the graph after dyng_hip_graph_apply,
@@ -283,7 +290,8 @@ int dyng_hip_api_from_arrays(dyng_hip_graph* g, int32_t source, const int64_t* d
                              const dyng_hip_options* opt, void* stream, dyng_hip_result** out);
 int dyng_hip_result_destroy(dyng_hip_result* r);
 int dyng_hip_result_clone(const dyng_hip_result* r, void* stream, dyng_hip_result** out);
-/* result accessors (sssp.hpp:183-234): device arrays of n entries. */
+/* result accessors (sssp.hpp:183-234): device arrays of n entries, read-only (the next update
+ * reuses the packed words the last run left for exactly these contents). */
 int dyng_hip_result_arrays(const dyng_hip_result* r, const int64_t** distances,
                            const int32_t** parents, int64_t* n);
 int dyng_hip_result_source(const dyng_hip_result* r, int32_t* source);
@@ -332,8 +340,9 @@ int dyng_hip_mlp_destroy(dyng_hip_mlp* m);
 /* All DYNG_HIP_MLP_PARAMS floats, host memory. */
 int dyng_hip_mlp_get_params(const dyng_hip_mlp* m, float* params, int64_t count);
 int dyng_hip_mlp_set_params(dyng_hip_mlp* m, const float* params, int64_t count);
-/* FP32 forward + backward of the mean squared error over `count` samples (x: count x 12,
- * t: count x 5, host): the DYNG_HIP_MLP_TRAINED gradients and the loss. */
+/* FP32 forward + backward of the squared error over `count` samples (x: count x 12, t: count x 5,
+ * host), summed over the known targets and divided by count x 5 (a NaN target, a schedule not
+ * measured on that sample, adds nothing): the DYNG_HIP_MLP_TRAINED gradients and the loss. */
 int dyng_hip_mlp_gradients(dyng_hip_mlp* m, const float* x, const float* t, int64_t count, float* grad,
                            float* loss);
 /* Adam (lr, 0.9, 0.999, 1e-8) over `steps` minibatches of `batch` samples drawn without
@@ -344,9 +353,11 @@ int dyng_hip_mlp_train(dyng_hip_mlp* m, const float* x, const float* t, int64_t 
 int dyng_hip_mlp_predict(dyng_hip_mlp* m, const float* x, int64_t count, float* y);
 
 /* sssp::update under the learned scheduler: the batch's features on G_t and the network's BF16
- * inference run on the device before anything changes (mlp = null: the plain schedule), then the
- * update runs under the schedule predicted fastest (*choice: its index). last_invalidated and
- * last_affected are the previous update's counters (the network's history features). */
+ * inference run on the device beside the apply, before it changes any row (mlp = null: the plain
+ * schedule), then the update runs under the predicted fastest schedule if it is predicted at
+ * least 5 % faster than the plain one, else under the plain one (*choice: its index).
+ * last_invalidated and last_affected are the previous update's counters (the network's history
+ * features). */
 int dyng_hip_api_update_scheduled(dyng_hip_graph* g, const dyng_hip_batch* batch, dyng_hip_result* r,
                                   dyng_hip_mlp* mlp, int64_t last_invalidated, int64_t last_affected,
                                   void* stream, dyng_hip_stats* stats, int32_t* choice);

@@ -43,9 +43,10 @@ def test_network_gradients_match_the_oracle(dg):
     p = _random_params(rng)
     net.params = p
     assert np.array_equal(net.params, p)
-    for B in (1, 7, 300):
+    for B, unknown in ((1, 0.0), (7, 0.0), (300, 0.0), (300, 0.8)):
         x = rng.normal(0, 2, (B, mlp.IN)).astype(np.float32)
         t = rng.normal(0, 1, (B, mlp.OUT)).astype(np.float32)
+        t[rng.random(t.shape) < unknown] = np.nan  # schedules not measured: ignored by the loss
         g, loss = net.gradients(x, t)
         want = mlp.gradients(p.astype(np.float64), x, t)
         # FP32 sums in another order than the FP64 oracle: relative 1e-4 of the largest gradient

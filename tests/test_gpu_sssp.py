@@ -616,6 +616,31 @@ def test_repair_after_apply_equals_update(dg):
         dg.sssp.repair(g2, r2)  # already current
 
 
+def test_results_sharing_the_workspace_interleave(dg):
+    """Two results of one graph repaired in turn, and a clone: each run finds the workspace holding
+    the other tree's words (or its own), so the engine must tell whether it may skip packing; every
+    tree stays equal to the oracle's canonical tree after every batch."""
+    rng = np.random.default_rng(23)
+    n, m = 4000, 40000
+    src, dst, w = rng.integers(0, n, m), rng.integers(0, n, m), rng.integers(1, 60, m)
+    g = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    r1, r2 = dg.sssp.compute(g, 5), dg.sssp.compute(g, 77)
+    for b in range(5):
+        x = _instance_batch(rng, og, n, 0.02, 60)
+        og, _, _ = oracle.update(og, oracle.compute(og, 5)[0], x[0], x[1], x[2].reshape(-1, 1), x[3], x[4])
+        g.apply(dg.EdgeBatch(insert=(x[0], x[1], x[2]), delete=(x[3], x[4])))
+        order = (r1, r2) if b % 2 == 0 else (r2, r1)
+        for r in order:
+            dg.sssp.repair(g, r)
+        if b == 2:
+            r1 = r1.clone()  # a copy of the tree whose words the workspace may hold
+        for r, s0 in ((r1, 5), (r2, 77)):
+            ot, _ = oracle.compute(og, s0)
+            d, p = r.to_numpy()
+            assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent), f"batch {b}, source {s0}"
+
+
 def test_graph_launch_replays_match_plain_launches(dg):
     """The engine's launch as a hipGraph, replayed for every update (with other eager work and
     synchronizations in between), gives the oracle's tree every time."""

@@ -29,6 +29,28 @@ def test_gradients_match_finite_differences():
         assert abs(fd - g[q]) <= 1e-6 + 1e-5 * abs(fd), (q, fd, g[q])
 
 
+def test_gradients_with_unknown_targets_match_finite_differences():
+    """NaN targets (schedules not measured on a batch) drop out of the loss and its gradient."""
+    rng = np.random.default_rng(4)
+    p = _params(rng)
+    x = rng.normal(0, 2, (11, mlp.IN))
+    t = rng.normal(0, 1, (11, mlp.OUT))
+    t[rng.random(t.shape) < 0.7] = np.nan
+    g = mlp.gradients(p, x, t)
+    assert np.isfinite(g).all()
+    h = 1e-6
+    for q in rng.choice(mlp.TRAINED, 60, replace=False).tolist() + [mlp.B3, mlp.TRAINED - 1]:
+        e = np.zeros(mlp.PARAMS)
+        e[q] = h
+        fd = (mlp.loss(p + e, x, t) - mlp.loss(p - e, x, t)) / (2 * h)
+        assert abs(fd - g[q]) <= 1e-6 + 1e-5 * abs(fd), (q, fd, g[q])
+    # an output with no known target in the minibatch gets no gradient on its own weights
+    t2 = t.copy()
+    t2[:, 3] = np.nan
+    g2 = mlp.gradients(p, x, t2)
+    assert np.all(g2[mlp.W3 + 3 * mlp.HIDDEN:mlp.W3 + 4 * mlp.HIDDEN] == 0) and g2[mlp.B3 + 3] == 0
+
+
 def test_planted_gradient_mistake_is_caught():
     """A backward pass that forgets the ReLU mask disagrees with finite differences."""
     rng = np.random.default_rng(2)
