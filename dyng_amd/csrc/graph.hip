@@ -859,34 +859,54 @@ DYNG_DEV row_plan plan_row(const merge_params& p, int32_t u, int2 mt, int64_t a,
 
 /// Before the merges: each touched row's mode (appended to its list) and, for full merges, the
 /// staging slots (its alive entries) and the tail slots (a row that outgrew its capacity), so the
-/// host can compact or grow first and no row is refused.
-__global__ void k_merge_plan(merge_params p) {
+/// host can compact or grow first and no row is refused. One row per thread; the lists are
+/// appended per workgroup (one atomic per mode and workgroup: the list counters are single
+/// addresses every workgroup hits).
+constexpr int kPlanBlock = 1024;
+
+__global__ void __launch_bounds__(kPlanBlock) k_merge_plan(merge_params p) {
+  __shared__ int wave_offset[kPlanBlock / 64][kModes];
+  __shared__ int base[kModes];
+  __shared__ long long part[kPlanBlock / 64];
   const int count = *p.touched_count;
+  const int t = blockIdx.x * kPlanBlock + threadIdx.x;
+  const int w = threadIdx.x / 64;
   long long reloc = 0, stage = 0;
-  const int stride = gridDim.x * blockDim.x;
-  // wave-uniform trip count: every lane takes part in the wave-aggregated list appends
-  for (int base = blockIdx.x * blockDim.x + threadIdx.x - lane(); base < count; base += stride) {
-    const int t = base + lane();
-    int mode = -1;
-    int32_t u = -1;
-    if (t < count) {
-      u = p.touched[t];
-      const int2 mt = p.meta[u];
-      const row_plan r = plan_row(p, u, mt, p.row_a[u], p.row_b[u]);
-      mode = r.mode;
-      if (r.mode == kFull) {
-        stage += r.alive_base + r.alive_tail;
-        if (r.new_len > p.cap[u]) reloc += grown_cap(r.new_len, p.headroom);
-      }
-    }
-    for (int m = 0; m < kModes; ++m) {
-      const int slot = wave_append(&p.list_count[m], mode == m);
-      if (mode == m) p.lists[m * (p.N + 1) + slot] = u;
+  int mode = -1;
+  int32_t u = -1;
+  if (t < count) {
+    u = p.touched[t];
+    const int2 mt = p.meta[u];
+    const row_plan r = plan_row(p, u, mt, p.row_a[u], p.row_b[u]);
+    mode = r.mode;
+    if (r.mode == kFull) {
+      stage = r.alive_base + r.alive_tail;
+      if (r.new_len > p.cap[u]) reloc = grown_cap(r.new_len, p.headroom);
     }
   }
-  reloc = static_cast<long long>(wave_sum(static_cast<u64>(reloc)));
-  stage = static_cast<long long>(wave_sum(static_cast<u64>(stage)));
-  if (lane() == 0) {
+  int rank = 0;  // among the wave's rows of the same mode
+#pragma unroll
+  for (int m = 0; m < kModes; ++m) {
+    const u64 b = __ballot(mode == m);
+    if (mode == m) rank = __popcll(b & lanes_below());
+    if (lane() == 0) wave_offset[w][m] = __popcll(b);
+  }
+  __syncthreads();
+  if (threadIdx.x < kModes) {
+    const int m = threadIdx.x;
+    int run = 0;
+    for (int i = 0; i < kPlanBlock / 64; ++i) {
+      const int c = wave_offset[i][m];
+      wave_offset[i][m] = run;
+      run += c;
+    }
+    base[m] = run > 0 ? atomicAdd(&p.list_count[m], run) : 0;
+  }
+  __syncthreads();
+  if (mode >= 0) p.lists[mode * (p.N + 1) + base[mode] + wave_offset[w][mode] + rank] = u;
+  reloc = block_reduce(reloc, part, op_sum{});
+  stage = block_reduce(stage, part, op_sum{});
+  if (threadIdx.x == 0) {
     if (reloc) atomicAdd(reinterpret_cast<unsigned long long*>(p.need_reloc), static_cast<unsigned long long>(reloc));
     if (stage) atomicAdd(reinterpret_cast<unsigned long long*>(p.need_stage), static_cast<unsigned long long>(stage));
   }
@@ -1918,7 +1938,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
       mp.headroom = g->props.headroom;
       return mp;
     };
-    const unsigned plan_grid = static_cast<unsigned>(std::min<int64_t>(blocks_for(N), 1024));
+    const unsigned plan_grid = static_cast<unsigned>((N + kPlanBlock - 1) / kPlanBlock);  // touched rows <= N
     if (wait != nullptr) DYNG_HIP_TRY(hipStreamWaitEvent(s, wait, 0));  // before the rows change
     for (int attempt = 0;; ++attempt) {
       hc->need_reloc_out = hc->need_reloc_in = hc->need_stage_out = hc->need_stage_in = 0;
@@ -1926,8 +1946,8 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
       DYNG_HIP_TRY(hipMemcpyAsync(&dc->need_reloc_out, &hc->need_reloc_out, 4 * sizeof(long long),
                                   hipMemcpyHostToDevice, s));
       DYNG_HIP_TRY(hipMemcpyAsync(dc->modes_out, hc->modes_out, 2 * sizeof(hc->modes_out), hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_merge_plan, dim3(plan_grid), dim3(kBlock), 0, s, merge_setup(true));
-      hipLaunchKernelGGL(k_merge_plan, dim3(plan_grid), dim3(kBlock), 0, s, merge_setup(false));
+      hipLaunchKernelGGL(k_merge_plan, dim3(plan_grid), dim3(kPlanBlock), 0, s, merge_setup(true));
+      hipLaunchKernelGGL(k_merge_plan, dim3(plan_grid), dim3(kPlanBlock), 0, s, merge_setup(false));
       DYNG_HIP_TRY(hipGetLastError());
       DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
       DYNG_HIP_TRY(hipStreamSynchronize(s));
