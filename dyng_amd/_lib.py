@@ -67,7 +67,8 @@ class Counters(C.Structure):
         ("pull_edges", C.c_int64), ("jump_rounds", C.c_int64), ("jump_items", C.c_int64),
         ("pull_vertices", C.c_int64), ("step1_ms", C.c_double), ("pull_ms", C.c_double),
         ("relax_ms", C.c_double), ("blocks", C.c_int64), ("kernel_ms", C.c_double), ("pack_ms", C.c_double),
-        ("jump_ms", C.c_double), ("unpack_ms", C.c_double), ("tree_token", C.c_uint64)]
+        ("jump_ms", C.c_double), ("unpack_ms", C.c_double), ("tree_token", C.c_uint64),
+        ("words_kept", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Options(C.Structure):

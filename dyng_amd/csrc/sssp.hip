@@ -1329,6 +1329,7 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   counters.unpack_ms = t7 >= t3 ? (t7 - t3) * tick_ms : 0.0;
   counters.blocks = blocks;
   counters.kernel_ms = kernel_ms;
+  counters.words_kept = p.resident;
   if (pk.bits > 0) {
     // `packed` now holds the words of the tree the run left: hand out a token for them
     static std::atomic<uint64_t> next_token{0};

@@ -225,6 +225,8 @@ typedef struct dyng_hip_counters {
   double jump_ms;         /* the pointer-jumping rounds, */
   double unpack_ms;       /* and the final unpack */
   uint64_t tree_token;    /* the token of the tree this run left (0: its words are not kept) */
+  int32_t words_kept;     /* 1: the run found the old tree's words in the workspace (no packing) */
+  int32_t pad_;
 } dyng_hip_counters;
 
 /* sssp_cuda_update (cuda.cu:271-307): Step 1 + Step 2 on the graph after dyng_hip_graph_apply,

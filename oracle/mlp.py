@@ -3,8 +3,9 @@ check the HIP training and inference kernels; nothing in the product imports it)
 
 A 12 -> 64 -> 64 -> 5 multilayer perceptron with ReLU hidden layers, the parameter layout of
 ``dyng_hip.h`` (DYNG_HIP_MLP_*): W1[64][12], b1[64], W2[64][64], b2[64], W3[5][64], b3[5], then the
-input normalization x_mean[12], x_scale[12] (x_n = (x - mean) * scale). The loss is the mean
-squared error over all outputs of a minibatch. The reference project has no such network, so
+input normalization x_mean[12], x_scale[12] (x_n = (x - mean) * scale). The loss is the squared
+error summed over the outputs whose target is known, divided by B * 5 for a minibatch of B: a
+NaN target (a schedule not measured on that batch) contributes nothing. The reference project has no such network, so
 nothing here restates reference code; the backward pass is pinned by finite differences
 (tests/test_oracle.py) and the BF16 forward by rounding each stage as IEEE bfloat16 does (round
 to nearest, ties to even, on the upper 16 bits of binary32).
@@ -46,7 +47,9 @@ def forward(p: np.ndarray, x: np.ndarray, dtype=np.float64):
 
 def loss(p, x, t, dtype=np.float64) -> float:
     y, _ = forward(p, x, dtype)
-    return float(np.mean((y - np.asarray(t, dtype)) ** 2))
+    t = np.asarray(t, dtype)
+    e = np.where(np.isnan(t), 0.0, y - np.nan_to_num(t))
+    return float(np.sum(e ** 2) / e.size)
 
 
 def gradients(p: np.ndarray, x: np.ndarray, t: np.ndarray, dtype=np.float64) -> np.ndarray:
@@ -54,7 +57,8 @@ def gradients(p: np.ndarray, x: np.ndarray, t: np.ndarray, dtype=np.float64) -> 
     w1, b1, w2, b2, w3, b3, mean, scale = (a.astype(dtype) for a in unpack(p))
     y, (xn, h1, h2) = forward(p, x, dtype)
     B = y.shape[0]
-    dy = 2.0 * (y - np.asarray(t, dtype)) / (B * OUT)
+    t = np.asarray(t, dtype)
+    dy = np.where(np.isnan(t), 0.0, 2.0 * (y - np.nan_to_num(t)) / (B * OUT))
     g = np.zeros(TRAINED, dtype)
     g[W3:B3] = (dy.T @ h2).reshape(-1)
     g[B3:TRAINED] = dy.sum(0)
