@@ -114,6 +114,7 @@ EXPORTS = (
     "dyng_hip_api_repair", "dyng_hip_batch_features", "dyng_hip_mlp_create", "dyng_hip_mlp_destroy",
     "dyng_hip_mlp_get_params", "dyng_hip_mlp_set_params", "dyng_hip_mlp_gradients", "dyng_hip_mlp_train",
     "dyng_hip_mlp_predict", "dyng_hip_result_equal", "dyng_hip_api_update_scheduled",
+    "dyng_hip_mosp_combined",
 )
 
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_TRAINED, MLP_PARAMS = 12, 64, 5, 5317, 5341
@@ -123,6 +124,11 @@ class BatchStats(C.Structure):
     _fields_ = [("num_ins", C.c_int64), ("num_del", C.c_int64), ("tree_deletions", C.c_int64),
                 ("touched", C.c_int64), ("mean_log_degree", C.c_double), ("max_degree", C.c_int64),
                 ("num_vertices", C.c_int64), ("num_edges", C.c_int64)]
+
+class MospStats(C.Structure):
+    _fields_ = [("combined_edges", C.c_int64), ("weight_sum", C.c_int64), ("preference_scale", C.c_int64),
+                ("base", C.c_int64), ("affected", C.c_int64), ("solve", Counters)]
+
 
 _lib = None
 
@@ -160,6 +166,8 @@ def lib() -> C.CDLL:
     L.dyng_hip_api_update.argtypes = [vp, C.POINTER(Batch), vp, vp, C.POINTER(Stats)]
     L.dyng_hip_api_compute_multi.argtypes = [vp, vp, C.c_int64, C.POINTER(Options), vp, C.POINTER(vp)]
     L.dyng_hip_api_repair.argtypes = [vp, vp, vp, C.POINTER(Stats)]
+    L.dyng_hip_mosp_combined.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, C.c_int32, vp, vp, C.c_int32, vp, vp,
+                                         C.POINTER(MospStats)]
     L.dyng_hip_batch_features.argtypes = [vp, C.POINTER(Batch), vp, C.c_int64, vp, C.POINTER(BatchStats)]
     L.dyng_hip_mlp_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
     L.dyng_hip_mlp_destroy.argtypes = [vp]

@@ -829,3 +829,98 @@ extern "C" int dyng_hip_synchronize(int device) {
     DYNG_HIP_TRY(hipDeviceSynchronize());
   });
 }
+
+// ---------------------------------------------------------------------------------------------
+// MOSP's combined solve (cpp/src/algorithms/mosp/mosp.cpp:264-330)
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int dyng_hip_mosp_combined(const dyng_hip_graph* g, int32_t source, int32_t num_objectives,
+                                      const int32_t* const* tree_parents, const int32_t* preferences,
+                                      int32_t count_changes, int64_t* distances, int32_t* parents,
+                                      int32_t cost_columns, int64_t* path_costs, void* stream,
+                                      dyng_hip_mosp_stats* out) {
+  return guarded([&] {
+    if (g == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: null graph");
+    const int64_t n = g->n;
+    if (num_objectives < 1 || num_objectives > 64) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: 1 to 64 objectives, got " + std::to_string(num_objectives));
+    }
+    if (tree_parents == nullptr || distances == nullptr || parents == nullptr) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the trees and the output arrays are required");
+    }
+    for (int k = 0; k < num_objectives; ++k) {
+      if (tree_parents[k] == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: tree " + std::to_string(k) + " is null");
+    }
+    if (n > 0 && (source < 0 || source >= n)) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: source " + std::to_string(source) + " is out of range [0, " +
+                                          std::to_string(n) + ")");
+    }
+    if (cost_columns < 0 || cost_columns > g->K || (cost_columns > 0 && path_costs == nullptr)) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: cost_columns must be in [0, " + std::to_string(g->K) +
+                                          "] with an output array");
+    }
+    // preferenceScale (mosp.cpp:114-131): L = lcm(preferences) <= 2^20; terms L / pref[k]
+    int64_t scale = 1;
+    if (preferences != nullptr) {
+      for (int k = 0; k < num_objectives; ++k) {
+        const int64_t pref = preferences[k];
+        if (pref < 1) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: preferences[" + std::to_string(k) + "] is " +
+                                                          std::to_string(pref) + "; a preference must be >= 1");
+        int64_t a = scale, b = pref;
+        while (b != 0) {
+          const int64_t t = a % b;
+          a = b;
+          b = t;
+        }
+        scale = scale / a * pref;
+        if (scale > (int64_t{1} << 20)) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: lcm(preferences) exceeds 2^20");
+      }
+    }
+    std::vector<int32_t> terms(static_cast<size_t>(num_objectives));
+    for (int k = 0; k < num_objectives; ++k) {
+      terms[k] = static_cast<int32_t>(preferences != nullptr ? scale / preferences[k] : scale);
+    }
+    const int64_t base = scale * (num_objectives + 1);
+    dyng_hip_mosp_stats st{};
+    st.preference_scale = scale;
+    st.base = base;
+    if (n == 0) {
+      if (out != nullptr) *out = st;
+      return;
+    }
+    scoped_device guard(g->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t edges = 0, weight_sum = 0;
+    std::unique_ptr<dyng_hip_graph> cg(mosp_build_combined(g->device, tree_parents, num_objectives, n, source,
+                                                           static_cast<int32_t>(base), terms.data(), s, &edges,
+                                                           &weight_sum));
+    st.combined_edges = edges;
+    st.weight_sum = weight_sum;
+    {
+      // the static solve of sssp_solve_view: the automatic width of the combined graph, the
+      // base weight as the largest weight; with count_changes a recompute that counts the pairs
+      // it changes against the arrays' previous tree
+      std::lock_guard<std::mutex> lk(pool().lock);
+      dyng_hip_sssp_run run{};
+      run.graph = cg.get();
+      run.objective = 0;
+      run.source = source;
+      run.delta = default_delta(std::max<int64_t>(edges, 1), n, weight_sum);
+      run.max_weight = base;
+      run.distances = distances;
+      run.parents = parents;
+      run.workspace = pool().get(g->device);
+      dyng_hip_counters c{};
+      if (count_changes) {
+        run.tuning.recompute = 1;
+        call(dyng_hip_sssp_update(&run, s, &c));
+        st.affected = c.affected;
+      } else {
+        call(dyng_hip_sssp_compute(&run, s, &c));
+      }
+      st.solve = c;
+    }
+    if (cost_columns > 0) mosp_path_costs(g, parents, n, source, cost_columns, path_costs, s);
+    if (out != nullptr) *out = st;
+  });
+}

@@ -364,6 +364,37 @@ int dyng_hip_api_update_scheduled(dyng_hip_graph* g, const dyng_hip_batch* batch
                                   dyng_hip_mlp* mlp, int64_t last_invalidated, int64_t last_affected,
                                   void* stream, dyng_hip_stats* stats, int32_t* choice);
 
+/* ============================================================================================
+ * (4) MOSP's combined solve (cpp/src/algorithms/mosp/mosp.cpp:264-330, which runs it through
+ *     sssp_solve_view, cpp/src/algorithms/sssp/sssp.cpp:1251-1286): the second caller of the
+ *     engine, a static solve on a graph built from K shortest-path trees.
+ * ==========================================================================================*/
+
+typedef struct dyng_hip_mosp_stats {
+  int64_t combined_edges;   /* edges of the combined graph */
+  int64_t weight_sum;       /* their weight sum (the solve's default delta comes from it) */
+  int64_t preference_scale; /* L = lcm(preferences), 1 without preferences */
+  int64_t base;             /* L (K + 1): the largest combined weight */
+  int64_t affected;         /* count_changes: vertices whose combined (distance, parent) changed */
+  dyng_hip_counters solve;  /* the combined solve's engine counters */
+} dyng_hip_mosp_stats;
+
+/* The combined graph of the K trees (tree_parents: a host array of K device parent arrays of g's
+ * n vertices; preferences: K values >= 1 with lcm <= 2^20, or null for all ones): for every
+ * vertex v but the source and every objective k with p = parent_k[v] >= 0, the edge (p, v) once,
+ * weighing L (K + 1) less L / pref[j] for every objective j whose tree holds it (MOSP's
+ * combinedEdge, cpp/src/algorithms/mosp/openmp.cpp:39-118). Its canonical shortest-path tree
+ * from the source goes to distances / parents (device, n); with count_changes they hold the
+ * previous combined tree and `affected` counts the changed pairs. With cost_columns > 0,
+ * path_costs (device, n x cost_columns, vertex-major) gets the first cost_columns weights of g
+ * summed along each vertex's MOSP path, each tree edge (p, v) taken as the first (p, v) of p's
+ * row (mosp_path_costs, cpp/src/algorithms/mosp/sequential.cpp:172-236); INF where the tree
+ * does not reach. */
+int dyng_hip_mosp_combined(const dyng_hip_graph* g, int32_t source, int32_t num_objectives,
+                           const int32_t* const* tree_parents, const int32_t* preferences, int32_t count_changes,
+                           int64_t* distances, int32_t* parents, int32_t cost_columns, int64_t* path_costs,
+                           void* stream, dyng_hip_mosp_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,3 +101,39 @@ def apply_cases() -> list[str]:
     base = os.path.join(GOLDEN, "mosp_graph_io")
     return [c for c in sorted(os.listdir(base))
             if os.path.isdir(os.path.join(base, c, "applied"))]
+
+
+def combined_cases() -> list[tuple[str, str, str, int, list[int]]]:
+    """(name, input directory, init directory, K, preferences) of mosp_combined/cases.txt."""
+    out = []
+    for r in open(os.path.join(GOLDEN, "mosp_combined", "cases.txt")):
+        parts = r.split()
+        if parts:
+            out.append((parts[0], os.path.join(GOLDEN, parts[1]), os.path.join(GOLDEN, parts[2]), int(parts[3]),
+                        [int(x) for x in parts[4].split(",")]))
+    return out
+
+
+def num_weight_columns(prefix: str) -> int:
+    """Weights per line of <prefix>Values.txt."""
+    with open(prefix + "Values.txt") as f:
+        for line in f:
+            if line.split():
+                return len(line.split())
+    return 1
+
+
+def read_costs(path: str, n: int) -> np.ndarray:
+    """mospCosts.txt: 'v c1 .. cK' per vertex, INF for unreachable."""
+    rows = {}
+    width = 0
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if parts:
+                rows[int(parts[0])] = [INF if x == "INF" else int(x) for x in parts[1:]]
+                width = len(parts) - 1
+    c = np.full((n, width), INF, np.int64)
+    for v, r in rows.items():
+        c[v] = r
+    return c
