@@ -6,7 +6,7 @@ HIP for gfx950). There is no CPU fallback: without the built library every call 
 NotSupportedError.
 """
 
-from . import batched, scheduler, sssp
+from . import batched, mosp, scheduler, sssp
 from .batch import EdgeBatch
 from .device import DeviceArray, device_count, synchronize
 from .errors import (DeviceError, DyngError, InvalidArgumentError, NotSupportedError, OutOfMemoryError,
@@ -16,7 +16,7 @@ from .resources import Resources
 
 __version__ = "0.1.0"
 
-__all__ = ["sssp", "batched", "scheduler", "Graph", "GraphProperties", "BatchSemantics", "ApplySummary", "EdgeBatch", "Resources",
+__all__ = ["sssp", "mosp", "batched", "scheduler", "Graph", "GraphProperties", "BatchSemantics", "ApplySummary", "EdgeBatch", "Resources",
            "DeviceArray", "device_count", "synchronize",
            "DyngError", "InvalidArgumentError", "StaleResultError", "NotSupportedError", "OutOfMemoryError",
            "DeviceError", "available", "__version__"]
