@@ -890,37 +890,38 @@ extern "C" int dyng_hip_mosp_combined(const dyng_hip_graph* g, int32_t source, i
     }
     scoped_device guard(g->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
+    // one lock over the build, the solve and the costs: the combined graph and its scratch are
+    // a per-device cache that the next call reuses, and the solve takes the pooled workspace
+    std::lock_guard<std::mutex> lk(pool().lock);
     int64_t edges = 0, weight_sum = 0;
-    std::unique_ptr<dyng_hip_graph> cg(mosp_build_combined(g->device, tree_parents, num_objectives, n, source,
-                                                           static_cast<int32_t>(base), terms.data(), s, &edges,
-                                                           &weight_sum));
+    dyng_hip_graph* cg = mosp_build_combined(g->device, tree_parents, num_objectives, n, source,
+                                             static_cast<int32_t>(base), terms.data(), s, &edges, &weight_sum);
     st.combined_edges = edges;
     st.weight_sum = weight_sum;
-    {
-      // the static solve of sssp_solve_view: the automatic width of the combined graph, the
-      // base weight as the largest weight; with count_changes a recompute that counts the pairs
-      // it changes against the arrays' previous tree
-      std::lock_guard<std::mutex> lk(pool().lock);
-      dyng_hip_sssp_run run{};
-      run.graph = cg.get();
-      run.objective = 0;
-      run.source = source;
-      run.delta = default_delta(std::max<int64_t>(edges, 1), n, weight_sum);
-      run.max_weight = base;
-      run.distances = distances;
-      run.parents = parents;
-      run.workspace = pool().get(g->device);
-      dyng_hip_counters c{};
-      if (count_changes) {
-        run.tuning.recompute = 1;
-        call(dyng_hip_sssp_update(&run, s, &c));
-        st.affected = c.affected;
-      } else {
-        call(dyng_hip_sssp_compute(&run, s, &c));
-      }
-      st.solve = c;
+    // the static solve of sssp_solve_view: the automatic width of the combined graph, the base
+    // weight as the largest weight; with count_changes a recompute that counts the pairs it
+    // changes against the arrays' previous tree
+    dyng_hip_sssp_run run{};
+    run.graph = cg;
+    run.objective = 0;
+    run.source = source;
+    run.delta = default_delta(std::max<int64_t>(edges, 1), n, weight_sum);
+    run.max_weight = base;
+    run.distances = distances;
+    run.parents = parents;
+    run.workspace = pool().get(g->device);
+    dyng_hip_counters c{};
+    if (count_changes) {
+      run.tuning.recompute = 1;
+      call(dyng_hip_sssp_update(&run, s, &c));
+      st.affected = c.affected;
+    } else {
+      call(dyng_hip_sssp_compute(&run, s, &c));
     }
+    st.solve = c;
     if (cost_columns > 0) mosp_path_costs(g, parents, n, source, cost_columns, path_costs, s);
+    // the cached graph must not be rebuilt while this call's work is queued
+    DYNG_HIP_TRY(hipStreamSynchronize(s));
     if (out != nullptr) *out = st;
   });
 }

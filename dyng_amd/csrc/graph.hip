@@ -212,6 +212,11 @@ __global__ void k_len64(const int2* meta, int64_t n, int64_t* out) {
 }
 
 /// Column summary of every objective: pass 0 max and sum, pass 1 the count of maximal weights.
+/// Each wave takes 64 consecutive rows at a time: a short row is scanned by its own lane, a long
+/// one by the whole wave (a wave per row would leave most lanes idle on sparse rows and chain one
+/// row's latency after another).
+constexpr int kShortRow = 16;
+
 __global__ void k_column_summary(const int2* meta, const int32_t* nbr, const int32_t* w, int64_t n, int64_t slot_cap,
                                  int K, int pass, apply_control* ctl) {
   const int64_t wave = (blockIdx.x * int64_t{blockDim.x} + threadIdx.x) / 64;
@@ -219,18 +224,31 @@ __global__ void k_column_summary(const int2* meta, const int32_t* nbr, const int
   for (int k = 0; k < K; ++k) {
     long long mx = 0, sum = 0, cnt = 0, bad = 0;
     const long long target = ctl->col_max[k];
-    for (int64_t u = wave; u < n; u += waves) {
-      const int2 mt = meta[u];
-      for (int j = lane(); j < mt.y; j += 64) {
-        if (nbr[mt.x + j] < 0) continue;  // dead
-        const long long x = w[k * slot_cap + mt.x + j];
-        if (pass == 0) {
-          mx = x > mx ? x : mx;
-          sum += x;
-          bad += x < 1 ? 1 : 0;
-        } else {
-          cnt += x == target ? 1 : 0;
-        }
+    const int32_t* col = w + k * slot_cap;
+    auto visit = [&](int64_t slot) {
+      if (nbr[slot] < 0) return;  // dead
+      const long long x = col[slot];
+      if (pass == 0) {
+        mx = x > mx ? x : mx;
+        sum += x;
+        bad += x < 1 ? 1 : 0;
+      } else {
+        cnt += x == target ? 1 : 0;
+      }
+    };
+    for (int64_t r0 = wave * 64; r0 < n; r0 += waves * 64) {
+      const int64_t u = r0 + lane();
+      const int2 mt = u < n ? meta[u] : make_int2(0, 0);
+      const bool is_long = mt.y > kShortRow;
+      if (!is_long) {
+        for (int j = 0; j < mt.y; ++j) visit(mt.x + int64_t{j});
+      }
+      u64 longs = __ballot(is_long);
+      while (longs) {
+        const int l = __ffsll(static_cast<unsigned long long>(longs)) - 1;
+        longs &= longs - 1;
+        const int off = __shfl(mt.x, l), len = __shfl(mt.y, l);
+        for (int j = lane(); j < len; j += 64) visit(off + int64_t{j});
       }
     }
     if (pass == 0) {
@@ -1429,7 +1447,7 @@ void recompute_columns(dyng_hip_graph* g, hipStream_t s) {
   std::memset(c, 0, sizeof(apply_control));
   DYNG_HIP_TRY(hipMemcpyAsync(g->ctl.p, c, sizeof(apply_control), hipMemcpyHostToDevice, s));
   // a bounded grid (one reduction per wave, then one atomic per wave and column)
-  const unsigned grid = std::min(blocks_for(g->n * 64), 2048u);
+  const unsigned grid = std::min(blocks_for(g->n), 2048u);
   hipLaunchKernelGGL(k_column_summary, dim3(grid), dim3(kBlock), 0, s, g->out.meta.p, g->out.nbr.p, g->out.w.p,
                      g->n, g->out.slot_cap, g->K, 0, g->ctl.p);
   DYNG_HIP_TRY(hipGetLastError());

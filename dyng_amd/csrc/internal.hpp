@@ -182,12 +182,14 @@ hipEvent_t mlp_predict_async(dyng_hip_mlp* m, const dyng_hip_graph* g, const bat
                              int64_t n_tree, int64_t last_invalidated, int64_t last_affected);
 int mlp_pick(const dyng_hip_mlp* m);
 /// MOSP's combined graph of K trees (device parent arrays, a host array of K pointers), built on
-/// the device as a graph of its own; the edge count and weight sum come back with it.
+/// the device as a solve-only graph (out-rows only) that a per-device cache owns and the next
+/// call rebuilds; the edge count and weight sum come back with it. Callers hold pool().lock.
 dyng_hip_graph* mosp_build_combined(int device, const int32_t* const* parents, int32_t num_objectives, int64_t n,
                                     int32_t source, int32_t base, const int32_t* terms, hipStream_t s,
                                     int64_t* edges, int64_t* weight_sum);
 /// The first `columns` objective costs of every vertex's path in the tree `parents` over the
 /// graph's out-rows (device, n x columns, vertex-major); throws if a tree edge is missing.
+/// Callers hold pool().lock (the doubling scratch is the per-device cache's).
 void mosp_path_costs(const dyng_hip_graph* g, const int32_t* parents, int64_t n, int32_t source, int32_t columns,
                      int64_t* costs, hipStream_t s);
 /// The apply after scan_batch. `wait` (optional): an event on another stream that must complete

@@ -1,12 +1,20 @@
 // MOSP's combined solve on the device (cpp/src/algorithms/mosp/mosp.cpp:264-330, the caller of
-// sssp_solve_view): the combined graph of K shortest-path trees, built as a device edge list
-// (one thread per vertex, its K tree edges; MOSP-OpenMP's combinedGraphSospCpu restated as
-// cpp/src/algorithms/mosp/openmp.cpp:39-118 does), and the path costs of the MOSP tree
-// (cpp/src/algorithms/mosp/sequential.cpp:172-236) by path doubling instead of a traversal from
-// the source. The solve itself is the persistent engine of sssp.hip on the combined graph.
+// sssp_solve_view): the combined graph of K shortest-path trees (MOSP-OpenMP's
+// combinedGraphSospCpu, restated by cpp/src/algorithms/mosp/openmp.cpp:39-118) written straight
+// into out-rows, and the path costs of the MOSP tree (cpp/src/algorithms/mosp/sequential.cpp:
+// 172-236) by path doubling instead of a traversal from the source. The solve itself is the
+// persistent engine of sssp.hip on the combined graph.
+//
+// The combined graph is a solve-only graph: out-rows in one arena (row u at [off[u], off[u+1]),
+// its entries in no particular order: the engine's tree is canonical whatever the row order), no
+// in-rows (a solve from the source never pulls), no slack. Its buffers live in a per-device cache
+// and are reused by the next call (they grow, never shrink, until the process ends).
 #include "internal.hpp"
 
 #include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 #include <vector>
@@ -44,123 +52,182 @@ DYNG_DEV bool combined_edge(const combine_params& c, int32_t v, int k, int32_t& 
   return true;
 }
 
-/// One thread per vertex: its combined edges, appended to the edge list with one atomic per
-/// workgroup (a workgroup scan of the threads' edge counts), plus the edges' weight sum.
-__global__ void __launch_bounds__(kBlock) k_combine(combine_params c, int32_t* src, int32_t* dst, int32_t* w,
-                                                    unsigned long long* count, unsigned long long* weight_sum) {
-  __shared__ int wave_total[kWaves];
-  __shared__ int block_base;
+/// Add the lanes' amounts to counter[key] with one atomic per distinct key of the wave (the lanes
+/// of a hub parent share one); each active lane gets the counter's old value plus the amounts of
+/// the lanes below it with the same key. `sub` subtracts instead (the fill pass counts down).
+/// Every lane of the wave calls it.
+DYNG_DEV int wave_keyed_add(int32_t* counter, int32_t key, bool active, bool sub) {
+  int mine = 0;
+  u64 pending = __ballot(active);
+  while (pending) {
+    const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+    const int32_t lkey = __shfl(key, leader);
+    const bool in = active && key == lkey && ((pending >> lane()) & 1ULL);
+    const u64 group = __ballot(in);
+    const int count = __popcll(group);
+    int old = 0;
+    if (lane() == leader) old = sub ? atomicSub(&counter[lkey], count) : atomicAdd(&counter[lkey], count);
+    old = __shfl(old, leader);
+    if (in) mine = sub ? old - count + __popcll(group & lanes_below()) : old + __popcll(group & lanes_below());
+    pending &= ~group;
+  }
+  return mine;
+}
+
+/// Pass 1, one thread per vertex v: the out-degree of every parent in the combined graph, the
+/// edge count and the weight sum.
+__global__ void __launch_bounds__(kBlock) k_combine_count(combine_params c, int32_t* deg,
+                                                          unsigned long long* totals) {
   __shared__ unsigned long long part[kWaves];
   const int32_t v = static_cast<int32_t>(blockIdx.x * kBlock + threadIdx.x);
-  int mine = 0;
-  unsigned long long wsum = 0;
-  if (v < c.n && v != c.source) {
-    for (int k = 0; k < c.K; ++k) {
-      int32_t p, x;
-      if (combined_edge(c, v, k, p, x)) {
-        ++mine;
-        wsum += static_cast<unsigned long long>(x);
-      }
+  const bool live = v < c.n && v != c.source;
+  unsigned long long edges = 0, wsum = 0;
+  for (int k = 0; k < c.K; ++k) {
+    int32_t p = -1, x = 0;
+    const bool has = live && combined_edge(c, v, k, p, x);
+    if (has) {
+      ++edges;
+      wsum += static_cast<unsigned long long>(x);
     }
+    (void)wave_keyed_add(deg, p, has, false);
   }
-  // exclusive scan of `mine` over the workgroup
-  int incl = mine;
-  for (int off = 1; off < 64; off <<= 1) {
-    const int y = __shfl_up(incl, off);
-    if (lane() >= off) incl += y;
-  }
-  const int w_id = static_cast<int>(threadIdx.x) / 64;
-  if (lane() == 63) wave_total[w_id] = incl;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int i = 0; i < kWaves; ++i) {
-      const int t = wave_total[i];
-      wave_total[i] = run;
-      run += t;
-    }
-    block_base = run > 0 ? static_cast<int>(atomicAdd(count, static_cast<unsigned long long>(run))) : 0;
-  }
-  __syncthreads();
-  int slot = block_base + wave_total[w_id] + incl - mine;
-  if (mine > 0) {
-    for (int k = 0; k < c.K; ++k) {
-      int32_t p, x;
-      if (combined_edge(c, v, k, p, x)) {
-        src[slot] = p;
-        dst[slot] = v;
-        w[slot] = x;
-        ++slot;
-      }
-    }
-  }
+  edges = block_reduce(edges, part, op_sum{});
   wsum = block_reduce(wsum, part, op_sum{});
-  if (threadIdx.x == 0 && wsum) atomicAdd(weight_sum, wsum);
+  if (threadIdx.x == 0 && edges) {
+    atomicAdd(&totals[0], edges);
+    atomicAdd(&totals[1], wsum);
+  }
+}
+
+/// Pass 2, one thread per vertex v: its combined edges into their parents' rows (slots counted
+/// down from each row's end), and v's own row bounds {offset, length}.
+__global__ void __launch_bounds__(kBlock) k_combine_fill(combine_params c, const int32_t* off, int32_t* left,
+                                                         int32_t* nbr, int32_t* w, int2* meta, int32_t* base_len) {
+  const int32_t v = static_cast<int32_t>(blockIdx.x * kBlock + threadIdx.x);
+  const bool live = v < c.n && v != c.source;
+  for (int k = 0; k < c.K; ++k) {
+    int32_t p = -1, x = 0;
+    const bool has = live && combined_edge(c, v, k, p, x);
+    const int slot = wave_keyed_add(left, p, has, true);
+    if (has) {
+      const int at = off[p] + slot;
+      nbr[at] = v;
+      w[at] = x;
+    }
+  }
+  if (v < c.n) {
+    const int a = off[v], len = off[v + 1] - a;
+    meta[v] = make_int2(a, len);
+    base_len[v] = len;
+  }
+}
+
+template <typename T>
+__global__ void k_fill_value(T* p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < n; i += int64_t{gridDim.x} * blockDim.x) p[i] = v;
 }
 
 /// Path costs, first step: every vertex's own tree edge (p, v) as the first alive (p, v) of p's
 /// out-row (the base run, then the tail: MOSP's row order), its first `columns` weights; the
 /// source starts at 0 and an unreachable vertex at INF, both with no ancestor. A tree edge that
-/// is not an edge of the graph is reported (the lowest such vertex).
+/// is not an edge of the graph is reported (the lowest such vertex). active[0] is set when some
+/// vertex has an ancestor.
 __global__ void k_tree_edge_costs(const int2* meta, const int32_t* base, const int32_t* nbr, const int32_t* w,
                                   int64_t slot_cap, int columns, const int32_t* parents, int32_t n, int32_t source,
-                                  int64_t* acc, int32_t* jump, int* missing) {
+                                  int64_t* acc, int32_t* jump, int* missing, int* active) {
   const int32_t v = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (v >= n) return;
-  const int32_t p = v == source ? -1 : parents[v];
-  int64_t* a = acc + static_cast<int64_t>(v) * columns;
-  if (p < 0) {
-    const int64_t start = v == source ? 0 : sssp_infinity;
-    for (int c = 0; c < columns; ++c) a[c] = start;
-    jump[v] = -1;
-    return;
-  }
-  const int2 mt = meta[p];
-  const int32_t* row = nbr + mt.x;
-  const int bl = base[p];
-  int hit = -1;
-  for (int r = 0; r < 2 && hit < 0; ++r) {
-    const int lo = r == 0 ? 0 : bl, hi = r == 0 ? bl : mt.y;
-    for (int x = row_lower_bound(row, lo, hi, v); x < hi && (row[x] & 0x7fffffff) == v; ++x) {
-      if (row[x] >= 0) {
-        hit = x;
-        break;
+  int32_t link = -1;
+  if (v < n) {
+    const int32_t p = v == source ? -1 : parents[v];
+    int64_t* a = acc + static_cast<int64_t>(v) * columns;
+    int hit = -1;
+    int2 mt = make_int2(0, 0);
+    if (p >= 0) {
+      mt = meta[p];
+      const int32_t* row = nbr + mt.x;
+      const int bl = base[p];
+      for (int r = 0; r < 2 && hit < 0; ++r) {
+        const int lo = r == 0 ? 0 : bl, hi = r == 0 ? bl : mt.y;
+        for (int x = row_lower_bound(row, lo, hi, v); x < hi && (row[x] & 0x7fffffff) == v; ++x) {
+          if (row[x] >= 0) {
+            hit = x;
+            break;
+          }
+        }
       }
+      if (hit < 0) atomicMin(missing, v);
     }
+    if (hit >= 0) {
+      for (int c = 0; c < columns; ++c) a[c] = w[c * slot_cap + mt.x + hit];
+      link = p;
+    } else {
+      const int64_t start = v == source ? 0 : sssp_infinity;
+      for (int c = 0; c < columns; ++c) a[c] = start;
+    }
+    jump[v] = link;
   }
-  if (hit < 0) {
-    atomicMin(missing, v);
-    for (int c = 0; c < columns; ++c) a[c] = sssp_infinity;
-    jump[v] = -1;
-    return;
-  }
-  for (int c = 0; c < columns; ++c) a[c] = w[c * slot_cap + mt.x + hit];
-  jump[v] = p;
+  if (__syncthreads_or(link >= 0) && threadIdx.x == 0) active[0] = 1;
 }
 
-/// Path doubling: each vertex adds its ancestor's sum and jumps to the ancestor's ancestor, so
-/// after ceil(log2 n) + 1 rounds every sum runs back to the source.
+/// Path doubling, round r: each vertex adds its ancestor's sum and jumps to the ancestor's
+/// ancestor. A round whose input has no vertex left with an ancestor does nothing; a round that
+/// runs records itself in `last` and sets active[r + 1] when some vertex still has an ancestor.
 __global__ void k_path_double(const int64_t* acc_in, const int32_t* jump_in, int64_t* acc_out, int32_t* jump_out,
-                              int32_t n, int columns) {
+                              int32_t n, int columns, int round, int* active, int* last) {
+  if (active[round] == 0) return;  // uniform: every thread reads the same flag
   const int32_t v = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (v >= n) return;
-  const int32_t j = jump_in[v];
-  const int64_t* a = acc_in + static_cast<int64_t>(v) * columns;
-  int64_t* o = acc_out + static_cast<int64_t>(v) * columns;
-  if (j < 0) {
-    for (int c = 0; c < columns; ++c) o[c] = a[c];
-    jump_out[v] = -1;
-    return;
+  int32_t next = -1;
+  if (v < n) {
+    const int32_t j = jump_in[v];
+    const int64_t* a = acc_in + static_cast<int64_t>(v) * columns;
+    int64_t* o = acc_out + static_cast<int64_t>(v) * columns;
+    if (j < 0) {
+      for (int c = 0; c < columns; ++c) o[c] = a[c];
+    } else {
+      const int64_t* b = acc_in + static_cast<int64_t>(j) * columns;
+      for (int c = 0; c < columns; ++c) o[c] = a[c] + b[c];
+      next = jump_in[j];
+    }
+    jump_out[v] = next;
   }
-  const int64_t* b = acc_in + static_cast<int64_t>(j) * columns;
-  for (int c = 0; c < columns; ++c) o[c] = a[c] + b[c];
-  jump_out[v] = jump_in[j];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *last = round;
+  if (__syncthreads_or(next >= 0) && threadIdx.x == 0) active[round + 1] = 1;
 }
 
-unsigned int grid(int64_t work, int block) { return static_cast<unsigned int>(std::max<int64_t>(1, (work + block - 1) / block)); }
+/// Round r writes buffer (r + 1) % 2 (buffer 0 is the caller's, which the first step wrote): the
+/// sums are in `acc` only when the last round that ran is even; then copy them home.
+__global__ void k_costs_home(const int64_t* acc, int64_t* costs, int64_t count, const int* last) {
+  const int r = *last;
+  if (r < 0 || (r & 1) == 1) return;
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < count; i += int64_t{gridDim.x} * blockDim.x) {
+    costs[i] = acc[i];
+  }
+}
 
-void check_status(int status) {
-  if (status != DYNG_HIP_OK) throw status_error(status, dyng_hip_last_error());
+unsigned int grid(int64_t work, int block) {
+  return static_cast<unsigned int>(std::max<int64_t>(1, (work + block - 1) / block));
+}
+
+/// The solve-only combined graph and the scratch of its build, one per device.
+struct combined_cache {
+  std::unique_ptr<dyng_hip_graph> g;
+  dbuf<int32_t> deg, off;
+  dbuf<unsigned char> scan_tmp;
+  dbuf<const int32_t*> parents;
+  dbuf<int32_t> terms;
+  dbuf<unsigned long long> totals;
+  dbuf<int64_t> acc;
+  dbuf<int32_t> jump_a, jump_b;
+  dbuf<int> flags;  ///< missing vertex, last round, then the rounds' active flags
+  const int2* in_meta_cleared = nullptr;  ///< the in-row meta array last cleared
+};
+
+combined_cache& cache_for(int device) {
+  static std::vector<std::unique_ptr<combined_cache>> caches;
+  if (device < 0) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: bad device");
+  if (static_cast<size_t>(device) >= caches.size()) caches.resize(static_cast<size_t>(device) + 1);
+  if (!caches[device]) caches[device] = std::make_unique<combined_cache>();
+  return *caches[device];
 }
 
 }  // namespace
@@ -168,71 +235,113 @@ void check_status(int status) {
 dyng_hip_graph* mosp_build_combined(int device, const int32_t* const* parents, int32_t num_objectives, int64_t n,
                                     int32_t source, int32_t base, const int32_t* terms, hipStream_t s,
                                     int64_t* edges, int64_t* weight_sum) {
-  dbuf<const int32_t*> dparents;
-  dbuf<int32_t> dterms;
-  dbuf<int32_t> esrc, edst, ew;
-  dbuf<unsigned long long> totals;
-  dparents.reserve(num_objectives);
-  dterms.reserve(num_objectives);
-  const int64_t cap = std::max<int64_t>(1, n * num_objectives);
-  esrc.reserve(cap);
-  edst.reserve(cap);
-  ew.reserve(cap);
-  totals.reserve(2);
-  // small synchronous copies from the caller's host arrays (done before anything is queued)
-  DYNG_HIP_TRY(hipMemcpy(dparents.p, parents, num_objectives * sizeof(const int32_t*), hipMemcpyHostToDevice));
-  DYNG_HIP_TRY(hipMemcpy(dterms.p, terms, num_objectives * sizeof(int32_t), hipMemcpyHostToDevice));
-  DYNG_HIP_TRY(hipMemsetAsync(totals.p, 0, 2 * sizeof(unsigned long long), s));
-  combine_params c{dparents.p, dterms.p, num_objectives, static_cast<int32_t>(n), source, base};
-  hipLaunchKernelGGL(k_combine, dim3(grid(n, kBlock)), dim3(kBlock), 0, s, c, esrc.p, edst.p, ew.p, totals.p,
-                     totals.p + 1);
-  DYNG_HIP_TRY(hipGetLastError());
-  unsigned long long host[2] = {0, 0};
-  DYNG_HIP_TRY(hipMemcpyAsync(host, totals.p, sizeof(host), hipMemcpyDeviceToHost, s));
+  combined_cache& cc = cache_for(device);
+  if (!cc.g) cc.g = std::make_unique<dyng_hip_graph>();
+  dyng_hip_graph* g = cc.g.get();
+  cc.parents.reserve(num_objectives);
+  cc.terms.reserve(num_objectives);
+  cc.totals.reserve(2);
+  cc.deg.reserve(n + 1);
+  cc.off.reserve(n + 1);
+  // small synchronous copies from the caller's host arrays (queued work on `s` may still read
+  // the previous call's values, so wait for it first)
   DYNG_HIP_TRY(hipStreamSynchronize(s));
-  *edges = static_cast<int64_t>(host[0]);
+  DYNG_HIP_TRY(hipMemcpy(cc.parents.p, parents, num_objectives * sizeof(const int32_t*), hipMemcpyHostToDevice));
+  DYNG_HIP_TRY(hipMemcpy(cc.terms.p, terms, num_objectives * sizeof(int32_t), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_fill_value<unsigned long long>, dim3(1), dim3(64), 0, s, cc.totals.p, int64_t{2}, 0ULL);
+  hipLaunchKernelGGL(k_fill_value<int32_t>, dim3(std::min(grid(n + 1, 256), 4096u)), dim3(256), 0, s, cc.deg.p,
+                     n + 1, 0);
+  combine_params c{cc.parents.p, cc.terms.p, num_objectives, static_cast<int32_t>(n), source, base};
+  hipLaunchKernelGGL(k_combine_count, dim3(grid(n, kBlock)), dim3(kBlock), 0, s, c, cc.deg.p, cc.totals.p);
+  DYNG_HIP_TRY(hipGetLastError());
+  size_t bytes = 0;
+  DYNG_HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, cc.deg.p, cc.off.p, 0, static_cast<size_t>(n + 1),
+                                       rocprim::plus<int32_t>(), s));
+  cc.scan_tmp.reserve(bytes);
+  DYNG_HIP_TRY(rocprim::exclusive_scan(static_cast<void*>(cc.scan_tmp.p), bytes, cc.deg.p, cc.off.p, 0,
+                                       static_cast<size_t>(n + 1), rocprim::plus<int32_t>(), s));
+  unsigned long long host[2] = {0, 0};
+  DYNG_HIP_TRY(hipMemcpyAsync(host, cc.totals.p, sizeof(host), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  const int64_t m = static_cast<int64_t>(host[0]);
+  if (m >= INT32_MAX) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the combined graph has 2^31 or more edges");
+  *edges = m;
   *weight_sum = static_cast<int64_t>(host[1]);
-  // a simple directed graph with sorted rows; the combined edges are distinct pairs already
-  dyng_hip_graph_props props{1, 1, 0, 0, 0, 0, 1, 1, 0.25};
-  dyng_hip_graph* cg = nullptr;
-  check_status(dyng_hip_graph_create_from_edges(device, &props, n, *edges, 1, esrc.p, edst.p, ew.p, 1, s, &cg));
-  return cg;
+  // the graph: out-rows in one arena of exactly m slots, rows in place of their offsets; the
+  // degree array counts down to zero as the fill pass claims slots
+  g->device = device;
+  g->props = dyng_hip_graph_props{1, 1, 0, 0, 0, 0, 1, 1, 0.0};
+  g->K = 1;
+  g->n = n;
+  g->m = m;
+  g->n_cap = std::max<int64_t>(g->n_cap, n);
+  g->out.meta.reserve(g->n_cap);
+  g->out.base.reserve(g->n_cap);
+  g->out.nbr.reserve(std::max<int64_t>(m, 1));
+  g->out.w.reserve(std::max<int64_t>(m, 1));
+  g->out.slot_cap = std::max<int64_t>(m, 1);
+  g->out.tail = m;
+  g->out.garbage = 0;
+  // no in-rows: every vertex has an empty in-row
+  g->in.meta.reserve(g->n_cap);
+  if (cc.in_meta_cleared != g->in.meta.p) {  // a new allocation
+    hipLaunchKernelGGL(k_fill_value<int2>, dim3(std::min(grid(g->n_cap, 256), 4096u)), dim3(256), 0, s,
+                       g->in.meta.p, g->n_cap, make_int2(0, 0));
+    cc.in_meta_cleared = g->in.meta.p;
+  }
+  g->in.nbr.reserve(1);
+  g->in.w.reserve(1);
+  g->in.slot_cap = 0;
+  g->wmax.assign(1, std::max<int64_t>(base, 1));
+  g->wsum.assign(1, *weight_sum);
+  g->wmax_count.assign(1, 0);
+  g->wmin.assign(1, 1);
+  g->last.valid = false;
+  g->num_del_slots = 0;
+  g->num_ins_slots = 0;
+  hipLaunchKernelGGL(k_combine_fill, dim3(grid(n, kBlock)), dim3(kBlock), 0, s, c, cc.off.p, cc.deg.p, g->out.nbr.p,
+                     g->out.w.p, g->out.meta.p, g->out.base.p);
+  DYNG_HIP_TRY(hipGetLastError());
+  return g;
 }
 
 void mosp_path_costs(const dyng_hip_graph* g, const int32_t* parents, int64_t n, int32_t source, int32_t columns,
                      int64_t* costs, hipStream_t s) {
   if (n == 0 || columns == 0) return;
-  dbuf<int64_t> acc;
-  dbuf<int32_t> jump_a, jump_b;
-  dbuf<int> missing;
-  acc.reserve(static_cast<size_t>(n) * columns);
-  jump_a.reserve(n);
-  jump_b.reserve(n);
-  missing.reserve(1);
-  const int first_missing = INT32_MAX;
-  DYNG_HIP_TRY(hipMemcpy(missing.p, &first_missing, sizeof(int), hipMemcpyHostToDevice));
-  // the first round writes the caller's array; the rounds then alternate between it and `acc`
-  int64_t* bufs[2] = {costs, acc.p};
-  int32_t* jumps[2] = {jump_a.p, jump_b.p};
-  hipLaunchKernelGGL(k_tree_edge_costs, dim3(grid(n, 256)), dim3(256), 0, s, g->out.meta.p, g->out.base.p, g->out.nbr.p,
-                     g->out.w.p, g->out.slot_cap, columns, parents, static_cast<int32_t>(n), source, bufs[0], jumps[0],
-                     missing.p);
+  combined_cache& cc = cache_for(g->device);
   int rounds = 1;
   while ((int64_t{1} << rounds) < n) ++rounds;
   rounds += 1;
+  cc.acc.reserve(static_cast<size_t>(n) * columns);
+  cc.jump_a.reserve(n);
+  cc.jump_b.reserve(n);
+  cc.flags.reserve(rounds + 3);
+  // flags: [0] the lowest vertex whose tree edge is missing, [1] the last round that ran, then
+  // active[0 .. rounds]
+  std::vector<int> init(static_cast<size_t>(rounds) + 3, 0);
+  init[0] = INT32_MAX;
+  init[1] = -1;
+  DYNG_HIP_TRY(hipMemcpyAsync(cc.flags.p, init.data(), init.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  int* missing = cc.flags.p;
+  int* last = cc.flags.p + 1;
+  int* active = cc.flags.p + 2;
+  // the first step writes the caller's array; the rounds then alternate between it and `acc`
+  int64_t* bufs[2] = {costs, cc.acc.p};
+  int32_t* jumps[2] = {cc.jump_a.p, cc.jump_b.p};
+  hipLaunchKernelGGL(k_tree_edge_costs, dim3(grid(n, 256)), dim3(256), 0, s, g->out.meta.p, g->out.base.p, g->out.nbr.p,
+                     g->out.w.p, g->out.slot_cap, columns, parents, static_cast<int32_t>(n), source, bufs[0], jumps[0],
+                     missing, active);
   int cur = 0;
   for (int r = 0; r < rounds; ++r) {
     hipLaunchKernelGGL(k_path_double, dim3(grid(n, 256)), dim3(256), 0, s, bufs[cur], jumps[cur], bufs[cur ^ 1],
-                       jumps[cur ^ 1], static_cast<int32_t>(n), columns);
+                       jumps[cur ^ 1], static_cast<int32_t>(n), columns, r, active, last);
     cur ^= 1;
   }
+  hipLaunchKernelGGL(k_costs_home, dim3(std::min(grid(n * columns, 256), 8192u)), dim3(256), 0, s, cc.acc.p, costs,
+                     n * columns, last);
   DYNG_HIP_TRY(hipGetLastError());
-  if (cur != 0) {
-    DYNG_HIP_TRY(hipMemcpyAsync(costs, acc.p, static_cast<size_t>(n) * columns * sizeof(int64_t),
-                                hipMemcpyDeviceToDevice, s));
-  }
   int host_missing = INT32_MAX;
-  DYNG_HIP_TRY(hipMemcpyAsync(&host_missing, missing.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipMemcpyAsync(&host_missing, missing, sizeof(int), hipMemcpyDeviceToHost, s));
   DYNG_HIP_TRY(hipStreamSynchronize(s));
   if (host_missing != INT32_MAX) {
     fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the MOSP tree edge (" + std::to_string(host_missing) +

@@ -37,6 +37,19 @@ class DeviceArray:
         if a.nbytes:
             _lib.check(_lib.lib().dyng_hip_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes))
 
+    @classmethod
+    def empty(cls, shape, dtype, device: int = 0) -> "DeviceArray":
+        """An uninitialised device array (no host copy is made)."""
+        self = object.__new__(cls)
+        self.shape = tuple(int(x) for x in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.device = device
+        self.ptr = 0
+        p = C.c_void_p()
+        _lib.check(_lib.lib().dyng_hip_malloc(device, self.nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+        return self
+
     @property
     def size(self) -> int:
         return int(np.prod(self.shape)) if self.shape else 1

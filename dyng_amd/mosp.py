@@ -129,7 +129,7 @@ class Result:
     (mosp::result, python/dyng/mosp.py:186-403)."""
 
     __slots__ = ("_trees", "_options", "_resources", "_source", "_count", "_dist", "_par", "_costs",
-                 "_costs_valid", "_scale", "_combined_edges", "__weakref__")
+                 "_costs_host", "_costs_valid", "_scale", "_combined_edges", "__weakref__")
 
     def __init__(self):
         raise TypeError("use dyng_amd.mosp.compute() or dyng_amd.mosp.Result.from_arrays()")
@@ -144,7 +144,8 @@ class Result:
         self._count = count
         self._dist = None
         self._par = None
-        self._costs = None
+        self._costs = None  # device (n, K) int64
+        self._costs_host = None  # its host copy, made on first read after each compute / update
         self._costs_valid = False
         self._scale = 1
         self._combined_edges = 0
@@ -193,11 +194,24 @@ class Result:
 
     @property
     def path_costs(self) -> np.ndarray:
-        """(n, K) int64 host array: the K objective values of every vertex's MOSP path."""
+        """(n, K) int64 host array: the K objective values of every vertex's MOSP path
+        (INFINITE_DISTANCE where the MOSP tree does not reach). Computed on the device by every
+        compute / update; copied to the host on the first read after it."""
         if not self._costs_valid:
             raise InvalidArgumentError("mosp.Result.path_costs: the last compute or update ran with "
                                        "compute_path_costs=False")
-        return self._costs.copy()
+        if self._costs_host is None:
+            self._costs_host = self._costs.to_numpy() if self._costs is not None else \
+                np.zeros((0, self._count), np.int64)
+        return self._costs_host.copy()
+
+    def path_costs_device(self) -> tuple[int, int, int]:
+        """(path costs pointer, n, K) in device memory (vertex-major, int64)."""
+        if not self._costs_valid:
+            raise InvalidArgumentError("mosp.Result.path_costs_device: the last compute or update ran with "
+                                       "compute_path_costs=False")
+        n = self._costs.shape[0] if self._costs is not None else 0
+        return (self._costs.ptr if self._costs is not None else 0, n, self._count)
 
     @property
     def preference_scale(self) -> int:
@@ -245,7 +259,8 @@ class Result:
         if self._dist is not None:
             out._dist = DeviceArray(self._dist.to_numpy(), device=res.device)
             out._par = DeviceArray(self._par.to_numpy(), device=res.device)
-        out._costs = None if self._costs is None else self._costs.copy()
+        if self._costs is not None:
+            out._costs = DeviceArray(self._costs.to_numpy(), device=res.device)
         out._costs_valid = self._costs_valid
         out._scale = self._scale
         out._combined_edges = self._combined_edges
@@ -316,14 +331,16 @@ def _combine(graph: Graph, result: Result, *, count_changes: bool) -> _lib.MospS
     prefs = result._options.preferences
     pref_arr = (C.c_int32 * count)(*prefs) if prefs else None
     columns = count if result._options.compute_path_costs else 0
-    costs_dev = DeviceArray(np.empty((n, columns), np.int64), device=res.device) if columns and n else None
+    costs = result._costs
+    if columns and n and (costs is None or costs.shape != (n, columns)):
+        costs = DeviceArray.empty((n, columns), np.int64, device=res.device)
     st = _lib.MospStats()
     _lib.check(_lib.lib().dyng_hip_mosp_combined(
         graph._handle, result._source, count, ptrs, pref_arr, int(bool(count_changes)), result._dist.ptr,
-        result._par.ptr, columns, costs_dev.ptr if costs_dev is not None else None, res.stream_handle,
-        C.byref(st)))
+        result._par.ptr, columns, costs.ptr if (columns and n) else None, res.stream_handle, C.byref(st)))
+    result._costs_host = None
     if columns:
-        result._costs = costs_dev.to_numpy() if costs_dev is not None else np.zeros((n, columns), np.int64)
+        result._costs = costs if n else None
         result._costs_valid = True
     else:
         result._costs_valid = False
