@@ -814,7 +814,7 @@ constexpr int kMergeBlock = 1024;  ///< threads of a merge workgroup
 constexpr int kMergeWaves = kMergeBlock / 64;
 constexpr int kMergeSeg = 2048;    ///< operations of one row a workgroup stages in LDS
 constexpr int kRegChunks = kMergeSmall / 64;  ///< a short tail held in registers, 64 entries per chunk
-constexpr int kRegWeights = 2;                ///< weight columns a tail merge holds in registers
+constexpr int kRegWeights = 2;                ///< weight columns a tail merge moves per pass
 constexpr int kTailMin = 64;                  ///< a tail may always grow to this many entries
 constexpr int kTailMax = 4096;                ///< and never past this (a workgroup's registers)
 constexpr int kTailPerThread = kTailMax / kMergeBlock;
@@ -862,7 +862,7 @@ DYNG_DEV row_plan plan_row(const merge_params& p, int32_t u, int2 mt, int64_t a,
   r.new_len = r.alive_base + r.new_tail;
   const int nops = static_cast<int>(b - a);
   const bool tail_ok = r.new_tail <= tail_limit(r.base_len) && r.base_len + r.new_tail <= p.cap[u] &&
-                       p.dead[u] <= dead_limit(r.base_len) && p.K <= kRegWeights;
+                       p.dead[u] <= dead_limit(r.base_len);
   if (tail_ok && r.tail_len <= kLaneTail && nops <= kLaneOps) {
     r.mode = kTailLane;
   } else if (tail_ok && r.tail_len <= kMergeSmall && nops <= 64) {
@@ -931,12 +931,12 @@ __global__ void __launch_bounds__(kPlanBlock) k_merge_plan(merge_params p) {
 }
 
 /// One thread per row whose tail holds at most kLaneTail entries and that has at most kLaneOps
-/// operations (no more than kRegWeights weight columns): the tail and the operations are read
-/// into registers with independent loads, every entry's place is counted with static indices
-/// (an alive entry: the alive entries before it plus the pending entries with a smaller
-/// neighbour; a pending entry: the alive entries up to its neighbour plus the pending entries
-/// before it), then each is stored at its place. Old entries come first among equal neighbours,
-/// as in the wave merge below.
+/// operations: the tail and the operations are read into registers with independent loads, every
+/// entry's place is counted with static indices (an alive entry: the alive entries before it plus
+/// the pending entries with a smaller neighbour; a pending entry: the alive entries up to its
+/// neighbour plus the pending entries before it), then each is stored at its place. The weights
+/// move kRegWeights columns per pass, each pass reading its columns before writing them. Old
+/// entries come first among equal neighbours, as in the wave merge below.
 __global__ void __launch_bounds__(256) k_tail_lane(merge_params p) {
   const int count = p.list_count[kTailLane];
   const int32_t* list = p.lists + kTailLane * (p.N + 1);
@@ -951,54 +951,62 @@ __global__ void __launch_bounds__(256) k_tail_lane(merge_params p) {
     const int tl = mt.y - bl;
     const int64_t at = mt.x + int64_t{bl};  // the tail's first slot
     int32_t x[kLaneTail];
-    int32_t wr[kRegWeights][kLaneTail];
 #pragma unroll
-    for (int j = 0; j < kLaneTail; ++j) {
-      const bool in = j < tl;
-      x[j] = in ? p.nbr[at + j] : -1;
-#pragma unroll
-      for (int k = 0; k < kRegWeights; ++k) wr[k][j] = (in && k < K) ? p.w[k * p.slot_cap + at + j] : 0;
-    }
+    for (int j = 0; j < kLaneTail; ++j) x[j] = j < tl ? p.nbr[at + j] : -1;
     int32_t ov[kLaneOps];
-    int32_t ow[kRegWeights][kLaneOps];
     bool pend[kLaneOps];
 #pragma unroll
     for (int j = 0; j < kLaneOps; ++j) {
       pend[j] = j < nops && p.pend_alive[a + j] != 0;
       ov[j] = pend[j] ? static_cast<int32_t>(p.keys[a + j] & mask) : 0;
-#pragma unroll
-      for (int k = 0; k < kRegWeights; ++k) ow[k][j] = (pend[j] && k < K) ? p.pend_w[k * p.N + a + j] : 0;
     }
-    // alive entries, at their places (the tail is sorted, dead entries carry the sign bit)
-    int alive = 0;
+    int alive = 0, pending = 0;
+    for (int k0 = 0; k0 == 0 || k0 < K; k0 += kRegWeights) {
+      int32_t wr[kRegWeights][kLaneTail];
+      int32_t ow[kRegWeights][kLaneOps];
 #pragma unroll
-    for (int i = 0; i < kLaneTail; ++i) {
-      if (x[i] >= 0) {
-        int pos = alive;
-#pragma unroll
-        for (int j = 0; j < kLaneOps; ++j) pos += (pend[j] && ov[j] < x[i]) ? 1 : 0;
-        p.nbr[at + pos] = x[i];
+      for (int j = 0; j < kLaneTail; ++j) {
 #pragma unroll
         for (int k = 0; k < kRegWeights; ++k) {
-          if (k < K) p.w[k * p.slot_cap + at + pos] = wr[k][i];
+          wr[k][j] = (j < tl && k0 + k < K) ? p.w[(k0 + k) * p.slot_cap + at + j] : 0;
         }
-        ++alive;
       }
-    }
-    // pending entries (the operations are sorted by neighbour, then batch order)
-    int pending = 0;
 #pragma unroll
-    for (int j = 0; j < kLaneOps; ++j) {
-      if (pend[j]) {
-        int pos = pending;
+      for (int j = 0; j < kLaneOps; ++j) {
 #pragma unroll
-        for (int i = 0; i < kLaneTail; ++i) pos += (x[i] >= 0 && x[i] <= ov[j]) ? 1 : 0;
-        p.nbr[at + pos] = ov[j];
+        for (int k = 0; k < kRegWeights; ++k) ow[k][j] = (pend[j] && k0 + k < K) ? p.pend_w[(k0 + k) * p.N + a + j] : 0;
+      }
+      // alive entries, at their places (the tail is sorted, dead entries carry the sign bit)
+      alive = 0;
 #pragma unroll
-        for (int k = 0; k < kRegWeights; ++k) {
-          if (k < K) p.w[k * p.slot_cap + at + pos] = ow[k][j];
+      for (int i = 0; i < kLaneTail; ++i) {
+        if (x[i] >= 0) {
+          int pos = alive;
+#pragma unroll
+          for (int j = 0; j < kLaneOps; ++j) pos += (pend[j] && ov[j] < x[i]) ? 1 : 0;
+          if (k0 == 0) p.nbr[at + pos] = x[i];
+#pragma unroll
+          for (int k = 0; k < kRegWeights; ++k) {
+            if (k0 + k < K) p.w[(k0 + k) * p.slot_cap + at + pos] = wr[k][i];
+          }
+          ++alive;
         }
-        ++pending;
+      }
+      // pending entries (the operations are sorted by neighbour, then batch order)
+      pending = 0;
+#pragma unroll
+      for (int j = 0; j < kLaneOps; ++j) {
+        if (pend[j]) {
+          int pos = pending;
+#pragma unroll
+          for (int i = 0; i < kLaneTail; ++i) pos += (x[i] >= 0 && x[i] <= ov[j]) ? 1 : 0;
+          if (k0 == 0) p.nbr[at + pos] = ov[j];
+#pragma unroll
+          for (int k = 0; k < kRegWeights; ++k) {
+            if (k0 + k < K) p.w[(k0 + k) * p.slot_cap + at + pos] = ow[k][j];
+          }
+          ++pending;
+        }
       }
     }
     p.meta[u] = make_int2(mt.x, bl + alive + pending);
@@ -1027,14 +1035,10 @@ __global__ void __launch_bounds__(kMergeBlock) k_tail_wave(merge_params p) {
     const int tl = mt.y - bl;
     const int64_t at = mt.x + int64_t{bl};  // the tail's first slot
     int32_t x[kRegChunks];
-    int32_t wr[kRegWeights][kRegChunks];
 #pragma unroll
     for (int c = 0; c < kRegChunks; ++c) {
       const int jx = c * 64 + lane();
-      const bool in = jx < tl;
-      x[c] = in ? p.nbr[at + jx] : -1;
-#pragma unroll
-      for (int k = 0; k < kRegWeights; ++k) wr[k][c] = (in && k < K) ? p.w[k * p.slot_cap + at + jx] : 0;
+      x[c] = jx < tl ? p.nbr[at + jx] : -1;
     }
     const bool has = lane() < nops;
     const int32_t ov = has ? static_cast<int32_t>(p.keys[a + lane()] & mask) : INT32_MAX;
@@ -1059,24 +1063,41 @@ __global__ void __launch_bounds__(kMergeBlock) k_tail_wave(merge_params p) {
       for (int c = 0; c < kRegChunks; ++c) y += __popcll(__ballot(x[c] >= 0 && x[c] <= vj));
       if (lane() == j) ppos = at + y + __popcll(pend_mask & ((1ULL << j) - 1));
     }
+    // the stores, kRegWeights weight columns per pass: every lane of the wave reads its entries'
+    // old weights of the pass's columns before any lane writes them
     int carry = 0;
+    for (int k0 = 0; k0 == 0 || k0 < K; k0 += kRegWeights) {
+      int32_t wr[kRegWeights][kRegChunks];
 #pragma unroll
-    for (int c = 0; c < kRegChunks; ++c) {
-      const bool alive = x[c] >= 0;
-      const u64 ballot = __ballot(alive);
-      if (alive) {
-        const int64_t pos = at + carry + __popcll(ballot & lanes_below()) + before[c];
-        p.nbr[pos] = x[c];
+      for (int c = 0; c < kRegChunks; ++c) {
+        const int jx = c * 64 + lane();
 #pragma unroll
         for (int k = 0; k < kRegWeights; ++k) {
-          if (k < K) p.w[k * p.slot_cap + pos] = wr[k][c];
+          wr[k][c] = (jx < tl && k0 + k < K) ? p.w[(k0 + k) * p.slot_cap + at + jx] : 0;
         }
       }
-      carry += __popcll(ballot);
-    }
-    if (opend) {
-      p.nbr[ppos] = ov;
-      for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + ppos] = p.pend_w[k * p.N + a + lane()];
+      carry = 0;
+#pragma unroll
+      for (int c = 0; c < kRegChunks; ++c) {
+        const bool alive = x[c] >= 0;
+        const u64 ballot = __ballot(alive);
+        if (alive) {
+          const int64_t pos = at + carry + __popcll(ballot & lanes_below()) + before[c];
+          if (k0 == 0) p.nbr[pos] = x[c];
+#pragma unroll
+          for (int k = 0; k < kRegWeights; ++k) {
+            if (k0 + k < K) p.w[(k0 + k) * p.slot_cap + pos] = wr[k][c];
+          }
+        }
+        carry += __popcll(ballot);
+      }
+      if (opend) {
+        if (k0 == 0) p.nbr[ppos] = ov;
+#pragma unroll
+        for (int k = 0; k < kRegWeights; ++k) {
+          if (k0 + k < K) p.w[(k0 + k) * p.slot_cap + ppos] = p.pend_w[(k0 + k) * p.N + a + lane()];
+        }
+      }
     }
     if (lane() == 0) p.meta[u] = make_int2(mt.x, bl + carry + __popcll(pend_mask));
   }
@@ -1143,22 +1164,39 @@ __global__ void __launch_bounds__(kMergeBlock) k_tail_block(merge_params p) {
       if (x[e] >= 0) s_t[apos[e]] = x[e];
     }
     __syncthreads();
-    // pending entries: the alive tail entries up to their neighbour, the pending ones before them
-    for (int j = tid; j < nops; j += kMergeBlock) {
-      if (s_pp[j + 1] == s_pp[j]) continue;  // not pending
-      const int32_t v = s_v[j];
-      const int64_t pos = at + lds_bound(s_t, alive_tail, v, true) + s_pp[j];
-      p.nbr[pos] = v;
-      for (int k = 0; k < K; ++k) p.w[k * p.slot_cap + pos] = p.pend_w[k * p.N + a + j];
-    }
+    // the stores, kRegWeights weight columns per pass (the first pass's weights were read with
+    // the tail; a later pass reads its columns, then waits for the whole workgroup before writing)
+    for (int k0 = 0; k0 == 0 || k0 < K; k0 += kRegWeights) {
+      if (k0 > 0) {
 #pragma unroll
-    for (int e = 0; e < kTailPerThread; ++e) {
-      if (x[e] < 0) continue;
-      const int64_t pos = at + apos[e] + s_pp[lds_bound(s_v, nops, x[e], false)];
-      p.nbr[pos] = x[e];
+        for (int e = 0; e < kTailPerThread; ++e) {
+          const int jx = e * kMergeBlock + tid;
 #pragma unroll
-      for (int k = 0; k < kRegWeights; ++k) {
-        if (k < K) p.w[k * p.slot_cap + pos] = wr[k][e];
+          for (int k = 0; k < kRegWeights; ++k) {
+            wr[k][e] = (jx < tl && k0 + k < K) ? p.w[(k0 + k) * p.slot_cap + at + jx] : 0;
+          }
+        }
+        __syncthreads();
+      }
+      // pending entries: the alive tail entries up to their neighbour, the pending ones before them
+      for (int j = tid; j < nops; j += kMergeBlock) {
+        if (s_pp[j + 1] == s_pp[j]) continue;  // not pending
+        const int32_t v = s_v[j];
+        const int64_t pos = at + lds_bound(s_t, alive_tail, v, true) + s_pp[j];
+        if (k0 == 0) p.nbr[pos] = v;
+        for (int k = 0; k < kRegWeights; ++k) {
+          if (k0 + k < K) p.w[(k0 + k) * p.slot_cap + pos] = p.pend_w[(k0 + k) * p.N + a + j];
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < kTailPerThread; ++e) {
+        if (x[e] < 0) continue;
+        const int64_t pos = at + apos[e] + s_pp[lds_bound(s_v, nops, x[e], false)];
+        if (k0 == 0) p.nbr[pos] = x[e];
+#pragma unroll
+        for (int k = 0; k < kRegWeights; ++k) {
+          if (k0 + k < K) p.w[(k0 + k) * p.slot_cap + pos] = wr[k][e];
+        }
       }
     }
     if (tid == 0) p.meta[u] = make_int2(mt.x, bl + alive_tail + s_pp[nops]);

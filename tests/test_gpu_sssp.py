@@ -499,6 +499,41 @@ def test_long_rows_with_many_operations(dg):
         _assert_same_rows(hg, og)
 
 
+@pytest.mark.parametrize("K,allow_parallel", [(3, True), (5, False), (8, True)])
+def test_rows_with_many_weight_columns(dg, K, allow_parallel):
+    """Graphs with more weight columns than a tail merge moves per pass (every merge mode: short
+    tails by a thread or a wave, long tails by a workgroup, whole rows): after every batch the rows
+    and all K columns equal the oracle's, and the tree of the last objective equals Dijkstra's."""
+    _say(f"[columns] K={K} parallel={allow_parallel}")
+    rng = np.random.default_rng(40 + K)
+    n = 6000
+    op, gp = _props_pair(dg, allow_parallel=allow_parallel, sorted_rows=not allow_parallel)
+    hub = rng.permutation(np.arange(2, n))[:3000]
+    src = np.concatenate([np.zeros(3000, np.int64), rng.integers(0, n, 24000)])
+    dst = np.concatenate([hub, rng.integers(0, n, 24000)])
+    w = rng.integers(1, 60, (len(src), K))
+    og = oracle.build_from_edges(n, src, dst, w, op)
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n, properties=gp)
+    _assert_same_rows(hg, og)
+    r = dg.sssp.compute(hg, 0, objective=K - 1)
+    for b in range(4):
+        ni = int(rng.integers(200, 3000))
+        nd = int(rng.integers(200, 3000))
+        # many operations on the hub's row (a long tail), the rest spread thin (short tails)
+        ins_s = np.concatenate([np.zeros(ni // 3, np.int64), rng.integers(0, n, ni - ni // 3)])
+        ins_d = rng.integers(1, n, ni)
+        ins_w = rng.integers(1, 60, (ni, K))
+        rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+        pick = rng.integers(0, og.m, nd)
+        del_s, del_d = rows[pick], og.col[pick]
+        og = oracle.apply_batch(og, ins_s, ins_d, ins_w, del_s, del_d, op).g
+        st = dg.sssp.update(hg, dg.EdgeBatch(insert=(ins_s, ins_d, ins_w), delete=(del_s, del_d)), r)
+        _assert_same_rows(hg, og)
+        d, p = oracle.dijkstra(og, 0, K - 1)
+        assert np.array_equal(r.distances, d) and np.array_equal(r.parents, p), f"batch {b}"
+        assert st.batch.num_vertices_after == og.n
+
+
 def test_grid_with_deletions_on_the_tree(dg):
     """The high-diameter configuration at a size the oracle runs in a moment: a 128 x 128 grid
     (4-neighbour arcs both ways, 0.1 % shortcuts, w ~ U[1, 1000], source at the corner) and
