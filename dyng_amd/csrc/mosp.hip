@@ -5,9 +5,9 @@
 // 172-236) by path doubling instead of a traversal from the source. The solve itself is the
 // persistent engine of sssp.hip on the combined graph.
 //
-// The combined graph is a solve-only graph: out-rows in one arena (row u at [off[u], off[u+1]),
-// its entries in no particular order: the engine's tree is canonical whatever the row order), no
-// in-rows (a solve from the source never pulls), no slack. Its buffers live in a per-device cache
+// The combined graph is a solve-only graph: out-rows in one arena (the edge list sorted by parent,
+// row u its run; the order within a row is that of the list, which the engine's canonical tree
+// does not depend on), no in-rows (a solve from the source never pulls), no slack. Its buffers live in a per-device cache
 // and are reused by the next call (they grow, never shrink, until the process ends).
 #include "internal.hpp"
 
@@ -52,74 +52,87 @@ DYNG_DEV bool combined_edge(const combine_params& c, int32_t v, int k, int32_t& 
   return true;
 }
 
-/// Add the lanes' amounts to counter[key] with one atomic per distinct key of the wave (the lanes
-/// of a hub parent share one); each active lane gets the counter's old value plus the amounts of
-/// the lanes below it with the same key. `sub` subtracts instead (the fill pass counts down).
-/// Every lane of the wave calls it.
-DYNG_DEV int wave_keyed_add(int32_t* counter, int32_t key, bool active, bool sub) {
-  int mine = 0;
-  u64 pending = __ballot(active);
-  while (pending) {
-    const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-    const int32_t lkey = __shfl(key, leader);
-    const bool in = active && key == lkey && ((pending >> lane()) & 1ULL);
-    const u64 group = __ballot(in);
-    const int count = __popcll(group);
-    int old = 0;
-    if (lane() == leader) old = sub ? atomicSub(&counter[lkey], count) : atomicAdd(&counter[lkey], count);
-    old = __shfl(old, leader);
-    if (in) mine = sub ? old - count + __popcll(group & lanes_below()) : old + __popcll(group & lanes_below());
-    pending &= ~group;
-  }
-  return mine;
-}
-
-/// Pass 1, one thread per vertex v: the out-degree of every parent in the combined graph, the
-/// edge count and the weight sum.
-__global__ void __launch_bounds__(kBlock) k_combine_count(combine_params c, int32_t* deg,
-                                                          unsigned long long* totals) {
+/// One thread per vertex v: its combined edges appended to a flat list (parent key, then
+/// (v, weight) as one word) with one atomic per workgroup (a workgroup scan of the threads' edge
+/// counts), plus the weight sum.
+__global__ void __launch_bounds__(kBlock) k_combine(combine_params c, uint32_t* keys, u64* vals,
+                                                    unsigned long long* totals) {
+  __shared__ int wave_total[kWaves];
+  __shared__ int block_base;
   __shared__ unsigned long long part[kWaves];
   const int32_t v = static_cast<int32_t>(blockIdx.x * kBlock + threadIdx.x);
   const bool live = v < c.n && v != c.source;
-  unsigned long long edges = 0, wsum = 0;
-  for (int k = 0; k < c.K; ++k) {
-    int32_t p = -1, x = 0;
-    const bool has = live && combined_edge(c, v, k, p, x);
-    if (has) {
-      ++edges;
-      wsum += static_cast<unsigned long long>(x);
+  int mine = 0;
+  unsigned long long wsum = 0;
+  if (live) {
+    for (int k = 0; k < c.K; ++k) {
+      int32_t p, x;
+      if (combined_edge(c, v, k, p, x)) {
+        ++mine;
+        wsum += static_cast<unsigned long long>(x);
+      }
     }
-    (void)wave_keyed_add(deg, p, has, false);
   }
-  edges = block_reduce(edges, part, op_sum{});
+  int incl = mine;  // inclusive scan of `mine` over the wave
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(incl, off);
+    if (lane() >= off) incl += y;
+  }
+  const int w_id = static_cast<int>(threadIdx.x) / 64;
+  if (lane() == 63) wave_total[w_id] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int i = 0; i < kWaves; ++i) {
+      const int t = wave_total[i];
+      wave_total[i] = run;
+      run += t;
+    }
+    block_base = run > 0 ? static_cast<int>(atomicAdd(&totals[0], static_cast<unsigned long long>(run))) : 0;
+  }
+  __syncthreads();
+  int slot = block_base + wave_total[w_id] + incl - mine;
+  if (mine > 0) {
+    for (int k = 0; k < c.K; ++k) {
+      int32_t p, x;
+      if (combined_edge(c, v, k, p, x)) {
+        keys[slot] = static_cast<uint32_t>(p);
+        vals[slot] = (static_cast<u64>(static_cast<uint32_t>(v)) << 32) | static_cast<uint32_t>(x);
+        ++slot;
+      }
+    }
+  }
   wsum = block_reduce(wsum, part, op_sum{});
-  if (threadIdx.x == 0 && edges) {
-    atomicAdd(&totals[0], edges);
-    atomicAdd(&totals[1], wsum);
-  }
+  if (threadIdx.x == 0 && wsum) atomicAdd(&totals[1], wsum);
 }
 
-/// Pass 2, one thread per vertex v: its combined edges into their parents' rows (slots counted
-/// down from each row's end), and v's own row bounds {offset, length}.
-__global__ void __launch_bounds__(kBlock) k_combine_fill(combine_params c, const int32_t* off, int32_t* left,
-                                                         int32_t* nbr, int32_t* w, int2* meta, int32_t* base_len) {
-  const int32_t v = static_cast<int32_t>(blockIdx.x * kBlock + threadIdx.x);
-  const bool live = v < c.n && v != c.source;
-  for (int k = 0; k < c.K; ++k) {
-    int32_t p = -1, x = 0;
-    const bool has = live && combined_edge(c, v, k, p, x);
-    const int slot = wave_keyed_add(left, p, has, true);
-    if (has) {
-      const int at = off[p] + slot;
-      nbr[at] = v;
-      w[at] = x;
-    }
+/// One thread per edge of the list sorted by parent: its (neighbour, weight) into the row arena
+/// at its index, and at the first edge of each parent's run that row's bounds {offset, length}
+/// (the run's end found by a galloping search). Rows of vertices without edges were cleared.
+__global__ void k_combined_rows(const uint32_t* keys, const u64* vals, int64_t m, int32_t* nbr, int32_t* w,
+                                int2* meta, int32_t* base_len) {
+  const int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x;
+  if (i >= m) return;
+  const u64 e = vals[i];
+  nbr[i] = static_cast<int32_t>(e >> 32);
+  w[i] = static_cast<int32_t>(static_cast<uint32_t>(e));
+  const uint32_t u = keys[i];
+  if (i > 0 && keys[i - 1] == u) return;
+  // the run of u is [i, end): gallop to a bracket, then bisect
+  int64_t lo = i, step = 1;  // keys[lo] == u
+  int64_t hi = i + 1;
+  while (hi < m && keys[hi] == u) {
+    lo = hi;
+    step <<= 1;
+    hi = lo + step < m ? lo + step : m;
   }
-  if (v < c.n) {
-    const int a = off[v], len = off[v + 1] - a;
-    meta[v] = make_int2(a, len);
-    base_len[v] = len;
+  while (hi - lo > 1) {  // keys[lo] == u, and hi == m or keys[hi] != u
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (keys[mid] == u) lo = mid; else hi = mid;
   }
+  const int len = static_cast<int>(hi - i);
+  meta[u] = make_int2(static_cast<int>(i), len);
+  base_len[u] = len;
 }
 
 template <typename T>
@@ -127,38 +140,58 @@ __global__ void k_fill_value(T* p, int64_t n, T v) {
   for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < n; i += int64_t{gridDim.x} * blockDim.x) p[i] = v;
 }
 
+/// The slot of the first alive entry with neighbour `x` in a row of two sorted runs ([0, bl) then
+/// [bl, len)), or -1.
+DYNG_DEV int first_alive(const int32_t* row, int bl, int len, int32_t x) {
+  for (int r = 0; r < 2; ++r) {
+    const int lo = r == 0 ? 0 : bl, hi = r == 0 ? bl : len;
+    for (int j = row_lower_bound(row, lo, hi, x); j < hi && (row[j] & 0x7fffffff) == x; ++j) {
+      if (row[j] >= 0) return j;
+    }
+  }
+  return -1;
+}
+
 /// Path costs, first step: every vertex's own tree edge (p, v) as the first alive (p, v) of p's
 /// out-row (the base run, then the tail: MOSP's row order), its first `columns` weights; the
-/// source starts at 0 and an unreachable vertex at INF, both with no ancestor. A tree edge that
-/// is not an edge of the graph is reported (the lowest such vertex). active[0] is set when some
-/// vertex has an ancestor.
-__global__ void k_tree_edge_costs(const int2* meta, const int32_t* base, const int32_t* nbr, const int32_t* w,
-                                  int64_t slot_cap, int columns, const int32_t* parents, int32_t n, int32_t source,
-                                  int64_t* acc, int32_t* jump, int* missing, int* active) {
+/// source starts at 0 and an unreachable vertex at INF, both with no ancestor. Without parallel
+/// edges the pair (p, v) is one edge, found in v's in-row instead (short and read in vertex
+/// order, where p's out-row may be a hub's). A tree edge that is not an edge of the graph is
+/// reported (the lowest such vertex). active[0] is set when some vertex has an ancestor.
+struct edge_rows {
+  const int2* meta;
+  const int32_t* base;
+  const int32_t* nbr;
+  const int32_t* w;
+  int64_t slot_cap;
+};
+
+__global__ void k_tree_edge_costs(edge_rows out, edge_rows in, bool by_in_row, int columns, const int32_t* parents,
+                                  int32_t n, int32_t source, int64_t* acc, int32_t* jump, int* missing, int* active) {
   const int32_t v = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
   int32_t link = -1;
   if (v < n) {
     const int32_t p = v == source ? -1 : parents[v];
     int64_t* a = acc + static_cast<int64_t>(v) * columns;
-    int hit = -1;
-    int2 mt = make_int2(0, 0);
+    int64_t slot = -1;
+    const int32_t* w = nullptr;
+    int64_t cap = 0;
     if (p >= 0) {
-      mt = meta[p];
-      const int32_t* row = nbr + mt.x;
-      const int bl = base[p];
-      for (int r = 0; r < 2 && hit < 0; ++r) {
-        const int lo = r == 0 ? 0 : bl, hi = r == 0 ? bl : mt.y;
-        for (int x = row_lower_bound(row, lo, hi, v); x < hi && (row[x] & 0x7fffffff) == v; ++x) {
-          if (row[x] >= 0) {
-            hit = x;
-            break;
-          }
-        }
+      const edge_rows& er = by_in_row ? in : out;
+      const int32_t row_of = by_in_row ? v : p;
+      const int32_t want = by_in_row ? p : v;
+      const int2 mt = er.meta[row_of];
+      const int hit = first_alive(er.nbr + mt.x, er.base[row_of], mt.y, want);
+      if (hit >= 0) {
+        slot = mt.x + int64_t{hit};
+        w = er.w;
+        cap = er.slot_cap;
+      } else {
+        atomicMin(missing, v);
       }
-      if (hit < 0) atomicMin(missing, v);
     }
-    if (hit >= 0) {
-      for (int c = 0; c < columns; ++c) a[c] = w[c * slot_cap + mt.x + hit];
+    if (slot >= 0) {
+      for (int c = 0; c < columns; ++c) a[c] = w[c * cap + slot];
       link = p;
     } else {
       const int64_t start = v == source ? 0 : sssp_infinity;
@@ -211,8 +244,9 @@ unsigned int grid(int64_t work, int block) {
 /// The solve-only combined graph and the scratch of its build, one per device.
 struct combined_cache {
   std::unique_ptr<dyng_hip_graph> g;
-  dbuf<int32_t> deg, off;
-  dbuf<unsigned char> scan_tmp;
+  dbuf<uint32_t> keys_a, keys_b;
+  dbuf<u64> vals_a, vals_b;
+  dbuf<unsigned char> sort_tmp;
   dbuf<const int32_t*> parents;
   dbuf<int32_t> terms;
   dbuf<unsigned long long> totals;
@@ -241,25 +275,19 @@ dyng_hip_graph* mosp_build_combined(int device, const int32_t* const* parents, i
   cc.parents.reserve(num_objectives);
   cc.terms.reserve(num_objectives);
   cc.totals.reserve(2);
-  cc.deg.reserve(n + 1);
-  cc.off.reserve(n + 1);
-  // small synchronous copies from the caller's host arrays (queued work on `s` may still read
-  // the previous call's values, so wait for it first)
-  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  const int64_t cap = std::max<int64_t>(1, n * num_objectives);
+  cc.keys_a.reserve(cap);
+  cc.keys_b.reserve(cap);
+  cc.vals_a.reserve(cap);
+  cc.vals_b.reserve(cap);
+  // small synchronous copies from the caller's host arrays (the previous call's work on the cache
+  // ended before it returned)
   DYNG_HIP_TRY(hipMemcpy(cc.parents.p, parents, num_objectives * sizeof(const int32_t*), hipMemcpyHostToDevice));
   DYNG_HIP_TRY(hipMemcpy(cc.terms.p, terms, num_objectives * sizeof(int32_t), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_fill_value<unsigned long long>, dim3(1), dim3(64), 0, s, cc.totals.p, int64_t{2}, 0ULL);
-  hipLaunchKernelGGL(k_fill_value<int32_t>, dim3(std::min(grid(n + 1, 256), 4096u)), dim3(256), 0, s, cc.deg.p,
-                     n + 1, 0);
   combine_params c{cc.parents.p, cc.terms.p, num_objectives, static_cast<int32_t>(n), source, base};
-  hipLaunchKernelGGL(k_combine_count, dim3(grid(n, kBlock)), dim3(kBlock), 0, s, c, cc.deg.p, cc.totals.p);
+  hipLaunchKernelGGL(k_combine, dim3(grid(n, kBlock)), dim3(kBlock), 0, s, c, cc.keys_a.p, cc.vals_a.p, cc.totals.p);
   DYNG_HIP_TRY(hipGetLastError());
-  size_t bytes = 0;
-  DYNG_HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, cc.deg.p, cc.off.p, 0, static_cast<size_t>(n + 1),
-                                       rocprim::plus<int32_t>(), s));
-  cc.scan_tmp.reserve(bytes);
-  DYNG_HIP_TRY(rocprim::exclusive_scan(static_cast<void*>(cc.scan_tmp.p), bytes, cc.deg.p, cc.off.p, 0,
-                                       static_cast<size_t>(n + 1), rocprim::plus<int32_t>(), s));
   unsigned long long host[2] = {0, 0};
   DYNG_HIP_TRY(hipMemcpyAsync(host, cc.totals.p, sizeof(host), hipMemcpyDeviceToHost, s));
   DYNG_HIP_TRY(hipStreamSynchronize(s));
@@ -267,8 +295,19 @@ dyng_hip_graph* mosp_build_combined(int device, const int32_t* const* parents, i
   if (m >= INT32_MAX) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the combined graph has 2^31 or more edges");
   *edges = m;
   *weight_sum = static_cast<int64_t>(host[1]);
-  // the graph: out-rows in one arena of exactly m slots, rows in place of their offsets; the
-  // degree array counts down to zero as the fill pass claims slots
+  // group the edges by parent (a stable radix sort over the bits of a vertex id)
+  int bits = 1;
+  while ((int64_t{1} << bits) < n) ++bits;
+  if (m > 0) {
+    size_t bytes = 0;
+    DYNG_HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, cc.keys_a.p, cc.keys_b.p, cc.vals_a.p, cc.vals_b.p,
+                                           static_cast<size_t>(m), 0, bits, s));
+    cc.sort_tmp.reserve(bytes);
+    DYNG_HIP_TRY(rocprim::radix_sort_pairs(static_cast<void*>(cc.sort_tmp.p), bytes, cc.keys_a.p, cc.keys_b.p,
+                                           cc.vals_a.p, cc.vals_b.p, static_cast<size_t>(m), 0, bits, s));
+  }
+  // the graph: out-rows in one arena of exactly m slots, each row where its run lies in the
+  // sorted list
   g->device = device;
   g->props = dyng_hip_graph_props{1, 1, 0, 0, 0, 0, 1, 1, 0.0};
   g->K = 1;
@@ -299,8 +338,14 @@ dyng_hip_graph* mosp_build_combined(int device, const int32_t* const* parents, i
   g->last.valid = false;
   g->num_del_slots = 0;
   g->num_ins_slots = 0;
-  hipLaunchKernelGGL(k_combine_fill, dim3(grid(n, kBlock)), dim3(kBlock), 0, s, c, cc.off.p, cc.deg.p, g->out.nbr.p,
-                     g->out.w.p, g->out.meta.p, g->out.base.p);
+  hipLaunchKernelGGL(k_fill_value<int2>, dim3(std::min(grid(n, 256), 4096u)), dim3(256), 0, s, g->out.meta.p, n,
+                     make_int2(0, 0));
+  hipLaunchKernelGGL(k_fill_value<int32_t>, dim3(std::min(grid(n, 256), 4096u)), dim3(256), 0, s, g->out.base.p, n,
+                     0);
+  if (m > 0) {
+    hipLaunchKernelGGL(k_combined_rows, dim3(grid(m, 256)), dim3(256), 0, s, cc.keys_b.p, cc.vals_b.p, m,
+                       g->out.nbr.p, g->out.w.p, g->out.meta.p, g->out.base.p);
+  }
   DYNG_HIP_TRY(hipGetLastError());
   return g;
 }
@@ -328,9 +373,11 @@ void mosp_path_costs(const dyng_hip_graph* g, const int32_t* parents, int64_t n,
   // the first step writes the caller's array; the rounds then alternate between it and `acc`
   int64_t* bufs[2] = {costs, cc.acc.p};
   int32_t* jumps[2] = {cc.jump_a.p, cc.jump_b.p};
-  hipLaunchKernelGGL(k_tree_edge_costs, dim3(grid(n, 256)), dim3(256), 0, s, g->out.meta.p, g->out.base.p, g->out.nbr.p,
-                     g->out.w.p, g->out.slot_cap, columns, parents, static_cast<int32_t>(n), source, bufs[0], jumps[0],
-                     missing, active);
+  const edge_rows out{g->out.meta.p, g->out.base.p, g->out.nbr.p, g->out.w.p, g->out.slot_cap};
+  const edge_rows in{g->in.meta.p, g->in.base.p, g->in.nbr.p, g->in.w.p, g->in.slot_cap};
+  const bool by_in_row = g->props.allow_parallel == 0;
+  hipLaunchKernelGGL(k_tree_edge_costs, dim3(grid(n, 256)), dim3(256), 0, s, out, in, by_in_row, columns, parents,
+                     static_cast<int32_t>(n), source, bufs[0], jumps[0], missing, active);
   int cur = 0;
   for (int r = 0; r < rounds; ++r) {
     hipLaunchKernelGGL(k_path_double, dim3(grid(n, 256)), dim3(256), 0, s, bufs[cur], jumps[cur], bufs[cur ^ 1],

@@ -140,22 +140,26 @@ def _random_graph(rng, n, m, K, wmax):
     return src, dst, w
 
 
-@pytest.mark.parametrize("seed,n,m,K,prefs", [
-    (1, 300, 1500, 3, [4, 1, 4]),
-    (2, 1000, 6000, 2, [3, 5]),
-    (3, 2000, 9000, 4, []),
-    (4, 500, 1500, 3, [1, 2, 7]),
+@pytest.mark.parametrize("seed,n,m,K,prefs,simple", [
+    (1, 300, 1500, 3, [4, 1, 4], False),
+    (2, 1000, 6000, 2, [3, 5], False),
+    (3, 2000, 9000, 4, [], False),
+    (4, 500, 1500, 3, [1, 2, 7], False),
+    (5, 1500, 9000, 3, [4, 1, 4], True),
+    (6, 800, 3000, 5, [1, 2, 3, 4, 5], True),
 ])
-def test_random_updates_match_the_oracle(dg, seed, n, m, K, prefs):
-    """Random multigraphs (parallel edges, self-loops, unreachable parts), several batches with
+def test_random_updates_match_the_oracle(dg, seed, n, m, K, prefs, simple):
+    """Random multigraphs (parallel edges, self-loops, unreachable parts) or simple graphs with
+    sorted rows (whose tree edges the costs look up in the in-rows), several batches with
     insertions, deletions, weight changes and vertex growth: after every update the combined
     arrays, the costs and the counters equal the oracle's."""
-    _say(f"[mosp random] seed={seed} n={n} m={m} K={K} prefs={prefs}")
+    _say(f"[mosp random] seed={seed} n={n} m={m} K={K} prefs={prefs} simple={simple}")
     rng = np.random.default_rng(seed)
     src, dst, w = _random_graph(rng, n, m, K, 20)
-    op = oracle.Props.mosp_compatible()
+    op = oracle.Props() if simple else oracle.Props.mosp_compatible()
+    gp = dg.GraphProperties() if simple else dg.GraphProperties.mosp_compatible()
     og = oracle.build_from_edges(n, src, dst, w, op)
-    g = dg.Graph.from_edges(src, dst, w, num_vertices=n, properties=dg.GraphProperties.mosp_compatible())
+    g = dg.Graph.from_edges(src, dst, w, num_vertices=n, properties=gp)
     r = dg.mosp.compute(g, 0, preferences=prefs)
     prev_d, prev_p, _ = omosp.solve([oracle.dijkstra(og, 0, k)[1] for k in range(K)], 0, prefs)
     assert np.array_equal(r.combined_distances, prev_d) and np.array_equal(r.combined_parents, prev_p)
