@@ -114,7 +114,7 @@ EXPORTS = (
     "dyng_hip_api_repair", "dyng_hip_batch_features", "dyng_hip_mlp_create", "dyng_hip_mlp_destroy",
     "dyng_hip_mlp_get_params", "dyng_hip_mlp_set_params", "dyng_hip_mlp_gradients", "dyng_hip_mlp_train",
     "dyng_hip_mlp_predict", "dyng_hip_result_equal", "dyng_hip_api_update_scheduled",
-    "dyng_hip_mosp_combined",
+    "dyng_hip_mosp_combined", "dyng_hip_api_update_each",
 )
 
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_TRAINED, MLP_PARAMS = 12, 64, 5, 5317, 5341
@@ -166,6 +166,7 @@ def lib() -> C.CDLL:
     L.dyng_hip_api_update.argtypes = [vp, C.POINTER(Batch), vp, vp, C.POINTER(Stats)]
     L.dyng_hip_api_compute_multi.argtypes = [vp, vp, C.c_int64, C.POINTER(Options), vp, C.POINTER(vp)]
     L.dyng_hip_api_repair.argtypes = [vp, vp, vp, C.POINTER(Stats)]
+    L.dyng_hip_api_update_each.argtypes = [vp, C.POINTER(Batch), vp, C.c_int64, vp, vp]
     L.dyng_hip_mosp_combined.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, C.c_int32, vp, vp, C.c_int32, vp, vp,
                                          C.POINTER(MospStats)]
     L.dyng_hip_batch_features.argtypes = [vp, C.POINTER(Batch), vp, C.c_int64, vp, C.POINTER(BatchStats)]

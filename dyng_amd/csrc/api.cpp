@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <exception>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -467,6 +468,101 @@ extern "C" int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batc
       stats->apply_ms = ev_apply.ms();
       stats->engine_ms = engine_ms;
     }
+  });
+}
+
+extern "C" int dyng_hip_api_update_each(dyng_hip_graph* g, const dyng_hip_batch* batch,
+                                        dyng_hip_result* const* results, int64_t count, void* stream,
+                                        dyng_hip_stats* stats) {
+  return guarded([&] {
+    // ---- every result's checks and preparation on G_t before the commit (update.hpp:263-364:
+    // strong guarantee for what is found here) ----
+    if (g == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng::update_each: null graph");
+    if (results == nullptr || count < 1) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "dyng::update_each: the list of results is empty (applying a batch "
+                                      "without updating any result would leave every result of the graph "
+                                      "stale; use the graph's apply)");
+    }
+    std::vector<const dyng_hip_result*> seen;
+    seen.reserve(static_cast<size_t>(count));
+    for (int64_t i = 0; i < count; ++i) {
+      const dyng_hip_result* r = results[i];
+      if (r == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng::update_each: result " + std::to_string(i) + " is null");
+      if (std::find(seen.begin(), seen.end(), r) != seen.end()) {
+        fail(DYNG_HIP_INVALID_ARGUMENT, "dyng::update_each: result " + std::to_string(i) + " is listed twice");
+      }
+      seen.push_back(r);
+      expect_usable(r, "dyng::update_each");
+      if (r->graph_state != g->state_id || r->version != g->version) {
+        throw status_error(DYNG_HIP_STALE_RESULT,
+                           "dyng: dyng::update_each: result " + std::to_string(i) + " matches graph version " +
+                               std::to_string(r->version) + " but the graph is at version " +
+                               std::to_string(g->version) + " (or it belongs to another graph); recompute it");
+      }
+      if (r->device != g->device) {
+        fail(DYNG_HIP_INVALID_ARGUMENT, "dyng::update_each: result " + std::to_string(i) + " is on another device");
+      }
+      expect_graph(g, r->opt, "dyng::update_each");
+      if (r->n != g->n) {
+        fail(DYNG_HIP_INVALID_ARGUMENT, "dyng::update_each: result " + std::to_string(i) + " has " +
+                                            std::to_string(r->n) + " vertices but the graph has " +
+                                            std::to_string(g->n));
+      }
+    }
+    scoped_device guard(g->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    batch_scan sc;
+    scan_batch(g, batch, s, sc);
+    std::vector<int64_t> max_weight(static_cast<size_t>(count)), delta(static_cast<size_t>(count));
+    for (int64_t i = 0; i < count; ++i) {
+      const dyng_hip_result* r = results[i];
+      const int k = r->opt.objective;
+      if (batch->num_ins > 0 && sc.wmin[k] < 1) {
+        fail(DYNG_HIP_INVALID_ARGUMENT, "dyng::update_each: an insertion has a weight below 1 in objective " +
+                                            std::to_string(k) + "; sssp needs weights in [1, 2^31 - 1]");
+      }
+      max_weight[i] = std::max<int64_t>(std::max<int64_t>(sc.wmax[k], 1), g->wmax[k]);
+      delta[i] = r->opt.delta > 0 ? r->opt.delta : default_delta(std::max<int64_t>(g->m, 1), g->n, g->wsum[k]);
+      if (!distances_fit(sc.n_after, max_weight[i])) {
+        fail(DYNG_HIP_INVALID_ARGUMENT, "dyng::update_each: distances up to " + std::to_string(max_weight[i]) +
+                                            " * " + std::to_string(sc.n_after - 1) + " do not fit in 62 bits");
+      }
+    }
+    // ---- commit: the batch applied once ----
+    event_pair ev_apply;
+    dyng_hip_apply_summary summary{};
+    DYNG_HIP_TRY(hipEventRecord(ev_apply.a, s));
+    try {
+      apply_scanned(g, sc, s, &summary);
+    } catch (...) {
+      for (int64_t i = 0; i < count; ++i) results[i]->poisoned = true;  // the graph may be partly updated
+      throw;
+    }
+    DYNG_HIP_TRY(hipEventRecord(ev_apply.b, s));
+    // ---- every result repaired on G_{t+1}; a failing one is poisoned, the others still brought
+    // up to date, and the first error reported ----
+    std::exception_ptr first;
+    for (int64_t i = 0; i < count; ++i) {
+      dyng_hip_counters c{};
+      double engine_ms = 0.0;
+      try {
+        repair_core(g, results[i], delta[i], max_weight[i], s, c, engine_ms);
+      } catch (...) {
+        if (!first) first = std::current_exception();
+        continue;
+      }
+      if (stats != nullptr) {
+        std::memset(&stats[i], 0, sizeof(stats[i]));
+        fill_stats(&stats[i], c);
+        stats[i].batch = summary;
+        stats[i].engine_ms = engine_ms;
+      }
+    }
+    if (stats != nullptr) {
+      DYNG_HIP_TRY(hipEventSynchronize(ev_apply.b));
+      stats[0].apply_ms = ev_apply.ms();
+    }
+    if (first) std::rethrow_exception(first);
   });
 }
 

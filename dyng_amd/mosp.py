@@ -31,7 +31,7 @@ import numpy as np
 from . import _lib, sssp as _sssp
 from .batch import EdgeBatch
 from .device import DeviceArray
-from .errors import InvalidArgumentError, NotSupportedError, StaleResultError
+from .errors import InvalidArgumentError, NotSupportedError
 from .graph import ApplySummary, Graph
 from .resources import Resources, resolve
 
@@ -366,23 +366,9 @@ def compute(graph: Graph, source: int, *, options: Options | None = None, resour
     return out
 
 
-def update(graph: Graph, batch: EdgeBatch, result: Result, *, resources: Resources | None = None) -> Stats:
-    """Apply ``batch`` to ``graph`` once, repair the K trees and rebuild the combined graph, the
-    MOSP tree and the path costs (mosp::update, python/dyng/mosp.py:458-509). Afterwards
-    ``result`` equals ``compute(graph, result.source, options=result.options)``."""
-    _check_graph(graph)
-    if not isinstance(result, Result):
-        raise TypeError("dyng_amd.mosp.update: result must be a dyng_amd.mosp.Result")
-    if not isinstance(batch, EdgeBatch):
-        raise TypeError("dyng_amd.mosp.update: batch must be a dyng_amd.EdgeBatch")
-    version = graph.version
-    for k, t in enumerate(result._trees):
-        if t.graph_version != version:
-            raise StaleResultError(f"dyng_amd.mosp.update: objective {k}'s tree matches graph version "
-                                   f"{t.graph_version}, the graph is at {version}")
-    res = resolve(resources, graph.resources)
-    summary = graph.apply(batch)
-    per = tuple(_sssp.repair(graph, t, resources=res) for t in result._trees)
+def _finish_update(graph: Graph, result: Result, per: tuple) -> Stats:
+    """After the K trees were repaired on the new graph: the combined graph, the MOSP tree and
+    the costs, and the update's stats."""
     st = _combine(graph, result, count_changes=True)
     solve = st.solve
     return Stats(affected=int(st.affected),
@@ -391,5 +377,22 @@ def update(graph: Graph, batch: EdgeBatch, result: Result, *, resources: Resourc
                  fallback_used=any(s.fallback_used for s in per),
                  converged=all(s.converged for s in per),
                  engine_used=per[0].engine_used if per else "fused",
-                 batch=summary, objectives=per, combined_edges=int(st.combined_edges),
+                 batch=per[0].batch, objectives=per, combined_edges=int(st.combined_edges),
                  preference_scale=int(st.preference_scale))
+
+
+def update(graph: Graph, batch: EdgeBatch, result: Result, *, resources: Resources | None = None) -> Stats:
+    """Apply ``batch`` to ``graph`` once, repair the K trees and rebuild the combined graph, the
+    MOSP tree and the path costs (mosp::update, python/dyng/mosp.py:458-509). Afterwards
+    ``result`` equals ``compute(graph, result.source, options=result.options)``. The batch is
+    applied and the trees repaired in one call (dyng::update_each); a stale result changes
+    nothing."""
+    from ._update import update_trees
+
+    _check_graph(graph)
+    if not isinstance(result, Result):
+        raise TypeError("dyng_amd.mosp.update: result must be a dyng_amd.mosp.Result")
+    if not isinstance(batch, EdgeBatch):
+        raise TypeError("dyng_amd.mosp.update: batch must be a dyng_amd.EdgeBatch")
+    per = tuple(update_trees(graph, batch, result._trees, resources))
+    return _finish_update(graph, result, per)

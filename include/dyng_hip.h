@@ -285,6 +285,14 @@ int dyng_hip_api_repair(dyng_hip_graph* g, dyng_hip_result* r, void* stream, dyn
 /* native.sssp_update (bindings/sssp.cpp:98-110 -> sssp::update, sssp.hpp:484-489). */
 int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batch, dyng_hip_result* r,
                         void* stream, dyng_hip_stats* stats);
+/* dyng::update_each (cpp/include/dyng/update.hpp:311-364; Python dyng.update,
+ * python/dyng/_update.py): the batch applied once and `count` results of the graph (distinct,
+ * each current; any objectives and sources) repaired on the new graph, one stats entry each
+ * (the apply time in the first). Every check runs before the batch is applied (nothing changes
+ * on an error found there); after the commit a result whose repair fails is poisoned, the others
+ * are still brought up to date, and the first error is returned. */
+int dyng_hip_api_update_each(dyng_hip_graph* g, const dyng_hip_batch* batch, dyng_hip_result* const* results,
+                             int64_t count, void* stream, dyng_hip_stats* stats);
 /* native.sssp_from_arrays (bindings/sssp.cpp:112-135 -> result::from_arrays,
  * sssp.cpp:1118-1198): host arrays, imported, canonicalized and validated on the host. */
 int dyng_hip_api_from_arrays(dyng_hip_graph* g, int32_t source, const int64_t* distances,

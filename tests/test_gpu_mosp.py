@@ -217,3 +217,75 @@ def test_options_and_errors(dg):
     with pytest.raises(dg.StaleResultError):
         dg.mosp.update(g2, dg.EdgeBatch(insert=([1], [0], [[1, 1]])), c)
     assert np.array_equal(c.combined_parents, r.combined_parents)
+
+
+# ------------------------------------------------------------------------------------------------
+# dyng_amd.update: one batch, several results (cpp/include/dyng/update.hpp:263-364)
+# ------------------------------------------------------------------------------------------------
+
+
+def test_update_several_results_equals_each_computed_alone(dg):
+    """Trees of different objectives and sources plus a mosp result, one batch applied once: each
+    equals what compute gives on the new graph, and the graph's version grows by one."""
+    _say("[update_each] mixed results")
+    rng = np.random.default_rng(77)
+    n, m, K = 3000, 20000, 3
+    src, dst, w = _random_graph(rng, n, m, K, 30)
+    op = oracle.Props()
+    og = oracle.build_from_edges(n, src, dst, w, op)
+    g = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    t0 = dg.sssp.compute(g, 0, objective=0)
+    t1 = dg.sssp.compute(g, 17, objective=2)
+    t2 = dg.sssp.compute(g, 5, objective=1, delta=7)
+    mr = dg.mosp.compute(g, 0, preferences=[2, 3, 1])
+    for b in range(3):
+        ni, nd = 400, 300
+        ins_s, ins_d = rng.integers(0, n, ni), rng.integers(0, n, ni)
+        ins_w = rng.integers(1, 31, (ni, K))
+        rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+        pick = rng.integers(0, og.m, nd)
+        del_s, del_d = rows[pick], og.col[pick]
+        version = g.version
+        stats = dg.update(g, dg.EdgeBatch(insert=(ins_s, ins_d, ins_w), delete=(del_s, del_d)), t0, t1, mr, t2)
+        assert g.version == version + 1
+        assert len(stats) == 4 and isinstance(stats[2], dg.mosp.Stats) and isinstance(stats[0], dg.sssp.Stats)
+        og = oracle.apply_batch(og, ins_s, ins_d, ins_w, del_s, del_d, op).g
+        for t, (s, k) in zip((t0, t1, t2), ((0, 0), (17, 2), (5, 1))):
+            d, p = oracle.dijkstra(og, s, k)
+            assert np.array_equal(t.distances, d) and np.array_equal(t.parents, p), (b, s, k)
+            assert t.graph_version == g.version
+        trees = [oracle.dijkstra(og, 0, k)[1] for k in range(K)]
+        d, p, mc = omosp.solve(trees, 0, [2, 3, 1])
+        assert np.array_equal(mr.combined_distances, d) and np.array_equal(mr.combined_parents, p), b
+        assert np.array_equal(mr.path_costs, omosp.path_costs(og, p, 0, K)), b
+        assert stats[2].combined_edges == mc
+        for s in stats[:2] + stats[3:]:
+            assert s.batch.inserted_edges + s.batch.updated_edges == stats[2].batch.inserted_edges + \
+                stats[2].batch.updated_edges
+
+
+def test_update_checks_everything_before_the_batch(dg):
+    """No result, a result twice, a stale result, an insertion weight below 1: an error, and the
+    graph and every result unchanged."""
+    g = dg.Graph.from_edges([0, 1, 2], [1, 2, 0], np.array([[1, 2], [1, 2], [1, 2]]))
+    a = dg.sssp.compute(g, 0)
+    b = dg.sssp.compute(g, 1, objective=1)
+    batch = dg.EdgeBatch(delete=([2], [0]))
+    with pytest.raises(dg.InvalidArgumentError):
+        dg.update(g, batch)
+    with pytest.raises(dg.InvalidArgumentError):
+        dg.update(g, batch, a, b, a)
+    stale = dg.sssp.compute(g, 0)
+    dg.sssp.update(g, dg.EdgeBatch(insert=([0], [2], [[5, 5]])), stale)  # g moves on; a and b go stale
+    assert g.version == 1
+    with pytest.raises(dg.StaleResultError):
+        dg.update(g, batch, stale, a)
+    assert g.version == 1 and g.num_edges == 4
+    a = dg.sssp.compute(g, 0)
+    b = dg.sssp.compute(g, 1, objective=1)
+    with pytest.raises(dg.InvalidArgumentError):
+        dg.update(g, dg.EdgeBatch(insert=([1], [0], [[3, 0]])), a, b)  # objective 1 weight 0
+    assert g.version == 1 and g.num_edges == 4
+    st = dg.update(g, batch, a, b)
+    assert g.version == 2 and g.num_edges == 3 and len(st) == 2
+    assert a.distances.tolist() == [0, 1, 2] and b.distances.tolist()[1] == 0
