@@ -50,7 +50,13 @@ WORKLOADS = {
     # of 1 % of its edges (half insertions of new RMAT edges, half deletions) and repairs all 256
     # trees in one persistent launch replayed from a hipGraph, with autotuned schedule knobs
     "batched256_rmat": ("batched", 256, None, 100, 3, 6, (0, 0, 0)),
+    # MOSP on RMAT-22 with three weight columns (column 0 as rmat22_mixed100k draws it, columns 1-2
+    # U[1, 255] from their own seed), preferences (4, 1, 4): each step is one mosp.update (the
+    # batch applied once, three trees repaired, the combined graph, MOSP tree and path costs rebuilt)
+    "mosp_rmat22_k3": ("mosp", 22, 67_108_864, 255, 5, 20, (40_000, 40_000, 20_000)),
 }
+MOSP_OBJECTIVES = 3
+MOSP_PREFERENCES = [4, 1, 4]
 
 
 def log(*a):
@@ -388,6 +394,163 @@ def batched_bench(args) -> int:
     return 0 if (exact is None or exact["bit_exact"]) else 3
 
 
+def fused_solve_bytes(c, n: int) -> int:
+    """Algorithmic HBM bytes of a from-scratch launch with count_changes (the combined solve):
+    8 B per vertex to initialise the packed words, the relax bytes, 8 B per vertex to read the
+    words back and 24 B per vertex to compare and write (distance, parent)."""
+    return 40 * n + relax_bytes(c)
+
+
+def mosp_inputs(name: str, batches: int):
+    """The MOSP workload: an RMAT graph with MOSP_OBJECTIVES weight columns and its batches (the
+    extra columns of graph and insertions drawn from seeds of their own)."""
+    from dyng_amd import generators as G
+
+    kind, size, m, wmax, cfg, nb, (ni, nd, nr) = WORKLOADS[name]
+    nb = max(nb, batches)
+    n = 1 << size
+    src, dst, w0 = G.rmat(size, m, wmax, G.GRAPH_SEED ^ cfg)
+    rng = np.random.default_rng(G.GRAPH_SEED ^ cfg ^ 0x3C)
+    w = np.ascontiguousarray(np.stack([w0] + [rng.integers(1, wmax + 1, m, dtype=np.int32)
+                                              for _ in range(MOSP_OBJECTIVES - 1)], 1))
+    drawn = G.mixed_batches(src, dst, n, nb, ni, nd, nr, wmax, G.BATCH_SEED ^ cfg, rmat_scale=size)
+    rng = np.random.default_rng(G.BATCH_SEED ^ cfg ^ 0x3C)
+    out = []
+    for x in drawn:
+        iw = np.ascontiguousarray(np.stack([x[2]] + [rng.integers(1, wmax + 1, len(x[2]), dtype=np.int32)
+                                                     for _ in range(MOSP_OBJECTIVES - 1)], 1))
+        out.append((x[0], x[1], iw, x[3], x[4]))
+    return n, src, dst, w, out, G.max_out_degree_vertex(src, n), ni + nd + nr
+
+
+def mosp_cpu_baseline(n, src, dst, w, drawn, source, sample: int, threads: int):
+    """The oracle on `sample` batches: the batch applied once and the trees repaired by the OpenMP
+    engine (update_each), the combined graph, its tree (OpenMP engine) and the path costs."""
+    import oracle
+    from oracle import mosp as omosp
+
+    row = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(src, minlength=n), out=row[1:])
+    g = oracle.Csr(row, dst.astype(np.int32), np.ascontiguousarray(w.T.astype(np.int32)))
+    K = w.shape[1]
+    ws = [oracle.Workspace() for _ in range(K)]
+    trees = [oracle.compute(g, source, k, "omp", threads=threads, ws=ws[k])[0] for k in range(K)]
+    parts, results = [], []
+    for b in range(sample):
+        x = drawn[b]
+        t0 = time.perf_counter()
+        g, _, cs = oracle.update_many(g, trees, x[0], x[1], x[2], x[3], x[4], engine="omp", threads=threads,
+                                      workspaces=ws)
+        t1 = time.perf_counter()
+        cg, _, _ = omosp.combine_fast([t.parent for t in trees], source, MOSP_PREFERENCES)
+        t2 = time.perf_counter()
+        ct, _ = oracle.compute(cg, source, 0, "omp", threads=threads)
+        t3 = time.perf_counter()
+        costs = omosp.path_costs_fast(g, ct.parent, source, K)
+        t4 = time.perf_counter()
+        repair = sum(sum(c.t_hooks) for c in cs)
+        parts.append({"update_each_s": t1 - t0, "repairs_s": repair, "combine_s": t2 - t1, "solve_s": t3 - t2,
+                      "costs_s": t4 - t3})
+        results.append((ct.dist.copy(), ct.parent.copy(), costs))
+        log(f"[bench] cpu mosp batch {b}: update_each {t1 - t0:.2f}s (repairs {repair * 1e3:.1f} ms), combine "
+            f"{t2 - t1:.2f}s, solve {t3 - t2:.2f}s, costs {t4 - t3:.2f}s")
+    return parts, results
+
+
+def mosp_bench(args) -> int:
+    """The MOSP configuration: mosp.update on RMAT-22 with three objectives."""
+    import dyng_amd
+    from dyng_amd import mosp
+
+    K, W = args.steps, args.warmup
+    t0 = time.perf_counter()
+    n, src, dst, w, drawn, source, ops = mosp_inputs(args.workload, K + W)
+    log(f"[bench] inputs {args.workload}: n={n} m={len(src)} K={w.shape[1]} batches={len(drawn)}x{ops} in "
+        f"{time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    g = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
+    r = mosp.compute(g, source, preferences=MOSP_PREFERENCES)
+    dyng_amd.synchronize()
+    log(f"[bench] device graph + mosp.compute in {time.perf_counter() - t0:.1f}s")
+    D = dyng_amd.DeviceArray
+    dev = [dyng_amd.EdgeBatch(insert=(D(x[0]), D(x[1]), D(x[2])), delete=(D(x[3]), D(x[4])))
+           for x in drawn[:K + W]]
+    wall, stats = [], []
+    for b in range(W + K):
+        dyng_amd.synchronize()
+        t1 = time.perf_counter()
+        st = mosp.update(g, dev[b], r)
+        dt = (time.perf_counter() - t1) * 1e3
+        if b >= W:
+            wall.append(dt)
+            stats.append(st)
+    dyng_amd.synchronize()
+    p50 = statistics.median(wall)
+    value = ops / (p50 / 1e3)
+    log(f"[bench] gpu: p50 {p50:.3f} ms/update, {value:.3e} edges/s")
+    # the persistent kernel: three repair launches and the combined solve per step
+    kbytes, kms = [], []
+    for st in stats:
+        for o in st.objectives:
+            kbytes.append(fused_kernel_bytes(o.device, n, o.batch.deleted_edges + o.batch.ignored_deletions + ops,
+                                             ops))
+            kms.append(o.device.kernel_ms)
+        kbytes.append(fused_solve_bytes(st.combined_solve, n))
+        kms.append(st.combined_solve.kernel_ms)
+    achieved = (sum(kbytes) / len(kbytes)) / (statistics.mean(kms) / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload),
+                "kernel": "sssp_fused_kernel (3 repairs + the combined solve per step)",
+                "kernel_ms_mean": round(statistics.mean(kms), 4), "algorithmic_bytes_mean": int(sum(kbytes) / len(kbytes))}
+    cpu, exact = None, None
+    if not args.no_cpu and args.cpu_sample > 0:
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        sample = min(args.cpu_sample, 2)
+        parts, results = mosp_cpu_baseline(n, src, dst, w, drawn, source, sample, threads)
+        total = [sum(v for k, v in p.items() if k != "repairs_s") for p in parts]
+        cpu = {"value": round(ops / statistics.median(total), 1), "unit": "edges/s", "cores": threads,
+               "kind": "port", "sample": f"{sample} batches x {ops} ops after compute(); one mosp update = batch "
+               "apply + transpose + 3 OpenMP repairs, the combined graph (numpy), its OpenMP solve and the path "
+               "costs (numpy)",
+               "p50_s_update": round(statistics.median(total), 3),
+               "p50_ms_repairs": round(statistics.median(p["repairs_s"] for p in parts) * 1e3, 2)}
+        g2 = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
+        r2 = mosp.compute(g2, source, preferences=MOSP_PREFERENCES)
+        ok = True
+        for b in range(sample):
+            x = drawn[b]
+            mosp.update(g2, dyng_amd.EdgeBatch(insert=(x[0], x[1], x[2]), delete=(x[3], x[4])), r2)
+            cd, cp, cc = results[b]
+            same = bool(np.array_equal(r2.combined_distances, cd) and np.array_equal(r2.combined_parents, cp)
+                        and np.array_equal(r2.path_costs, cc))
+            ok = ok and same
+            log(f"[bench] parity batch {b}: {'bit-exact' if same else 'MISMATCH'}")
+        exact = {"batches_checked": sample, "bit_exact": ok,
+                 "against": "oracle (OpenMP repairs, combined graph, OpenMP solve, path costs)"}
+        del r2, g2
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "edges/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(sum(wall) / K, 4), "p50_ms": round(p50, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic",
+        "config": {"workload": args.workload, "vertices": int(n), "edges": int(len(src)), "objectives": int(w.shape[1]),
+                   "preferences": MOSP_PREFERENCES, "batch_ops": ops, "source": int(source),
+                   "apply_ms_p50": round(statistics.median(s.objectives[0].apply_ms for s in stats), 4),
+                   "repairs_ms_p50": round(statistics.median(sum(o.engine_ms for o in s.objectives)
+                                                             for s in stats), 4),
+                   "combined_solve_kernel_ms_p50": round(statistics.median(s.combined_solve.kernel_ms
+                                                                           for s in stats), 4),
+                   "combined_edges_p50": int(statistics.median(s.combined_edges for s in stats))},
+        "roofline": roofline, "cpu_baseline": cpu, "parity": exact,
+    }
+    print(json.dumps(out), flush=True)
+    del dev, r, g
+    import gc
+
+    gc.collect()
+    dyng_amd.synchronize()
+    return 0 if (exact is None or exact["bit_exact"]) else 3
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -417,6 +580,8 @@ def main() -> int:
         return stream_bench(args)
     if WORKLOADS[args.workload][0] == "batched":
         return batched_bench(args)
+    if WORKLOADS[args.workload][0] == "mosp":
+        return mosp_bench(args)
     K, W = args.steps, args.warmup
     n, src, dst, w, batch, source, ops = make_inputs(args.workload, K + W)
 

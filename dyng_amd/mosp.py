@@ -122,6 +122,7 @@ class Stats:
     objectives: tuple
     combined_edges: int
     preference_scale: int
+    combined_solve: Any = None  # dyng_amd.sssp.DeviceCounters of the combined solve (HIP path only)
 
 
 class Result:
@@ -378,7 +379,8 @@ def _finish_update(graph: Graph, result: Result, per: tuple) -> Stats:
                  converged=all(s.converged for s in per),
                  engine_used=per[0].engine_used if per else "fused",
                  batch=per[0].batch, objectives=per, combined_edges=int(st.combined_edges),
-                 preference_scale=int(st.preference_scale))
+                 preference_scale=int(st.preference_scale),
+                 combined_solve=_sssp.DeviceCounters._from_native(solve))
 
 
 def update(graph: Graph, batch: EdgeBatch, result: Result, *, resources: Resources | None = None) -> Stats:

@@ -94,6 +94,7 @@ class DeviceCounters:
     pack_ms: float
     jump_ms: float
     unpack_ms: float
+    words_kept: bool = False  # the workspace kept the tree's packed words from the last run (no pack)
 
     @classmethod
     def _from_native(cls, d) -> "DeviceCounters":
@@ -101,7 +102,7 @@ class DeviceCounters:
                    bool(d.packed_parents), int(d.relax_vertices), int(d.relax_edges), int(d.relax_improved),
                    int(d.pull_edges), int(d.jump_rounds), int(d.jump_items), int(d.pull_vertices),
                    float(d.step1_ms), float(d.pull_ms), float(d.relax_ms), int(d.blocks), float(d.kernel_ms),
-                   float(d.pack_ms), float(d.jump_ms), float(d.unpack_ms))
+                   float(d.pack_ms), float(d.jump_ms), float(d.unpack_ms), bool(d.words_kept))
 
 
 @dataclass(frozen=True)

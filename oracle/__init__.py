@@ -441,3 +441,40 @@ def update(g: Csr, tree: Tree, ins_src, ins_dst, ins_w, del_src, del_dst,
     c = run_engine(engine, True, g2, inn, k, tree.source, delta, max_weight, tree.dist,
                    tree.parent, cf, ct, applied.ins_dst, threads=threads, ws=ws)
     return g2, applied, c
+
+
+def update_many(g: Csr, trees: list, ins_src, ins_dst, ins_w, del_src, del_dst,
+                props: Props | None = None, engine: str = "seq", threads: int = 1,
+                workspaces: list | None = None) -> tuple[Csr, Applied, list]:
+    """dyng::update_each on the host backends (cpp/include/dyng/update.hpp:311-364): every tree's
+    preparation on G_t, the batch applied once, then each tree repaired on G_{t+1} (one
+    transpose shared by all). Returns (G_{t+1}, the apply, one Counters per tree)."""
+    props = props or Props()
+    m_in = len(np.atleast_1d(ins_src))
+    ins_w = np.asarray(ins_w, dtype=np.int64).reshape(m_in, -1) if m_in else np.zeros((0, g.K), np.int64)
+    prepared = []
+    for tree in trees:
+        k = tree.objective
+        if len(ins_w) and (ins_w[:, k] < 1).any():
+            raise OracleError(1, "sssp::update: an insertion has a weight below 1")
+        max_w, wsum = column_summary(g, k)
+        largest = int(ins_w[:, k].max()) if len(ins_w) else 1
+        delta = tree.delta if tree.delta > 0 else default_delta(max(g.m, 1), g.n, wsum)
+        prepared.append((max(largest, max_w), delta))
+    applied = apply_batch(g, ins_src, ins_dst, ins_w, del_src, del_dst, props)
+    g2 = applied.g
+    inn = transpose(g2)
+    out = []
+    for i, (tree, (max_weight, delta)) in enumerate(zip(trees, prepared)):
+        k = tree.objective
+        if not distances_fit(g2.n, max_weight):
+            raise OracleError(1, "sssp::update: distances do not fit in 62 bits")
+        if g2.n > len(tree.dist):
+            grow = g2.n - len(tree.dist)
+            tree.dist = np.concatenate([tree.dist, np.full(grow, INF, np.int64)])
+            tree.parent = np.concatenate([tree.parent, np.full(grow, -1, np.int32)])
+        cf, ct = applied.changes(k)
+        ws = workspaces[i] if workspaces is not None else None
+        out.append(run_engine(engine, True, g2, inn, k, tree.source, delta, max_weight, tree.dist,
+                              tree.parent, cf, ct, applied.ins_dst, threads=threads, ws=ws))
+    return g2, applied, out

@@ -15,6 +15,9 @@ Follows the reference's host path (paths relative to the reference root):
 - ``path_costs``: the objective values of the MOSP path to every vertex along the combined tree,
   each tree edge (p, v) taken as the first (p, v) of p's row of the graph
   (cpp/src/algorithms/mosp/sequential.cpp:172-236); unreachable vertices keep INF.
+- ``combine_fast`` / ``path_costs_fast``: the same two functions over whole arrays (numpy), for
+  graphs where the loops above take too long (the CPU baseline); tests/test_mosp_oracle.py checks
+  them against the loops.
 
 Pinned by the reference's committed MOSP combined fixtures (cpp/tests/data/mosp_combined:
 distances, trees and path costs of 8 preference cases; tests/test_oracle.py) and the thesis
@@ -119,4 +122,81 @@ def path_costs(g: Csr, parent, source: int, columns: int) -> np.ndarray:
             e = lo + int(hit[0])
             costs[v, :] = costs[p, :] + g.w[:columns, e].astype(np.int64)
             queue.append(v)
+    return costs
+
+
+def combine_fast(parents, source: int, preferences=None) -> tuple[Csr, int, int]:
+    """``combine`` over whole arrays: the same graph, weight sum and base."""
+    k_count = len(parents)
+    P = np.stack([np.asarray(p, np.int64) for p in parents])
+    n = P.shape[1]
+    terms, base, _ = terms_and_base(preferences, k_count)
+    v = np.arange(n, dtype=np.int64)
+    src, dst, w = [], [], []
+    for k in range(k_count):
+        p = P[k]
+        keep = (p >= 0) & (v != source)
+        for j in range(k):
+            keep &= P[j] != p
+        weight = np.full(n, base - terms[k], np.int64)
+        for j in range(k + 1, k_count):
+            weight -= np.where(P[j] == p, terms[j], 0)
+        src.append(p[keep])
+        dst.append(v[keep])
+        w.append(weight[keep])
+    src = np.concatenate(src) if src else np.zeros(0, np.int64)
+    dst = np.concatenate(dst) if dst else np.zeros(0, np.int64)
+    w = np.concatenate(w) if w else np.zeros(0, np.int64)
+    order = np.lexsort((dst, src))
+    src, dst, w = src[order], dst[order], w[order]
+    row = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(src, minlength=n), out=row[1:])
+    g = Csr(row, dst.astype(np.int32), w.astype(np.int32).reshape(1, -1))
+    return g, int(w.sum()), base
+
+
+def path_costs_fast(g: Csr, parent, source: int, columns: int) -> np.ndarray:
+    """``path_costs`` over whole arrays: each tree edge (p, v) located as the first (p, v) of p's
+    row by a stable sort of the (row, neighbour) keys, then path doubling up the tree."""
+    parent = np.asarray(parent, np.int64)
+    n = g.n
+    costs = np.full((n, columns), INF, np.int64)
+    if n == 0:
+        return costs
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(g.row_ptr))
+    keys = rows * n + g.col.astype(np.int64)
+    if len(keys) < 2 or bool((keys[1:] >= keys[:-1]).all()):
+        order = np.arange(len(keys))  # rows already sorted by neighbour
+        sorted_keys = keys
+    else:
+        order = np.argsort(keys, kind="stable")
+        sorted_keys = keys[order]
+    v = np.arange(n, dtype=np.int64)
+    has = (parent >= 0) & (v != source)
+    want = parent[has] * n + v[has]
+    at = np.searchsorted(sorted_keys, want, side="left")
+    found = at < len(sorted_keys)
+    found[found] = sorted_keys[at[found]] == want[found]
+    if not found.all():
+        bad = v[has][~found][0]
+        raise ValueError(f"mosp: the MOSP tree edge ({int(parent[bad])}, {int(bad)}) is not an edge of the graph")
+    acc = np.zeros((n, columns), np.int64)
+    acc[v[has]] = g.w[:columns, order[at]].T.astype(np.int64)
+    # acc[v] sums the edges from v up to end[v]; each round doubles the covered segments
+    end = np.where(has, parent, v)
+    rounds = 2
+    while (1 << rounds) < n:
+        rounds += 1
+    for _ in range(rounds + 1):
+        active = has[end]
+        if not active.any():
+            break
+        e = end[active]
+        acc[active] += acc[e]
+        nxt = end.copy()
+        nxt[active] = end[e]
+        end = nxt
+    # the vertices whose path runs back to the source hold costs (a parent cycle never ends)
+    reach = (end == source) & ~has[end]
+    costs[reach] = acc[reach]
     return costs

@@ -86,3 +86,56 @@ def test_planted_combine_mistake_is_caught():
     d_bad, _ = oracle.dijkstra(bad, 0, 0)
     d_good, _ = oracle.dijkstra(cg, 0, 0)
     assert not np.array_equal(d_bad, d_good)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_fast_combine_and_costs_equal_the_loops(seed):
+    """The whole-array combine and path costs (the CPU baseline's) equal the loop versions on
+    random multigraphs (parallel edges, unreachable vertices) and on every fixture case."""
+    rng = np.random.default_rng(seed)
+    n, m, K = 400, 1600, 3
+    src, dst = rng.integers(0, n, m), rng.integers(0, n, m)
+    w = rng.integers(1, 20, (m, K))
+    g = oracle.build_from_edges(n, src, dst, w, oracle.Props.mosp_compatible())
+    trees = [oracle.dijkstra(g, 0, k)[1] for k in range(K)]
+    prefs = [[], [1, 2, 3], [4, 1, 4], [7, 7, 2]][seed - 1]
+    a, sa, ba = mosp.combine(trees, 0, prefs)
+    b, sb, bb = mosp.combine_fast(trees, 0, prefs)
+    assert (sa, ba) == (sb, bb)
+    assert np.array_equal(a.row_ptr, b.row_ptr) and np.array_equal(a.col, b.col) and np.array_equal(a.w, b.w)
+    d, p = oracle.dijkstra(a, 0, 0)
+    assert np.array_equal(mosp.path_costs(g, p, 0, K), mosp.path_costs_fast(g, p, 0, K))
+
+
+def test_fast_costs_on_the_fixtures_and_a_missing_edge():
+    for name, inp, init, K, prefs in fio.combined_cases():
+        g = _updated_graph(inp)
+        trees = [oracle.dijkstra(g, 0, k)[1] for k in range(K)]
+        _, p, _ = mosp.solve(trees, 0, prefs)
+        want = fio.read_costs(os.path.join(fio.GOLDEN, "mosp_combined", name, "combined", "mospCosts.txt"), g.n)
+        assert np.array_equal(mosp.path_costs_fast(g, p, 0, want.shape[1]), want), name
+    g = oracle.build_from_edges(3, np.array([0]), np.array([1]), np.array([[1]]))
+    with pytest.raises(ValueError):
+        mosp.path_costs_fast(g, np.array([-1, 0, 1]), 0, 1)
+
+
+def test_update_many_equals_one_update_per_tree():
+    """oracle.update_many (dyng::update_each): one apply, K repairs = K separate updates."""
+    rng = np.random.default_rng(9)
+    n, m, K = 500, 4000, 3
+    src, dst = rng.integers(0, n, m), rng.integers(0, n, m)
+    w = rng.integers(1, 50, (m, K))
+    g = oracle.build_from_edges(n, src, dst, w)
+    many = [oracle.compute(g, s, k)[0] for s, k in ((0, 0), (3, 1), (0, 2))]
+    alone = [oracle.compute(g, s, k)[0] for s, k in ((0, 0), (3, 1), (0, 2))]
+    ins_s, ins_d = rng.integers(0, n + 2, 200), rng.integers(0, n + 2, 200)
+    ins_w = rng.integers(1, 50, (200, K))
+    rows = np.repeat(np.arange(g.n), np.diff(g.row_ptr))
+    pick = rng.integers(0, g.m, 150)
+    g2, _, cs = oracle.update_many(g, many, ins_s, ins_d, ins_w, rows[pick], g.col[pick])
+    for t, tm, c in zip(alone, many, cs):
+        _, _, ca = oracle.update(g, t, ins_s, ins_d, ins_w, rows[pick], g.col[pick])
+        assert np.array_equal(t.dist, tm.dist) and np.array_equal(t.parent, tm.parent)
+        assert (c.invalidated, c.affected) == (ca.invalidated, ca.affected)
+        d, p = oracle.dijkstra(g2, t.source, t.objective)
+        assert np.array_equal(tm.dist, d) and np.array_equal(tm.parent, p)
