@@ -6,7 +6,7 @@ HIP for gfx950). There is no CPU fallback: without the built library every call 
 NotSupportedError.
 """
 
-from . import batched, mosp, scheduler, sssp
+from . import batched, io, mosp, scheduler, sssp
 from ._update import update
 from .batch import EdgeBatch
 from .device import DeviceArray, device_count, synchronize
@@ -17,7 +17,7 @@ from .resources import Resources
 
 __version__ = "0.1.0"
 
-__all__ = ["update", "sssp", "mosp", "batched", "scheduler", "Graph", "GraphProperties", "BatchSemantics", "ApplySummary", "EdgeBatch", "Resources",
+__all__ = ["update", "sssp", "mosp", "io", "batched", "scheduler", "Graph", "GraphProperties", "BatchSemantics", "ApplySummary", "EdgeBatch", "Resources",
            "DeviceArray", "device_count", "synchronize",
            "DyngError", "InvalidArgumentError", "StaleResultError", "NotSupportedError", "OutOfMemoryError",
            "DeviceError", "available", "__version__"]

@@ -119,6 +119,36 @@ class EdgeBatch:
     def empty(self) -> bool:
         return self.num_insertions == 0 and self.num_deletions == 0
 
+    def _part(self, which, i):
+        part = self._ins if which == "insert" else self._del
+        if part is None:
+            if which == "insert" and i == 2:
+                return np.zeros((0, 0), np.int32)
+            return np.zeros(0, np.int32)
+        return part[i]
+
+    @property
+    def insert_src(self):
+        """Insertion sources (int32; a DeviceArray for a device batch)."""
+        return self._part("insert", 0)
+
+    @property
+    def insert_dst(self):
+        return self._part("insert", 1)
+
+    @property
+    def insert_weights(self):
+        """(num_insertions, K) int32 (a DeviceArray for a device batch)."""
+        return self._part("insert", 2)
+
+    @property
+    def delete_src(self):
+        return self._part("delete", 0)
+
+    @property
+    def delete_dst(self):
+        return self._part("delete", 1)
+
     def _native(self, graph_weights: int):
         """The C struct and the objects that must outlive the call."""
         ni, nd = self.num_insertions, self.num_deletions

@@ -350,6 +350,37 @@ def _combine(graph: Graph, result: Result, *, count_changes: bool) -> _lib.MospS
     return st
 
 
+def path_costs_over(graph: Graph, result: Result, columns: int) -> np.ndarray:
+    """The first ``columns`` weight columns of ``graph`` (up to all of them, more than the
+    objectives) summed along ``result``'s MOSP tree: (n, columns) int64 on the host, as MOSP's
+    mospCosts.txt holds them. ``result`` must match the graph's current version."""
+    if not isinstance(result, Result):
+        raise TypeError("dyng_amd.mosp.path_costs_over: result must be a dyng_amd.mosp.Result")
+    if result.graph_version != graph.version:
+        from .errors import StaleResultError
+
+        raise StaleResultError("dyng_amd.mosp.path_costs_over: the result does not match the graph's version")
+    columns = int(columns)
+    if not 1 <= columns <= graph.num_weights:
+        raise InvalidArgumentError(f"mosp.path_costs_over: columns must be in [1, {graph.num_weights}]")
+    res = result._resources
+    n = graph.num_vertices
+    if n == 0:
+        return np.zeros((0, columns), np.int64)
+    count = result._count
+    ptrs = (C.c_void_p * count)(*[t.device_arrays()[1] for t in result._trees])
+    prefs = result._options.preferences
+    pref_arr = (C.c_int32 * count)(*prefs) if prefs else None
+    dist = DeviceArray.empty(n, np.int64, device=res.device)
+    par = DeviceArray.empty(n, np.int32, device=res.device)
+    costs = DeviceArray.empty((n, columns), np.int64, device=res.device)
+    st = _lib.MospStats()
+    _lib.check(_lib.lib().dyng_hip_mosp_combined(graph._handle, result._source, count, ptrs, pref_arr, 0,
+                                                 dist.ptr, par.ptr, columns, costs.ptr, res.stream_handle,
+                                                 C.byref(st)))
+    return costs.to_numpy()
+
+
 def compute(graph: Graph, source: int, *, options: Options | None = None, resources: Resources | None = None,
             **kwargs: Any) -> Result:
     """The K canonical trees of ``graph`` from ``source``, the combined graph, the MOSP tree and
