@@ -104,6 +104,28 @@ void exclusive_scan(dbuf<unsigned char>& tmp, const T* in, T* out, size_t n, hip
                                        rocprim::plus<T>(), s));
 }
 
+/// The exclusive prefix sums of four int32 arrays of n entries in one scan (the pending and
+/// killed flags of both directions' operations).
+struct quad_plus {
+  using quad = rocprim::tuple<int32_t, int32_t, int32_t, int32_t>;
+  __device__ quad operator()(const quad& a, const quad& b) const {
+    return quad(rocprim::get<0>(a) + rocprim::get<0>(b), rocprim::get<1>(a) + rocprim::get<1>(b),
+                rocprim::get<2>(a) + rocprim::get<2>(b), rocprim::get<3>(a) + rocprim::get<3>(b));
+  }
+};
+
+void exclusive_scan4(dbuf<unsigned char>& tmp, const int32_t* a, const int32_t* b, const int32_t* c, const int32_t* d,
+                     int32_t* oa, int32_t* ob, int32_t* oc, int32_t* od, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  auto in = rocprim::make_zip_iterator(rocprim::make_tuple(a, b, c, d));
+  auto out = rocprim::make_zip_iterator(rocprim::make_tuple(oa, ob, oc, od));
+  const quad_plus::quad zero(0, 0, 0, 0);
+  size_t bytes = 0;
+  DYNG_HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, in, out, zero, n, quad_plus(), s));
+  tmp.reserve(bytes);
+  DYNG_HIP_TRY(rocprim::exclusive_scan(static_cast<void*>(tmp.p), bytes, in, out, zero, n, quad_plus(), s));
+}
+
 template <typename T>
 __global__ void k_fill(T* p, int64_t n, T v) {
   for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < n; i += int64_t{gridDim.x} * blockDim.x) p[i] = v;
@@ -1920,9 +1942,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     hipLaunchKernelGGL(k_out_pairs, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, op);
     DYNG_HIP_TRY(hipGetLastError());
     mark_stage(g, 5, s);
-    exclusive_scan<int32_t>(g->scan_tmp, g->pend_alive.p, g->pend_prefix.p, N + 1, s);
-    exclusive_scan<int32_t>(g->scan_tmp, g->kill.p, g->kill_prefix.p, N + 1, s);
-    mark_stage(g, 6, s);
+    mark_stage(g, 6, s);  // the out-rows' prefixes are scanned with the in-rows' below
     // ---- the in-rows' operations (they do not read the out-rows) ----
     mark_stage(g, 7, s);
     in_pair_params ip{};
@@ -1952,8 +1972,8 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     hipLaunchKernelGGL(k_in_pairs, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, ip);
     DYNG_HIP_TRY(hipGetLastError());
     mark_stage(g, 8, s);
-    exclusive_scan<int32_t>(g->scan_tmp, g->ipend_alive.p, g->ipend_prefix.p, N + 1, s);
-    exclusive_scan<int32_t>(g->scan_tmp, g->ikill.p, g->ikill_prefix.p, N + 1, s);
+    exclusive_scan4(g->scan_tmp, g->pend_alive.p, g->kill.p, g->ipend_alive.p, g->ikill.p, g->pend_prefix.p,
+                    g->kill_prefix.p, g->ipend_prefix.p, g->ikill_prefix.p, N + 1, s);
     mark_stage(g, 9, s);
     // ---- plan the merges: each row's mode, and the staging and tail slots they take ----
     g->merge_lists.reserve(2 * kModes * (N + 1));

@@ -82,9 +82,9 @@ typedef struct dyng_hip_apply_summary {
   int64_t inserted_edges, updated_edges, deleted_edges, ignored_deletions, dropped_self_loops;
   int64_t inserted_vertices, num_vertices_after, ignored_insertions, relocated_rows, compactions;
   /* device time of the apply's stages (HIP events, ms): batch scan, operation keys, key sort
-   * (both directions' keys in one sort), out-row pairing, out scans, (unused: 0), in-row pairing,
-   * in scans, merge planning (with its host round trip and any compaction), out-row merge,
-   * in-row merge */
+   * (both directions' keys in one sort), out-row pairing, (unused: 0), (unused: 0), in-row
+   * pairing, the prefix scans of both directions (one scan), merge planning (with its host round
+   * trip and any compaction), out-row merge, in-row merge */
   double stage_ms[11];
 } dyng_hip_apply_summary;
 
