@@ -37,7 +37,7 @@ SUMMARY_FIELDS = ("inserted_edges", "updated_edges", "deleted_edges", "ignored_d
                   "ignored_insertions", "relocated_rows", "compactions")
 
 
-APPLY_STAGES = ("scan", "make_ops", "sort_out", "pairs_out", "scans_out", "sort_in", "pairs_in", "scans_in",
+APPLY_STAGES = ("scan", "make_ops", "sort", "pairs_out", "unused_4", "unused_5", "pairs_in", "scans",
                 "plan", "merge_out", "merge_in")
 
 

@@ -36,6 +36,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 #include <rocprim/rocprim.hpp>
 
@@ -1722,7 +1723,18 @@ void copy_rows_to_host(const row_store& rs, int64_t n, int K, int64_t* row_ptr, 
 
 namespace {
 
+/// Whether the apply times its stages: DYNG_HIP_STAGE_TIMES=1 in the environment. Each timed
+/// event costs the stream a few microseconds, so the stages are not timed by default.
+bool stage_timing() {
+  static const bool on = [] {
+    const char* v = std::getenv("DYNG_HIP_STAGE_TIMES");
+    return v != nullptr && v[0] == '1';
+  }();
+  return on;
+}
+
 void mark_stage(dyng_hip_graph* g, int i, hipStream_t s) {
+  if (!stage_timing()) return;
   if (g->apply_ev[i] == nullptr) DYNG_HIP_TRY(hipEventCreate(&g->apply_ev[i]));
   DYNG_HIP_TRY(hipEventRecord(g->apply_ev[i], s));
 }
@@ -1730,6 +1742,7 @@ void mark_stage(dyng_hip_graph* g, int i, hipStream_t s) {
 /// Stage durations from the recorded events (after the stream has been synchronized).
 void stage_times(dyng_hip_graph* g, bool rows, dyng_hip_apply_summary* out) {
   for (double& t : out->stage_ms) t = 0.0;
+  if (!stage_timing()) return;
   float ms = 0.0f;
   if (hipEventElapsedTime(&ms, g->apply_ev[0], g->apply_ev[1]) == hipSuccess) out->stage_ms[0] = ms;
   if (!rows) return;

@@ -81,10 +81,11 @@ typedef struct dyng_hip_graph dyng_hip_graph;
 typedef struct dyng_hip_apply_summary {
   int64_t inserted_edges, updated_edges, deleted_edges, ignored_deletions, dropped_self_loops;
   int64_t inserted_vertices, num_vertices_after, ignored_insertions, relocated_rows, compactions;
-  /* device time of the apply's stages (HIP events, ms): batch scan, operation keys, key sort
-   * (both directions' keys in one sort), out-row pairing, (unused: 0), (unused: 0), in-row
-   * pairing, the prefix scans of both directions (one scan), merge planning (with its host round
-   * trip and any compaction), out-row merge, in-row merge */
+  /* device time of the apply's stages (HIP events, ms; all 0 unless the environment sets
+   * DYNG_HIP_STAGE_TIMES=1, since each timed event costs the stream a few microseconds): batch
+   * scan, operation keys, key sort (both directions' keys in one sort), out-row pairing, (unused),
+   * (unused), in-row pairing, the prefix scans of both directions (one scan), merge planning (with
+   * its host round trip and any compaction), out-row merge, in-row merge */
   double stage_ms[11];
 } dyng_hip_apply_summary;
 
