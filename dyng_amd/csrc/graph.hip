@@ -2030,11 +2030,16 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     const unsigned plan_grid = static_cast<unsigned>((N + kPlanBlock - 1) / kPlanBlock);  // touched rows <= N
     if (wait != nullptr) DYNG_HIP_TRY(hipStreamWaitEvent(s, wait, 0));  // before the rows change
     for (int attempt = 0;; ++attempt) {
-      hc->need_reloc_out = hc->need_reloc_in = hc->need_stage_out = hc->need_stage_in = 0;
-      for (int i = 0; i < kModes; ++i) hc->modes_out[i] = hc->modes_in[i] = 0;
-      DYNG_HIP_TRY(hipMemcpyAsync(&dc->need_reloc_out, &hc->need_reloc_out, 4 * sizeof(long long),
-                                  hipMemcpyHostToDevice, s));
-      DYNG_HIP_TRY(hipMemcpyAsync(dc->modes_out, hc->modes_out, 2 * sizeof(hc->modes_out), hipMemcpyHostToDevice, s));
+      if (attempt > 0) {
+        // planning again: clear what the last plan counted (the first plan finds them zero, as
+        // the control block was uploaded zeroed at the start of the apply)
+        hc->need_reloc_out = hc->need_reloc_in = hc->need_stage_out = hc->need_stage_in = 0;
+        for (int i = 0; i < kModes; ++i) hc->modes_out[i] = hc->modes_in[i] = 0;
+        DYNG_HIP_TRY(hipMemcpyAsync(&dc->need_reloc_out, &hc->need_reloc_out, 4 * sizeof(long long),
+                                    hipMemcpyHostToDevice, s));
+        DYNG_HIP_TRY(hipMemcpyAsync(dc->modes_out, hc->modes_out, 2 * sizeof(hc->modes_out), hipMemcpyHostToDevice,
+                                    s));
+      }
       hipLaunchKernelGGL(k_merge_plan, dim3(plan_grid), dim3(kPlanBlock), 0, s, merge_setup(true));
       hipLaunchKernelGGL(k_merge_plan, dim3(plan_grid), dim3(kPlanBlock), 0, s, merge_setup(false));
       DYNG_HIP_TRY(hipGetLastError());

@@ -429,9 +429,20 @@ DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best
 #define DYNG_GRID_SYNC_FENCED() DYNG_GRID_SYNC_AS(true)
 
 /// Clear the grid barrier (a kernel, not a memset, so the launch can live in a hipGraph).
-__global__ void k_reset_barrier(grid_sync_state* bar) {
+/// Before each run (one workgroup): the barrier state and the control block zeroed, the control
+/// block's minima at INF (a kernel rather than a host copy: it costs the stream no blit).
+static_assert(sizeof(control) % 8 == 0 && sizeof(grid_sync_state) % 8 == 0, "reset in 8-byte words");
+__global__ void k_reset_barrier(grid_sync_state* bar, control* ctl) {
   unsigned long long* w = reinterpret_cast<unsigned long long*>(bar);
   for (unsigned int i = threadIdx.x; i < sizeof(grid_sync_state) / 8; i += blockDim.x) w[i] = 0;
+  unsigned long long* c = reinterpret_cast<unsigned long long*>(ctl);
+  for (unsigned int i = threadIdx.x; i < sizeof(control) / 8; i += blockDim.x) c[i] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ctl->min_frontier = packed_inf;
+    ctl->minimum[0] = packed_inf;
+    ctl->minimum[1] = packed_inf;
+  }
 }
 
 __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) {
@@ -1236,17 +1247,16 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
     if (ws->graph_exec == nullptr) {
       // built once per workspace: the control block, the barrier and the events never move
       DYNG_HIP_TRY(hipGraphCreate(&ws->graph, 0));
-      hipGraphNode_t up, reset, start, stop, down;
-      DYNG_HIP_TRY(hipGraphAddMemcpyNode1D(&up, ws->graph, nullptr, 0, ws->ctl.p, hc, sizeof(control),
-                                           hipMemcpyHostToDevice));
+      hipGraphNode_t reset, start, stop, down;
       grid_sync_state* bar = ws->bar.p;
-      void* reset_args[] = {&bar};
+      control* ctl = ws->ctl.p;
+      void* reset_args[] = {&bar, &ctl};
       hipKernelNodeParams rp{};
       rp.func = reinterpret_cast<void*>(k_reset_barrier);
       rp.gridDim = dim3(1);
       rp.blockDim = dim3(64);
       rp.kernelParams = reset_args;
-      DYNG_HIP_TRY(hipGraphAddKernelNode(&reset, ws->graph, &up, 1, &rp));
+      DYNG_HIP_TRY(hipGraphAddKernelNode(&reset, ws->graph, nullptr, 0, &rp));
       DYNG_HIP_TRY(hipGraphAddEventRecordNode(&start, ws->graph, &reset, 1, ws->ev_start));
       DYNG_HIP_TRY(hipGraphAddKernelNode(&ws->graph_kernel, ws->graph, &start, 1, &kp));
       DYNG_HIP_TRY(hipGraphAddEventRecordNode(&stop, ws->graph, &ws->graph_kernel, 1, ws->ev_stop));
@@ -1258,8 +1268,7 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
     }
     DYNG_HIP_TRY(hipGraphLaunch(ws->graph_exec, s));
   } else {
-    DYNG_HIP_TRY(hipMemcpyAsync(ws->ctl.p, hc, sizeof(control), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, s, ws->bar.p);
+    hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, s, ws->bar.p, ws->ctl.p);
     DYNG_HIP_TRY(hipEventRecord(ws->ev_start, s));
     hipLaunchKernelGGL(sssp_fused_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
     DYNG_HIP_TRY(hipGetLastError());
