@@ -371,7 +371,7 @@ __global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* i
                            const int32_t* del_src, const int32_t* del_dst, int directed,
                            int drop_self_loops, int64_t n_after, int vbits, u64* keys, uint32_t* vals,
                            int32_t* e_del_src, int32_t* e_del_dst, int32_t* e_ins_src,
-                           int32_t* e_ins_dst, uint8_t* e_inc, int K, apply_control* ctl) {
+                           int32_t* e_ins_dst, uint8_t* e_inc, int32_t* e_fw, int K, apply_control* ctl) {
   const int64_t total = (L.num_first + L.num_second) * L.stride;
   unsigned long long dropped = 0, ignored = 0;
   for (int64_t e = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; e < total; e += int64_t{gridDim.x} * blockDim.x) {
@@ -410,6 +410,9 @@ __global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* i
       e_ins_src[loc] = valid ? u : -1;
       e_ins_dst[loc] = valid ? v : -1;
       for (int k = 0; k < K; ++k) e_inc[loc * K + k] = 0;
+      if (e_fw != nullptr) {
+        for (int k = 0; k < K; ++k) e_fw[loc * K + k] = -1;
+      }
     }
   }
   dropped = wave_sum(dropped);
@@ -452,6 +455,9 @@ struct out_pair_params {
   int32_t* kill;    ///< 1: the operation killed a tail entry
   int32_t* pend_w;  ///< k * N + position
   uint8_t* e_inc;
+  /// simple graphs: per insertion (batch slot, K weights) the pair's weights after the batch, -1
+  /// if the pair is gone (null with parallel edges, where several entries may hold the pair)
+  int32_t* e_fw;
   // what each operation (by batch slot e) does to the in-rows: kInNone, or the type and weights
   int32_t* iop_type;
   int32_t* iop_wold;
@@ -652,6 +658,27 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
               p.e_inc[loc * K + k] = wf[k * wf_stride] > orig ? 1 : 0;
             }
           }
+        }
+      }
+      // simple graphs: the pair's final weights for each of its insertions (what the engine offers
+      // the insertion's head, without searching the row again)
+      if (p.e_fw != nullptr) {
+        const int32_t* wf = nullptr;
+        int64_t wf_stride = 0;
+        const int final_hit = first_alive();
+        if (final_hit >= 0) {
+          wf = p.w + mt.x + final_hit;
+          wf_stride = p.slot_cap;
+        } else {
+          for (int64_t x = i; x < j; ++x) {
+            if (p.pend_alive[x]) { wf = p.pend_w + x; wf_stride = p.N; break; }
+          }
+        }
+        for (int64_t q = i; q < j; ++q) {
+          const int64_t e = p.vals[q];
+          if (p.L.is_del(e)) continue;
+          const int64_t loc = p.L.local(e);
+          for (int k = 0; k < K; ++k) p.e_fw[loc * K + k] = wf != nullptr ? wf[k * wf_stride] : -1;
         }
       }
       const bool first_touch = structural && atomicExch(&p.mark[u], p.mark_gen) != p.mark_gen;
@@ -1875,6 +1902,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
   g->ins_src.reserve(g->num_ins_slots);
   g->ins_dst.reserve(g->num_ins_slots);
   g->ins_inc.reserve(g->num_ins_slots * K);
+  if (!g->props.allow_parallel) g->ins_fw.reserve(g->num_ins_slots * K);
   g->keys_a.reserve(2 * N);  // out-row keys, then in-row keys
   g->keys_b.reserve(2 * N);
   g->vals_a.reserve(2 * N);
@@ -1914,7 +1942,8 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     mark_stage(g, 2, s);
     hipLaunchKernelGGL(k_make_ops, dim3(blocks_for(N)), dim3(kBlock), 0, s, L, ins_src, ins_dst, del_src, del_dst,
                        g->props.directed, g->props.on_self_loop == 1 ? 1 : 0, n_after, vbits, g->keys_a.p,
-                       g->vals_a.p, g->del_src.p, g->del_dst.p, g->ins_src.p, g->ins_dst.p, g->ins_inc.p, K, dc);
+                       g->vals_a.p, g->del_src.p, g->del_dst.p, g->ins_src.p, g->ins_dst.p, g->ins_inc.p,
+                       g->props.allow_parallel ? nullptr : g->ins_fw.p, K, dc);
     DYNG_HIP_TRY(hipGetLastError());
     mark_stage(g, 3, s);
     // one stable sort of both directions' keys (2N of them, 2 * vbits + 1 bits): one sort of 2N
@@ -1943,6 +1972,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     op.kill = g->kill.p;
     op.pend_w = g->pend_w.p;
     op.e_inc = g->ins_inc.p;
+    op.e_fw = g->props.allow_parallel ? nullptr : g->ins_fw.p;
     op.iop_type = g->iop_type.p;
     op.iop_wold = g->iop_wold.p;
     op.iop_wnew = g->iop_wnew.p;

@@ -225,6 +225,7 @@ struct dyng_hip_graph {
   // last apply: the effective operations (with holes, -1) and their weight-increase flags
   dyng_hip::dbuf<int32_t> del_src, del_dst, ins_src, ins_dst;
   dyng_hip::dbuf<uint8_t> ins_inc;
+  dyng_hip::dbuf<int32_t> ins_fw;  ///< simple graphs: each insertion's pair weights after the batch (-1: gone)
   int64_t num_del_slots = 0, num_ins_slots = 0;
 
   // apply scratch

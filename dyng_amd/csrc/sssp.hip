@@ -89,6 +89,7 @@ struct engine_params {
   const int32_t* ins_src;
   const int32_t* ins_dst;
   const uint8_t* ins_inc;
+  const int32_t* ins_fw;  ///< null, or each insertion's pair weights after the batch (simple graphs)
   int32_t inc_stride;
   int32_t inc_k;
   int64_t num_ins;
@@ -388,6 +389,17 @@ DYNG_DEV u64 edge_offer(const engine_params& p, int32_t u, int32_t v) {
     }
   }
   return best;
+}
+
+/// The offer of insertion i's pair (u, v) on a simple graph: u's distance plus the pair's weight
+/// after the batch, which the apply recorded (the pair is one entry of u's row, so no search).
+DYNG_DEV u64 final_offer(const engine_params& p, int32_t u, int64_t i) {
+  const u64 word = ld(&p.packed[u]);
+  if (word == packed_inf) return packed_inf;
+  const int32_t w = p.ins_fw[i * p.inc_stride + p.inc_k];
+  if (w < 1) return packed_inf;  // the batch left no (u, v)
+  const u64 nd = p.pk.distance(word) + static_cast<u64>(w);
+  return nd <= p.max_distance ? p.pk.pack(nd, u) : packed_inf;
 }
 
 /// Lower v's word to its best pull offer; v joins the frontier if its distance fell. The
@@ -695,7 +707,9 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         u64 best = packed_inf;
         if (i < p.num_ins) {
           v = p.ins_dst[i];
-          if (v >= 0 && v != p.source) best = edge_offer(p, p.ins_src[i], v);
+          if (v >= 0 && v != p.source) {
+            best = p.ins_fw != nullptr ? final_offer(p, p.ins_src[i], i) : edge_offer(p, p.ins_src[i], v);
+          }
         }
         pull_commit(p, best != packed_inf, v, best, generation, S, &c->frontier_count, &c->min_frontier);
       }
@@ -1166,6 +1180,7 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
     p.ins_src = g->ins_src.p;
     p.ins_dst = g->ins_dst.p;
     p.ins_inc = g->ins_inc.p;
+    p.ins_fw = g->props.allow_parallel ? nullptr : g->ins_fw.p;
     p.inc_stride = g->K;
     p.inc_k = k;
     p.num_ins = g->num_ins_slots;

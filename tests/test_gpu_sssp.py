@@ -277,6 +277,11 @@ STREAM_CASES = [
     # long rows (past the 16-entry bisection) with several parallel copies of each edge
     (9, 64, 20000, 50, {"allow_parallel": True}),
     (10, 400, 40000, 100, {"allow_parallel": True, "deletions_first": False}),
+    # simple graphs where a batch inserts and then deletes a pair, or ignores re-insertions (the
+    # engine offers each insertion's head the pair's weights after the batch, as the apply left them)
+    (11, 300, 2000, 50, {"deletions_first": False}),
+    (12, 200, 1500, 30, {"existing": "ignore", "deletions_first": False}),
+    (13, 250, 1800, 40, {"directed": False, "deletions_first": False}),
 ]
 
 
@@ -302,8 +307,10 @@ def test_random_streams_match_the_oracle(dg, seed, n, m, wmax, kw):
         ins = (rng.integers(0, n + grow, ni), rng.integers(0, n + grow, ni), rng.integers(1, wmax + 1, ni))
         rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
         pick = rng.integers(0, og.m, nd) if og.m else np.zeros(0, np.int64)
-        dels = (np.concatenate([rows[pick], rng.integers(0, n, 2)]),
-                np.concatenate([og.col[pick], rng.integers(0, n, 2)]))  # plus possibly missing ones
+        # plus possibly missing pairs, and pairs this batch also inserts (inserted and deleted in
+        # one batch, in either order)
+        dels = (np.concatenate([rows[pick], rng.integers(0, n, 2), ins[0][:3]]),
+                np.concatenate([og.col[pick], rng.integers(0, n, 2), ins[1][:3]]))
         og, a, c = oracle.update(og, ot, ins[0], ins[1], ins[2].reshape(-1, 1), dels[0], dels[1], props=op)
         st = dg.sssp.update(hg, dg.EdgeBatch(insert=ins, delete=dels), hr)
         d, p = hr.to_numpy()
