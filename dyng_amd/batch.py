@@ -158,7 +158,7 @@ class EdgeBatch:
             if a is None:
                 return None
             keep.append(a)
-            return a.ptr if self._device else a.ctypes.data
+            return a.ptr if self._device else a.__array_interface__["data"][0]
 
         ins = self._ins or (None, None, None)
         dele = self._del or (None, None)

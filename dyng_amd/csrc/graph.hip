@@ -73,14 +73,6 @@ DYNG_DEV int64_t lower_key(const u64* keys, int64_t lo, int64_t hi, u64 key) {
   return lo;
 }
 
-DYNG_DEV int64_t upper_key(const u64* keys, int64_t lo, int64_t hi, u64 key) {
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (keys[mid] <= key) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
 // ---------------------------------------------------------------------------------------------
 // rocPRIM wrappers
 // ---------------------------------------------------------------------------------------------

@@ -105,32 +105,72 @@ class DeviceCounters:
                    float(d.pack_ms), float(d.jump_ms), float(d.unpack_ms), bool(d.words_kept))
 
 
-@dataclass(frozen=True)
 class Stats:
-    """sssp::stats (sssp.hpp:100-117) plus the device counters and stream times of this update."""
+    """sssp::stats (sssp.hpp:100-117) plus the device counters and stream times of this update.
 
-    affected: int
-    iterations: int
-    frontier_visits: int
-    fallback_used: bool
-    converged: bool
-    engine_used: str
-    batch: ApplySummary
-    invalidated: int
-    epochs: int
-    pushes: int
-    packed_parents: bool
-    device: DeviceCounters
-    apply_ms: float
-    engine_ms: float
+    Read-only. The fields decode from the library's record when first read (an update's stats
+    cost the caller no Python work it does not use): ``affected``, ``iterations``,
+    ``frontier_visits``, ``fallback_used``, ``converged``, ``engine_used``, ``batch``
+    (:class:`dyng_amd.ApplySummary`), ``invalidated``, ``epochs``, ``pushes``,
+    ``packed_parents``, ``device`` (:class:`DeviceCounters`), ``apply_ms``, ``engine_ms``.
+    """
+
+    __slots__ = ("_raw", "_batch", "_device")
+    FIELDS = ("affected", "iterations", "frontier_visits", "fallback_used", "converged", "engine_used", "batch",
+              "invalidated", "epochs", "pushes", "packed_parents", "device", "apply_ms", "engine_ms")
+
+    def __init__(self):
+        raise TypeError("dyng_amd.sssp.Stats comes from update, repair and dyng_amd.update")
 
     @classmethod
     def _from_native(cls, s: _lib.Stats) -> "Stats":
-        dev = DeviceCounters._from_native(s.device)
-        return cls(int(s.affected), int(s.iterations), int(s.frontier_visits), bool(s.fallback_used),
-                   bool(s.converged), _lib.ENGINE_NAMES.get(s.engine_used, "fused"),
-                   ApplySummary._from_native(s.batch), int(s.invalidated), int(s.epochs), int(s.pushes),
-                   bool(s.packed_parents), dev, float(s.apply_ms), float(s.engine_ms))
+        self = object.__new__(cls)
+        self._raw = s
+        self._batch = None
+        self._device = None
+        return self
+
+    affected = property(lambda self: int(self._raw.affected))
+    iterations = property(lambda self: int(self._raw.iterations))
+    frontier_visits = property(lambda self: int(self._raw.frontier_visits))
+    fallback_used = property(lambda self: bool(self._raw.fallback_used))
+    converged = property(lambda self: bool(self._raw.converged))
+    engine_used = property(lambda self: _lib.ENGINE_NAMES.get(self._raw.engine_used, "fused"))
+    invalidated = property(lambda self: int(self._raw.invalidated))
+    epochs = property(lambda self: int(self._raw.epochs))
+    pushes = property(lambda self: int(self._raw.pushes))
+    packed_parents = property(lambda self: bool(self._raw.packed_parents))
+    apply_ms = property(lambda self: float(self._raw.apply_ms))
+    engine_ms = property(lambda self: float(self._raw.engine_ms))
+
+    @property
+    def batch(self) -> ApplySummary:
+        if self._batch is None:
+            self._batch = ApplySummary._from_native(self._raw.batch)
+        return self._batch
+
+    @property
+    def device(self) -> DeviceCounters:
+        if self._device is None:
+            self._device = DeviceCounters._from_native(self._raw.device)
+        return self._device
+
+    def __setattr__(self, name, value):
+        if name in Stats.__slots__:
+            object.__setattr__(self, name, value)
+        else:
+            raise AttributeError("dyng_amd.sssp.Stats is read-only")
+
+    def __eq__(self, other) -> bool:
+        if not isinstance(other, Stats):
+            return NotImplemented
+        return all(getattr(self, f) == getattr(other, f) for f in Stats.FIELDS)
+
+    __hash__ = None
+
+    def __repr__(self) -> str:
+        inner = ", ".join(f"{f}={getattr(self, f)!r}" for f in Stats.FIELDS if f not in ("batch", "device"))
+        return f"dyng_amd.sssp.Stats({inner})"
 
 
 class Result:
