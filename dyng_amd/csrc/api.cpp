@@ -42,18 +42,34 @@ struct scoped_device {
   ~scoped_device() { (void)hipSetDevice(previous); }
 };
 
-/// The pooled device workspace of each device (ADR 0015: sized once, shared by every result run
-/// on the device; runs on one device are serialized by the pool's mutex).
+/// The pooled device workspaces of each device (ADR 0015: sized once, shared by the result runs
+/// on the device; runs on one device are serialized by the pool's mutex). Slot 0 serves one
+/// result at a time; an update of several results gives result i slot i % kWorkspaceSlots, so
+/// those results keep their own packed words from one update to the next and two of their runs
+/// can be in flight at once (on the caller's stream and the device's second stream).
+constexpr int kWorkspaceSlots = 4;
+
 struct workspace_pool {
   std::mutex lock;
-  std::map<int, dyng_hip_workspace*> per_device;
-  dyng_hip_workspace* get(int device) {
-    auto it = per_device.find(device);
-    if (it != per_device.end()) return it->second;
+  std::map<std::pair<int, int>, dyng_hip_workspace*> per_slot;
+  std::map<int, hipStream_t> second_stream;
+  dyng_hip_workspace* get(int device, int slot = 0) {
+    auto it = per_slot.find({device, slot});
+    if (it != per_slot.end()) return it->second;
     dyng_hip_workspace* ws = nullptr;
     if (dyng_hip_workspace_create(device, &ws) != DYNG_HIP_OK) fail(DYNG_HIP_OUT_OF_MEMORY, dyng_hip_last_error());
-    per_device[device] = ws;
+    per_slot[{device, slot}] = ws;
     return ws;
+  }
+  /// A second stream of the device (default priority; non-blocking, so that it runs beside a
+  /// caller's work on the null stream), joined to the caller's stream by events.
+  hipStream_t stream2(int device) {
+    auto it = second_stream.find(device);
+    if (it != second_stream.end()) return it->second;
+    hipStream_t st = nullptr;
+    DYNG_HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    second_stream[device] = st;
+    return st;
   }
 };
 
@@ -542,21 +558,74 @@ extern "C" int dyng_hip_api_update_each(dyng_hip_graph* g, const dyng_hip_batch*
     // ---- every result repaired on G_{t+1}; a failing one is poisoned, the others still brought
     // up to date, and the first error reported ----
     std::exception_ptr first;
-    for (int64_t i = 0; i < count; ++i) {
+    if (count == 1) {
       dyng_hip_counters c{};
       double engine_ms = 0.0;
       try {
-        repair_core(g, results[i], delta[i], max_weight[i], s, c, engine_ms);
+        repair_core(g, results[0], delta[0], max_weight[0], s, c, engine_ms);
+        if (stats != nullptr) {
+          std::memset(&stats[0], 0, sizeof(stats[0]));
+          fill_stats(&stats[0], c);
+          stats[0].batch = summary;
+          stats[0].engine_ms = engine_ms;
+        }
       } catch (...) {
-        if (!first) first = std::current_exception();
-        continue;
+        first = std::current_exception();
       }
-      if (stats != nullptr) {
-        std::memset(&stats[i], 0, sizeof(stats[i]));
-        fill_stats(&stats[i], c);
-        stats[i].batch = summary;
-        stats[i].engine_ms = engine_ms;
+    } else {
+      // several results: result i runs on workspace slot i % kWorkspaceSlots, the even ones on the
+      // caller's stream and the odd ones on the device's second stream, so two persistent
+      // launches are in flight at a time; each stream's runs follow one another
+      for (int64_t i = 0; i < count; ++i) grow_result(results[i], g->n, s);
+      std::lock_guard<std::mutex> lk(pool().lock);
+      hipStream_t s2 = pool().stream2(g->device);
+      event_pair ev_join;
+      DYNG_HIP_TRY(hipEventRecord(ev_join.a, s));  // the apply and the growth are done
+      DYNG_HIP_TRY(hipStreamWaitEvent(s2, ev_join.a, 0));
+      std::vector<engine_pending> pend(static_cast<size_t>(count));
+      std::vector<char> ok(static_cast<size_t>(count), 0);
+      auto finish = [&](int64_t i) {
+        if (!ok[i]) return;
+        ok[i] = 0;
+        dyng_hip_result* r = results[i];
+        dyng_hip_counters c{};
+        try {
+          engine_finish(pend[i], &c);
+        } catch (...) {
+          r->poisoned = true;
+          if (!first) first = std::current_exception();
+          return;
+        }
+        r->tree_token = c.tree_token;
+        r->version = g->version;
+        r->trusted = true;
+        r->engine_tree = true;
+        if (stats != nullptr) {
+          std::memset(&stats[i], 0, sizeof(stats[i]));
+          fill_stats(&stats[i], c);
+          stats[i].batch = summary;
+          stats[i].engine_ms = c.kernel_ms;
+        }
+      };
+      for (int64_t i = 0; i < count; ++i) {
+        // a workspace holds one run at a time: the run kWorkspaceSlots back used this slot
+        if (i >= kWorkspaceSlots) finish(i - kWorkspaceSlots);
+        dyng_hip_result* r = results[i];
+        try {
+          dyng_hip_sssp_run run = make_run(g, r, delta[i], max_weight[i]);
+          run.workspace = pool().get(g->device, static_cast<int>(i % kWorkspaceSlots));
+          r->tree_token = 0;
+          pend[i] = engine_launch(&run, true, (i % 2 == 0) ? s : s2);
+          ok[i] = 1;
+        } catch (...) {
+          r->poisoned = true;
+          if (!first) first = std::current_exception();
+        }
       }
+      for (int64_t i = std::max<int64_t>(0, count - kWorkspaceSlots); i < count; ++i) finish(i);
+      // the caller's stream does not run ahead of the second stream's runs
+      DYNG_HIP_TRY(hipEventRecord(ev_join.b, s2));
+      DYNG_HIP_TRY(hipStreamWaitEvent(s, ev_join.b, 0));
     }
     if (stats != nullptr) {
       DYNG_HIP_TRY(hipEventSynchronize(ev_apply.b));

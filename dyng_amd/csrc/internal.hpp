@@ -172,6 +172,27 @@ struct dyng_hip_graph;
 struct dyng_hip_mlp;
 namespace dyng_hip {
 void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch_scan& sc);
+
+/// One engine run in flight: engine_launch queues it on `s` (the checks, the launch and the
+/// counters' read-back, nothing waited for), engine_finish waits for it and reports (or throws).
+/// dyng_hip_sssp_update / _compute are the two back to back; runs on different workspaces and
+/// streams may be in flight together (each persistent launch takes one workgroup per CU, half of
+/// what fits, so two are resident at once).
+struct engine_pending {
+  dyng_hip_workspace* ws = nullptr;
+  hipStream_t s = nullptr;
+  bool launched = false;
+  bool update = false;
+  int bits = 0;
+  int blocks = 0;
+  int resident = 0;
+  int64_t n = 0;
+  int64_t timeout_us = 0;
+  int64_t* distances = nullptr;
+  dyng_hip_counters counters{};
+};
+engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s);
+void engine_finish(engine_pending& pend, dyng_hip_counters* out);
 /// The scheduler's pick for a scanned batch on G_t: features, BF16 inference, argmin (one sync).
 int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents, int64_t n_tree,
                int64_t last_invalidated, int64_t last_affected, hipStream_t s);

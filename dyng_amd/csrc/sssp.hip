@@ -1113,19 +1113,27 @@ int bits_for_parents(int64_t n) {
   return bits;
 }
 
-void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_hip_counters* out) {
+}  // namespace
+
+engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s) {
   if (run == nullptr || run->graph == nullptr || run->workspace == nullptr) {
     fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: sssp run: graph and workspace are required");
   }
   const dyng_hip_graph* g = run->graph;
   dyng_hip_workspace* ws = run->workspace;
   const int64_t n = g->n;
-  dyng_hip_counters counters{};
+  engine_pending pend{};
+  pend.ws = ws;
+  pend.s = s;
+  pend.update = update;
+  pend.n = n;
+  pend.distances = run->distances;
+  dyng_hip_counters& counters = pend.counters;
   counters.engine_used = DYNG_HIP_ENGINE_FUSED;
   counters.packed_parents = 1;
   if (n == 0) {
-    if (out) *out = counters;
-    return;
+    pend.launched = false;
+    return pend;
   }
   if (run->delta <= 0) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: the near-far width delta must be > 0, got " + std::to_string(run->delta));
   if (run->source == -1) {
@@ -1290,6 +1298,26 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
     DYNG_HIP_TRY(hipEventRecord(ws->ev_stop, s));
     DYNG_HIP_TRY(hipMemcpyAsync(hc, ws->ctl.p, sizeof(control), hipMemcpyDeviceToHost, s));
   }
+  pend.launched = true;
+  pend.bits = pk.bits;
+  pend.blocks = blocks;
+  pend.resident = p.resident;
+  pend.timeout_us = timeout_us;
+  return pend;
+}
+
+void engine_finish(engine_pending& pend, dyng_hip_counters* out) {
+  dyng_hip_counters counters = pend.counters;
+  if (!pend.launched) {
+    if (out) *out = counters;
+    return;
+  }
+  pend.launched = false;
+  dyng_hip_workspace* ws = pend.ws;
+  hipStream_t s = pend.s;
+  const bool update = pend.update;
+  const int64_t timeout_us = pend.timeout_us;
+  control* hc = ws->host_ctl.p;
   DYNG_HIP_TRY(hipStreamSynchronize(s));
   float kernel_ms = 0.0f;
   DYNG_HIP_TRY(hipEventElapsedTime(&kernel_ms, ws->ev_start, ws->ev_stop));
@@ -1330,7 +1358,7 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   counters.epochs = hc->epochs;
   counters.pushes = hc->pushes;
   counters.affected = static_cast<int64_t>(hc->affected);
-  counters.packed_parents = pk.bits > 0 ? 1 : 0;
+  counters.packed_parents = pend.bits > 0 ? 1 : 0;
   counters.relax_vertices = hc->pushes;
   counters.relax_edges = static_cast<int64_t>(hc->relax_edges);
   counters.relax_improved = static_cast<int64_t>(hc->relax_improved);
@@ -1351,19 +1379,26 @@ void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_h
   counters.pack_ms = update && ta >= t0 ? (ta - t0) * tick_ms : 0.0;
   counters.jump_ms = update && tb >= ta ? (tb - ta) * tick_ms : 0.0;
   counters.unpack_ms = t7 >= t3 ? (t7 - t3) * tick_ms : 0.0;
-  counters.blocks = blocks;
+  counters.blocks = pend.blocks;
   counters.kernel_ms = kernel_ms;
-  counters.words_kept = p.resident;
-  if (pk.bits > 0) {
+  counters.words_kept = pend.resident;
+  if (pend.bits > 0) {
     // `packed` now holds the words of the tree the run left: hand out a token for them
     static std::atomic<uint64_t> next_token{0};
     ws->words_token = ++next_token;
-    ws->words_n = n;
-    ws->words_bits = pk.bits;
-    ws->words_distances = run->distances;
+    ws->words_n = pend.n;
+    ws->words_bits = pend.bits;
+    ws->words_distances = pend.distances;
     counters.tree_token = ws->words_token;
   }
   if (out) *out = counters;
+}
+
+namespace {
+
+void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_hip_counters* out) {
+  engine_pending pend = engine_launch(run, update, s);
+  engine_finish(pend, out);
 }
 
 }  // namespace

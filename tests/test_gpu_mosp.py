@@ -289,3 +289,30 @@ def test_update_checks_everything_before_the_batch(dg):
     st = dg.update(g, batch, a, b)
     assert g.version == 2 and g.num_edges == 3 and len(st) == 2
     assert a.distances.tolist() == [0, 1, 2] and b.distances.tolist()[1] == 0
+
+
+def test_update_many_results_reuse_workspaces(dg):
+    """Nine results (more than the workspace slots: runs of one slot follow one another, two are
+    in flight at once) through three batches: each tree equals Dijkstra's on the new graph."""
+    _say("[update_each] nine results")
+    rng = np.random.default_rng(91)
+    n, m, K = 2500, 15000, 3
+    src, dst, w = _random_graph(rng, n, m, K, 25)
+    op = oracle.Props()
+    og = oracle.build_from_edges(n, src, dst, w, op)
+    g = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    picks = [(int(rng.integers(0, n)), int(rng.integers(0, K))) for _ in range(9)]
+    trees = [dg.sssp.compute(g, s, objective=k) for s, k in picks]
+    for b in range(3):
+        ni, nd = 300, 300
+        ins_s, ins_d = rng.integers(0, n, ni), rng.integers(0, n, ni)
+        ins_w = rng.integers(1, 26, (ni, K))
+        rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+        pick = rng.integers(0, og.m, nd)
+        del_s, del_d = rows[pick], og.col[pick]
+        stats = dg.update(g, dg.EdgeBatch(insert=(ins_s, ins_d, ins_w), delete=(del_s, del_d)), *trees)
+        og = oracle.apply_batch(og, ins_s, ins_d, ins_w, del_s, del_d, op).g
+        for t, (s, k), st in zip(trees, picks, stats):
+            d, p = oracle.dijkstra(og, s, k)
+            assert np.array_equal(t.distances, d) and np.array_equal(t.parents, p), (b, s, k)
+            assert t.graph_version == g.version and st.converged
