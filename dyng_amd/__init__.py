@@ -1,9 +1,12 @@
 """dyng_amd: an MI355X-native implementation of dynG's batch-dynamic SSSP path.
 
 ``dyng_amd.sssp.compute`` / ``dyng_amd.sssp.update`` keep the reference's Python API
-(python/dyng/sssp.py) on a device-resident graph; the work runs in libdyng_hip.so (hand-written
-HIP for gfx950). There is no CPU fallback: without the built library every call raises
-NotSupportedError.
+(python/dyng/sssp.py) on a device-resident graph; ``dyng_amd.update`` applies one batch for
+several results (python/dyng/_update.py), ``dyng_amd.mosp`` runs MOSP's combined solve on the same
+engine (python/dyng/mosp.py), ``dyng_amd.io`` reads and writes MOSP's files and
+``python -m dyng_amd`` is the ``sssp`` / ``mosp`` command line. The work runs in libdyng_hip.so
+(hand-written HIP for gfx950). There is no CPU fallback: without the built library every call
+raises NotSupportedError.
 """
 
 from . import batched, io, mosp, scheduler, sssp
