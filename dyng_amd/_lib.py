@@ -114,7 +114,8 @@ EXPORTS = (
     "dyng_hip_api_repair", "dyng_hip_batch_features", "dyng_hip_mlp_create", "dyng_hip_mlp_destroy",
     "dyng_hip_mlp_get_params", "dyng_hip_mlp_set_params", "dyng_hip_mlp_gradients", "dyng_hip_mlp_train",
     "dyng_hip_mlp_predict", "dyng_hip_result_equal", "dyng_hip_api_update_scheduled",
-    "dyng_hip_mosp_combined", "dyng_hip_api_update_each",
+    "dyng_hip_mosp_combined", "dyng_hip_api_update_each", "dyng_hip_mosp_create", "dyng_hip_mosp_update",
+    "dyng_hip_mosp_arrays", "dyng_hip_mosp_path_costs", "dyng_hip_mosp_destroy",
 )
 
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_TRAINED, MLP_PARAMS = 12, 64, 5, 5317, 5341
@@ -127,7 +128,8 @@ class BatchStats(C.Structure):
 
 class MospStats(C.Structure):
     _fields_ = [("combined_edges", C.c_int64), ("weight_sum", C.c_int64), ("preference_scale", C.c_int64),
-                ("base", C.c_int64), ("affected", C.c_int64), ("solve", Counters)]
+                ("base", C.c_int64), ("affected", C.c_int64), ("deletions", C.c_int64), ("insertions", C.c_int64),
+                ("solve", Counters)]
 
 
 _lib = None
@@ -167,6 +169,11 @@ def lib() -> C.CDLL:
     L.dyng_hip_api_compute_multi.argtypes = [vp, vp, C.c_int64, C.POINTER(Options), vp, C.POINTER(vp)]
     L.dyng_hip_api_repair.argtypes = [vp, vp, vp, C.POINTER(Stats)]
     L.dyng_hip_api_update_each.argtypes = [vp, C.POINTER(Batch), vp, C.c_int64, vp, vp]
+    L.dyng_hip_mosp_create.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, C.POINTER(vp), C.POINTER(MospStats)]
+    L.dyng_hip_mosp_update.argtypes = [vp, vp, vp, vp, C.POINTER(MospStats)]
+    L.dyng_hip_mosp_arrays.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_int64)]
+    L.dyng_hip_mosp_path_costs.argtypes = [vp, vp, C.c_int32, vp, vp]
+    L.dyng_hip_mosp_destroy.argtypes = [vp]
     L.dyng_hip_mosp_combined.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, C.c_int32, vp, vp, C.c_int32, vp, vp,
                                          C.POINTER(MospStats)]
     L.dyng_hip_batch_features.argtypes = [vp, C.POINTER(Batch), vp, C.c_int64, vp, C.POINTER(BatchStats)]

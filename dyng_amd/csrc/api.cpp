@@ -999,53 +999,74 @@ extern "C" int dyng_hip_synchronize(int device) {
 // MOSP's combined solve (cpp/src/algorithms/mosp/mosp.cpp:264-330)
 // ---------------------------------------------------------------------------------------------
 
+namespace {
+
+/// MOSP's checks of the trees and the preferences, and the combined weights' terms
+/// (preferenceScale, mosp.cpp:114-131: L = lcm(preferences) <= 2^20; terms L / pref[k]; base
+/// L (K + 1)).
+struct mosp_terms {
+  int64_t scale = 1;
+  int64_t base = 0;
+  std::vector<int32_t> terms;
+};
+
+mosp_terms mosp_check(const dyng_hip_graph* g, int32_t source, int32_t num_objectives,
+                      const int32_t* const* tree_parents, const int32_t* preferences) {
+  if (g == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: null graph");
+  const int64_t n = g->n;
+  if (num_objectives < 1 || num_objectives > 64) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: 1 to 64 objectives, got " + std::to_string(num_objectives));
+  }
+  if (tree_parents == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the trees are required");
+  for (int k = 0; k < num_objectives; ++k) {
+    if (tree_parents[k] == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: tree " + std::to_string(k) + " is null");
+  }
+  if (n > 0 && (source < 0 || source >= n)) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: source " + std::to_string(source) + " is out of range [0, " +
+                                        std::to_string(n) + ")");
+  }
+  mosp_terms t;
+  if (preferences != nullptr) {
+    for (int k = 0; k < num_objectives; ++k) {
+      const int64_t pref = preferences[k];
+      if (pref < 1) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: preferences[" + std::to_string(k) + "] is " +
+                                                        std::to_string(pref) + "; a preference must be >= 1");
+      int64_t a = t.scale, b = pref;
+      while (b != 0) {
+        const int64_t r = a % b;
+        a = b;
+        b = r;
+      }
+      t.scale = t.scale / a * pref;
+      if (t.scale > (int64_t{1} << 20)) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: lcm(preferences) exceeds 2^20");
+    }
+  }
+  t.terms.resize(static_cast<size_t>(num_objectives));
+  for (int k = 0; k < num_objectives; ++k) {
+    t.terms[k] = static_cast<int32_t>(preferences != nullptr ? t.scale / preferences[k] : t.scale);
+  }
+  t.base = t.scale * (num_objectives + 1);
+  return t;
+}
+
+}  // namespace
+
 extern "C" int dyng_hip_mosp_combined(const dyng_hip_graph* g, int32_t source, int32_t num_objectives,
                                       const int32_t* const* tree_parents, const int32_t* preferences,
                                       int32_t count_changes, int64_t* distances, int32_t* parents,
                                       int32_t cost_columns, int64_t* path_costs, void* stream,
                                       dyng_hip_mosp_stats* out) {
   return guarded([&] {
-    if (g == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: null graph");
+    const mosp_terms mt = mosp_check(g, source, num_objectives, tree_parents, preferences);
     const int64_t n = g->n;
-    if (num_objectives < 1 || num_objectives > 64) {
-      fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: 1 to 64 objectives, got " + std::to_string(num_objectives));
-    }
-    if (tree_parents == nullptr || distances == nullptr || parents == nullptr) {
-      fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the trees and the output arrays are required");
-    }
-    for (int k = 0; k < num_objectives; ++k) {
-      if (tree_parents[k] == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: tree " + std::to_string(k) + " is null");
-    }
-    if (n > 0 && (source < 0 || source >= n)) {
-      fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: source " + std::to_string(source) + " is out of range [0, " +
-                                          std::to_string(n) + ")");
-    }
+    if (distances == nullptr || parents == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the output arrays are required");
     if (cost_columns < 0 || cost_columns > g->K || (cost_columns > 0 && path_costs == nullptr)) {
       fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: cost_columns must be in [0, " + std::to_string(g->K) +
                                           "] with an output array");
     }
-    // preferenceScale (mosp.cpp:114-131): L = lcm(preferences) <= 2^20; terms L / pref[k]
-    int64_t scale = 1;
-    if (preferences != nullptr) {
-      for (int k = 0; k < num_objectives; ++k) {
-        const int64_t pref = preferences[k];
-        if (pref < 1) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: preferences[" + std::to_string(k) + "] is " +
-                                                          std::to_string(pref) + "; a preference must be >= 1");
-        int64_t a = scale, b = pref;
-        while (b != 0) {
-          const int64_t t = a % b;
-          a = b;
-          b = t;
-        }
-        scale = scale / a * pref;
-        if (scale > (int64_t{1} << 20)) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: lcm(preferences) exceeds 2^20");
-      }
-    }
-    std::vector<int32_t> terms(static_cast<size_t>(num_objectives));
-    for (int k = 0; k < num_objectives; ++k) {
-      terms[k] = static_cast<int32_t>(preferences != nullptr ? scale / preferences[k] : scale);
-    }
-    const int64_t base = scale * (num_objectives + 1);
+    const int64_t scale = mt.scale;
+    const std::vector<int32_t>& terms = mt.terms;
+    const int64_t base = mt.base;
     dyng_hip_mosp_stats st{};
     st.preference_scale = scale;
     st.base = base;
@@ -1089,4 +1110,191 @@ extern "C" int dyng_hip_mosp_combined(const dyng_hip_graph* g, int32_t source, i
     DYNG_HIP_TRY(hipStreamSynchronize(s));
     if (out != nullptr) *out = st;
   });
+}
+
+// ---- MOSP's combined state kept on the device: the combined graph and its tree ----
+
+struct dyng_hip_mosp {
+  int device = 0;
+  int32_t source = 0;
+  int32_t K = 0;
+  mosp_terms terms;
+  dyng_hip_graph* cg = nullptr;   ///< the combined graph (a graph of its own, one weight column)
+  dyng_hip_result* cr = nullptr;  ///< its canonical tree from the source
+  int64_t n = 0;                  ///< vertices of the trees it was last built from
+  int64_t cap = 0;                ///< vertices `prev` holds per tree
+  dyng_hip::dbuf<int32_t> prev;   ///< those trees' parents, tree k at k * cap
+  dyng_hip::dbuf<const int32_t*> old_ptrs, new_ptrs;
+  dyng_hip::dbuf<int32_t> dterms;
+  dyng_hip::dbuf<int32_t> l0, l1, l2, l3, l4;  ///< the edge list, then the batches
+  dyng_hip::dbuf<unsigned long long> totals;
+  dyng_hip_mosp() = default;
+  dyng_hip_mosp(const dyng_hip_mosp&) = delete;
+  dyng_hip_mosp& operator=(const dyng_hip_mosp&) = delete;
+  ~dyng_hip_mosp() {
+    if (cr != nullptr) (void)dyng_hip_result_destroy(cr);
+    if (cg != nullptr) (void)dyng_hip_graph_destroy(cg);
+  }
+};
+
+namespace {
+
+/// Keep the trees' parents for the next diff.
+void mosp_save_trees(dyng_hip_mosp* m, const int32_t* const* tree_parents, int64_t n, hipStream_t s) {
+  if (n > m->cap) {
+    m->cap = std::max<int64_t>(n, m->cap + m->cap / 2);
+    m->prev.reserve(static_cast<size_t>(m->cap) * m->K);
+  }
+  for (int k = 0; k < m->K; ++k) {
+    if (n > 0) {
+      DYNG_HIP_TRY(hipMemcpyAsync(m->prev.p + static_cast<int64_t>(k) * m->cap, tree_parents[k], n * sizeof(int32_t),
+                                  hipMemcpyDeviceToDevice, s));
+    }
+  }
+  m->n = n;
+}
+
+void mosp_stats_of(const dyng_hip_mosp* m, dyng_hip_mosp_stats* out) {
+  out->preference_scale = m->terms.scale;
+  out->base = m->terms.base;
+  if (m->cg != nullptr) {
+    out->combined_edges = m->cg->m;
+    out->weight_sum = m->cg->wsum.empty() ? 0 : m->cg->wsum[0];
+  }
+}
+
+}  // namespace
+
+extern "C" int dyng_hip_mosp_create(const dyng_hip_graph* g, int32_t source, int32_t num_objectives,
+                                    const int32_t* const* tree_parents, const int32_t* preferences, void* stream,
+                                    dyng_hip_mosp** out, dyng_hip_mosp_stats* stats) {
+  return guarded([&] {
+    if (out == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: no output handle");
+    auto m = std::make_unique<dyng_hip_mosp>();
+    m->terms = mosp_check(g, source, num_objectives, tree_parents, preferences);
+    m->device = g->device;
+    m->source = source;
+    m->K = num_objectives;
+    const int64_t n = g->n;
+    scoped_device guard(g->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    m->dterms.reserve(num_objectives);
+    m->new_ptrs.reserve(num_objectives);
+    m->old_ptrs.reserve(num_objectives);
+    m->totals.reserve(4);
+    DYNG_HIP_TRY(hipStreamSynchronize(s));  // the trees are final before the pointer copies below
+    DYNG_HIP_TRY(hipMemcpy(m->dterms.p, m->terms.terms.data(), num_objectives * sizeof(int32_t), hipMemcpyHostToDevice));
+    DYNG_HIP_TRY(hipMemcpy(m->new_ptrs.p, tree_parents, num_objectives * sizeof(const int32_t*), hipMemcpyHostToDevice));
+    const int64_t list_cap = std::max<int64_t>(1, n * num_objectives);
+    m->l0.reserve(list_cap);
+    m->l1.reserve(list_cap);
+    m->l2.reserve(list_cap);
+    int64_t weight_sum = 0;
+    const int64_t edges =
+        n > 0 ? mosp_combined_list(m->new_ptrs.p, num_objectives, n, source, static_cast<int32_t>(m->terms.base),
+                                   m->dterms.p, m->l0.p, m->l1.p, m->l2.p, m->totals.p, s, &weight_sum)
+              : 0;
+    // a simple graph with sorted rows that keeps MOSP's batch rules (upserts, deletions first) and
+    // grows with the trees
+    const dyng_hip_graph_props props{1, 1, 0, 0, 0, 0, 1, 1, 0.25};
+    call(dyng_hip_graph_create_from_edges(g->device, &props, n, edges, 1, m->l0.p, m->l1.p, m->l2.p, 1, s, &m->cg));
+    if (n > 0) {
+      const dyng_hip_options opt{0, 0, DYNG_HIP_ENGINE_AUTOMATIC, 1};
+      call(dyng_hip_api_compute(m->cg, source, &opt, s, &m->cr));
+    }
+    mosp_save_trees(m.get(), tree_parents, n, s);
+    DYNG_HIP_TRY(hipStreamSynchronize(s));
+    if (stats != nullptr) {
+      *stats = dyng_hip_mosp_stats{};
+      mosp_stats_of(m.get(), stats);
+      stats->weight_sum = weight_sum;
+    }
+    *out = m.release();
+  });
+}
+
+extern "C" int dyng_hip_mosp_update(dyng_hip_mosp* m, const dyng_hip_graph* g, const int32_t* const* tree_parents,
+                                    void* stream, dyng_hip_mosp_stats* stats) {
+  return guarded([&] {
+    if (m == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: null state");
+    (void)mosp_check(g, m->source, m->K, tree_parents, nullptr);
+    if (g->device != m->device) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the graph is on another device");
+    const int64_t n_old = m->n, n_new = g->n;
+    if (n_new < n_old) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the trees have fewer vertices than before");
+    scoped_device guard(g->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dyng_hip_mosp_stats st{};
+    if (n_new == 0) {
+      mosp_stats_of(m, &st);
+      if (stats != nullptr) *stats = st;
+      return;
+    }
+    std::vector<const int32_t*> olds(static_cast<size_t>(m->K));
+    for (int k = 0; k < m->K; ++k) olds[k] = m->prev.p + static_cast<int64_t>(k) * m->cap;
+    DYNG_HIP_TRY(hipStreamSynchronize(s));  // the repaired trees are final; the pointer arrays are free
+    DYNG_HIP_TRY(hipMemcpy(m->new_ptrs.p, tree_parents, m->K * sizeof(const int32_t*), hipMemcpyHostToDevice));
+    DYNG_HIP_TRY(hipMemcpy(m->old_ptrs.p, olds.data(), m->K * sizeof(const int32_t*), hipMemcpyHostToDevice));
+    const int64_t ops_cap = std::max<int64_t>(1, std::max(n_old, n_new) * m->K);
+    for (auto* b : {&m->l0, &m->l1, &m->l2, &m->l3, &m->l4}) b->reserve(ops_cap);
+    int64_t nd = 0, ni = 0;
+    mosp_combined_diff(m->old_ptrs.p, n_old, m->new_ptrs.p, n_new, m->K, m->source,
+                       static_cast<int32_t>(m->terms.base), m->dterms.p, m->l0.p, m->l1.p, m->l2.p, m->l3.p, m->l4.p,
+                       m->totals.p, s, &nd, &ni);
+    if (m->cr == nullptr) {  // the first trees had no vertex: build as on create
+      fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the combined state was created on an empty graph; create it again");
+    }
+    // vertices the trees gained: rows of their own in the combined graph even before an edge
+    // names them (a new vertex may stay unreached)
+    if (n_new > m->cg->n) {
+      graph_grow(m->cg, n_new, s);
+      grow_result(m->cr, n_new, s);
+    }
+    // the change between the two combined graphs, applied and repaired like any batch
+    dyng_hip_batch b{m->l2.p, m->l3.p, m->l4.p, ni, m->l0.p, m->l1.p, nd, 1, 1};
+    dyng_hip_stats ss{};
+    call(dyng_hip_api_update(m->cg, &b, m->cr, s, &ss));
+    mosp_save_trees(m, tree_parents, n_new, s);
+    DYNG_HIP_TRY(hipStreamSynchronize(s));
+    mosp_stats_of(m, &st);
+    st.affected = ss.affected;
+    st.deletions = nd;
+    st.insertions = ni;
+    st.solve = ss.device;
+    if (stats != nullptr) *stats = st;
+  });
+}
+
+extern "C" int dyng_hip_mosp_arrays(const dyng_hip_mosp* m, const int64_t** distances, const int32_t** parents,
+                                    int64_t* n) {
+  return guarded([&] {
+    if (m == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: null state");
+    if (m->cr == nullptr) {
+      if (distances) *distances = nullptr;
+      if (parents) *parents = nullptr;
+      if (n) *n = 0;
+      return;
+    }
+    if (distances) *distances = m->cr->distances.p;
+    if (parents) *parents = m->cr->parents.p;
+    if (n) *n = m->cr->n;
+  });
+}
+
+extern "C" int dyng_hip_mosp_path_costs(const dyng_hip_mosp* m, const dyng_hip_graph* g, int32_t columns,
+                                        int64_t* costs, void* stream) {
+  return guarded([&] {
+    if (m == nullptr || g == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: null state or graph");
+    if (columns < 1 || columns > g->K || costs == nullptr) {
+      fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: columns must be in [1, " + std::to_string(g->K) + "] with an output array");
+    }
+    if (m->cr == nullptr || g->n == 0) return;
+    if (m->cr->n != g->n) fail(DYNG_HIP_INVALID_ARGUMENT, "mosp: the combined tree does not match the graph");
+    scoped_device guard(g->device);
+    std::lock_guard<std::mutex> lk(pool().lock);  // the doubling scratch is the device's cache
+    mosp_path_costs(g, m->cr->parents.p, g->n, m->source, columns, costs, static_cast<hipStream_t>(stream));
+  });
+}
+
+extern "C" int dyng_hip_mosp_destroy(dyng_hip_mosp* m) {
+  return guarded([&] { delete m; });
 }

@@ -1772,6 +1772,13 @@ void stage_times(dyng_hip_graph* g, bool rows, dyng_hip_apply_summary* out) {
 
 }  // namespace
 
+void graph_grow(dyng_hip_graph* g, int64_t n, hipStream_t s) {
+  if (n <= g->n) return;
+  if (n >= INT32_MAX) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: the HIP backend takes int32 vertex ids");
+  grow_vertices(g, n, s);
+  g->n = n;
+}
+
 void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch_scan& sc) {
   if (b == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: graph apply: no batch");
   mark_stage(g, 0, s);

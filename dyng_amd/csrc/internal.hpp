@@ -208,6 +208,19 @@ int mlp_pick(const dyng_hip_mlp* m);
 dyng_hip_graph* mosp_build_combined(int device, const int32_t* const* parents, int32_t num_objectives, int64_t n,
                                     int32_t source, int32_t base, const int32_t* terms, hipStream_t s,
                                     int64_t* edges, int64_t* weight_sum);
+/// Grow the graph to n vertices without edges (new rows empty; no batch, no new version).
+void graph_grow(dyng_hip_graph* g, int64_t n, hipStream_t s);
+/// The combined graph of K trees (device pointer array `parents_dev`, device terms) as an edge
+/// list into src / dst / w (capacity n * K); returns the edge count (and the weight sum).
+int64_t mosp_combined_list(const int32_t* const* parents_dev, int32_t num_objectives, int64_t n, int32_t source,
+                           int32_t base, const int32_t* terms_dev, int32_t* src, int32_t* dst, int32_t* w,
+                           unsigned long long* totals, hipStream_t s, int64_t* weight_sum);
+/// The batch from the combined graph of the old trees to that of the new ones (device pointer
+/// arrays; capacity max(old_n, new_n) * K per list): deletions, then upserts.
+void mosp_combined_diff(const int32_t* const* old_parents, int64_t old_n, const int32_t* const* new_parents,
+                        int64_t new_n, int32_t num_objectives, int32_t source, int32_t base, const int32_t* terms_dev,
+                        int32_t* del_src, int32_t* del_dst, int32_t* ins_src, int32_t* ins_dst, int32_t* ins_w,
+                        unsigned long long* totals, hipStream_t s, int64_t* num_del, int64_t* num_ins);
 /// The first `columns` objective costs of every vertex's path in the tree `parents` over the
 /// graph's out-rows (device, n x columns, vertex-major); throws if a tree edge is missing.
 /// Callers hold pool().lock (the doubling scratch is the per-device cache's).

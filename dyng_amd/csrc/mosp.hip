@@ -237,6 +237,231 @@ __global__ void k_costs_home(const int64_t* acc, int64_t* costs, int64_t count, 
   }
 }
 
+/// Counts of the threads of a workgroup turned into slots of a shared list: the workgroup's
+/// exclusive scan of `mine`, then one atomic per workgroup on `counter`. Every thread calls it;
+/// returns this thread's first slot. `wave_total` and `block_base` are LDS scratch.
+DYNG_DEV int block_slots(int mine, unsigned long long* counter, int* wave_total, int* block_base) {
+  int incl = mine;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(incl, off);
+    if (lane() >= off) incl += y;
+  }
+  const int w_id = static_cast<int>(threadIdx.x) / 64;
+  __syncthreads();
+  if (lane() == 63) wave_total[w_id] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int i = 0; i < kWaves; ++i) {
+      const int t = wave_total[i];
+      wave_total[i] = run;
+      run += t;
+    }
+    *block_base = run > 0 ? static_cast<int>(atomicAdd(counter, static_cast<unsigned long long>(run))) : 0;
+  }
+  __syncthreads();
+  return *block_base + wave_total[w_id] + incl - mine;
+}
+
+/// The combined graph as an edge list (src, dst, weight), one thread per vertex, with its weight
+/// sum: the input of the kept combined graph's first build.
+__global__ void __launch_bounds__(kBlock) k_combine_list(combine_params c, int32_t* src, int32_t* dst, int32_t* w,
+                                                         unsigned long long* totals) {
+  __shared__ int wave_total[kWaves];
+  __shared__ int block_base;
+  __shared__ unsigned long long part[kWaves];
+  const int32_t v = static_cast<int32_t>(blockIdx.x * kBlock + threadIdx.x);
+  const bool live = v < c.n && v != c.source;
+  int mine = 0;
+  unsigned long long wsum = 0;
+  if (live) {
+    for (int k = 0; k < c.K; ++k) {
+      int32_t p, x;
+      if (combined_edge(c, v, k, p, x)) {
+        ++mine;
+        wsum += static_cast<unsigned long long>(x);
+      }
+    }
+  }
+  int slot = block_slots(mine, &totals[0], wave_total, &block_base);
+  if (mine > 0) {
+    for (int k = 0; k < c.K; ++k) {
+      int32_t p, x;
+      if (combined_edge(c, v, k, p, x)) {
+        src[slot] = p;
+        dst[slot] = v;
+        w[slot] = x;
+        ++slot;
+      }
+    }
+  }
+  wsum = block_reduce(wsum, part, op_sum{});
+  if (threadIdx.x == 0 && wsum) atomicAdd(&totals[1], wsum);
+}
+
+/// Up to kRegTrees trees: a vertex's K parents and the K terms held in registers (static
+/// indices only), so each parent is read once instead of once per comparison.
+constexpr int kRegTrees = 8;
+
+struct reg_trees {
+  int32_t p[kRegTrees];
+  int32_t t[kRegTrees];
+};
+
+DYNG_DEV void load_trees(const combine_params& c, int32_t v, bool live, reg_trees& r) {
+#pragma unroll
+  for (int k = 0; k < kRegTrees; ++k) {
+    r.p[k] = (live && k < c.K) ? c.parents[k][v] : -1;
+    r.t[k] = k < c.K ? c.terms[k] : 0;
+  }
+}
+
+/// combined_edge over the registers: objective k's edge (p, v) if it is the first objective with
+/// that parent, weighing base less the terms of every objective holding it.
+DYNG_DEV bool reg_edge(const reg_trees& r, int32_t base, int k, int32_t& p, int32_t& w) {
+  p = -1;
+  w = 0;
+#pragma unroll
+  for (int i = 0; i < kRegTrees; ++i) {
+    if (i == k) p = r.p[i];
+  }
+  if (p < 0) return false;
+  int32_t weight = base;
+#pragma unroll
+  for (int j = 0; j < kRegTrees; ++j) {
+    if (r.p[j] == p) {
+      if (j < k) return false;
+      weight -= r.t[j];
+    }
+  }
+  w = weight;
+  return true;
+}
+
+/// k_combine_diff for at most kRegTrees trees, from registers.
+__global__ void __launch_bounds__(kBlock) k_combine_diff_reg(combine_params old_c, combine_params new_c,
+                                                             int32_t* del_src, int32_t* del_dst, int32_t* ins_src,
+                                                             int32_t* ins_dst, int32_t* ins_w,
+                                                             unsigned long long* totals) {
+  __shared__ int wave_total[kWaves];
+  __shared__ int block_base;
+  const int32_t v = static_cast<int32_t>(blockIdx.x * kBlock + threadIdx.x);
+  const bool has_old = v < old_c.n && v != old_c.source;
+  const bool has_new = v < new_c.n && v != new_c.source;
+  reg_trees ro, rn;
+  load_trees(old_c, v, has_old, ro);
+  load_trees(new_c, v, has_new, rn);
+  // an old edge whose parent no new tree gives v is deleted; a new edge is inserted unless the old
+  // trees gave the same (p, v) the same weight
+  unsigned del_mask = 0, ins_mask = 0;
+#pragma unroll
+  for (int k = 0; k < kRegTrees; ++k) {
+    int32_t p, x;
+    if (reg_edge(ro, old_c.base, k, p, x)) {
+      bool kept = false;
+#pragma unroll
+      for (int j = 0; j < kRegTrees; ++j) kept = kept || rn.p[j] == p;
+      if (!kept) del_mask |= 1u << k;
+    }
+    if (reg_edge(rn, new_c.base, k, p, x)) {
+      bool same = false;
+#pragma unroll
+      for (int j = 0; j < kRegTrees; ++j) {
+        int32_t po, xo;
+        same = same || (reg_edge(ro, old_c.base, j, po, xo) && po == p && xo == x);
+      }
+      if (!same) ins_mask |= 1u << k;
+    }
+  }
+  int del_at = block_slots(__popc(del_mask), &totals[2], wave_total, &block_base);
+  int ins_at = block_slots(__popc(ins_mask), &totals[3], wave_total, &block_base);
+#pragma unroll
+  for (int k = 0; k < kRegTrees; ++k) {
+    int32_t p, x;
+    if ((del_mask >> k) & 1u) {
+      reg_edge(ro, old_c.base, k, p, x);
+      del_src[del_at] = p;
+      del_dst[del_at] = v;
+      ++del_at;
+    }
+    if ((ins_mask >> k) & 1u) {
+      reg_edge(rn, new_c.base, k, p, x);
+      ins_src[ins_at] = p;
+      ins_dst[ins_at] = v;
+      ins_w[ins_at] = x;
+      ++ins_at;
+    }
+  }
+}
+
+/// Whether any of the K trees gives v the parent p.
+DYNG_DEV bool some_tree_has(const combine_params& c, int32_t v, int32_t p) {
+  for (int j = 0; j < c.K; ++j) {
+    if (c.parents[j][v] == p) return true;
+  }
+  return false;
+}
+
+/// The batch that turns the combined graph of the old trees into that of the new ones, one
+/// thread per vertex v: a combined in-edge (p, v) of the old trees that the new ones lack is
+/// deleted, one the new trees have is inserted (an upsert) unless the old trees gave it the same
+/// weight. Vertices past old.n had no combined edges. Each thread counts its operations, the
+/// workgroup claims their slots (totals[2], totals[3]: deletions, insertions), then the thread
+/// writes them.
+__global__ void __launch_bounds__(kBlock) k_combine_diff(combine_params old_c, combine_params new_c,
+                                                         int32_t* del_src, int32_t* del_dst, int32_t* ins_src,
+                                                         int32_t* ins_dst, int32_t* ins_w,
+                                                         unsigned long long* totals) {
+  __shared__ int wave_total[kWaves];
+  __shared__ int block_base;
+  const int32_t v = static_cast<int32_t>(blockIdx.x * kBlock + threadIdx.x);
+  const bool has_old = v < old_c.n && v != old_c.source;
+  const bool has_new = v < new_c.n && v != new_c.source;
+  int dels = 0, ins = 0;
+  auto each = [&](bool write, int del_at, int ins_at) {
+    if (has_old) {
+      for (int k = 0; k < old_c.K; ++k) {
+        int32_t p, x;
+        if (combined_edge(old_c, v, k, p, x) && !(has_new && some_tree_has(new_c, v, p))) {
+          if (write) {
+            del_src[del_at] = p;
+            del_dst[del_at] = v;
+          }
+          ++del_at;
+        }
+      }
+    }
+    if (has_new) {
+      for (int k = 0; k < new_c.K; ++k) {
+        int32_t p, x;
+        if (!combined_edge(new_c, v, k, p, x)) continue;
+        bool same = false;
+        if (has_old) {
+          for (int j = 0; j < old_c.K && !same; ++j) {
+            int32_t po, xo;
+            same = combined_edge(old_c, v, j, po, xo) && po == p && xo == x;
+          }
+        }
+        if (!same) {
+          if (write) {
+            ins_src[ins_at] = p;
+            ins_dst[ins_at] = v;
+            ins_w[ins_at] = x;
+          }
+          ++ins_at;
+        }
+      }
+    }
+    dels = del_at;
+    ins = ins_at;
+  };
+  each(false, 0, 0);
+  const int nd = dels, ni = ins;
+  const int del_at = block_slots(nd, &totals[2], wave_total, &block_base);
+  const int ins_at = block_slots(ni, &totals[3], wave_total, &block_base);
+  if (nd > 0 || ni > 0) each(true, del_at, ins_at);
+}
+
 unsigned int grid(int64_t work, int block) {
   return static_cast<unsigned int>(std::max<int64_t>(1, (work + block - 1) / block));
 }
@@ -348,6 +573,43 @@ dyng_hip_graph* mosp_build_combined(int device, const int32_t* const* parents, i
   }
   DYNG_HIP_TRY(hipGetLastError());
   return g;
+}
+
+int64_t mosp_combined_list(const int32_t* const* parents_dev, int32_t num_objectives, int64_t n, int32_t source,
+                           int32_t base, const int32_t* terms_dev, int32_t* src, int32_t* dst, int32_t* w,
+                           unsigned long long* totals, hipStream_t s, int64_t* weight_sum) {
+  hipLaunchKernelGGL(k_fill_value<unsigned long long>, dim3(1), dim3(64), 0, s, totals, int64_t{2}, 0ULL);
+  combine_params c{parents_dev, terms_dev, num_objectives, static_cast<int32_t>(n), source, base};
+  hipLaunchKernelGGL(k_combine_list, dim3(grid(n, kBlock)), dim3(kBlock), 0, s, c, src, dst, w, totals);
+  DYNG_HIP_TRY(hipGetLastError());
+  unsigned long long host[2] = {0, 0};
+  DYNG_HIP_TRY(hipMemcpyAsync(host, totals, sizeof(host), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  *weight_sum = static_cast<int64_t>(host[1]);
+  return static_cast<int64_t>(host[0]);
+}
+
+void mosp_combined_diff(const int32_t* const* old_parents, int64_t old_n, const int32_t* const* new_parents,
+                        int64_t new_n, int32_t num_objectives, int32_t source, int32_t base, const int32_t* terms_dev,
+                        int32_t* del_src, int32_t* del_dst, int32_t* ins_src, int32_t* ins_dst, int32_t* ins_w,
+                        unsigned long long* totals, hipStream_t s, int64_t* num_del, int64_t* num_ins) {
+  hipLaunchKernelGGL(k_fill_value<unsigned long long>, dim3(1), dim3(64), 0, s, totals, int64_t{4}, 0ULL);
+  combine_params oc{old_parents, terms_dev, num_objectives, static_cast<int32_t>(old_n), source, base};
+  combine_params nc{new_parents, terms_dev, num_objectives, static_cast<int32_t>(new_n), source, base};
+  const int64_t n = std::max(old_n, new_n);
+  if (num_objectives <= kRegTrees) {
+    hipLaunchKernelGGL(k_combine_diff_reg, dim3(grid(n, kBlock)), dim3(kBlock), 0, s, oc, nc, del_src, del_dst,
+                       ins_src, ins_dst, ins_w, totals);
+  } else {
+    hipLaunchKernelGGL(k_combine_diff, dim3(grid(n, kBlock)), dim3(kBlock), 0, s, oc, nc, del_src, del_dst, ins_src,
+                       ins_dst, ins_w, totals);
+  }
+  DYNG_HIP_TRY(hipGetLastError());
+  unsigned long long host[4] = {0, 0, 0, 0};
+  DYNG_HIP_TRY(hipMemcpyAsync(host, totals, sizeof(host), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  *num_del = static_cast<int64_t>(host[2]);
+  *num_ins = static_cast<int64_t>(host[3]);
 }
 
 void mosp_path_costs(const dyng_hip_graph* g, const int32_t* parents, int64_t n, int32_t source, int32_t columns,

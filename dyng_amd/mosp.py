@@ -11,7 +11,10 @@ repairs the K trees on the device and rebuilds the combined graph, the MOSP tree
 Names and behaviour follow the reference's python/dyng/mosp.py (Options :93-136, Stats :139-183,
 Result :186-403, compute :420-455, update :458-509) over the C++ path of
 cpp/src/algorithms/mosp/mosp.cpp:264-330: the combined graph and the path costs are built by the
-kernels of mosp.hip, the combined tree by the persistent sssp engine (``dyng_hip_mosp_combined``).
+kernels of mosp.hip, the combined tree by the persistent sssp engine. A result keeps its combined
+graph on the device (``dyng_hip_mosp_create``): an update turns it into the new trees' combined
+graph with one batch and repairs its tree (``dyng_hip_mosp_update``), the same tree the
+reference's solve from scratch gives.
 The trees and the combined arrays stay in device memory; the path costs are returned as host
 memory, as in the reference.
 
@@ -31,7 +34,7 @@ import numpy as np
 from . import _lib, sssp as _sssp
 from .batch import EdgeBatch
 from .device import DeviceArray
-from .errors import InvalidArgumentError, NotSupportedError
+from .errors import InvalidArgumentError, NotSupportedError, StaleResultError
 from .graph import ApplySummary, Graph
 from .resources import Resources, resolve
 
@@ -122,14 +125,15 @@ class Stats:
     objectives: tuple
     combined_edges: int
     preference_scale: int
-    combined_solve: Any = None  # dyng_amd.sssp.DeviceCounters of the combined solve (HIP path only)
+    combined_solve: Any = None  # dyng_amd.sssp.DeviceCounters of the combined tree's repair (HIP path only)
+    combined_batch: tuple = (0, 0)  # the combined graph's batch: (deletions, insertions or re-weights)
 
 
 class Result:
     """The K trees, the MOSP tree and the path costs kept up to date by :func:`update`
     (mosp::result, python/dyng/mosp.py:186-403)."""
 
-    __slots__ = ("_trees", "_options", "_resources", "_source", "_count", "_dist", "_par", "_costs",
+    __slots__ = ("_trees", "_options", "_resources", "_source", "_count", "_state", "_graph", "_costs",
                  "_costs_host", "_costs_valid", "_scale", "_combined_edges", "__weakref__")
 
     def __init__(self):
@@ -143,8 +147,8 @@ class Result:
         self._resources = res
         self._source = int(source)
         self._count = count
-        self._dist = None
-        self._par = None
+        self._state = None  # dyng_hip_mosp*: the combined graph and its tree on the device
+        self._graph = None  # the graph the result belongs to
         self._costs = None  # device (n, K) int64
         self._costs_host = None  # its host copy, made on first read after each compute / update
         self._costs_valid = False
@@ -180,18 +184,28 @@ class Result:
 
     def combined_device_arrays(self) -> tuple[int, int, int]:
         """(combined distances pointer, MOSP parents pointer, n) in device memory."""
-        n = self._dist.size if self._dist is not None else 0
-        return (self._dist.ptr if self._dist is not None else 0, self._par.ptr if self._par is not None else 0, n)
+        d, p, n = C.c_void_p(), C.c_void_p(), C.c_int64()
+        if self._state is None:
+            return 0, 0, 0
+        _lib.check(_lib.lib().dyng_hip_mosp_arrays(self._state, C.byref(d), C.byref(p), C.byref(n)))
+        return d.value or 0, p.value or 0, n.value
+
+    def _download(self, which: int, dtype) -> np.ndarray:
+        ptrs = self.combined_device_arrays()
+        out = np.empty(ptrs[2], dtype)
+        if out.nbytes:
+            _lib.check(_lib.lib().dyng_hip_memcpy_dtoh(out.ctypes.data, ptrs[which], out.nbytes))
+        return out
 
     @property
     def combined_distances(self) -> np.ndarray:
         """Host copy of the combined distances (units of 1/L, L = preference_scale)."""
-        return self._dist.to_numpy()
+        return self._download(0, np.int64)
 
     @property
     def combined_parents(self) -> np.ndarray:
         """Host copy of the MOSP tree (-1 for the source and unreachable vertices)."""
-        return self._par.to_numpy()
+        return self._download(1, np.int32)
 
     @property
     def path_costs(self) -> np.ndarray:
@@ -254,18 +268,21 @@ class Result:
         return np.dtype(np.int32)
 
     def clone(self, resources: Resources | None = None) -> "Result":
-        """A deep copy (trees, combined arrays, costs, options)."""
+        """A deep copy (trees, combined graph and tree, costs, options) of a result that matches
+        its graph's current version."""
+        if self._graph is None or self.graph_version != self._graph.version:
+            raise StaleResultError("dyng_amd.mosp.Result.clone: the result does not match its graph's current "
+                                   "version")
         res = resolve(resources, self._resources)
         out = Result._new([t.clone(res) for t in self._trees], self.options, self._count, self._source, res)
-        if self._dist is not None:
-            out._dist = DeviceArray(self._dist.to_numpy(), device=res.device)
-            out._par = DeviceArray(self._par.to_numpy(), device=res.device)
-        if self._costs is not None:
-            out._costs = DeviceArray(self._costs.to_numpy(), device=res.device)
-        out._costs_valid = self._costs_valid
-        out._scale = self._scale
-        out._combined_edges = self._combined_edges
+        _combine(self._graph, out, update=False)
         return out
+
+    def __del__(self):
+        h = getattr(self, "_state", None)
+        if h and _lib._lib is not None:
+            _lib._lib.dyng_hip_mosp_destroy(h)
+            self._state = None
 
     @classmethod
     def from_arrays(cls, graph: Graph, source: int, distances, parents, *, canonicalize: bool = True,
@@ -286,7 +303,7 @@ class Result:
         trees = [_sssp.Result.from_arrays(graph, source, distances[k], parents[k], canonicalize=canonicalize,
                                           options=opt._sssp(k), resources=res) for k in range(count)]
         out = cls._new(trees, opt, count, source, res)
-        _combine(graph, out, count_changes=False)
+        _combine(graph, out, update=False)
         return out
 
     def __copy__(self) -> "Result":
@@ -300,7 +317,7 @@ class Result:
                         "dyng_amd.mosp.Result.from_arrays()")
 
     def __repr__(self) -> str:
-        n = self._par.size if self._par is not None else 0
+        n = self.combined_device_arrays()[2]
         return (f"dyng_amd.mosp.Result(source={self._source}, num_objectives={self._count}, num_vertices={n}, "
                 f"preference_scale={self._scale}, graph_version={self.graph_version})")
 
@@ -312,33 +329,35 @@ def _check_graph(graph: Graph) -> None:
         raise NotSupportedError("dyng_amd.mosp needs integer weights: one int32 weight column per objective")
 
 
-def _combine(graph: Graph, result: Result, *, count_changes: bool) -> _lib.MospStats:
+def _combine(graph: Graph, result: Result, *, update: bool) -> _lib.MospStats:
     """The combined graph, the MOSP tree and the path costs of ``result``'s K trees
-    (mosp.cpp:264-330) in one call of dyng_hip_mosp_combined."""
+    (mosp.cpp:264-330): built on the device (compute, from_arrays), or, after the trees were
+    repaired, brought up to date with one batch and a repair (update)."""
     res = result._resources
     n = graph.num_vertices
-    old = 0 if result._par is None else result._par.size
-    if result._dist is None or old != n:
-        # first solve, or the graph grew: the previous tree, extended with unreached vertices
-        d = np.full(n, INFINITE_DISTANCE, np.int64)
-        p = np.full(n, -1, np.int32)
-        if old:
-            d[:old] = result._dist.to_numpy()[:n]
-            p[:old] = result._par.to_numpy()[:n]
-        result._dist = DeviceArray(d, device=res.device)
-        result._par = DeviceArray(p, device=res.device)
     count = result._count
     ptrs = (C.c_void_p * count)(*[t.device_arrays()[1] for t in result._trees])
-    prefs = result._options.preferences
-    pref_arr = (C.c_int32 * count)(*prefs) if prefs else None
+    st = _lib.MospStats()
+    if update and result._state is not None:
+        _lib.check(_lib.lib().dyng_hip_mosp_update(result._state, graph._handle, ptrs, res.stream_handle,
+                                                   C.byref(st)))
+    else:
+        prefs = result._options.preferences
+        pref_arr = (C.c_int32 * count)(*prefs) if prefs else None
+        h = C.c_void_p()
+        _lib.check(_lib.lib().dyng_hip_mosp_create(graph._handle, result._source, count, ptrs, pref_arr,
+                                                   res.stream_handle, C.byref(h), C.byref(st)))
+        if result._state is not None:
+            _lib.lib().dyng_hip_mosp_destroy(result._state)
+        result._state = h.value
+    result._graph = graph
     columns = count if result._options.compute_path_costs else 0
     costs = result._costs
     if columns and n and (costs is None or costs.shape != (n, columns)):
         costs = DeviceArray.empty((n, columns), np.int64, device=res.device)
-    st = _lib.MospStats()
-    _lib.check(_lib.lib().dyng_hip_mosp_combined(
-        graph._handle, result._source, count, ptrs, pref_arr, int(bool(count_changes)), result._dist.ptr,
-        result._par.ptr, columns, costs.ptr if (columns and n) else None, res.stream_handle, C.byref(st)))
+    if columns and n:
+        _lib.check(_lib.lib().dyng_hip_mosp_path_costs(result._state, graph._handle, columns, costs.ptr,
+                                                       res.stream_handle))
     result._costs_host = None
     if columns:
         result._costs = costs if n else None
@@ -357,8 +376,6 @@ def path_costs_over(graph: Graph, result: Result, columns: int) -> np.ndarray:
     if not isinstance(result, Result):
         raise TypeError("dyng_amd.mosp.path_costs_over: result must be a dyng_amd.mosp.Result")
     if result.graph_version != graph.version:
-        from .errors import StaleResultError
-
         raise StaleResultError("dyng_amd.mosp.path_costs_over: the result does not match the graph's version")
     columns = int(columns)
     if not 1 <= columns <= graph.num_weights:
@@ -394,14 +411,14 @@ def compute(graph: Graph, source: int, *, options: Options | None = None, resour
     count = opt.num_objectives or graph.num_weights
     trees = [_sssp.compute(graph, source, options=opt._sssp(k), resources=res) for k in range(count)]
     out = Result._new(trees, opt, count, source, res)
-    _combine(graph, out, count_changes=False)
+    _combine(graph, out, update=False)
     return out
 
 
 def _finish_update(graph: Graph, result: Result, per: tuple) -> Stats:
     """After the K trees were repaired on the new graph: the combined graph, the MOSP tree and
     the costs, and the update's stats."""
-    st = _combine(graph, result, count_changes=True)
+    st = _combine(graph, result, update=True)
     solve = st.solve
     return Stats(affected=int(st.affected),
                  iterations=sum(s.iterations for s in per) + int(solve.iterations),
@@ -411,7 +428,8 @@ def _finish_update(graph: Graph, result: Result, per: tuple) -> Stats:
                  engine_used=per[0].engine_used if per else "fused",
                  batch=per[0].batch, objectives=per, combined_edges=int(st.combined_edges),
                  preference_scale=int(st.preference_scale),
-                 combined_solve=_sssp.DeviceCounters._from_native(solve))
+                 combined_solve=_sssp.DeviceCounters._from_native(solve),
+                 combined_batch=(int(st.deletions), int(st.insertions)))
 
 
 def update(graph: Graph, batch: EdgeBatch, result: Result, *, resources: Resources | None = None) -> Stats:

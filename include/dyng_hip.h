@@ -384,8 +384,10 @@ typedef struct dyng_hip_mosp_stats {
   int64_t weight_sum;       /* their weight sum (the solve's default delta comes from it) */
   int64_t preference_scale; /* L = lcm(preferences), 1 without preferences */
   int64_t base;             /* L (K + 1): the largest combined weight */
-  int64_t affected;         /* count_changes: vertices whose combined (distance, parent) changed */
-  dyng_hip_counters solve;  /* the combined solve's engine counters */
+  int64_t affected;         /* count_changes / an update: vertices whose combined (distance, parent) changed */
+  int64_t deletions;        /* dyng_hip_mosp_update: combined edges the batch deleted */
+  int64_t insertions;       /* and inserted or re-weighted */
+  dyng_hip_counters solve;  /* the combined solve's (or repair's) engine counters */
 } dyng_hip_mosp_stats;
 
 /* The combined graph of the K trees (tree_parents: a host array of K device parent arrays of g's
@@ -403,6 +405,32 @@ int dyng_hip_mosp_combined(const dyng_hip_graph* g, int32_t source, int32_t num_
                            const int32_t* const* tree_parents, const int32_t* preferences, int32_t count_changes,
                            int64_t* distances, int32_t* parents, int32_t cost_columns, int64_t* path_costs,
                            void* stream, dyng_hip_mosp_stats* out);
+
+/* MOSP's combined state kept on the device between updates (the combined arrays of
+ * mosp::result, python/dyng/mosp.py:186-403): the combined graph of the K trees kept as a device
+ * graph and its canonical tree kept as an sssp result. dyng_hip_mosp_create builds both from the
+ * trees (tree_parents: a host array of K device parent arrays of g's n vertices; preferences as
+ * for dyng_hip_mosp_combined). dyng_hip_mosp_update takes the trees after an update of the same
+ * graph (as many vertices or more) and turns the combined graph into theirs with one batch (the
+ * combined edges the new trees lack deleted, the new or re-weighted ones upserted), applied and
+ * repaired like any sssp update: the same tree the combined solve gives (the reference solves
+ * it from scratch, mosp.cpp:319-330), `affected` counting the changed (distance, parent) pairs,
+ * `solve` the repair's counters. Calls on one state are not thread-safe. */
+typedef struct dyng_hip_mosp dyng_hip_mosp;
+int dyng_hip_mosp_create(const dyng_hip_graph* g, int32_t source, int32_t num_objectives,
+                         const int32_t* const* tree_parents, const int32_t* preferences, void* stream,
+                         dyng_hip_mosp** out, dyng_hip_mosp_stats* stats);
+int dyng_hip_mosp_update(dyng_hip_mosp* m, const dyng_hip_graph* g, const int32_t* const* tree_parents,
+                         void* stream, dyng_hip_mosp_stats* stats);
+/* The combined distances and the MOSP tree (device, n entries, read-only; valid until the next
+ * update or the destroy). */
+int dyng_hip_mosp_arrays(const dyng_hip_mosp* m, const int64_t** distances, const int32_t** parents,
+                         int64_t* n);
+/* The path costs of the state's MOSP tree over the first `columns` weights of g (device, n x
+ * columns, vertex-major), as dyng_hip_mosp_combined computes them. */
+int dyng_hip_mosp_path_costs(const dyng_hip_mosp* m, const dyng_hip_graph* g, int32_t columns,
+                             int64_t* costs, void* stream);
+int dyng_hip_mosp_destroy(dyng_hip_mosp* m);
 
 #ifdef __cplusplus
 }

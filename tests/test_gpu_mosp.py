@@ -147,6 +147,8 @@ def _random_graph(rng, n, m, K, wmax):
     (4, 500, 1500, 3, [1, 2, 7], False),
     (5, 1500, 9000, 3, [4, 1, 4], True),
     (6, 800, 3000, 5, [1, 2, 3, 4, 5], True),
+    # more trees than the combined diff holds in registers (its generic path)
+    (7, 400, 2400, 10, [1, 2, 3, 4, 5, 6, 7, 8, 9, 10], False),
 ])
 def test_random_updates_match_the_oracle(dg, seed, n, m, K, prefs, simple):
     """Random multigraphs (parallel edges, self-loops, unreachable parts) or simple graphs with
