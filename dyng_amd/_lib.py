@@ -38,7 +38,7 @@ SUMMARY_FIELDS = ("inserted_edges", "updated_edges", "deleted_edges", "ignored_d
 
 
 APPLY_STAGES = ("scan", "make_ops", "sort", "pairs_out", "unused_4", "unused_5", "pairs_in", "scans",
-                "plan", "merge_out", "merge_in")
+                "plan", "merge_tails", "merge_full")
 
 
 class ApplySummary(C.Structure):

@@ -852,6 +852,24 @@ struct merge_params {
   double headroom;
 };
 
+/// The in-place tail merges of one mode for both stores in one launch: workgroups [0, split)
+/// merge out-rows, the others in-rows. The two stores share nothing these merges write (the error
+/// bits aside, which they set atomically), so the halves run side by side.
+struct merge_sides {
+  merge_params side[2];
+  int split;
+};
+
+/// Runs `body(p, block, blocks)` with this workgroup's store and its index and count among that
+/// store's workgroups (one copy of the body: the store's parameters are selected, not branched on).
+template <typename Body>
+DYNG_DEV void for_side(const merge_sides& m, Body&& body) {
+  const int b = static_cast<int>(blockIdx.x);
+  const bool in = b >= m.split;
+  const merge_params p = in ? m.side[1] : m.side[0];
+  body(p, in ? b - m.split : b, in ? static_cast<int>(gridDim.x) - m.split : m.split);
+}
+
 constexpr int kMergeBlock = 1024;  ///< threads of a merge workgroup
 constexpr int kMergeWaves = kMergeBlock / 64;
 constexpr int kMergeSeg = 2048;    ///< operations of one row a workgroup stages in LDS
@@ -979,12 +997,12 @@ __global__ void __launch_bounds__(kPlanBlock) k_merge_plan(merge_params p) {
 /// neighbour plus the pending entries before it), then each is stored at its place. The weights
 /// move kRegWeights columns per pass, each pass reading its columns before writing them. Old
 /// entries come first among equal neighbours, as in the wave merge below.
-__global__ void __launch_bounds__(256) k_tail_lane(merge_params p) {
+DYNG_DEV void tail_lane(const merge_params& p, int block, int blocks) {
   const int count = p.list_count[kTailLane];
   const int32_t* list = p.lists + kTailLane * (p.N + 1);
   const u64 mask = (1ULL << p.vbits) - 1;
   const int K = p.K;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+  for (int t = block * blockDim.x + threadIdx.x; t < count; t += blocks * blockDim.x) {
     const int32_t u = list[t];
     const int2 mt = p.meta[u];
     const int64_t a = p.row_a[u];
@@ -1055,18 +1073,22 @@ __global__ void __launch_bounds__(256) k_tail_lane(merge_params p) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_tail_lane(merge_sides m) {
+  for_side(m, [](const merge_params& p, int block, int blocks) { tail_lane(p, block, blocks); });
+}
+
 /// One wave per row whose tail is short (at most kMergeSmall entries, 64 operations): the tail
 /// is read into registers and the merged tail (its alive entries and the pending ones, by
 /// neighbour id, old entries first among equal ones) written back in place. An alive entry's
 /// place is the alive entries before it plus the pending entries with a smaller neighbour (the
 /// operations one per lane); a pending entry's place is the alive entries up to its neighbour
 /// (ballots over the registers) plus the pending entries before it.
-__global__ void __launch_bounds__(kMergeBlock) k_tail_wave(merge_params p) {
+DYNG_DEV void tail_wave(const merge_params& p, int block, int blocks) {
   const int count = p.list_count[kTailWave];
   const int32_t* list = p.lists + kTailWave * (p.N + 1);
   const u64 mask = (1ULL << p.vbits) - 1;
-  const int wave = static_cast<int>((blockIdx.x * int64_t{blockDim.x} + threadIdx.x) / 64);
-  const int waves = static_cast<int>((int64_t{gridDim.x} * blockDim.x) / 64);
+  const int wave = static_cast<int>((block * int64_t{blockDim.x} + threadIdx.x) / 64);
+  const int waves = static_cast<int>((int64_t{blocks} * blockDim.x) / 64);
   const int K = p.K;
   for (int t = wave; t < count; t += waves) {
     const int32_t u = list[t];
@@ -1145,10 +1167,15 @@ __global__ void __launch_bounds__(kMergeBlock) k_tail_wave(merge_params p) {
   }
 }
 
+__global__ void __launch_bounds__(kMergeBlock) k_tail_wave(merge_sides m) {
+  for_side(m, [](const merge_params& p, int block, int blocks) { tail_wave(p, block, blocks); });
+}
+
 /// One workgroup per row whose tail needs more than a wave (up to kTailMax entries, kMergeSeg
 /// operations): the tail in registers (kTailPerThread per thread), the operations and the
-/// alive tail in LDS, the merged tail written back in place.
-__global__ void __launch_bounds__(kMergeBlock) k_tail_block(merge_params p) {
+/// alive tail in LDS, the merged tail written back in place. The LDS arrays are the function's
+/// own (one set per workgroup, whichever store it merges).
+DYNG_DEV void tail_block(const merge_params& p, int block, int blocks) {
   __shared__ int32_t s_v[kMergeSeg];       // neighbour of each operation
   __shared__ int32_t s_pp[kMergeSeg + 1];  // pending entries before each operation
   __shared__ int32_t s_t[kTailMax];        // the alive tail entries, compacted
@@ -1159,7 +1186,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_tail_block(merge_params p) {
   const int tid = static_cast<int>(threadIdx.x);
   const int wid = tid / 64;
   const int K = p.K;
-  for (int t = blockIdx.x; t < count; t += gridDim.x) {
+  for (int t = block; t < count; t += blocks) {
     const int32_t u = list[t];
     const int2 mt = p.meta[u];
     const int64_t a = p.row_a[u], b = p.row_b[u];
@@ -1244,6 +1271,10 @@ __global__ void __launch_bounds__(kMergeBlock) k_tail_block(merge_params p) {
     if (tid == 0) p.meta[u] = make_int2(mt.x, bl + alive_tail + s_pp[nops]);
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(kMergeBlock) k_tail_block(merge_sides m) {
+  for_side(m, [](const merge_params& p, int block, int blocks) { tail_block(p, block, blocks); });
 }
 
 /// Count of the entries of a sorted run in global memory that are < v (or <= v when `upper`).
@@ -2103,28 +2134,33 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     mark_stage(g, 10, s);
     // ---- the merges, with grids sized from the plan ----
     if (hc->error_bits == 0) {
+      // the in-place tail merges: one launch per mode for both stores (out-rows in the first
+      // workgroups, in-rows in the rest)
+      merge_sides ms{};
+      ms.side[0] = merge_setup(true);
+      ms.side[1] = merge_setup(false);
+      auto launch_tails = [&](void (*kernel)(merge_sides), int mode, int block, int rows_per_group, int cap) {
+        const int out_groups = std::min((hc->modes_out[mode] + rows_per_group - 1) / rows_per_group, cap);
+        const int in_groups = std::min((hc->modes_in[mode] + rows_per_group - 1) / rows_per_group, cap);
+        if (out_groups + in_groups == 0) return;
+        ms.split = out_groups;
+        hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(out_groups + in_groups)), dim3(block), 0, s, ms);
+      };
+      launch_tails(k_tail_lane, kTailLane, 256, 256, 8192);
+      launch_tails(k_tail_wave, kTailWave, kMergeBlock, kMergeWaves, 8192);
+      launch_tails(k_tail_block, kTailBlock, kMergeBlock, 1, 4096);
+      DYNG_HIP_TRY(hipGetLastError());
+      mark_stage(g, 11, s);
+      // whole rows through the staging arena, which both stores use: one store at a time
       for (int side = 0; side < 2; ++side) {
-        const merge_params mp = merge_setup(side == 0);
-        const int* modes = side == 0 ? hc->modes_out : hc->modes_in;
-        if (modes[kTailLane] > 0) {
-          const int64_t blocks = std::min<int64_t>((modes[kTailLane] + 255) / 256, 8192);
-          hipLaunchKernelGGL(k_tail_lane, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, mp);
+        const int rows = side == 0 ? hc->modes_out[kFull] : hc->modes_in[kFull];
+        if (rows > 0) {
+          hipLaunchKernelGGL(k_full_merge, dim3(static_cast<unsigned>(std::min(rows, 4096))), dim3(kMergeBlock), 0,
+                             s, ms.side[side]);
         }
-        if (modes[kTailWave] > 0) {
-          const int64_t blocks = std::min<int64_t>((modes[kTailWave] + kMergeWaves - 1) / kMergeWaves, 8192);
-          hipLaunchKernelGGL(k_tail_wave, dim3(static_cast<unsigned>(blocks)), dim3(kMergeBlock), 0, s, mp);
-        }
-        if (modes[kTailBlock] > 0) {
-          hipLaunchKernelGGL(k_tail_block, dim3(static_cast<unsigned>(std::min(modes[kTailBlock], 4096))),
-                             dim3(kMergeBlock), 0, s, mp);
-        }
-        if (modes[kFull] > 0) {
-          hipLaunchKernelGGL(k_full_merge, dim3(static_cast<unsigned>(std::min(modes[kFull], 4096))),
-                             dim3(kMergeBlock), 0, s, mp);
-        }
-        DYNG_HIP_TRY(hipGetLastError());
-        mark_stage(g, 11 + side, s);
       }
+      DYNG_HIP_TRY(hipGetLastError());
+      mark_stage(g, 12, s);
     }
   }
   if (wait != nullptr) DYNG_HIP_TRY(hipStreamWaitEvent(s, wait, 0));

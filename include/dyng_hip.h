@@ -85,7 +85,8 @@ typedef struct dyng_hip_apply_summary {
    * DYNG_HIP_STAGE_TIMES=1, since each timed event costs the stream a few microseconds): batch
    * scan, operation keys, key sort (both directions' keys in one sort), out-row pairing, (unused),
    * (unused), in-row pairing, the prefix scans of both directions (one scan), merge planning (with
-   * its host round trip and any compaction), out-row merge, in-row merge */
+   * its host round trip and any compaction), the in-place tail merges of both directions, the
+   * whole-row merges through the staging arena */
   double stage_ms[11];
 } dyng_hip_apply_summary;
 
