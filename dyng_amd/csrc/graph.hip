@@ -942,12 +942,12 @@ DYNG_DEV row_plan plan_row(const merge_params& p, int32_t u, int2 mt, int64_t a,
 /// addresses every workgroup hits).
 constexpr int kPlanBlock = 1024;
 
-__global__ void __launch_bounds__(kPlanBlock) k_merge_plan(merge_params p) {
+DYNG_DEV void merge_plan(const merge_params& p, int block) {
   __shared__ int wave_offset[kPlanBlock / 64][kModes];
   __shared__ int base[kModes];
   __shared__ long long part[kPlanBlock / 64];
   const int count = *p.touched_count;
-  const int t = blockIdx.x * kPlanBlock + threadIdx.x;
+  const int t = block * kPlanBlock + threadIdx.x;
   const int w = threadIdx.x / 64;
   long long reloc = 0, stage = 0;
   int mode = -1;
@@ -988,6 +988,10 @@ __global__ void __launch_bounds__(kPlanBlock) k_merge_plan(merge_params p) {
     if (reloc) atomicAdd(reinterpret_cast<unsigned long long*>(p.need_reloc), static_cast<unsigned long long>(reloc));
     if (stage) atomicAdd(reinterpret_cast<unsigned long long*>(p.need_stage), static_cast<unsigned long long>(stage));
   }
+}
+
+__global__ void __launch_bounds__(kPlanBlock) k_merge_plan(merge_sides m) {
+  for_side(m, [](const merge_params& p, int block, int) { merge_plan(p, block); });
 }
 
 /// One thread per row whose tail holds at most kLaneTail entries and that has at most kLaneOps
@@ -2100,8 +2104,11 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
         DYNG_HIP_TRY(hipMemcpyAsync(dc->modes_out, hc->modes_out, 2 * sizeof(hc->modes_out), hipMemcpyHostToDevice,
                                     s));
       }
-      hipLaunchKernelGGL(k_merge_plan, dim3(plan_grid), dim3(kPlanBlock), 0, s, merge_setup(true));
-      hipLaunchKernelGGL(k_merge_plan, dim3(plan_grid), dim3(kPlanBlock), 0, s, merge_setup(false));
+      merge_sides plan{};  // both stores' plans in one launch: out-rows first
+      plan.side[0] = merge_setup(true);
+      plan.side[1] = merge_setup(false);
+      plan.split = static_cast<int>(plan_grid);
+      hipLaunchKernelGGL(k_merge_plan, dim3(2 * plan_grid), dim3(kPlanBlock), 0, s, plan);
       DYNG_HIP_TRY(hipGetLastError());
       DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
       DYNG_HIP_TRY(hipStreamSynchronize(s));
