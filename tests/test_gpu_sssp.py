@@ -506,6 +506,37 @@ def test_long_rows_with_many_operations(dg):
         _assert_same_rows(hg, og)
 
 
+@pytest.mark.parametrize("hub_is_source", [True, False])
+def test_one_store_per_merge_mode(dg, hub_is_source):
+    """A batch whose operations all touch one hub: the hub's row (out-row when it is the source of
+    every operation, else in-row) takes a workgroup merge while every other touched row takes a
+    lane merge in the other store, so each merge launch has rows of one store only."""
+    _say(f"[one store per mode] hub_is_source={hub_is_source}")
+    rng = np.random.default_rng(21)
+    n = 5000
+    src = rng.integers(0, n, 40000)
+    dst = rng.integers(0, n, 40000)
+    w = rng.integers(1, 50, len(src))
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    ot, _ = oracle.compute(og, 0)
+    hr = dg.sssp.compute(hg, 0)
+    hub = 7
+    for b, size in enumerate((3000, 40, 2500)):
+        others = rng.permutation(np.arange(n))[:size]
+        others = others[others != hub]
+        hubs = np.full(len(others), hub, np.int64)
+        ins_src, ins_dst = (hubs, others) if hub_is_source else (others, hubs)
+        ins_w = rng.integers(1, 50, len(others))
+        empty = np.zeros(0, np.int64)
+        og, a, c = oracle.update(og, ot, ins_src, ins_dst, ins_w.reshape(-1, 1), empty, empty)
+        st = dg.sssp.update(hg, dg.EdgeBatch(insert=(ins_src, ins_dst, ins_w)), hr)
+        d, p = hr.to_numpy()
+        assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent), f"batch {b}"
+        assert st.batch.inserted_edges == a.summary["inserted_edges"]
+        _assert_same_rows(hg, og)
+
+
 @pytest.mark.parametrize("K,allow_parallel", [(3, True), (5, False), (8, True)])
 def test_rows_with_many_weight_columns(dg, K, allow_parallel):
     """Graphs with more weight columns than a tail merge moves per pass (every merge mode: short
