@@ -205,21 +205,26 @@ __global__ void k_tree_edge_costs(edge_rows out, edge_rows in, bool by_in_row, i
 /// Path doubling, round r: each vertex adds its ancestor's sum and jumps to the ancestor's
 /// ancestor. A round whose input has no vertex left with an ancestor does nothing; a round that
 /// runs records itself in `last` and sets active[r + 1] when some vertex still has an ancestor.
+/// A vertex's jump is -1 in the round after its sum became final (that round copies the sum to
+/// the other buffer) and -2 from then on (both buffers hold it: the round only passes the -2 on).
 __global__ void k_path_double(const int64_t* acc_in, const int32_t* jump_in, int64_t* acc_out, int32_t* jump_out,
                               int32_t n, int columns, int round, int* active, int* last) {
   if (active[round] == 0) return;  // uniform: every thread reads the same flag
   const int32_t v = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
-  int32_t next = -1;
+  int32_t next = -2;
   if (v < n) {
     const int32_t j = jump_in[v];
-    const int64_t* a = acc_in + static_cast<int64_t>(v) * columns;
-    int64_t* o = acc_out + static_cast<int64_t>(v) * columns;
-    if (j < 0) {
+    if (j == -1) {
+      const int64_t* a = acc_in + static_cast<int64_t>(v) * columns;
+      int64_t* o = acc_out + static_cast<int64_t>(v) * columns;
       for (int c = 0; c < columns; ++c) o[c] = a[c];
-    } else {
+    } else if (j >= 0) {
+      const int64_t* a = acc_in + static_cast<int64_t>(v) * columns;
+      int64_t* o = acc_out + static_cast<int64_t>(v) * columns;
       const int64_t* b = acc_in + static_cast<int64_t>(j) * columns;
       for (int c = 0; c < columns; ++c) o[c] = a[c] + b[c];
-      next = jump_in[j];
+      const int32_t up = jump_in[j];
+      next = up < 0 ? -1 : up;
     }
     jump_out[v] = next;
   }
