@@ -210,9 +210,11 @@ __global__ void k_tree_edge_costs(edge_rows out, edge_rows in, bool by_in_row, i
 __global__ void k_path_double(const int64_t* acc_in, const int32_t* jump_in, int64_t* acc_out, int32_t* jump_out,
                               int32_t n, int columns, int round, int* active, int* last) {
   if (active[round] == 0) return;  // uniform: every thread reads the same flag
-  const int32_t v = static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x);
-  int32_t next = -2;
-  if (v < n) {
+  bool more = false;  // some vertex of this thread still has an ancestor
+  const int64_t stride = int64_t{gridDim.x} * blockDim.x;
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < n; i += stride) {
+    const int32_t v = static_cast<int32_t>(i);
+    int32_t next = -2;
     const int32_t j = jump_in[v];
     if (j == -1) {
       const int64_t* a = acc_in + static_cast<int64_t>(v) * columns;
@@ -227,9 +229,10 @@ __global__ void k_path_double(const int64_t* acc_in, const int32_t* jump_in, int
       next = up < 0 ? -1 : up;
     }
     jump_out[v] = next;
+    more = more || next >= 0;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *last = round;
-  if (__syncthreads_or(next >= 0) && threadIdx.x == 0) active[round + 1] = 1;
+  if (__syncthreads_or(more) && threadIdx.x == 0) active[round + 1] = 1;
 }
 
 /// Round r writes buffer (r + 1) % 2 (buffer 0 is the caller's, which the first step wrote): the
@@ -647,8 +650,9 @@ void mosp_path_costs(const dyng_hip_graph* g, const int32_t* parents, int64_t n,
                      static_cast<int32_t>(n), source, bufs[0], jumps[0], missing, active);
   int cur = 0;
   for (int r = 0; r < rounds; ++r) {
-    hipLaunchKernelGGL(k_path_double, dim3(grid(n, 256)), dim3(256), 0, s, bufs[cur], jumps[cur], bufs[cur ^ 1],
-                       jumps[cur ^ 1], static_cast<int32_t>(n), columns, r, active, last);
+    // a bounded grid: a round after the last one still launches, and returns at once
+    hipLaunchKernelGGL(k_path_double, dim3(std::min(grid(n, 256), 2048u)), dim3(256), 0, s, bufs[cur], jumps[cur],
+                       bufs[cur ^ 1], jumps[cur ^ 1], static_cast<int32_t>(n), columns, r, active, last);
     cur ^= 1;
   }
   hipLaunchKernelGGL(k_costs_home, dim3(std::min(grid(n * columns, 256), 8192u)), dim3(256), 0, s, cc.acc.p, costs,
