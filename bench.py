@@ -133,32 +133,70 @@ def make_inputs(name: str, batches_needed: int):
     return n, src, dst, w, batch, source, ni + nd + nr
 
 
-def cpu_baseline(n, src, dst, w, batch, source, sample: int, threads: int):
-    """The oracle's OpenMP engine (openmp.cpp restated) on `sample` batches: repair time per
-    batch (identify_affected + seed + loop + finalize) and the full host update (MOSP apply +
-    transpose + repair). Returns (summary, trees after each sampled batch)."""
+def host_cpus() -> dict:
+    """The host cores this process may run on: CPU model, logical CPUs, physical cores and the
+    sockets they sit on (from /proc/cpuinfo and sysfs topology)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores, sockets = {}, {}
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(base + "physical_package_id") as f:
+                pkg = int(f.read())
+            with open(base + "core_id") as f:
+                core = int(f.read())
+        except OSError:
+            pkg, core = 0, c
+        cores.setdefault((pkg, core), c)
+        sockets.setdefault(pkg, set()).add((pkg, core))
+    one_socket = max(len(v) for v in sockets.values()) if sockets else len(cpus)
+    return {"model": model, "logical_cpus": len(cpus), "physical_cores": len(cores), "sockets": len(sockets),
+            "cores_one_socket": one_socket}
+
+
+def cpu_baseline(n, src, dst, w, batch, source, sample: int, thread_counts):
+    """The oracle's OpenMP engine (openmp.cpp restated) on `sample` batches, once per thread
+    count: repair time per batch (identify_affected + seed + loop + finalize) and the full host
+    update (MOSP apply + transpose + repair). Returns (summary per thread count, trees after each
+    sampled batch of the first run, the sampled batches)."""
     import oracle
 
     row = np.zeros(n + 1, np.int64)
     np.cumsum(np.bincount(src, minlength=n), out=row[1:])
-    g = oracle.Csr(row, dst.astype(np.int32), w.astype(np.int32).reshape(1, -1))
-    ws = oracle.Workspace()
-    t0 = time.perf_counter()
-    tree, _ = oracle.compute(g, source, 0, "omp", threads=threads, ws=ws)
-    t_compute = time.perf_counter() - t0
-    repair, full, trees, drawn = [], [], [], []
-    for b in range(sample):
-        drawn.append(batch(b, tree.parent))
-        ins_s, ins_d, ins_w, del_s, del_d = drawn[-1]
-        t1 = time.perf_counter()
-        g, _, c = oracle.update(g, tree, ins_s, ins_d, ins_w.reshape(-1, 1), del_s, del_d, engine="omp",
-                                threads=threads, ws=ws)
-        full.append(time.perf_counter() - t1)
-        repair.append(sum(c.t_hooks))
-        trees.append((tree.dist.copy(), tree.parent.copy(), c.invalidated, c.affected))
-        log(f"[bench] cpu batch {b}: repair {repair[-1] * 1e3:.1f} ms, full update {full[-1] * 1e3:.0f} ms")
-    return ({"compute_s": t_compute, "repair_ms": [x * 1e3 for x in repair], "update_ms": [x * 1e3 for x in full]},
-            trees, drawn)
+    summaries, trees, drawn = {}, [], []
+    for run, threads in enumerate(thread_counts):
+        g = oracle.Csr(row, dst.astype(np.int32), w.astype(np.int32).reshape(1, -1))
+        ws = oracle.Workspace()
+        t0 = time.perf_counter()
+        tree, _ = oracle.compute(g, source, 0, "omp", threads=threads, ws=ws)
+        t_compute = time.perf_counter() - t0
+        repair, full = [], []
+        for b in range(sample):
+            if run == 0:
+                drawn.append(batch(b, tree.parent))
+            ins_s, ins_d, ins_w, del_s, del_d = drawn[b]
+            t1 = time.perf_counter()
+            g, _, c = oracle.update(g, tree, ins_s, ins_d, ins_w.reshape(-1, 1), del_s, del_d, engine="omp",
+                                    threads=threads, ws=ws)
+            full.append(time.perf_counter() - t1)
+            repair.append(sum(c.t_hooks))
+            if run == 0:
+                trees.append((tree.dist.copy(), tree.parent.copy(), c.invalidated, c.affected))
+            log(f"[bench] cpu batch {b} ({threads} threads): repair {repair[-1] * 1e3:.1f} ms, full update "
+                f"{full[-1] * 1e3:.0f} ms")
+        summaries[threads] = {"compute_s": t_compute, "repair_ms": [x * 1e3 for x in repair],
+                              "update_ms": [x * 1e3 for x in full]}
+        del g, ws, tree
+    return summaries, trees, drawn
 
 
 def learned_vs_plain(n, src, dst, w, source, train, evaluate, log_prefix="learned"):
@@ -548,13 +586,17 @@ def mosp_bench(args) -> int:
 
 
 def main() -> int:
+    # the CPU baseline's OpenMP threads, pinned one per core (read when the OpenMP runtime loads,
+    # so before the generators' or the oracle's library is opened)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=19)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rmat22_mixed100k", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-sample", type=int, default=3, help="batches timed on the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core of this host")
+    ap.add_argument("--cpu-sample", type=int, default=5, help="batches timed on the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every physical core this process may use")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
     ap.add_argument("--train-batches", type=int, default=500, help="stream workloads: exploration batches the "
@@ -633,14 +675,22 @@ def main() -> int:
     cpu = None
     exact = None
     if not args.no_cpu and args.cpu_sample > 0:
-        threads = args.cpu_threads or len(os.sched_getaffinity(0))
-        summary, trees, drawn = cpu_baseline(n, src, dst, w, batch, source, args.cpu_sample, threads)
+        hc = host_cpus()
+        threads = args.cpu_threads or hc["physical_cores"]
+        counts = [threads] + ([hc["cores_one_socket"]] if hc["cores_one_socket"] < threads else [])
+        summaries, trees, drawn = cpu_baseline(n, src, dst, w, batch, source, args.cpu_sample, counts)
+        summary = summaries[threads]
         rep = statistics.median(summary["repair_ms"])
         cpu = {"value": round(ops / (rep / 1e3), 1), "unit": "edges/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_sample} batches x {ops} ops after compute(); repair = identify_affected + "
-                         f"seed + loop + finalize of the OpenMP engine",
+                         f"seed + loop + finalize of the OpenMP engine (median)",
+               "samples": args.cpu_sample, "cpu_model": hc["model"], "host": hc,
+               "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
                "p50_ms_repair": round(rep, 3), "p50_ms_update_incl_host_apply": round(
-                   statistics.median(summary["update_ms"]), 1), "compute_s": round(summary["compute_s"], 2)}
+                   statistics.median(summary["update_ms"]), 1), "compute_s": round(summary["compute_s"], 2),
+               "one_socket": {"threads": hc["cores_one_socket"],
+                              "p50_ms_repair": round(statistics.median(summaries[counts[-1]]["repair_ms"]), 3),
+                              "same_as_all_cores": len(counts) == 1}}
         # the same batches on a fresh device graph, compared after each
         g2 = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
         t2 = sssp.compute(g2, source)

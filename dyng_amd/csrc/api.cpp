@@ -33,15 +33,6 @@ namespace {
 
 constexpr int64_t kInf = sssp_infinity;
 
-struct scoped_device {
-  int previous = 0;
-  explicit scoped_device(int d) {
-    DYNG_HIP_TRY(hipGetDevice(&previous));
-    if (previous != d) DYNG_HIP_TRY(hipSetDevice(d));
-  }
-  ~scoped_device() { (void)hipSetDevice(previous); }
-};
-
 /// The pooled device workspaces of each device (ADR 0015: sized once, shared by the result runs
 /// on the device; runs on one device are serialized by the pool's mutex). Slot 0 serves one
 /// result at a time; an update of several results gives result i slot i % kWorkspaceSlots, so

@@ -359,11 +359,25 @@ struct op_layout {
   DYNG_DEV int64_t ins_base() const { return first_is_del ? num_first * stride : 0; }
 };
 
+/// Ranks of offending operations in the error modes, in the order the reference's host apply
+/// meets them (apply_host.cpp:80-252): first the deletions naming a vertex past the vertex count
+/// (batch order), then row by row (ascending u), within a row the first phase's operations (the
+/// deletions when deletions_first) before the second phase's, each in batch order. The lowest
+/// rank is the error reported. Layout: bit 63 row phase, bits 62-32 the row, bit 31 the second
+/// phase, bits 30-0 the operation's batch index.
+constexpr unsigned long long kNoError = ~0ULL;
+DYNG_DEV unsigned long long error_rank_prescan(int64_t j) { return static_cast<unsigned long long>(j); }
+DYNG_DEV unsigned long long error_rank_row(int32_t u, int second_phase, int64_t j) {
+  return (1ULL << 63) | (static_cast<unsigned long long>(u) << 32) | (static_cast<unsigned long long>(second_phase) << 31) |
+         static_cast<unsigned long long>(j);
+}
+
 __global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* ins_dst,
                            const int32_t* del_src, const int32_t* del_dst, int directed,
                            int drop_self_loops, int64_t n_after, int vbits, u64* keys, uint32_t* vals,
                            int32_t* e_del_src, int32_t* e_del_dst, int32_t* e_ins_src,
-                           int32_t* e_ins_dst, uint8_t* e_inc, int32_t* e_fw, int K, apply_control* ctl) {
+                           int32_t* e_ins_dst, uint8_t* e_inc, int32_t* e_fw, int K, int missing_delete_error,
+                           apply_control* ctl) {
   const int64_t total = (L.num_first + L.num_second) * L.stride;
   unsigned long long dropped = 0, ignored = 0;
   for (int64_t e = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; e < total; e += int64_t{gridDim.x} * blockDim.x) {
@@ -383,6 +397,9 @@ __global__ void k_make_ops(op_layout L, const int32_t* ins_src, const int32_t* i
     if (valid && del && (u >= n_after || v >= n_after)) {
       valid = false;
       ignored += 1;
+      // no such vertex, so no such edge: in the error mode, the batch's first such deletion
+      // (apply_host.cpp:116-138 finds these before any row)
+      if (missing_delete_error) atomicMin(&ctl->first_error, error_rank_prescan(j));
     }
     // Both directions' keys go through one sort: the out-row key (u, v) at e and the in-row key
     // (v, u) at total + e, tagged with bit 2 * vbits so that every out-key sorts before every
@@ -478,6 +495,51 @@ DYNG_DEV run_pair find_runs(const int32_t* row, int base_len, int len, int32_t v
   r.tl = row_lower_bound(row, base_len, len, v);
   r.tu = row_upper_bound_from(row, r.tl, len, v);
   return r;
+}
+
+/// The error modes, checked before anything changes (read only): one thread per (u, v) segment of
+/// the sorted out-row keys replays the segment's operations in batch order on counts of the alive
+/// (u, v) entries (old ones in the row, pending ones of the batch). A deletion that finds none is a
+/// missing edge, an insertion that finds one an existing edge (apply_host.cpp:117-120, 206-226);
+/// the lowest rank of an offending operation lands in ctl->first_error.
+__global__ void __launch_bounds__(kBlock) k_check_pairs(op_layout L, const u64* keys, const uint32_t* vals,
+                                                         int64_t N, int vbits, int64_t n_before, const int2* meta,
+                                                         const int32_t* base, const int32_t* nbr,
+                                                         int missing_delete_error, int existing_insert_error,
+                                                         apply_control* ctl) {
+  const int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x;
+  if (i >= N) return;
+  const u64 key = keys[i];
+  if (key == kNoKey || (i > 0 && keys[i - 1] == key)) return;
+  const u64 mask = (1ULL << vbits) - 1;
+  const int32_t u = static_cast<int32_t>(key >> vbits);
+  const int32_t v = static_cast<int32_t>(key & mask);
+  int alive = 0;
+  if (u < n_before) {
+    const int2 mt = meta[u];
+    const int32_t* row = nbr + mt.x;
+    const run_pair rr = find_runs(row, base[u], mt.y, v);
+    for (int x = rr.lb; x < rr.ub; ++x) alive += row[x] >= 0 ? 1 : 0;
+    for (int x = rr.tl; x < rr.tu; ++x) alive += row[x] >= 0 ? 1 : 0;
+  }
+  unsigned long long first = kNoError;
+  for (int64_t q = i; q < N && keys[q] == key; ++q) {
+    const int64_t e = vals[q];
+    const int second = e >= L.num_first * L.stride ? 1 : 0;
+    const int64_t j = L.local(e) / L.stride;
+    if (L.is_del(e)) {
+      if (alive > 0) {
+        --alive;
+      } else if (missing_delete_error) {
+        first = min(first, error_rank_row(u, second, j));
+      }
+    } else if (alive > 0) {
+      if (existing_insert_error) first = min(first, error_rank_row(u, second, j));
+    } else {
+      ++alive;  // appended
+    }
+  }
+  if (first != kNoError) atomicMin(&ctl->first_error, first);
 }
 
 /// Record where row r's operations start and end in the sorted keys (row = key >> vbits).
@@ -1463,15 +1525,6 @@ __global__ void k_caps_from_len(const int2* meta, int64_t n, double headroom, in
 
 hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
-struct scoped_device {
-  int previous = 0;
-  explicit scoped_device(int d) {
-    DYNG_HIP_TRY(hipGetDevice(&previous));
-    if (previous != d) DYNG_HIP_TRY(hipSetDevice(d));
-  }
-  ~scoped_device() { (void)hipSetDevice(previous); }
-};
-
 int64_t arena_slots(int64_t rows_slots, int64_t m, int64_t extra) {
   return rows_slots + std::max<int64_t>(m + m / 2, 1 << 16) + extra;
 }
@@ -1650,10 +1703,11 @@ dyng_hip_graph* create_from_edge_list(int device, const dyng_hip_graph_props* pr
     fail(DYNG_HIP_NOT_SUPPORTED, "dyng: the HIP sssp backend takes 1.." + std::to_string(max_weights) +
                                      " weight columns, got " + std::to_string(K));
   }
-  if (props->on_existing_insert == 1 || props->on_missing_delete == 1) {
-    fail(DYNG_HIP_NOT_SUPPORTED,
-         "dyng: the HIP device apply supports on_existing_insert = upsert or ignore and "
-         "on_missing_delete = ignore");
+  if (props->on_existing_insert < 0 || props->on_existing_insert > 2 || props->on_missing_delete < 0 ||
+      props->on_missing_delete > 1) {
+    fail(DYNG_HIP_INVALID_ARGUMENT,
+         "dyng: on_existing_insert must be 0 (upsert), 1 (error) or 2 (ignore) and on_missing_delete 0 "
+         "(ignore) or 1 (error)");
   }
   scoped_device guard(device);
   auto g = std::make_unique<dyng_hip_graph>();
@@ -1895,6 +1949,34 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
   }
 }
 
+/// Throw the invalid-argument error of the offending operation of rank `rank` (k_check_pairs):
+/// the message names the operation's kind, batch index and (u, v), as apply_host.cpp:117-120 and
+/// 222-225 do. Nothing has changed yet.
+[[noreturn]] void report_batch_error(dyng_hip_graph* g, const batch_scan& sc, const op_layout& L,
+                                     unsigned long long rank, hipStream_t s) {
+  const bool row_phase = (rank >> 63) != 0;
+  const int64_t j = static_cast<int64_t>(rank & 0x7fffffffULL);
+  const bool second = ((rank >> 31) & 1ULL) != 0;
+  const bool del = !row_phase || (second ? L.first_is_del == 0 : L.first_is_del != 0);
+  int32_t uv[2] = {0, 0};
+  const int32_t* src = del ? sc.del_src : sc.ins_src;
+  const int32_t* dst = del ? sc.del_dst : sc.ins_dst;
+  DYNG_HIP_TRY(hipMemcpyAsync(&uv[0], src + j, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipMemcpyAsync(&uv[1], dst + j, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  if (row_phase) {
+    // the arc of the row: (v, u) for the reverse arc of an undirected edge
+    const int32_t row = static_cast<int32_t>((rank >> 32) & 0x7fffffffULL);
+    if (row != uv[0]) std::swap(uv[0], uv[1]);
+  }
+  (void)g;
+  const std::string op = std::string(del ? "deletion " : "insertion ") + std::to_string(j) + " (" +
+                         std::to_string(uv[0]) + ", " + std::to_string(uv[1]) + ")";
+  fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: " + op +
+                                      (del ? " names a missing edge and batch_semantics::on_missing_delete is error"
+                                           : " names an existing edge and batch_semantics::on_existing_insert is error"));
+}
+
 void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out,
                    hipEvent_t wait) {
   const int K = g->K;
@@ -1967,6 +2049,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
   }
   // ---- control block: counters, tails, the column targets ----
   std::memset(hc, 0, sizeof(apply_control));
+  hc->first_error = kNoError;
   for (int k = 0; k < K; ++k) hc->col_max[k] = std::max<int64_t>(g->wmax[k], sc.wmax[k]);
   hc->tail_out = g->out.tail;
   hc->tail_in = g->in.tail;
@@ -1977,7 +2060,8 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     hipLaunchKernelGGL(k_make_ops, dim3(blocks_for(N)), dim3(kBlock), 0, s, L, ins_src, ins_dst, del_src, del_dst,
                        g->props.directed, g->props.on_self_loop == 1 ? 1 : 0, n_after, vbits, g->keys_a.p,
                        g->vals_a.p, g->del_src.p, g->del_dst.p, g->ins_src.p, g->ins_dst.p, g->ins_inc.p,
-                       g->props.allow_parallel ? nullptr : g->ins_fw.p, K, dc);
+                       g->props.allow_parallel ? nullptr : g->ins_fw.p, K, g->props.on_missing_delete == 1 ? 1 : 0,
+                       dc);
     DYNG_HIP_TRY(hipGetLastError());
     mark_stage(g, 3, s);
     // one stable sort of both directions' keys (2N of them, 2 * vbits + 1 bits): one sort of 2N
@@ -1985,6 +2069,18 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     sort_pairs(g->sort_tmp, g->keys_a.p, g->keys_b.p, g->vals_a.p, g->vals_b.p, 2 * N, 2 * vbits + 1, s);
     hipLaunchKernelGGL(k_untag, dim3(std::min<unsigned>(blocks_for(2 * N), 1024)), dim3(kBlock), 0, s, g->keys_b.p,
                        2 * N, vbits);
+    if (g->props.on_missing_delete == 1 || g->props.on_existing_insert == 1) {
+      // the error modes: every operation checked before anything changes (one more launch and
+      // synchronization, only in these modes)
+      hipLaunchKernelGGL(k_check_pairs, dim3(blocks_for(N)), dim3(kBlock), 0, s, L, g->keys_b.p, g->vals_b.p, N,
+                         vbits, n_before, g->out.meta.p, g->out.base.p, g->out.nbr.p,
+                         g->props.on_missing_delete == 1 ? 1 : 0, g->props.on_existing_insert == 1 ? 1 : 0, dc);
+      DYNG_HIP_TRY(hipGetLastError());
+      DYNG_HIP_TRY(hipMemcpyAsync(&hc->first_error, &dc->first_error, sizeof(hc->first_error),
+                                  hipMemcpyDeviceToHost, s));
+      DYNG_HIP_TRY(hipStreamSynchronize(s));
+      if (hc->first_error != kNoError) report_batch_error(g, sc, L, hc->first_error, s);
+    }
     mark_stage(g, 4, s);
     out_pair_params op{};
     op.L = L;

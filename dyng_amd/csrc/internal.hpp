@@ -52,6 +52,19 @@ int guarded(fn_t&& fn) {
   }
 }
 
+/// Makes device `d` current for a scope and restores the caller's device afterwards (the
+/// reference's scoped_device, cuda_runtime.hpp:77-94).
+struct scoped_device {
+  int previous = 0;
+  explicit scoped_device(int d) {
+    DYNG_HIP_TRY(hipGetDevice(&previous));
+    if (previous != d) DYNG_HIP_TRY(hipSetDevice(d));
+  }
+  ~scoped_device() { (void)hipSetDevice(previous); }
+  scoped_device(const scoped_device&) = delete;
+  scoped_device& operator=(const scoped_device&) = delete;
+};
+
 /// A device array that grows on demand (hipMalloc; the contents are not kept on growth unless
 /// `keep` is set).
 template <typename T>
@@ -156,6 +169,9 @@ struct apply_control {
   long long col_bad[max_weights];  ///< weights below 1 per column (column summary)
   // merge plan: tail slots of the relocating rows, staging slots of the staged rows
   long long need_reloc_out, need_reloc_in, need_stage_out, need_stage_in;
+  // error modes (on_missing_delete / on_existing_insert = error): the first offending operation
+  // in the reference's order, as a rank (all ones: none); see k_check_pairs
+  unsigned long long first_error;
 };
 
 /// What the batch scan found (before anything changes) and where the batch is on the device.

@@ -482,7 +482,7 @@ int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, c
 extern "C" int dyng_hip_mlp_create(int device, uint64_t seed, dyng_hip_mlp** out) {
   return guarded([&] {
     if (out == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: null output handle");
-    DYNG_HIP_TRY(hipSetDevice(device));
+    scoped_device guard(device);
     auto m = std::make_unique<dyng_hip_mlp>();
     m->device = device;
     m->params.reserve(kAll);
@@ -521,7 +521,7 @@ extern "C" int dyng_hip_mlp_destroy(dyng_hip_mlp* m) {
 extern "C" int dyng_hip_mlp_get_params(const dyng_hip_mlp* m, float* out, int64_t count) {
   return guarded([&] {
     if (m == nullptr || out == nullptr || count != kAll) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: get_params needs DYNG_HIP_MLP_PARAMS floats");
-    DYNG_HIP_TRY(hipSetDevice(m->device));
+    scoped_device guard(m->device);
     DYNG_HIP_TRY(hipStreamSynchronize(m->stream));  // a scheduled update may still read the buffers
     DYNG_HIP_TRY(hipMemcpy(out, m->params.p, kAll * sizeof(float), hipMemcpyDeviceToHost));
   });
@@ -533,7 +533,7 @@ extern "C" int dyng_hip_mlp_set_params(dyng_hip_mlp* m, const float* in, int64_t
     for (int i = 0; i < kIn; ++i) {
       if (!(in[kScale + i] > 0.0f) || !std::isfinite(in[kScale + i])) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: input scales must be positive");
     }
-    DYNG_HIP_TRY(hipSetDevice(m->device));
+    scoped_device guard(m->device);
     DYNG_HIP_TRY(hipStreamSynchronize(m->stream));  // a scheduled update may still read the buffers
     DYNG_HIP_TRY(hipMemcpy(m->params.p, in, kAll * sizeof(float), hipMemcpyHostToDevice));
     DYNG_HIP_TRY(hipMemset(m->adam_m.p, 0, kTrained * sizeof(float)));
@@ -549,7 +549,7 @@ extern "C" int dyng_hip_mlp_gradients(dyng_hip_mlp* m, const float* x, const flo
     if (m == nullptr || x == nullptr || t == nullptr || grad == nullptr || count < 1 || count > (1 << 20)) {
       fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: gradients need 1 to 2^20 samples and output arrays");
     }
-    DYNG_HIP_TRY(hipSetDevice(m->device));
+    scoped_device guard(m->device);
     DYNG_HIP_TRY(hipStreamSynchronize(m->stream));  // a scheduled update may still read the buffers
     upload_set(m, x, t, count);
     std::vector<int32_t> idx(static_cast<size_t>(count));
@@ -570,7 +570,7 @@ extern "C" int dyng_hip_mlp_train(dyng_hip_mlp* m, const float* x, const float* 
         batch > 65536 || !(lr > 0.0f)) {
       fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: train needs samples, steps >= 0, a minibatch in [1, 65536], lr > 0");
     }
-    DYNG_HIP_TRY(hipSetDevice(m->device));
+    scoped_device guard(m->device);
     DYNG_HIP_TRY(hipStreamSynchronize(m->stream));  // a scheduled update may still read the buffers
     upload_set(m, x, t, count);
     const int B = static_cast<int>(std::min<int64_t>(batch, count));
@@ -619,7 +619,7 @@ extern "C" int dyng_hip_mlp_predict(dyng_hip_mlp* m, const float* x, int64_t cou
     if (m == nullptr || x == nullptr || y == nullptr || count < 1 || count > 65535) {
       fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: mlp: predict needs 1 to 65535 samples");
     }
-    DYNG_HIP_TRY(hipSetDevice(m->device));
+    scoped_device guard(m->device);
     DYNG_HIP_TRY(hipStreamSynchronize(m->stream));  // a scheduled update may still read the buffers
     refresh_bf16(m, nullptr);
     m->X.reserve(count * kIn);
@@ -637,7 +637,7 @@ extern "C" int dyng_hip_batch_features(const dyng_hip_graph* g, const dyng_hip_b
   return guarded([&] {
     if (g == nullptr || b == nullptr || out == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: batch_features: null argument");
     if (b->num_ins < 0 || b->num_del < 0) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: batch_features: negative sizes");
-    DYNG_HIP_TRY(hipSetDevice(g->device));
+    scoped_device guard(g->device);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int32_t *ins_src = b->ins_src, *del_src = b->del_src, *del_dst = b->del_dst;
     dbuf<int32_t> hi, hs, hd;

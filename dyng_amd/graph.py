@@ -20,8 +20,8 @@ from .resources import Resources, resolve
 class BatchSemantics:
     """batch_semantics (cpp/include/dyng/graph/graph_properties.hpp:67-188)."""
 
-    on_existing_insert: str = "upsert"  # upsert | ignore (error: not on this backend)
-    on_missing_delete: str = "ignore"
+    on_existing_insert: str = "upsert"  # upsert | error | ignore
+    on_missing_delete: str = "ignore"  # ignore | error
     on_self_loop: str = "keep"  # keep | drop | error
     deletions_first: bool = True
     allow_vertex_growth: bool = True

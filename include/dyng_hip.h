@@ -64,8 +64,8 @@ typedef struct dyng_hip_graph_props {
   int32_t directed;            /* 1: directed; 0: a batch edge (u,v) changes both arcs */
   int32_t sorted_rows;         /* row_order::sorted (1) or append (0); see the layout note */
   int32_t allow_parallel;      /* multi_edges::allow (1) or forbid (0) */
-  int32_t on_existing_insert;  /* 0 upsert, 2 ignore (1 = error is not supported here) */
-  int32_t on_missing_delete;   /* 0 ignore (1 = error is not supported here) */
+  int32_t on_existing_insert;  /* 0 upsert, 1 error, 2 ignore */
+  int32_t on_missing_delete;   /* 0 ignore, 1 error (both error modes: checked before anything changes) */
   int32_t on_self_loop;        /* 0 keep, 1 drop (2 = error: checked on the host side) */
   int32_t deletions_first;     /* 1: deletions, then insertions */
   int32_t allow_vertex_growth; /* 1: an insertion naming v >= n grows the vertex set */
