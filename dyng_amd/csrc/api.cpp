@@ -459,6 +459,8 @@ extern "C" int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batc
     DYNG_HIP_TRY(hipEventRecord(ev_apply.a, s));
     try {
       apply_scanned(g, sc, s, &summary);
+    } catch (const batch_refused&) {
+      throw;  // refused before anything changed: the result stays current
     } catch (...) {
       r->poisoned = true;  // the graph may be partly updated
       throw;
@@ -541,6 +543,8 @@ extern "C" int dyng_hip_api_update_each(dyng_hip_graph* g, const dyng_hip_batch*
     DYNG_HIP_TRY(hipEventRecord(ev_apply.a, s));
     try {
       apply_scanned(g, sc, s, &summary);
+    } catch (const batch_refused&) {
+      throw;  // refused before anything changed: every result stays current
     } catch (...) {
       for (int64_t i = 0; i < count; ++i) results[i]->poisoned = true;  // the graph may be partly updated
       throw;
@@ -674,6 +678,8 @@ extern "C" int dyng_hip_api_update_scheduled(dyng_hip_graph* g, const dyng_hip_b
     DYNG_HIP_TRY(hipEventRecord(ev_apply.a, s));
     try {
       apply_scanned(g, sc, s, &summary, predicted);
+    } catch (const batch_refused&) {
+      throw;  // refused before anything changed: the result stays current
     } catch (...) {
       r->poisoned = true;  // the graph may be partly updated
       throw;

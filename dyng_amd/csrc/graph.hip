@@ -1972,9 +1972,10 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
   (void)g;
   const std::string op = std::string(del ? "deletion " : "insertion ") + std::to_string(j) + " (" +
                          std::to_string(uv[0]) + ", " + std::to_string(uv[1]) + ")";
-  fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: " + op +
-                                      (del ? " names a missing edge and batch_semantics::on_missing_delete is error"
-                                           : " names an existing edge and batch_semantics::on_existing_insert is error"));
+  throw batch_refused(DYNG_HIP_INVALID_ARGUMENT,
+                      "dyng: " + op +
+                          (del ? " names a missing edge and batch_semantics::on_missing_delete is error"
+                               : " names an existing edge and batch_semantics::on_existing_insert is error"));
 }
 
 void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out,

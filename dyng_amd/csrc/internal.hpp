@@ -23,6 +23,12 @@ struct status_error : std::runtime_error {
       : std::runtime_error(m), code(c), device_bits(bits) {}
 };
 
+/// A batch that the apply refused before it changed anything (the batch semantics' error modes):
+/// the graph and every result stay as they were.
+struct batch_refused : status_error {
+  using status_error::status_error;
+};
+
 [[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
 [[noreturn]] inline void fail(int code, const std::string& m) { throw status_error(code, m); }
 void set_last_error(const std::string& m);
