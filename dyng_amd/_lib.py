@@ -116,7 +116,10 @@ EXPORTS = (
     "dyng_hip_mlp_predict", "dyng_hip_result_equal", "dyng_hip_api_update_scheduled",
     "dyng_hip_mosp_combined", "dyng_hip_api_update_each", "dyng_hip_mosp_create", "dyng_hip_mosp_update",
     "dyng_hip_mosp_arrays", "dyng_hip_mosp_path_costs", "dyng_hip_mosp_destroy",
+    "dyng_hip_sssp_update_noalloc",
 )
+
+NOALLOC_SCRATCH_BYTES = 64 << 20  # DYNG_HIP_NOALLOC_SCRATCH_BYTES
 
 MLP_INPUTS, MLP_HIDDEN, MLP_OUTPUTS, MLP_TRAINED, MLP_PARAMS = 12, 64, 5, 5317, 5341
 
@@ -164,6 +167,7 @@ def lib() -> C.CDLL:
     L.dyng_hip_workspace_destroy.argtypes = [vp]
     L.dyng_hip_sssp_update.argtypes = [C.POINTER(SsspRun), vp, C.POINTER(Counters)]
     L.dyng_hip_sssp_compute.argtypes = [C.POINTER(SsspRun), vp, C.POINTER(Counters)]
+    L.dyng_hip_sssp_update_noalloc.argtypes = [C.POINTER(SsspRun), vp, C.POINTER(Counters)]
     L.dyng_hip_api_compute.argtypes = [vp, C.c_int32, C.POINTER(Options), vp, C.POINTER(vp)]
     L.dyng_hip_api_update.argtypes = [vp, C.POINTER(Batch), vp, vp, C.POINTER(Stats)]
     L.dyng_hip_api_compute_multi.argtypes = [vp, vp, C.c_int64, C.POINTER(Options), vp, C.POINTER(vp)]

@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 namespace dyng_hip {
@@ -1070,8 +1071,11 @@ namespace dyng_hip {
 
 namespace {
 
+/// Chunk-list entries a run over n vertices and a row arena of `slots` slots may need.
+int64_t chunks_for(int64_t n, int64_t slots) { return slots / 16 + n + 4096; }
+
 void ws_reserve(dyng_hip_workspace* ws, int64_t n, int64_t slots, hipStream_t s) {
-  const int64_t want_chunks = slots / 16 + n + 4096;
+  const int64_t want_chunks = chunks_for(n, slots);
   if (n > ws->capacity) {
     const int64_t cap = std::max<int64_t>(n, 1);
     ws->packed.reserve(cap);
@@ -1115,26 +1119,24 @@ int bits_for_parents(int64_t n) {
 
 }  // namespace
 
-engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s) {
-  if (run == nullptr || run->graph == nullptr || run->workspace == nullptr) {
-    fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: sssp run: graph and workspace are required");
-  }
+namespace {
+
+/// Where a run's scratch lives: the device arrays, the control block and the barrier state.
+struct engine_scratch {
+  u64* packed;
+  int *stamp, *in_far, *flag, *ancestor;
+  int *cand, *frontier, *near_a, *near_b, *far_a, *far_b;
+  int4* chunks;
+  int64_t chunk_cap;
+  control* ctl;
+  grid_sync_state* bar;
+};
+
+/// The checks and the parameters of a run that do not depend on where its scratch lives (a
+/// workspace, or the no-allocation entry point's static array). n > 0.
+void engine_setup(const dyng_hip_sssp_run* run, bool update, engine_params& p, int64_t& timeout_us) {
   const dyng_hip_graph* g = run->graph;
-  dyng_hip_workspace* ws = run->workspace;
   const int64_t n = g->n;
-  engine_pending pend{};
-  pend.ws = ws;
-  pend.s = s;
-  pend.update = update;
-  pend.n = n;
-  pend.distances = run->distances;
-  dyng_hip_counters& counters = pend.counters;
-  counters.engine_used = DYNG_HIP_ENGINE_FUSED;
-  counters.packed_parents = 1;
-  if (n == 0) {
-    pend.launched = false;
-    return pend;
-  }
   if (run->delta <= 0) fail(DYNG_HIP_INVALID_ARGUMENT, "sssp: the near-far width delta must be > 0, got " + std::to_string(run->delta));
   if (run->source == -1) {
     if (run->sources == nullptr || run->num_sources < 1 || run->num_sources > n) {
@@ -1166,12 +1168,7 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
     pk.bits = 0;
     pk.no_parent = 0;
   }
-  ws_reserve(ws, n, std::max(g->out.slot_cap, g->in.slot_cap), s);
-  if (ws->generation > INT32_MAX / 2) {
-    DYNG_HIP_TRY(hipMemsetAsync(ws->stamp.p, 0, ws->capacity * sizeof(int), s));
-    ws->generation = 0;
-  }
-  engine_params p{};
+  p = engine_params{};
   p.n = static_cast<int32_t>(n);
   const int k = run->objective;
   p.out_meta = g->out.meta.p;
@@ -1208,47 +1205,146 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
   int rounds = 0;
   while ((1LL << rounds) < static_cast<long long>(n)) ++rounds;
   p.max_rounds = rounds + 1;
+  p.distances = reinterpret_cast<long long*>(run->distances);
+  p.parents = run->parents;
+  p.light = run->tuning.light_degree > 0 ? run->tuning.light_degree : 4096;
+  p.chunk_edges = run->tuning.chunk_edges > 0 ? std::max(16, run->tuning.chunk_edges) : 256;
+  // the real-time counter runs at 100 MHz: 100 ticks per microsecond
+  timeout_us = run->tuning.barrier_timeout_us > 0   ? run->tuning.barrier_timeout_us
+               : run->tuning.barrier_timeout_us == 0 ? 4000000
+                                                     : 0;
+  p.barrier_ticks = static_cast<unsigned long long>(timeout_us) * 100ULL;
+}
+
+void attach_scratch(engine_params& p, const engine_scratch& sc) {
+  p.packed = sc.packed;
+  p.stamp = sc.stamp;
+  p.in_far = sc.in_far;
+  p.flag = sc.flag;
+  p.ancestor = sc.ancestor;
+  p.cand = sc.cand;
+  p.frontier = sc.frontier;
+  p.near_a = sc.near_a;
+  p.near_b = sc.near_b;
+  p.far_a = sc.far_a;
+  p.far_b = sc.far_b;
+  p.chunks = sc.chunks;
+  p.chunk_cap = sc.chunk_cap;
+  p.ctl = sc.ctl;
+  p.bar = sc.bar;
+}
+
+/// Every workgroup must be resident for the grid barrier. The launch stays at half of the
+/// occupancy the runtime reports (one 1024-thread workgroup per CU by default), never at the edge
+/// of it; the barrier's bounded wait turns a grid that is not resident after all into an error.
+int grid_blocks(const dyng_hip_tuning& t, int occupancy, int cu_count) {
+  const int resident = occupancy * cu_count;  // what the runtime says fits at once
+  const int cap = std::max(1, resident / 2);
+  const int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : 1;
+  return std::max(1, std::min(per_cu * cu_count, cap));
+}
+
+/// The errors a run's control block reports, as the reference's device errors.
+void throw_on_errors(const control* hc, int64_t timeout_us) {
+  if (hc->overflow != 0) {
+    throw status_error(DYNG_HIP_DEVICE_ERROR,
+                       "sssp: an input distance is negative or larger than (n - 1) * max weight; the initial tree "
+                       "does not belong to this graph",
+                       DYNG_HIP_DEVICE_ERROR_INVALID_INPUT);
+  }
+  if (hc->error_bits & DYNG_HIP_DEVICE_ERROR_TIMEOUT) {
+    throw status_error(DYNG_HIP_DEVICE_ERROR,
+                       "sssp: a grid barrier wait of the persistent kernel ran past " + std::to_string(timeout_us) +
+                           " us (its workgroups were not all resident: is other work filling the device?)",
+                       hc->error_bits);
+  }
+  if (hc->error_bits & DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE) {
+    throw status_error(DYNG_HIP_DEVICE_ERROR,
+                       "sssp: the input shortest-path tree has a parent cycle (pointer jumping did not reach a root "
+                       "in ceil(log2 n) + 1 rounds)",
+                       DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE);
+  }
+  if (hc->error_bits != 0) {
+    throw status_error(DYNG_HIP_DEVICE_ERROR, "sssp: the device reported error bits " + std::to_string(hc->error_bits),
+                       hc->error_bits);
+  }
+}
+
+/// The counters of a finished run from its control block.
+void read_counters(const control* hc, bool update, int bits, int blocks, dyng_hip_counters& counters) {
+  counters.invalidated = hc->invalidated;
+  counters.iterations = hc->iterations;
+  counters.epochs = hc->epochs;
+  counters.pushes = hc->pushes;
+  counters.affected = static_cast<int64_t>(hc->affected);
+  counters.packed_parents = bits > 0 ? 1 : 0;
+  counters.relax_vertices = hc->pushes;
+  counters.relax_edges = static_cast<int64_t>(hc->relax_edges);
+  counters.relax_improved = static_cast<int64_t>(hc->relax_improved);
+  counters.pull_edges = static_cast<int64_t>(hc->pull_edges);
+  counters.jump_rounds = hc->rounds;
+  counters.jump_items = static_cast<int64_t>(hc->jump_items);
+  counters.pull_vertices = static_cast<int64_t>(hc->pull_vertices);
+  // s_memrealtime runs at 100 MHz
+  const double tick_ms = 1e-5;
+  const unsigned long long t0 = hc->t_phase[0];
+  const unsigned long long t1 = update ? hc->t_phase[1] : t0;
+  const unsigned long long t2 = hc->t_phase[2];
+  const unsigned long long t3 = hc->t_phase[3];
+  counters.step1_ms = (t2 >= t0 ? t2 - t0 : 0) * tick_ms;
+  counters.pull_ms = (update && t2 >= t1 ? t2 - t1 : 0) * tick_ms;
+  counters.relax_ms = (t3 >= t2 ? t3 - t2 : 0) * tick_ms;
+  const unsigned long long ta = hc->t_phase[4], tb = hc->t_phase[5], t7 = hc->t_phase[7];
+  counters.pack_ms = update && ta >= t0 ? (ta - t0) * tick_ms : 0.0;
+  counters.jump_ms = update && tb >= ta ? (tb - ta) * tick_ms : 0.0;
+  counters.unpack_ms = t7 >= t3 ? (t7 - t3) * tick_ms : 0.0;
+  counters.blocks = blocks;
+}
+
+}  // namespace
+
+engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s) {
+  if (run == nullptr || run->graph == nullptr || run->workspace == nullptr) {
+    fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: sssp run: graph and workspace are required");
+  }
+  const dyng_hip_graph* g = run->graph;
+  dyng_hip_workspace* ws = run->workspace;
+  const int64_t n = g->n;
+  engine_pending pend{};
+  pend.ws = ws;
+  pend.s = s;
+  pend.update = update;
+  pend.n = n;
+  pend.distances = run->distances;
+  dyng_hip_counters& counters = pend.counters;
+  counters.engine_used = DYNG_HIP_ENGINE_FUSED;
+  counters.packed_parents = 1;
+  if (n == 0) {
+    pend.launched = false;
+    return pend;
+  }
+  engine_params p{};
+  int64_t timeout_us = 0;
+  engine_setup(run, update, p, timeout_us);
+  ws_reserve(ws, n, std::max(g->out.slot_cap, g->in.slot_cap), s);
+  if (ws->generation > INT32_MAX / 2) {
+    DYNG_HIP_TRY(hipMemsetAsync(ws->stamp.p, 0, ws->capacity * sizeof(int), s));
+    ws->generation = 0;
+  }
   p.generation = ++ws->generation;
   // the workspace still holds this tree's words when the caller's token is the one the last
   // successful run handed out for them (same size, packing and arrays)
-  p.resident = update && !recompute && !run->check_cycles && pk.bits > 0 && run->tree_token != 0 &&
-                       run->tree_token == ws->words_token && ws->words_n == n && ws->words_bits == pk.bits &&
+  const bool recompute = update && run->tuning.recompute != 0;
+  p.resident = update && !recompute && !run->check_cycles && p.pk.bits > 0 && run->tree_token != 0 &&
+                       run->tree_token == ws->words_token && ws->words_n == n && ws->words_bits == p.pk.bits &&
                        ws->words_distances == run->distances
                    ? 1
                    : 0;
   ws->words_token = 0;  // until this run ends well
-  p.distances = reinterpret_cast<long long*>(run->distances);
-  p.parents = run->parents;
-  p.packed = ws->packed.p;
-  p.stamp = ws->stamp.p;
-  p.in_far = ws->in_far.p;
-  p.flag = ws->flag.p;
-  p.ancestor = ws->ancestor.p;
-  p.cand = ws->cand.p;
-  p.frontier = ws->frontier.p;
-  p.near_a = ws->near_a.p;
-  p.near_b = ws->near_b.p;
-  p.far_a = ws->far_a.p;
-  p.far_b = ws->far_b.p;
-  p.chunks = ws->chunks.p;
-  p.chunk_cap = ws->chunk_cap;
-  p.ctl = ws->ctl.p;
-  p.bar = ws->bar.p;
-  p.light = run->tuning.light_degree > 0 ? run->tuning.light_degree : 4096;
-  p.chunk_edges = run->tuning.chunk_edges > 0 ? std::max(16, run->tuning.chunk_edges) : 256;
-  // the real-time counter runs at 100 MHz: 100 ticks per microsecond
-  const int64_t timeout_us = run->tuning.barrier_timeout_us > 0   ? run->tuning.barrier_timeout_us
-                             : run->tuning.barrier_timeout_us == 0 ? 4000000
-                                                                   : 0;
-  p.barrier_ticks = static_cast<unsigned long long>(timeout_us) * 100ULL;
-  // Every workgroup must be resident for the grid barrier. The launch stays at half of the
-  // occupancy the runtime reports (one 1024-thread workgroup per CU by default), never at the
-  // edge of it; the barrier's bounded wait turns a grid that is not resident after all into an
-  // error.
-  const int resident = ws->occupancy * ws->cu_count;  // what the runtime says fits at once
-  const int cap = std::max(1, resident / 2);
-  const int per_cu = run->tuning.blocks_per_cu > 0 ? run->tuning.blocks_per_cu : 1;
-  const int blocks = std::max(1, std::min(per_cu * ws->cu_count, cap));
+  attach_scratch(p, engine_scratch{ws->packed.p, ws->stamp.p, ws->in_far.p, ws->flag.p, ws->ancestor.p, ws->cand.p,
+                                   ws->frontier.p, ws->near_a.p, ws->near_b.p, ws->far_a.p, ws->far_b.p,
+                                   ws->chunks.p, ws->chunk_cap, ws->ctl.p, ws->bar.p});
+  const int blocks = grid_blocks(run->tuning, ws->occupancy, ws->cu_count);
   control* hc = ws->host_ctl.p;
   std::memset(hc, 0, sizeof(control));
   hc->min_frontier = packed_inf;
@@ -1299,7 +1395,7 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
     DYNG_HIP_TRY(hipMemcpyAsync(hc, ws->ctl.p, sizeof(control), hipMemcpyDeviceToHost, s));
   }
   pend.launched = true;
-  pend.bits = pk.bits;
+  pend.bits = p.pk.bits;
   pend.blocks = blocks;
   pend.resident = p.resident;
   pend.timeout_us = timeout_us;
@@ -1330,56 +1426,9 @@ void engine_finish(engine_pending& pend, dyng_hip_counters* out) {
     DYNG_HIP_TRY(hipStreamSynchronize(s));
     ws->generation = 0;
   }
-  if (hc->overflow != 0) {
-    throw status_error(DYNG_HIP_DEVICE_ERROR,
-                       "sssp: an input distance is negative or larger than (n - 1) * max weight; the initial tree "
-                       "does not belong to this graph",
-                       DYNG_HIP_DEVICE_ERROR_INVALID_INPUT);
-  }
-  if (hc->error_bits & DYNG_HIP_DEVICE_ERROR_TIMEOUT) {
-    throw status_error(DYNG_HIP_DEVICE_ERROR,
-                       "sssp: a grid barrier wait of the persistent kernel ran past " + std::to_string(timeout_us) +
-                           " us (its workgroups were not all resident: is other work filling the device?)",
-                       hc->error_bits);
-  }
-  if (hc->error_bits & DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE) {
-    throw status_error(DYNG_HIP_DEVICE_ERROR,
-                       "sssp: the input shortest-path tree has a parent cycle (pointer jumping did not reach a root "
-                       "in ceil(log2 n) + 1 rounds)",
-                       DYNG_HIP_DEVICE_ERROR_PARENT_CYCLE);
-  }
-  if (hc->error_bits != 0) {
-    throw status_error(DYNG_HIP_DEVICE_ERROR, "sssp: the device reported error bits " + std::to_string(hc->error_bits),
-                       hc->error_bits);
-  }
+  throw_on_errors(hc, timeout_us);
   ws->generation = std::max(ws->generation, hc->generation);
-  counters.invalidated = hc->invalidated;
-  counters.iterations = hc->iterations;
-  counters.epochs = hc->epochs;
-  counters.pushes = hc->pushes;
-  counters.affected = static_cast<int64_t>(hc->affected);
-  counters.packed_parents = pend.bits > 0 ? 1 : 0;
-  counters.relax_vertices = hc->pushes;
-  counters.relax_edges = static_cast<int64_t>(hc->relax_edges);
-  counters.relax_improved = static_cast<int64_t>(hc->relax_improved);
-  counters.pull_edges = static_cast<int64_t>(hc->pull_edges);
-  counters.jump_rounds = hc->rounds;
-  counters.jump_items = static_cast<int64_t>(hc->jump_items);
-  counters.pull_vertices = static_cast<int64_t>(hc->pull_vertices);
-  // s_memrealtime runs at 100 MHz
-  const double tick_ms = 1e-5;
-  const unsigned long long t0 = hc->t_phase[0];
-  const unsigned long long t1 = update ? hc->t_phase[1] : t0;
-  const unsigned long long t2 = hc->t_phase[2];
-  const unsigned long long t3 = hc->t_phase[3];
-  counters.step1_ms = (t2 >= t0 ? t2 - t0 : 0) * tick_ms;
-  counters.pull_ms = (update && t2 >= t1 ? t2 - t1 : 0) * tick_ms;
-  counters.relax_ms = (t3 >= t2 ? t3 - t2 : 0) * tick_ms;
-  const unsigned long long ta = hc->t_phase[4], tb = hc->t_phase[5], t7 = hc->t_phase[7];
-  counters.pack_ms = update && ta >= t0 ? (ta - t0) * tick_ms : 0.0;
-  counters.jump_ms = update && tb >= ta ? (tb - ta) * tick_ms : 0.0;
-  counters.unpack_ms = t7 >= t3 ? (t7 - t3) * tick_ms : 0.0;
-  counters.blocks = pend.blocks;
+  read_counters(hc, update, pend.bits, pend.blocks, counters);
   counters.kernel_ms = kernel_ms;
   counters.words_kept = pend.resident;
   if (pend.bits > 0) {
@@ -1399,6 +1448,187 @@ namespace {
 void run_engine(const dyng_hip_sssp_run* run, bool update, hipStream_t s, dyng_hip_counters* out) {
   engine_pending pend = engine_launch(run, update, s);
   engine_finish(pend, out);
+}
+
+}  // namespace
+
+}  // namespace dyng_hip
+
+// ---------------------------------------------------------------------------------------------
+// The update that allocates nothing: its scratch is one static device array
+// ---------------------------------------------------------------------------------------------
+
+/// The scratch of dyng_hip_sssp_update_noalloc: the loader places one copy in each device's memory
+/// when the code object loads there, and it stays until the process ends. Host code never passes
+/// this name to a kernel or a copy (on the host it only stands for the symbol): it uses the device
+/// address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[DYNG_HIP_NOALLOC_SCRATCH_BYTES];
+
+namespace dyng_hip {
+
+namespace {
+
+__global__ void k_clear_scratch(unsigned long long* p, int64_t words) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = 0;
+}
+
+void clear_scratch(void* p, int64_t bytes, hipStream_t s) {
+  const int64_t words = bytes / 8;  // every region is a multiple of 256 bytes
+  if (words <= 0) return;
+  const int64_t groups = std::min<int64_t>((words + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_clear_scratch, dim3(static_cast<unsigned>(groups)), dim3(256), 0, s,
+                     static_cast<unsigned long long*>(p), words);
+  DYNG_HIP_TRY(hipGetLastError());
+}
+
+/// Where the arrays of a run over n vertices (row arenas of `slots` slots) sit in the scratch:
+/// byte offsets, each region rounded up to 256 bytes.
+struct scratch_layout {
+  int64_t packed, stamp, in_far, flag, ancestor, cand, frontier, near_a, near_b, far_a, far_b, chunks, ctl, bar;
+  int64_t chunk_cap;
+  int64_t total;
+};
+
+scratch_layout layout_for(int64_t n, int64_t slots) {
+  scratch_layout L{};
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) {
+    const int64_t at = off;
+    off += (bytes + 255) / 256 * 256;
+    return at;
+  };
+  L.packed = take(8 * n);
+  L.stamp = take(4 * n);
+  L.in_far = take(4 * n);
+  L.flag = take(4 * n);
+  L.ancestor = take(4 * n);
+  L.cand = take(4 * (2 * n + 64));
+  L.frontier = take(4 * n);
+  L.near_a = take(4 * n);
+  L.near_b = take(4 * n);
+  L.far_a = take(4 * n);
+  L.far_b = take(4 * n);
+  L.chunk_cap = chunks_for(n, slots);
+  L.chunks = take(static_cast<int64_t>(sizeof(int4)) * L.chunk_cap);
+  L.ctl = take(sizeof(control));
+  L.bar = take(sizeof(grid_sync_state));
+  L.total = off;
+  return L;
+}
+
+/// What the host keeps per device for the static scratch: its device address, the stamps'
+/// generation, the launch shape, and the control block read back into plain host memory.
+struct noalloc_device {
+  bool ready = false;
+  unsigned char* ptr = nullptr;
+  int generation = 0;
+  int64_t last_n = 0;  ///< the vertex count the stamps, far marks and flags were laid out for
+  int cu_count = 0;
+  int occupancy = 0;
+  control host_ctl{};
+};
+
+constexpr int kNoallocDevices = 64;
+noalloc_device noalloc_state[kNoallocDevices];
+std::mutex noalloc_setup_lock;
+
+/// Once per device, with the device current: the scratch's address on it (hipGetSymbolAddress),
+/// cleared once, so stamps, far marks and flags start at zero; and the launch shape.
+noalloc_device& noalloc_for(int device, hipStream_t s) {
+  if (device < 0 || device >= kNoallocDevices) {
+    fail(DYNG_HIP_NOT_SUPPORTED, "dyng: sssp update (no allocation): device " + std::to_string(device) +
+                                     " is past the " + std::to_string(kNoallocDevices) + " devices it serves");
+  }
+  noalloc_device& st = noalloc_state[device];
+  std::lock_guard<std::mutex> hold(noalloc_setup_lock);
+  if (st.ready) return st;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    (void)hipGetLastError();
+    fail(DYNG_HIP_RUNTIME_ERROR, std::string("dyng: sssp update (no allocation): hipGetSymbolAddress of the scratch "
+                                             "array failed: ") +
+                                     hipGetErrorString(e));
+  }
+  DYNG_HIP_TRY(hipDeviceGetAttribute(&st.cu_count, hipDeviceAttributeMultiprocessorCount, device));
+  DYNG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&st.occupancy,
+                                                            reinterpret_cast<const void*>(sssp_fused_kernel), kBlock,
+                                                            0));
+  if (st.occupancy < 1) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: sssp: the persistent kernel does not fit on a CU");
+  clear_scratch(ptr, DYNG_HIP_NOALLOC_SCRATCH_BYTES, s);
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  st.ptr = static_cast<unsigned char*>(ptr);
+  st.generation = 0;
+  st.ready = true;
+  return st;
+}
+
+void run_engine_noalloc(const dyng_hip_sssp_run* run, hipStream_t s, dyng_hip_counters* out) {
+  if (run == nullptr || run->graph == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: sssp run: a graph is required");
+  const dyng_hip_graph* g = run->graph;
+  const int64_t n = g->n;
+  dyng_hip_counters counters{};
+  counters.engine_used = DYNG_HIP_ENGINE_FUSED;
+  counters.packed_parents = 1;
+  if (n == 0) {
+    if (out) *out = counters;
+    return;
+  }
+  engine_params p{};
+  int64_t timeout_us = 0;
+  engine_setup(run, true, p, timeout_us);
+  const scratch_layout L = layout_for(n, std::max(g->out.slot_cap, g->in.slot_cap));
+  if (L.total > static_cast<int64_t>(DYNG_HIP_NOALLOC_SCRATCH_BYTES)) {
+    fail(DYNG_HIP_INVALID_ARGUMENT,
+         "dyng: sssp update (no allocation): a graph of " + std::to_string(n) + " vertices and " +
+             std::to_string(std::max(g->out.slot_cap, g->in.slot_cap)) + " row slots needs " +
+             std::to_string(L.total) + " bytes of scratch, more than the " +
+             std::to_string(static_cast<int64_t>(DYNG_HIP_NOALLOC_SCRATCH_BYTES)) +
+             " of the static array; use dyng_hip_sssp_update with a workspace");
+  }
+  scoped_device guard(g->device);
+  noalloc_device& st = noalloc_for(g->device, s);
+  unsigned char* base = st.ptr;
+  // Stamps, far marks and flags must be zero (or of an older generation) when a run starts, and a
+  // run leaves them so. Their place in the array moves with n: a call on another vertex count
+  // (or after a run that stopped early) finds the last call's other arrays there, so it clears
+  // them first.
+  if (n != st.last_n || st.generation > INT32_MAX / 2) {
+    clear_scratch(base + L.stamp, 4 * n, s);
+    clear_scratch(base + L.in_far, 4 * n, s);
+    clear_scratch(base + L.flag, 4 * n, s);
+    st.generation = 0;
+    st.last_n = n;
+  }
+  p.generation = ++st.generation;
+  p.resident = 0;  // the old tree's words are always packed from its arrays
+  attach_scratch(p, engine_scratch{reinterpret_cast<u64*>(base + L.packed), reinterpret_cast<int*>(base + L.stamp),
+                                   reinterpret_cast<int*>(base + L.in_far), reinterpret_cast<int*>(base + L.flag),
+                                   reinterpret_cast<int*>(base + L.ancestor), reinterpret_cast<int*>(base + L.cand),
+                                   reinterpret_cast<int*>(base + L.frontier), reinterpret_cast<int*>(base + L.near_a),
+                                   reinterpret_cast<int*>(base + L.near_b), reinterpret_cast<int*>(base + L.far_a),
+                                   reinterpret_cast<int*>(base + L.far_b), reinterpret_cast<int4*>(base + L.chunks),
+                                   L.chunk_cap, reinterpret_cast<control*>(base + L.ctl),
+                                   reinterpret_cast<grid_sync_state*>(base + L.bar)});
+  const int blocks = grid_blocks(run->tuning, st.occupancy, st.cu_count);
+  hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, s, p.bar, p.ctl);
+  hipLaunchKernelGGL(sssp_fused_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
+  DYNG_HIP_TRY(hipGetLastError());
+  control* hc = &st.host_ctl;
+  DYNG_HIP_TRY(hipMemcpyAsync(hc, p.ctl, sizeof(control), hipMemcpyDeviceToHost, s));
+  DYNG_HIP_TRY(hipStreamSynchronize(s));
+  if (hc->overflow != 0 || hc->error_bits != 0) {
+    // a run that stopped early can leave flags, far marks and stamps of any later generation
+    // behind: the next call clears them
+    st.last_n = 0;  // the next call clears them
+  }
+  throw_on_errors(hc, timeout_us);
+  st.generation = std::max(st.generation, hc->generation);
+  read_counters(hc, true, p.pk.bits, blocks, counters);
+  // no events (creating one may allocate): the kernel's own clock from its first to its last stamp
+  counters.kernel_ms = hc->t_phase[7] >= hc->t_phase[0] ? (hc->t_phase[7] - hc->t_phase[0]) * 1e-5 : 0.0;
+  if (out) *out = counters;
 }
 
 }  // namespace
@@ -1425,4 +1655,8 @@ extern "C" int dyng_hip_sssp_update(const dyng_hip_sssp_run* run, void* stream, 
 
 extern "C" int dyng_hip_sssp_compute(const dyng_hip_sssp_run* run, void* stream, dyng_hip_counters* out) {
   return guarded([&] { run_engine(run, false, static_cast<hipStream_t>(stream), out); });
+}
+
+extern "C" int dyng_hip_sssp_update_noalloc(const dyng_hip_sssp_run* run, void* stream, dyng_hip_counters* out) {
+  return guarded([&] { run_engine_noalloc(run, static_cast<hipStream_t>(stream), out); });
 }

@@ -237,6 +237,18 @@ int dyng_hip_sssp_update(const dyng_hip_sssp_run* run, void* stream, dyng_hip_co
 /* sssp_cuda_compute (cuda.cu:309-335): the canonical tree from the source. */
 int dyng_hip_sssp_compute(const dyng_hip_sssp_run* run, void* stream, dyng_hip_counters* out);
 
+/* No counterpart in the reference: dyng_hip_sssp_update for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * same run and result, but the engine's scratch is one static device array of
+ * DYNG_HIP_NOALLOC_SCRATCH_BYTES per device instead of run->workspace, which is ignored, as are
+ * tree_token (the old tree is always packed from its arrays) and tuning.launch (a plain launch).
+ * It calls no hipMalloc, hipFree or allocating library routine. A graph that needs more scratch
+ * (68 B per vertex plus 1 B per row slot, plus 64 KiB) is refused with DYNG_HIP_INVALID_ARGUMENT
+ * before anything runs. Every call on a device shares that device's array, which keeps the last
+ * call's contents: calls that overlap on one device race. */
+#define DYNG_HIP_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int dyng_hip_sssp_update_noalloc(const dyng_hip_sssp_run* run, void* stream, dyng_hip_counters* out);
+
 /* Device memory and synchronization for callers without their own HIP runtime plumbing (the
  * reference does this through rmm-like memory resources, cpp/include/dyng/core/memory.hpp):
  * device count, allocation, host<->device copies, a device-wide synchronize. */
