@@ -133,10 +133,16 @@ def make_inputs(name: str, batches_needed: int):
     return n, src, dst, w, batch, source, ni + nd + nr
 
 
+# The CPUs this process may run on, read at start: once an OpenMP region has run with
+# OMP_PROC_BIND set (the generators' library runs one), the main thread is bound to one core and
+# its own affinity shrinks to that core.
+STARTUP_CPUS = sorted(os.sched_getaffinity(0))
+
+
 def host_cpus() -> dict:
     """The host cores this process may run on: CPU model, logical CPUs, physical cores and the
     sockets they sit on (from /proc/cpuinfo and sysfs topology)."""
-    cpus = sorted(os.sched_getaffinity(0))
+    cpus = STARTUP_CPUS
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -318,7 +324,7 @@ def batched_bench(args) -> int:
 
     from concurrent.futures import ThreadPoolExecutor
 
-    with ThreadPoolExecutor(max_workers=min(16, len(os.sched_getaffinity(0)))) as pool:
+    with ThreadPoolExecutor(max_workers=min(16, len(STARTUP_CPUS))) as pool:
         made = list(pool.map(make, range(count)))  # the generators release the GIL
     graphs = [x[0] for x in made]
     sources = [x[1] for x in made]
@@ -382,7 +388,7 @@ def batched_bench(args) -> int:
     if not args.no_cpu and args.cpu_sample > 0:
         import oracle
 
-        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        threads = args.cpu_threads or len(STARTUP_CPUS)
         order = np.argsort(scales, kind="stable")
         sample = sorted(set(order[:4].tolist() + order[len(order) // 2::max(1, len(order) // 8)][:4].tolist()))
         d_all, p_all = tree.to_numpy()
@@ -537,7 +543,7 @@ def mosp_bench(args) -> int:
                 "kernel_ms_mean": round(statistics.mean(kms), 4), "algorithmic_bytes_mean": int(sum(kbytes) / len(kbytes))}
     cpu, exact = None, None
     if not args.no_cpu and args.cpu_sample > 0:
-        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        threads = args.cpu_threads or len(STARTUP_CPUS)
         sample = min(args.cpu_sample, 2)
         parts, results = mosp_cpu_baseline(n, src, dst, w, drawn, source, sample, threads)
         total = [sum(v for k, v in p.items() if k != "repairs_s") for p in parts]

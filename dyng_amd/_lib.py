@@ -52,7 +52,7 @@ class Batch(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("engine", C.c_int32), ("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
+    _fields_ = [("engine", C.c_int32), ("grid_blocks", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("light_degree", C.c_int32), ("chunk_edges", C.c_int32), ("launch", C.c_int32),
                 ("recompute", C.c_int32), ("barrier_timeout_us", C.c_int32)]
 

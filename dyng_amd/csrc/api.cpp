@@ -610,7 +610,7 @@ extern "C" int dyng_hip_api_update_each(dyng_hip_graph* g, const dyng_hip_batch*
           dyng_hip_sssp_run run = make_run(g, r, delta[i], max_weight[i]);
           run.workspace = pool().get(g->device, static_cast<int>(i % kWorkspaceSlots));
           r->tree_token = 0;
-          pend[i] = engine_launch(&run, true, (i % 2 == 0) ? s : s2);
+          pend[i] = engine_launch(&run, true, (i % 2 == 0) ? s : s2, /*concurrent=*/count > 1);
           ok[i] = 1;
         } catch (...) {
           r->poisoned = true;

@@ -213,7 +213,9 @@ struct engine_pending {
   int64_t* distances = nullptr;
   dyng_hip_counters counters{};
 };
-engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s);
+/// `concurrent`: the run is launched to overlap another persistent launch on the device (each
+/// then takes at most half of what fits).
+engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s, bool concurrent = false);
 void engine_finish(engine_pending& pend, dyng_hip_counters* out);
 /// The scheduler's pick for a scanned batch on G_t: features, BF16 inference, argmin (one sync).
 int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents, int64_t n_tree,

@@ -254,16 +254,22 @@ DYNG_DEV int block_scan(tile_shared& L, int x, int& total) {
   return L.scan[w] + v - x;
 }
 
+/// Edge slots a lane takes per step of an expansion: their loads are issued together (each slot's
+/// row entry, then the words it points at), so a lane waits for one round of gathers per K slots
+/// instead of one per slot.
+constexpr int kSlots = 1;
+
 /// Edge-balanced expansion of list items [0, count): the items are cut into one tile per
 /// workgroup (up to kBlock items), `load(index, v, off, len, word)` fills a thread's item (len 0:
 /// nothing to expand in the tile, e.g. a long row sent to the chunk list), and the edge slots of
-/// the tile's rows are dealt out to the workgroup's threads, one slot per thread per step:
-/// `edge(act, item, slot)` runs with every lane of a wave (act false on padding lanes). `done(valid,
-/// item)` runs on every thread after the tile's edges, and `flush()` on every thread of the
-/// workgroup after each tile (to publish staged appends). Uniform per workgroup: every thread
-/// calls it.
-template <class Load, class Edge, class Done, class Flush>
-DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edge edge, Done done, Flush flush) {
+/// the tile's rows are dealt out to the workgroup's threads, kSlots slots per thread per step
+/// (slot j of a step kBlock apart from slot j - 1, so each j is one coalesced run per wave):
+/// `edges(act, item, slot)` runs with every lane of a wave, on arrays of kSlots (act false on
+/// padding). `done(valid, item)` runs on every thread after the tile's edges, and `flush()` on
+/// every thread of the workgroup after each tile (to publish staged appends). Uniform per
+/// workgroup: every thread calls it.
+template <class Load, class Edges, class Done, class Flush>
+DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edges edges, Done done, Flush flush) {
   const int G = static_cast<int>(gridDim.x);
   const int ts = min(kBlock, max(1, (count + G - 1) / G));
   const int i = static_cast<int>(threadIdx.x);
@@ -282,23 +288,29 @@ DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edge edge, Done do
     }
     if (i == 0) L.start[cnt] = total;
     __syncthreads();
-    for (int base = i - lane(); base < total; base += kBlock) {
-      const int e = base + lane();
-      const bool act = e < total;
-      int item = 0;
-      if (act) {
-        int lo = 0, hi = cnt;  // the last item whose first slot is <= e (its row holds e)
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (L.start[mid] <= e) {
-            lo = mid;
-          } else {
-            hi = mid;
+    for (int base = i - lane(); base < total; base += kSlots * kBlock) {
+      bool act[kSlots];
+      int item[kSlots], slot[kSlots];
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const int e = base + j * kBlock + lane();
+        act[j] = e < total;
+        int lo = 0;
+        if (act[j]) {
+          int hi = cnt;  // the last item whose first slot is <= e (its row holds e)
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (L.start[mid] <= e) {
+              lo = mid;
+            } else {
+              hi = mid;
+            }
           }
         }
-        item = lo;
+        item[j] = lo;
+        slot[j] = act[j] ? L.off[lo] + (e - L.start[lo]) : 0;
       }
-      edge(act, item, act ? L.off[item] + (e - L.start[item]) : 0);
+      edges(act, item, slot);
     }
     __syncthreads();
     done(i < cnt, i);
@@ -314,17 +326,20 @@ DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edge edge, Done do
 /// generation a claim compares with, so it claims nothing. The unpack visits the stamped ones.
 DYNG_DEV void mark_changed(const engine_params& p, int32_t w) { atomicMax(&p.stamp[w], p.generation); }
 
-/// One relaxation of edge (u, w) of weight wt with u at distance du (fused.cuh:341-362): the
-/// candidate word joins the next near list or the far pile when it lowered w's word.
-DYNG_DEV void relax(const engine_params& p, bool active, int32_t u, u64 du, int32_t w, int32_t wt,
-                    u64 threshold, int generation, stage_shared& S, int* next, int* next_count, int* far,
-                    int* far_count, unsigned long long* improved) {
-  bool to_near = false, to_far = false;
-  if (active && w != p.source) {
+/// One relaxation of edge (u, w) of weight wt with u at distance du (fused.cuh:341-362), given
+/// w's word as read before (`seen`; it only falls during the relax, so a stale read is never
+/// below the truth and only costs an atomic): the candidate word joins the next near list or the
+/// far pile when it lowered w's word. Returns the lists to append w to.
+DYNG_DEV void relax_decide(const engine_params& p, bool active, int32_t u, u64 du, int32_t w, int32_t wt, u64 seen,
+                           u64 threshold, int generation, unsigned long long* improved, bool& to_near,
+                           bool& to_far) {
+  to_near = false;
+  to_far = false;
+  if (active) {
     const u64 nd = du + static_cast<u64>(wt);
     if (nd <= p.max_distance) {
       const u64 cand = p.pk.pack(nd, u);
-      if (cand < ld(&p.packed[w])) {
+      if (cand < seen) {
         const u64 old = atomicMin(&p.packed[w], cand);
         // Any lowered word joins a list, also one lowered only to a lower parent id at the same
         // distance (fused.cuh:341-362 lists only a lower distance): the extra expansion is rare
@@ -340,31 +355,71 @@ DYNG_DEV void relax(const engine_params& p, bool active, int32_t u, u64 du, int3
       }
     }
   }
-  stage_append(S, 0, to_near, w, next, next_count);
-  stage_append(S, 1, to_far, w, far, far_count);
 }
 
-/// Chunks of long rows, one wave per chunk: `edge(act, u, slot)` with every lane of the wave.
-template <class Edge>
-DYNG_DEV void chunk_rows(const engine_params& p, int nchunks, Edge edge) {
+/// kSlots relaxations per lane: the row entries first, then the words they point at, then the
+/// atomics and the list appends. Every lane of the wave calls it.
+DYNG_DEV void relax_slots(const engine_params& p, const bool (&act)[kSlots], const int32_t (&u)[kSlots],
+                          const u64 (&du)[kSlots], const int (&slot)[kSlots], u64 threshold, int generation,
+                          stage_shared& S, int* next, int* next_count, int* far, int* far_count,
+                          unsigned long long* improved) {
+  int32_t w[kSlots], wt[kSlots];
+  u64 seen[kSlots];
+  bool live[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    w[j] = act[j] ? p.out_nbr[slot[j]] : -1;
+    wt[j] = act[j] ? p.out_w[slot[j]] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    live[j] = act[j] && w[j] >= 0 && w[j] != p.source;  // a dead entry is negative
+    seen[j] = live[j] ? ld(&p.packed[w[j]]) : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    bool to_near, to_far;
+    relax_decide(p, live[j], u[j], du[j], w[j], wt[j], seen[j], threshold, generation, improved, to_near, to_far);
+    stage_append(S, 0, to_near, w[j], next, next_count);
+    stage_append(S, 1, to_far, w[j], far, far_count);
+  }
+}
+
+/// Chunks of long rows, one wave per chunk: `edges(act, u, slot)` with every lane of the wave, on
+/// arrays of kSlots slots (64 apart).
+template <class Edges>
+DYNG_DEV void chunk_rows(const engine_params& p, int nchunks, Edges edges) {
   const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / 64);
   const int waves = static_cast<int>((gridDim.x * blockDim.x) / 64);
   for (int c = wave; c < nchunks; c += waves) {
     const int4 ch = ld_chunk(&p.chunks[c]);
-    for (int b = ch.y; b < ch.z; b += 64) {
-      const int e = b + lane();
-      edge(e < ch.z, ch.x, e);
+    for (int b = ch.y; b < ch.z; b += 64 * kSlots) {
+      bool act[kSlots];
+      int32_t u[kSlots];
+      int slot[kSlots];
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const int e = b + j * 64 + lane();
+        act[j] = e < ch.z;
+        u[j] = ch.x;
+        slot[j] = act[j] ? e : 0;
+      }
+      edges(act, u, slot);
     }
   }
 }
 
-/// Pull offer of in-edge (u -> v, wt): the candidate word, or INF (also for a dead entry, u < 0).
-DYNG_DEV u64 pull_offer(const engine_params& p, int32_t u, int32_t wt) {
-  if (u < 0) return packed_inf;
-  const u64 word = ld(&p.packed[u]);
-  if (word == packed_inf) return packed_inf;
+/// Pull offer of in-edge (u -> v, wt) given u's word: the candidate word, or INF (also for a dead
+/// entry, u < 0).
+DYNG_DEV u64 offer_of(const engine_params& p, int32_t u, int32_t wt, u64 word) {
+  if (u < 0 || word == packed_inf) return packed_inf;
   const u64 nd = p.pk.distance(word) + static_cast<u64>(wt);
   return nd <= p.max_distance ? p.pk.pack(nd, u) : packed_inf;
+}
+
+/// Pull offer of in-edge (u -> v, wt): the candidate word, or INF (also for a dead entry, u < 0).
+DYNG_DEV u64 pull_offer(const engine_params& p, int32_t u, int32_t wt) {
+  return u < 0 ? packed_inf : offer_of(p, u, wt, ld(&p.packed[u]));
 }
 
 /// The best offer to v over the alive entries (u, v) of u's out-row (both sorted runs), or INF.
@@ -547,17 +602,24 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       // of its own, so every thread reads the same level bounds. A walk still going after
       // kWalkLevels levels (a deep tree) is finished by pointer jumping ----
       int lo = 0, hi = roots, levels = 0;
-      auto visit = [&](bool act, int32_t u, int slot, int* level, int* found) {
-        bool add = false;
-        int32_t w = -1;
-        if (act) {
-          w = p.out_nbr[slot];  // a dead entry (deleted, not yet compacted) is negative
-          if (w >= 0 && p.parents[w] == u && ld(&p.flag[w]) == 0 && atomicExch(&p.flag[w], 1) == 0) {
-            st(&p.stamp[w], generation + 1);
+      // kSlots out-edges (u, w) per lane: the entries, then parent[w] (the tree is read-only
+      // here), then the flag claims of the tree children (rare: one in-edge of each child)
+      auto visit = [&](const bool (&act)[kSlots], const int32_t (&u)[kSlots], const int (&slot)[kSlots], int* level,
+                       int* found) {
+        int32_t w[kSlots], par[kSlots];
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) w[j] = act[j] ? p.out_nbr[slot[j]] : -1;  // dead entries are negative
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) par[j] = w[j] >= 0 ? p.parents[w[j]] : -2;
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) {
+          bool add = false;
+          if (w[j] >= 0 && par[j] == u[j] && atomicExch(&p.flag[w[j]], 1) == 0) {
+            st(&p.stamp[w[j]], generation + 1);
             add = true;
           }
+          stage_append(S, 0, add, w[j], level, found);
         }
-        stage_append(S, 0, add, w, level, found);
       };
       while (hi > lo && levels < kWalkLevels) {
         jump_items += hi - lo;
@@ -582,13 +644,20 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
                 emit_chunks(p, chunk_count, v, mt.x, mt.y);
               }
             },
-            [&](bool act, int item, int slot) { visit(act, act ? L.vert[item] : -1, slot, level, found); },
+            [&](const bool (&act)[kSlots], const int (&item)[kSlots], const int (&slot)[kSlots]) {
+              int32_t u[kSlots];
+#pragma unroll
+              for (int j = 0; j < kSlots; ++j) u[j] = act[j] ? L.vert[item[j]] : -1;
+              visit(act, u, slot, level, found);
+            },
             [&](bool, int) {}, [&] { stage_flush(S, level, found); });
         DYNG_GRID_SYNC();
         const int nchunks = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap)));
         ++cstep;
         if (nchunks > 0) {
-          chunk_rows(p, nchunks, [&](bool act, int32_t u, int slot) { visit(act, u, slot, level, found); });
+          chunk_rows(p, nchunks, [&](const bool (&act)[kSlots], const int32_t (&u)[kSlots], const int (&slot)[kSlots]) {
+            visit(act, u, slot, level, found);
+          });
           stage_flush(S, level, found);
           DYNG_GRID_SYNC();
         }
@@ -730,10 +799,20 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
             emit_chunks(p, chunk_count, v, mt.x, mt.y);
           }
         },
-        [&](bool act, int item, int slot) {
-          if (act) {
-            const u64 o = pull_offer(p, p.in_nbr[slot], p.in_w[slot]);
-            if (o < L.word[item]) atomicMin(&L.word[item], o);
+        [&](const bool (&act)[kSlots], const int (&item)[kSlots], const int (&slot)[kSlots]) {
+          int32_t u[kSlots], wt[kSlots];
+          u64 word[kSlots];
+#pragma unroll
+          for (int j = 0; j < kSlots; ++j) {
+            u[j] = act[j] ? p.in_nbr[slot[j]] : -1;
+            wt[j] = act[j] ? p.in_w[slot[j]] : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < kSlots; ++j) word[j] = u[j] >= 0 ? ld(&p.packed[u[j]]) : packed_inf;
+#pragma unroll
+          for (int j = 0; j < kSlots; ++j) {
+            const u64 o = offer_of(p, u[j], wt[j], word[j]);
+            if (act[j] && o < L.word[item[j]]) atomicMin(&L.word[item[j]], o);
           }
         },
         [&](bool valid, int item) {
@@ -841,9 +920,9 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           L, near_count,
           [&](int idx, int32_t& v, int32_t& off, int32_t& len, u64& word) {
             v = ld(&cur[idx]);
+            const int2 mt = p.out_meta[v];  // issued together with the word's load
             word = ld(&p.packed[v]);
             if (word == packed_inf) return;
-            const int2 mt = p.out_meta[v];
             if (mt.y <= p.light) {
               off = mt.x;
               len = mt.y;
@@ -852,13 +931,16 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
               emit_chunks(p, chunk_count, v, mt.x, mt.y);
             }
           },
-          [&](bool act, int item, int slot) {
-            const int32_t u = act ? L.vert[item] : 0;
-            const u64 du = act ? p.pk.distance(L.word[item]) : 0;
-            const int32_t w = act ? p.out_nbr[slot] : 0;
-            const int32_t wt = act ? p.out_w[slot] : 0;
-            relax(p, act && w >= 0, u, du, w, wt, threshold, generation, S, next, next_count, far_list, far_counter,
-                  relax_improved);
+          [&](const bool (&act)[kSlots], const int (&item)[kSlots], const int (&slot)[kSlots]) {
+            int32_t u[kSlots];
+            u64 du[kSlots];
+#pragma unroll
+            for (int j = 0; j < kSlots; ++j) {
+              u[j] = act[j] ? L.vert[item[j]] : 0;
+              du[j] = act[j] ? p.pk.distance(L.word[item[j]]) : 0;
+            }
+            relax_slots(p, act, u, du, slot, threshold, generation, S, next, next_count, far_list, far_counter,
+                        relax_improved);
           },
           [&](bool, int) {}, [&] { stage_flush(S, next, next_count, far_list, far_counter); });
       DYNG_GRID_SYNC();
@@ -873,13 +955,21 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           const u64 word = ld(&p.packed[u]);
           const u64 du = p.pk.distance(word);
           if (lane() == 0) atomicAdd(relax_edges, static_cast<unsigned long long>(ch.z - ch.y));
-          for (int b = ch.y; b < ch.z; b += 64) {
-            const int e = b + lane();
-            const bool act = word != packed_inf && e < ch.z;
-            const int32_t w = act ? p.out_nbr[e] : 0;
-            const int32_t wt = act ? p.out_w[e] : 0;
-            relax(p, act && w >= 0, u, du, w, wt, threshold, generation, S, next, next_count, far_list, far_counter,
-                  relax_improved);
+          for (int b = ch.y; b < ch.z; b += 64 * kSlots) {
+            bool act[kSlots];
+            int32_t uu[kSlots];
+            u64 dd[kSlots];
+            int slot[kSlots];
+#pragma unroll
+            for (int j = 0; j < kSlots; ++j) {
+              const int e = b + j * 64 + lane();
+              act[j] = word != packed_inf && e < ch.z;
+              uu[j] = u;
+              dd[j] = du;
+              slot[j] = act[j] ? e : 0;
+            }
+            relax_slots(p, act, uu, dd, slot, threshold, generation, S, next, next_count, far_list, far_counter,
+                        relax_improved);
           }
         }
         stage_flush(S, next, next_count, far_list, far_counter);
@@ -1234,13 +1324,17 @@ void attach_scratch(engine_params& p, const engine_scratch& sc) {
   p.bar = sc.bar;
 }
 
-/// Every workgroup must be resident for the grid barrier. The launch stays at half of the
-/// occupancy the runtime reports (one 1024-thread workgroup per CU by default), never at the edge
-/// of it; the barrier's bounded wait turns a grid that is not resident after all into an error.
-int grid_blocks(const dyng_hip_tuning& t, int occupancy, int cu_count) {
-  const int resident = occupancy * cu_count;  // what the runtime says fits at once
-  const int cap = std::max(1, resident / 2);
-  const int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : 1;
+/// Every workgroup must be resident for the grid barrier. A run alone on the device fills it (two
+/// 1024-thread workgroups per CU: the waves of one hide the other's workgroup barriers and gathers,
+/// 1.6x on a from-scratch solve of RMAT-22 against one per CU). Runs launched to overlap on one
+/// device (`concurrent`) take at most half of what fits each, so two of them are resident at
+/// once. The barrier's bounded wait turns a grid that is not resident after all into an error.
+int grid_blocks(const dyng_hip_tuning& t, int occupancy, int cu_count, bool concurrent) {
+  const int resident = std::max(1, occupancy * cu_count);  // what the runtime says fits at once
+  // an explicit count may go up to everything that fits (the caller runs nothing else beside it)
+  if (t.grid_blocks > 0) return std::min(t.grid_blocks, resident);
+  const int cap = concurrent ? std::max(1, resident / 2) : resident;
+  const int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : (concurrent ? 1 : occupancy);
   return std::max(1, std::min(per_cu * cu_count, cap));
 }
 
@@ -1303,7 +1397,7 @@ void read_counters(const control* hc, bool update, int bits, int blocks, dyng_hi
 
 }  // namespace
 
-engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s) {
+engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s, bool concurrent) {
   if (run == nullptr || run->graph == nullptr || run->workspace == nullptr) {
     fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: sssp run: graph and workspace are required");
   }
@@ -1344,7 +1438,7 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
   attach_scratch(p, engine_scratch{ws->packed.p, ws->stamp.p, ws->in_far.p, ws->flag.p, ws->ancestor.p, ws->cand.p,
                                    ws->frontier.p, ws->near_a.p, ws->near_b.p, ws->far_a.p, ws->far_b.p,
                                    ws->chunks.p, ws->chunk_cap, ws->ctl.p, ws->bar.p});
-  const int blocks = grid_blocks(run->tuning, ws->occupancy, ws->cu_count);
+  const int blocks = grid_blocks(run->tuning, ws->occupancy, ws->cu_count, concurrent);
   control* hc = ws->host_ctl.p;
   std::memset(hc, 0, sizeof(control));
   hc->min_frontier = packed_inf;
@@ -1611,7 +1705,7 @@ void run_engine_noalloc(const dyng_hip_sssp_run* run, hipStream_t s, dyng_hip_co
                                    reinterpret_cast<int*>(base + L.far_b), reinterpret_cast<int4*>(base + L.chunks),
                                    L.chunk_cap, reinterpret_cast<control*>(base + L.ctl),
                                    reinterpret_cast<grid_sync_state*>(base + L.bar)});
-  const int blocks = grid_blocks(run->tuning, st.occupancy, st.cu_count);
+  const int blocks = grid_blocks(run->tuning, st.occupancy, st.cu_count, false);
   hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, s, p.bar, p.ctl);
   hipLaunchKernelGGL(sssp_fused_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
   DYNG_HIP_TRY(hipGetLastError());

@@ -61,12 +61,14 @@ class Tuning:
     launch: str = "plain"  # "plain" or "graph" (the launch built once as a hipGraph and replayed)
     recompute: bool = False  # updates recompute the tree from scratch instead of repairing it
     barrier_timeout_us: int = 0
+    grid_blocks: int = 0  # workgroups of the persistent launch (0: blocks_per_cu per CU)
 
     def _native(self) -> _lib.Tuning:
         launches = {"plain": 0, "graph": 1}
         if self.launch not in launches:
             raise InvalidArgumentError(f"Tuning.launch must be 'plain' or 'graph', got {self.launch!r}")
-        return _lib.Tuning(_lib.ENGINES[self.engine], 0, int(self.blocks_per_cu), int(self.light_degree),
+        return _lib.Tuning(_lib.ENGINES[self.engine], int(self.grid_blocks), int(self.blocks_per_cu),
+                           int(self.light_degree),
                            int(self.chunk_edges), launches[self.launch], int(bool(self.recompute)),
                            int(self.barrier_timeout_us))
 

@@ -166,8 +166,11 @@ int dyng_hip_workspace_destroy(dyng_hip_workspace* ws);
 
 typedef struct dyng_hip_tuning {
   int32_t engine;          /* DYNG_HIP_ENGINE_* */
-  int32_t block_size;      /* threads per workgroup of the persistent kernel (256, 512, 1024) */
-  int32_t blocks_per_cu;   /* resident workgroups per CU (<= occupancy) */
+  int32_t grid_blocks;     /* workgroups of the persistent launch, at most what fits at once (0:
+                              blocks_per_cu per CU) */
+  int32_t blocks_per_cu;   /* workgroups per CU (0: all that fit, or one when the library overlaps
+                              two runs on the device); a caller that overlaps its own runs on one
+                              device asks for 1 */
   int32_t light_degree;    /* rows up to this length: edge-balanced inside a workgroup (0: 4096) */
   int32_t chunk_edges;     /* longer rows: split in chunks of this many edges, one wave each (0: 256) */
   int32_t launch;             /* DYNG_HIP_LAUNCH_*: a plain launch, or a hipGraph replayed per run */

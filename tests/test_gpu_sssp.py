@@ -323,9 +323,10 @@ def test_random_streams_match_the_oracle(dg, seed, n, m, wmax, kw):
         _assert_same_rows(hg, og)
 
 
-@pytest.mark.parametrize("light,chunk,bpc", [(1, 16, 1), (4, 64, 2), (64, 1024, 0)])
-def test_schedule_knobs_never_change_the_result(dg, light, chunk, bpc):
-    _say(f"[tuning] light={light} chunk={chunk} bpc={bpc}")
+@pytest.mark.parametrize("light,chunk,bpc,gb", [(1, 16, 1, 0), (4, 64, 2, 0), (64, 1024, 0, 0), (4096, 256, 0, 1),
+                                                (32, 64, 0, 7)])
+def test_schedule_knobs_never_change_the_result(dg, light, chunk, bpc, gb):
+    _say(f"[tuning] light={light} chunk={chunk} bpc={bpc} grid={gb}")
     rng = np.random.default_rng(99)
     n, m = 3000, 40000
     src = np.concatenate([rng.integers(0, n, m), np.zeros(3000, np.int64)])  # a hub row
@@ -335,7 +336,7 @@ def test_schedule_knobs_never_change_the_result(dg, light, chunk, bpc):
     hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
     ot, _ = oracle.compute(og, 0)
     hr = dg.sssp.compute(hg, 0)
-    hr.set_tuning(dg.sssp.Tuning(light_degree=light, chunk_edges=chunk, blocks_per_cu=bpc))
+    hr.set_tuning(dg.sssp.Tuning(light_degree=light, chunk_edges=chunk, blocks_per_cu=bpc, grid_blocks=gb))
     rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
     for _ in range(3):
         pick = rng.integers(0, og.m, 300)
