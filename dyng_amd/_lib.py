@@ -34,7 +34,7 @@ class GraphProps(C.Structure):
 
 SUMMARY_FIELDS = ("inserted_edges", "updated_edges", "deleted_edges", "ignored_deletions",
                   "dropped_self_loops", "inserted_vertices", "num_vertices_after",
-                  "ignored_insertions", "relocated_rows", "compactions")
+                  "ignored_insertions", "relocated_rows", "compactions", "host_plans")
 
 
 APPLY_STAGES = ("scan", "make_ops", "sort", "pairs_out", "unused_4", "unused_5", "pairs_in", "scans",

@@ -290,26 +290,58 @@ struct event_pair {
 
 /// The engine part of an update (sssp_problem::resume + the engine run, sssp.cpp:691-823) for a
 /// result that is one apply behind the graph; a failure poisons the result.
+/// An apply queued ahead of a repair and not finished yet (apply_launch done, apply_finish not).
+struct apply_in_flight {
+  apply_pending* ap;
+  const batch_scan* sc;
+  dyng_hip_apply_summary* summary;
+};
+
 void repair_core(dyng_hip_graph* g, dyng_hip_result* r, int64_t delta, int64_t max_weight, hipStream_t s,
-                 dyng_hip_counters& c, double& engine_ms, int choice = 0) {
+                 dyng_hip_counters& c, double& engine_ms, int choice = 0, const apply_in_flight* inflight = nullptr) {
   event_pair ev;
   try {
     grow_result(r, g->n, s);
     std::lock_guard<std::mutex> lk(pool().lock);
-    dyng_hip_sssp_run run = make_run(g, r, delta, max_weight);
-    // the learned scheduler's schedules (they change the time, never the tree)
-    switch (choice) {
-      case 1: run.delta = 4 * delta; break;
-      case 2: run.delta = std::max<int64_t>(1, delta / 4); break;
-      case 3: run.tuning.light_degree = 256; run.tuning.chunk_edges = 64; break;
-      case 4: run.tuning.recompute = 1; break;
-      default: break;
-    }
+    auto run_for = [&]() {
+      dyng_hip_sssp_run run = make_run(g, r, delta, max_weight);
+      // the learned scheduler's schedules (they change the time, never the tree)
+      switch (choice) {
+        case 1: run.delta = 4 * delta; break;
+        case 2: run.delta = std::max<int64_t>(1, delta / 4); break;
+        case 3: run.tuning.light_degree = 256; run.tuning.chunk_edges = 64; break;
+        case 4: run.tuning.recompute = 1; break;
+        default: break;
+      }
+      return run;
+    };
+    dyng_hip_sssp_run run = run_for();
     DYNG_HIP_TRY(hipEventRecord(ev.a, s));
     r->tree_token = 0;
-    call(dyng_hip_sssp_update(&run, s, &c));
-    r->tree_token = c.tree_token;
+    // queued right behind the apply when it is in flight: one synchronization for both
+    engine_pending pend = engine_launch(&run, true, s, false, inflight != nullptr ? apply_hold(g) : nullptr);
     DYNG_HIP_TRY(hipEventRecord(ev.b, s));
+    if (inflight != nullptr) {
+      try {
+        apply_finish(g, *inflight->sc, *inflight->ap, s, inflight->summary);
+      } catch (...) {
+        try {
+          engine_finish(pend, nullptr);
+        } catch (...) {  // NOLINT(bugprone-empty-catch): the apply's error is the one reported
+        }
+        throw;
+      }
+    }
+    engine_finish(pend, &c);
+    if (pend.held) {
+      // the apply's merges were planned again on the host (its arenas may have moved): the run now
+      run = run_for();
+      DYNG_HIP_TRY(hipEventRecord(ev.a, s));
+      pend = engine_launch(&run, true, s);
+      DYNG_HIP_TRY(hipEventRecord(ev.b, s));
+      engine_finish(pend, &c);
+    }
+    r->tree_token = c.tree_token;
     DYNG_HIP_TRY(hipEventSynchronize(ev.b));
   } catch (...) {
     r->poisoned = true;
@@ -453,12 +485,14 @@ extern "C" int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batc
       fail(DYNG_HIP_INVALID_ARGUMENT, "sssp::update: distances up to " + std::to_string(max_weight) + " * " +
                                           std::to_string(sc.n_after - 1) + " do not fit in 62 bits");
     }
-    // ---- commit: the device apply ----
+    // ---- commit: the device apply, with the engine queued right behind it (one synchronization
+    // for both; apply_finish reports the apply's errors before the engine's) ----
     event_pair ev_apply;
     dyng_hip_apply_summary summary{};
     DYNG_HIP_TRY(hipEventRecord(ev_apply.a, s));
+    apply_pending ap;
     try {
-      apply_scanned(g, sc, s, &summary);
+      ap = apply_launch(g, sc, s, &summary);
     } catch (const batch_refused&) {
       throw;  // refused before anything changed: the result stays current
     } catch (...) {
@@ -469,7 +503,8 @@ extern "C" int dyng_hip_api_update(dyng_hip_graph* g, const dyng_hip_batch* batc
     // ---- resume + the engine; a failure from here on poisons the result ----
     dyng_hip_counters c{};
     double engine_ms = 0.0;
-    repair_core(g, r, delta, max_weight, s, c, engine_ms);
+    const apply_in_flight inflight{&ap, &sc, &summary};
+    repair_core(g, r, delta, max_weight, s, c, engine_ms, 0, &inflight);
     if (stats != nullptr) {
       std::memset(stats, 0, sizeof(*stats));
       fill_stats(stats, c);
@@ -672,12 +707,14 @@ extern "C" int dyng_hip_api_update_scheduled(dyng_hip_graph* g, const dyng_hip_b
     if (mlp != nullptr) {
       predicted = mlp_predict_async(mlp, g, sc, r->parents.p, r->n, last_invalidated, last_affected);
     }
-    // ---- commit: the device apply ----
+    // ---- commit: the device apply, the engine queued right behind it under the predicted
+    // schedule (the prediction is waited for after the apply was queued) ----
     event_pair ev_apply;
     dyng_hip_apply_summary summary{};
     DYNG_HIP_TRY(hipEventRecord(ev_apply.a, s));
+    apply_pending ap;
     try {
-      apply_scanned(g, sc, s, &summary, predicted);
+      ap = apply_launch(g, sc, s, &summary, predicted);
     } catch (const batch_refused&) {
       throw;  // refused before anything changed: the result stays current
     } catch (...) {
@@ -688,9 +725,11 @@ extern "C" int dyng_hip_api_update_scheduled(dyng_hip_graph* g, const dyng_hip_b
     // ---- resume + the engine; a failure from here on poisons the result ----
     dyng_hip_counters c{};
     double engine_ms = 0.0;
+    if (predicted != nullptr) DYNG_HIP_TRY(hipEventSynchronize(predicted));
     const int pick = mlp != nullptr ? mlp_pick(mlp) : 0;
     if (choice != nullptr) *choice = pick;
-    repair_core(g, r, delta, max_weight, s, c, engine_ms, pick);
+    const apply_in_flight inflight{&ap, &sc, &summary};
+    repair_core(g, r, delta, max_weight, s, c, engine_ms, pick, &inflight);
     if (stats != nullptr) {
       std::memset(stats, 0, sizeof(*stats));
       fill_stats(stats, c);

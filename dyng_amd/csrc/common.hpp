@@ -145,10 +145,6 @@ struct op_max {
   DYNG_DEV T operator()(T a, T b) const { return b > a ? b : a; }
 };
 
-/// Claim a vertex for a list in this generation: true for the first claimant only.
-DYNG_DEV bool claim(int* stamp, int32_t v, int generation) {
-  return atomicExch(&stamp[v], generation) != generation;
-}
 
 /// Grid-wide barrier of a persistent kernel whose workgroups are all resident (the grid is sized
 /// to at most half of what the occupancy calculator says fits at once).

@@ -912,6 +912,7 @@ struct merge_params {
   long long* need_stage;
   int* error_bits;
   double headroom;
+  const int* hold;  ///< null, or a flag that makes the merge do nothing (see k_merge_gate)
 };
 
 /// The in-place tail merges of one mode for both stores in one launch: workgroups [0, split)
@@ -929,6 +930,7 @@ DYNG_DEV void for_side(const merge_sides& m, Body&& body) {
   const int b = static_cast<int>(blockIdx.x);
   const bool in = b >= m.split;
   const merge_params p = in ? m.side[1] : m.side[0];
+  if (p.hold != nullptr && *p.hold != 0) return;  // uniform: the gate held the merges back
   body(p, in ? b - m.split : b, in ? static_cast<int>(gridDim.x) - m.split : m.split);
 }
 
@@ -1054,6 +1056,20 @@ DYNG_DEV void merge_plan(const merge_params& p, int block) {
 
 __global__ void __launch_bounds__(kPlanBlock) k_merge_plan(merge_sides m) {
   for_side(m, [](const merge_params& p, int block, int) { merge_plan(p, block); });
+}
+
+/// Slots a relocating merge leaves free at the arena's tail (the host's plan keeps the same).
+constexpr long long kArenaMargin = 1024;
+
+/// After a plan that the host does not read back: whether the merges fit as planned (the
+/// arenas' free tails take the relocations, the staging arena the full merges) and no error came
+/// first. If not, `hold` makes the merges and the engine run behind them do nothing, and the host
+/// plans again after its synchronization. One thread.
+__global__ void k_merge_gate(apply_control* c, long long slot_cap_out, long long slot_cap_in, long long stage_cap) {
+  const bool fits = c->error_bits == 0 && c->need_reloc_out + kArenaMargin <= slot_cap_out - c->tail_out &&
+                    c->need_reloc_in + kArenaMargin <= slot_cap_in - c->tail_in &&
+                    c->need_stage_out <= stage_cap && c->need_stage_in <= stage_cap;
+  c->hold = fits ? 0 : 1;
 }
 
 /// One thread per row whose tail holds at most kLaneTail entries and that has at most kLaneOps
@@ -1393,6 +1409,7 @@ __global__ void __launch_bounds__(kMergeBlock) k_full_merge(merge_params p) {
   __shared__ int s_tot[kMergeWaves];
   __shared__ long long s_stage, s_dest;
   __shared__ int s_newcap;
+  if (p.hold != nullptr && *p.hold != 0) return;  // uniform: the gate held the merges back
   const int count = p.list_count[kFull];
   const int32_t* list = p.lists + kFull * (p.N + 1);
   const u64 mask = (1ULL << p.vbits) - 1;
@@ -1949,6 +1966,172 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
   }
 }
 
+/// The merge parameters of one row store for a batch of N operations (keys of `vbits` bits).
+merge_params merge_setup(dyng_hip_graph* g, bool outs, int64_t N, int vbits, const int* hold) {
+  apply_control* dc = g->ctl.p;
+  row_store& rs = outs ? g->out : g->in;
+  merge_params mp{};
+  mp.meta = rs.meta.p;
+  mp.cap = rs.cap.p;
+  mp.base = rs.base.p;
+  mp.dead = rs.dead.p;
+  mp.nbr = rs.nbr.p;
+  mp.w = rs.w.p;
+  mp.slot_cap = rs.slot_cap;
+  mp.K = g->K;
+  mp.keys = outs ? g->keys_b.p : g->keys_b.p + N;
+  mp.N = N;
+  mp.vbits = vbits;
+  mp.pend_alive = outs ? g->pend_alive.p : g->ipend_alive.p;
+  mp.pend_prefix = outs ? g->pend_prefix.p : g->ipend_prefix.p;
+  mp.kill_prefix = outs ? g->kill_prefix.p : g->ikill_prefix.p;
+  mp.pend_w = outs ? g->pend_w.p : g->ipend_w.p;
+  mp.touched = outs ? g->touched_out.p : g->touched_in.p;
+  mp.touched_count = outs ? &dc->touched_out : &dc->touched_in;
+  mp.row_a = outs ? g->row_a_out.p : g->row_a_in.p;
+  mp.row_b = outs ? g->row_b_out.p : g->row_b_in.p;
+  mp.lists = g->merge_lists.p + (outs ? 0 : kModes * (N + 1));
+  mp.list_count = outs ? dc->modes_out : dc->modes_in;
+  mp.stage_nbr = g->stage_nbr.p;
+  mp.stage_w = g->stage_w.p;
+  mp.stage_cap = g->stage_cap;
+  mp.stage_used = outs ? &dc->stage_out : &dc->stage_in;
+  mp.tail = outs ? &dc->tail_out : &dc->tail_in;
+  mp.garbage = outs ? &dc->garbage_out : &dc->garbage_in;
+  mp.relocated = &dc->relocated_rows;
+  mp.need_reloc = outs ? &dc->need_reloc_out : &dc->need_reloc_in;
+  mp.need_stage = outs ? &dc->need_stage_out : &dc->need_stage_in;
+  mp.error_bits = &dc->error_bits;
+  mp.headroom = g->props.headroom;
+  mp.hold = hold;
+  return mp;
+}
+
+unsigned plan_groups(int64_t N) { return static_cast<unsigned>((N + kPlanBlock - 1) / kPlanBlock); }  // rows <= N
+
+/// Launch the tail merges of one mode for both stores: `groups` workgroups per store.
+void launch_tails(void (*kernel)(merge_sides), merge_sides ms, int out_groups, int in_groups, int block,
+                  hipStream_t s) {
+  if (out_groups + in_groups == 0) return;
+  ms.split = out_groups;
+  hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(out_groups + in_groups)), dim3(block), 0, s, ms);
+}
+
+/// The merge plan and the merges with the plan read back first: the launches sized from it, the
+/// arenas compacted and the staging arena grown as it needs. `replan`: a plan already ran (its
+/// counts are cleared first).
+void merges_exact(dyng_hip_graph* g, int64_t N, int vbits, hipStream_t s, dyng_hip_apply_summary* out, bool replan) {
+  apply_control* hc = g->host_ctl.p;
+  apply_control* dc = g->ctl.p;
+  const int K = g->K;
+  for (int attempt = 0;; ++attempt) {
+    if (attempt > 0 || replan) {
+      // planning again: clear what the last plan counted (the first plan of a batch finds them
+      // zero, as the control block was uploaded zeroed at the start of the apply)
+      hc->need_reloc_out = hc->need_reloc_in = hc->need_stage_out = hc->need_stage_in = 0;
+      for (int i = 0; i < kModes; ++i) hc->modes_out[i] = hc->modes_in[i] = 0;
+      DYNG_HIP_TRY(hipMemcpyAsync(&dc->need_reloc_out, &hc->need_reloc_out, 4 * sizeof(long long),
+                                  hipMemcpyHostToDevice, s));
+      DYNG_HIP_TRY(hipMemcpyAsync(dc->modes_out, hc->modes_out, 2 * sizeof(hc->modes_out), hipMemcpyHostToDevice,
+                                  s));
+    }
+    merge_sides plan{};  // both stores' plans in one launch: out-rows first
+    plan.side[0] = merge_setup(g, true, N, vbits, nullptr);
+    plan.side[1] = merge_setup(g, false, N, vbits, nullptr);
+    plan.split = static_cast<int>(plan_groups(N));
+    hipLaunchKernelGGL(k_merge_plan, dim3(2 * plan_groups(N)), dim3(kPlanBlock), 0, s, plan);
+    DYNG_HIP_TRY(hipGetLastError());
+    DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
+    DYNG_HIP_TRY(hipStreamSynchronize(s));
+    if (out) out->host_plans += 1;
+    if (hc->error_bits != 0) return;  // reported by the caller
+    bool compacted = false;
+    for (int side = 0; side < 2; ++side) {
+      row_store& rs = side == 0 ? g->out : g->in;
+      const int64_t need = (side == 0 ? hc->need_reloc_out : hc->need_reloc_in) + kArenaMargin;
+      if (rs.slot_cap - rs.tail < need) {
+        if (attempt > 2) fail(DYNG_HIP_RUNTIME_ERROR, "dyng: graph apply: the row arena cannot take the batch");
+        compact_store(g, rs, 2 * need, s);
+        if (out) out->compactions += 1;
+        compacted = true;
+      }
+    }
+    if (!compacted) break;
+    // compaction recomputed the capacities: plan again on the fresh arenas
+    hc->tail_out = g->out.tail;
+    hc->tail_in = g->in.tail;
+    hc->garbage_out = hc->garbage_in = 0;
+    DYNG_HIP_TRY(hipMemcpyAsync(&dc->tail_out, &hc->tail_out, 2 * sizeof(long long), hipMemcpyHostToDevice, s));
+    DYNG_HIP_TRY(hipMemcpyAsync(&dc->garbage_out, &hc->garbage_out, 2 * sizeof(long long), hipMemcpyHostToDevice, s));
+  }
+  const int64_t stage_need = std::max(hc->need_stage_out, hc->need_stage_in);
+  if (stage_need > g->stage_cap) {
+    g->stage_cap = stage_need + stage_need / 4 + 4096;
+    g->stage_nbr.reserve(g->stage_cap);
+    g->stage_w.reserve(g->stage_cap * K);
+  }
+  mark_stage(g, 10, s);
+  // the in-place tail merges: one launch per mode for both stores (out-rows in the first
+  // workgroups, in-rows in the rest), grids from the plan
+  merge_sides ms{};
+  ms.side[0] = merge_setup(g, true, N, vbits, nullptr);
+  ms.side[1] = merge_setup(g, false, N, vbits, nullptr);
+  auto groups = [](int rows, int per_group, int cap) { return std::min((rows + per_group - 1) / per_group, cap); };
+  launch_tails(k_tail_lane, ms, groups(hc->modes_out[kTailLane], 256, 8192), groups(hc->modes_in[kTailLane], 256, 8192),
+               256, s);
+  launch_tails(k_tail_wave, ms, groups(hc->modes_out[kTailWave], kMergeWaves, 8192),
+               groups(hc->modes_in[kTailWave], kMergeWaves, 8192), kMergeBlock, s);
+  launch_tails(k_tail_block, ms, groups(hc->modes_out[kTailBlock], 1, 4096), groups(hc->modes_in[kTailBlock], 1, 4096),
+               kMergeBlock, s);
+  DYNG_HIP_TRY(hipGetLastError());
+  mark_stage(g, 11, s);
+  // whole rows through the staging arena, which both stores use: one store at a time
+  for (int side = 0; side < 2; ++side) {
+    const int rows = side == 0 ? hc->modes_out[kFull] : hc->modes_in[kFull];
+    if (rows > 0) {
+      hipLaunchKernelGGL(k_full_merge, dim3(static_cast<unsigned>(std::min(rows, 4096))), dim3(kMergeBlock), 0, s,
+                         ms.side[side]);
+    }
+  }
+  DYNG_HIP_TRY(hipGetLastError());
+  mark_stage(g, 12, s);
+}
+
+/// The merge plan and the merges queued together for a small batch, with no read-back in between:
+/// a gate on the device checks that the arenas take what the plan asks (k_merge_gate) and holds
+/// the merges back otherwise; the grids are sized for the batch (every touched row is one of its
+/// at most N + 1 rows per store) and the merges loop over the rows the plan listed.
+void merges_speculative(dyng_hip_graph* g, int64_t N, int vbits, hipStream_t s) {
+  apply_control* dc = g->ctl.p;
+  const int* hold = &dc->hold;
+  merge_sides plan{};
+  plan.side[0] = merge_setup(g, true, N, vbits, nullptr);
+  plan.side[1] = merge_setup(g, false, N, vbits, nullptr);
+  plan.split = static_cast<int>(plan_groups(N));
+  hipLaunchKernelGGL(k_merge_plan, dim3(2 * plan_groups(N)), dim3(kPlanBlock), 0, s, plan);
+  hipLaunchKernelGGL(k_merge_gate, dim3(1), dim3(1), 0, s, dc, static_cast<long long>(g->out.slot_cap),
+                     static_cast<long long>(g->in.slot_cap), static_cast<long long>(g->stage_cap));
+  DYNG_HIP_TRY(hipGetLastError());
+  mark_stage(g, 10, s);
+  merge_sides ms{};
+  ms.side[0] = merge_setup(g, true, N, vbits, hold);
+  ms.side[1] = merge_setup(g, false, N, vbits, hold);
+  const int rows = static_cast<int>(N + 1);
+  const int lane = std::min((rows + 255) / 256, 1024);
+  const int wave = std::min((rows + kMergeWaves - 1) / kMergeWaves, 512);
+  const int block = std::min(rows, 256);
+  launch_tails(k_tail_lane, ms, lane, lane, 256, s);
+  launch_tails(k_tail_wave, ms, wave, wave, kMergeBlock, s);
+  launch_tails(k_tail_block, ms, block, block, kMergeBlock, s);
+  DYNG_HIP_TRY(hipGetLastError());
+  mark_stage(g, 11, s);
+  for (int side = 0; side < 2; ++side) {
+    hipLaunchKernelGGL(k_full_merge, dim3(static_cast<unsigned>(block)), dim3(kMergeBlock), 0, s, ms.side[side]);
+  }
+  DYNG_HIP_TRY(hipGetLastError());
+  mark_stage(g, 12, s);
+}
+
 /// Throw the invalid-argument error of the offending operation of rank `rank` (k_check_pairs):
 /// the message names the operation's kind, batch index and (u, v), as apply_host.cpp:117-120 and
 /// 222-225 do. Nothing has changed yet.
@@ -1978,8 +2161,8 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
                                : " names an existing edge and batch_semantics::on_existing_insert is error"));
 }
 
-void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out,
-                   hipEvent_t wait) {
+apply_pending apply_launch(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out,
+                           hipEvent_t wait) {
   const int K = g->K;
   // G_t as the batch found it (the repair of a result one apply behind reads it)
   g->last.valid = false;
@@ -2007,6 +2190,11 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
   // deletions below the old vertex count that name vertices of the growth are fine; ids in the
   // batch >= n_after only occur in deletions (handled in Step 0)
   const int vbits = bits_for(std::max<int64_t>(n_after, 1));
+  apply_pending ap;
+  ap.n_before = n_before;
+  ap.N = N;
+  ap.vbits = vbits;
+  ap.wait = wait;
   op_layout L;
   L.first_is_del = g->props.deletions_first ? 1 : 0;
   L.num_first = L.first_is_del ? num_del : num_ins;
@@ -2151,130 +2339,46 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     mark_stage(g, 9, s);
     // ---- plan the merges: each row's mode, and the staging and tail slots they take ----
     g->merge_lists.reserve(2 * kModes * (N + 1));
-    auto merge_setup = [&](bool outs) {
-      row_store& rs = outs ? g->out : g->in;
-      merge_params mp{};
-      mp.meta = rs.meta.p;
-      mp.cap = rs.cap.p;
-      mp.base = rs.base.p;
-      mp.dead = rs.dead.p;
-      mp.nbr = rs.nbr.p;
-      mp.w = rs.w.p;
-      mp.slot_cap = rs.slot_cap;
-      mp.K = K;
-      mp.keys = outs ? g->keys_b.p : g->keys_b.p + N;
-      mp.N = N;
-      mp.vbits = vbits;
-      mp.pend_alive = outs ? g->pend_alive.p : g->ipend_alive.p;
-      mp.pend_prefix = outs ? g->pend_prefix.p : g->ipend_prefix.p;
-      mp.kill_prefix = outs ? g->kill_prefix.p : g->ikill_prefix.p;
-      mp.pend_w = outs ? g->pend_w.p : g->ipend_w.p;
-      mp.touched = outs ? g->touched_out.p : g->touched_in.p;
-      mp.touched_count = outs ? &dc->touched_out : &dc->touched_in;
-      mp.row_a = outs ? g->row_a_out.p : g->row_a_in.p;
-      mp.row_b = outs ? g->row_b_out.p : g->row_b_in.p;
-      mp.lists = g->merge_lists.p + (outs ? 0 : kModes * (N + 1));
-      mp.list_count = outs ? dc->modes_out : dc->modes_in;
-      mp.stage_nbr = g->stage_nbr.p;
-      mp.stage_w = g->stage_w.p;
-      mp.stage_cap = g->stage_cap;
-      mp.stage_used = outs ? &dc->stage_out : &dc->stage_in;
-      mp.tail = outs ? &dc->tail_out : &dc->tail_in;
-      mp.garbage = outs ? &dc->garbage_out : &dc->garbage_in;
-      mp.relocated = &dc->relocated_rows;
-      mp.need_reloc = outs ? &dc->need_reloc_out : &dc->need_reloc_in;
-      mp.need_stage = outs ? &dc->need_stage_out : &dc->need_stage_in;
-      mp.error_bits = &dc->error_bits;
-      mp.headroom = g->props.headroom;
-      return mp;
-    };
-    const unsigned plan_grid = static_cast<unsigned>((N + kPlanBlock - 1) / kPlanBlock);  // touched rows <= N
     if (wait != nullptr) DYNG_HIP_TRY(hipStreamWaitEvent(s, wait, 0));  // before the rows change
-    for (int attempt = 0;; ++attempt) {
-      if (attempt > 0) {
-        // planning again: clear what the last plan counted (the first plan finds them zero, as
-        // the control block was uploaded zeroed at the start of the apply)
-        hc->need_reloc_out = hc->need_reloc_in = hc->need_stage_out = hc->need_stage_in = 0;
-        for (int i = 0; i < kModes; ++i) hc->modes_out[i] = hc->modes_in[i] = 0;
-        DYNG_HIP_TRY(hipMemcpyAsync(&dc->need_reloc_out, &hc->need_reloc_out, 4 * sizeof(long long),
-                                    hipMemcpyHostToDevice, s));
-        DYNG_HIP_TRY(hipMemcpyAsync(dc->modes_out, hc->modes_out, 2 * sizeof(hc->modes_out), hipMemcpyHostToDevice,
-                                    s));
-      }
-      merge_sides plan{};  // both stores' plans in one launch: out-rows first
-      plan.side[0] = merge_setup(true);
-      plan.side[1] = merge_setup(false);
-      plan.split = static_cast<int>(plan_grid);
-      hipLaunchKernelGGL(k_merge_plan, dim3(2 * plan_grid), dim3(kPlanBlock), 0, s, plan);
-      DYNG_HIP_TRY(hipGetLastError());
-      DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
-      DYNG_HIP_TRY(hipStreamSynchronize(s));
-      if (hc->error_bits != 0) break;  // reported below
-      bool compacted = false;
-      for (int side = 0; side < 2; ++side) {
-        row_store& rs = side == 0 ? g->out : g->in;
-        const int64_t need = (side == 0 ? hc->need_reloc_out : hc->need_reloc_in) + 1024;
-        if (rs.slot_cap - rs.tail < need) {
-          if (attempt > 2) fail(DYNG_HIP_RUNTIME_ERROR, "dyng: graph apply: the row arena cannot take the batch");
-          compact_store(g, rs, 2 * need, s);
-          if (out) out->compactions += 1;
-          compacted = true;
-        }
-      }
-      if (!compacted) break;
-      // compaction recomputed the capacities: plan again on the fresh arenas
-      hc->tail_out = g->out.tail;
-      hc->tail_in = g->in.tail;
-      hc->garbage_out = hc->garbage_in = 0;
-      DYNG_HIP_TRY(hipMemcpyAsync(&dc->tail_out, &hc->tail_out, 2 * sizeof(long long), hipMemcpyHostToDevice, s));
-      DYNG_HIP_TRY(hipMemcpyAsync(&dc->garbage_out, &hc->garbage_out, 2 * sizeof(long long), hipMemcpyHostToDevice, s));
-    }
-    const int64_t stage_need = std::max(hc->need_stage_out, hc->need_stage_in);
-    if (stage_need > g->stage_cap) {
-      g->stage_cap = stage_need + stage_need / 4 + 4096;
-      g->stage_nbr.reserve(g->stage_cap);
-      g->stage_w.reserve(g->stage_cap * K);
-    }
-    mark_stage(g, 10, s);
-    // ---- the merges, with grids sized from the plan ----
-    if (hc->error_bits == 0) {
-      // the in-place tail merges: one launch per mode for both stores (out-rows in the first
-      // workgroups, in-rows in the rest)
-      merge_sides ms{};
-      ms.side[0] = merge_setup(true);
-      ms.side[1] = merge_setup(false);
-      auto launch_tails = [&](void (*kernel)(merge_sides), int mode, int block, int rows_per_group, int cap) {
-        const int out_groups = std::min((hc->modes_out[mode] + rows_per_group - 1) / rows_per_group, cap);
-        const int in_groups = std::min((hc->modes_in[mode] + rows_per_group - 1) / rows_per_group, cap);
-        if (out_groups + in_groups == 0) return;
-        ms.split = out_groups;
-        hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(out_groups + in_groups)), dim3(block), 0, s, ms);
-      };
-      launch_tails(k_tail_lane, kTailLane, 256, 256, 8192);
-      launch_tails(k_tail_wave, kTailWave, kMergeBlock, kMergeWaves, 8192);
-      launch_tails(k_tail_block, kTailBlock, kMergeBlock, 1, 4096);
-      DYNG_HIP_TRY(hipGetLastError());
-      mark_stage(g, 11, s);
-      // whole rows through the staging arena, which both stores use: one store at a time
-      for (int side = 0; side < 2; ++side) {
-        const int rows = side == 0 ? hc->modes_out[kFull] : hc->modes_in[kFull];
-        if (rows > 0) {
-          hipLaunchKernelGGL(k_full_merge, dim3(static_cast<unsigned>(std::min(rows, 4096))), dim3(kMergeBlock), 0,
-                             s, ms.side[side]);
-        }
-      }
-      DYNG_HIP_TRY(hipGetLastError());
-      mark_stage(g, 12, s);
+    ap.speculative = N <= kSpeculativeOps;
+    if (ap.speculative) {
+      merges_speculative(g, N, vbits, s);
+    } else {
+      merges_exact(g, N, vbits, s, out, false);
     }
   }
   if (wait != nullptr) DYNG_HIP_TRY(hipStreamWaitEvent(s, wait, 0));
   DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
+  // from here on the graph has the batch's vertices (work queued behind reads them)
+  g->n = n_after;
+  return ap;
+}
+
+void apply_finish(dyng_hip_graph* g, const batch_scan& sc, apply_pending& ap, hipStream_t s,
+                  dyng_hip_apply_summary* out) {
+  const int K = g->K;
+  apply_control* hc = g->host_ctl.p;
+  apply_control* dc = g->ctl.p;
+  const int64_t n_before = ap.n_before;
+  const int64_t n_after = g->n;
   DYNG_HIP_TRY(hipStreamSynchronize(s));
+  if (ap.speculative && hc->hold != 0 && hc->error_bits == 0) {
+    // the arenas could not take the merges as the device planned them: plan again with the
+    // host's compaction and staging growth (nothing was merged; the rows are as the pairing left
+    // them)
+    hc->hold = 0;
+    DYNG_HIP_TRY(hipMemcpyAsync(&dc->hold, &hc->hold, sizeof(int), hipMemcpyHostToDevice, s));
+    merges_exact(g, ap.N, ap.vbits, s, out, true);
+    if (ap.wait != nullptr) DYNG_HIP_TRY(hipStreamWaitEvent(s, ap.wait, 0));
+    DYNG_HIP_TRY(hipMemcpyAsync(hc, dc, sizeof(apply_control), hipMemcpyDeviceToHost, s));
+    DYNG_HIP_TRY(hipStreamSynchronize(s));
+    ap.replanned = true;
+  }
   if (hc->error_bits != 0) {
+    g->n = n_before;
     throw status_error(DYNG_HIP_DEVICE_ERROR, "dyng: the device apply reported an internal error",
                        static_cast<uint32_t>(hc->error_bits));
   }
-  g->n = n_after;
   g->m += static_cast<int64_t>(hc->inserted_edges) - static_cast<int64_t>(hc->deleted_edges);
   for (int k = 0; k < K; ++k) g->wmin[k] = std::min<int64_t>(g->wmin[k], sc.wmin[k]);
   g->out.tail = hc->tail_out;
@@ -2301,7 +2405,7 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
     out->inserted_vertices = n_after - n_before;
     out->num_vertices_after = n_after;
     out->relocated_rows = static_cast<int64_t>(hc->relocated_rows);
-    stage_times(g, N > 0 && hc->error_bits == 0, out);
+    stage_times(g, ap.N > 0 && hc->error_bits == 0, out);
     g->last.summary = *out;
   }
   g->last.valid = true;
@@ -2309,6 +2413,14 @@ void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_
   // block, so only after the counters above were read)
   if (dirty) recompute_columns(g, s);
 }
+
+void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out,
+                   hipEvent_t wait) {
+  apply_pending ap = apply_launch(g, sc, s, out, wait);
+  apply_finish(g, sc, ap, s, out);
+}
+
+const int* apply_hold(const dyng_hip_graph* g) { return &g->ctl.p->hold; }
 
 }  // namespace dyng_hip
 

@@ -178,6 +178,11 @@ struct apply_control {
   // error modes (on_missing_delete / on_existing_insert = error): the first offending operation
   // in the reference's order, as a rank (all ones: none); see k_check_pairs
   unsigned long long first_error;
+  /// nonzero: the merges were queued without the host reading the plan, and the arenas could
+  /// not take them as planned (or an error came first): the merges and any engine run queued
+  /// behind them did nothing, and the host plans again (k_merge_gate)
+  int hold;
+  int pad1;
 };
 
 /// What the batch scan found (before anything changes) and where the batch is on the device.
@@ -198,8 +203,8 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
 /// One engine run in flight: engine_launch queues it on `s` (the checks, the launch and the
 /// counters' read-back, nothing waited for), engine_finish waits for it and reports (or throws).
 /// dyng_hip_sssp_update / _compute are the two back to back; runs on different workspaces and
-/// streams may be in flight together (each persistent launch takes one workgroup per CU, half of
-/// what fits, so two are resident at once).
+/// streams may be in flight together when launched as `concurrent` (each persistent launch then
+/// takes half of what fits, so two are resident at once).
 struct engine_pending {
   dyng_hip_workspace* ws = nullptr;
   hipStream_t s = nullptr;
@@ -211,11 +216,15 @@ struct engine_pending {
   int64_t n = 0;
   int64_t timeout_us = 0;
   int64_t* distances = nullptr;
+  bool held = false;  ///< set by engine_finish: the run did nothing (see engine_launch's `hold`)
   dyng_hip_counters counters{};
 };
 /// `concurrent`: the run is launched to overlap another persistent launch on the device (each
-/// then takes at most half of what fits).
-engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s, bool concurrent = false);
+/// then takes at most half of what fits). `hold`: queued behind apply_launch, the apply's
+/// apply_hold flag: when the apply's merges were held back the run does nothing, and
+/// engine_finish sets `held` (the caller launches it again after apply_finish).
+engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s, bool concurrent = false,
+                             const int* hold = nullptr);
 void engine_finish(engine_pending& pend, dyng_hip_counters* out);
 /// The scheduler's pick for a scanned batch on G_t: features, BF16 inference, argmin (one sync).
 int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents, int64_t n_tree,
@@ -254,6 +263,27 @@ void mosp_path_costs(const dyng_hip_graph* g, const int32_t* parents, int64_t n,
 /// before the merges change the rows (and before the apply's read-backs).
 void apply_scanned(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out,
                    hipEvent_t wait = nullptr);
+/// The apply in two halves: apply_launch queues the device work (the graph's vertex count is
+/// n_after from then on; for a batch of up to kSpeculativeOps operations the merges are queued
+/// without reading the merge plan back, guarded by a device flag, apply_hold) and the control
+/// block's read-back; work queued on the stream after it sees the applied rows. apply_finish
+/// waits for the stream, plans and merges again from the host if the guard held the merges
+/// back (then `replanned` is set: engine runs queued in between did nothing), reports errors
+/// and updates the graph's counts. Between the two only engine runs may be queued.
+struct apply_pending {
+  bool speculative = false;
+  bool replanned = false;
+  int64_t n_before = 0, N = 0;
+  int vbits = 0;
+  hipEvent_t wait = nullptr;
+};
+constexpr int64_t kSpeculativeOps = int64_t{1} << 17;
+apply_pending apply_launch(dyng_hip_graph* g, const batch_scan& sc, hipStream_t s, dyng_hip_apply_summary* out,
+                           hipEvent_t wait = nullptr);
+void apply_finish(dyng_hip_graph* g, const batch_scan& sc, apply_pending& ap, hipStream_t s,
+                  dyng_hip_apply_summary* out);
+/// The device flag an engine run queued between apply_launch and apply_finish checks first.
+const int* apply_hold(const dyng_hip_graph* g);
 }  // namespace dyng_hip
 
 /// The device graph behind the dyng_hip_graph handle.

@@ -54,6 +54,8 @@ struct control {
   int cand_count;
   int heads_count;
   int frontier_count;
+  int touched_count;  ///< entries of the unpack's list of vertices first stamped by this run's claims
+  int held;           ///< 1: the run did nothing (the apply ahead of it was held back, see engine_params::hold)
   int roots;
   int overflow;
   int invalidated;
@@ -119,6 +121,7 @@ struct engine_params {
   int* near_b;
   int* far_a;
   int* far_b;
+  int* touched;  ///< a repair's vertices whose words its claims may have changed (see claim_first)
   int4* chunks;
   int64_t chunk_cap;
   control* ctl;
@@ -126,6 +129,7 @@ struct engine_params {
   int32_t light;
   int32_t chunk_edges;
   unsigned long long barrier_ticks;  ///< longest wait at a grid barrier (s_memrealtime ticks)
+  const int* hold;  ///< null, or a flag of the apply queued ahead: nonzero, the run does nothing
 };
 
 /// Append v to a list when pred (wave-aggregated reservation).
@@ -143,9 +147,11 @@ constexpr int kStage = 2048;  ///< entries a workgroup stages per list between f
 /// the workgroup's staged entries with one atomicAdd per list. Entries past the stage's capacity
 /// go to the list directly.
 struct stage_shared {
-  int count[2];
-  int base[2];
-  int item[2][kStage];
+  int count[3];
+  int base[3];
+  int item[3][kStage];
+  int* list2;     ///< list 2 is always the unpack's list (engine_params::touched)
+  int* counter2;  ///< and its counter
 };
 
 /// Stage v for list k (`list`, `counter`) when pred. Every active lane of the wave calls it.
@@ -161,26 +167,35 @@ DYNG_DEV void stage_append(stage_shared& S, int k, bool pred, int32_t v, int* li
   append(list, counter, pred && slot >= kStage, v);
 }
 
-/// Publish the staged entries of list 0 (and of list 1 when list1 is not null). Every thread of
-/// the workgroup calls it; the stage is empty afterwards.
+/// Publish the staged entries of list 0, of list 1 when list1 is not null, and of list 2 (the
+/// unpack's list). Every thread of the workgroup calls it; the stage is empty afterwards.
 DYNG_DEV void stage_flush(stage_shared& S, int* list0, int* counter0, int* list1 = nullptr, int* counter1 = nullptr) {
-  const int lists = list1 ? 2 : 1;
   __syncthreads();  // every staged entry is in LDS
-  if (static_cast<int>(threadIdx.x) < lists) {
+  if (threadIdx.x < 3) {
     const int k = threadIdx.x;
-    const int cnt = min(S.count[k], kStage);
+    int* counter = k == 0 ? counter0 : k == 1 ? counter1 : S.counter2;
+    const int cnt = counter != nullptr ? min(S.count[k], kStage) : 0;
     S.count[k] = cnt;
-    S.base[k] = cnt > 0 ? atomicAdd(k == 0 ? counter0 : counter1, cnt) : 0;
+    S.base[k] = cnt > 0 ? atomicAdd(counter, cnt) : 0;
   }
   __syncthreads();
-  for (int k = 0; k < lists; ++k) {
-    int* list = k == 0 ? list0 : list1;
+  for (int k = 0; k < 3; ++k) {
+    int* list = k == 0 ? list0 : k == 1 ? list1 : S.list2;
     const int cnt = S.count[k], b = S.base[k];
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) st(&list[b + i], S.item[k][i]);
   }
   __syncthreads();  // every thread has read count and base
-  if (static_cast<int>(threadIdx.x) < lists) S.count[threadIdx.x] = 0;
+  if (threadIdx.x < 3) S.count[threadIdx.x] = 0;
   __syncthreads();  // the reset lands before the next stage_append
+}
+
+/// Claim v for the list of generation `generation` (the stamp exchange): true when this call
+/// claimed it. `first`: v carried no stamp of this run before (stamps of this run are >= its first
+/// generation p.generation), so the unpack's list must take it.
+DYNG_DEV bool claim_first(const engine_params& p, int32_t v, int generation, bool& first) {
+  const int old = atomicExch(&p.stamp[v], generation);
+  first = old < p.generation;
+  return old != generation;
 }
 
 /// A chunk {vertex, begin, end} of the chunk list, written and read with agent-scope operations.
@@ -324,7 +339,7 @@ DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edges edges, Done 
 /// of this run are >= its first generation p.generation): an invalidated vertex or one whose
 /// distance fell was claimed for a list; this one gets p.generation itself, which is below every
 /// generation a claim compares with, so it claims nothing. The unpack visits the stamped ones.
-DYNG_DEV void mark_changed(const engine_params& p, int32_t w) { atomicMax(&p.stamp[w], p.generation); }
+DYNG_DEV bool mark_changed(const engine_params& p, int32_t w) { return atomicMax(&p.stamp[w], p.generation) < p.generation; }
 
 /// One relaxation of edge (u, w) of weight wt with u at distance du (fused.cuh:341-362), given
 /// w's word as read before (`seen`; it only falls during the relax, so a stale read is never
@@ -332,9 +347,10 @@ DYNG_DEV void mark_changed(const engine_params& p, int32_t w) { atomicMax(&p.sta
 /// far pile when it lowered w's word. Returns the lists to append w to.
 DYNG_DEV void relax_decide(const engine_params& p, bool active, int32_t u, u64 du, int32_t w, int32_t wt, u64 seen,
                            u64 threshold, int generation, unsigned long long* improved, bool& to_near,
-                           bool& to_far) {
+                           bool& to_far, bool& first) {
   to_near = false;
   to_far = false;
+  first = false;
   if (active) {
     const u64 nd = du + static_cast<u64>(wt);
     if (nd <= p.max_distance) {
@@ -347,7 +363,7 @@ DYNG_DEV void relax_decide(const engine_params& p, bool active, int32_t u, u64 d
         if (cand < old) {
           atomicAdd(improved, 1ULL);
           if (nd < threshold) {
-            to_near = claim(p.stamp, w, generation);
+            to_near = claim_first(p, w, generation, first);
           } else {
             to_far = atomicExch(&p.in_far[w], 1) == 0;  // stamped when it leaves the far pile
           }
@@ -378,10 +394,12 @@ DYNG_DEV void relax_slots(const engine_params& p, const bool (&act)[kSlots], con
   }
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
-    bool to_near, to_far;
-    relax_decide(p, live[j], u[j], du[j], w[j], wt[j], seen[j], threshold, generation, improved, to_near, to_far);
+    bool to_near, to_far, first;
+    relax_decide(p, live[j], u[j], du[j], w[j], wt[j], seen[j], threshold, generation, improved, to_near, to_far,
+                 first);
     stage_append(S, 0, to_near, w[j], next, next_count);
     stage_append(S, 1, to_far, w[j], far, far_count);
+    stage_append(S, 2, first, w[j], S.list2, S.counter2);
   }
 }
 
@@ -462,7 +480,7 @@ DYNG_DEV u64 final_offer(const engine_params& p, int32_t u, int64_t i) {
 /// smallest lowered distance goes to *minimum (the first near-far threshold).
 DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best, int generation, stage_shared& S,
                           int* counter, u64* minimum) {
-  bool add = false;
+  bool add = false, first = false;
   u64 low = packed_inf;
   if (pred) {
     const u64 current = ld(&p.packed[v]);
@@ -470,13 +488,14 @@ DYNG_DEV void pull_commit(const engine_params& p, bool pred, int32_t v, u64 best
       const u64 old = atomicMin(&p.packed[v], best);
       if (p.pk.distance(best) < p.pk.distance(old)) {
         low = p.pk.distance(best);
-        add = claim(p.stamp, v, generation);
+        add = claim_first(p, v, generation, first);
       } else if (best < old) {
-        mark_changed(p, v);
+        first = mark_changed(p, v);
       }
     }
   }
   stage_append(S, 0, add, v, p.frontier, counter);
+  stage_append(S, 2, first, v, S.list2, S.counter2);
   low = wave_min(low);
   if (lane() == 0 && low != packed_inf) atomicMin(minimum, low);
 }
@@ -514,12 +533,21 @@ __global__ void k_reset_barrier(grid_sync_state* bar, control* ctl) {
 }
 
 __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) {
+  // queued right behind an apply whose merges were held back: nothing to do (uniform)
+  if (p.hold != nullptr && *p.hold != 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.ctl->held = 1;
+    return;
+  }
   __shared__ tile_shared L;
   // work counters of the workgroup, in LDS: relax edges, improving atomics, pull edges, affected
   __shared__ unsigned long long counts[4];
   __shared__ stage_shared S;
   if (threadIdx.x < 4) counts[threadIdx.x] = 0;
-  if (threadIdx.x < 2) S.count[threadIdx.x] = 0;
+  if (threadIdx.x < 3) S.count[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    S.list2 = p.touched;
+    S.counter2 = &p.ctl->touched_count;
+  }
   __syncthreads();
   unsigned long long* const relax_edges = &counts[0];
   unsigned long long* const relax_improved = &counts[1];
@@ -746,12 +774,13 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     for (int64_t base = tid - lane(); p.check_cycles && base < p.num_ins; base += threads) {
       const int64_t i = base + lane();
       int32_t v = -1;
-      bool add = false;
+      bool add = false, first = false;
       if (i < p.num_ins) {
         v = p.ins_dst[i];
-        add = v >= 0 && v != p.source && claim(p.stamp, v, generation);
+        add = v >= 0 && v != p.source && claim_first(p, v, generation, first);
       }
       append(p.cand + inv, &c->heads_count, add, v);
+      append(p.touched, &c->touched_count, first, v);
     }
     DYNG_GRID_SYNC();
     // ---- D: pull ----
@@ -839,14 +868,16 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           const u64 current = ld(&p.packed[ch.x]);
           if (best < current) {
             const u64 old = atomicMin(&p.packed[ch.x], best);
-            if (best < old && p.pk.distance(best) == p.pk.distance(old)) mark_changed(p, ch.x);
+            bool first = false;
+            if (best < old && p.pk.distance(best) == p.pk.distance(old)) first = mark_changed(p, ch.x);
             if (p.pk.distance(best) < p.pk.distance(old)) {
               atomicMin(&c->min_frontier, p.pk.distance(best));
-              if (claim(p.stamp, ch.x, generation)) {
+              if (claim_first(p, ch.x, generation, first)) {
                 const int slot = atomicAdd(&c->frontier_count, 1);
                 st(&p.frontier[slot], ch.x);
               }
             }
+            if (first) st(&p.touched[atomicAdd(&c->touched_count, 1)], ch.x);
           }
         }
       }
@@ -872,19 +903,20 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     for (int base = tid - lane(); base < frontier_count; base += threads) {
       const int i = base + lane();
       int32_t v = -1;
-      bool to_near = false, to_far = false;
+      bool to_near = false, to_far = false, first = false;
       if (i < frontier_count) {
         v = ld(&p.frontier[i]);
         const u64 word = ld(&p.packed[v]);
         const u64 d = word == packed_inf ? packed_inf : p.pk.distance(word);
         if (d < threshold) {
-          to_near = claim(p.stamp, v, generation);
+          to_near = claim_first(p, v, generation, first);
         } else {
           to_far = atomicExch(&p.in_far[v], 1) == 0;
         }
       }
       stage_append(S, 0, to_near, v, current, near_count);
       stage_append(S, 1, to_far, v, far[fb], &c->far_count[fb]);
+      stage_append(S, 2, first, v, S.list2, S.counter2);
     }
     stage_flush(S, current, near_count, far[fb], &c->far_count[fb]);
     DYNG_GRID_SYNC();
@@ -1004,20 +1036,21 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     for (int base = tid - lane(); base < far_count; base += threads) {
       const int i = base + lane();
       int32_t v = -1;
-      bool to_near = false, keep = false;
+      bool to_near = false, keep = false, first = false;
       if (i < far_count) {
         v = ld(&far[fb][i]);
         const u64 word = ld(&p.packed[v]);
         const u64 d = word == packed_inf ? packed_inf : p.pk.distance(word);
         if (d < threshold) {
           st(&p.in_far[v], 0);
-          to_near = claim(p.stamp, v, generation);
+          to_near = claim_first(p, v, generation, first);
         } else {
           keep = true;
         }
       }
       stage_append(S, 0, to_near, v, current, split_count);
       stage_append(S, 1, keep, v, far[fb ^ 1], &c->far_count[fb ^ 1]);
+      stage_append(S, 2, first, v, S.list2, S.counter2);
     }
     stage_flush(S, current, split_count, far[fb ^ 1], &c->far_count[fb ^ 1]);
     DYNG_GRID_SYNC();
@@ -1032,8 +1065,9 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   unsigned long long changed = 0;
   if (p.pk.bits > 0) {
     // A repair changed only words of vertices it stamped (see mark_changed); every other word
-    // still equals its vertex's packed (distance, parent). The stamps cost 4 B per vertex to
-    // scan (four per load) instead of 20 B to unpack. A run from the sources visits every vertex.
+    // still equals its vertex's packed (distance, parent), so only the stamped ones are visited,
+    // from the lists that stamped them (no pass over every vertex). A run from the sources visits
+    // every vertex.
     auto unpack = [&](int32_t v) {
       const u64 word = ld(&p.packed[v]);
       const long long d = word == packed_inf ? sssp_infinity : static_cast<long long>(p.pk.distance(word));
@@ -1050,18 +1084,12 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     if (p.from_scratch) {
       for (int v = tid; v < n; v += threads) unpack(v);
     } else {
-      const int quads = n / 4;  // the stamp array is 16-byte aligned
-      const int4* stamp4 = reinterpret_cast<const int4*>(p.stamp);
-      for (int q = tid; q < quads; q += threads) {
-        const int4 s4 = ld_int4(&stamp4[q]);
-        if (s4.x >= p.generation) unpack(4 * q);
-        if (s4.y >= p.generation) unpack(4 * q + 1);
-        if (s4.z >= p.generation) unpack(4 * q + 2);
-        if (s4.w >= p.generation) unpack(4 * q + 3);
-      }
-      for (int v = 4 * quads + tid; v < n; v += threads) {
-        if (ld(&p.stamp[v]) >= p.generation) unpack(v);
-      }
+      // the invalidated vertices (cand[0, invalidated)) and the vertices whose first stamp of
+      // this run came from a claim or a same-distance lowering (the touched list): disjoint, as
+      // the invalidated ones were stamped before any claim
+      const int inv = uniform(ld(&c->invalidated));
+      const int total = inv + uniform(ld(&c->touched_count));
+      for (int i = tid; i < total; i += threads) unpack(i < inv ? ld(&p.cand[i]) : ld(&p.touched[i - inv]));
     }
   } else {
     // distance-only words: lowest-id tight parents (fused.cuh:452-485)
@@ -1127,7 +1155,7 @@ struct dyng_hip_workspace {
   int generation = 0;
   dyng_hip::dbuf<dyng_hip::u64> packed;
   dyng_hip::dbuf<int> stamp, in_far, flag, ancestor;
-  dyng_hip::dbuf<int> cand, frontier, near_a, near_b, far_a, far_b;
+  dyng_hip::dbuf<int> cand, frontier, near_a, near_b, far_a, far_b, touched;
   /// the tree whose packed words `packed` holds after the last successful run: its token (0:
   /// none), size, parent bits and distance array
   uint64_t words_token = 0;
@@ -1179,6 +1207,7 @@ void ws_reserve(dyng_hip_workspace* ws, int64_t n, int64_t slots, hipStream_t s)
     ws->near_b.reserve(cap);
     ws->far_a.reserve(cap);
     ws->far_b.reserve(cap);
+    ws->touched.reserve(cap);
     DYNG_HIP_TRY(hipMemsetAsync(ws->stamp.p, 0, cap * sizeof(int), s));
     DYNG_HIP_TRY(hipMemsetAsync(ws->in_far.p, 0, cap * sizeof(int), s));
     DYNG_HIP_TRY(hipMemsetAsync(ws->flag.p, 0, cap * sizeof(int), s));
@@ -1215,7 +1244,7 @@ namespace {
 struct engine_scratch {
   u64* packed;
   int *stamp, *in_far, *flag, *ancestor;
-  int *cand, *frontier, *near_a, *near_b, *far_a, *far_b;
+  int *cand, *frontier, *near_a, *near_b, *far_a, *far_b, *touched;
   int4* chunks;
   int64_t chunk_cap;
   control* ctl;
@@ -1318,6 +1347,7 @@ void attach_scratch(engine_params& p, const engine_scratch& sc) {
   p.near_b = sc.near_b;
   p.far_a = sc.far_a;
   p.far_b = sc.far_b;
+  p.touched = sc.touched;
   p.chunks = sc.chunks;
   p.chunk_cap = sc.chunk_cap;
   p.ctl = sc.ctl;
@@ -1397,7 +1427,8 @@ void read_counters(const control* hc, bool update, int bits, int blocks, dyng_hi
 
 }  // namespace
 
-engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s, bool concurrent) {
+engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s, bool concurrent,
+                             const int* hold) {
   if (run == nullptr || run->graph == nullptr || run->workspace == nullptr) {
     fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: sssp run: graph and workspace are required");
   }
@@ -1420,6 +1451,7 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
   engine_params p{};
   int64_t timeout_us = 0;
   engine_setup(run, update, p, timeout_us);
+  p.hold = hold;
   ws_reserve(ws, n, std::max(g->out.slot_cap, g->in.slot_cap), s);
   if (ws->generation > INT32_MAX / 2) {
     DYNG_HIP_TRY(hipMemsetAsync(ws->stamp.p, 0, ws->capacity * sizeof(int), s));
@@ -1437,7 +1469,7 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
   ws->words_token = 0;  // until this run ends well
   attach_scratch(p, engine_scratch{ws->packed.p, ws->stamp.p, ws->in_far.p, ws->flag.p, ws->ancestor.p, ws->cand.p,
                                    ws->frontier.p, ws->near_a.p, ws->near_b.p, ws->far_a.p, ws->far_b.p,
-                                   ws->chunks.p, ws->chunk_cap, ws->ctl.p, ws->bar.p});
+                                   ws->touched.p, ws->chunks.p, ws->chunk_cap, ws->ctl.p, ws->bar.p});
   const int blocks = grid_blocks(run->tuning, ws->occupancy, ws->cu_count, concurrent);
   control* hc = ws->host_ctl.p;
   std::memset(hc, 0, sizeof(control));
@@ -1509,6 +1541,12 @@ void engine_finish(engine_pending& pend, dyng_hip_counters* out) {
   const int64_t timeout_us = pend.timeout_us;
   control* hc = ws->host_ctl.p;
   DYNG_HIP_TRY(hipStreamSynchronize(s));
+  if (hc->held != 0) {
+    // nothing ran (the apply ahead of the run was held back): the caller runs it again
+    pend.held = true;
+    if (out) *out = counters;
+    return;
+  }
   float kernel_ms = 0.0f;
   DYNG_HIP_TRY(hipEventElapsedTime(&kernel_ms, ws->ev_start, ws->ev_stop));
   if (hc->overflow != 0 || hc->error_bits != 0) {
@@ -1579,7 +1617,8 @@ void clear_scratch(void* p, int64_t bytes, hipStream_t s) {
 /// Where the arrays of a run over n vertices (row arenas of `slots` slots) sit in the scratch:
 /// byte offsets, each region rounded up to 256 bytes.
 struct scratch_layout {
-  int64_t packed, stamp, in_far, flag, ancestor, cand, frontier, near_a, near_b, far_a, far_b, chunks, ctl, bar;
+  int64_t packed, stamp, in_far, flag, ancestor, cand, frontier, near_a, near_b, far_a, far_b, touched, chunks, ctl,
+      bar;
   int64_t chunk_cap;
   int64_t total;
 };
@@ -1603,6 +1642,7 @@ scratch_layout layout_for(int64_t n, int64_t slots) {
   L.near_b = take(4 * n);
   L.far_a = take(4 * n);
   L.far_b = take(4 * n);
+  L.touched = take(4 * n);
   L.chunk_cap = chunks_for(n, slots);
   L.chunks = take(static_cast<int64_t>(sizeof(int4)) * L.chunk_cap);
   L.ctl = take(sizeof(control));
@@ -1702,7 +1742,8 @@ void run_engine_noalloc(const dyng_hip_sssp_run* run, hipStream_t s, dyng_hip_co
                                    reinterpret_cast<int*>(base + L.ancestor), reinterpret_cast<int*>(base + L.cand),
                                    reinterpret_cast<int*>(base + L.frontier), reinterpret_cast<int*>(base + L.near_a),
                                    reinterpret_cast<int*>(base + L.near_b), reinterpret_cast<int*>(base + L.far_a),
-                                   reinterpret_cast<int*>(base + L.far_b), reinterpret_cast<int4*>(base + L.chunks),
+                                   reinterpret_cast<int*>(base + L.far_b), reinterpret_cast<int*>(base + L.touched),
+                                   reinterpret_cast<int4*>(base + L.chunks),
                                    L.chunk_cap, reinterpret_cast<control*>(base + L.ctl),
                                    reinterpret_cast<grid_sync_state*>(base + L.bar)});
   const int blocks = grid_blocks(run->tuning, st.occupancy, st.cu_count, false);

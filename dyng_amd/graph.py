@@ -75,6 +75,7 @@ class ApplySummary:
     ignored_insertions: int = 0
     relocated_rows: int = 0
     compactions: int = 0
+    host_plans: int = field(default=0, compare=False)  # merge plans read back before the merges
     stage_ms: dict = field(default_factory=dict, compare=False)  # device time per apply stage
 
     @classmethod

@@ -81,6 +81,8 @@ typedef struct dyng_hip_graph dyng_hip_graph;
 typedef struct dyng_hip_apply_summary {
   int64_t inserted_edges, updated_edges, deleted_edges, ignored_deletions, dropped_self_loops;
   int64_t inserted_vertices, num_vertices_after, ignored_insertions, relocated_rows, compactions;
+  int64_t host_plans; /* merge plans read back by the host before the merges (0: a batch of at most
+                         2^17 operations whose merges the device planned and ran alone) */
   /* device time of the apply's stages (HIP events, ms; all 0 unless the environment sets
    * DYNG_HIP_STAGE_TIMES=1, since each timed event costs the stream a few microseconds): batch
    * scan, operation keys, key sort (both directions' keys in one sort), out-row pairing, (unused),
@@ -246,7 +248,7 @@ int dyng_hip_sssp_compute(const dyng_hip_sssp_run* run, void* stream, dyng_hip_c
  * DYNG_HIP_NOALLOC_SCRATCH_BYTES per device instead of run->workspace, which is ignored, as are
  * tree_token (the old tree is always packed from its arrays) and tuning.launch (a plain launch).
  * It calls no hipMalloc, hipFree or allocating library routine. A graph that needs more scratch
- * (68 B per vertex plus 1 B per row slot, plus 64 KiB) is refused with DYNG_HIP_INVALID_ARGUMENT
+ * (72 B per vertex plus 1 B per row slot, plus 64 KiB) is refused with DYNG_HIP_INVALID_ARGUMENT
  * before anything runs. Every call on a device shares that device's array, which keeps the last
  * call's contents: calls that overlap on one device race. */
 #define DYNG_HIP_NOALLOC_SCRATCH_BYTES (64ull << 20)

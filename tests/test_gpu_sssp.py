@@ -472,6 +472,43 @@ def test_barrier_timeout_fails_cleanly(dg):
     _stream(dg, og, hg, ot, fresh, rng, 2, 200, 1, 40)
 
 
+def test_merges_planned_on_the_device_and_again_on_the_host(dg):
+    """A small batch's merges are planned and run on the device with no read-back of the plan
+    (host_plans 0); a batch whose full merge needs staging the graph does not have yet is held
+    back on the device (the merges and the engine run queued behind them do nothing) and planned
+    again on the host (host_plans >= 1). Trees, counts and rows equal the oracle's either way."""
+    _say("[speculative merges] n=3000")
+    rng = np.random.default_rng(31)
+    n = 3000
+    src = np.concatenate([np.zeros(200, np.int64), rng.integers(1, n, 30000)])
+    dst = np.concatenate([rng.permutation(np.arange(1, n))[:200], rng.integers(0, n, 30000)])
+    w = rng.integers(1, 60, len(src))
+    og = oracle.build_from_edges(n, src, dst, w.reshape(-1, 1))
+    hg = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    ot, _ = oracle.compute(og, 0)
+    hr = dg.sssp.compute(hg, 0)
+    plans = []
+    for b in range(4):
+        rows = np.repeat(np.arange(og.n), np.diff(og.row_ptr))
+        pick = rng.choice(og.m, 20, replace=False)
+        ins_s, ins_d = rng.integers(1, n, 20), rng.integers(0, n, 20)
+        if b % 2 == 1:  # vertex 0's row grows past its tail limit: a full merge through staging
+            ins_s = np.concatenate([ins_s, np.zeros(300, np.int64)])
+            ins_d = np.concatenate([ins_d, rng.integers(1, n, 300)])
+        ins = (ins_s, ins_d, rng.integers(1, 60, len(ins_s)))
+        dels = (rows[pick], og.col[pick].copy())
+        og, a, c = oracle.update(og, ot, ins[0], ins[1], ins[2].reshape(-1, 1), dels[0], dels[1])
+        st = dg.sssp.update(hg, dg.EdgeBatch(insert=ins, delete=dels), hr)
+        d, p = hr.to_numpy()
+        assert np.array_equal(d, ot.dist) and np.array_equal(p, ot.parent), f"batch {b}"
+        assert (st.invalidated, st.affected) == (c.invalidated, c.affected), f"batch {b}"
+        _assert_same_rows(hg, og)
+        plans.append(st.batch.host_plans)
+    _say(f"[speculative merges] host plans per batch {plans}")
+    assert plans[0] == 0, "a small batch of tail merges reads no plan back"
+    assert plans[1] >= 1, "the first full merge (no staging yet) is planned again on the host"
+
+
 def test_long_rows_with_many_operations(dg):
     """Rows longer than a wave's share and batches with more operations on one row than the merge
     stages in LDS (both directions: vertex 0's out-row and vertex 1's in-row)."""
