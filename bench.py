@@ -66,7 +66,8 @@ def log(*a):
 
 def measured_traffic(workload: str):
     """HBM bytes per persistent-kernel launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    passes of this workload (scaled by a copy of known size), or None if none was taken."""
+    passes of this workload (scaled by a copy of known size), or None if none was taken. A
+    workload name ending in "_compute" names the passes over its solves from the source."""
     path = os.path.join(ROOT, "profiles", f"sssp_fused_kernel_traffic_{workload}.md")
     try:
         with open(path) as f:
@@ -80,16 +81,24 @@ def measured_traffic(workload: str):
 
 def fused_kernel_bytes(c, n: int, num_changes: int, num_ins: int) -> int:
     """Algorithmic HBM bytes of one persistent-kernel launch (update), from its work counters:
-    the O(n) passes (packing 24 B/vertex unless the workspace kept the old tree's words, then the
-    unpack's scan: 4 B of stamps per vertex when the words were kept, 20 B per vertex otherwise),
-    20 B per pointer-jumping visit, 20 B per invalidated vertex, 12 B per change-list entry, 8 B
-    per insertion head, 20 B per pull candidate plus 16 B per in-edge it scans, 20 B per expanded
-    vertex + 16 B per scanned out-edge + 16 B per improving atomicMin, 20 B per changed
-    (distance, parent) pair read and written."""
-    per_vertex = 4 if c.words_kept else 44
+    packing the old tree, 24 B/vertex, unless the workspace kept its words; 20 B per
+    pointer-jumping visit, 20 B per invalidated vertex, 12 B per change-list entry, 8 B per
+    insertion head, 20 B per pull candidate plus 16 B per in-edge it scans, 20 B per expanded
+    vertex + 16 B per scanned out-edge + 16 B per improving atomicMin; the unpack visits the
+    vertices it stamped (24 B each: list entry, word, old pair; the invalidated ones and, at
+    least, the changed ones) and writes 12 B per changed (distance, parent) pair."""
+    per_vertex = 0 if c.words_kept else 24
     return (per_vertex * n + 20 * c.jump_items + 20 * c.invalidated + 12 * num_changes + 8 * num_ins
             + 20 * c.pull_vertices + 16 * c.pull_edges
-            + 20 * c.relax_vertices + 16 * c.relax_edges + 16 * c.relax_improved + 20 * c.affected)
+            + 20 * c.relax_vertices + 16 * c.relax_edges + 16 * c.relax_improved
+            + 24 * (c.invalidated + c.affected) + 12 * c.affected)
+
+
+def compute_bytes(c, n: int) -> int:
+    """Algorithmic HBM bytes of a solve from the source: every word set (8 B/vertex), the
+    near-far propagation (20 P + 16 E_s + 16 I), every vertex unpacked (word read, distance and
+    parent written: 20 B/vertex)."""
+    return 8 * n + relax_bytes(c) + 20 * n
 
 
 def relax_bytes(c) -> int:
@@ -268,17 +277,17 @@ def learned_vs_plain(n, src, dst, w, source, train, evaluate, log_prefix="learne
     return out
 
 
-def stream_bench(args) -> int:
-    """The learned-scheduler configuration: RMAT-23 and a stream of uneven batches."""
+def stream_inputs(workload: str, count: int, train_batches: int):
+    """The learned-scheduler workload: its graph, source, training batches (held-out seeds) and a
+    generator of `count` evaluated stream batches."""
     from dyng_amd import generators as G
 
-    kind, scale, m, wmax, cfg, nb, (lo, hi, _) = WORKLOADS[args.workload]
+    kind, scale, m, wmax, cfg, nb, (lo, hi, _) = WORKLOADS[workload]
     n = 1 << scale
     t0 = time.perf_counter()
     src, dst, w = G.rmat(scale, m, wmax, G.GRAPH_SEED ^ cfg)
     source = G.max_out_degree_vertex(src, n)
-    count = args.stream_batches or nb
-    train = G.stream(src, dst, n, args.train_batches, lo, hi, wmax, (G.BATCH_SEED ^ cfg) ^ 0x7EA1, scale)
+    train = G.stream(src, dst, n, train_batches, lo, hi, wmax, (G.BATCH_SEED ^ cfg) ^ 0x7EA1, scale)
 
     def evaluate():
         # drawn in chunks of 500 batches (a whole stream of large batches would not fit in host
@@ -286,8 +295,15 @@ def stream_bench(args) -> int:
         for c in range(0, count, 500):
             yield from G.stream(src, dst, n, min(500, count - c), lo, hi, wmax, (G.BATCH_SEED ^ cfg) + c // 500, scale)
 
-    log(f"[bench] inputs {args.workload}: n={n} m={m} train {len(train)} batches in {time.perf_counter() - t0:.1f}s; "
+    log(f"[bench] inputs {workload}: n={n} m={m} train {len(train)} batches in {time.perf_counter() - t0:.1f}s; "
         f"evaluating {count} batches")
+    return n, m, src, dst, w, source, train, evaluate, (lo, hi)
+
+
+def stream_bench(args) -> int:
+    """The learned-scheduler configuration: RMAT-23 and a stream of uneven batches."""
+    count = args.stream_batches or WORKLOADS[args.workload][5]
+    n, m, src, dst, w, source, train, evaluate, (lo, hi) = stream_inputs(args.workload, count, args.train_batches)
     res = learned_vs_plain(n, src, dst, w, source, train, evaluate())
     out = {"metric": METRIC, "value": round(res["ops"] / (res["learned_ms_total"] / 1e3), 1), "unit": "edges/s",
            "n_gpus": 1, "steps": count, "warmup": 0,
@@ -608,6 +624,10 @@ def main() -> int:
     ap.add_argument("--train-batches", type=int, default=500, help="stream workloads: exploration batches the "
                     "scheduler trains on")
     ap.add_argument("--stream-batches", type=int, default=0, help="stream workloads: evaluated batches")
+    ap.add_argument("--computes", type=int, default=3, help="solves from the source timed for the second "
+                    "roofline figure")
+    ap.add_argument("--compute-only", action="store_true", help="build the graph and run only the timed "
+                    "solves from the source (for counter passes over the solve)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (the path does "
@@ -640,6 +660,33 @@ def main() -> int:
     def device_batch(b):
         x = batch(b, tree.to_numpy()[1] if batch.dynamic else None)
         return dyng_amd.EdgeBatch(insert=(D(x[0]), D(x[1]), D(x[2])), delete=(D(x[3]), D(x[4])))
+
+    def solves(count):
+        """`count` solves from the source on the graph as it is: their counters and the
+        roofline figure of the persistent kernel over them."""
+        cs = []
+        for _ in range(count):
+            r2 = sssp.compute(g, source)
+            cs.append(r2.counters)
+            del r2
+        cb = [compute_bytes(c, n) for c in cs]
+        cms = [c.kernel_ms for c in cs]
+        got = (sum(cb) / len(cb)) / (statistics.mean(cms) / 1e3) / 1e9
+        log(f"[bench] solve from the source: {statistics.mean(cms):.3f} ms, {got:.1f} GB/s algorithmic, "
+            f"{cs[-1].iterations} iterations, {cs[-1].relax_edges} edges scanned, {cs[-1].blocks} workgroups")
+        return {"bound": "hbm", "achieved": round(got, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(got / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.workload + "_compute"),
+                "kernel": "sssp_fused_kernel (solve from the source)", "kernel_ms_mean": round(statistics.mean(cms), 4),
+                "algorithmic_bytes_mean": int(sum(cb) / len(cb)), "solves": count,
+                "relax_edges": int(cs[-1].relax_edges), "iterations": int(cs[-1].iterations),
+                "blocks": int(cs[-1].blocks)}
+
+    if args.compute_only:
+        res = solves(max(1, args.computes))
+        print(json.dumps({"compute_roofline": res}), flush=True)
+        del tree, g
+        dyng_amd.synchronize()
+        return 0
 
     # mixed batches are uploaded up front; a tree batch is drawn (outside the timed call) against
     # the tree the previous update left
@@ -677,6 +724,23 @@ def main() -> int:
                    "relax_ms_mean": round(statistics.mean(rms), 4), "bytes_mean": int(sum(rbytes) / len(rbytes)),
                    "teps": round(sum(s.device.relax_edges for s in stats) / max(sum(rms), 1e-9) * 1e3, 1)}
 
+    # ---- where the persistent kernel's time goes: its phases (s_memrealtime stamps) and steps ----
+    def mean(f):
+        return round(statistics.mean(f(s) for s in stats), 4)
+
+    phases = {"pack_ms": mean(lambda s: s.device.pack_ms), "walk_ms": mean(lambda s: s.device.jump_ms),
+              "invalidate_and_pull_ms": mean(lambda s: s.device.step1_ms - s.device.pack_ms - s.device.jump_ms),
+              "pull_ms": mean(lambda s: s.device.pull_ms), "relax_ms": mean(lambda s: s.device.relax_ms),
+              "unpack_ms": mean(lambda s: s.device.unpack_ms), "kernel_ms": mean(lambda s: s.device.kernel_ms),
+              "walk_levels": mean(lambda s: s.device.jump_rounds), "relax_iterations": mean(lambda s: s.iterations),
+              "epochs": mean(lambda s: s.epochs), "invalidated": mean(lambda s: s.invalidated),
+              "relax_edges": mean(lambda s: s.device.relax_edges)}
+    phases["us_per_walk_level"] = round(1e3 * phases["walk_ms"] / max(phases["walk_levels"], 1e-9), 2)
+    phases["us_per_relax_step"] = round(1e3 * phases["relax_ms"] / max(phases["relax_iterations"]
+                                                                        + phases["epochs"], 1e-9), 2)
+    log(f"[bench] phases {phases}")
+    compute_roofline = solves(args.computes) if args.computes > 0 else None
+
     # ---- CPU baseline + bit-exactness on the sampled batches ----
     cpu = None
     exact = None
@@ -712,15 +776,17 @@ def main() -> int:
             log(f"[bench] parity batch {b}: {'bit-exact' if same else 'MISMATCH'}")
         exact = {"batches_checked": args.cpu_sample, "bit_exact": ok, "against": "OpenMP engine (oracle)"}
 
-    # ---- the learned scheduler against the plain schedule on the same batches ----
+    # ---- the learned scheduler against the plain schedule, on the stream it is meant for (a
+    # slice of the RMAT-23 stream of uneven batches) ----
     learned = None
-    if not args.no_learned and not batch.dynamic:
-        from dyng_amd import generators as G
-
-        kind, size, _, wmax, cfg, _, (ni, nd, nr) = WORKLOADS[args.workload]
-        train = G.mixed_batches(src, dst, n, 100, ni, nd, nr, wmax, (G.BATCH_SEED ^ cfg) ^ 0x7EA1,
-                                rmat_scale=size if kind == "rmat" else 0)
-        learned = learned_vs_plain(n, src, dst, w, source, train, [batch(b, None) for b in range(W + K)])
+    if not args.no_learned:
+        count = args.stream_batches or 1000
+        sn, sm, ssrc, sdst, sw, ssource, train, evaluate, (lo, hi) = stream_inputs("rmat23_learned", count,
+                                                                               args.train_batches)
+        learned = learned_vs_plain(sn, ssrc, sdst, sw, ssource, train, evaluate())
+        learned["workload"] = {"name": "rmat23_learned", "vertices": sn, "edges": sm, "batches": count,
+                               "batch_ops": f"log-uniform [{lo}, {hi}]", "delete_fraction": "uniform [0, 1]"}
+        del ssrc, sdst, sw, train
 
     s0 = stats[0].device
     out = {
@@ -732,7 +798,8 @@ def main() -> int:
                    "apply_ms_p50": round(statistics.median(s.apply_ms for s in stats), 4),
                    "engine_ms_p50": round(statistics.median(s.engine_ms for s in stats), 4),
                    "blocks": int(s0.blocks)},
-        "roofline": roofline, "relax_phase": relax_phase, "cpu_baseline": cpu, "parity": exact,
+        "roofline": roofline, "relax_phase": relax_phase, "phases": phases, "compute_roofline": compute_roofline,
+        "cpu_baseline": cpu, "parity": exact,
         "learned_vs_plain": learned,
     }
     print(json.dumps(out), flush=True)

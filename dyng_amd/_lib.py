@@ -116,7 +116,7 @@ EXPORTS = (
     "dyng_hip_mlp_predict", "dyng_hip_result_equal", "dyng_hip_api_update_scheduled",
     "dyng_hip_mosp_combined", "dyng_hip_api_update_each", "dyng_hip_mosp_create", "dyng_hip_mosp_update",
     "dyng_hip_mosp_arrays", "dyng_hip_mosp_path_costs", "dyng_hip_mosp_destroy",
-    "dyng_hip_sssp_update_noalloc",
+    "dyng_hip_sssp_update_noalloc", "dyng_hip_result_counters",
 )
 
 NOALLOC_SCRATCH_BYTES = 64 << 20  # DYNG_HIP_NOALLOC_SCRATCH_BYTES
@@ -197,6 +197,7 @@ def lib() -> C.CDLL:
     L.dyng_hip_result_clone.argtypes = [vp, vp, C.POINTER(vp)]
     L.dyng_hip_result_arrays.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), _i64p]
     L.dyng_hip_result_source.argtypes = [vp, _i32p]
+    L.dyng_hip_result_counters.argtypes = [vp, C.POINTER(Counters)]
     L.dyng_hip_result_version.argtypes = [vp, C.POINTER(C.c_uint64)]
     L.dyng_hip_result_options.argtypes = [vp, C.POINTER(Options)]
     L.dyng_hip_result_set_options.argtypes = [vp, C.POINTER(Options)]

@@ -127,6 +127,8 @@ struct dyng_hip_result {
   /// the token of the engine run that left this tree (its packed words may still be in the
   /// device's workspace); 0 for a tree the engine did not leave, or that changed since
   uint64_t tree_token = 0;
+  /// the work counters of the engine run that left this tree (compute, update or repair)
+  dyng_hip_counters last{};
 };
 
 using namespace dyng_hip;
@@ -309,7 +311,7 @@ void repair_core(dyng_hip_graph* g, dyng_hip_result* r, int64_t delta, int64_t m
       switch (choice) {
         case 1: run.delta = 4 * delta; break;
         case 2: run.delta = std::max<int64_t>(1, delta / 4); break;
-        case 3: run.tuning.light_degree = 256; run.tuning.chunk_edges = 64; break;
+        case 3: run.tuning.blocks_per_cu = 2; break;  // the full grid (capped at what fits)
         case 4: run.tuning.recompute = 1; break;
         default: break;
       }
@@ -342,6 +344,7 @@ void repair_core(dyng_hip_graph* g, dyng_hip_result* r, int64_t delta, int64_t m
       engine_finish(pend, &c);
     }
     r->tree_token = c.tree_token;
+    r->last = c;
     DYNG_HIP_TRY(hipEventSynchronize(ev.b));
   } catch (...) {
     r->poisoned = true;
@@ -394,6 +397,7 @@ extern "C" int dyng_hip_api_compute(dyng_hip_graph* g, int32_t source, const dyn
     dyng_hip_counters c{};
     call(dyng_hip_sssp_compute(&run, s, &c));
     r->tree_token = c.tree_token;
+    r->last = c;
     *out = r.release();
   });
 }
@@ -446,6 +450,7 @@ extern "C" int dyng_hip_api_compute_multi(dyng_hip_graph* g, const int32_t* sour
     dyng_hip_counters c{};
     call(dyng_hip_sssp_compute(&run, s, &c));
     r->tree_token = c.tree_token;
+    r->last = c;
     *out = r.release();
   });
 }
@@ -936,6 +941,13 @@ extern "C" int dyng_hip_result_arrays(const dyng_hip_result* r, const int64_t** 
 
 extern "C" int dyng_hip_result_source(const dyng_hip_result* r, int32_t* source) {
   return guarded([&] { *source = r ? r->source : -1; });
+}
+
+extern "C" int dyng_hip_result_counters(const dyng_hip_result* r, dyng_hip_counters* out) {
+  return guarded([&] {
+    if (r == nullptr || out == nullptr) fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: null result or output");
+    *out = r->last;
+  });
 }
 
 extern "C" int dyng_hip_result_version(const dyng_hip_result* r, uint64_t* version) {

@@ -1354,17 +1354,19 @@ void attach_scratch(engine_params& p, const engine_scratch& sc) {
   p.bar = sc.bar;
 }
 
-/// Every workgroup must be resident for the grid barrier. A run alone on the device fills it (two
-/// 1024-thread workgroups per CU: the waves of one hide the other's workgroup barriers and gathers,
-/// 1.6x on a from-scratch solve of RMAT-22 against one per CU). Runs launched to overlap on one
-/// device (`concurrent`) take at most half of what fits each, so two of them are resident at
-/// once. The barrier's bounded wait turns a grid that is not resident after all into an error.
-int grid_blocks(const dyng_hip_tuning& t, int occupancy, int cu_count, bool concurrent) {
+/// Every workgroup must be resident for the grid barrier. A solve from the sources alone on the
+/// device fills it (two 1024-thread workgroups per CU: on its wide frontiers the waves of one hide
+/// the other's workgroup barriers and gathers, 1.6x on RMAT-22 against one per CU); a repair, whose
+/// frontiers are narrow and whose steps are chains of dependent accesses, runs one per CU (measured
+/// 4 % faster on RMAT-22 batches of 100,000 operations than two). Runs launched to overlap on one
+/// device (`concurrent`) take at most half of what fits each, so two of them are resident at once.
+/// The barrier's bounded wait turns a grid that is not resident after all into an error.
+int grid_blocks(const dyng_hip_tuning& t, int occupancy, int cu_count, bool concurrent, bool from_sources) {
   const int resident = std::max(1, occupancy * cu_count);  // what the runtime says fits at once
   // an explicit count may go up to everything that fits (the caller runs nothing else beside it)
   if (t.grid_blocks > 0) return std::min(t.grid_blocks, resident);
   const int cap = concurrent ? std::max(1, resident / 2) : resident;
-  const int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : (concurrent ? 1 : occupancy);
+  const int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : (concurrent || !from_sources ? 1 : occupancy);
   return std::max(1, std::min(per_cu * cu_count, cap));
 }
 
@@ -1470,7 +1472,7 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
   attach_scratch(p, engine_scratch{ws->packed.p, ws->stamp.p, ws->in_far.p, ws->flag.p, ws->ancestor.p, ws->cand.p,
                                    ws->frontier.p, ws->near_a.p, ws->near_b.p, ws->far_a.p, ws->far_b.p,
                                    ws->touched.p, ws->chunks.p, ws->chunk_cap, ws->ctl.p, ws->bar.p});
-  const int blocks = grid_blocks(run->tuning, ws->occupancy, ws->cu_count, concurrent);
+  const int blocks = grid_blocks(run->tuning, ws->occupancy, ws->cu_count, concurrent, p.from_scratch != 0);
   control* hc = ws->host_ctl.p;
   std::memset(hc, 0, sizeof(control));
   hc->min_frontier = packed_inf;
@@ -1746,7 +1748,7 @@ void run_engine_noalloc(const dyng_hip_sssp_run* run, hipStream_t s, dyng_hip_co
                                    reinterpret_cast<int4*>(base + L.chunks),
                                    L.chunk_cap, reinterpret_cast<control*>(base + L.ctl),
                                    reinterpret_cast<grid_sync_state*>(base + L.bar)});
-  const int blocks = grid_blocks(run->tuning, st.occupancy, st.cu_count, false);
+  const int blocks = grid_blocks(run->tuning, st.occupancy, st.cu_count, false, p.from_scratch != 0);
   hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, s, p.bar, p.ctl);
   hipLaunchKernelGGL(sssp_fused_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
   DYNG_HIP_TRY(hipGetLastError());

@@ -5,7 +5,8 @@ same tree (bit for bit); only the time differs. Without a trained network it is 
 schedule.
 
 The five schedules (``CHOICES``): repair with the default near-far width, repair with a 4x wider
-or a 4x narrower width, repair with fine-grained tiles (short-row threshold 256, chunks of 64),
+or a 4x narrower width, repair on the full grid (two workgroups per CU, as a solve from the source
+runs: faster on the wide frontiers of large batches, slower on the narrow ones of small batches),
 and recomputing the tree from scratch.
 
 The network (12 -> 64 -> 64 -> 5, ReLU) keeps FP32 master weights and is trained on the device by
@@ -29,7 +30,7 @@ from .batch import EdgeBatch
 from .errors import InvalidArgumentError
 from .graph import Graph
 
-CHOICES = ("repair", "repair_wide", "repair_narrow", "repair_fine", "recompute")
+CHOICES = ("repair", "repair_wide", "repair_narrow", "repair_full_grid", "recompute")
 INPUTS, HIDDEN, OUTPUTS = _lib.MLP_INPUTS, _lib.MLP_HIDDEN, _lib.MLP_OUTPUTS
 FEATURES = ("log2 ops", "delete fraction", "log2 deletions", "log2 insertions", "log2 vertices",
             "log2 average degree", "tree-edge share of deletions", "log2 tree-edge deletions",
@@ -164,7 +165,7 @@ def settings(choice: int, delta: int) -> tuple[sssp.Tuning, int]:
     if choice == 2:
         return sssp.Tuning(), max(1, delta // 4)
     if choice == 3:
-        return sssp.Tuning(light_degree=256, chunk_edges=64), delta
+        return sssp.Tuning(blocks_per_cu=2), delta
     if choice == 4:
         return sssp.Tuning(recompute=True), delta
     raise InvalidArgumentError(f"scheduler: choice {choice} outside [0, {len(CHOICES)})")

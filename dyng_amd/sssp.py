@@ -205,6 +205,14 @@ class Result:
         _lib.check(_lib.lib().dyng_hip_result_source(self._handle, C.byref(s)))
         return s.value
 
+    @property
+    def counters(self) -> DeviceCounters:
+        """The work counters and timings of the engine run that left this tree (compute, update
+        or repair)."""
+        c = _lib.Counters()
+        _lib.check(_lib.lib().dyng_hip_result_counters(self._handle, C.byref(c)))
+        return DeviceCounters._from_native(c)
+
     def device_arrays(self) -> tuple[int, int, int]:
         """(distances pointer, parents pointer, n) in device memory."""
         d, p, n = C.c_void_p(), C.c_void_p(), C.c_int64()

@@ -170,9 +170,9 @@ typedef struct dyng_hip_tuning {
   int32_t engine;          /* DYNG_HIP_ENGINE_* */
   int32_t grid_blocks;     /* workgroups of the persistent launch, at most what fits at once (0:
                               blocks_per_cu per CU) */
-  int32_t blocks_per_cu;   /* workgroups per CU (0: all that fit, or one when the library overlaps
-                              two runs on the device); a caller that overlaps its own runs on one
-                              device asks for 1 */
+  int32_t blocks_per_cu;   /* workgroups per CU (0: all that fit for a solve from the sources, one
+                              for a repair or when the library overlaps two runs on the device); a
+                              caller that overlaps its own runs on one device asks for 1 */
   int32_t light_degree;    /* rows up to this length: edge-balanced inside a workgroup (0: 4096) */
   int32_t chunk_edges;     /* longer rows: split in chunks of this many edges, one wave each (0: 256) */
   int32_t launch;             /* DYNG_HIP_LAUNCH_*: a plain launch, or a hipGraph replayed per run */
@@ -324,6 +324,9 @@ int dyng_hip_result_clone(const dyng_hip_result* r, void* stream, dyng_hip_resul
 int dyng_hip_result_arrays(const dyng_hip_result* r, const int64_t** distances,
                            const int32_t** parents, int64_t* n);
 int dyng_hip_result_source(const dyng_hip_result* r, int32_t* source);
+/* The work counters and timings of the engine run that left the result's tree (compute, update
+ * or repair); no counterpart in the reference, whose compute() reports no stats. */
+int dyng_hip_result_counters(const dyng_hip_result* r, dyng_hip_counters* out);
 int dyng_hip_result_version(const dyng_hip_result* r, uint64_t* version);
 int dyng_hip_result_options(const dyng_hip_result* r, dyng_hip_options* opt);
 int dyng_hip_result_set_options(dyng_hip_result* r, const dyng_hip_options* opt);
