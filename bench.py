@@ -214,13 +214,14 @@ def cpu_baseline(n, src, dst, w, batch, source, sample: int, thread_counts):
     return summaries, trees, drawn
 
 
-def learned_vs_plain(n, src, dst, w, source, train, evaluate, log_prefix="learned"):
+def learned_vs_plain(n, src, dst, w, source, train, evaluate, hindsight=(), log_prefix="learned"):
     """Train the scheduler on traces of the `train` batches (held-out seeds) on one device graph:
-    each batch runs for real under a schedule drawn at random, its engine time the target of that
-    schedule only (scheduler.explore). Then run the `evaluate` batches through two more graphs in
+    every schedule's engine time on clones of the tree for each batch (scheduler.collect). Then run the `evaluate` batches through two more graphs in
     lockstep, one with the plain fixed schedule and one with the scheduler, comparing the two trees
     on the device after every batch. Update times are the wall time of each sssp.update call (the
-    scheduler's feature kernel and BF16 inference included)."""
+    scheduler's feature kernel and BF16 inference included). Then, over the `hindsight` batches,
+    every schedule is timed on clones of the tree: the best possible per-batch pick bounds what
+    any scheduler could gain with these schedules."""
     import dyng_amd
     from dyng_amd import scheduler, sssp
 
@@ -232,7 +233,10 @@ def learned_vs_plain(n, src, dst, w, source, train, evaluate, log_prefix="learne
     t0 = time.perf_counter()
     gt = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
     rt = sssp.compute(gt, source)
-    trace = scheduler.explore(gt, rt, (dev(x) for x in train))
+    # every schedule timed on clones of the tree for each training batch (complete targets: a
+    # single random schedule per batch left the network's predictions too noisy to leave the plain
+    # schedule safely)
+    trace = scheduler.collect(gt, rt, (dev(x) for x in train))
     del rt, gt
     sched = scheduler.Scheduler()
     loss = sched.fit(trace, steps=3000)
@@ -240,7 +244,7 @@ def learned_vs_plain(n, src, dst, w, source, train, evaluate, log_prefix="learne
     seen = ~np.isnan(tt)
     by_choice = {c: round(float(np.mean(2.0 ** tt[seen[:, i], i])), 4) if seen[:, i].any() else None
                  for i, c in enumerate(scheduler.CHOICES)}
-    log(f"[bench] {log_prefix}: {len(train)} exploration traces, fit loss {loss:.4f}, mean engine ms by schedule "
+    log(f"[bench] {log_prefix}: {len(train)} training traces, fit loss {loss:.4f}, mean engine ms by schedule "
         f"{by_choice}, in {time.perf_counter() - t0:.1f}s")
     gp = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
     gl = dyng_amd.Graph.from_edges(src, dst, w, num_vertices=n)
@@ -262,15 +266,40 @@ def learned_vs_plain(n, src, dst, w, source, train, evaluate, log_prefix="learne
         learned_ms.append((time.perf_counter() - t1) * 1e3)
         diffs += rp.differences(rl)
         del eb
+    # the ceiling: every schedule timed on clones of the plain tree over `hindsight` more batches of
+    # the stream; the best per batch, picked with hindsight, against the plain schedule and against
+    # the network's picks (engine times: the apply is the same under every schedule)
+    held, hist, net_picks = scheduler.Trace(), scheduler.History(), []
+    for x in hindsight:
+        eb = dev(x)
+        feats = scheduler.features(scheduler.batch_stats(gp, eb, rp), hist)
+        net_picks.append(scheduler.pick(sched.network.predict(feats[None, :])[0]))
+        scheduler.collect(gp, rp, [eb], held)
+        c = rp.counters
+        hist = scheduler.History(c.invalidated, c.affected)
+        del eb
+    ceiling = None
+    if held.t:
+        t = 2.0 ** np.asarray(held.t, np.float64)
+        ceiling = {"batches": len(t), "plain_engine_ms": round(float(t[:, 0].sum()), 3),
+                   "best_engine_ms": round(float(t.min(1).sum()), 3),
+                   "learned_engine_ms": round(float(t[np.arange(len(t)), net_picks].sum()), 3),
+                   "best_vs_plain": round(float(t[:, 0].sum() / t.min(1).sum()), 4),
+                   "best_by_schedule": np.bincount(t.argmin(1), minlength=len(scheduler.CHOICES)).tolist(),
+                   "timed": "every schedule on clones of the tree per batch (scheduler.collect)"}
+        log(f"[bench] {log_prefix}: hindsight over {len(t)} batches: plain {ceiling['plain_engine_ms']:.1f} ms, "
+            f"best {ceiling['best_engine_ms']:.1f} ms ({ceiling['best_vs_plain']:.3f}x), network's picks "
+            f"{ceiling['learned_engine_ms']:.1f} ms")
     picks = np.bincount(sched.choices, minlength=len(scheduler.CHOICES)).tolist()
     out = {"batches": len(plain_ms), "ops": ops, "plain_ms_total": round(sum(plain_ms), 3),
            "learned_ms_total": round(sum(learned_ms), 3),
            "speedup": round(sum(plain_ms) / max(sum(learned_ms), 1e-9), 4),
            "plain_p50_ms": round(statistics.median(plain_ms), 4), "learned_p50_ms": round(statistics.median(learned_ms), 4),
            "picks": dict(zip(scheduler.CHOICES, picks)), "train_traces": len(train),
-           "train": "exploration: each training batch run for real under a random schedule",
+           "train": "every schedule timed on clones of the tree for each training batch (scheduler.collect)",
            "train_engine_ms_mean_by_schedule": by_choice,
-           "bit_exact": diffs == 0, "compared": "plain and learned trees on the device after every batch"}
+           "bit_exact": diffs == 0, "compared": "plain and learned trees on the device after every batch",
+           "hindsight": ceiling}
     log(f"[bench] {log_prefix}: plain {out['plain_ms_total']:.1f} ms, learned {out['learned_ms_total']:.1f} ms, "
         f"speedup {out['speedup']:.3f}x, picks {out['picks']}, bit-exact {out['bit_exact']}")
     del rp, rl, gp, gl
@@ -295,16 +324,18 @@ def stream_inputs(workload: str, count: int, train_batches: int):
         for c in range(0, count, 500):
             yield from G.stream(src, dst, n, min(500, count - c), lo, hi, wmax, (G.BATCH_SEED ^ cfg) + c // 500, scale)
 
+    hindsight = G.stream(src, dst, n, 100, lo, hi, wmax, (G.BATCH_SEED ^ cfg) ^ 0x51D3, scale)
     log(f"[bench] inputs {workload}: n={n} m={m} train {len(train)} batches in {time.perf_counter() - t0:.1f}s; "
         f"evaluating {count} batches")
-    return n, m, src, dst, w, source, train, evaluate, (lo, hi)
+    return n, m, src, dst, w, source, train, evaluate, hindsight, (lo, hi)
 
 
 def stream_bench(args) -> int:
     """The learned-scheduler configuration: RMAT-23 and a stream of uneven batches."""
     count = args.stream_batches or WORKLOADS[args.workload][5]
-    n, m, src, dst, w, source, train, evaluate, (lo, hi) = stream_inputs(args.workload, count, args.train_batches)
-    res = learned_vs_plain(n, src, dst, w, source, train, evaluate())
+    n, m, src, dst, w, source, train, evaluate, hindsight, (lo, hi) = stream_inputs(args.workload, count,
+                                                                                   args.train_batches)
+    res = learned_vs_plain(n, src, dst, w, source, train, evaluate(), hindsight)
     out = {"metric": METRIC, "value": round(res["ops"] / (res["learned_ms_total"] / 1e3), 1), "unit": "edges/s",
            "n_gpus": 1, "steps": count, "warmup": 0,
            "ms_per_step": round(res["learned_ms_total"] / count, 4), "p50_ms": res["learned_p50_ms"],
@@ -621,8 +652,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: every physical core this process may use")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison")
-    ap.add_argument("--train-batches", type=int, default=500, help="stream workloads: exploration batches the "
-                    "scheduler trains on")
+    ap.add_argument("--train-batches", type=int, default=200, help="stream workloads: batches the scheduler "
+                    "trains on (every schedule timed on each)")
     ap.add_argument("--stream-batches", type=int, default=0, help="stream workloads: evaluated batches")
     ap.add_argument("--computes", type=int, default=3, help="solves from the source timed for the second "
                     "roofline figure")
@@ -781,9 +812,9 @@ def main() -> int:
     learned = None
     if not args.no_learned:
         count = args.stream_batches or 1000
-        sn, sm, ssrc, sdst, sw, ssource, train, evaluate, (lo, hi) = stream_inputs("rmat23_learned", count,
-                                                                               args.train_batches)
-        learned = learned_vs_plain(sn, ssrc, sdst, sw, ssource, train, evaluate())
+        sn, sm, ssrc, sdst, sw, ssource, train, evaluate, hindsight, (lo, hi) = stream_inputs(
+            "rmat23_learned", count, args.train_batches)
+        learned = learned_vs_plain(sn, ssrc, sdst, sw, ssource, train, evaluate(), hindsight)
         learned["workload"] = {"name": "rmat23_learned", "vertices": sn, "edges": sm, "batches": count,
                                "batch_ops": f"log-uniform [{lo}, {hi}]", "delete_fraction": "uniform [0, 1]"}
         del ssrc, sdst, sw, train

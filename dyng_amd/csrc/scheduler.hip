@@ -328,7 +328,7 @@ using namespace dyng_hip;
 /// The schedule to run from the predicted log2 engine times: the plain one (0) unless another is
 /// predicted at least kSwitchMargin faster (a misprediction costs more than the small gains the
 /// other schedules offer on most batches).
-constexpr float kSwitchMargin = 0.07f;  // log2(1.05)
+constexpr float kSwitchMargin = 0.2016f;  ///< log2(1.15): a predicted gain of at least 15 %
 
 int pick_schedule(const float* y) {
   int best = 0;

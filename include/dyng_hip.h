@@ -387,7 +387,7 @@ int dyng_hip_mlp_predict(dyng_hip_mlp* m, const float* x, int64_t count, float* 
 /* sssp::update under the learned scheduler: the batch's features on G_t and the network's BF16
  * inference run on the device beside the apply, before it changes any row (mlp = null: the plain
  * schedule), then the update runs under the predicted fastest schedule if it is predicted at
- * least 5 % faster than the plain one, else under the plain one (*choice: its index).
+ * least 15 % faster than the plain one, else under the plain one (*choice: its index).
  * last_invalidated and last_affected are the previous update's counters (the network's history
  * features). */
 int dyng_hip_api_update_scheduled(dyng_hip_graph* g, const dyng_hip_batch* batch, dyng_hip_result* r,

@@ -163,11 +163,12 @@ def test_trained_scheduler_against_the_plain_schedule(dg):
     hr = dg.sssp.compute(hg, 0)
     held = dg.scheduler.Trace()
     history = dg.scheduler.History()
-    picked, plain, learned, best = [], 0.0, 0.0, 0.0
+    picked, plain, learned, best, ranked = [], 0.0, 0.0, 0.0, 0
     for b in batches:
         batch = dg.EdgeBatch(insert=b[:3], delete=b[3:])
         x = dg.scheduler.features(dg.scheduler.batch_stats(hg, batch, hr), history)
-        choice = dg.scheduler.pick(sched.network.predict(x[None, :])[0])
+        y = sched.network.predict(x[None, :])[0]
+        choice = dg.scheduler.pick(y)
         dg.scheduler.collect(hg, hr, [batch], held)  # times every schedule, then repairs hr
         og, _, c = oracle.update(og, ot, b[0], b[1], b[2].reshape(-1, 1), b[3], b[4])
         d, p = hr.to_numpy()
@@ -178,6 +179,14 @@ def test_trained_scheduler_against_the_plain_schedule(dg):
         plain += times[0]
         learned += times[choice]
         best += times.min()
+        ranked += int(np.argmax(y) == int(np.argmax(times)))
     _say(f"[scheduler] held out: plain {plain:.2f} ms, learned {learned:.2f} ms, best possible {best:.2f} ms, "
-         f"gain {plain / learned:.3f}x, picks {np.bincount(picked, minlength=5).tolist()}")
-    assert learned <= 1.05 * plain
+         f"gain {plain / learned:.3f}x, picks {np.bincount(picked, minlength=5).tolist()}, slowest schedule "
+         f"predicted on {ranked}/{len(batches)} batches")
+    # what the trained network knows: the schedule that costs most (recomputing from scratch, 10x
+    # to 100x a repair here) is the one it predicts slowest on held-out batches, so it never runs it
+    # where a repair wins; and leaving the plain schedule only on a predicted 15 % gain, its picks
+    # cost no more than the plain schedule beyond timing noise. The schedules' costs differ too
+    # little on these batches for a larger gain (the best possible pick is reported beside).
+    assert ranked >= 0.9 * len(batches)
+    assert learned <= 1.03 * plain
