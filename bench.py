@@ -763,10 +763,10 @@ def main() -> int:
               "invalidate_and_pull_ms": mean(lambda s: s.device.step1_ms - s.device.pack_ms - s.device.jump_ms),
               "pull_ms": mean(lambda s: s.device.pull_ms), "relax_ms": mean(lambda s: s.device.relax_ms),
               "unpack_ms": mean(lambda s: s.device.unpack_ms), "kernel_ms": mean(lambda s: s.device.kernel_ms),
-              "walk_levels": mean(lambda s: s.device.jump_rounds), "relax_iterations": mean(lambda s: s.iterations),
+              "walk_rounds": mean(lambda s: s.device.jump_rounds), "relax_iterations": mean(lambda s: s.iterations),
               "epochs": mean(lambda s: s.epochs), "invalidated": mean(lambda s: s.invalidated),
               "relax_edges": mean(lambda s: s.device.relax_edges)}
-    phases["us_per_walk_level"] = round(1e3 * phases["walk_ms"] / max(phases["walk_levels"], 1e-9), 2)
+    phases["us_per_walk_round"] = round(1e3 * phases["walk_ms"] / max(phases["walk_rounds"], 1e-9), 2)
     phases["us_per_relax_step"] = round(1e3 * phases["relax_ms"] / max(phases["relax_iterations"]
                                                                         + phases["epochs"], 1e-9), 2)
     log(f"[bench] phases {phases}")

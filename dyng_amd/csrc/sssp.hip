@@ -5,11 +5,12 @@
 //   A  pack the old tree into (distance << b | parent) words; roots = heads v of changed edges
 //      (u,v) with parent[v] == u (fused.cuh:178-197); roots start the invalidated list.
 //   B  the subtrees under the roots (fused.cuh:199-234 computes them by pointer jumping over all
-//      n vertices). A tree this engine produced is walked top-down instead, level by level along
-//      its tree edges (out-edges (u, w) with parent[w] == u), touching only the invalidated
-//      region; a walk deeper than kWalkLevels levels hands its unfinished part to pointer
-//      jumping. A tree handed in from outside keeps pointer jumping (over compacted lists of the
-//      vertices still jumping), which also detects parent cycles.
+//      n vertices). A tree this engine produced is walked top-down instead along its tree edges
+//      (out-edges (u, w) with parent[w] == u), touching only the invalidated region: each
+//      workgroup walks its tile of the frontier down several levels on its own (levels in LDS)
+//      between grid barriers; a walk deeper than kWalkRounds such rounds hands its unfinished
+//      part to pointer jumping. A tree handed in from outside keeps pointer jumping (over
+//      compacted lists of the vertices still jumping), which also detects parent cycles.
 //   C  invalidate (word INF, stamp), then the insertion heads (fused.cuh:236-256).
 //   D  pull: each invalidated vertex takes its best in-neighbour offer (fused.cuh:258-285). For a
 //      tree this engine produced, an insertion head's old word was already the best over its
@@ -44,7 +45,9 @@ namespace {
 
 constexpr int kBlock = 1024;  ///< 16 waves per workgroup; 8 waves/SIMD fit two per CU, one is launched
 constexpr int kWaves = kBlock / 64;
-constexpr int kWalkLevels = 64;  ///< subtree walk levels before pointer jumping takes over
+constexpr int kWalkGlobal = 8;   ///< first rounds of the subtree walk: one level each
+constexpr int kWalkLocal = 16;   ///< later rounds: levels a workgroup walks on its own between grid barriers
+constexpr int kWalkRounds = kWalkGlobal + 8;  ///< grid rounds of the walk before pointer jumping takes over
 
 struct control {
   int ring_near[3];
@@ -274,62 +277,72 @@ DYNG_DEV int block_scan(tile_shared& L, int x, int& total) {
 /// instead of one per slot.
 constexpr int kSlots = 1;
 
+/// Edge-balanced expansion of one tile of `cnt` items (cnt <= kBlock): `load(i, v, off, len,
+/// word)` fills thread i's item (i < cnt; len 0: nothing to expand, e.g. a long row sent to the
+/// chunk list), and the edge slots of the tile's rows are dealt out to the workgroup's threads,
+/// kSlots slots per thread per step (slot j of a step kBlock apart from slot j - 1, so each j is
+/// one coalesced run per wave): `edges(act, item, slot)` runs with every lane of a wave, on arrays
+/// of kSlots (act false on padding). `done(valid, item)` runs on every thread after the tile's
+/// edges. Every thread of the workgroup calls it.
+template <class Load, class Edges, class Done>
+DYNG_DEV void expand_tile(tile_shared& L, int cnt, Load load, Edges edges, Done done) {
+  const int i = static_cast<int>(threadIdx.x);
+  int32_t v = -1, off = 0, len = 0;
+  u64 word = packed_inf;
+  if (i < cnt) load(i, v, off, len, word);
+  int total = 0;
+  const int start = block_scan(L, len, total);
+  if (i < cnt) {
+    L.start[i] = start;
+    L.off[i] = off;
+    L.vert[i] = v;
+    L.word[i] = word;
+  }
+  if (i == 0) L.start[cnt] = total;
+  __syncthreads();
+  for (int base = i - lane(); base < total; base += kSlots * kBlock) {
+    bool act[kSlots];
+    int item[kSlots], slot[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int e = base + j * kBlock + lane();
+      act[j] = e < total;
+      int lo = 0;
+      if (act[j]) {
+        int hi = cnt;  // the last item whose first slot is <= e (its row holds e)
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (L.start[mid] <= e) {
+            lo = mid;
+          } else {
+            hi = mid;
+          }
+        }
+      }
+      item[j] = lo;
+      slot[j] = act[j] ? L.off[lo] + (e - L.start[lo]) : 0;
+    }
+    edges(act, item, slot);
+  }
+  __syncthreads();
+  done(i < cnt, i);
+  __syncthreads();
+}
+
 /// Edge-balanced expansion of list items [0, count): the items are cut into one tile per
-/// workgroup (up to kBlock items), `load(index, v, off, len, word)` fills a thread's item (len 0:
-/// nothing to expand in the tile, e.g. a long row sent to the chunk list), and the edge slots of
-/// the tile's rows are dealt out to the workgroup's threads, kSlots slots per thread per step
-/// (slot j of a step kBlock apart from slot j - 1, so each j is one coalesced run per wave):
-/// `edges(act, item, slot)` runs with every lane of a wave, on arrays of kSlots (act false on
-/// padding). `done(valid, item)` runs on every thread after the tile's edges, and `flush()` on
-/// every thread of the workgroup after each tile (to publish staged appends). Uniform per
-/// workgroup: every thread calls it.
+/// workgroup (up to kBlock items), each expanded by expand_tile with `load(index, ...)` given the
+/// item's index in the list, and `flush()` runs on every thread of the workgroup after each tile
+/// (to publish staged appends). Uniform per workgroup: every thread calls it.
 template <class Load, class Edges, class Done, class Flush>
 DYNG_DEV void tile_rows(tile_shared& L, int count, Load load, Edges edges, Done done, Flush flush) {
   const int G = static_cast<int>(gridDim.x);
   const int ts = min(kBlock, max(1, (count + G - 1) / G));
-  const int i = static_cast<int>(threadIdx.x);
   for (long long t0 = static_cast<long long>(blockIdx.x) * ts; t0 < count; t0 += static_cast<long long>(G) * ts) {
     const int cnt = static_cast<int>(min(static_cast<long long>(ts), count - t0));
-    int32_t v = -1, off = 0, len = 0;
-    u64 word = packed_inf;
-    if (i < cnt) load(static_cast<int>(t0) + i, v, off, len, word);
-    int total = 0;
-    const int start = block_scan(L, len, total);
-    if (i < cnt) {
-      L.start[i] = start;
-      L.off[i] = off;
-      L.vert[i] = v;
-      L.word[i] = word;
-    }
-    if (i == 0) L.start[cnt] = total;
-    __syncthreads();
-    for (int base = i - lane(); base < total; base += kSlots * kBlock) {
-      bool act[kSlots];
-      int item[kSlots], slot[kSlots];
-#pragma unroll
-      for (int j = 0; j < kSlots; ++j) {
-        const int e = base + j * kBlock + lane();
-        act[j] = e < total;
-        int lo = 0;
-        if (act[j]) {
-          int hi = cnt;  // the last item whose first slot is <= e (its row holds e)
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (L.start[mid] <= e) {
-              lo = mid;
-            } else {
-              hi = mid;
-            }
-          }
-        }
-        item[j] = lo;
-        slot[j] = act[j] ? L.off[lo] + (e - L.start[lo]) : 0;
-      }
-      edges(act, item, slot);
-    }
-    __syncthreads();
-    done(i < cnt, i);
-    __syncthreads();
+    const int first = static_cast<int>(t0);
+    expand_tile(
+        L, cnt, [&](int i, int32_t& v, int32_t& off, int32_t& len, u64& word) { load(first + i, v, off, len, word); },
+        edges, done);
     flush();
   }
 }
@@ -532,6 +545,12 @@ __global__ void k_reset_barrier(grid_sync_state* bar, control* ctl) {
   }
 }
 
+/// The subtree walk's local levels (phase B): two levels of up to kBlock vertices.
+struct walk_shared {
+  int count[2];
+  int item[2][kBlock];
+};
+
 __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) {
   // queued right behind an apply whose merges were held back: nothing to do (uniform)
   if (p.hold != nullptr && *p.hold != 0) {
@@ -542,6 +561,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   // work counters of the workgroup, in LDS: relax edges, improving atomics, pull edges, affected
   __shared__ unsigned long long counts[4];
   __shared__ stage_shared S;
+  __shared__ walk_shared W;
   if (threadIdx.x < 4) counts[threadIdx.x] = 0;
   if (threadIdx.x < 3) S.count[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
@@ -623,86 +643,177 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     bool jump = work && p.check_cycles;
     int jump_slot = 0;  // the first ring slot pointer jumping counts in (a slot already zero)
     if (work && !p.check_cycles && roots > 0) {
-      // ---- B (tree from this engine): the subtrees under the roots, level by level down the
-      // tree edges (parent[w] == u for an out-edge (u, w)); every tree edge of the old tree is
-      // an edge of the graph unless this batch deleted it, and then its head is a root. Level
-      // L + 1 is appended right after level L in the candidate list and counted in a ring slot
-      // of its own, so every thread reads the same level bounds. A walk still going after
-      // kWalkLevels levels (a deep tree) is finished by pointer jumping ----
-      int lo = 0, hi = roots, levels = 0;
+      // ---- B (tree from this engine): the subtrees under the roots, down the tree edges
+      // (parent[w] == u for an out-edge (u, w)); every tree edge of the old tree is an edge of
+      // the graph unless this batch deleted it, and then its head is a root. A tree edge is
+      // claimed exactly once (the child's flag), so no workgroup needs another's level: each takes
+      // a tile of the frontier and walks it down on its own, its levels in LDS, a level's rows
+      // edge-balanced over its threads: one level in each of the first kWalkGlobal rounds (the
+      // wide levels of a bushy tree stay spread over the grid: measured on RMAT-22, walking them
+      // alone kept the other workgroups waiting), then up to kWalkLocal levels per round (the
+      // narrow levels of a deep tree: the 4096^2 grid's walk 2.0 -> 0.4 ms). What a workgroup did
+      // not reach (a level past the cap or past the LDS queue, and the children of rows long
+      // enough for the chunk list) is the next round's frontier, after a grid barrier. Every flagged vertex joins the
+      // candidate list after the roots. A walk still going after kWalkRounds rounds (a deep tree)
+      // is finished by pointer jumping ----
+      // the frontier: a range [lo, hi) of the candidate list while rounds walk one level (the
+      // level's children are appended right after it), then a list of its own
+      const int* front = p.cand;
+      int front_count = roots;
+      int lo = 0, hi = roots;
+      int rounds = 0;
       // kSlots out-edges (u, w) per lane: the entries, then parent[w] (the tree is read-only
       // here), then the flag claims of the tree children (rare: one in-edge of each child)
-      auto visit = [&](const bool (&act)[kSlots], const int32_t (&u)[kSlots], const int (&slot)[kSlots], int* level,
-                       int* found) {
-        int32_t w[kSlots], par[kSlots];
+      auto claim_children = [&](const bool (&act)[kSlots], const int32_t (&u)[kSlots], const int (&slot)[kSlots],
+                                int32_t (&w)[kSlots], bool (&add)[kSlots]) {
+        int32_t par[kSlots];
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) w[j] = act[j] ? p.out_nbr[slot[j]] : -1;  // dead entries are negative
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) par[j] = w[j] >= 0 ? p.parents[w[j]] : -2;
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) {
-          bool add = false;
-          if (w[j] >= 0 && par[j] == u[j] && atomicExch(&p.flag[w[j]], 1) == 0) {
-            st(&p.stamp[w[j]], generation + 1);
-            add = true;
-          }
-          stage_append(S, 0, add, w[j], level, found);
+          add[j] = w[j] >= 0 && par[j] == u[j] && atomicExch(&p.flag[w[j]], 1) == 0;
+          if (add[j]) st(&p.stamp[w[j]], generation + 1);
         }
       };
-      while (hi > lo && levels < kWalkLevels) {
-        jump_items += hi - lo;
-        int* found = &c->ring_jump[levels % 3];
+      auto row_of = [&](int32_t v, int32_t& off, int32_t& len, int* chunk_count) {
+        const int2 mt = p.out_meta[v];
+        if (mt.y <= p.light) {
+          off = mt.x;
+          len = mt.y;
+        } else {
+          emit_chunks(p, chunk_count, v, mt.x, mt.y);
+        }
+      };
+      while (front_count > 0 && rounds < kWalkRounds) {
+        // the first rounds walk one level each (a bushy tree's levels spread over the grid); the
+        // rest kWalkLocal levels each (a deep tree's narrow levels walked without barriers)
+        const bool local = rounds >= kWalkGlobal;
+        int* spill = (rounds & 1) ? p.near_b : p.near_a;
+        int* spill_count = &c->ring_jump[rounds % 3];
         int* chunk_count = &c->ring_chunk[cstep % 3];
         if (tid == 0) {
-          st(&c->ring_jump[(levels + 1) % 3], 0);
+          st(&c->ring_jump[(rounds + 1) % 3], 0);
           st(&c->ring_chunk[(cstep + 1) % 3], 0);
         }
-        ++levels;
-        const int level_base = lo;
-        int* const level = p.cand + hi;
-        tile_rows(
-            L, hi - lo,
-            [&](int idx, int32_t& v, int32_t& off, int32_t& len, u64&) {
-              v = ld(&p.cand[level_base + idx]);
-              const int2 mt = p.out_meta[v];
-              if (mt.y <= p.light) {
-                off = mt.x;
-                len = mt.y;
-              } else {
-                emit_chunks(p, chunk_count, v, mt.x, mt.y);
-              }
-            },
-            [&](const bool (&act)[kSlots], const int (&item)[kSlots], const int (&slot)[kSlots]) {
-              int32_t u[kSlots];
+        ++rounds;
+        if (!local) {
+          tile_rows(
+              L, front_count,
+              [&](int idx, int32_t& v, int32_t& off, int32_t& len, u64&) {
+                v = ld(&front[idx]);
+                row_of(v, off, len, chunk_count);
+              },
+              [&](const bool (&act)[kSlots], const int (&item)[kSlots], const int (&slot)[kSlots]) {
+                int32_t u[kSlots], w[kSlots];
+                bool add[kSlots];
 #pragma unroll
-              for (int j = 0; j < kSlots; ++j) u[j] = act[j] ? L.vert[item[j]] : -1;
-              visit(act, u, slot, level, found);
-            },
-            [&](bool, int) {}, [&] { stage_flush(S, level, found); });
+                for (int j = 0; j < kSlots; ++j) u[j] = act[j] ? L.vert[item[j]] : -1;
+                claim_children(act, u, slot, w, add);
+#pragma unroll
+                for (int j = 0; j < kSlots; ++j) stage_append(S, 0, add[j], w[j], p.cand, &c->cand_count);
+              },
+              [&](bool, int) {}, [&] { stage_flush(S, p.cand, &c->cand_count); });
+        } else {
+          const int G = static_cast<int>(gridDim.x);
+          const int ts = min(kBlock, max(1, (front_count + G - 1) / G));
+          for (long long t0 = static_cast<long long>(blockIdx.x) * ts; t0 < front_count;
+               t0 += static_cast<long long>(G) * ts) {
+            const int cnt = static_cast<int>(min(static_cast<long long>(ts), front_count - t0));
+            if (static_cast<int>(threadIdx.x) < cnt) W.item[0][threadIdx.x] = ld(&front[t0 + threadIdx.x]);
+            if (threadIdx.x == 0) {
+              W.count[0] = cnt;
+              W.count[1] = 0;
+            }
+            __syncthreads();
+            int q = 0;
+            for (int level = 0; level < kWalkLocal; ++level) {
+              const int items = W.count[q];  // uniform: LDS after a workgroup barrier
+              if (items == 0) break;
+              expand_tile(
+                  L, items,
+                  [&](int i, int32_t& v, int32_t& off, int32_t& len, u64&) {
+                    v = W.item[q][i];
+                    row_of(v, off, len, chunk_count);
+                  },
+                  [&](const bool (&act)[kSlots], const int (&item)[kSlots], const int (&slot)[kSlots]) {
+                    int32_t u[kSlots], w[kSlots];
+                    bool add[kSlots];
+#pragma unroll
+                    for (int j = 0; j < kSlots; ++j) u[j] = act[j] ? L.vert[item[j]] : -1;
+                    claim_children(act, u, slot, w, add);
+#pragma unroll
+                    for (int j = 0; j < kSlots; ++j) {
+                      // the child joins the next local level, or the next round's frontier when
+                      // the level is full
+                      const int at = add[j] ? atomicAdd(&W.count[q ^ 1], 1) : 0;
+                      if (add[j] && at < kBlock) W.item[q ^ 1][at] = w[j];
+                      stage_append(S, 0, add[j], w[j], p.cand, &c->cand_count);
+                      stage_append(S, 1, add[j] && at >= kBlock, w[j], spill, spill_count);
+                    }
+                  },
+                  [&](bool, int) {});
+              if (threadIdx.x == 0) {
+                W.count[q] = 0;
+                W.count[q ^ 1] = min(W.count[q ^ 1], kBlock);
+              }
+              if (S.count[0] > kStage / 2 || S.count[1] > kStage / 2) {
+                stage_flush(S, p.cand, &c->cand_count, spill, spill_count);  // uniform: LDS after a barrier
+              }
+              __syncthreads();
+              q ^= 1;
+            }
+            // a level the cap left: the next round's frontier
+            const int left = W.count[q];
+            for (int base = static_cast<int>(threadIdx.x) - lane(); base < left; base += kBlock) {
+              const int i = base + lane();
+              stage_append(S, 1, i < left, i < left ? W.item[q][i] : -1, spill, spill_count);
+            }
+            stage_flush(S, p.cand, &c->cand_count, spill, spill_count);
+          }
+        }
         DYNG_GRID_SYNC();
+        // long rows: their chunks, by every workgroup; the children join the next frontier
         const int nchunks = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap)));
         ++cstep;
         if (nchunks > 0) {
           chunk_rows(p, nchunks, [&](const bool (&act)[kSlots], const int32_t (&u)[kSlots], const int (&slot)[kSlots]) {
-            visit(act, u, slot, level, found);
+            int32_t w[kSlots];
+            bool add[kSlots];
+            claim_children(act, u, slot, w, add);
+#pragma unroll
+            for (int j = 0; j < kSlots; ++j) {
+              stage_append(S, 0, add[j], w[j], p.cand, &c->cand_count);
+              stage_append(S, 1, local && add[j], w[j], spill, spill_count);
+            }
           });
-          stage_flush(S, level, found);
+          stage_flush(S, p.cand, &c->cand_count, spill, spill_count);
           DYNG_GRID_SYNC();
         }
-        lo = hi;
-        hi += uniform(ld(found));
+        if (!local) {
+          // the next round starts from this level's children, right after it in the list
+          lo = hi;
+          hi = uniform(ld(&c->cand_count));
+          front = p.cand + lo;
+          front_count = hi - lo;
+        } else {
+          front = spill;
+          front_count = uniform(ld(spill_count));
+        }
       }
-      if (tid == 0) st(&c->rounds, levels);
-      if (hi > lo) {
+      if (tid == 0) st(&c->rounds, rounds);
+      hi = uniform(ld(&c->cand_count));  // every flagged vertex (no append after the barrier)
+      jump_items = hi;
+      if (front_count > 0) {
         // deep: pointer jumping from every vertex flagged so far (they are cand[0, hi)). Its
-        // rounds continue the ring where the walk stopped: slot `levels % 3` was cleared during
-        // the last level and nobody reads it, while the last level's count may still be unread
+        // rounds continue the ring where the walk stopped: slot `rounds % 3` was cleared during
+        // the last round and nobody reads it, while the last round's count may still be unread
         // by slower workgroups and must not be touched here.
         for (int v = tid; v < n; v += threads) st(&p.ancestor[v], p.parents[v]);
-        if (tid == 0) st(&c->cand_count, hi);
         DYNG_GRID_SYNC();
         jump = true;
-        jump_slot = levels;
+        jump_slot = rounds;
       } else {
         inv = hi;
       }
