@@ -36,14 +36,13 @@ constexpr int64_t kInf = sssp_infinity;
 /// The pooled device workspaces of each device (ADR 0015: sized once, shared by the result runs
 /// on the device; runs on one device are serialized by the pool's mutex). Slot 0 serves one
 /// result at a time; an update of several results gives result i slot i % kWorkspaceSlots, so
-/// those results keep their own packed words from one update to the next and two of their runs
-/// can be in flight at once (on the caller's stream and the device's second stream).
+/// those results keep their own packed words from one update to the next (up to kWorkspaceSlots
+/// results) and kWorkspaceSlots of their runs can be queued ahead of the host.
 constexpr int kWorkspaceSlots = 4;
 
 struct workspace_pool {
   std::mutex lock;
   std::map<std::pair<int, int>, dyng_hip_workspace*> per_slot;
-  std::map<int, hipStream_t> second_stream;
   dyng_hip_workspace* get(int device, int slot = 0) {
     auto it = per_slot.find({device, slot});
     if (it != per_slot.end()) return it->second;
@@ -51,16 +50,6 @@ struct workspace_pool {
     if (dyng_hip_workspace_create(device, &ws) != DYNG_HIP_OK) fail(DYNG_HIP_OUT_OF_MEMORY, dyng_hip_last_error());
     per_slot[{device, slot}] = ws;
     return ws;
-  }
-  /// A second stream of the device (default priority; non-blocking, so that it runs beside a
-  /// caller's work on the null stream), joined to the caller's stream by events.
-  hipStream_t stream2(int device) {
-    auto it = second_stream.find(device);
-    if (it != second_stream.end()) return it->second;
-    hipStream_t st = nullptr;
-    DYNG_HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    second_stream[device] = st;
-    return st;
   }
 };
 
@@ -321,7 +310,7 @@ void repair_core(dyng_hip_graph* g, dyng_hip_result* r, int64_t delta, int64_t m
     DYNG_HIP_TRY(hipEventRecord(ev.a, s));
     r->tree_token = 0;
     // queued right behind the apply when it is in flight: one synchronization for both
-    engine_pending pend = engine_launch(&run, true, s, false, inflight != nullptr ? apply_hold(g) : nullptr);
+    engine_pending pend = engine_launch(&run, true, s, inflight != nullptr ? apply_hold(g) : nullptr);
     DYNG_HIP_TRY(hipEventRecord(ev.b, s));
     if (inflight != nullptr) {
       try {
@@ -608,15 +597,13 @@ extern "C" int dyng_hip_api_update_each(dyng_hip_graph* g, const dyng_hip_batch*
         first = std::current_exception();
       }
     } else {
-      // several results: result i runs on workspace slot i % kWorkspaceSlots, the even ones on the
-      // caller's stream and the odd ones on the device's second stream, so two persistent
-      // launches are in flight at a time; each stream's runs follow one another
+      // several results: result i runs on workspace slot i % kWorkspaceSlots, all of them queued
+      // on the caller's stream, kWorkspaceSlots runs ahead of the host. They run one after
+      // another: two persistent launches resident at once on MI355X gave wrong trees (each
+      // lane keeps a little state in scratch memory, which the other launch's waves corrupted),
+      // so runs never overlap
       for (int64_t i = 0; i < count; ++i) grow_result(results[i], g->n, s);
       std::lock_guard<std::mutex> lk(pool().lock);
-      hipStream_t s2 = pool().stream2(g->device);
-      event_pair ev_join;
-      DYNG_HIP_TRY(hipEventRecord(ev_join.a, s));  // the apply and the growth are done
-      DYNG_HIP_TRY(hipStreamWaitEvent(s2, ev_join.a, 0));
       std::vector<engine_pending> pend(static_cast<size_t>(count));
       std::vector<char> ok(static_cast<size_t>(count), 0);
       auto finish = [&](int64_t i) {
@@ -650,7 +637,7 @@ extern "C" int dyng_hip_api_update_each(dyng_hip_graph* g, const dyng_hip_batch*
           dyng_hip_sssp_run run = make_run(g, r, delta[i], max_weight[i]);
           run.workspace = pool().get(g->device, static_cast<int>(i % kWorkspaceSlots));
           r->tree_token = 0;
-          pend[i] = engine_launch(&run, true, (i % 2 == 0) ? s : s2, /*concurrent=*/count > 1);
+          pend[i] = engine_launch(&run, true, s);
           ok[i] = 1;
         } catch (...) {
           r->poisoned = true;
@@ -658,9 +645,6 @@ extern "C" int dyng_hip_api_update_each(dyng_hip_graph* g, const dyng_hip_batch*
         }
       }
       for (int64_t i = std::max<int64_t>(0, count - kWorkspaceSlots); i < count; ++i) finish(i);
-      // the caller's stream does not run ahead of the second stream's runs
-      DYNG_HIP_TRY(hipEventRecord(ev_join.b, s2));
-      DYNG_HIP_TRY(hipStreamWaitEvent(s, ev_join.b, 0));
     }
     if (stats != nullptr) {
       DYNG_HIP_TRY(hipEventSynchronize(ev_apply.b));

@@ -202,9 +202,9 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
 
 /// One engine run in flight: engine_launch queues it on `s` (the checks, the launch and the
 /// counters' read-back, nothing waited for), engine_finish waits for it and reports (or throws).
-/// dyng_hip_sssp_update / _compute are the two back to back; runs on different workspaces and
-/// streams may be in flight together when launched as `concurrent` (each persistent launch then
-/// takes half of what fits, so two are resident at once).
+/// dyng_hip_sssp_update / _compute are the two back to back. Runs on one device are queued on one
+/// stream, one after another: two persistent launches resident together gave wrong trees on
+/// MI355X (the kernel keeps a little per-lane state in scratch memory), so runs never overlap.
 struct engine_pending {
   dyng_hip_workspace* ws = nullptr;
   hipStream_t s = nullptr;
@@ -219,12 +219,10 @@ struct engine_pending {
   bool held = false;  ///< set by engine_finish: the run did nothing (see engine_launch's `hold`)
   dyng_hip_counters counters{};
 };
-/// `concurrent`: the run is launched to overlap another persistent launch on the device (each
-/// then takes at most half of what fits). `hold`: queued behind apply_launch, the apply's
+/// `hold`: queued behind apply_launch, the apply's
 /// apply_hold flag: when the apply's merges were held back the run does nothing, and
 /// engine_finish sets `held` (the caller launches it again after apply_finish).
-engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s, bool concurrent = false,
-                             const int* hold = nullptr);
+engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s, const int* hold = nullptr);
 void engine_finish(engine_pending& pend, dyng_hip_counters* out);
 /// The scheduler's pick for a scanned batch on G_t: features, BF16 inference, argmin (one sync).
 int mlp_choose(dyng_hip_mlp* m, const dyng_hip_graph* g, const batch_scan& sc, const int32_t* parents, int64_t n_tree,

@@ -375,11 +375,11 @@ DYNG_DEV void relax_decide(const engine_params& p, bool active, int32_t u, u64 d
         // and offers nothing new, and being claimed stamps the vertex for the unpack.
         if (cand < old) {
           atomicAdd(improved, 1ULL);
-          if (nd < threshold) {
-            to_near = claim_first(p, w, generation, first);
-          } else {
-            to_far = atomicExch(&p.in_far[w], 1) == 0;  // stamped when it leaves the far pile
-          }
+          // both flags as values (a store through a reference chosen by the branch would keep
+          // them in scratch memory); a far vertex is stamped when it leaves the far pile
+          const bool near = nd < threshold;
+          to_near = near && claim_first(p, w, generation, first);
+          to_far = !near && atomicExch(&p.in_far[w], 1) == 0;
         }
       }
     }
@@ -824,10 +824,10 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       // parent cycle ----
       int count = n;
       const int* list = nullptr;
-      int* outs[2] = {p.near_a, p.near_b};
+      // (selects, not a two-entry array: a dynamically indexed private array lives in scratch)
       for (int r = 0; r < p.max_rounds; ++r) {
         jump_items += count;
-        int* out = outs[r & 1];
+        int* out = (r & 1) ? p.near_b : p.near_a;
         int* out_count = &c->ring_jump[(r + jump_slot) % 3];
         if (tid == 0) {
           st(&c->ring_jump[(r + jump_slot + 1) % 3], 0);
@@ -1005,7 +1005,9 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   u64 threshold = (smallest == packed_inf ? 0 : smallest) + p.delta;
   int* current = p.near_a;
   int* next = p.near_b;
-  int* far[2] = {p.far_a, p.far_b};
+  // the far piles: far_of(fb) is the current one (a select, not a two-entry array: a dynamically
+  // indexed private array lives in scratch)
+  auto far_of = [&](int b) { return b ? p.far_b : p.far_a; };
   int fb = 0;
   ++generation;
   {
@@ -1019,17 +1021,17 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         v = ld(&p.frontier[i]);
         const u64 word = ld(&p.packed[v]);
         const u64 d = word == packed_inf ? packed_inf : p.pk.distance(word);
-        if (d < threshold) {
-          to_near = claim_first(p, v, generation, first);
-        } else {
-          to_far = atomicExch(&p.in_far[v], 1) == 0;
-        }
+        // both flags as values (a store through a pointer chosen by the branch would keep them
+        // in scratch memory)
+        const bool near = d < threshold;
+        to_near = near && claim_first(p, v, generation, first);
+        to_far = !near && atomicExch(&p.in_far[v], 1) == 0;
       }
       stage_append(S, 0, to_near, v, current, near_count);
-      stage_append(S, 1, to_far, v, far[fb], &c->far_count[fb]);
+      stage_append(S, 1, to_far, v, far_of(fb), &c->far_count[fb]);
       stage_append(S, 2, first, v, S.list2, S.counter2);
     }
-    stage_flush(S, current, near_count, far[fb], &c->far_count[fb]);
+    stage_flush(S, current, near_count, far_of(fb), &c->far_count[fb]);
     DYNG_GRID_SYNC();
   }
   int near_count = uniform(ld(&c->ring_near[step % 3]));
@@ -1056,7 +1058,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         st(&c->ring_near[(step + 1) % 3], 0);
         st(&c->ring_chunk[(cstep + 1) % 3], 0);
       }
-      int* far_list = far[fb];
+      int* far_list = far_of(fb);
       int* far_counter = &c->far_count[fb];
       const int* cur = current;
       tile_rows(
@@ -1133,7 +1135,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     if (tid == 0) st(&c->minimum[ms ^ 1], packed_inf);
     local = packed_inf;
     for (int i = tid; i < far_count; i += threads) {
-      const u64 word = ld(&p.packed[ld(&far[fb][i])]);
+      const u64 word = ld(&p.packed[ld(&far_of(fb)[i])]);
       if (word != packed_inf) local = min(local, p.pk.distance(word));
     }
     local = wave_min(local);
@@ -1149,21 +1151,19 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       int32_t v = -1;
       bool to_near = false, keep = false, first = false;
       if (i < far_count) {
-        v = ld(&far[fb][i]);
+        v = ld(&far_of(fb)[i]);
         const u64 word = ld(&p.packed[v]);
         const u64 d = word == packed_inf ? packed_inf : p.pk.distance(word);
-        if (d < threshold) {
-          st(&p.in_far[v], 0);
-          to_near = claim_first(p, v, generation, first);
-        } else {
-          keep = true;
-        }
+        const bool near = d < threshold;
+        if (near) st(&p.in_far[v], 0);
+        to_near = near && claim_first(p, v, generation, first);
+        keep = !near;
       }
       stage_append(S, 0, to_near, v, current, split_count);
-      stage_append(S, 1, keep, v, far[fb ^ 1], &c->far_count[fb ^ 1]);
+      stage_append(S, 1, keep, v, far_of(fb ^ 1), &c->far_count[fb ^ 1]);
       stage_append(S, 2, first, v, S.list2, S.counter2);
     }
-    stage_flush(S, current, split_count, far[fb ^ 1], &c->far_count[fb ^ 1]);
+    stage_flush(S, current, split_count, far_of(fb ^ 1), &c->far_count[fb ^ 1]);
     DYNG_GRID_SYNC();
     near_count = uniform(ld(split_count));
     ++step;
@@ -1469,16 +1469,15 @@ void attach_scratch(engine_params& p, const engine_scratch& sc) {
 /// device fills it (two 1024-thread workgroups per CU: on its wide frontiers the waves of one hide
 /// the other's workgroup barriers and gathers, 1.6x on RMAT-22 against one per CU); a repair, whose
 /// frontiers are narrow and whose steps are chains of dependent accesses, runs one per CU (measured
-/// 4 % faster on RMAT-22 batches of 100,000 operations than two). Runs launched to overlap on one
-/// device (`concurrent`) take at most half of what fits each, so two of them are resident at once.
-/// The barrier's bounded wait turns a grid that is not resident after all into an error.
-int grid_blocks(const dyng_hip_tuning& t, int occupancy, int cu_count, bool concurrent, bool from_sources) {
+/// 4 % faster on RMAT-22 batches of 100,000 operations than two). Runs never overlap on a device
+/// (they are queued on one stream). The barrier's bounded wait turns a grid that is not resident
+/// after all into an error.
+int grid_blocks(const dyng_hip_tuning& t, int occupancy, int cu_count, bool from_sources) {
   const int resident = std::max(1, occupancy * cu_count);  // what the runtime says fits at once
   // an explicit count may go up to everything that fits (the caller runs nothing else beside it)
   if (t.grid_blocks > 0) return std::min(t.grid_blocks, resident);
-  const int cap = concurrent ? std::max(1, resident / 2) : resident;
-  const int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : (concurrent || !from_sources ? 1 : occupancy);
-  return std::max(1, std::min(per_cu * cu_count, cap));
+  const int per_cu = t.blocks_per_cu > 0 ? t.blocks_per_cu : (!from_sources ? 1 : occupancy);
+  return std::max(1, std::min(per_cu * cu_count, resident));
 }
 
 /// The errors a run's control block reports, as the reference's device errors.
@@ -1540,8 +1539,7 @@ void read_counters(const control* hc, bool update, int bits, int blocks, dyng_hi
 
 }  // namespace
 
-engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s, bool concurrent,
-                             const int* hold) {
+engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStream_t s, const int* hold) {
   if (run == nullptr || run->graph == nullptr || run->workspace == nullptr) {
     fail(DYNG_HIP_INVALID_ARGUMENT, "dyng: sssp run: graph and workspace are required");
   }
@@ -1583,7 +1581,7 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
   attach_scratch(p, engine_scratch{ws->packed.p, ws->stamp.p, ws->in_far.p, ws->flag.p, ws->ancestor.p, ws->cand.p,
                                    ws->frontier.p, ws->near_a.p, ws->near_b.p, ws->far_a.p, ws->far_b.p,
                                    ws->touched.p, ws->chunks.p, ws->chunk_cap, ws->ctl.p, ws->bar.p});
-  const int blocks = grid_blocks(run->tuning, ws->occupancy, ws->cu_count, concurrent, p.from_scratch != 0);
+  const int blocks = grid_blocks(run->tuning, ws->occupancy, ws->cu_count, p.from_scratch != 0);
   control* hc = ws->host_ctl.p;
   std::memset(hc, 0, sizeof(control));
   hc->min_frontier = packed_inf;
@@ -1859,7 +1857,7 @@ void run_engine_noalloc(const dyng_hip_sssp_run* run, hipStream_t s, dyng_hip_co
                                    reinterpret_cast<int4*>(base + L.chunks),
                                    L.chunk_cap, reinterpret_cast<control*>(base + L.ctl),
                                    reinterpret_cast<grid_sync_state*>(base + L.bar)});
-  const int blocks = grid_blocks(run->tuning, st.occupancy, st.cu_count, false, p.from_scratch != 0);
+  const int blocks = grid_blocks(run->tuning, st.occupancy, st.cu_count, p.from_scratch != 0);
   hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, s, p.bar, p.ctl);
   hipLaunchKernelGGL(sssp_fused_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
   DYNG_HIP_TRY(hipGetLastError());

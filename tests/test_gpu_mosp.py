@@ -148,10 +148,7 @@ def _random_graph(rng, n, m, K, wmax):
     (5, 1500, 9000, 3, [4, 1, 4], True),
     (6, 800, 3000, 5, [1, 2, 3, 4, 5], True),
     # more trees than the combined diff holds in registers (its generic path)
-    pytest.param(7, 400, 2400, 10, [1, 2, 3, 4, 5, 6, 7, 8, 9, 10], False, marks=pytest.mark.skip(
-        reason="left the GPU faulted once (ten trees repaired as overlapping pairs of persistent launches, with "
-               "an engine variant since taken out); no cause found by reading the kernels, so the case stays off "
-               "the GPU until a CPU build of the engine kernel shows one")),
+    (7, 400, 2400, 10, [1, 2, 3, 4, 5, 6, 7, 8, 9, 10], False),
 ])
 def test_random_updates_match_the_oracle(dg, seed, n, m, K, prefs, simple):
     """Random multigraphs (parallel edges, self-loops, unreachable parts) or simple graphs with
