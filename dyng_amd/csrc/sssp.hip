@@ -48,6 +48,10 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kWalkGlobal = 8;   ///< first rounds of the subtree walk: one level each
 constexpr int kWalkLocal = 16;   ///< later rounds: levels a workgroup walks on its own between grid barriers
 constexpr int kWalkRounds = kWalkGlobal + 8;  ///< grid rounds of the walk before pointer jumping takes over
+constexpr int kRelaxGlobal = 8;  ///< relax iterations of a repair before local rounds may start
+constexpr int kRelaxLocal = 16;  ///< local rounds of a workgroup in one relax iteration
+constexpr int kRelaxNarrow = 64; ///< a near frontier of at most this many vertices per workgroup is narrow
+constexpr int kLocalGenerations = 1600000000;  ///< no local relax iteration starts past this generation
 
 struct control {
   int ring_near[3];
@@ -386,12 +390,31 @@ DYNG_DEV void relax_decide(const engine_params& p, bool active, int32_t u, u64 d
   }
 }
 
+/// The subtree walk's and the local relax rounds' levels in LDS: two levels of up to kBlock
+/// vertices.
+struct walk_shared {
+  int count[2];
+  int item[2][kBlock];
+};
+
+/// Where a relax step sends the vertices it claimed for the near frontier. In a global iteration
+/// (`local` null), the next near list. In a workgroup's local rounds, its next local level first
+/// (`local`->item[q]), and the next near list only when that is full; a vertex enters that list
+/// once per iteration there (`listed[v]` takes the iteration's `tag` when it does), since the
+/// local rounds' claims use a generation per workgroup and round and may claim it again.
+struct near_sink {
+  walk_shared* local;
+  int q;
+  int* listed;
+  int tag;
+};
+
 /// kSlots relaxations per lane: the row entries first, then the words they point at, then the
 /// atomics and the list appends. Every lane of the wave calls it.
 DYNG_DEV void relax_slots(const engine_params& p, const bool (&act)[kSlots], const int32_t (&u)[kSlots],
                           const u64 (&du)[kSlots], const int (&slot)[kSlots], u64 threshold, int generation,
                           stage_shared& S, int* next, int* next_count, int* far, int* far_count,
-                          unsigned long long* improved) {
+                          unsigned long long* improved, const near_sink& sink) {
   int32_t w[kSlots], wt[kSlots];
   u64 seen[kSlots];
   bool live[kSlots];
@@ -410,7 +433,16 @@ DYNG_DEV void relax_slots(const engine_params& p, const bool (&act)[kSlots], con
     bool to_near, to_far, first;
     relax_decide(p, live[j], u[j], du[j], w[j], wt[j], seen[j], threshold, generation, improved, to_near, to_far,
                  first);
-    stage_append(S, 0, to_near, w[j], next, next_count);
+    bool to_list = to_near;
+    if (to_near && sink.local != nullptr) {
+      const int at = atomicAdd(&sink.local->count[sink.q], 1);
+      if (at < kBlock) {
+        sink.local->item[sink.q][at] = w[j];
+        to_list = false;
+      }
+    }
+    if (to_list && sink.listed != nullptr) to_list = atomicExch(&sink.listed[w[j]], sink.tag) != sink.tag;
+    stage_append(S, 0, to_list, w[j], next, next_count);
     stage_append(S, 1, to_far, w[j], far, far_count);
     stage_append(S, 2, first, w[j], S.list2, S.counter2);
   }
@@ -545,12 +577,6 @@ __global__ void k_reset_barrier(grid_sync_state* bar, control* ctl) {
   }
 }
 
-/// The subtree walk's local levels (phase B): two levels of up to kBlock vertices.
-struct walk_shared {
-  int count[2];
-  int item[2][kBlock];
-};
-
 __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) {
   // queued right behind an apply whose merges were held back: nothing to do (uniform)
   if (p.hold != nullptr && *p.hold != 0) {
@@ -558,11 +584,12 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     return;
   }
   __shared__ tile_shared L;
-  // work counters of the workgroup, in LDS: relax edges, improving atomics, pull edges, affected
-  __shared__ unsigned long long counts[4];
+  // work counters of the workgroup, in LDS: relax edges, improving atomics, pull edges, affected,
+  // expansions of local relax rounds
+  __shared__ unsigned long long counts[5];
   __shared__ stage_shared S;
   __shared__ walk_shared W;
-  if (threadIdx.x < 4) counts[threadIdx.x] = 0;
+  if (threadIdx.x < 5) counts[threadIdx.x] = 0;
   if (threadIdx.x < 3) S.count[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     S.list2 = p.touched;
@@ -573,6 +600,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   unsigned long long* const relax_improved = &counts[1];
   unsigned long long* const pull_edges = &counts[2];
   unsigned long long* const affected = &counts[3];
+  unsigned long long* const local_pushes = &counts[4];
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int threads = gridDim.x * blockDim.x;
   const int n = p.n;
@@ -1048,10 +1076,28 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     }
     if (near_count > 0) {
       // -- one relax iteration: rows up to `light` edges edge-balanced per workgroup, longer
-      // rows in chunks after a barrier --
-      ++generation;
+      // rows in chunks after a barrier. After the first kRelaxGlobal iterations of a repair, an
+      // iteration over a narrow near frontier lets each workgroup relax its share for up to
+      // kRelaxLocal rounds on its own: the vertices it claims go to its next local level (LDS)
+      // rather than to the next near list, each round claiming with a generation of its own
+      // (generation0 + round * workgroups + workgroup), so a vertex improved after a workgroup
+      // expanded it is claimed again and expanded again (every improvement is expanded, as in a
+      // global iteration; the fixed point, and so the tree, is the same). A level past LDS and
+      // the last level go to the next near list, once per vertex (the `ancestor` array, free in
+      // this phase, tags them with the iteration). The deep, narrow repairs of a high-diameter
+      // graph take hundreds of iterations; each global one costs a grid barrier and a list round
+      // trip --
+      // (a local iteration takes workgroups * kRelaxLocal + 1 generations: past kLocalGenerations
+      // the iterations stay global, one generation each, so a run of millions of levels cannot
+      // carry the stamps past INT32_MAX)
+      const bool local = !p.from_scratch && iterations >= kRelaxGlobal &&
+                         near_count <= static_cast<int>(gridDim.x) * kRelaxNarrow && generation < kLocalGenerations;
       ++iterations;
       pushes += near_count;
+      const int gen0 = generation + 1;
+      // a global iteration claims with one generation; a local one reserves one per workgroup
+      // and round, then one for its chunk pass
+      generation += local ? static_cast<int>(gridDim.x) * kRelaxLocal + 1 : 1;
       int* next_count = &c->ring_near[step % 3];
       int* chunk_count = &c->ring_chunk[cstep % 3];
       if (tid == 0) {
@@ -1061,37 +1107,104 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       int* far_list = far_of(fb);
       int* far_counter = &c->far_count[fb];
       const int* cur = current;
-      tile_rows(
-          L, near_count,
-          [&](int idx, int32_t& v, int32_t& off, int32_t& len, u64& word) {
-            v = ld(&cur[idx]);
-            const int2 mt = p.out_meta[v];  // issued together with the word's load
-            word = ld(&p.packed[v]);
-            if (word == packed_inf) return;
-            if (mt.y <= p.light) {
-              off = mt.x;
-              len = mt.y;
-              atomicAdd(relax_edges, static_cast<unsigned long long>(mt.y));
-            } else {
-              emit_chunks(p, chunk_count, v, mt.x, mt.y);
-            }
-          },
-          [&](const bool (&act)[kSlots], const int (&item)[kSlots], const int (&slot)[kSlots]) {
-            int32_t u[kSlots];
-            u64 du[kSlots];
+      const int tag = local ? -2 - generation : 0;  // a local iteration's list tag: below every vertex id and -1
+      auto load_row = [&](int32_t v, int32_t& off, int32_t& len, u64& word) {
+        const int2 mt = p.out_meta[v];  // issued together with the word's load
+        word = ld(&p.packed[v]);
+        if (word == packed_inf) return;
+        if (mt.y <= p.light) {
+          off = mt.x;
+          len = mt.y;
+          atomicAdd(relax_edges, static_cast<unsigned long long>(mt.y));
+        } else {
+          emit_chunks(p, chunk_count, v, mt.x, mt.y);
+        }
+      };
+      if (!local) {
+        const near_sink sink{nullptr, 0, nullptr, 0};
+        tile_rows(
+            L, near_count,
+            [&](int idx, int32_t& v, int32_t& off, int32_t& len, u64& word) {
+              v = ld(&cur[idx]);
+              load_row(v, off, len, word);
+            },
+            [&](const bool (&act)[kSlots], const int (&item)[kSlots], const int (&slot)[kSlots]) {
+              int32_t u[kSlots];
+              u64 du[kSlots];
 #pragma unroll
-            for (int j = 0; j < kSlots; ++j) {
-              u[j] = act[j] ? L.vert[item[j]] : 0;
-              du[j] = act[j] ? p.pk.distance(L.word[item[j]]) : 0;
+              for (int j = 0; j < kSlots; ++j) {
+                u[j] = act[j] ? L.vert[item[j]] : 0;
+                du[j] = act[j] ? p.pk.distance(L.word[item[j]]) : 0;
+              }
+              relax_slots(p, act, u, du, slot, threshold, generation, S, next, next_count, far_list, far_counter,
+                          relax_improved, sink);
+            },
+            [&](bool, int) {}, [&] { stage_flush(S, next, next_count, far_list, far_counter); });
+      } else {
+        const int G = static_cast<int>(gridDim.x);
+        const int ts = min(kBlock, max(1, (near_count + G - 1) / G));
+        for (long long t0 = static_cast<long long>(blockIdx.x) * ts; t0 < near_count;
+             t0 += static_cast<long long>(G) * ts) {
+          const int cnt = static_cast<int>(min(static_cast<long long>(ts), near_count - t0));
+          if (static_cast<int>(threadIdx.x) < cnt) W.item[0][threadIdx.x] = ld(&cur[t0 + threadIdx.x]);
+          if (threadIdx.x == 0) {
+            W.count[0] = cnt;
+            W.count[1] = 0;
+          }
+          __syncthreads();
+          int q = 0;
+          for (int round = 0; round < kRelaxLocal; ++round) {
+            const int items = W.count[q];  // uniform: LDS after a workgroup barrier
+            if (items == 0) break;
+            if (round > 0 && threadIdx.x == 0) *local_pushes += static_cast<unsigned long long>(items);
+            const int gen = gen0 + round * G + static_cast<int>(blockIdx.x);
+            const near_sink sink{&W, q ^ 1, p.ancestor, tag};
+            expand_tile(
+                L, items,
+                [&](int i, int32_t& v, int32_t& off, int32_t& len, u64& word) {
+                  v = W.item[q][i];
+                  load_row(v, off, len, word);
+                },
+                [&](const bool (&act)[kSlots], const int (&item)[kSlots], const int (&slot)[kSlots]) {
+                  int32_t u[kSlots];
+                  u64 du[kSlots];
+#pragma unroll
+                  for (int j = 0; j < kSlots; ++j) {
+                    u[j] = act[j] ? L.vert[item[j]] : 0;
+                    du[j] = act[j] ? p.pk.distance(L.word[item[j]]) : 0;
+                  }
+                  relax_slots(p, act, u, du, slot, threshold, gen, S, next, next_count, far_list, far_counter,
+                              relax_improved, sink);
+                },
+                [&](bool, int) {});
+            if (threadIdx.x == 0) {
+              W.count[q] = 0;
+              W.count[q ^ 1] = min(W.count[q ^ 1], kBlock);
             }
-            relax_slots(p, act, u, du, slot, threshold, generation, S, next, next_count, far_list, far_counter,
-                        relax_improved);
-          },
-          [&](bool, int) {}, [&] { stage_flush(S, next, next_count, far_list, far_counter); });
+            if (S.count[0] > kStage / 2 || S.count[1] > kStage / 2 || S.count[2] > kStage / 2) {
+              stage_flush(S, next, next_count, far_list, far_counter);  // uniform: LDS after a barrier
+            }
+            __syncthreads();
+            q ^= 1;
+          }
+          // the level the last round left: the next near list, once per vertex
+          const int left = W.count[q];
+          for (int base = static_cast<int>(threadIdx.x) - lane(); base < left; base += kBlock) {
+            const int i = base + lane();
+            const int32_t v = i < left ? W.item[q][i] : -1;
+            const bool add = i < left && atomicExch(&p.ancestor[v], tag) != tag;
+            stage_append(S, 0, add, v, next, next_count);
+          }
+          stage_flush(S, next, next_count, far_list, far_counter);
+        }
+      }
       DYNG_GRID_SYNC();
       const int nchunks = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap)));
       ++cstep;
       if (nchunks > 0) {
+        // the chunk pass claims with the iteration's last generation; in a local iteration its
+        // near vertices enter the next list once, as above
+        const near_sink sink{nullptr, 0, local ? p.ancestor : nullptr, tag};
         const int wave = tid / 64;
         const int waves = threads / 64;
         for (int q = wave; q < nchunks; q += waves) {
@@ -1114,7 +1227,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
               slot[j] = act[j] ? e : 0;
             }
             relax_slots(p, act, uu, dd, slot, threshold, generation, S, next, next_count, far_list, far_counter,
-                        relax_improved);
+                        relax_improved, sink);
           }
         }
         stage_flush(S, next, next_count, far_list, far_counter);
@@ -1240,12 +1353,13 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
     if (*relax_edges) atomicAdd(&c->relax_edges, *relax_edges);
     if (*relax_improved) atomicAdd(&c->relax_improved, *relax_improved);
     if (*pull_edges) atomicAdd(&c->pull_edges, *pull_edges);
+    if (*local_pushes) atomicAdd(reinterpret_cast<unsigned long long*>(&c->pushes), *local_pushes);
   }
   if (tid == 0) {
     c->t_phase[7] = __builtin_amdgcn_s_memrealtime();
     c->iterations = iterations;
     c->epochs = epochs;
-    c->pushes = pushes;
+    atomicAdd(reinterpret_cast<unsigned long long*>(&c->pushes), static_cast<unsigned long long>(pushes));
     c->generation = generation;
   }
 }
@@ -1322,6 +1436,7 @@ void ws_reserve(dyng_hip_workspace* ws, int64_t n, int64_t slots, hipStream_t s)
     DYNG_HIP_TRY(hipMemsetAsync(ws->stamp.p, 0, cap * sizeof(int), s));
     DYNG_HIP_TRY(hipMemsetAsync(ws->in_far.p, 0, cap * sizeof(int), s));
     DYNG_HIP_TRY(hipMemsetAsync(ws->flag.p, 0, cap * sizeof(int), s));
+    DYNG_HIP_TRY(hipMemsetAsync(ws->ancestor.p, 0, cap * sizeof(int), s));
     ws->capacity = cap;
     ws->generation = 0;
     ws->words_token = 0;
@@ -1565,7 +1680,9 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
   p.hold = hold;
   ws_reserve(ws, n, std::max(g->out.slot_cap, g->in.slot_cap), s);
   if (ws->generation > INT32_MAX / 2) {
+    // stamps and the relax phase's list tags (-2 - generation, in `ancestor`) restart
     DYNG_HIP_TRY(hipMemsetAsync(ws->stamp.p, 0, ws->capacity * sizeof(int), s));
+    DYNG_HIP_TRY(hipMemsetAsync(ws->ancestor.p, 0, ws->capacity * sizeof(int), s));
     ws->generation = 0;
   }
   p.generation = ++ws->generation;
@@ -1666,6 +1783,7 @@ void engine_finish(engine_pending& pend, dyng_hip_counters* out) {
     DYNG_HIP_TRY(hipMemsetAsync(ws->stamp.p, 0, ws->capacity * sizeof(int), s));
     DYNG_HIP_TRY(hipMemsetAsync(ws->in_far.p, 0, ws->capacity * sizeof(int), s));
     DYNG_HIP_TRY(hipMemsetAsync(ws->flag.p, 0, ws->capacity * sizeof(int), s));
+    DYNG_HIP_TRY(hipMemsetAsync(ws->ancestor.p, 0, ws->capacity * sizeof(int), s));
     DYNG_HIP_TRY(hipStreamSynchronize(s));
     ws->generation = 0;
   }
@@ -1836,13 +1954,14 @@ void run_engine_noalloc(const dyng_hip_sssp_run* run, hipStream_t s, dyng_hip_co
   noalloc_device& st = noalloc_for(g->device, s);
   unsigned char* base = st.ptr;
   // Stamps, far marks and flags must be zero (or of an older generation) when a run starts, and a
-  // run leaves them so. Their place in the array moves with n: a call on another vertex count
+  // run leaves them so; `ancestor` must hold no list tag (-2 - generation) of a later generation. Their place in the array moves with n: a call on another vertex count
   // (or after a run that stopped early) finds the last call's other arrays there, so it clears
   // them first.
   if (n != st.last_n || st.generation > INT32_MAX / 2) {
     clear_scratch(base + L.stamp, 4 * n, s);
     clear_scratch(base + L.in_far, 4 * n, s);
     clear_scratch(base + L.flag, 4 * n, s);
+    clear_scratch(base + L.ancestor, 4 * n, s);
     st.generation = 0;
     st.last_n = n;
   }
