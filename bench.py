@@ -55,6 +55,10 @@ WORKLOADS = {
     # batch applied once, three trees repaired, the kept combined graph brought up to date and its
     # tree repaired, the path costs recomputed)
     "mosp_rmat22_k3": ("mosp", 22, 67_108_864, 255, 5, 20, (40_000, 40_000, 20_000)),
+    # hyper_sssp: 200,000 hyperedges of 2-6 members drawn uniformly over 100,000 vertices, weights
+    # U[1, 100], the source the vertex in most hyperedges; each step is one hyper_sssp.update with
+    # 500 hyperedge deletions, 500 insertions, 500 incidence removals and 500 additions
+    "hyper_uniform": ("hyper", 100_000, 200_000, 100, None, 20, (500, 500, 500)),
 }
 MOSP_OBJECTIVES = 3
 MOSP_PREFERENCES = [4, 1, 4]
@@ -638,6 +642,77 @@ def mosp_bench(args) -> int:
     return 0 if (exact is None or exact["bit_exact"]) else 3
 
 
+def hyper_bench(args) -> int:
+    """hyper_sssp.update on a uniform random hypergraph: the whole update (host bookkeeping of the
+    hypergraph and the line graph's edge batch, then the device apply and repair) and its device
+    part apart."""
+    import dyng_amd
+    from dyng_amd import hyper_sssp as hs
+
+    K, W = args.steps, args.warmup
+    _, V, H, wmax, _, _, (nd, ni, ne) = WORKLOADS[args.workload]
+    from dyng_amd import generators as G
+
+    rng = np.random.default_rng(G.GRAPH_SEED ^ 7)
+    sizes = rng.integers(2, 7, H)
+    members = rng.integers(0, V, int(sizes.sum()))
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    lists = [members[off[i]:off[i + 1]].tolist() for i in range(H)]
+    weights = rng.integers(1, wmax + 1, H).tolist()
+    t0 = time.perf_counter()
+    h = dyng_amd.Hypergraph.from_lists(lists, weights, num_vertices=V)
+    source = int(np.argmax(np.bincount(members, minlength=V)))
+    r = hs.compute(h, source)
+    dyng_amd.synchronize()
+    lg_edges = r._lg.num_edges
+    log(f"[bench] hypergraph V={V} H={H}, line graph {lg_edges} edges, compute in {time.perf_counter() - t0:.1f}s")
+    wall, dev_ms, host_ms, lg_changes = [], [], [], []
+    for b in range(W + K):
+        bound = h.hyperedge_id_bound
+        dels = rng.integers(0, bound, nd).tolist()
+        ins_sizes = rng.integers(2, 7, ni)
+        ins = [rng.integers(0, V, int(k)).tolist() for k in ins_sizes]
+        rem = (rng.integers(0, bound, ne).tolist(), rng.integers(0, V, ne).tolist())
+        add = (rng.integers(0, bound, ne).tolist(), rng.integers(0, V, ne).tolist())
+        batch = dyng_amd.HyperedgeBatch(dels, ins, rng.integers(1, wmax + 1, ni).tolist(),
+                                        delete_incidences=rem, insert_incidences=add)
+        dyng_amd.synchronize()
+        t1 = time.perf_counter()
+        st = hs.update(h, batch, r)
+        dt = (time.perf_counter() - t1) * 1e3
+        if b >= W:
+            d = st.device
+            wall.append(dt)
+            dev_ms.append(d.apply_ms + d.engine_ms)
+            host_ms.append(dt - d.apply_ms - d.engine_ms)
+            lg_changes.append(d.batch.inserted_edges + d.batch.deleted_edges)
+    p50 = statistics.median(wall)
+    ops = nd + ni + 2 * ne
+    value = ops / (p50 / 1e3)
+    again = hs.compute(h, source)
+    exact = bool(np.array_equal(again.hyperedge_distances, r.hyperedge_distances)
+                 and np.array_equal(again.hyperedge_parents, r.hyperedge_parents))
+    log(f"[bench] gpu: p50 {p50:.2f} ms/update (device {statistics.median(dev_ms):.3f} ms), {value:.3e} ops/s, "
+        f"updated == compute: {exact}")
+    out = {
+        "metric": "hyperedge ops/s & p50 ms/update, hyper_sssp on a uniform hypergraph", "value": round(value, 1),
+        "unit": "ops/s", "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(sum(wall) / K, 4),
+        "p50_ms": round(p50, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "vertices": V, "hyperedges": H, "line_graph_edges": int(lg_edges),
+                   "batch_ops": ops, "source": source,
+                   "device_ms_p50": round(statistics.median(dev_ms), 4),
+                   "host_ms_p50": round(statistics.median(host_ms), 4),
+                   "line_graph_changes_p50": int(statistics.median(lg_changes)),
+                   "device": "sssp.update on the line graph: the device apply of its edge batch and the "
+                             "persistent repair",
+                   "host": "the hypergraph's bookkeeping and the line graph's edge batch (Python)"},
+        "parity": {"bit_exact": exact, "against": "hyper_sssp.compute on the final hypergraph"},
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if exact else 3
+
+
 def main() -> int:
     # the CPU baseline's OpenMP threads, pinned one per core (read when the OpenMP runtime loads,
     # so before the generators' or the oracle's library is opened)
@@ -677,6 +752,8 @@ def main() -> int:
         return batched_bench(args)
     if WORKLOADS[args.workload][0] == "mosp":
         return mosp_bench(args)
+    if WORKLOADS[args.workload][0] == "hyper":
+        return hyper_bench(args)
     K, W = args.steps, args.warmup
     n, src, dst, w, batch, source, ops = make_inputs(args.workload, K + W)
 

@@ -296,30 +296,28 @@ def _edge_batch(hg: Hypergraph, applied: Any, source: int) -> tuple[EdgeBatch, i
         after = hg.neighbours(t)
         w_pre = applied.weights_pre.get(t, 0)
         w_post = post_w[t] if t < len(post_w) else 0
-        for g in before | after:
-            was, now = g in before, g in after
-            pair = (min(g, t), max(g, t))
-            if was and not now:
-                lost.add(pair)
-            elif now and not was:
-                gained.add(pair)
-            # the in-edge (g, t): weight w(t)
-            if was and not now:
-                del_s.append(g + 1)
-                del_d.append(t + 1)
-            elif now and (not was or w_pre != w_post):
+        # only the pairs that changed, and the in-edges of a reused id whose weight changed
+        for g in before - after:
+            lost.add((g, t) if g < t else (t, g))
+            del_s.append(g + 1)
+            del_d.append(t + 1)
+            if g not in touched:  # a touched g handles the edge (t, g) as its in-edge
+                del_s.append(t + 1)
+                del_d.append(g + 1)
+        for g in after - before:
+            gained.add((g, t) if g < t else (t, g))
+            ins_s.append(g + 1)
+            ins_d.append(t + 1)
+            ins_w.append(w_post)
+            if g not in touched:
+                ins_s.append(t + 1)
+                ins_d.append(g + 1)
+                ins_w.append(post_w[g])
+        if w_pre != w_post:
+            for g in after & before:
                 ins_s.append(g + 1)
                 ins_d.append(t + 1)
                 ins_w.append(w_post)
-            # the out-edge (t, g) of an untouched g (a touched g handles it as its in-edge)
-            if g not in touched:
-                if was and not now:
-                    del_s.append(t + 1)
-                    del_d.append(g + 1)
-                elif now and not was:
-                    ins_s.append(t + 1)
-                    ins_d.append(g + 1)
-                    ins_w.append(post_w[g])
         # the virtual source's edge (0, t)
         was, now = source in applied.pre[t], source in applied.post[t]
         if was and not now:

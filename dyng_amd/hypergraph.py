@@ -124,11 +124,13 @@ class _IdAllocator:
         self.tail_capacity = [0] + [_padded4(c) - 1 for c in counts]
         self.occupancy = [0] + list(counts)
         self.tails: dict[int, list[int]] = {}
+        self.dead_keys: set[int] = set()
         self.max_key = n
         self.live = n
         self.dead = 0
         self.bump = size
         self.compactions = 0
+        self._log: list | None = None  # (key, state, length, count, tail_capacity, occupancy, tail) to undo
 
     def _at(self, key: int) -> None:
         while len(self.state) <= key:
@@ -138,10 +140,41 @@ class _IdAllocator:
             self.tail_capacity.append(0)
             self.occupancy.append(0)
 
+    def _touch(self, key: int) -> None:
+        """Record a key's slot before it changes (undone by rollback())."""
+        if self._log is not None and key < len(self.state):
+            self._log.append((key, self.state[key], self.length[key], self.count[key], self.tail_capacity[key],
+                              self.occupancy[key], list(self.tails[key]) if key in self.tails else None))
+
+    def begin(self) -> None:
+        """Start recording: rollback() restores the state of this moment (a failed apply)."""
+        self._log = []
+        self._saved = (len(self.state), set(self.dead_keys), self.max_key, self.live, self.dead, self.bump,
+                       self.capacity, self.compactions)
+
+    def commit(self) -> None:
+        self._log = None
+
+    def rollback(self) -> None:
+        log, self._log = self._log, None
+        for key, st, ln, ct, tc, oc, tail in reversed(log or []):
+            self.state[key], self.length[key], self.count[key] = st, ln, ct
+            self.tail_capacity[key], self.occupancy[key] = tc, oc
+            if tail is None:
+                self.tails.pop(key, None)
+            else:
+                self.tails[key] = tail
+        size, dead_keys, self.max_key, self.live, self.dead, self.bump, self.capacity, self.compactions = self._saved
+        self.dead_keys = dead_keys
+        for lst in (self.state, self.length, self.count, self.tail_capacity, self.occupancy):
+            del lst[size:]
+
     def _clear(self, key: int) -> None:
+        self._touch(key)
         self.state[key] = _ABSENT
         self.length[key] = self.count[key] = self.tail_capacity[key] = self.occupancy[key] = 0
         self.tails.pop(key, None)
+        self.dead_keys.discard(key)
 
     def _grown(self, needed: int) -> int:
         """store_policy::grown_capacity (store_policy.cpp:100-116)."""
@@ -181,6 +214,7 @@ class _IdAllocator:
             raise CapacityError(f"dyng: CBST h2v: the compacted rows need {total} payload slots")
         self._ensure(total)
         for key in range(1, self.max_key + 1):
+            self._touch(key)
             if self.state[key] == _LIVE:
                 self.length[key] = _padded4(self.count[key])
                 self.tail_capacity[key] = self.length[key] - 1
@@ -196,7 +230,9 @@ class _IdAllocator:
         """Mark live keys as deleted slots (id_allocator.cpp:265-282)."""
         for key in keys:
             if 0 < key < len(self.state) and self.state[key] == _LIVE:
+                self._touch(key)
                 self.state[key] = _DEAD
+                self.dead_keys.add(key)
                 self.live -= 1
                 self.dead += 1
                 self.tails.pop(key, None)
@@ -212,7 +248,7 @@ class _IdAllocator:
         matched = [False] * num
         reuse_k = min(num, self.dead)
         if reuse_k > 0:
-            dead_keys = [k for k in range(1, self.max_key + 1) if self.state[k] == _DEAD]
+            dead_keys = sorted(self.dead_keys)
             cap = [self.length[k] - 1 for k in dead_keys]
             slot_order = sorted(range(len(dead_keys)), key=lambda j: cap[j])  # stable: key order on ties
             sorted_cap = [cap[j] for j in slot_order]
@@ -225,7 +261,9 @@ class _IdAllocator:
                 if assigned >= len(dead_keys):
                     break
                 key = dead_keys[slot_order[assigned]]
+                self._touch(key)
                 self.state[key] = _LIVE
+                self.dead_keys.discard(key)
                 self.count[key] = counts[item]
                 self.tail_capacity[key] = self.length[key] - 1
                 self.occupancy[key] = counts[item]
@@ -242,9 +280,8 @@ class _IdAllocator:
             self.bump += slots
             # the rebuild: deleted slots not reused are dropped; the surplus rows get the keys after
             # the last live key, in item order
-            for key in range(1, self.max_key + 1):
-                if self.state[key] == _DEAD:
-                    self._clear(key)
+            for key in list(self.dead_keys):
+                self._clear(key)
             self.dead = 0
             last_live = self.max_key
             while last_live > 0 and self.state[last_live] != _LIVE:
@@ -253,6 +290,7 @@ class _IdAllocator:
             for i in surplus:
                 a = _aligned(counts[i], self.alignment)
                 self._at(nxt)
+                self._touch(nxt)
                 self.state[nxt] = _LIVE
                 self.length[nxt] = a + 1
                 self.count[nxt] = counts[i]
@@ -280,6 +318,7 @@ class _IdAllocator:
         taken, segment = [], []
         total = 0
         for key, values in groups:
+            self._touch(key)
             occ = min(max(self.occupancy[key], 0), self.tail_capacity[key])
             room = self.tail_capacity[key] - occ
             k = len(values)
@@ -320,6 +359,7 @@ class _IdAllocator:
         for key, values in groups:
             if not (0 < key < len(self.state)) or self.state[key] != _LIVE:
                 continue
+            self._touch(key)
             removed = set(values)
             self.count[key] -= len(removed)
             if key not in self.tails:
@@ -333,6 +373,8 @@ class _IdAllocator:
         out = object.__new__(_IdAllocator)
         out.__dict__.update({k: (list(v) if isinstance(v, list) else v) for k, v in self.__dict__.items()})
         out.tails = {k: list(v) for k, v in self.tails.items()}
+        out.dead_keys = set(self.dead_keys)
+        out._log = None
         return out
 
 
@@ -569,6 +611,7 @@ class Hypergraph:
         for h, r in enumerate(rows):
             for v in r:
                 self._v2h[v].add(h)
+        self._live = sum(1 for r in rows if r)
         return self
 
     # ---- views ----
@@ -579,7 +622,7 @@ class Hypergraph:
 
     @property
     def num_hyperedges(self) -> int:
-        return sum(1 for r in self._rows if r)
+        return self._live
 
     @property
     def hyperedge_id_bound(self) -> int:
@@ -643,6 +686,7 @@ class Hypergraph:
         out._props = self._props
         out._alloc = self._alloc.copy()
         out._v2h = [set(s) for s in self._v2h]
+        out._live = self._live
         return out
 
     def __copy__(self) -> "Hypergraph":
@@ -746,18 +790,24 @@ class Hypergraph:
         applied_edits = removed + added
         skipped_edits = nrem + len(add_h) - applied_edits
         # the store phase (hypergraph.cpp:945-1075): the allocator in the order of id_reuse
-        alloc = self._alloc.copy()  # committed only when every step succeeded
+        alloc = self._alloc
         counts = [len(r) for r in rows_in]
-        if self._props.reuse == "insert_first":
-            keys = alloc.insert(counts)
-            alloc.erase([h + 1 for h in deleted])
-            alloc.remove_values(removals)
-            alloc.append(additions)
-        else:
-            alloc.erase([h + 1 for h in deleted])
-            alloc.remove_values(removals)
-            alloc.append(additions)
-            keys = alloc.insert(counts)
+        alloc.begin()  # a step that fails (CapacityError) leaves the allocator as it was
+        try:
+            if self._props.reuse == "insert_first":
+                keys = alloc.insert(counts)
+                alloc.erase([h + 1 for h in deleted])
+                alloc.remove_values(removals)
+                alloc.append(additions)
+            else:
+                alloc.erase([h + 1 for h in deleted])
+                alloc.remove_values(removals)
+                alloc.append(additions)
+                keys = alloc.insert(counts)
+        except BaseException:
+            alloc.rollback()
+            raise
+        alloc.commit()
         ids = [k - 1 for k in keys]
         # commit the rows
         pre: dict[int, list[int]] = {}
@@ -786,7 +836,6 @@ class Hypergraph:
                 while len(self._w) <= h:
                     self._w.append(0)
                 self._w[h] = int(batch._weights[k])
-        self._alloc = alloc
         for h in pre:
             post[h] = row_of(h)
         self._version += 1
@@ -800,6 +849,7 @@ class Hypergraph:
     def _set_row(self, h: int, row: list[int]) -> None:
         while len(self._rows) <= h:
             self._rows.append([])
+        self._live += (1 if row else 0) - (1 if self._rows[h] else 0)
         for v in self._rows[h]:
             self._v2h[v].discard(h)
         self._rows[h] = list(row)
