@@ -318,3 +318,41 @@ def test_update_many_results_reuse_workspaces(dg):
             d, p = oracle.dijkstra(og, s, k)
             assert np.array_equal(t.distances, d) and np.array_equal(t.parents, p), (b, s, k)
             assert t.graph_version == g.version and st.converged
+
+
+def test_path_costs_down_a_deep_chain(dg):
+    """The path doubling of the costs on a tree deeper than its grid covers in one stride: a chain
+    of 600,000 vertices (the costs' grid is capped at 2048 x 256 = 524,288 threads, so threads take
+    two vertices each and the doubling runs 20 rounds through the settled states), with a few
+    shortcuts that the trees do not take, then a batch that cuts the chain and adds a bypass. The
+    combined tree and every objective's cost equal the oracle's after each step."""
+    _say("[mosp deep chain] n=600000 K=2")
+    n, K = 600_000, 2
+    rng = np.random.default_rng(11)
+    src = np.arange(n - 1, dtype=np.int64)
+    dst = src + 1
+    w = rng.integers(1, 10, (n - 1, K))
+    # shortcuts far heavier than the chain below them: never on a shortest path
+    sc_u = rng.integers(0, n - 1000, 50)
+    sc_v = sc_u + rng.integers(2, 1000, 50)
+    src = np.concatenate([src, sc_u])
+    dst = np.concatenate([dst, sc_v])
+    w = np.concatenate([w, np.full((50, K), 20_000, np.int64)])
+    props = dg.GraphProperties.mosp_compatible()
+    g = dg.Graph.from_edges(src, dst, w, num_vertices=n, properties=props)
+    og = oracle.build_from_edges(n, src, dst, w, oracle.Props.mosp_compatible())
+    r = dg.mosp.compute(g, 0, preferences=[1, 2])
+
+    def check(og):
+        trees = [oracle.dijkstra(og, 0, k)[1] for k in range(K)]
+        d, p, _ = omosp.solve(trees, 0, [1, 2])
+        assert np.array_equal(r.combined_distances, d) and np.array_equal(r.combined_parents, p)
+        assert np.array_equal(r.path_costs, omosp.path_costs_fast(og, p, 0, K))
+
+    check(og)
+    # cut the chain at 300,000 and bypass it from 299,990 to 300,005
+    ins_s, ins_d, ins_w = [299_990], [300_005], [[7, 7]]
+    del_s, del_d = [300_000], [300_001]
+    dg.mosp.update(g, dg.EdgeBatch(insert=(ins_s, ins_d, ins_w), delete=(del_s, del_d)), r)
+    og = oracle.apply_batch(og, ins_s, ins_d, ins_w, del_s, del_d, oracle.Props.mosp_compatible()).g
+    check(og)
