@@ -17,6 +17,7 @@ NOT_SUPPORTED = 3
 OUT_OF_MEMORY = 4
 DEVICE_ERROR = 5
 RUNTIME_ERROR = 6
+CAPACITY = 7
 
 ENGINES = {"automatic": 0, "fused": 1, "operators": 2}
 ENGINE_NAMES = {v: k for k, v in ENGINES.items()}
@@ -117,6 +118,9 @@ EXPORTS = (
     "dyng_hip_mosp_combined", "dyng_hip_api_update_each", "dyng_hip_mosp_create", "dyng_hip_mosp_update",
     "dyng_hip_mosp_arrays", "dyng_hip_mosp_path_costs", "dyng_hip_mosp_destroy",
     "dyng_hip_sssp_update_noalloc", "dyng_hip_result_counters",
+    "dyng_hip_hypergraph_create", "dyng_hip_hypergraph_destroy", "dyng_hip_hypergraph_clone",
+    "dyng_hip_hypergraph_info", "dyng_hip_hypergraph_rows", "dyng_hip_hypergraph_apply",
+    "dyng_hip_hyper_line_graph", "dyng_hip_hyper_line_graph_delta",
 )
 
 NOALLOC_SCRATCH_BYTES = 64 << 20  # DYNG_HIP_NOALLOC_SCRATCH_BYTES
@@ -133,6 +137,37 @@ class MospStats(C.Structure):
     _fields_ = [("combined_edges", C.c_int64), ("weight_sum", C.c_int64), ("preference_scale", C.c_int64),
                 ("base", C.c_int64), ("affected", C.c_int64), ("deletions", C.c_int64), ("insertions", C.c_int64),
                 ("solve", Counters)]
+
+
+class HyperProps(C.Structure):
+    _fields_ = [("id_reuse", C.c_int32), ("initial_headroom", C.c_double), ("initial_slack", C.c_int64),
+                ("min_initial_payload", C.c_int64), ("grow_factor", C.c_double), ("compact_before_grow", C.c_int32),
+                ("max_payload", C.c_int64), ("row_alignment", C.c_int32), ("compaction_headroom", C.c_int32)]
+
+
+class HyperBatch(C.Structure):
+    _fields_ = [("delete_hyperedges", C.c_void_p), ("num_deletions", C.c_int64),
+                ("insert_offsets", C.c_void_p), ("insert_members", C.c_void_p), ("insert_weights", C.c_void_p),
+                ("num_insertions", C.c_int64),
+                ("delete_incidence_hyperedge", C.c_void_p), ("delete_incidence_vertex", C.c_void_p),
+                ("num_removals", C.c_int64),
+                ("insert_incidence_hyperedge", C.c_void_p), ("insert_incidence_vertex", C.c_void_p),
+                ("num_additions", C.c_int64)]
+
+
+HYPER_SUMMARY_FIELDS = ("inserted_hyperedges", "deleted_hyperedges", "inserted_incidences", "deleted_incidences",
+                        "skipped_ops", "ignored_deletions", "dropped_members", "num_hyperedges_after",
+                        "id_bound_after")
+
+
+class HyperSummary(C.Structure):
+    _fields_ = [(f, C.c_int64) for f in HYPER_SUMMARY_FIELDS]
+
+
+class HyperDelta(C.Structure):
+    _fields_ = [("num_insertions", C.c_int64), ("insert_src", _i32p), ("insert_dst", _i32p),
+                ("insert_weights", _i32p), ("num_deletions", C.c_int64), ("delete_src", _i32p),
+                ("delete_dst", _i32p), ("line_graph_insertions", C.c_int64), ("line_graph_deletions", C.c_int64)]
 
 
 _lib = None
@@ -209,6 +244,15 @@ def lib() -> C.CDLL:
     L.dyng_hip_memcpy_htod.argtypes = [vp, vp, C.c_int64]
     L.dyng_hip_memcpy_dtoh.argtypes = [vp, vp, C.c_int64]
     L.dyng_hip_synchronize.argtypes = [C.c_int]
+    L.dyng_hip_hypergraph_create.argtypes = [vp, vp, C.c_int64, vp, C.c_int32, C.POINTER(HyperProps), C.POINTER(vp)]
+    L.dyng_hip_hypergraph_destroy.argtypes = [vp]
+    L.dyng_hip_hypergraph_clone.argtypes = [vp, C.POINTER(vp)]
+    L.dyng_hip_hypergraph_info.argtypes = [vp, _i32p, _i64p, _i64p, _i64p, _i32p]
+    L.dyng_hip_hypergraph_rows.argtypes = [vp, C.POINTER(_i64p), C.POINTER(_i32p), C.POINTER(_i64p)]
+    L.dyng_hip_hypergraph_apply.argtypes = [vp, C.POINTER(HyperBatch), C.c_int64, C.POINTER(HyperSummary), vp]
+    L.dyng_hip_hyper_line_graph.argtypes = [vp, C.c_int32, _i64p, C.POINTER(_i32p), C.POINTER(_i32p),
+                                            C.POINTER(_i32p)]
+    L.dyng_hip_hyper_line_graph_delta.argtypes = [vp, C.c_int32, C.POINTER(HyperDelta)]
     _lib = L
     return L
 
@@ -224,6 +268,7 @@ def check(status: int) -> None:
         NOT_SUPPORTED: errors.NotSupportedError,
         OUT_OF_MEMORY: errors.OutOfMemoryError,
         RUNTIME_ERROR: errors.DeviceError,
+        CAPACITY: errors.CapacityError,
     }.get(status)
     if status == DEVICE_ERROR:
         # invalid input trees (parent cycle, distances out of range) are invalid_argument_error in

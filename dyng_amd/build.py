@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdyng_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["graph.hip", "sssp.hip", "scheduler.hip", "mosp.hip", "api.cpp"]
+SOURCES = ["graph.hip", "sssp.hip", "scheduler.hip", "mosp.hip", "api.cpp", "hyper.cpp"]
 ARCH = os.environ.get("DYNG_HIP_ARCH", "gfx950")
 
 

@@ -23,3 +23,7 @@ class OutOfMemoryError(DyngError, MemoryError):
 
 class DeviceError(DyngError, RuntimeError):
     """The HIP runtime or a kernel reported an error (the reference's cuda_error)."""
+
+
+class CapacityError(DyngError):
+    """capacity_error (error.hpp): a storage cannot hold the batch."""

@@ -16,8 +16,8 @@ hyperedges that share a vertex (the node weight of the head, as the reference's 
 The shortest-path tree from vertex 0 is the tree above: D(h) is the distance of h + 1, and the
 lowest-id tight parent of h + 1 is 0 exactly for the seeded hyperedges (every other tight
 parent g + 1 would need D(g) = 0). ``compute`` solves it with the fused engine; ``update``
-applies the batch to the hypergraph (host bookkeeping, :mod:`dyng_amd.hypergraph`), turns the
-line graph's change into an edge batch (the pairs it lost and gained, and the in-edges of a
+applies the batch to the hypergraph (the library's host container, :mod:`dyng_amd.hypergraph`),
+which turns the line graph's change into an edge batch (the pairs it lost and gained, and the in-edges of a
 reused id whose weight changed) and repairs the tree with ``dyng_amd.sssp.update`` (the device
 apply and the persistent kernel). Afterwards the result equals ``compute`` on the new
 hypergraph, distances and parents.
@@ -30,12 +30,14 @@ end: ``fallback_used`` is False); ``invalidated``, ``iterations``, ``frontier_vi
 
 from __future__ import annotations
 
+import ctypes as C
 import weakref
 from dataclasses import dataclass, field, replace
 from typing import Any
 
 import numpy as np
 
+from . import _lib
 from . import sssp as _sssp
 from .batch import EdgeBatch
 from .errors import InvalidArgumentError, NotSupportedError, StaleResultError
@@ -93,31 +95,33 @@ class Stats:
 
 
 def _line_graph_edges(hg: Hypergraph, source: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
-    """Every edge of a result's device line graph: (0, h + 1) for the hyperedges that hold the
-    source, (g + 1, h + 1) for the ordered pairs that share a vertex; weight w(h)."""
-    w = hg._w
-    src, dst = [], []
-    for v in range(hg.num_vertices):
-        hs = np.fromiter(hg._v2h[v], np.int64)
-        d = len(hs)
-        if d > 1:
-            a = np.repeat(hs, d)
-            b = np.tile(hs, d)
-            keep = a != b
-            src.append(a[keep])
-            dst.append(b[keep])
-    if src:
-        key = np.unique(np.concatenate(src) * (1 << 32) + np.concatenate(dst))
-        s_all, d_all = key >> 32, key & 0xffffffff
-    else:
-        s_all, d_all = np.zeros(0, np.int64), np.zeros(0, np.int64)
-    seeds = np.asarray(sorted(hg._v2h[source]) if 0 <= source < hg.num_vertices else [], np.int64)
-    wt = np.asarray(w, np.int64) if w else np.zeros(0, np.int64)
-    s = np.concatenate([np.zeros(len(seeds), np.int64), s_all + 1])
-    d = np.concatenate([seeds + 1, d_all + 1])
-    weights = np.concatenate([wt[seeds] if len(seeds) else np.zeros(0, np.int64), wt[d_all] if len(d_all) else
-                              np.zeros(0, np.int64)])
-    return s, d, weights
+    """Every edge of a result's device line graph (dyng_hip_hyper_line_graph): (0, h + 1) for the
+    hyperedges that hold the source, (g + 1, h + 1) for the ordered pairs that share a vertex;
+    weight w(h)."""
+    count = C.c_int64()
+    src, dst, w = _lib._i32p(), _lib._i32p(), _lib._i32p()
+    _lib.check(_lib.lib().dyng_hip_hyper_line_graph(hg._h, int(source), C.byref(count), C.byref(src), C.byref(dst),
+                                                    C.byref(w)))
+    m = count.value
+    if m == 0:
+        z = np.zeros(0, np.int32)
+        return z, z.copy(), z.copy()
+    return tuple(np.ctypeslib.as_array(x, shape=(m,)).copy() for x in (src, dst, w))
+
+
+def _edge_batch(hg: Hypergraph, source: int) -> tuple[EdgeBatch, int, int]:
+    """The change of a result's device line graph over the hypergraph's last apply
+    (dyng_hip_hyper_line_graph_delta), and the unordered pairs the line graph gained and lost."""
+    d = _lib.HyperDelta()
+    _lib.check(_lib.lib().dyng_hip_hyper_line_graph_delta(hg._h, int(source), C.byref(d)))
+
+    def arr(p, k):
+        return np.ctypeslib.as_array(p, shape=(k,)).copy() if k else np.zeros(0, np.int32)
+
+    ni, nd = d.num_insertions, d.num_deletions
+    batch = EdgeBatch(insert=(arr(d.insert_src, ni), arr(d.insert_dst, ni), arr(d.insert_weights, ni)),
+                      delete=(arr(d.delete_src, nd), arr(d.delete_dst, nd)))
+    return batch, int(d.line_graph_insertions), int(d.line_graph_deletions)
 
 
 class Result:
@@ -178,7 +182,7 @@ class Result:
         if v == self._source:
             return 0
         d = self._arrays()[0]
-        return int(min((d[h] for h in hg._v2h[v]), default=INFINITE_DISTANCE))
+        return int(min((d[h] for h, row in enumerate(hg.rows()) if v in row), default=INFINITE_DISTANCE))
 
     def vertex_distances(self) -> np.ndarray:
         """The distance of every vertex (int64, host memory)."""
@@ -263,75 +267,6 @@ def compute(hypergraph: Hypergraph, source: int, *, options: Options | None = No
     return r
 
 
-def _edge_batch(hg: Hypergraph, applied: Any, source: int) -> tuple[EdgeBatch, int, int]:
-    """The change of a result's device line graph, and the unordered pairs the line graph gained
-    and lost (hyper_sssp_reference.cpp:248-289's delta)."""
-    touched = set(applied.pre)
-    post_w = hg._w
-    # the hyperedges of a vertex before the batch: today's, with the touched ids as they were
-    pre_v2h: dict[int, set[int]] = {}
-    for t in touched:
-        for v in set(applied.pre[t]) | set(applied.post[t]):
-            if v not in pre_v2h:
-                pre_v2h[v] = set(hg._v2h[v])
-    for t in touched:
-        for v in applied.post[t]:
-            pre_v2h[v].discard(t)
-    for t in touched:
-        for v in applied.pre[t]:
-            pre_v2h[v].add(t)
-
-    def pre_nbrs(t: int) -> set[int]:
-        out: set[int] = set()
-        for v in applied.pre[t]:
-            out |= pre_v2h[v]
-        out.discard(t)
-        return out
-
-    ins_s, ins_d, ins_w, del_s, del_d = [], [], [], [], []
-    gained: set[tuple[int, int]] = set()
-    lost: set[tuple[int, int]] = set()
-    for t in touched:
-        before = pre_nbrs(t)
-        after = hg.neighbours(t)
-        w_pre = applied.weights_pre.get(t, 0)
-        w_post = post_w[t] if t < len(post_w) else 0
-        # only the pairs that changed, and the in-edges of a reused id whose weight changed
-        for g in before - after:
-            lost.add((g, t) if g < t else (t, g))
-            del_s.append(g + 1)
-            del_d.append(t + 1)
-            if g not in touched:  # a touched g handles the edge (t, g) as its in-edge
-                del_s.append(t + 1)
-                del_d.append(g + 1)
-        for g in after - before:
-            gained.add((g, t) if g < t else (t, g))
-            ins_s.append(g + 1)
-            ins_d.append(t + 1)
-            ins_w.append(w_post)
-            if g not in touched:
-                ins_s.append(t + 1)
-                ins_d.append(g + 1)
-                ins_w.append(post_w[g])
-        if w_pre != w_post:
-            for g in after & before:
-                ins_s.append(g + 1)
-                ins_d.append(t + 1)
-                ins_w.append(w_post)
-        # the virtual source's edge (0, t)
-        was, now = source in applied.pre[t], source in applied.post[t]
-        if was and not now:
-            del_s.append(0)
-            del_d.append(t + 1)
-        elif now and (not was or w_pre != w_post):
-            ins_s.append(0)
-            ins_d.append(t + 1)
-            ins_w.append(w_post)
-    batch = EdgeBatch(insert=(np.asarray(ins_s, np.int64), np.asarray(ins_d, np.int64), np.asarray(ins_w, np.int64)),
-                      delete=(np.asarray(del_s, np.int64), np.asarray(del_d, np.int64)))
-    return batch, len(gained), len(lost)
-
-
 def update(hypergraph: Hypergraph, batch: HyperedgeBatch, result: Result, *, resources: Any = None) -> Stats:
     """Apply `batch` to `hypergraph` and update `result` (hyper_sssp::update). Afterwards the result
     equals compute() on the new hypergraph."""
@@ -343,9 +278,9 @@ def update(hypergraph: Hypergraph, batch: HyperedgeBatch, result: Result, *, res
     if result._poisoned or result._hg() is not hg or result._version != hg.version:
         raise StaleResultError("dyng_amd.hyper_sssp.update: the result does not match the hypergraph's current "
                                "state (it was computed on another hypergraph, or the hypergraph changed without it)")
-    applied = hg._apply(batch, validate_weights=result._options.validate_inputs)
+    summary = hg._apply(batch, MAX_WEIGHT if result._options.validate_inputs else 0)
     try:
-        eb, gained, lost = _edge_batch(hg, applied, result._source)
+        eb, gained, lost = _edge_batch(hg, result._source)
         st = _sssp.update(result._lg, eb, result._sp)
     except BaseException:
         result._poisoned = True
@@ -354,6 +289,6 @@ def update(hypergraph: Hypergraph, batch: HyperedgeBatch, result: Result, *, res
     result._bound = hg.hyperedge_id_bound
     dev = st.device
     return Stats(affected=st.invalidated, iterations=st.iterations, frontier_visits=st.frontier_visits,
-                 fallback_used=False, converged=True, engine_used="operators", batch=applied.summary,
+                 fallback_used=False, converged=True, engine_used="operators", batch=summary,
                  line_graph_insertions=gained, line_graph_deletions=lost, invalidated=st.invalidated,
                  relaxations=int(dev.relax_edges + dev.pull_edges), device=st)

@@ -37,7 +37,8 @@ typedef enum dyng_hip_status {
   DYNG_HIP_NOT_SUPPORTED = 3,    /* not_supported_error */
   DYNG_HIP_OUT_OF_MEMORY = 4,    /* out_of_memory_error */
   DYNG_HIP_DEVICE_ERROR = 5,     /* a device error bit was raised (see device_errors) */
-  DYNG_HIP_RUNTIME_ERROR = 6     /* the HIP runtime reported an error (cuda_error) */
+  DYNG_HIP_RUNTIME_ERROR = 6,    /* the HIP runtime reported an error (cuda_error) */
+  DYNG_HIP_CAPACITY = 7          /* capacity_error: a storage cannot hold the batch */
 } dyng_hip_status;
 
 /* Device error bits (cpp/src/util/device_error_flags.hpp). */
@@ -452,6 +453,100 @@ int dyng_hip_mosp_arrays(const dyng_hip_mosp* m, const int64_t** distances, cons
 int dyng_hip_mosp_path_costs(const dyng_hip_mosp* m, const dyng_hip_graph* g, int32_t columns,
                              int64_t* costs, void* stream);
 int dyng_hip_mosp_destroy(dyng_hip_mosp* m);
+
+/* ---- hyper_sssp's hypergraph (host side) ----
+ * A weighted hypergraph with slack_csr storage (dyng::hypergraph<int32_t, int32_t>,
+ * cpp/include/dyng/hypergraph/hypergraph.hpp; python/dyng/hypergraph.py), kept in host memory:
+ * hyperedge i of the construction gets id i, members sorted and deduplicated. The device side of
+ * hyper_sssp is a dyng_hip_graph of its line graph (vertex 0 a virtual source, vertex h + 1 the
+ * hyperedge h), built from dyng_hip_hyper_line_graph and carried across each apply by
+ * dyng_hip_hyper_line_graph_delta's edge batch. */
+typedef struct dyng_hip_hypergraph dyng_hip_hypergraph;
+
+/* hypergraph_properties' id_reuse and capacity_policy (hypergraph_properties.hpp:60-152). */
+typedef struct {
+  int32_t id_reuse; /* 0 erase_first (deletions, removals, additions, insertions), 1 insert_first */
+  double initial_headroom;
+  int64_t initial_slack;
+  int64_t min_initial_payload;
+  double grow_factor; /* 0: never grow (capacity_error) */
+  int32_t compact_before_grow;
+  int64_t max_payload; /* 0: 2^31 - 1 */
+  int32_t row_alignment;
+  int32_t compaction_headroom;
+} dyng_hip_hyper_props;
+
+/* hyperedge_batch_view (hyperedge_batch.hpp:41-51): hyperedge deletions, insertions (a CSR of
+ * members, one weight each on a weighted hypergraph), incidence removals and additions. Host
+ * arrays. */
+typedef struct {
+  const int64_t* delete_hyperedges;
+  int64_t num_deletions;
+  const int64_t* insert_offsets; /* num_insertions + 1 */
+  const int32_t* insert_members;
+  const int64_t* insert_weights;
+  int64_t num_insertions;
+  const int64_t* delete_incidence_hyperedge;
+  const int32_t* delete_incidence_vertex;
+  int64_t num_removals;
+  const int64_t* insert_incidence_hyperedge;
+  const int32_t* insert_incidence_vertex;
+  int64_t num_additions;
+} dyng_hip_hyper_batch;
+
+/* hyper_apply_summary (hyper_apply_summary.hpp:20-36); the inserted ids go to a caller array. */
+typedef struct {
+  int64_t inserted_hyperedges, deleted_hyperedges, inserted_incidences, deleted_incidences;
+  int64_t skipped_ops, ignored_deletions, dropped_members, num_hyperedges_after, id_bound_after;
+} dyng_hip_hyper_summary;
+
+/* The line graph's change of the last apply, for a result from `source`: the edges of its device
+ * graph to delete and to insert or re-weight (upserts), and the unordered pairs the line graph
+ * gained and lost (dyng::testing::line_graph_delta, hyper_sssp_reference.cpp:248-289). Host
+ * arrays owned by the handle, valid until its next call. */
+typedef struct {
+  int64_t num_insertions;
+  const int32_t* insert_src;
+  const int32_t* insert_dst;
+  const int32_t* insert_weights;
+  int64_t num_deletions;
+  const int32_t* delete_src;
+  const int32_t* delete_dst;
+  int64_t line_graph_insertions;
+  int64_t line_graph_deletions;
+} dyng_hip_hyper_delta;
+
+/* hypergraph::from_lists / from_hyperedges (hypergraph.hpp): `weights` null for an unweighted
+ * hypergraph (every weight >= 1 otherwise); members in [0, num_vertices). */
+int dyng_hip_hypergraph_create(const int64_t* offsets, const int32_t* members, int64_t num_hyperedges,
+                               const int64_t* weights, int32_t num_vertices, const dyng_hip_hyper_props* props,
+                               dyng_hip_hypergraph** out);
+int dyng_hip_hypergraph_destroy(dyng_hip_hypergraph* h);
+/* hypergraph::clone (the ids, the store's state and the version kept). */
+int dyng_hip_hypergraph_clone(const dyng_hip_hypergraph* h, dyng_hip_hypergraph** out);
+/* num_vertices, num_hyperedges (live), hyperedge_id_bound, version, weighted. */
+int dyng_hip_hypergraph_info(const dyng_hip_hypergraph* h, int32_t* num_vertices, int64_t* num_live,
+                             int64_t* id_bound, int64_t* version, int32_t* weighted);
+/* h2v(): the members of every id below the bound (offsets id_bound + 1 entries, a dead id's row
+ * empty) and the weights by id (null when unweighted). Host arrays owned by the handle, valid
+ * until its next call. */
+int dyng_hip_hypergraph_rows(dyng_hip_hypergraph* h, const int64_t** offsets, const int32_t** members,
+                             const int64_t** weights);
+/* hypergraph::apply (hypergraph.cpp:573-1240): the phases, skip rules and counts of the reference,
+ * the ids of its best-fit allocator under the hypergraph's id_reuse. Inserted weights above
+ * `max_weight` (> 0) are refused before anything changes (hyper_sssp's validate_inputs);
+ * `inserted_ids` gets num_insertions ids. A storage that cannot hold the batch under a policy
+ * that does not grow raises DYNG_HIP_CAPACITY, the hypergraph unchanged. */
+int dyng_hip_hypergraph_apply(dyng_hip_hypergraph* h, const dyng_hip_hyper_batch* b, int64_t max_weight,
+                              dyng_hip_hyper_summary* out, int64_t* inserted_ids);
+/* The device line graph of a hyper_sssp result from the vertex `source`, as an edge list: (0, h + 1)
+ * of weight w(h) for the hyperedges that hold the source, (g + 1, h + 1) of weight w(h) for every
+ * ordered pair of live hyperedges sharing a vertex (operators.cu:350-357,457: the head's node
+ * weight). Host arrays owned by the handle, valid until its next call. */
+int dyng_hip_hyper_line_graph(dyng_hip_hypergraph* h, int32_t source, int64_t* count, const int32_t** src,
+                              const int32_t** dst, const int32_t** weight);
+/* The change of that line graph over the last apply (see dyng_hip_hyper_delta). */
+int dyng_hip_hyper_line_graph_delta(dyng_hip_hypergraph* h, int32_t source, dyng_hip_hyper_delta* out);
 
 #ifdef __cplusplus
 }

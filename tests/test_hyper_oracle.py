@@ -1,6 +1,7 @@
-"""CPU tests of the hyper_sssp oracle (oracle/hyper.py) and of the host side of
-dyng_amd.hyper_sssp: the hypergraph container (ids, skip rules, counts) and the edge batch that
-carries a result's device line graph from one hypergraph version to the next. No GPU needed.
+"""CPU tests of the hyper_sssp oracles (oracle/hyper.py, oracle/hypergraph_model.py) and of the
+host side of dyng_amd.hyper_sssp in libdyng_hip.so: the hypergraph container (ids, skip rules,
+counts) and the edge batch that carries a result's device line graph from one hypergraph version
+to the next. The library's host calls run without a GPU.
 
 Pins of the oracle: the reference's worked example and test expectations
 (python/dyng/hyper_sssp.py docstring; python/tests/test_hyper_sssp.py:63-82,104-112),
@@ -18,6 +19,7 @@ import pytest
 import dyng_amd as dg
 from dyng_amd import hyper_sssp as hs
 from oracle import hyper as oh
+from oracle import hypergraph_model as hm
 
 INF = oh.INF
 LISTS = [[0, 1], [1, 2], [2, 3], [0, 3], [4, 5]]
@@ -249,11 +251,12 @@ def test_edge_batch_carries_the_line_graph(reuse, seed):
     h = dg.Hypergraph.from_lists(lists, rng.integers(1, 9, 9).tolist(), num_vertices=n, id_reuse=reuse)
     source = int(rng.integers(0, n))
     edges = _edges(h, source)
+    assert edges == _brute_edges(h.rows(), h.weights.tolist(), source)
     for _ in range(10):
         pre_rows = h.rows()
         dels, ins, w, rem, add = _random_batch(rng, h, n)
-        applied = h._apply(dg.HyperedgeBatch(dels, ins, w, delete_incidences=rem, insert_incidences=add))
-        eb, gained, lost = hs._edge_batch(h, applied, source)
+        h.apply(dg.HyperedgeBatch(dels, ins, w, delete_incidences=rem, insert_incidences=add))
+        eb, gained, lost = hs._edge_batch(h, source)
         ins_s, ins_d, ins_w = (np.asarray(a) for a in eb._ins) if eb._ins else ([], [], [])
         del_s, del_d = (np.asarray(a) for a in eb._del) if eb._del else ([], [])
         for a, b in zip(del_s, del_d):
@@ -261,6 +264,78 @@ def test_edge_batch_carries_the_line_graph(reuse, seed):
             del edges[(int(a), int(b))]
         for a, b, x in zip(ins_s, ins_d, np.asarray(ins_w).reshape(-1)):
             edges[(int(a), int(b))] = int(x)
-        assert edges == _edges(h, source)
+        assert edges == _edges(h, source) == _brute_edges(h.rows(), h.weights.tolist(), source)
         deleted, inserted = oh.line_graph_delta(pre_rows, h.rows())
         assert (gained, lost) == (len(inserted), len(deleted))
+
+
+def _brute_edges(rows, weights, source):
+    """The device line graph by brute force: (0, h + 1, w(h)) for the hyperedges that hold the
+    source, (g + 1, h + 1, w(h)) for the ordered pairs of live hyperedges sharing a vertex."""
+    out = {}
+    for h, r in enumerate(rows):
+        if source in r:
+            out[(0, h + 1)] = weights[h]
+        for g, q in enumerate(rows):
+            if g != h and r and q and set(r) & set(q):
+                out[(g + 1, h + 1)] = weights[h]
+    return out
+
+
+# ---- the library's container against the restated rules ----
+
+POLICIES = {
+    "dyng_default": hm.CapacityPolicy(),
+    "escher_gpu": hm.CapacityPolicy.escher_gpu(),
+    # a small payload: compactions, growth and the headroom after a compaction on every batch
+    "tight": hm.CapacityPolicy(initial_headroom=0.0, initial_slack=8, compaction_headroom=8),
+    "tight_no_compaction": hm.CapacityPolicy(initial_headroom=0.0, initial_slack=8, compact_before_grow=False),
+}
+
+
+def _props_of(p):
+    return dg.CapacityPolicy(p.initial_headroom, p.initial_slack, p.min_initial_payload, p.grow_factor,
+                             p.compact_before_grow, p.max_payload, p.row_alignment, p.compaction_headroom)
+
+
+@pytest.mark.parametrize("policy", sorted(POLICIES))
+@pytest.mark.parametrize("reuse", ["erase_first", "insert_first"])
+@pytest.mark.parametrize("seed", range(4))
+def test_container_against_the_restated_rules(policy, reuse, seed):
+    """The library's container gives the ids, rows and counts of the restated allocator and apply
+    rules, batch after batch, under payload policies that compact and grow often."""
+    rng = np.random.default_rng(300 + seed)
+    n = 16
+    lists = [rng.integers(0, n, int(rng.integers(1, 6))).tolist() for _ in range(12)]
+    w = rng.integers(1, 9, 12).tolist()
+    pol = POLICIES[policy]
+    h = dg.Hypergraph.from_lists(lists, w, num_vertices=n, id_reuse=reuse, capacity=_props_of(pol))
+    model = hm.Model(lists, w, n, reuse, pol)
+    for _ in range(15):
+        dels, ins, iw, rem, add = _random_batch(rng, h, n, size=8)
+        s = h.apply(dg.HyperedgeBatch(dels, ins, iw, delete_incidences=rem, insert_incidences=add))
+        counts, ids = model.apply(dels, ins, iw, rem, add)
+        assert s.inserted_ids.tolist() == ids
+        for name, value in counts.items():
+            assert getattr(s, name) == value, name
+        assert h.rows() == model.rows_by_id()
+        assert h.weights.tolist() == [model.w[i] if i < len(model.w) and model.rows_by_id()[i] else 0
+                                      for i in range(h.hyperedge_id_bound)]
+
+
+def test_capacity_error_changes_nothing():
+    """A policy that never grows: a batch whose surplus rows do not fit raises CapacityError and
+    leaves the hypergraph as it was (the restated allocator raises too)."""
+    pol = hm.CapacityPolicy(initial_headroom=0.0, initial_slack=5, grow_factor=0.0)  # 8 + 5 payload slots
+    lists = [[0, 1], [1, 2]]
+    h = dg.Hypergraph.from_lists(lists, [1, 1], num_vertices=4, capacity=_props_of(pol))
+    model = hm.Model(lists, [1, 1], 4, "erase_first", pol)
+    big = [[0, 1, 2, 3]] * 3
+    with pytest.raises(hm.CapacityError):
+        model.apply([], big, [1, 1, 1])
+    before = (h.version, h.rows(), h.hyperedge_id_bound)
+    with pytest.raises(dg.CapacityError):
+        h.apply(dg.HyperedgeBatch(insert_hyperedges=big, insert_weights=[1, 1, 1]))
+    assert (h.version, h.rows(), h.hyperedge_id_bound) == before
+    s = h.apply(dg.HyperedgeBatch(insert_hyperedges=[[3]], insert_weights=[2]))
+    assert s.inserted_ids.tolist() == model.apply([], [[3]], [2])[1]
