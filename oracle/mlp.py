@@ -90,3 +90,30 @@ def forward_bf16(p: np.ndarray, x: np.ndarray) -> np.ndarray:
     h1 = bf16(np.maximum(xn @ bf16(w1).T + bf16(b1), 0).astype(np.float32))
     h2 = bf16(np.maximum(h1 @ bf16(w2).T + bf16(b2), 0).astype(np.float32))
     return (h2 @ bf16(w3).T + bf16(b3)).astype(np.float32)
+
+
+def batch_features(out_degree, parents, ins_src, del_src, del_dst) -> dict:
+    """The scheduler's raw batch features on G_t (before the batch): the deletions (u, v) that hit
+    the old tree (parents[v] == u), the operations whose tail u is a vertex of the graph
+    ("touched"), the mean of log2(1 + out-degree(u)) over them (each term rounded to 1/1024, as the
+    kernel accumulates it in integers) and the largest such out-degree. `out_degree` is the row
+    length of every vertex, `parents` the old tree (or None)."""
+    deg = np.asarray(out_degree, np.int64)
+    n = len(deg)
+    ins_src = np.asarray(ins_src, np.int64)
+    del_src = np.asarray(del_src, np.int64)
+    del_dst = np.asarray(del_dst, np.int64)
+    hits = 0
+    if parents is not None and len(del_src):
+        par = np.asarray(parents, np.int64)
+        ok = (del_dst >= 0) & (del_dst < len(par))
+        hits = int(np.count_nonzero(par[del_dst[ok]] == del_src[ok]))
+    tails = np.concatenate([del_src, ins_src])
+    tails = tails[(tails >= 0) & (tails < n)]
+    d = deg[tails]
+    terms = np.floor(np.log2(1.0 + d.astype(np.float32)).astype(np.float32) * np.float32(1024.0) + np.float32(0.5))
+    touched = len(tails)
+    return {"tree_deletions": hits, "touched": touched,
+            "log_degree_sum": int(terms.astype(np.int64).sum()),
+            "mean_log_degree": float(terms.sum() / 1024.0 / touched) if touched else 0.0,
+            "max_degree": int(d.max()) if touched else 0}

@@ -190,3 +190,43 @@ def test_trained_scheduler_against_the_plain_schedule(dg):
     # little on these batches for a larger gain (the best possible pick is reported beside).
     assert ranked >= 0.9 * len(batches)
     assert learned <= 1.03 * plain
+
+
+def test_batch_features_match_the_oracle(dg):
+    """The scheduler's raw batch features from the device (k_batch_features) equal the oracle's
+    restatement (oracle/mlp.py batch_features) on a skewed graph: deletions of tree edges, of other
+    edges and of absent pairs, insertions (some at new vertices), with and without the tree."""
+    from dyng_amd import scheduler as sch
+
+    _say("[scheduler] batch features vs the oracle")
+    rng = np.random.default_rng(23)
+    n, m = 6000, 90_000
+    src = np.minimum(rng.zipf(1.6, m) - 1, n - 1)  # a few hubs with long rows
+    dst = rng.integers(0, n, m)
+    w = rng.integers(1, 50, m)
+    g = dg.Graph.from_edges(src, dst, w, num_vertices=n)
+    og = oracle.build_from_edges(n, src, dst, w)
+    deg = np.diff(og.row_ptr)
+    r = dg.sssp.compute(g, int(src[0]))
+    parents = r.parents
+    rows = np.repeat(np.arange(n), deg)
+    for step in range(4):
+        k = int(rng.integers(1, 4000))
+        pick = rng.integers(0, og.m, k)
+        tree_v = rng.choice(np.nonzero(parents >= 0)[0], 200)
+        del_s = np.concatenate([rows[pick], parents[tree_v], rng.integers(0, n, 50)])
+        del_d = np.concatenate([og.col[pick], tree_v, rng.integers(0, n, 50)])
+        ins_s = np.concatenate([rng.integers(0, n, k), rng.integers(n, n + 20, 5)])
+        ins_d = rng.integers(0, n, len(ins_s))
+        ins_w = rng.integers(1, 50, len(ins_s))
+        batch = dg.EdgeBatch(insert=(ins_s, ins_d, ins_w), delete=(del_s, del_d))
+        for res, par in ((r, parents), (None, None)):
+            st = sch.batch_stats(g, batch, res)
+            want = mlp.batch_features(deg, par, ins_s, del_s, del_d)
+            assert (st.num_ins, st.num_del) == (len(ins_s), len(del_s)), step
+            assert (st.num_vertices, st.num_edges) == (n, og.m), step
+            assert st.tree_deletions == want["tree_deletions"], step
+            assert st.touched == want["touched"], step
+            assert st.max_degree == want["max_degree"], step
+            # each term is log2f rounded to 1/1024 on both sides; allow one step of 1/1024 in a term
+            assert abs(st.mean_log_degree - want["mean_log_degree"]) <= 1.0 / 1024, step

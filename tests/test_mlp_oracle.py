@@ -82,3 +82,30 @@ def test_bf16_forward_is_close_to_fp64():
     y64, _ = mlp.forward(p, x)
     yb = mlp.forward_bf16(p, x)
     assert np.max(np.abs(yb - y64)) <= 0.05 * (1 + np.max(np.abs(y64)))
+
+
+def test_batch_features_against_a_loop():
+    """oracle.mlp.batch_features against a plain loop over the operations (its definition written
+    out), and a planted mistake (tree hits tested as parents[u] == v) that the loop catches."""
+    import math
+
+    from oracle import mlp as om
+
+    rng = np.random.default_rng(4)
+    n = 50
+    deg = rng.integers(0, 300, n)
+    parents = np.where(rng.random(n) < 0.8, rng.integers(0, n, n), -1)
+    for _ in range(20):
+        ins_s = rng.integers(0, n + 5, int(rng.integers(0, 40)))
+        del_s = rng.integers(0, n, int(rng.integers(0, 40)))
+        del_d = rng.integers(0, n + 3, len(del_s))
+        hits = sum(1 for u, v in zip(del_s, del_d) if v < n and parents[v] == u)
+        tails = [u for u in list(del_s) + list(ins_s) if 0 <= u < n]
+        terms = [math.floor(float(np.float32(np.log2(np.float32(1 + deg[u])))) * 1024 + 0.5) for u in tails]
+        got = om.batch_features(deg, parents, ins_s, del_s, del_d)
+        assert got["tree_deletions"] == hits and got["touched"] == len(tails)
+        assert got["log_degree_sum"] == sum(terms)
+        assert got["max_degree"] == (max(deg[u] for u in tails) if tails else 0)
+        wrong = sum(1 for u, v in zip(del_s, del_d) if u < n and v < n and parents[u] == v)
+        if hits != wrong:
+            assert got["tree_deletions"] != wrong
