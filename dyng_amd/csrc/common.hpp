@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "row_search.hpp"
+
 namespace dyng_hip {
 
 using u64 = unsigned long long;
@@ -42,35 +44,6 @@ DYNG_DEV T ld(const T* p) {
 template <typename T>
 DYNG_DEV void st(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---- searches in the sorted runs of a row (a neighbour id with the sign bit as the dead mark) ----
-
-/// First index in [lo, hi) whose neighbour (sign bit masked) is >= v, or hi. Each round probes
-/// seven evenly spaced entries with independent loads, so a long row costs about log8 of its
-/// length in dependent memory round trips instead of log2; the last 16 entries are bisected.
-DYNG_DEV int row_lower_bound(const int32_t* a, int lo, int hi, int32_t v) {
-  while (hi - lo > 16) {  // the answer lies in [lo, hi]
-    const int step = (hi - lo) >> 3;
-    int below = 0;  // probes lo + k * step (k = 1..7) whose neighbour is < v: a prefix of them
-#pragma unroll
-    for (int k = 1; k <= 7; ++k) below += (a[lo + k * step] & 0x7fffffff) < v ? 1 : 0;
-    const int next_lo = below == 0 ? lo : lo + below * step + 1;
-    hi = below == 7 ? hi : lo + (below + 1) * step;
-    lo = next_lo;
-  }
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if ((a[mid] & 0x7fffffff) < v) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
-/// One past the last entry of v's run, given lb = row_lower_bound(a, ., hi, v): parallel edges
-/// are rare, so one load usually settles it.
-DYNG_DEV int row_upper_bound_from(const int32_t* a, int lb, int hi, int32_t v) {
-  if (lb >= hi || (a[lb] & 0x7fffffff) != v) return lb;
-  return row_lower_bound(a, lb + 1, hi, v + 1);  // vertex ids are below INT32_MAX
 }
 
 DYNG_DEV int lane() { return static_cast<int>(__lane_id()); }

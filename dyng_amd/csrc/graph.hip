@@ -740,31 +740,39 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
       if (first_touch) p.touched[t] = u;
     }
   }
-  __shared__ unsigned long long part[kBlock / 64];
-  c_ins = block_reduce(c_ins, part, op_sum{});
-  c_upd = block_reduce(c_upd, part, op_sum{});
-  c_del = block_reduce(c_del, part, op_sum{});
-  c_ign_del = block_reduce(c_ign_del, part, op_sum{});
-  c_ign_ins = block_reduce(c_ign_ins, part, op_sum{});
-  const long long ds[4] = {static_cast<long long>(block_reduce(static_cast<u64>(ds0), part, op_sum{})),
-                           static_cast<long long>(block_reduce(static_cast<u64>(ds1), part, op_sum{})),
-                           static_cast<long long>(block_reduce(static_cast<u64>(ds2), part, op_sum{})),
-                           static_cast<long long>(block_reduce(static_cast<u64>(ds3), part, op_sum{}))};
-  const long long dc[4] = {static_cast<long long>(block_reduce(static_cast<u64>(dc0), part, op_sum{})),
-                           static_cast<long long>(block_reduce(static_cast<u64>(dc1), part, op_sum{})),
-                           static_cast<long long>(block_reduce(static_cast<u64>(dc2), part, op_sum{})),
-                           static_cast<long long>(block_reduce(static_cast<u64>(dc3), part, op_sum{}))};
+  // the workgroup's counters in one pass: wave sums by shuffles, one barrier, then thread 0 adds
+  // the waves' sums and bumps the device counters (13 values)
+  constexpr int kSums = 13;
+  u64 sum[kSums] = {c_ins, c_upd, c_del, c_ign_del, c_ign_ins,
+                    static_cast<u64>(ds0), static_cast<u64>(ds1), static_cast<u64>(ds2), static_cast<u64>(ds3),
+                    static_cast<u64>(dc0), static_cast<u64>(dc1), static_cast<u64>(dc2), static_cast<u64>(dc3)};
+  __shared__ u64 part[kBlock / 64][kSums];
+#pragma unroll
+  for (int q = 0; q < kSums; ++q) {
+    for (int off = 32; off > 0; off >>= 1) sum[q] += static_cast<u64>(__shfl_xor(sum[q], off));
+  }
+  if (lane() == 0) {
+#pragma unroll
+    for (int q = 0; q < kSums; ++q) part[threadIdx.x / 64][q] = sum[q];
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if (c_ins) atomicAdd(&p.ctl->inserted_edges, c_ins);
-    if (c_upd) atomicAdd(&p.ctl->updated_edges, c_upd);
-    if (c_del) atomicAdd(&p.ctl->deleted_edges, c_del);
-    if (c_ign_del) atomicAdd(&p.ctl->ignored_deletions, c_ign_del);
-    if (c_ign_ins) atomicAdd(&p.ctl->ignored_insertions, c_ign_ins);
+#pragma unroll
+    for (int q = 0; q < kSums; ++q) {
+      u64 t = part[0][q];
+      for (int wv = 1; wv < kBlock / 64; ++wv) t += part[wv][q];
+      sum[q] = t;
+    }
+    if (sum[0]) atomicAdd(&p.ctl->inserted_edges, sum[0]);
+    if (sum[1]) atomicAdd(&p.ctl->updated_edges, sum[1]);
+    if (sum[2]) atomicAdd(&p.ctl->deleted_edges, sum[2]);
+    if (sum[3]) atomicAdd(&p.ctl->ignored_deletions, sum[3]);
+    if (sum[4]) atomicAdd(&p.ctl->ignored_insertions, sum[4]);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (k >= K) break;
-      if (ds[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dsum[k]), static_cast<unsigned long long>(ds[k]));
-      if (dc[k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dcount[k]), static_cast<unsigned long long>(dc[k]));
+      if (sum[5 + k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dsum[k]), sum[5 + k]);
+      if (sum[9 + k]) atomicAdd(reinterpret_cast<unsigned long long*>(&p.ctl->col_dcount[k]), sum[9 + k]);
     }
   }
 }
