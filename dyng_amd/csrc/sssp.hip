@@ -45,10 +45,10 @@ namespace {
 
 constexpr int kBlock = 1024;  ///< 16 waves per workgroup; 8 waves/SIMD fit two per CU, one is launched
 constexpr int kWaves = kBlock / 64;
-constexpr int kWalkGlobal = 8;   ///< first rounds of the subtree walk: one level each
+constexpr int kWalkGlobal = 2;   ///< first rounds of the subtree walk: one level each
 constexpr int kWalkLocal = 16;   ///< later rounds: levels a workgroup walks on its own between grid barriers
 constexpr int kWalkRounds = kWalkGlobal + 8;  ///< grid rounds of the walk before pointer jumping takes over
-constexpr int kRelaxGlobal = 8;  ///< relax iterations of a repair before local rounds may start
+constexpr int kRelaxGlobal = 2;  ///< relax iterations of a repair before local rounds may start
 constexpr int kRelaxLocal = 16;  ///< local rounds of a workgroup in one relax iteration
 constexpr int kRelaxNarrow = 64; ///< a near frontier of at most this many vertices per workgroup is narrow
 constexpr int kLocalGenerations = 1600000000;  ///< no local relax iteration starts past this generation
@@ -677,8 +677,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       // claimed exactly once (the child's flag), so no workgroup needs another's level: each takes
       // a tile of the frontier and walks it down on its own, its levels in LDS, a level's rows
       // edge-balanced over its threads: one level in each of the first kWalkGlobal rounds (the
-      // wide levels of a bushy tree stay spread over the grid: measured on RMAT-22, walking them
-      // alone kept the other workgroups waiting), then up to kWalkLocal levels per round (the
+      // widest levels, next to the roots, stay spread over the grid: measured on RMAT-22 and
+      // ER-64K, two such rounds beat one and eight), then up to kWalkLocal levels per round (the
       // narrow levels of a deep tree: the 4096^2 grid's walk 2.0 -> 0.4 ms). What a workgroup did
       // not reach (a level past the cap or past the LDS queue, and the children of rows long
       // enough for the chunk list) is the next round's frontier, after a grid barrier. Every flagged vertex joins the
