@@ -1085,8 +1085,10 @@ __global__ void k_merge_gate(apply_control* c, long long slot_cap_out, long long
 /// entry's place is counted with static indices (an alive entry: the alive entries before it plus
 /// the pending entries with a smaller neighbour; a pending entry: the alive entries up to its
 /// neighbour plus the pending entries before it), then each is stored at its place. The weights
-/// move kRegWeights columns per pass, each pass reading its columns before writing them. Old
+/// move KW columns per pass, each pass reading its columns before writing them (KW = 1 for a
+/// single weight column: 162 VGPRs instead of 186, three waves per SIMD instead of two). Old
 /// entries come first among equal neighbours, as in the wave merge below.
+template <int KW>
 DYNG_DEV void tail_lane(const merge_params& p, int block, int blocks) {
   const int count = p.list_count[kTailLane];
   const int32_t* list = p.lists + kTailLane * (p.N + 1);
@@ -1111,20 +1113,20 @@ DYNG_DEV void tail_lane(const merge_params& p, int block, int blocks) {
       ov[j] = pend[j] ? static_cast<int32_t>(p.keys[a + j] & mask) : 0;
     }
     int alive = 0, pending = 0;
-    for (int k0 = 0; k0 == 0 || k0 < K; k0 += kRegWeights) {
-      int32_t wr[kRegWeights][kLaneTail];
-      int32_t ow[kRegWeights][kLaneOps];
+    for (int k0 = 0; k0 == 0 || k0 < K; k0 += KW) {
+      int32_t wr[KW][kLaneTail];
+      int32_t ow[KW][kLaneOps];
 #pragma unroll
       for (int j = 0; j < kLaneTail; ++j) {
 #pragma unroll
-        for (int k = 0; k < kRegWeights; ++k) {
+        for (int k = 0; k < KW; ++k) {
           wr[k][j] = (j < tl && k0 + k < K) ? p.w[(k0 + k) * p.slot_cap + at + j] : 0;
         }
       }
 #pragma unroll
       for (int j = 0; j < kLaneOps; ++j) {
 #pragma unroll
-        for (int k = 0; k < kRegWeights; ++k) ow[k][j] = (pend[j] && k0 + k < K) ? p.pend_w[(k0 + k) * p.N + a + j] : 0;
+        for (int k = 0; k < KW; ++k) ow[k][j] = (pend[j] && k0 + k < K) ? p.pend_w[(k0 + k) * p.N + a + j] : 0;
       }
       // alive entries, at their places (the tail is sorted, dead entries carry the sign bit)
       alive = 0;
@@ -1136,7 +1138,7 @@ DYNG_DEV void tail_lane(const merge_params& p, int block, int blocks) {
           for (int j = 0; j < kLaneOps; ++j) pos += (pend[j] && ov[j] < x[i]) ? 1 : 0;
           if (k0 == 0) p.nbr[at + pos] = x[i];
 #pragma unroll
-          for (int k = 0; k < kRegWeights; ++k) {
+          for (int k = 0; k < KW; ++k) {
             if (k0 + k < K) p.w[(k0 + k) * p.slot_cap + at + pos] = wr[k][i];
           }
           ++alive;
@@ -1152,7 +1154,7 @@ DYNG_DEV void tail_lane(const merge_params& p, int block, int blocks) {
           for (int i = 0; i < kLaneTail; ++i) pos += (x[i] >= 0 && x[i] <= ov[j]) ? 1 : 0;
           if (k0 == 0) p.nbr[at + pos] = ov[j];
 #pragma unroll
-          for (int k = 0; k < kRegWeights; ++k) {
+          for (int k = 0; k < KW; ++k) {
             if (k0 + k < K) p.w[(k0 + k) * p.slot_cap + at + pos] = ow[k][j];
           }
           ++pending;
@@ -1163,8 +1165,9 @@ DYNG_DEV void tail_lane(const merge_params& p, int block, int blocks) {
   }
 }
 
+template <int KW>
 __global__ void __launch_bounds__(256) k_tail_lane(merge_sides m) {
-  for_side(m, [](const merge_params& p, int block, int blocks) { tail_lane(p, block, blocks); });
+  for_side(m, [](const merge_params& p, int block, int blocks) { tail_lane<KW>(p, block, blocks); });
 }
 
 /// One wave per row whose tail is short (at most kMergeSmall entries, 64 operations): the tail
@@ -2085,8 +2088,8 @@ void merges_exact(dyng_hip_graph* g, int64_t N, int vbits, hipStream_t s, dyng_h
   ms.side[0] = merge_setup(g, true, N, vbits, nullptr);
   ms.side[1] = merge_setup(g, false, N, vbits, nullptr);
   auto groups = [](int rows, int per_group, int cap) { return std::min((rows + per_group - 1) / per_group, cap); };
-  launch_tails(k_tail_lane, ms, groups(hc->modes_out[kTailLane], 256, 8192), groups(hc->modes_in[kTailLane], 256, 8192),
-               256, s);
+  launch_tails(g->K == 1 ? k_tail_lane<1> : k_tail_lane<kRegWeights>, ms, groups(hc->modes_out[kTailLane], 256, 8192),
+               groups(hc->modes_in[kTailLane], 256, 8192), 256, s);
   launch_tails(k_tail_wave, ms, groups(hc->modes_out[kTailWave], kMergeWaves, 8192),
                groups(hc->modes_in[kTailWave], kMergeWaves, 8192), kMergeBlock, s);
   launch_tails(k_tail_block, ms, groups(hc->modes_out[kTailBlock], 1, 4096), groups(hc->modes_in[kTailBlock], 1, 4096),
@@ -2128,7 +2131,7 @@ void merges_speculative(dyng_hip_graph* g, int64_t N, int vbits, hipStream_t s) 
   const int lane = std::min((rows + 255) / 256, 1024);
   const int wave = std::min((rows + kMergeWaves - 1) / kMergeWaves, 512);
   const int block = std::min(rows, 256);
-  launch_tails(k_tail_lane, ms, lane, lane, 256, s);
+  launch_tails(g->K == 1 ? k_tail_lane<1> : k_tail_lane<kRegWeights>, ms, lane, lane, 256, s);
   launch_tails(k_tail_wave, ms, wave, wave, kMergeBlock, s);
   launch_tails(k_tail_block, ms, block, block, kMergeBlock, s);
   DYNG_HIP_TRY(hipGetLastError());

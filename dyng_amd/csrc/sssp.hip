@@ -51,6 +51,11 @@ constexpr int kWalkRounds = kWalkGlobal + 8;  ///< grid rounds of the walk befor
 constexpr int kRelaxGlobal = 2;  ///< relax iterations of a repair before local rounds may start
 constexpr int kRelaxLocal = 16;  ///< local rounds of a workgroup in one relax iteration
 constexpr int kRelaxNarrow = 64; ///< a near frontier of at most this many vertices per workgroup is narrow
+/// A workgroup's local levels of one walk round or relax iteration stop once they have expanded
+/// this many edges; what they claimed next goes to the next grid round, spread over the grid (a
+/// local level that reaches a bushy part of the graph would otherwise keep the grid waiting on one
+/// workgroup: measured on RMAT-22, one local relax iteration of 124 us where the others took 12).
+constexpr int kLocalEdgeBudget = 8192;
 constexpr int kLocalGenerations = 1600000000;  ///< no local relax iteration starts past this generation
 
 struct control {
@@ -228,18 +233,22 @@ DYNG_DEV int4 ld_int4(const int4* q4) {
 }
 
 /// Cut the row [begin, begin + len) of vertex v into chunks of `chunk` edges (one wave each).
-DYNG_DEV void emit_chunks(const engine_params& p, int* counter, int32_t v, int32_t begin, int32_t len) {
+DYNG_DEV void emit_chunks_to(const engine_params& p, int4* out, int64_t cap, int* counter, int32_t v, int32_t begin,
+                              int32_t len) {
   const int pieces = (len + p.chunk_edges - 1) / p.chunk_edges;
   const int base = atomicAdd(counter, pieces);
   for (int c = 0; c < pieces; ++c) {
-    if (base + c >= p.chunk_cap) {
+    if (base + c >= cap) {
       atomicOr(&p.ctl->error_bits, DYNG_HIP_DEVICE_ERROR_CAPACITY);
       return;
     }
     const int b = begin + c * p.chunk_edges;
     const int e = min(begin + len, b + p.chunk_edges);
-    st_chunk(&p.chunks[base + c], v, b, e);
+    st_chunk(&out[base + c], v, b, e);
   }
+}
+DYNG_DEV void emit_chunks(const engine_params& p, int* counter, int32_t v, int32_t begin, int32_t len) {
+  emit_chunks_to(p, p.chunks, p.chunk_cap, counter, v, begin, len);
 }
 
 /// The workgroup's view of one tile of a vertex list: per item the first edge slot of its rows in
@@ -756,7 +765,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
             }
             __syncthreads();
             int q = 0;
-            for (int level = 0; level < kWalkLocal; ++level) {
+            int spent = 0;  // edges this tile's local levels expanded
+            for (int level = 0; level < kWalkLocal && spent < kLocalEdgeBudget; ++level) {
               const int items = W.count[q];  // uniform: LDS after a workgroup barrier
               if (items == 0) break;
               expand_tile(
@@ -782,6 +792,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
                     }
                   },
                   [&](bool, int) {});
+              spent += L.start[items];  // the level's edges (LDS after the expansion's barrier: uniform)
               if (threadIdx.x == 0) {
                 W.count[q] = 0;
                 W.count[q ^ 1] = min(W.count[q ^ 1], kBlock);
@@ -1066,6 +1077,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
   ++step;
   int iterations = 0, epochs = 0;
   long long pushes = 0;
+  int pending = 0;  // chunks of long rows the previous relax iteration left for the next one
   // every relax iteration expands a vertex whose distance fell since its last expansion; a run
   // past this bound is a bug: stop with an internal error instead of spinning
   const long long guard = 4LL * n + 100000;
@@ -1074,9 +1086,13 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       if (tid == 0) atomicOr(&c->error_bits, DYNG_HIP_DEVICE_ERROR_INTERNAL);
       break;
     }
-    if (near_count > 0) {
-      // -- one relax iteration: rows up to `light` edges edge-balanced per workgroup, longer
-      // rows in chunks after a barrier. After the first kRelaxGlobal iterations of a repair, an
+    if (near_count > 0 || pending > 0) {
+      // -- one relax iteration, one grid barrier: rows up to `light` edges edge-balanced per
+      // workgroup; a longer row is cut into chunks of chunk_edges, which every wave of the grid
+      // relaxes at the start of the next iteration, before its tile (the chunks of iteration k
+      // and k + 1 sit in the two halves of the chunk list; a vertex a chunk improves is claimed
+      // with the next iteration's generation and expanded one iteration later, and the fixed
+      // point, so the tree, is the same). After the first kRelaxGlobal iterations of a repair, an
       // iteration over a narrow near frontier lets each workgroup relax its share for up to
       // kRelaxLocal rounds on its own: the vertices it claims go to its next local level (LDS)
       // rather than to the next near list, each round claiming with a generation of its own
@@ -1087,19 +1103,22 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
       // this phase, tags them with the iteration). The deep, narrow repairs of a high-diameter
       // graph take hundreds of iterations; each global one costs a grid barrier and a list round
       // trip --
-      // (a local iteration takes workgroups * kRelaxLocal + 1 generations: past kLocalGenerations
+      // (a local iteration takes workgroups * kRelaxLocal generations: past kLocalGenerations
       // the iterations stay global, one generation each, so a run of millions of levels cannot
-      // carry the stamps past INT32_MAX)
-      const bool local = !p.from_scratch && iterations >= kRelaxGlobal &&
+      // carry the stamps past INT32_MAX). An iteration with chunks to relax is global.
+      const bool local = !p.from_scratch && iterations >= kRelaxGlobal && pending == 0 &&
                          near_count <= static_cast<int>(gridDim.x) * kRelaxNarrow && generation < kLocalGenerations;
       ++iterations;
       pushes += near_count;
       const int gen0 = generation + 1;
       // a global iteration claims with one generation; a local one reserves one per workgroup
-      // and round, then one for its chunk pass
-      generation += local ? static_cast<int>(gridDim.x) * kRelaxLocal + 1 : 1;
+      // and round
+      generation += local ? static_cast<int>(gridDim.x) * kRelaxLocal : 1;
       int* next_count = &c->ring_near[step % 3];
       int* chunk_count = &c->ring_chunk[cstep % 3];
+      const int64_t chunk_half = p.chunk_cap / 2;
+      int4* chunk_out = p.chunks + (cstep & 1) * chunk_half;               // this iteration's chunks
+      const int4* chunk_in = p.chunks + ((cstep + 1) & 1) * chunk_half;    // the previous one's
       if (tid == 0) {
         st(&c->ring_near[(step + 1) % 3], 0);
         st(&c->ring_chunk[(cstep + 1) % 3], 0);
@@ -1117,9 +1136,40 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           len = mt.y;
           atomicAdd(relax_edges, static_cast<unsigned long long>(mt.y));
         } else {
-          emit_chunks(p, chunk_count, v, mt.x, mt.y);
+          emit_chunks_to(p, chunk_out, chunk_half, chunk_count, v, mt.x, mt.y);
         }
       };
+      if (pending > 0) {
+        // the previous iteration's chunks: one wave per chunk, grid-wide, then a flush (a tile
+        // may not follow)
+        const near_sink sink{nullptr, 0, nullptr, 0};
+        const int wave = tid / 64;
+        const int waves = threads / 64;
+        for (int q = wave; q < pending; q += waves) {
+          const int4 ch = ld_chunk(&chunk_in[q]);
+          const int32_t u = ch.x;
+          const u64 word = ld(&p.packed[u]);
+          const u64 du = p.pk.distance(word);
+          if (lane() == 0) atomicAdd(relax_edges, static_cast<unsigned long long>(ch.z - ch.y));
+          for (int b = ch.y; b < ch.z; b += 64 * kSlots) {
+            bool act[kSlots];
+            int32_t uu[kSlots];
+            u64 dd[kSlots];
+            int slot[kSlots];
+#pragma unroll
+            for (int j = 0; j < kSlots; ++j) {
+              const int e = b + j * 64 + lane();
+              act[j] = word != packed_inf && e < ch.z;
+              uu[j] = u;
+              dd[j] = du;
+              slot[j] = act[j] ? e : 0;
+            }
+            relax_slots(p, act, uu, dd, slot, threshold, generation, S, next, next_count, far_list, far_counter,
+                        relax_improved, sink);
+          }
+        }
+        stage_flush(S, next, next_count, far_list, far_counter);
+      }
       if (!local) {
         const near_sink sink{nullptr, 0, nullptr, 0};
         tile_rows(
@@ -1153,7 +1203,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
           }
           __syncthreads();
           int q = 0;
-          for (int round = 0; round < kRelaxLocal; ++round) {
+          int spent = 0;  // edges this tile's local rounds expanded
+          for (int round = 0; round < kRelaxLocal && spent < kLocalEdgeBudget; ++round) {
             const int items = W.count[q];  // uniform: LDS after a workgroup barrier
             if (items == 0) break;
             if (round > 0 && threadIdx.x == 0) *local_pushes += static_cast<unsigned long long>(items);
@@ -1177,6 +1228,7 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
                               relax_improved, sink);
                 },
                 [&](bool, int) {});
+            spent += L.start[items];  // the round's edges (LDS after the expansion's barrier: uniform)
             if (threadIdx.x == 0) {
               W.count[q] = 0;
               W.count[q ^ 1] = min(W.count[q ^ 1], kBlock);
@@ -1199,40 +1251,8 @@ __global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) 
         }
       }
       DYNG_GRID_SYNC();
-      const int nchunks = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), p.chunk_cap)));
+      pending = uniform(static_cast<int>(min(static_cast<int64_t>(ld(chunk_count)), chunk_half)));
       ++cstep;
-      if (nchunks > 0) {
-        // the chunk pass claims with the iteration's last generation; in a local iteration its
-        // near vertices enter the next list once, as above
-        const near_sink sink{nullptr, 0, local ? p.ancestor : nullptr, tag};
-        const int wave = tid / 64;
-        const int waves = threads / 64;
-        for (int q = wave; q < nchunks; q += waves) {
-          const int4 ch = ld_chunk(&p.chunks[q]);
-          const int32_t u = ch.x;
-          const u64 word = ld(&p.packed[u]);
-          const u64 du = p.pk.distance(word);
-          if (lane() == 0) atomicAdd(relax_edges, static_cast<unsigned long long>(ch.z - ch.y));
-          for (int b = ch.y; b < ch.z; b += 64 * kSlots) {
-            bool act[kSlots];
-            int32_t uu[kSlots];
-            u64 dd[kSlots];
-            int slot[kSlots];
-#pragma unroll
-            for (int j = 0; j < kSlots; ++j) {
-              const int e = b + j * 64 + lane();
-              act[j] = word != packed_inf && e < ch.z;
-              uu[j] = u;
-              dd[j] = du;
-              slot[j] = act[j] ? e : 0;
-            }
-            relax_slots(p, act, uu, dd, slot, threshold, generation, S, next, next_count, far_list, far_counter,
-                        relax_improved, sink);
-          }
-        }
-        stage_flush(S, next, next_count, far_list, far_counter);
-        DYNG_GRID_SYNC();
-      }
       near_count = uniform(ld(next_count));
       ++step;
       int* t = current;
@@ -1414,8 +1434,10 @@ namespace dyng_hip {
 
 namespace {
 
-/// Chunk-list entries a run over n vertices and a row arena of `slots` slots may need.
-int64_t chunks_for(int64_t n, int64_t slots) { return slots / 16 + n + 4096; }
+/// Chunk-list entries a run over n vertices and a row arena of `slots` slots may need: one pass
+/// over every row in chunks of at least 16 edges takes at most slots / 16 + n; the relax
+/// iterations keep two such lists (one iteration's chunks are relaxed during the next).
+int64_t chunks_for(int64_t n, int64_t slots) { return 2 * (slots / 16 + n + 4096); }
 
 void ws_reserve(dyng_hip_workspace* ws, int64_t n, int64_t slots, hipStream_t s) {
   const int64_t want_chunks = chunks_for(n, slots);

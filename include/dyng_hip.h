@@ -249,7 +249,7 @@ int dyng_hip_sssp_compute(const dyng_hip_sssp_run* run, void* stream, dyng_hip_c
  * DYNG_HIP_NOALLOC_SCRATCH_BYTES per device instead of run->workspace, which is ignored, as are
  * tree_token (the old tree is always packed from its arrays) and tuning.launch (a plain launch).
  * It calls no hipMalloc, hipFree or allocating library routine. A graph that needs more scratch
- * (72 B per vertex plus 1 B per row slot, plus 64 KiB) is refused with DYNG_HIP_INVALID_ARGUMENT
+ * (88 B per vertex plus 2 B per row slot, plus 128 KiB) is refused with DYNG_HIP_INVALID_ARGUMENT
  * before anything runs. Every call on a device shares that device's array, which keeps the last
  * call's contents: calls that overlap on one device race. */
 #define DYNG_HIP_NOALLOC_SCRATCH_BYTES (64ull << 20)

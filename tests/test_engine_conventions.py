@@ -41,7 +41,7 @@ SHARED = ("packed", "stamp", "in_far", "flag", "ancestor", "cand", "frontier", "
           "far_a", "far_b", "touched", "chunks")
 # Local pointers and helper parameters that hold one of the shared arrays.
 ALIASES = ("list", "next", "far", "spill", "out", "far_list", "current", "cur", "front", "listed", "q",
-           "q4", "ch")
+           "chunk_in", "chunk_out", "q4", "ch")
 TREE = ("parents", "distances")
 HELPER = re.compile(r"^(ld|st|ld_chunk|st_chunk|ld_int4|atomic[A-Za-z]+|__hip_atomic_[a-z_]+)$")
 KERNEL_START = "sssp_fused_kernel(engine_params p)"

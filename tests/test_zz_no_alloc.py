@@ -52,7 +52,7 @@ def test_noalloc_update_matches_the_workspace_entry_point_twice(dg):
 def test_noalloc_update_refuses_a_graph_past_its_scratch(dg):
     from dyng_amd import _lib
 
-    n = _lib.NOALLOC_SCRATCH_BYTES // 72 + 1000  # 72 B per vertex alone is past the array
+    n = _lib.NOALLOC_SCRATCH_BYTES // 88 + 1000  # 88 B per vertex alone is past the array
     case = seam.make_case(dg, n, 2 * n, 100, 23)
     dist, par = dg.DeviceArray(case.old_dist), dg.DeviceArray(case.old_parent)
     run = seam.fill_run(dg, case, dist, par)
