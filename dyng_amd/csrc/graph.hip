@@ -287,62 +287,77 @@ __global__ void k_column_summary(const int2* meta, const int32_t* nbr, const int
 // Apply
 // ---------------------------------------------------------------------------------------------
 
-/// Batch scan (before anything changes): id range, self-loops, weight range per objective.
-__global__ void k_batch_scan(const int32_t* ins_src, const int32_t* ins_dst, const int32_t* ins_w,
-                             int64_t num_ins, const int32_t* del_src, const int32_t* del_dst,
-                             int64_t num_del, int K, apply_control* ctl) {
-  __shared__ long long part[kBlock / 64];
+/// Batch scan (before anything changes): id range, self-loops, weight range per objective. The
+/// insertions and deletions are read in one loop (their loads in flight together); each
+/// workgroup reduces its nine values in one pass (wave shuffles, one barrier) and adds them to
+/// the control block with one atomic per value: the launch is 64 workgroups of 1024 threads, so
+/// few workgroups queue on those nine addresses, which share a cache line.
+constexpr int kScanBlock = 1024;
+__global__ void __launch_bounds__(kScanBlock) k_batch_scan(const int32_t* ins_src, const int32_t* ins_dst,
+                                                            const int32_t* ins_w, int64_t num_ins,
+                                                            const int32_t* del_src, const int32_t* del_dst,
+                                                            int64_t num_del, int K, apply_control* ctl) {
   long long mn = INT32_MAX, mx = -1, sl_i = 0, sl_d = 0, wmin = INT32_MAX;
-  long long lmax[4] = {0, 0, 0, 0};
+  long long lmax0 = 0, lmax1 = 0, lmax2 = 0, lmax3 = 0;  // named: a private array would sit in scratch
   const int64_t stride = int64_t{gridDim.x} * blockDim.x;
-  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < num_ins; i += stride) {
-    const int32_t u = ins_src[i], v = ins_dst[i];
-    mn = min(mn, static_cast<long long>(min(u, v)));
-    mx = max(mx, static_cast<long long>(max(u, v)));
-    sl_i += u == v ? 1 : 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (k < K) {
-        const long long x = ins_w[i * K + k];
-        wmin = min(wmin, x);
-        lmax[k] = max(lmax[k], x);
+  const int64_t total = num_ins > num_del ? num_ins : num_del;
+  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < total; i += stride) {
+    const bool ins = i < num_ins, del = i < num_del;
+    const int32_t u = ins ? ins_src[i] : 0, v = ins ? ins_dst[i] : 0;
+    const int32_t du = del ? del_src[i] : 0, dv = del ? del_dst[i] : 0;
+    if (ins) {
+      mn = min(mn, static_cast<long long>(min(u, v)));
+      mx = max(mx, static_cast<long long>(max(u, v)));
+      sl_i += u == v ? 1 : 0;
+      for (int k = 0; k < K; ++k) {
+        const int32_t x = ins_w[i * K + k];
+        wmin = min(wmin, static_cast<long long>(x));
+        if (k == 0) lmax0 = max(lmax0, static_cast<long long>(x));
+        else if (k == 1) lmax1 = max(lmax1, static_cast<long long>(x));
+        else if (k == 2) lmax2 = max(lmax2, static_cast<long long>(x));
+        else if (k == 3) lmax3 = max(lmax3, static_cast<long long>(x));
+        else atomicMax(&ctl->batch_wmax[k], x);
+        if (x < 1) atomicMin(&ctl->batch_wmin[k], x);  // rare: the objective checks are the host's
       }
     }
-    for (int k = 4; k < K; ++k) {
-      const int32_t x = ins_w[i * K + k];
-      wmin = min(wmin, static_cast<long long>(x));
-      atomicMax(&ctl->batch_wmax[k], x);
+    if (del) {
+      mn = min(mn, static_cast<long long>(min(du, dv)));
+      sl_d += du == dv ? 1 : 0;
     }
   }
-  // per-column minimum (rarely below 1; objective checks happen on the host)
-  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < num_ins; i += stride) {
-    for (int k = 0; k < K; ++k) {
-      const int32_t x = ins_w[i * K + k];
-      if (x < 1) atomicMin(&ctl->batch_wmin[k], x);
-    }
-  }
-  for (int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x; i < num_del; i += stride) {
-    const int32_t u = del_src[i], v = del_dst[i];
-    mn = min(mn, static_cast<long long>(min(u, v)));
-    sl_d += u == v ? 1 : 0;
-  }
-  // one atomic per workgroup and counter
-  mn = block_reduce(mn, part, op_min{});
-  mx = block_reduce(mx, part, op_max{});
-  sl_i = block_reduce(sl_i, part, op_sum{});
-  sl_d = block_reduce(sl_d, part, op_sum{});
-  wmin = block_reduce(wmin, part, op_min{});
+  constexpr int kVals = 9;
+  long long val[kVals] = {mn, mx, sl_i, sl_d, wmin, lmax0, lmax1, lmax2, lmax3};
+  auto combine = [](int q, long long a, long long b) -> long long {
+    if (q == 0 || q == 4) return b < a ? b : a;  // minima
+    if (q == 2 || q == 3) return a + b;          // counts
+    return b > a ? b : a;                        // maxima
+  };
 #pragma unroll
-  for (int k = 0; k < 4; ++k) lmax[k] = block_reduce(lmax[k], part, op_max{});
+  for (int q = 0; q < kVals; ++q) {
+    for (int off = 32; off > 0; off >>= 1) val[q] = combine(q, val[q], static_cast<long long>(__shfl_xor(val[q], off)));
+  }
+  __shared__ long long part[kScanBlock / 64][kVals];
+  if (lane() == 0) {
+#pragma unroll
+    for (int q = 0; q < kVals; ++q) part[threadIdx.x / 64][q] = val[q];
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if (mn != INT32_MAX) atomicMin(&ctl->min_id, static_cast<int>(mn));
-    if (mx >= 0) atomicMax(&ctl->max_id, static_cast<int>(mx));
-    if (sl_i) atomicAdd(&ctl->self_loops_ins, static_cast<int>(sl_i));
-    if (sl_d) atomicAdd(&ctl->self_loops_del, static_cast<int>(sl_d));
-    if (wmin != INT32_MAX) atomicMin(&ctl->min_weight, static_cast<int>(wmin));
+    const int waves = static_cast<int>(blockDim.x / 64);
+#pragma unroll
+    for (int q = 0; q < kVals; ++q) {
+      long long t = part[0][q];
+      for (int w = 1; w < waves; ++w) t = combine(q, t, part[w][q]);
+      val[q] = t;
+    }
+    if (val[0] != INT32_MAX) atomicMin(&ctl->min_id, static_cast<int>(val[0]));
+    if (val[1] >= 0) atomicMax(&ctl->max_id, static_cast<int>(val[1]));
+    if (val[2]) atomicAdd(&ctl->self_loops_ins, static_cast<int>(val[2]));
+    if (val[3]) atomicAdd(&ctl->self_loops_del, static_cast<int>(val[3]));
+    if (val[4] != INT32_MAX) atomicMin(&ctl->min_weight, static_cast<int>(val[4]));
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      if (k < K && lmax[k] > 0) atomicMax(&ctl->batch_wmax[k], static_cast<int>(lmax[k]));
+      if (k < K && val[5 + k] > 0) atomicMax(&ctl->batch_wmax[k], static_cast<int>(val[5 + k]));
     }
   }
 }
@@ -1939,8 +1954,8 @@ void scan_batch(dyng_hip_graph* g, const dyng_hip_batch* b, hipStream_t s, batch
   for (int k = 0; k < max_weights; ++k) hc->batch_wmin[k] = 1;
   DYNG_HIP_TRY(hipMemcpyAsync(g->ctl.p, hc, sizeof(apply_control), hipMemcpyHostToDevice, s));
   if (num_ins + num_del > 0) {
-    hipLaunchKernelGGL(k_batch_scan, dim3(std::min<unsigned>(blocks_for(std::max(num_ins, num_del)), 256)),
-                       dim3(kBlock), 0, s, ins_src, ins_dst, ins_w, num_ins, del_src, del_dst, num_del, K, g->ctl.p);
+    hipLaunchKernelGGL(k_batch_scan, dim3(std::min<unsigned>(blocks_for(std::max(num_ins, num_del), kScanBlock), 64)),
+                       dim3(kScanBlock), 0, s, ins_src, ins_dst, ins_w, num_ins, del_src, del_dst, num_del, K, g->ctl.p);
     DYNG_HIP_TRY(hipGetLastError());
   }
   mark_stage(g, 1, s);
