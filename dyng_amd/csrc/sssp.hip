@@ -586,7 +586,12 @@ __global__ void k_reset_barrier(grid_sync_state* bar, control* ctl) {
   }
 }
 
-__global__ void __launch_bounds__(kBlock, 8) sssp_fused_kernel(engine_params p) {
+/// Two builds of the persistent kernel: kWaves = 8 waves per SIMD keeps it at 64 VGPRs, so two
+/// 1024-thread workgroups fit a CU (solves from the sources), at the price of 17 VGPRs spilled to
+/// scratch; kWaves = 4 gives it 128 (it takes 85, nothing spilled) for grids of one workgroup per
+/// CU (repairs).
+template <int kWaves>
+__global__ void __launch_bounds__(kBlock, kWaves) sssp_fused_kernel(engine_params p) {
   // queued right behind an apply whose merges were held back: nothing to do (uniform)
   if (p.hold != nullptr && *p.hold != 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) p.ctl->held = 1;
@@ -1414,7 +1419,8 @@ struct dyng_hip_workspace {
   dyng_hip::dbuf<dyng_hip::grid_sync_state> bar;
   int grid_blocks = 0;
   int cu_count = 0;
-  int occupancy = 0;
+  int occupancy = 0;         ///< workgroups per CU of the 64-VGPR build
+  int occupancy_narrow = 0;  ///< and of the 128-VGPR build
   hipEvent_t ev_start = nullptr;
   hipEvent_t ev_stop = nullptr;
   /// the launch as a hipGraph (control upload, barrier reset, events, the kernel, control
@@ -1422,6 +1428,7 @@ struct dyng_hip_workspace {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   hipGraphNode_t graph_kernel = nullptr;
+  bool graph_narrow = false;  ///< which build the graph's kernel node launches
   ~dyng_hip_workspace() {
     if (graph_exec != nullptr) (void)hipGraphExecDestroy(graph_exec);
     if (graph != nullptr) (void)hipGraphDestroy(graph);
@@ -1433,6 +1440,15 @@ struct dyng_hip_workspace {
 namespace dyng_hip {
 
 namespace {
+
+/// The build of the persistent kernel a grid takes: the 128-VGPR one when the grid fits at its
+/// occupancy (one workgroup per CU), else the 64-VGPR one.
+const void* fused_kernel(bool narrow) {
+  return narrow ? reinterpret_cast<const void*>(sssp_fused_kernel<4>) : reinterpret_cast<const void*>(sssp_fused_kernel<8>);
+}
+bool narrow_grid(int blocks, int occupancy_narrow, int cu_count) {
+  return occupancy_narrow >= 1 && blocks <= occupancy_narrow * cu_count;
+}
 
 /// Chunk-list entries a run over n vertices and a row arena of `slots` slots may need: one pass
 /// over every row in chunks of at least 16 edges takes at most slots / 16 + n; the relax
@@ -1471,9 +1487,8 @@ void ws_reserve(dyng_hip_workspace* ws, int64_t n, int64_t slots, hipStream_t s)
   ws->bar.reserve(1);
   if (ws->cu_count == 0) {
     DYNG_HIP_TRY(hipDeviceGetAttribute(&ws->cu_count, hipDeviceAttributeMultiprocessorCount, ws->device));
-    DYNG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ws->occupancy,
-                                                              reinterpret_cast<const void*>(sssp_fused_kernel),
-                                                              kBlock, 0));
+    DYNG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ws->occupancy, fused_kernel(false), kBlock, 0));
+    DYNG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&ws->occupancy_narrow, fused_kernel(true), kBlock, 0));
     if (ws->occupancy < 1) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: sssp: the persistent kernel does not fit on a CU");
   }
 }
@@ -1721,6 +1736,7 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
                                    ws->frontier.p, ws->near_a.p, ws->near_b.p, ws->far_a.p, ws->far_b.p,
                                    ws->touched.p, ws->chunks.p, ws->chunk_cap, ws->ctl.p, ws->bar.p});
   const int blocks = grid_blocks(run->tuning, ws->occupancy, ws->cu_count, p.from_scratch != 0);
+  const bool narrow = narrow_grid(blocks, ws->occupancy_narrow, ws->cu_count);
   control* hc = ws->host_ctl.p;
   std::memset(hc, 0, sizeof(control));
   hc->min_frontier = packed_inf;
@@ -1733,14 +1749,22 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
   void* args[] = {&p};
   if (run->tuning.launch == DYNG_HIP_LAUNCH_GRAPH) {
     hipKernelNodeParams kp{};
-    kp.func = reinterpret_cast<void*>(sssp_fused_kernel);
+    kp.func = const_cast<void*>(fused_kernel(narrow));
     kp.gridDim = dim3(blocks);
     kp.blockDim = dim3(kBlock);
     kp.sharedMemBytes = 0;
     kp.kernelParams = args;
     kp.extra = nullptr;
+    if (ws->graph_exec != nullptr && ws->graph_narrow != narrow) {
+      // the other build of the kernel: the graph is built again around it
+      DYNG_HIP_TRY(hipGraphExecDestroy(ws->graph_exec));
+      DYNG_HIP_TRY(hipGraphDestroy(ws->graph));
+      ws->graph_exec = nullptr;
+      ws->graph = nullptr;
+    }
     if (ws->graph_exec == nullptr) {
       // built once per workspace: the control block, the barrier and the events never move
+      ws->graph_narrow = narrow;
       DYNG_HIP_TRY(hipGraphCreate(&ws->graph, 0));
       hipGraphNode_t reset, start, stop, down;
       grid_sync_state* bar = ws->bar.p;
@@ -1765,7 +1789,11 @@ engine_pending engine_launch(const dyng_hip_sssp_run* run, bool update, hipStrea
   } else {
     hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, s, ws->bar.p, ws->ctl.p);
     DYNG_HIP_TRY(hipEventRecord(ws->ev_start, s));
-    hipLaunchKernelGGL(sssp_fused_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
+    if (narrow) {
+      hipLaunchKernelGGL(sssp_fused_kernel<4>, dim3(blocks), dim3(kBlock), 0, s, p);
+    } else {
+      hipLaunchKernelGGL(sssp_fused_kernel<8>, dim3(blocks), dim3(kBlock), 0, s, p);
+    }
     DYNG_HIP_TRY(hipGetLastError());
     DYNG_HIP_TRY(hipEventRecord(ws->ev_stop, s));
     DYNG_HIP_TRY(hipMemcpyAsync(hc, ws->ctl.p, sizeof(control), hipMemcpyDeviceToHost, s));
@@ -1911,6 +1939,7 @@ struct noalloc_device {
   int64_t last_n = 0;  ///< the vertex count the stamps, far marks and flags were laid out for
   int cu_count = 0;
   int occupancy = 0;
+  int occupancy_narrow = 0;
   control host_ctl{};
 };
 
@@ -1937,9 +1966,8 @@ noalloc_device& noalloc_for(int device, hipStream_t s) {
                                      hipGetErrorString(e));
   }
   DYNG_HIP_TRY(hipDeviceGetAttribute(&st.cu_count, hipDeviceAttributeMultiprocessorCount, device));
-  DYNG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&st.occupancy,
-                                                            reinterpret_cast<const void*>(sssp_fused_kernel), kBlock,
-                                                            0));
+  DYNG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&st.occupancy, fused_kernel(false), kBlock, 0));
+  DYNG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&st.occupancy_narrow, fused_kernel(true), kBlock, 0));
   if (st.occupancy < 1) fail(DYNG_HIP_NOT_SUPPORTED, "dyng: sssp: the persistent kernel does not fit on a CU");
   clear_scratch(ptr, DYNG_HIP_NOALLOC_SCRATCH_BYTES, s);
   DYNG_HIP_TRY(hipStreamSynchronize(s));
@@ -2000,7 +2028,11 @@ void run_engine_noalloc(const dyng_hip_sssp_run* run, hipStream_t s, dyng_hip_co
                                    reinterpret_cast<grid_sync_state*>(base + L.bar)});
   const int blocks = grid_blocks(run->tuning, st.occupancy, st.cu_count, p.from_scratch != 0);
   hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, s, p.bar, p.ctl);
-  hipLaunchKernelGGL(sssp_fused_kernel, dim3(blocks), dim3(kBlock), 0, s, p);
+  if (narrow_grid(blocks, st.occupancy_narrow, st.cu_count)) {
+    hipLaunchKernelGGL(sssp_fused_kernel<4>, dim3(blocks), dim3(kBlock), 0, s, p);
+  } else {
+    hipLaunchKernelGGL(sssp_fused_kernel<8>, dim3(blocks), dim3(kBlock), 0, s, p);
+  }
   DYNG_HIP_TRY(hipGetLastError());
   control* hc = &st.host_ctl;
   DYNG_HIP_TRY(hipMemcpyAsync(hc, p.ctl, sizeof(control), hipMemcpyDeviceToHost, s));
