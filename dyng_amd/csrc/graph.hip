@@ -565,8 +565,9 @@ DYNG_DEV void mark_row_range(const u64* keys, int64_t i, int64_t N, int vbits, i
   if (i + 1 == N || (keys[i + 1] >> vbits) != r) row_b[r] = static_cast<int32_t>(i + 1);
 }
 
-__global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
-  const int64_t i = blockIdx.x * int64_t{blockDim.x} + threadIdx.x;
+/// The out-row pass of thread i (every thread of the workgroup calls it: it ends with the
+/// workgroup's counter reduction).
+DYNG_DEV void out_pairs(const out_pair_params& p, int64_t i) {
   unsigned long long c_ins = 0, c_upd = 0, c_del = 0, c_ign_del = 0, c_ign_ins = 0;
   // weight sum and count-at-maximum deltas of columns 0..3 (named, not an array: a private
   // array indexed by a runtime column would live in scratch memory)
@@ -792,7 +793,14 @@ __global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
   }
 }
 
+__global__ void __launch_bounds__(kBlock) k_out_pairs(out_pair_params p) {
+  out_pairs(p, blockIdx.x * int64_t{blockDim.x} + threadIdx.x);
+}
+
 struct in_pair_params {
+  op_layout L;
+  const int32_t* ins_w;  ///< batch insertion weights (j * K + k), for the simple graphs' own replay
+  int upsert;            ///< on_existing_insert: 1 upsert, 0 ignore
   const u64* keys;  ///< sorted in-operation keys (v << b | u)
   const uint32_t* vals;  ///< out-operation position of each
   int64_t N;
@@ -900,6 +908,104 @@ __global__ void __launch_bounds__(kBlock) k_in_pairs(in_pair_params p) {
   const bool first_touch = structural && atomicExch(&p.mark[v], p.mark_gen) != p.mark_gen;
   const int t = wave_append(&p.ctl->touched_in, first_touch);
   if (first_touch) p.touched[t] = v;
+}
+
+/// The in-row pass of a simple graph, thread i. A simple graph holds at most one alive (u, v), and
+/// its in-row entry (v, u) carries the same weights, so replaying the segment's operations in
+/// batch order on the in-row as the out-row pass replays them on the out-row (the first alive old
+/// entry, else the first alive pending one; deletions remove it, insertions upsert or append)
+/// reaches the same decisions and weights without that pass's results: both passes run in one
+/// launch (k_pairs_simple). With parallel edges the copy an operation takes is matched by weight
+/// to the out-row's (k_in_pairs, after k_out_pairs).
+DYNG_DEV void in_pairs_simple(const in_pair_params& p, int64_t i) {
+  if (i == p.N) {
+    p.pend_alive[i] = 0;
+    p.kill[i] = 0;
+  }
+  if (i >= p.N) return;
+  const u64 key = p.keys[i];
+  if (key == kNoKey) {
+    p.pend_alive[i] = 0;
+    p.kill[i] = 0;
+    return;
+  }
+  mark_row_range(p.keys, i, p.N, p.vbits, p.row_a, p.row_b);
+  if (i > 0 && p.keys[i - 1] == key) return;
+  int64_t j = i + 1;
+  while (j < p.N && p.keys[j] == key) ++j;
+  const u64 mask = (1ULL << p.vbits) - 1;
+  const int32_t v = static_cast<int32_t>(key >> p.vbits);
+  const int32_t u = static_cast<int32_t>(key & mask);
+  const int2 mt = p.meta[v];
+  const int32_t* row = p.nbr + mt.x;
+  const run_pair rr = find_runs(row, p.base[v], mt.y, u);
+  const int K = p.K;
+  auto first_alive = [&]() -> int {
+    for (int x = rr.lb; x < rr.ub; ++x) {
+      if (row[x] >= 0) return x;
+    }
+    for (int x = rr.tl; x < rr.tu; ++x) {
+      if (row[x] >= 0) return x;
+    }
+    return -1;
+  };
+  bool structural = false;
+  for (int64_t q = i; q < j; ++q) {
+    p.pend_alive[q] = 0;
+    p.kill[q] = 0;
+    const int64_t e = p.vals[q];
+    const int old_hit = first_alive();
+    int64_t pend_hit = -1;
+    if (old_hit < 0) {
+      for (int64_t x = i; x < q; ++x) {
+        if (p.pend_alive[x]) {
+          pend_hit = x;
+          break;
+        }
+      }
+    }
+    if (p.L.is_del(e)) {
+      if (old_hit >= 0) {
+        p.nbr[mt.x + old_hit] = u | kDead;
+        if (old_hit < rr.bl) {
+          atomicAdd(&p.dead[v], 1);
+        } else {
+          p.kill[q] = 1;  // a tail entry
+        }
+        structural = true;
+      } else if (pend_hit >= 0) {
+        p.pend_alive[pend_hit] = 0;
+      }
+    } else {
+      const int64_t bj = p.L.local(e) / p.L.stride;
+      if (old_hit >= 0 || pend_hit >= 0) {
+        if (p.upsert) {
+          for (int k = 0; k < K; ++k) {
+            const int32_t wn = p.ins_w[bj * K + k];
+            if (old_hit >= 0) p.w[k * p.slot_cap + mt.x + old_hit] = wn; else p.pend_w[k * p.N + pend_hit] = wn;
+          }
+        }
+      } else {
+        p.pend_alive[q] = 1;
+        for (int k = 0; k < K; ++k) p.pend_w[k * p.N + q] = p.ins_w[bj * K + k];
+        structural = true;
+      }
+    }
+  }
+  const bool first_touch = structural && atomicExch(&p.mark[v], p.mark_gen) != p.mark_gen;
+  const int t = wave_append(&p.ctl->touched_in, first_touch);
+  if (first_touch) p.touched[t] = v;
+}
+
+/// Both pairing passes of a simple graph in one launch: workgroups [0, split) the out-rows,
+/// the rest the in-rows (their own replay, independent of the out-row pass).
+__global__ void __launch_bounds__(kBlock) k_pairs_simple(out_pair_params po, in_pair_params pi, int split) {
+  const int b = static_cast<int>(blockIdx.x);
+  if (b < split) {
+    out_pairs(po, b * int64_t{blockDim.x} + threadIdx.x);
+  } else {
+    in_pairs_simple(pi, (b - split) * int64_t{blockDim.x} + threadIdx.x);
+  }
 }
 
 struct merge_params {
@@ -2327,13 +2433,11 @@ apply_pending apply_launch(dyng_hip_graph* g, const batch_scan& sc, hipStream_t 
     op.row_a = g->row_a_out.p;
     op.row_b = g->row_b_out.p;
     op.ctl = dc;
-    hipLaunchKernelGGL(k_out_pairs, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, op);
-    DYNG_HIP_TRY(hipGetLastError());
-    mark_stage(g, 5, s);
-    mark_stage(g, 6, s);  // the out-rows' prefixes are scanned with the in-rows' below
     // ---- the in-rows' operations (they do not read the out-rows) ----
-    mark_stage(g, 7, s);
     in_pair_params ip{};
+    ip.L = L;
+    ip.ins_w = ins_w;
+    ip.upsert = op.upsert;
     ip.keys = g->keys_b.p + N;
     ip.vals = g->vals_b.p + N;
     ip.N = N;
@@ -2357,8 +2461,24 @@ apply_pending apply_launch(dyng_hip_graph* g, const batch_scan& sc, hipStream_t 
     ip.row_a = g->row_a_in.p;
     ip.row_b = g->row_b_in.p;
     ip.ctl = dc;
-    hipLaunchKernelGGL(k_in_pairs, dim3(blocks_for(N + 1)), dim3(kBlock), 0, s, ip);
-    DYNG_HIP_TRY(hipGetLastError());
+    const unsigned pair_groups = blocks_for(N + 1);
+    if (!g->props.allow_parallel) {
+      // a simple graph: the in-row pass replays the operations itself, in the same launch
+      mark_stage(g, 5, s);
+      hipLaunchKernelGGL(k_pairs_simple, dim3(2 * pair_groups), dim3(kBlock), 0, s, op, ip,
+                         static_cast<int>(pair_groups));
+      DYNG_HIP_TRY(hipGetLastError());
+      mark_stage(g, 6, s);
+      mark_stage(g, 7, s);
+    } else {
+      mark_stage(g, 5, s);
+      hipLaunchKernelGGL(k_out_pairs, dim3(pair_groups), dim3(kBlock), 0, s, op);
+      DYNG_HIP_TRY(hipGetLastError());
+      mark_stage(g, 6, s);
+      mark_stage(g, 7, s);
+      hipLaunchKernelGGL(k_in_pairs, dim3(pair_groups), dim3(kBlock), 0, s, ip);
+      DYNG_HIP_TRY(hipGetLastError());
+    }
     mark_stage(g, 8, s);
     exclusive_scan4(g->scan_tmp, g->pend_alive.p, g->kill.p, g->ipend_alive.p, g->ikill.p, g->pend_prefix.p,
                     g->kill_prefix.p, g->ipend_prefix.p, g->ikill_prefix.p, N + 1, s);
