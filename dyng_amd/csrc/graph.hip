@@ -97,26 +97,143 @@ void exclusive_scan(dbuf<unsigned char>& tmp, const T* in, T* out, size_t n, hip
                                        rocprim::plus<T>(), s));
 }
 
-/// The exclusive prefix sums of four int32 arrays of n entries in one scan (the pending and
-/// killed flags of both directions' operations).
-struct quad_plus {
-  using quad = rocprim::tuple<int32_t, int32_t, int32_t, int32_t>;
-  __device__ quad operator()(const quad& a, const quad& b) const {
-    return quad(rocprim::get<0>(a) + rocprim::get<0>(b), rocprim::get<1>(a) + rocprim::get<1>(b),
-                rocprim::get<2>(a) + rocprim::get<2>(b), rocprim::get<3>(a) + rocprim::get<3>(b));
-  }
+/// The exclusive prefix sums of four int32 arrays of n entries (the pending and killed flags of
+/// both directions' operations), in two launches: the sums of tiles of kScan4Tile entries, then
+/// each tile scanned from the sum of the tiles before it (each workgroup adds those up itself).
+/// No chain of look-backs between workgroups: on a few hundred thousand entries a decoupled
+/// look-back scan is a serial chain of device-scope round trips (rocPRIM's took 16 us here, its
+/// state initialisation included).
+constexpr int kScan4Block = 1024;
+constexpr int kScan4Items = 4;
+constexpr int kScan4Tile = kScan4Block * kScan4Items;
+
+struct scan4_args {
+  const int32_t* in[4];
+  int32_t* out[4];
+  int64_t n;
+  int4* tile_sums;
 };
+
+/// Sum over the workgroup of one int4 per thread; thread 0 gets it.
+DYNG_DEV int4 block_sum4(int4 v, int4* part) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_xor(v.x, off);
+    v.y += __shfl_xor(v.y, off);
+    v.z += __shfl_xor(v.z, off);
+    v.w += __shfl_xor(v.w, off);
+  }
+  if (lane() == 0) part[threadIdx.x / 64] = v;
+  __syncthreads();
+  int4 t = make_int4(0, 0, 0, 0);
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < static_cast<int>(blockDim.x / 64); ++w) {
+      t.x += part[w].x;
+      t.y += part[w].y;
+      t.z += part[w].z;
+      t.w += part[w].w;
+    }
+  }
+  return t;
+}
+
+__global__ void __launch_bounds__(kScan4Block) k_scan4_tiles(scan4_args a) {
+  __shared__ int4 part[kScan4Block / 64];
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kScan4Tile;
+  int4 v = make_int4(0, 0, 0, 0);
+  for (int j = 0; j < kScan4Items; ++j) {
+    const int64_t i = t0 + static_cast<int64_t>(j) * kScan4Block + threadIdx.x;
+    if (i < a.n) {
+      v.x += a.in[0][i];
+      v.y += a.in[1][i];
+      v.z += a.in[2][i];
+      v.w += a.in[3][i];
+    }
+  }
+  const int4 t = block_sum4(v, part);
+  if (threadIdx.x == 0) a.tile_sums[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kScan4Block) k_scan4_apply(scan4_args a) {
+  __shared__ int4 part[kScan4Block / 64];
+  __shared__ int4 wave_sum[kScan4Block / 64];
+  __shared__ int4 base;
+  // the sum of the tiles before this one
+  int4 v = make_int4(0, 0, 0, 0);
+  for (int b = threadIdx.x; b < static_cast<int>(blockIdx.x); b += blockDim.x) {
+    const int4 x = a.tile_sums[b];
+    v.x += x.x;
+    v.y += x.y;
+    v.z += x.z;
+    v.w += x.w;
+  }
+  const int4 before = block_sum4(v, part);
+  if (threadIdx.x == 0) base = before;
+  // this thread's kScan4Items consecutive entries, their sums, then the workgroup's exclusive scan
+  const int64_t first = static_cast<int64_t>(blockIdx.x) * kScan4Tile + static_cast<int64_t>(threadIdx.x) * kScan4Items;
+  int32_t x[4][kScan4Items];
+  int4 own = make_int4(0, 0, 0, 0);
+#pragma unroll
+  for (int j = 0; j < kScan4Items; ++j) {
+    const int64_t i = first + j;
+    const bool in = i < a.n;
+    x[0][j] = in ? a.in[0][i] : 0;
+    x[1][j] = in ? a.in[1][i] : 0;
+    x[2][j] = in ? a.in[2][i] : 0;
+    x[3][j] = in ? a.in[3][i] : 0;
+    own.x += x[0][j];
+    own.y += x[1][j];
+    own.z += x[2][j];
+    own.w += x[3][j];
+  }
+  int4 inc = own;  // inclusive scan over the wave
+  for (int off = 1; off < 64; off <<= 1) {
+    const int4 y = make_int4(__shfl_up(inc.x, off), __shfl_up(inc.y, off), __shfl_up(inc.z, off), __shfl_up(inc.w, off));
+    if (lane() >= off) {
+      inc.x += y.x;
+      inc.y += y.y;
+      inc.z += y.z;
+      inc.w += y.w;
+    }
+  }
+  const int wv = static_cast<int>(threadIdx.x / 64);
+  if (lane() == 63) wave_sum[wv] = inc;
+  __syncthreads();  // wave sums and base are in LDS
+  int4 run = base;
+  for (int w = 0; w < wv; ++w) {
+    run.x += wave_sum[w].x;
+    run.y += wave_sum[w].y;
+    run.z += wave_sum[w].z;
+    run.w += wave_sum[w].w;
+  }
+  run.x += inc.x - own.x;
+  run.y += inc.y - own.y;
+  run.z += inc.z - own.z;
+  run.w += inc.w - own.w;
+#pragma unroll
+  for (int j = 0; j < kScan4Items; ++j) {
+    const int64_t i = first + j;
+    if (i < a.n) {
+      a.out[0][i] = run.x;
+      a.out[1][i] = run.y;
+      a.out[2][i] = run.z;
+      a.out[3][i] = run.w;
+    }
+    run.x += x[0][j];
+    run.y += x[1][j];
+    run.z += x[2][j];
+    run.w += x[3][j];
+  }
+}
 
 void exclusive_scan4(dbuf<unsigned char>& tmp, const int32_t* a, const int32_t* b, const int32_t* c, const int32_t* d,
                      int32_t* oa, int32_t* ob, int32_t* oc, int32_t* od, size_t n, hipStream_t s) {
   if (n == 0) return;
-  auto in = rocprim::make_zip_iterator(rocprim::make_tuple(a, b, c, d));
-  auto out = rocprim::make_zip_iterator(rocprim::make_tuple(oa, ob, oc, od));
-  const quad_plus::quad zero(0, 0, 0, 0);
-  size_t bytes = 0;
-  DYNG_HIP_TRY(rocprim::exclusive_scan(nullptr, bytes, in, out, zero, n, quad_plus(), s));
-  tmp.reserve(bytes);
-  DYNG_HIP_TRY(rocprim::exclusive_scan(static_cast<void*>(tmp.p), bytes, in, out, zero, n, quad_plus(), s));
+  const int64_t tiles = (static_cast<int64_t>(n) + kScan4Tile - 1) / kScan4Tile;
+  tmp.reserve(static_cast<size_t>(tiles) * sizeof(int4));
+  scan4_args args{{a, b, c, d}, {oa, ob, oc, od}, static_cast<int64_t>(n), reinterpret_cast<int4*>(tmp.p)};
+  hipLaunchKernelGGL(k_scan4_tiles, dim3(static_cast<unsigned>(tiles)), dim3(kScan4Block), 0, s, args);
+  hipLaunchKernelGGL(k_scan4_apply, dim3(static_cast<unsigned>(tiles)), dim3(kScan4Block), 0, s, args);
+  DYNG_HIP_TRY(hipGetLastError());
 }
 
 template <typename T>
