@@ -104,7 +104,7 @@ void exclusive_scan(dbuf<unsigned char>& tmp, const T* in, T* out, size_t n, hip
 /// look-back scan is a serial chain of device-scope round trips (rocPRIM's took 16 us here, its
 /// state initialisation included).
 constexpr int kScan4Block = 1024;
-constexpr int kScan4Items = 4;
+constexpr int kScan4Items = 1;  ///< entries per thread: tiles of 1024 (four per thread measured 5 us slower)
 constexpr int kScan4Tile = kScan4Block * kScan4Items;
 
 struct scan4_args {

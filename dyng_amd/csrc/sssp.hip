@@ -50,7 +50,7 @@ constexpr int kWalkLocal = 16;   ///< later rounds: levels a workgroup walks on 
 constexpr int kWalkRounds = kWalkGlobal + 8;  ///< grid rounds of the walk before pointer jumping takes over
 constexpr int kRelaxGlobal = 2;  ///< relax iterations of a repair before local rounds may start
 constexpr int kRelaxLocal = 16;  ///< local rounds of a workgroup in one relax iteration
-constexpr int kRelaxNarrow = 64; ///< a near frontier of at most this many vertices per workgroup is narrow
+constexpr int kRelaxNarrow = 128; ///< a near frontier of at most this many vertices per workgroup is narrow
 /// A workgroup's local levels of one walk round or relax iteration stop once they have expanded
 /// this many edges; what they claimed next goes to the next grid round, spread over the grid (a
 /// local level that reaches a bushy part of the graph would otherwise keep the grid waiting on one
