@@ -326,14 +326,17 @@ using namespace dyng_hip;
 /// The network's state on one device: FP32 parameters, their BF16 copy, Adam's moments and the
 /// scratch of a training minibatch.
 /// The schedule to run from the predicted log2 engine times: the plain one (0) unless another is
-/// predicted at least kSwitchMargin faster (a misprediction costs more than the small gains the
-/// other schedules offer on most batches).
+/// predicted at least kSwitchMargin faster and at least kMinGainMs sooner (a misprediction costs
+/// more than the small gains the other schedules offer on most batches, and a predicted saving
+/// below one run's timing noise is no evidence: a repair of tens of microseconds stays plain).
 constexpr float kSwitchMargin = 0.2016f;  ///< log2(1.15): a predicted gain of at least 15 %
+constexpr float kMinGainMs = 0.02f;       ///< and of at least 20 us
 
 int pick_schedule(const float* y) {
   int best = 0;
   for (int o = 1; o < kOut; ++o) best = y[o] < y[best] ? o : best;
-  return (best != 0 && y[best] < y[0] - kSwitchMargin) ? best : 0;
+  const bool gain = y[best] < y[0] - kSwitchMargin && std::exp2(y[0]) - std::exp2(y[best]) >= kMinGainMs;
+  return (best != 0 && gain) ? best : 0;
 }
 
 struct dyng_hip_mlp {
