@@ -1589,9 +1589,10 @@ void engine_setup(const dyng_hip_sssp_run* run, bool update, engine_params& p, i
   p.max_rounds = rounds + 1;
   p.distances = reinterpret_cast<long long*>(run->distances);
   p.parents = run->parents;
-  // rows up to 8192 edges in the tiles (4096 measured 6 % slower on RMAT-22 repairs: more rows
-  // waited a relax iteration as chunks)
-  p.light = run->tuning.light_degree > 0 ? run->tuning.light_degree : 8192;
+  // rows up to 8192 edges in a repair's tiles (4096 measured 6 % slower on RMAT-22 repairs: more
+  // rows waited a relax iteration as chunks), up to 4096 in a solve's (its wide frontiers balance
+  // better with the long rows in chunks: 8192 measured 5.9 -> 8.7 ms on RMAT-22)
+  p.light = run->tuning.light_degree > 0 ? run->tuning.light_degree : (p.from_scratch ? 4096 : 8192);
   p.chunk_edges = run->tuning.chunk_edges > 0 ? std::max(16, run->tuning.chunk_edges) : 256;
   // the real-time counter runs at 100 MHz: 100 ticks per microsecond
   timeout_us = run->tuning.barrier_timeout_us > 0   ? run->tuning.barrier_timeout_us

@@ -174,7 +174,7 @@ typedef struct dyng_hip_tuning {
   int32_t blocks_per_cu;   /* workgroups per CU (0: all that fit for a solve from the sources, one
                               for a repair or when the library overlaps two runs on the device); a
                               caller that overlaps its own runs on one device asks for 1 */
-  int32_t light_degree;    /* rows up to this length: edge-balanced inside a workgroup (0: 8192) */
+  int32_t light_degree;    /* rows up to this length: edge-balanced inside a workgroup (0: 8192 in a repair, 4096 in a solve) */
   int32_t chunk_edges;     /* longer rows: split in chunks of this many edges, one wave each (0: 256) */
   int32_t launch;             /* DYNG_HIP_LAUNCH_*: a plain launch, or a hipGraph replayed per run */
   int32_t recompute;          /* update: recompute the tree from the source(s) instead of repairing it
