@@ -1053,33 +1053,11 @@ __global__ void __launch_bounds__(kBlock, kWaves) sssp_fused_kernel(engine_param
   // indexed private array lives in scratch)
   auto far_of = [&](int b) { return b ? p.far_b : p.far_a; };
   int fb = 0;
-  ++generation;
-  {
-    int* near_count = &c->ring_near[step % 3];
-    if (tid == 0) st(&c->ring_near[(step + 1) % 3], 0);
-    for (int base = tid - lane(); base < frontier_count; base += threads) {
-      const int i = base + lane();
-      int32_t v = -1;
-      bool to_near = false, to_far = false, first = false;
-      if (i < frontier_count) {
-        v = ld(&p.frontier[i]);
-        const u64 word = ld(&p.packed[v]);
-        const u64 d = word == packed_inf ? packed_inf : p.pk.distance(word);
-        // both flags as values (a store through a pointer chosen by the branch would keep them
-        // in scratch memory)
-        const bool near = d < threshold;
-        to_near = near && claim_first(p, v, generation, first);
-        to_far = !near && atomicExch(&p.in_far[v], 1) == 0;
-      }
-      stage_append(S, 0, to_near, v, current, near_count);
-      stage_append(S, 1, to_far, v, far_of(fb), &c->far_count[fb]);
-      stage_append(S, 2, first, v, S.list2, S.counter2);
-    }
-    stage_flush(S, current, near_count, far_of(fb), &c->far_count[fb]);
-    DYNG_GRID_SYNC();
-  }
-  int near_count = uniform(ld(&c->ring_near[step % 3]));
-  ++step;
+  // The pull's frontier is the first relax iteration's list (each vertex once, claimed and
+  // stamped by the pull): a vertex at or past the threshold goes to the far pile when its row
+  // would be loaded, so no separate split pass and grid barrier come before the first iteration.
+  const int* seed = p.frontier;
+  int near_count = frontier_count;
   int iterations = 0, epochs = 0;
   long long pushes = 0;
   int pending = 0;  // chunks of long rows the previous relax iteration left for the next one
@@ -1130,12 +1108,19 @@ __global__ void __launch_bounds__(kBlock, kWaves) sssp_fused_kernel(engine_param
       }
       int* far_list = far_of(fb);
       int* far_counter = &c->far_count[fb];
-      const int* cur = current;
+      const bool seeding = seed != nullptr;
+      const int* cur = seeding ? seed : current;
+      seed = nullptr;
       const int tag = local ? -2 - generation : 0;  // a local iteration's list tag: below every vertex id and -1
       auto load_row = [&](int32_t v, int32_t& off, int32_t& len, u64& word) {
         const int2 mt = p.out_meta[v];  // issued together with the word's load
         word = ld(&p.packed[v]);
         if (word == packed_inf) return;
+        if (seeding && p.pk.distance(word) >= threshold) {
+          // a vertex of the pull's frontier past the threshold waits in the far pile
+          stage_append(S, 1, atomicExch(&p.in_far[v], 1) == 0, v, far_list, far_counter);
+          return;
+        }
         if (mt.y <= p.light) {
           off = mt.x;
           len = mt.y;
