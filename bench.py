@@ -706,7 +706,7 @@ def hyper_bench(args) -> int:
                    "line_graph_changes_p50": int(statistics.median(lg_changes)),
                    "device": "sssp.update on the line graph: the device apply of its edge batch and the "
                              "persistent repair",
-                   "host": "the hypergraph's bookkeeping and the line graph's edge batch (Python)"},
+                   "host": "the hypergraph's bookkeeping and the line graph's edge batch (the C++ container, called from Python)"},
         "parity": {"bit_exact": exact, "against": "hyper_sssp.compute on the final hypergraph"},
     }
     print(json.dumps(out), flush=True)

@@ -7,6 +7,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <limits>
 #include <climits>
 #include <cstring>
 #include <map>
@@ -359,8 +360,17 @@ struct dyng_hip_hypergraph {
   std::vector<std::vector<int32_t>> pre_rows;
   std::vector<int64_t> pre_w;
   // scratch and the arrays the last query returned (valid until the next call on the handle)
-  std::vector<int64_t> mark_a, mark_b;
-  int64_t stamp = 0;
+  std::vector<int64_t> mark_a, mark_b, mark_t;
+  int64_t stamp = 0;  ///< marks below it are stale: each query takes fresh values above it
+  std::vector<std::pair<int32_t, int64_t>> pre_pairs;  ///< (vertex, touched id) before the apply
+
+  /// The three mark arrays at least b long; entries past their old length start stale.
+  void marks_for(int64_t b) {
+    const size_t need = static_cast<size_t>(std::max<int64_t>(b, 1));
+    if (mark_a.size() < need) mark_a.resize(need, -1);
+    if (mark_b.size() < need) mark_b.resize(need, -1);
+    if (mark_t.size() < need) mark_t.resize(need, -1);
+  }
   std::vector<int32_t> out_src, out_dst, out_w, out_del_src, out_del_dst;
   std::vector<int64_t> out_offsets;
   std::vector<int32_t> out_members;
@@ -581,10 +591,29 @@ extern "C" int dyng_hip_hypergraph_apply(dyng_hip_hypergraph* h, const dyng_hip_
       s = e;
     }
     const int64_t skipped_edits = nrem + nadd - removed - added;
-    // ---- the store phase in the order of id_reuse; the allocator commits only if every step does
-    dyng_hip::allocator alloc = h->alloc;
+    // ---- the store phase in the order of id_reuse; the allocator commits only if every step does.
+    // Its steps fail only when a payload would pass the policy's limits (a payload it may not
+    // grow, or one past max_payload or the int32 offsets). No step of one apply asks for more than
+    // the payload before it, again (a row relocated whole), plus four slots per value of the batch
+    // and a row's padding: when a growing policy has room for twice the payload, the capacity and
+    // that, the steps run
+    // in place; otherwise on a copy committed at the end, so a failed apply changes nothing.
     std::vector<int32_t> counts(rows_in.size());
-    for (size_t i = 0; i < rows_in.size(); ++i) counts[i] = static_cast<int32_t>(rows_in[i].size());
+    int64_t batch_values = 0;
+    for (size_t i = 0; i < rows_in.size(); ++i) {
+      counts[i] = static_cast<int32_t>(rows_in[i].size());
+      batch_values += counts[i];
+    }
+    for (const auto& a : additions) batch_values += static_cast<int64_t>(a.second.size());
+    const dyng_hip::allocator& now = h->alloc;
+    const int64_t room = std::min<int64_t>(now.pol.max_payload, dyng_hip::kIntMax);
+    const bool in_place =
+        now.pol.grows && 2 * now.bump + now.capacity + 4 * batch_values +
+                             now.pol.alignment * static_cast<int64_t>(counts.size() + additions.size()) + 65536 <
+                         room;
+    dyng_hip::allocator copy;
+    if (!in_place) copy = h->alloc;
+    dyng_hip::allocator& alloc = in_place ? h->alloc : copy;
     std::vector<int64_t> del_keys;
     for (int64_t id : deleted) del_keys.push_back(id + 1);
     std::vector<int64_t> keys;
@@ -600,7 +629,7 @@ extern "C" int dyng_hip_hypergraph_apply(dyng_hip_hypergraph* h, const dyng_hip_
       keys = alloc.insert(counts);
     }
     // ---- commit: the rows, the weights, the delta of this apply ----
-    h->alloc = std::move(alloc);
+    if (!in_place) h->alloc = std::move(copy);
     h->touched.clear();
     h->pre_rows.clear();
     h->pre_w.clear();
@@ -647,7 +676,9 @@ extern "C" int dyng_hip_hyper_line_graph(dyng_hip_hypergraph* h, int32_t source,
     h->out_dst.clear();
     h->out_w.clear();
     const int64_t b = h->id_bound();
-    h->mark_a.assign(static_cast<size_t>(std::max<int64_t>(b, 1)), -1);
+    h->marks_for(b);
+    const int64_t base = h->stamp;  // id t marks its neighbours with base + t
+    h->stamp += b + 1;
     if (source >= 0 && source < h->n) {
       std::vector<int64_t> seeds = h->v2h[static_cast<size_t>(source)];
       std::sort(seeds.begin(), seeds.end());
@@ -661,7 +692,7 @@ extern "C" int dyng_hip_hyper_line_graph(dyng_hip_hypergraph* h, int32_t source,
       const auto& row = h->rows[static_cast<size_t>(t)];
       if (row.empty()) continue;
       const int32_t wt = static_cast<int32_t>(h->w[static_cast<size_t>(t)]);
-      h->neighbours(row, t, h->mark_a, t, [&](int64_t g) {  // the in-edges (g, t): weight w(t)
+      h->neighbours(row, t, h->mark_a, base + t, [&](int64_t g) {  // the in-edges (g, t): weight w(t)
         h->out_src.push_back(static_cast<int32_t>(g + 1));
         h->out_dst.push_back(static_cast<int32_t>(t + 1));
         h->out_w.push_back(wt);
@@ -683,15 +714,20 @@ extern "C" int dyng_hip_hyper_line_graph_delta(dyng_hip_hypergraph* h, int32_t s
     h->out_del_src.clear();
     h->out_del_dst.clear();
     const int64_t b = std::max<int64_t>(h->id_bound(), static_cast<int64_t>(h->rows.size()));
-    h->mark_a.assign(static_cast<size_t>(std::max<int64_t>(b, 1)), -1);
-    h->mark_b.assign(static_cast<size_t>(std::max<int64_t>(b, 1)), -1);
-    std::vector<char> is_touched(static_cast<size_t>(std::max<int64_t>(b, 1)), 0);
-    for (int64_t t : h->touched) is_touched[static_cast<size_t>(t)] = 1;
-    // the touched ids that held each vertex before the apply
-    std::map<int32_t, std::vector<int64_t>> pre_holders;
+    // marks of this query take values above every earlier one (nothing is cleared): id i of the
+    // touched list marks with base + 2i and base + 2i + 1, the touched ids with base + 2k
+    h->marks_for(b);
+    const int64_t base = h->stamp;
+    const int64_t touched_mark = base + 2 * static_cast<int64_t>(h->touched.size());
+    h->stamp = touched_mark + 1;
+    auto is_touched = [&](int64_t g) { return h->mark_t[static_cast<size_t>(g)] == touched_mark; };
+    for (int64_t t : h->touched) h->mark_t[static_cast<size_t>(t)] = touched_mark;
+    // the touched ids that held each vertex before the apply: (vertex, id) pairs by vertex
+    h->pre_pairs.clear();
     for (size_t i = 0; i < h->touched.size(); ++i) {
-      for (int32_t v : h->pre_rows[i]) pre_holders[v].push_back(h->touched[i]);
+      for (int32_t v : h->pre_rows[i]) h->pre_pairs.emplace_back(v, h->touched[i]);
     }
+    std::sort(h->pre_pairs.begin(), h->pre_pairs.end());
     auto ins = [&](int64_t a, int64_t c, int64_t wt) {
       h->out_src.push_back(static_cast<int32_t>(a));
       h->out_dst.push_back(static_cast<int32_t>(c));
@@ -709,43 +745,43 @@ extern "C" int dyng_hip_hyper_line_graph_delta(dyng_hip_hypergraph* h, int32_t s
       const auto& post = h->rows[static_cast<size_t>(t)];
       const int64_t w_pre = h->pre_w[i];
       const int64_t w_post = h->w[static_cast<size_t>(t)];
-      const int64_t sa = 2 * static_cast<int64_t>(i), sb = 2 * static_cast<int64_t>(i) + 1;
+      const int64_t sa = base + 2 * static_cast<int64_t>(i), sb = sa + 1;
       // before: the untouched holders of its old members (unchanged by the apply) and the touched
       // ids that held them before; after: today's holders of its members
       before.clear();
       after.clear();
       for (int32_t v : pre) {
         for (int64_t g : h->v2h[static_cast<size_t>(v)]) {
-          if (g != t && !is_touched[static_cast<size_t>(g)] && h->mark_a[static_cast<size_t>(g)] != sa) {
+          if (g != t && !is_touched(g) && h->mark_a[static_cast<size_t>(g)] != sa) {
             h->mark_a[static_cast<size_t>(g)] = sa;
             before.push_back(g);
           }
         }
-        auto it = pre_holders.find(v);
-        if (it != pre_holders.end()) {
-          for (int64_t g : it->second) {
-            if (g != t && h->mark_a[static_cast<size_t>(g)] != sa) {
-              h->mark_a[static_cast<size_t>(g)] = sa;
-              before.push_back(g);
-            }
+        auto it = std::lower_bound(h->pre_pairs.begin(), h->pre_pairs.end(),
+                                   std::make_pair(v, std::numeric_limits<int64_t>::min()));
+        for (; it != h->pre_pairs.end() && it->first == v; ++it) {
+          const int64_t g = it->second;
+          if (g != t && h->mark_a[static_cast<size_t>(g)] != sa) {
+            h->mark_a[static_cast<size_t>(g)] = sa;
+            before.push_back(g);
           }
         }
       }
       h->neighbours(post, t, h->mark_b, sb, [&](int64_t g) { after.push_back(g); });
       for (int64_t g : before) {
         if (h->mark_b[static_cast<size_t>(g)] == sb) continue;  // kept
-        if (!is_touched[static_cast<size_t>(g)] || t < g) ++lost;
+        if (!is_touched(g) || t < g) ++lost;
         del(g + 1, t + 1);
-        if (!is_touched[static_cast<size_t>(g)]) del(t + 1, g + 1);  // a touched g deletes it as its in-edge
+        if (!is_touched(g)) del(t + 1, g + 1);  // a touched g deletes it as its in-edge
       }
       for (int64_t g : after) {
         if (h->mark_a[static_cast<size_t>(g)] == sa) {
           if (w_pre != w_post) ins(g + 1, t + 1, w_post);  // kept, the head's weight changed
           continue;
         }
-        if (!is_touched[static_cast<size_t>(g)] || t < g) ++gained;
+        if (!is_touched(g) || t < g) ++gained;
         ins(g + 1, t + 1, w_post);
-        if (!is_touched[static_cast<size_t>(g)]) ins(t + 1, g + 1, h->w[static_cast<size_t>(g)]);
+        if (!is_touched(g)) ins(t + 1, g + 1, h->w[static_cast<size_t>(g)]);
       }
       const bool was = std::binary_search(pre.begin(), pre.end(), source);
       const bool now = std::binary_search(post.begin(), post.end(), source);
