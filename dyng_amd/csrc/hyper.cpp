@@ -360,16 +360,19 @@ struct dyng_hip_hypergraph {
   std::vector<std::vector<int32_t>> pre_rows;
   std::vector<int64_t> pre_w;
   // scratch and the arrays the last query returned (valid until the next call on the handle)
-  std::vector<int64_t> mark_a, mark_b, mark_t;
+  /// Marks of one id, side by side so that a visited id costs one cache line: `a` and `b` hold
+  /// the stamps of the two neighbour sets being built, `t` the touched-id stamp.
+  struct id_marks {
+    int64_t a = -1, b = -1, t = -1;
+  };
+  std::vector<id_marks> marks;
   int64_t stamp = 0;  ///< marks below it are stale: each query takes fresh values above it
   std::vector<std::pair<int32_t, int64_t>> pre_pairs;  ///< (vertex, touched id) before the apply
 
-  /// The three mark arrays at least b long; entries past their old length start stale.
+  /// The marks at least b ids long; entries past their old length start stale.
   void marks_for(int64_t b) {
     const size_t need = static_cast<size_t>(std::max<int64_t>(b, 1));
-    if (mark_a.size() < need) mark_a.resize(need, -1);
-    if (mark_b.size() < need) mark_b.resize(need, -1);
-    if (mark_t.size() < need) mark_t.resize(need, -1);
+    if (marks.size() < need) marks.resize(need);
   }
   std::vector<int32_t> out_src, out_dst, out_w, out_del_src, out_del_dst;
   std::vector<int64_t> out_offsets;
@@ -390,14 +393,14 @@ struct dyng_hip_hypergraph {
     old = std::move(row);
   }
 
-  /// The live ids sharing a vertex with h (rows as they are now), marked with `stamp` in `mark`.
+  /// The live ids sharing a vertex with h (rows as they are now), marked with `st` in `field`.
   template <class Visit>
-  void neighbours(const std::vector<int32_t>& row, int64_t h, std::vector<int64_t>& mark, int64_t st,
-                  Visit visit) const {
+  void neighbours(const std::vector<int32_t>& row, int64_t h, int64_t id_marks::*field, int64_t st, Visit visit) {
     for (int32_t v : row) {
       for (int64_t g : v2h[static_cast<size_t>(v)]) {
-        if (g != h && mark[static_cast<size_t>(g)] != st) {
-          mark[static_cast<size_t>(g)] = st;
+        int64_t& m = marks[static_cast<size_t>(g)].*field;
+        if (g != h && m != st) {
+          m = st;
           visit(g);
         }
       }
@@ -692,7 +695,7 @@ extern "C" int dyng_hip_hyper_line_graph(dyng_hip_hypergraph* h, int32_t source,
       const auto& row = h->rows[static_cast<size_t>(t)];
       if (row.empty()) continue;
       const int32_t wt = static_cast<int32_t>(h->w[static_cast<size_t>(t)]);
-      h->neighbours(row, t, h->mark_a, base + t, [&](int64_t g) {  // the in-edges (g, t): weight w(t)
+      h->neighbours(row, t, &dyng_hip_hypergraph::id_marks::a, base + t, [&](int64_t g) {  // the in-edges (g, t): weight w(t)
         h->out_src.push_back(static_cast<int32_t>(g + 1));
         h->out_dst.push_back(static_cast<int32_t>(t + 1));
         h->out_w.push_back(wt);
@@ -720,8 +723,9 @@ extern "C" int dyng_hip_hyper_line_graph_delta(dyng_hip_hypergraph* h, int32_t s
     const int64_t base = h->stamp;
     const int64_t touched_mark = base + 2 * static_cast<int64_t>(h->touched.size());
     h->stamp = touched_mark + 1;
-    auto is_touched = [&](int64_t g) { return h->mark_t[static_cast<size_t>(g)] == touched_mark; };
-    for (int64_t t : h->touched) h->mark_t[static_cast<size_t>(t)] = touched_mark;
+    auto& marks = h->marks;
+    auto is_touched = [&](int64_t g) { return marks[static_cast<size_t>(g)].t == touched_mark; };
+    for (int64_t t : h->touched) marks[static_cast<size_t>(t)].t = touched_mark;
     // the touched ids that held each vertex before the apply: (vertex, id) pairs by vertex
     h->pre_pairs.clear();
     for (size_t i = 0; i < h->touched.size(); ++i) {
@@ -752,8 +756,9 @@ extern "C" int dyng_hip_hyper_line_graph_delta(dyng_hip_hypergraph* h, int32_t s
       after.clear();
       for (int32_t v : pre) {
         for (int64_t g : h->v2h[static_cast<size_t>(v)]) {
-          if (g != t && !is_touched(g) && h->mark_a[static_cast<size_t>(g)] != sa) {
-            h->mark_a[static_cast<size_t>(g)] = sa;
+          auto& m = marks[static_cast<size_t>(g)];
+          if (g != t && m.t != touched_mark && m.a != sa) {
+            m.a = sa;
             before.push_back(g);
           }
         }
@@ -761,21 +766,22 @@ extern "C" int dyng_hip_hyper_line_graph_delta(dyng_hip_hypergraph* h, int32_t s
                                    std::make_pair(v, std::numeric_limits<int64_t>::min()));
         for (; it != h->pre_pairs.end() && it->first == v; ++it) {
           const int64_t g = it->second;
-          if (g != t && h->mark_a[static_cast<size_t>(g)] != sa) {
-            h->mark_a[static_cast<size_t>(g)] = sa;
+          auto& m = marks[static_cast<size_t>(g)];
+          if (g != t && m.a != sa) {
+            m.a = sa;
             before.push_back(g);
           }
         }
       }
-      h->neighbours(post, t, h->mark_b, sb, [&](int64_t g) { after.push_back(g); });
+      h->neighbours(post, t, &dyng_hip_hypergraph::id_marks::b, sb, [&](int64_t g) { after.push_back(g); });
       for (int64_t g : before) {
-        if (h->mark_b[static_cast<size_t>(g)] == sb) continue;  // kept
+        if (marks[static_cast<size_t>(g)].b == sb) continue;  // kept
         if (!is_touched(g) || t < g) ++lost;
         del(g + 1, t + 1);
         if (!is_touched(g)) del(t + 1, g + 1);  // a touched g deletes it as its in-edge
       }
       for (int64_t g : after) {
-        if (h->mark_a[static_cast<size_t>(g)] == sa) {
+        if (marks[static_cast<size_t>(g)].a == sa) {
           if (w_pre != w_post) ins(g + 1, t + 1, w_post);  // kept, the head's weight changed
           continue;
         }
