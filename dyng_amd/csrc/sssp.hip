@@ -23,9 +23,10 @@
 //
 // Rows are expanded edge-balanced: a workgroup takes a tile of the list, scans the row lengths in
 // LDS and deals the tile's edge slots out one per thread, so a thread's critical path is one edge
-// rather than one row. Rows longer than `light` are cut into chunks that whole waves expand after
-// a barrier. Per-iteration list counters live in rings of three slots, so an iteration needs one
-// barrier (two when it has chunked rows). Workgroups exchange data only through agent-scope
+// rather than one row. Rows longer than `light` are cut into chunks that whole waves expand: in the
+// walk and the pull after a barrier of their own, in the relax phase at the start of the next
+// iteration. Per-iteration list counters live in rings of three slots, so a relax iteration needs
+// one barrier. Workgroups exchange data only through agent-scope
 // atomics and loads/stores that bypass the non-coherent caches, so the grid barrier itself needs
 // no cache write-back or invalidation (the distance-only unpack, which reads plain stores of
 // other workgroups, uses the fenced barrier).
