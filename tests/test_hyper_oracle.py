@@ -245,16 +245,27 @@ def test_edge_batch_carries_the_line_graph(reuse, seed):
     """Applying the edge batch of an update to the line graph of the old hypergraph (deletions
     first, then upserts) gives the line graph of the new one, weights included, and the pair
     counts equal the oracle's delta."""
-    rng = np.random.default_rng(7 + seed)
-    n = 12
-    lists = [rng.integers(0, n, int(rng.integers(1, 5))).tolist() for _ in range(9)]
-    h = dg.Hypergraph.from_lists(lists, rng.integers(1, 9, 9).tolist(), num_vertices=n, id_reuse=reuse)
+    _check_edge_batches(np.random.default_rng(7 + seed), reuse, n=12, m=9, size=6, rounds=10)
+
+
+@pytest.mark.parametrize("reuse", ["erase_first", "insert_first"])
+@pytest.mark.parametrize("seed", range(3))
+def test_edge_batch_carries_the_line_graph_mid_size(reuse, seed):
+    """The same on 200 hyperedges over 60 vertices (about eight holders per vertex) with batches of
+    up to 60 operations of each kind: many touched ids share vertices with each other, so pairs of
+    touched ids and ids that are both deleted from and added to a row's neighbours are common."""
+    _check_edge_batches(np.random.default_rng(70 + seed), reuse, n=60, m=200, size=60, rounds=8)
+
+
+def _check_edge_batches(rng, reuse, n, m, size, rounds):
+    lists = [rng.integers(0, n, int(rng.integers(1, 5))).tolist() for _ in range(m)]
+    h = dg.Hypergraph.from_lists(lists, rng.integers(1, 9, m).tolist(), num_vertices=n, id_reuse=reuse)
     source = int(rng.integers(0, n))
     edges = _edges(h, source)
     assert edges == _brute_edges(h.rows(), h.weights.tolist(), source)
-    for _ in range(10):
+    for _ in range(rounds):
         pre_rows = h.rows()
-        dels, ins, w, rem, add = _random_batch(rng, h, n)
+        dels, ins, w, rem, add = _random_batch(rng, h, n, size=size)
         h.apply(dg.HyperedgeBatch(dels, ins, w, delete_incidences=rem, insert_incidences=add))
         eb, gained, lost = hs._edge_batch(h, source)
         ins_s, ins_d, ins_w = (np.asarray(a) for a in eb._ins) if eb._ins else ([], [], [])
