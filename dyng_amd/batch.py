@@ -78,6 +78,17 @@ class EdgeBatch:
             raise InvalidArgumentError("EdgeBatch: weights must fit int32")
         return ids[0], ids[1], np.ascontiguousarray(w.astype(np.int32))
 
+    @classmethod
+    def _from_library(cls, insert, delete) -> "EdgeBatch":
+        """A host batch from contiguous int32 arrays the library produced: (src, dst, weights) and
+        (src, dst), one weight per insertion. It skips the checks and conversions of the
+        constructor; the fields are the ones the constructor would build from the same arrays."""
+        b = cls.__new__(cls)
+        b._device = False
+        b._ins = (insert[0], insert[1], insert[2].reshape(-1, 1))
+        b._del = (delete[0], delete[1])
+        return b
+
     @staticmethod
     def _check_device(parts, with_weights):
         out = list(parts)

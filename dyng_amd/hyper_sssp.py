@@ -119,8 +119,8 @@ def _edge_batch(hg: Hypergraph, source: int) -> tuple[EdgeBatch, int, int]:
         return np.ctypeslib.as_array(p, shape=(k,)).copy() if k else np.zeros(0, np.int32)
 
     ni, nd = d.num_insertions, d.num_deletions
-    batch = EdgeBatch(insert=(arr(d.insert_src, ni), arr(d.insert_dst, ni), arr(d.insert_weights, ni)),
-                      delete=(arr(d.delete_src, nd), arr(d.delete_dst, nd)))
+    batch = EdgeBatch._from_library((arr(d.insert_src, ni), arr(d.insert_dst, ni), arr(d.insert_weights, ni)),
+                                    (arr(d.delete_src, nd), arr(d.delete_dst, nd)))
     return batch, int(d.line_graph_insertions), int(d.line_graph_deletions)
 
 

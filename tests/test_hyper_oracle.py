@@ -268,6 +268,11 @@ def _check_edge_batches(rng, reuse, n, m, size, rounds):
         dels, ins, w, rem, add = _random_batch(rng, h, n, size=size)
         h.apply(dg.HyperedgeBatch(dels, ins, w, delete_incidences=rem, insert_incidences=add))
         eb, gained, lost = hs._edge_batch(h, source)
+        # the library's arrays skip the constructor's checks: its fields are what the constructor builds
+        built = dg.EdgeBatch(insert=tuple(a.reshape(-1) for a in eb._ins), delete=eb._del)
+        for got, want in zip(eb._ins + eb._del, built._ins + built._del):
+            assert got.dtype == want.dtype == np.int32 and got.shape == want.shape and got.flags.c_contiguous
+            assert np.array_equal(got, want)
         ins_s, ins_d, ins_w = (np.asarray(a) for a in eb._ins) if eb._ins else ([], [], [])
         del_s, del_d = (np.asarray(a) for a in eb._del) if eb._del else ([], [])
         for a, b in zip(del_s, del_d):
