@@ -587,9 +587,9 @@ extern "C" int dyng_hip_hypergraph_apply(dyng_hip_hypergraph* h, const dyng_hip_
             ++added;
           }
         }
-        if (!rem.empty() || !add.empty()) edited.emplace_back(id, row);
-        if (!rem.empty()) removals.emplace_back(id + 1, rem);
-        if (!add.empty()) additions.emplace_back(id + 1, add);
+        if (!rem.empty() || !add.empty()) edited.emplace_back(id, std::move(row));
+        if (!rem.empty()) removals.emplace_back(id + 1, std::move(rem));
+        if (!add.empty()) additions.emplace_back(id + 1, std::move(add));
       }
       s = e;
     }
@@ -636,17 +636,18 @@ extern "C" int dyng_hip_hypergraph_apply(dyng_hip_hypergraph* h, const dyng_hip_
     h->touched.clear();
     h->pre_rows.clear();
     h->pre_w.clear();
-    std::set<int64_t> recorded;
-    auto record = [&](int64_t id) {
-      if (!recorded.insert(id).second) return;
-      h->touched.push_back(id);
+    // the ids whose rows change, once each in increasing order, with their rows and weights
+    // before the apply
+    for (int64_t id : deleted) h->touched.push_back(id);
+    for (const auto& e : edited) h->touched.push_back(e.first);
+    for (int64_t k : keys) h->touched.push_back(k - 1);
+    std::sort(h->touched.begin(), h->touched.end());
+    h->touched.erase(std::unique(h->touched.begin(), h->touched.end()), h->touched.end());
+    for (int64_t id : h->touched) {
       const bool has = id < static_cast<int64_t>(h->rows.size());
       h->pre_rows.push_back(has ? h->rows[static_cast<size_t>(id)] : std::vector<int32_t>{});
       h->pre_w.push_back(has ? h->w[static_cast<size_t>(id)] : 0);
-    };
-    for (int64_t id : deleted) record(id);
-    for (const auto& e : edited) record(e.first);
-    for (int64_t k : keys) record(k - 1);
+    }
     for (int64_t id : deleted) h->set_row(id, {});
     for (auto& e : edited) h->set_row(e.first, std::move(e.second));
     for (size_t i = 0; i < keys.size(); ++i) {
